@@ -1,0 +1,213 @@
+// hv_fft.hpp -- LDS-staged mixed-radix Stockham FFT engine for gfx950.
+//
+// One transform of length N is carried by NT = N / E threads, each holding
+// E complex values in registers.  Stage s applies radix R_s (16, 8, 4 or 2)
+// with Ns = R_0 ... R_{s-1}: thread j reads x[j + r N/R], multiplies by
+// exp(dir 2 pi i r (j mod Ns) / (Ns R)), does a radix-R DFT in registers and
+// writes x'[(j / Ns) Ns R + (j mod Ns) + r Ns].  Stage 0 reads through a
+// caller-supplied loader (global memory, fused conversion / chirp), the last
+// stage writes through a caller-supplied storer (global memory, fused lens,
+// ring slot, image epilogue), and stages exchange through LDS in between.
+// Twiddles come from a 4096-entry table of exp(-2 pi i k / 4096) rounded
+// from double (hv_twiddles.inc); compile-time indices fold to constants.
+#pragma once
+
+#include "hv_common.hpp"
+#include "hv_twiddles.inc"  // generated into _build/gen by build.py
+
+namespace hv {
+
+template <int I, int S, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < S) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, S>(f);
+    }
+}
+
+// exp(dir * 2 pi i * k / n) for n | 4096
+template <int DIR>
+__device__ __forceinline__ float2 twiddle(int k, int log2n) {
+    const int idx = (k << (12 - log2n)) & (kTwN - 1);
+    const float c = kTw[idx][0], s = kTw[idx][1];
+    return DIR < 0 ? make_float2(c, s) : make_float2(c, -s);
+}
+
+// d * exp(dir * 2 pi i * k / m), m <= 16, k compile-time after unrolling.
+template <int DIR>
+__device__ __forceinline__ float2 tw_small(float2 d, int k, int m) {
+    const int k16 = (k * (16 / m)) & 15;
+    if (k16 == 0) return d;
+    if (k16 == 8) return make_float2(-d.x, -d.y);
+    if (k16 == 4)  // exp(dir i pi/2) = dir * i
+        return DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+    if (k16 == 12) return DIR < 0 ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    const float c = kTw[k16 * 256][0];
+    const float s = DIR < 0 ? kTw[k16 * 256][1] : -kTw[k16 * 256][1];
+    return make_float2(d.x * c - d.y * s, d.x * s + d.y * c);
+}
+
+__host__ __device__ constexpr int bitrev(int x, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b)
+        if (x & (1 << b)) r |= 1 << (bits - 1 - b);
+    return r;
+}
+__host__ __device__ constexpr int clog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+// In-register radix-R DFT (R <= 16), natural order in and out.
+template <int R, int DIR>
+__device__ __forceinline__ void dft_reg(float2* v) {
+    if constexpr (R == 1) return;
+    // radix-2 decimation in frequency: natural in, bit-reversed out
+#pragma unroll
+    for (int half = R / 2; half >= 1; half >>= 1) {
+#pragma unroll
+        for (int blk = 0; blk < R; blk += 2 * half) {
+#pragma unroll
+            for (int k = 0; k < half; ++k) {
+                const float2 a = v[blk + k], b = v[blk + k + half];
+                v[blk + k] = cadd(a, b);
+                v[blk + k + half] = tw_small<DIR>(csub(a, b), k, 2 * half);
+            }
+        }
+    }
+    float2 t[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) t[i] = v[bitrev(i, clog2(R))];
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = t[i];
+}
+
+// Stage plan for length N with E values per thread.
+template <int N, int E>
+struct FftShape {
+    static constexpr int kE = E < N ? E : N;
+    static constexpr int NT = N / kE;
+    static constexpr int stages() {
+        int s = 0, rem = N;
+        while (rem > 1) {
+            rem /= (rem >= kE ? kE : rem);
+            ++s;
+        }
+        return s;
+    }
+    static constexpr int radix(int s) {
+        int rem = N;
+        for (int i = 0; i < s; ++i) rem /= (rem >= kE ? kE : rem);
+        return rem >= kE ? kE : rem;
+    }
+    static constexpr int ns(int s) {
+        int n = 1, rem = N;
+        for (int i = 0; i < s; ++i) {
+            const int r = rem >= kE ? kE : rem;
+            n *= r;
+            rem /= r;
+        }
+        return n;
+    }
+};
+
+// Runs one length-N transform on thread t (0 <= t < NT) of its group.
+//   load(n)      -> float2 : element n of the input (stage 0)
+//   store(n, v)            : element n of the output (last stage)
+//   lds(n)       -> float2&: this transform's LDS staging slot n
+// Every thread of the workgroup must call fft_run the same number of times
+// (it contains __syncthreads()).  STORE_SYNC puts a barrier before the last
+// stage's stores, for storers that write into the same LDS staging area.
+template <int N, int E, int DIR, bool STORE_SYNC = false, class Load, class Store, class Lds>
+__device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&& lds) {
+    using S = FftShape<N, E>;
+    constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
+    constexpr int LOGN = clog2(N);
+    float2 v[kE];
+    {
+        constexpr int R = S::radix(0), NB = kE / R;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int j = t + k * NT;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[k * R + r] = load(j + r * (N / R));
+        }
+    }
+    static_for<0, NS>([&](auto s_) {
+        constexpr int s = decltype(s_)::value;
+        constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R;
+        if constexpr (s > 0) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int j = t + k * NT;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[k * R + r] = lds(j + r * (N / R));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int j = t + k * NT;
+            if constexpr (Ns > 1) {
+                const int kk = j & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[k * R + r] = cmul(v[k * R + r], twiddle<DIR>(r * kk * (N / (Ns * R)), LOGN));
+            }
+            dft_reg<R, DIR>(&v[k * R]);
+        }
+        if constexpr (s == NS - 1) {
+            if constexpr (STORE_SYNC) __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int j = t + k * NT;
+                const int d = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+                for (int r = 0; r < R; ++r) store(d + r * Ns, v[k * R + r]);
+            }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int j = t + k * NT;
+                const int d = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds(d + r * Ns) = v[k * R + r];
+            }
+            __syncthreads();
+        }
+    });
+}
+
+// Whole transform in one thread's registers (N <= 64): radix-2 DIF with
+// table twiddles, natural order in and out.
+template <int N, int DIR>
+__device__ __forceinline__ void fft_reg(float2* v) {
+    if constexpr (N <= 16) {
+        dft_reg<N, DIR>(v);
+    } else {
+        constexpr int LOGN = clog2(N);
+#pragma unroll
+        for (int half = N / 2; half >= 1; half >>= 1) {
+#pragma unroll
+            for (int blk = 0; blk < N; blk += 2 * half) {
+#pragma unroll
+                for (int k = 0; k < half; ++k) {
+                    const float2 a = v[blk + k], b = v[blk + k + half];
+                    v[blk + k] = cadd(a, b);
+                    const float2 d = csub(a, b);
+                    // exp(dir 2 pi i k / (2 half))
+                    const int kk = k * (N / (2 * half));
+                    v[blk + k + half] = (kk == 0) ? d : cmul(d, twiddle<DIR>(kk, LOGN));
+                }
+            }
+        }
+        float2 t[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) t[i] = v[bitrev(i, LOGN)];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = t[i];
+    }
+}
+
+}  // namespace hv

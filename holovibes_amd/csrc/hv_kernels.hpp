@@ -1,0 +1,64 @@
+// hv_kernels.hpp -- host launchers of the HIP kernels (internal C++ API).
+// Arguments are validated by the C-ABI layer (hv_capi.cpp) before these run.
+#pragma once
+
+#include "hv_common.hpp"
+
+namespace hv {
+
+// hv_spatial.hip
+hv_status launch_convert(float2* out, const void* in, size_t n, int depth, hipStream_t st);
+hv_status launch_lens_as(float2* out, int nx, int ny, float z, float lambda, float xs, float ys, int precision,
+                         int shift, float scale, hipStream_t st);
+hv_status launch_lens_fresnel(float2* out, int side, int width, int height, float lambda, float z, float px,
+                              int precision, hipStream_t st);
+hv_status launch_shift_corners_c64(float2* buf, int batch, int sx, int sy, hipStream_t st);
+hv_status launch_shift_corners_f32(float* buf, int batch, int sx, int sy, hipStream_t st);
+hv_status launch_fft2(float2* buf, int batch, int width, int height, int direction, hipStream_t st);
+// x = IFFT2(FFT2(x) * lens) * post_scale
+hv_status launch_angular_spectrum(float2* buf, int batch, const float2* lens, int width, int height,
+                                  float post_scale, hipStream_t st);
+hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width, int height, hipStream_t st);
+hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
+                                   const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st);
+
+// hv_time.hip
+hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_t st);
+hv_status launch_complex_to_image(float* out, const float2* in, int img_type, int pmin, int pmax, size_t P,
+                                  hipStream_t st);
+// Fused STFT + image type.  mm: encoded min/max (may be null).
+hv_status launch_stft_image(float* out, const float2* ring, int T, size_t P, int img_type, int pmin, int pmax,
+                            uint32_t* mm, hipStream_t st);
+hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* ring, int T, size_t P, int f_start,
+                              int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st);
+
+// hv_render.hip
+hv_status launch_minmax_init(uint32_t* mm, hipStream_t st);
+hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st);
+hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st);
+hv_status launch_subsample_f32(float* out, const float* in, int width, int height, int step, hipStream_t st);
+hv_status launch_apply_contrast(float* img, size_t n, float mn, float mx, hipStream_t st);
+hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st);
+// range: 2 values, floats (encoded = 0) or ordered-uint min/max (encoded = 1).
+hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
+                                    int encoded, int invert, hipStream_t st);
+hv_status launch_scale(float* img, size_t n, float factor, hipStream_t st);
+size_t percentile_scratch_bytes();
+// Writes range[0..1] = values of rank k0, k1 (ascending) of the image (or its
+// subsample), optionally clamped to >= 1.
+hv_status launch_percentile(float* range, const float* img, int width, int height, int step, float lo, float hi,
+                            int clamp, void* scratch, hipStream_t st);
+hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st);
+hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st);
+size_t unwrap_scratch_bytes(int width, int height);
+
+// hv_pca.hip
+size_t pca_workspace_bytes(int T, size_t P, int precision);
+hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int precision, void* ws,
+                         hipStream_t st);
+size_t eig_workspace_bytes(int T);
+hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st);
+hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
+                           int pmin, int pmax, uint32_t* mm, hipStream_t st);
+
+}  // namespace hv

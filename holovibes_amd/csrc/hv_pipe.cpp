@@ -1,0 +1,584 @@
+// hv_pipe.cpp -- the per-batch orchestrator (Pipe::exec / refresh,
+// Backend/sources/core/pipe.cc:255-387, 650-680) and the multi-source server.
+//
+// A pipe owns every device buffer it will ever use (allocated at create, the
+// reference's refresh time): the T-slot time-window ring, the lens, the
+// image / contrast / u16 outputs, reduction and PCA scratch, and an input
+// staging buffer.  The control flow the reference runs as lambdas
+// (stride discard pipe.cc:400-416, ring fill wait :418-427, autocontrast once
+// rendering.cc:239-258) stays on the host; the device work of a steady-state
+// cycle is captured once per ring phase into a hipGraph and replayed.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "hv_kernels.hpp"
+
+using namespace hv;
+
+namespace {
+
+bool is_image_kind(int t) {
+    return t == HV_IMG_MODULUS || t == HV_IMG_SQUARED_MODULUS || t == HV_IMG_ARGUMENT;
+}
+bool is_moment_kind(int t) { return t == HV_IMG_MOMENTS_0 || t == HV_IMG_MOMENTS_1 || t == HV_IMG_MOMENTS_2; }
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+struct Layout {
+    size_t ring, lens, staging, image, cimage, tmp, out, small, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
+    size_t total;
+};
+
+Layout layout_of(const hv_compute_settings& s) {
+    Layout l{};
+    const size_t P = (size_t)s.width * s.height;
+    const size_t T = s.time_transformation_size;
+    l.ring = align256(T * P * sizeof(float2));
+    l.lens = s.space_transformation != HV_SPACE_NONE ? align256(P * sizeof(float2)) : 0;
+    l.staging = align256((size_t)s.batch_size * P * s.depth);
+    l.image = align256(P * sizeof(float));
+    l.cimage = align256(P * sizeof(float));
+    const bool unwrap = s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d;
+    l.tmp = unwrap ? align256(P * sizeof(float)) : 0;
+    l.out = align256(P * sizeof(uint16_t));
+    l.small = 256;  // min/max (2 x uint32) + range (2 floats)
+    l.pct = align256(percentile_scratch_bytes());
+    l.unwrap = unwrap ? align256(unwrap_scratch_bytes(s.width, s.height)) : 0;
+    if (s.time_transformation == HV_TIME_PCA) {
+        l.cov = align256(T * T * sizeof(float2));
+        l.evals = align256(T * sizeof(double));
+        l.evecs = align256(T * T * sizeof(double2));
+        l.pca_ws = align256(pca_workspace_bytes((int)T, P, s.pca_precision));
+        l.eig_ws = align256(eig_workspace_bytes((int)T));
+    }
+    l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.pct + l.unwrap + l.cov +
+              l.evals + l.evecs + l.pca_ws + l.eig_ws;
+    return l;
+}
+
+}  // namespace
+
+struct hv_pipe {
+    hv_compute_settings s{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t P = 0;
+    int T = 0, B = 0;
+    Layout lay{};
+    char* base = nullptr;
+    float2* ring = nullptr;
+    float2* lens = nullptr;
+    void* staging = nullptr;
+    float* image = nullptr;
+    float* cimage = nullptr;
+    float* tmp = nullptr;
+    uint16_t* out = nullptr;
+    uint32_t* mm = nullptr;
+    float* range = nullptr;
+    void* pct = nullptr;
+    void* unwrap_ws = nullptr;
+    float2* cov = nullptr;
+    double* evals = nullptr;
+    double2* evecs = nullptr;
+    void* pca_ws = nullptr;
+    void* eig_ws = nullptr;
+    // host mirror of the reference's state
+    uint32_t batch_index = 0;  // BatchEnv::batch_index
+    uint32_t q_start = 0, q_size = 0;
+    bool autocontrast_pending = true;
+    bool have_output = false;
+    // graphs, one per (ring phase, autocontrast-pending) pair
+    bool graph_enabled = false;
+    std::map<int, hipGraphExec_t> graphs;
+
+    hv_status image_work();
+    hv_status enqueue_cycle(int slot0);
+};
+
+// Device work of one cycle after the ring holds T frames (pipe.cc:333-380).
+hv_status hv_pipe::image_work() {
+    const int it = s.img_type;
+    const int Tn = T;
+    const bool minmax_mode = s.contrast_enabled && s.contrast_mode == HV_CONTRAST_MINMAX;
+    const bool unwrap = it == HV_IMG_ARGUMENT && s.unwrap_2d;
+    // the image's min/max is folded in by the producing kernel when it is the final image
+    uint32_t* mm_img = (minmax_mode && !unwrap) ? mm : nullptr;
+    if (mm_img) HV_TRY(launch_minmax_init(mm, stream));
+    if (is_image_kind(it)) {
+        // insert_compute_p_accu (converts.cc:76-92)
+        const int pmin = (int)s.p_start;
+        const int pmax = s.p_width ? std::min<int>(pmin + (int)s.p_width, Tn) : pmin;
+        if (s.time_transformation == HV_TIME_STFT) {
+            HV_TRY(launch_stft_image(image, ring, Tn, P, it, pmin, pmax, mm_img, stream));
+        } else if (s.time_transformation == HV_TIME_PCA) {
+            HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, stream));
+            HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, stream));
+            HV_TRY(launch_pca_image(image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, stream));
+        } else {
+            HV_TRY(launch_complex_to_image(image, ring, it, pmin, pmax, P, stream));
+            if (mm_img) HV_TRY(launch_minmax(mm, image, P, stream));
+        }
+        if (unwrap) {
+            HV_TRY(launch_unwrap_2d(tmp, image, s.width, s.height, unwrap_ws, stream));
+            HV_TRY(launch_rescale_unwrap(image, tmp, P, mm, stream));
+        }
+    } else {
+        // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
+        const int fs = (int)s.p_start;
+        const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
+        float* m0 = it == HV_IMG_MOMENTS_0 ? image : nullptr;
+        float* m1 = it == HV_IMG_MOMENTS_1 ? image : nullptr;
+        float* m2 = it == HV_IMG_MOMENTS_2 ? image : nullptr;
+        HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
+                                   it - HV_IMG_MOMENTS_0, stream));
+    }
+    if (s.fft_shift) HV_TRY(launch_shift_corners_f32(image, 1, s.width, s.height, stream));
+    if (!s.contrast_enabled) return launch_float_to_ushort(out, image, P, stream);
+    switch (s.contrast_mode) {
+        case HV_CONTRAST_MINMAX:
+            if (unwrap || s.fft_shift) {
+                // recompute on the final image (unwrap rescales; shift keeps min/max but is cheap to cover)
+                HV_TRY(launch_minmax_init(mm, stream));
+                HV_TRY(launch_minmax(mm, image, P, stream));
+            }
+            HV_TRY(launch_minmax_decode(range, mm, stream));
+            break;
+        case HV_CONTRAST_PERCENTILE:
+            HV_TRY(launch_percentile(range, image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
+                                     s.contrast_upper, 1, pct, stream));
+            break;
+        case HV_CONTRAST_REFERENCE:
+            if (autocontrast_pending)
+                HV_TRY(launch_percentile(range, image, s.width, s.height, (int)s.contrast_subsample,
+                                         s.contrast_lower, s.contrast_upper, 1, pct, stream));
+            break;
+        default:  // MANUAL: range uploaded at create
+            break;
+    }
+    return launch_contrast_to_ushort(out, cimage, image, P, range, 0, s.contrast_invert, stream);
+}
+
+hv_status hv_pipe::enqueue_cycle(int slot0) {
+    HV_TRY(launch_propagate_to_ring(staging, (int)s.depth, B, (int)s.width, (int)s.height, s.space_transformation,
+                                    lens, ring, T, slot0, stream));
+    return image_work();
+}
+
+extern "C" {
+
+void hv_default_settings(hv_compute_settings* s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    // Backend/includes/core/holovibes.hh:359-447
+    s->width = 512;
+    s->height = 512;
+    s->depth = HV_DEPTH_16;
+    s->batch_size = 1;
+    s->time_stride = 1;
+    s->time_transformation_size = 1;
+    s->space_transformation = HV_SPACE_NONE;
+    s->time_transformation = HV_TIME_NONE;
+    s->img_type = HV_IMG_MODULUS;
+    s->lambda = 852e-9f;
+    s->z_distance = 0.0f;
+    s->pixel_size = 12.0f;
+    s->p_start = 0;
+    s->p_width = 0;
+    s->unwrap_2d = 0;
+    s->fft_shift = 0;
+    s->contrast_enabled = 1;
+    s->contrast_invert = 0;
+    s->contrast_mode = HV_CONTRAST_REFERENCE;
+    s->contrast_min = 1.0f;
+    s->contrast_max = 65535.0f;
+    s->contrast_lower = 0.02f;
+    s->contrast_upper = 99.98f;
+    s->contrast_subsample = 1;
+    s->camera_fps = 0.0f;
+    s->lens_precision = HV_LENS_FP64;
+    s->pca_precision = HV_PCA_BF16_SPLIT;
+}
+
+hv_status hv_validate_settings(const hv_compute_settings* s) {
+    if (!s) return fail(HV_ERR_INVALID_ARG, "settings is null");
+    if (s->width == 0 || s->height == 0) return fail(HV_ERR_INVALID_ARG, "frame size must be positive");
+    if (s->space_transformation != HV_SPACE_NONE || s->time_transformation == HV_TIME_STFT ||
+        s->time_transformation == HV_TIME_PCA || (s->img_type == HV_IMG_ARGUMENT && s->unwrap_2d)) {
+        if (!fft_len_ok(s->width) || !fft_len_ok(s->height))
+            return fail(HV_ERR_UNSUPPORTED, "frame sides must be powers of two in [16, 4096]");
+    }
+    if (((size_t)s->width * s->height) % 256) return fail(HV_ERR_UNSUPPORTED, "frame must hold a multiple of 256 pixels");
+    if (s->depth != HV_DEPTH_8 && s->depth != HV_DEPTH_16 && s->depth != HV_DEPTH_32F)
+        return fail(HV_ERR_INVALID_ARG, "depth must be 1, 2 or 4");
+    // transform_api.cc:36-79: batch <= stride, stride a multiple of batch
+    if (s->batch_size == 0) return fail(HV_ERR_INVALID_ARG, "batch_size must be >= 1");
+    if (s->time_stride < s->batch_size || s->time_stride % s->batch_size)
+        return fail(HV_ERR_INVALID_ARG, "time_stride must be a multiple of batch_size");
+    // transform_api.cc:113-116
+    if (s->time_transformation_size == 0) return fail(HV_ERR_INVALID_ARG, "time_transformation_size must be >= 1");
+    if (!time_len_ok(s->time_transformation_size))
+        return fail(HV_ERR_UNSUPPORTED, "time_transformation_size must be a power of two <= 256");
+    if (s->space_transformation < HV_SPACE_NONE || s->space_transformation > HV_SPACE_ANGULARSP)
+        return fail(HV_ERR_INVALID_ARG, "unknown space transformation");
+    if (s->time_transformation < HV_TIME_NONE || s->time_transformation > HV_TIME_PCA)
+        return fail(HV_ERR_UNSUPPORTED, "time transformation not supported (SSA_STFT is not on this path)");
+    if (s->space_transformation != HV_SPACE_NONE && !(s->lambda > 0.f && s->pixel_size > 0.f))
+        return fail(HV_ERR_INVALID_ARG, "lambda and pixel_size must be positive");
+    if (s->space_transformation == HV_SPACE_FRESNELTR && s->z_distance == 0.f)
+        return fail(HV_ERR_INVALID_ARG, "Fresnel transform needs z_distance != 0");
+    const uint32_t T = s->time_transformation_size;
+    if (is_image_kind(s->img_type)) {
+        // pmax = start + width must stay inside the T frames the kernels read
+        if (s->p_start >= T || (s->p_width && s->p_start + s->p_width >= T))
+            return fail(HV_ERR_INVALID_ARG, "P range must satisfy p_start + p_width < T");
+    } else if (is_moment_kind(s->img_type)) {
+        if (s->p_start >= T) return fail(HV_ERR_INVALID_ARG, "p_start must be < T");
+        if (s->time_transformation != HV_TIME_STFT)
+            return fail(HV_ERR_UNSUPPORTED, "moments are computed from the STFT");
+    } else {
+        return fail(HV_ERR_UNSUPPORTED, "image type not on this path");
+    }
+    if (s->time_transformation == HV_TIME_PCA && (s->pca_precision < HV_PCA_FP32 || s->pca_precision > HV_PCA_FP16))
+        return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
+    if (s->contrast_mode < HV_CONTRAST_REFERENCE || s->contrast_mode > HV_CONTRAST_MANUAL)
+        return fail(HV_ERR_INVALID_ARG, "unknown contrast mode");
+    if (s->contrast_subsample > 1 && (s->width % s->contrast_subsample || s->height % s->contrast_subsample))
+        return fail(HV_ERR_INVALID_ARG, "contrast_subsample must divide the frame sides");
+    if (s->fft_shift && (s->width % 2 || s->height % 2)) return fail(HV_ERR_UNSUPPORTED, "fft shift needs even sides");
+    return HV_OK;
+}
+
+hv_status hv_pipe_memory_bytes(const hv_compute_settings* s, size_t* bytes) {
+    if (!bytes) return fail(HV_ERR_INVALID_ARG, "bytes is null");
+    HV_TRY(hv_validate_settings(s));
+    *bytes = layout_of(*s).total;
+    return HV_OK;
+}
+
+hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** out) {
+    if (!out) return fail(HV_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    HV_TRY(hv_validate_settings(s));
+    std::unique_ptr<hv_pipe> p(new hv_pipe);
+    p->s = *s;
+    p->P = (size_t)s->width * s->height;
+    p->T = (int)s->time_transformation_size;
+    p->B = (int)s->batch_size;
+    p->lay = layout_of(*s);
+    if (stream) {
+        p->stream = as_stream(stream);
+    } else {
+        HV_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        p->own_stream = true;
+    }
+    void* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, p->lay.total);
+    if (e != hipSuccess) {
+        if (p->own_stream) (void)hipStreamDestroy(p->stream);
+        return hip_fail(e, "hipMalloc(pipe)");
+    }
+    p->base = (char*)mem;
+    char* c = p->base;
+    auto take = [&](size_t n) {
+        char* r = n ? c : nullptr;
+        c += n;
+        return r;
+    };
+    const Layout& l = p->lay;
+    p->ring = (float2*)take(l.ring);
+    p->lens = (float2*)take(l.lens);
+    p->staging = take(l.staging);
+    p->image = (float*)take(l.image);
+    p->cimage = (float*)take(l.cimage);
+    p->tmp = (float*)take(l.tmp);
+    p->out = (uint16_t*)take(l.out);
+    char* small = take(l.small);
+    p->mm = (uint32_t*)small;
+    p->range = (float*)(small + 16);
+    p->pct = take(l.pct);
+    p->unwrap_ws = take(l.unwrap);
+    p->cov = (float2*)take(l.cov);
+    p->evals = (double*)take(l.evals);
+    p->evecs = (double2*)take(l.evecs);
+    p->pca_ws = take(l.pca_ws);
+    p->eig_ws = take(l.eig_ws);
+    // refresh-time work: lens, manual contrast range, zeroed ring
+    hv_status st = hv_prepare_lens(p->lens, s->space_transformation, s->width, s->height, s->lambda, s->z_distance,
+                                   s->pixel_size, s->lens_precision, p->stream);
+    if (st == HV_OK) {
+        const float r[2] = {s->contrast_min, s->contrast_max};
+        e = hipMemcpyAsync(p->range, r, sizeof(r), hipMemcpyHostToDevice, p->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(p->ring, 0, l.ring, p->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        if (e != hipSuccess) st = hip_fail(e, "pipe init");
+    }
+    if (st != HV_OK) {
+        hv_pipe_destroy(p.release());
+        return st;
+    }
+    *out = p.release();
+    return HV_OK;
+}
+
+hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames) {
+    if (!p || !frames) return fail(HV_ERR_INVALID_ARG, "null argument");
+    *frames = p->staging;
+    return HV_OK;
+}
+
+hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
+    if (!p || !frames) return fail(HV_ERR_INVALID_ARG, "null argument");
+    int made = 0;
+    if (produced) *produced = 0;
+    // insert_wait_time_stride (pipe.cc:400-416)
+    p->batch_index += p->B;
+    if (p->batch_index != p->s.time_stride) return HV_OK;
+    p->batch_index = 0;
+    if (frames != p->staging)
+        HV_HIP(hipMemcpyAsync(p->staging, frames, (size_t)p->B * p->P * p->s.depth, hipMemcpyDeviceToDevice,
+                              p->stream));
+    // Queue::enqueue_multiple bookkeeping (queue.cc:332-376)
+    const uint32_t maxq = (uint32_t)p->T;
+    const int slot0 = (int)((p->q_start + p->q_size) % maxq);
+    uint32_t nb = (uint32_t)p->B;
+    if (nb > maxq) {
+        p->q_start = (p->q_start + nb - maxq) % maxq;
+        nb = maxq;
+    }
+    p->q_size += nb;
+    if (p->q_size > maxq) {
+        p->q_start = (p->q_start + p->q_size - maxq) % maxq;
+        p->q_size = maxq;
+    }
+    // insert_wait_time_transformation_size (pipe.cc:418-427)
+    const bool full = p->q_size >= maxq;
+    if (!full) {
+        return launch_propagate_to_ring(p->staging, (int)p->s.depth, p->B, (int)p->s.width, (int)p->s.height,
+                                        p->s.space_transformation, p->lens, p->ring, p->T, slot0, p->stream);
+    }
+    const bool pending = p->s.contrast_enabled && p->s.contrast_mode == HV_CONTRAST_REFERENCE &&
+                         p->autocontrast_pending;
+    if (p->graph_enabled && !pending) {
+        auto it = p->graphs.find(slot0);
+        if (it == p->graphs.end()) {
+            hipGraph_t g = nullptr;
+            HV_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+            hv_status st = p->enqueue_cycle(slot0);
+            hipError_t e = hipStreamEndCapture(p->stream, &g);
+            if (st != HV_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+            it = p->graphs.emplace(slot0, ex).first;
+        }
+        HV_HIP(hipGraphLaunch(it->second, p->stream));
+    } else {
+        HV_TRY(p->enqueue_cycle(slot0));
+    }
+    if (pending) p->autocontrast_pending = false;  // rendering.cc:251-258
+    p->have_output = true;
+    made = 1;
+    if (produced) *produced = made;
+    return HV_OK;
+}
+
+hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out_f32_contrast,
+                         const float** out_f32_image) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    if (out_u16) *out_u16 = p->out;
+    if (out_f32_contrast) *out_f32_contrast = p->s.contrast_enabled ? p->cimage : p->image;
+    if (out_f32_image) *out_f32_image = p->image;
+    return p->have_output ? HV_OK : HV_ERR_NOT_READY;
+}
+
+hv_status hv_pipe_contrast_range(hv_pipe* p, const float** range_dev) {
+    if (!p || !range_dev) return fail(HV_ERR_INVALID_ARG, "null argument");
+    *range_dev = p->range;
+    return HV_OK;
+}
+
+hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    if (ring) *ring = p->ring;
+    if (start) *start = p->q_start;
+    if (size) *size = p->q_size;
+    return HV_OK;
+}
+
+hv_status hv_pipe_set_graph(hv_pipe* p, int enabled) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    p->graph_enabled = enabled != 0;
+    return HV_OK;
+}
+
+hv_status hv_pipe_stream(hv_pipe* p, void** stream) {
+    if (!p || !stream) return fail(HV_ERR_INVALID_ARG, "null argument");
+    *stream = p->stream;
+    return HV_OK;
+}
+
+hv_status hv_pipe_sync(hv_pipe* p) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    HV_HIP(hipStreamSynchronize(p->stream));
+    return HV_OK;
+}
+
+hv_status hv_pipe_destroy(hv_pipe* p) {
+    if (!p) return HV_OK;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (p->base) (void)hipFree(p->base);
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return HV_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// server: several sources, admission by HBM budget, latency accounting
+// ===========================================================================
+namespace {
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+struct Inflight {
+    int source;
+    hipEvent_t done;
+    int64_t t_arrival;
+    bool produced;
+};
+}  // namespace
+
+struct hv_server {
+    size_t budget = 0, used = 0;
+    int next_id = 0;
+    std::map<int, hv_pipe*> sources;
+    std::map<int, size_t> bytes;
+    std::deque<Inflight> inflight;
+    std::vector<hipEvent_t> free_events;
+};
+
+extern "C" {
+
+hv_status hv_server_create(size_t budget_bytes, hv_server** out) {
+    if (!out) return fail(HV_ERR_INVALID_ARG, "out is null");
+    *out = new hv_server;
+    (*out)->budget = budget_bytes;
+    return HV_OK;
+}
+
+hv_status hv_server_add_source(hv_server* s, const hv_compute_settings* settings, int* source_id) {
+    if (!s || !source_id) return fail(HV_ERR_INVALID_ARG, "null argument");
+    size_t need = 0;
+    HV_TRY(hv_pipe_memory_bytes(settings, &need));
+    if (s->used + need > s->budget) return fail(HV_ERR_BUDGET, "source does not fit in the HBM budget");
+    hv_pipe* p = nullptr;
+    HV_TRY(hv_pipe_create(settings, nullptr, &p));
+    hv_pipe_set_graph(p, 1);
+    const int id = s->next_id++;
+    s->sources[id] = p;
+    s->bytes[id] = need;
+    s->used += need;
+    *source_id = id;
+    return HV_OK;
+}
+
+hv_status hv_server_remove_source(hv_server* s, int source_id) {
+    if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
+    auto it = s->sources.find(source_id);
+    if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
+    hv_pipe_sync(it->second);
+    // retire its in-flight entries
+    for (auto& f : s->inflight)
+        if (f.source == source_id) f.source = -1;
+    hv_pipe_destroy(it->second);
+    s->used -= s->bytes[source_id];
+    s->bytes.erase(source_id);
+    s->sources.erase(it);
+    return HV_OK;
+}
+
+hv_status hv_server_submit(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns) {
+    if (!s || !frames) return fail(HV_ERR_INVALID_ARG, "null argument");
+    auto it = s->sources.find(source_id);
+    if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
+    hv_pipe* p = it->second;
+    int produced = 0;
+    HV_TRY(hv_pipe_push_batch(p, frames, &produced));
+    hipEvent_t ev;
+    if (!s->free_events.empty()) {
+        ev = s->free_events.back();
+        s->free_events.pop_back();
+    } else {
+        HV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    HV_HIP(hipEventRecord(ev, p->stream));
+    s->inflight.push_back({source_id, ev, t_arrival_ns, produced != 0});
+    return HV_OK;
+}
+
+hv_status hv_server_poll(hv_server* s, int* sources, int64_t* latency_ns, int max_n, int* n_out) {
+    if (!s || !n_out) return fail(HV_ERR_INVALID_ARG, "null argument");
+    int n = 0;
+    // completion order may differ across sources: scan every in-flight batch
+    for (auto itr = s->inflight.begin(); itr != s->inflight.end() && n < max_n;) {
+        hipError_t q = hipEventQuery(itr->done);
+        if (q == hipErrorNotReady) {
+            ++itr;
+            continue;
+        }
+        if (q != hipSuccess) return hip_fail(q, "hipEventQuery");
+        const int64_t t = now_ns();
+        if (itr->produced && itr->source >= 0) {
+            if (sources) sources[n] = itr->source;
+            if (latency_ns) latency_ns[n] = t - itr->t_arrival;
+            ++n;
+        }
+        s->free_events.push_back(itr->done);
+        itr = s->inflight.erase(itr);
+    }
+    *n_out = n;
+    return HV_OK;
+}
+
+hv_status hv_server_drain(hv_server* s) {
+    if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
+    for (auto& kv : s->sources) HV_TRY(hv_pipe_sync(kv.second));
+    return HV_OK;
+}
+
+hv_status hv_server_output(hv_server* s, int source_id, const uint16_t** out_u16, const float** out_f32_image) {
+    if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
+    auto it = s->sources.find(source_id);
+    if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
+    return hv_pipe_output(it->second, out_u16, nullptr, out_f32_image);
+}
+
+hv_status hv_server_memory_bytes(hv_server* s, size_t* bytes) {
+    if (!s || !bytes) return fail(HV_ERR_INVALID_ARG, "null argument");
+    *bytes = s->used;
+    return HV_OK;
+}
+
+hv_status hv_server_destroy(hv_server* s) {
+    if (!s) return HV_OK;
+    for (auto& kv : s->sources) hv_pipe_destroy(kv.second);
+    for (auto& f : s->inflight) (void)hipEventDestroy(f.done);
+    for (auto e : s->free_events) (void)hipEventDestroy(e);
+    delete s;
+    return HV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,408 @@
+// hv_render.hip -- contrast, float -> u16, subsample, percentile, 2-D unwrap.
+//
+// Reductions are device-resident: the min/max of an image is kept as the
+// order-preserving uint32 encoding of two floats and updated with one
+// atomicMin/atomicMax per workgroup after a 64-lane shuffle reduction, so no
+// stage reads anything back to the host (the reference's rescale_float,
+// rescale_float_unwrap2d and compute_percentile all synchronise the stream).
+// Percentiles are exact order statistics found by a 3-pass radix select
+// (11 + 11 + 10 bits of the ordered key) instead of a full sort.
+#include <cfloat>
+
+#include "hv_fft.hpp"
+#include "hv_kernels.hpp"
+
+namespace hv {
+
+static unsigned blocks_for(size_t n, int threads, size_t cap = 16384) {
+    size_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------
+// min / max
+// ---------------------------------------------------------------------------
+__global__ void k_minmax_init(uint32_t* mm) {
+    if (threadIdx.x == 0) {
+        mm[0] = float_to_ordered(INFINITY);
+        mm[1] = float_to_ordered(-INFINITY);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_minmax(uint32_t* mm, const float* __restrict__ in, size_t n) {
+    float a = INFINITY, b = -INFINITY;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = in[i];
+        a = fminf(a, v);
+        b = fmaxf(b, v);
+    }
+    block_minmax_atomic(a, b, mm);
+}
+
+__global__ void k_minmax_decode(float* out, const uint32_t* mm) {
+    if (threadIdx.x == 0) {
+        const float a = ordered_to_float(mm[0]), b = ordered_to_float(mm[1]);
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+hv_status launch_minmax_init(uint32_t* mm, hipStream_t st) {
+    k_minmax_init<<<1, 64, 0, st>>>(mm);
+    HV_CHECK_LAUNCH("k_minmax_init");
+    return HV_OK;
+}
+hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st) {
+    k_minmax<<<blocks_for(n, 256, 2048), 256, 0, st>>>(mm, in, n);
+    HV_CHECK_LAUNCH("k_minmax");
+    return HV_OK;
+}
+hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st) {
+    k_minmax_decode<<<1, 64, 0, st>>>(out, mm);
+    HV_CHECK_LAUNCH("k_minmax_decode");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// subsample_frame (frame_reshape.cu:278-305)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t subsample_index(size_t index, int in_w, int out_w, int step) {
+    const uint32_t x = index % out_w, y = index / out_w;
+    const uint32_t x_off = y % 2 == 0 ? 0 : step / 2;
+    const uint32_t y_off = x % 2 == 0 ? 0 : step / 2;
+    return (size_t)y * step * in_w + (size_t)y_off * in_w + (size_t)x * step + x_off;
+}
+
+__global__ void k_subsample(float* __restrict__ out, const float* __restrict__ in, int in_w, int out_w, int step,
+                            size_t n_out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_out) out[i] = in[subsample_index(i, in_w, out_w, step)];
+}
+
+hv_status launch_subsample_f32(float* out, const float* in, int width, int height, int step, hipStream_t st) {
+    const size_t n = (size_t)(width / step) * (height / step);
+    k_subsample<<<blocks_for(n, 256, 1u << 30), 256, 0, st>>>(out, in, width, width / step, step, n);
+    HV_CHECK_LAUNCH("k_subsample");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// contrast and float -> u16 (contrast_correction.cu:5-17, tools_conversion.cu:350-359)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2u16(float x) {
+    if (x <= 0.0f) return 0;
+    if (x >= 65536.0f) return 65535;  // uint(x) > 65535 (the cast saturates on the GPU)
+    return (uint16_t)(uint32_t)x;
+}
+
+__global__ void __launch_bounds__(256) k_apply_contrast(float* img, size_t n, float mn, float mx) {
+    const float factor = (unsigned short)65535 / (mx - mn + FLT_EPSILON);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        img[i] = factor * (img[i] - mn);
+}
+
+__global__ void __launch_bounds__(256) k_float_to_ushort(uint16_t* out, const float* in, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = f2u16(in[i]);
+}
+
+__global__ void __launch_bounds__(256)
+    k_contrast_u16(uint16_t* __restrict__ out, float* __restrict__ out_f32, const float* __restrict__ in, size_t n,
+                   const void* range, int encoded, int invert) {
+    float mn, mx;
+    if (encoded) {
+        mn = ordered_to_float(((const uint32_t*)range)[0]);
+        mx = ordered_to_float(((const uint32_t*)range)[1]);
+    } else {
+        mn = ((const float*)range)[0];
+        mx = ((const float*)range)[1];
+    }
+    if (invert) {
+        const float t = mn;
+        mn = mx;
+        mx = t;
+    }
+    const float factor = (unsigned short)65535 / (mx - mn + FLT_EPSILON);
+    const size_t n4 = n / 4;
+    const float4* in4 = reinterpret_cast<const float4*>(in);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        float4 v = in4[i];
+        v.x = factor * (v.x - mn);
+        v.y = factor * (v.y - mn);
+        v.z = factor * (v.z - mn);
+        v.w = factor * (v.w - mn);
+        if (out_f32) reinterpret_cast<float4*>(out_f32)[i] = v;
+        ushort4 u;
+        u.x = f2u16(v.x);
+        u.y = f2u16(v.y);
+        u.z = f2u16(v.z);
+        u.w = f2u16(v.w);
+        reinterpret_cast<ushort4*>(out)[i] = u;
+    }
+    for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = factor * (in[i] - mn);
+        if (out_f32) out_f32[i] = v;
+        out[i] = f2u16(v);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scale(float* img, size_t n, float f) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        img[i] = img[i] * f;
+}
+
+hv_status launch_apply_contrast(float* img, size_t n, float mn, float mx, hipStream_t st) {
+    k_apply_contrast<<<blocks_for(n, 256), 256, 0, st>>>(img, n, mn, mx);
+    HV_CHECK_LAUNCH("k_apply_contrast");
+    return HV_OK;
+}
+hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st) {
+    k_float_to_ushort<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n);
+    HV_CHECK_LAUNCH("k_float_to_ushort");
+    return HV_OK;
+}
+hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
+                                    int encoded, int invert, hipStream_t st) {
+    k_contrast_u16<<<blocks_for((n + 3) / 4, 256), 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert);
+    HV_CHECK_LAUNCH("k_contrast_u16");
+    return HV_OK;
+}
+hv_status launch_scale(float* img, size_t n, float factor, hipStream_t st) {
+    k_scale<<<blocks_for(n, 256), 256, 0, st>>>(img, n, factor);
+    HV_CHECK_LAUNCH("k_scale");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// percentiles: exact radix select on ordered keys
+// ---------------------------------------------------------------------------
+constexpr int kPctBins = 2048;
+struct PctScratch {
+    uint32_t hist[2][kPctBins];
+    uint32_t prefix[2];
+    uint32_t rank[2];
+};
+size_t percentile_scratch_bytes() { return sizeof(PctScratch); }
+
+__global__ void k_pct_init(PctScratch* s, uint32_t r0, uint32_t r1) {
+    for (int i = threadIdx.x; i < 2 * kPctBins; i += blockDim.x) (&s->hist[0][0])[i] = 0;
+    if (threadIdx.x == 0) {
+        s->prefix[0] = s->prefix[1] = 0;
+        s->rank[0] = r0;
+        s->rank[1] = r1;
+    }
+}
+
+// Histogram of digit [shift, shift + width) over keys whose higher bits equal
+// the running prefix of each of the two selections.
+__global__ void __launch_bounds__(256)
+    k_pct_hist(PctScratch* s, const float* __restrict__ img, int in_w, int out_w, int step, size_t n, int shift,
+               int width) {
+    __shared__ uint32_t h[2][kPctBins];
+    for (int i = threadIdx.x; i < 2 * kPctBins; i += blockDim.x) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t p0 = s->prefix[0], p1 = s->prefix[1];
+    const int hs = shift + width;
+    const uint32_t mask = (1u << width) - 1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t src = step > 1 ? subsample_index(i, in_w, out_w, step) : i;
+        const uint32_t key = float_to_ordered(img[src]);
+        const uint32_t digit = (key >> shift) & mask;
+        const uint32_t high = hs >= 32 ? 0u : (key >> hs);
+        if (high == p0) atomicAdd(&h[0][digit], 1u);
+        if (high == p1) atomicAdd(&h[1][digit], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * kPctBins; i += blockDim.x) {
+        const uint32_t v = (&h[0][0])[i];
+        if (v) atomicAdd(&(&s->hist[0][0])[i], v);
+    }
+}
+
+// One block per selection q: find the bin holding rank[q], extend the prefix,
+// reduce the rank, clear the histogram for the next pass.  On the last pass
+// write the selected value (clamped to >= 1 when requested).
+__global__ void __launch_bounds__(256)
+    k_pct_select(PctScratch* s, int width, int last, float* out, int clamp) {
+    const int q = blockIdx.x;
+    constexpr int PER = kPctBins / 256;
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t found_bin, found_before;
+    const int t = threadIdx.x;
+    uint32_t local[PER];
+    uint32_t sum = 0;
+    for (int k = 0; k < PER; ++k) {
+        local[k] = (t * PER + k) < (1 << width) ? s->hist[q][t * PER + k] : 0u;
+        sum += local[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    // inclusive Hillis-Steele scan of the 256 partial sums
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const uint32_t rank = s->rank[q];
+    const uint32_t before = t > 0 ? part[t - 1] : 0u;
+    if (rank >= before && rank < part[t]) {
+        uint32_t c = before;
+        for (int k = 0; k < PER; ++k) {
+            if (rank < c + local[k]) {
+                found_bin = t * PER + k;
+                found_before = c;
+                break;
+            }
+            c += local[k];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < PER; ++k) s->hist[q][t * PER + k] = 0;
+    if (t == 0) {
+        const uint32_t prefix = (s->prefix[q] << width) | found_bin;
+        s->prefix[q] = prefix;
+        s->rank[q] = rank - found_before;
+        if (last) {
+            float v = ordered_to_float(prefix);
+            if (clamp) v = v > 1.0f ? v : 1.0f;  // contrast_api.cc:139-140
+            out[q] = v;
+        }
+    }
+}
+
+hv_status launch_percentile(float* range, const float* img, int width, int height, int step, float lo, float hi,
+                            int clamp, void* scratch, hipStream_t st) {
+    if (step < 1) step = 1;
+    const int out_w = width / step, out_h = height / step;
+    const size_t n = (size_t)out_w * out_h;
+    if (n == 0) return fail(HV_ERR_INVALID_ARG, "empty image");
+    // compute_percentile (percentile.cu:84-91): index = uint(p / 100 * N), float arithmetic
+    auto rank_of = [n](float pct) {
+        const float f = pct / 100 * (float)n;
+        uint32_t r = f <= 0.f ? 0u : (uint32_t)f;
+        if (r >= n) r = (uint32_t)n - 1;  // p = 100 would read one past the end
+        return r;
+    };
+    PctScratch* s = reinterpret_cast<PctScratch*>(scratch);
+    k_pct_init<<<1, 256, 0, st>>>(s, rank_of(lo), rank_of(hi));
+    HV_CHECK_LAUNCH("k_pct_init");
+    const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
+    const unsigned g = blocks_for(n, 256 * 8, 1024);
+    for (int d = 0; d < 3; ++d) {
+        k_pct_hist<<<g, 256, 0, st>>>(s, img, width, out_w, step, n, shifts[d], widths[d]);
+        HV_CHECK_LAUNCH("k_pct_hist");
+        k_pct_select<<<2, 256, 0, st>>>(s, widths[d], d == 2, range, clamp);
+        HV_CHECK_LAUNCH("k_pct_select");
+    }
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rescale_float_unwrap2d (tools_conversion.cu:315-335), device min/max
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float* in, size_t n, const uint32_t* mm) {
+    const float mn = ordered_to_float(mm[0]), mx = ordered_to_float(mm[1]);
+    const float m = (unsigned short)65535;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float v = in[i];
+        out[i] = mn < 0.f ? (v + fabsf(mn)) / (fabsf(mn) + mx) * m : (v - mn) / (mx - mn) * m;
+    }
+}
+
+hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st) {
+    HV_TRY(launch_minmax_init(mm, st));
+    HV_TRY(launch_minmax(mm, in, n, st));
+    k_rescale_unwrap<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n, mm);
+    HV_CHECK_LAUNCH("k_rescale_unwrap");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// unwrap_2d (tools_unwrap.cu:269-369): periodic least-squares unwrapping
+// ---------------------------------------------------------------------------
+// k: the FFT frequency index of the circularly shifted fx/fy grids
+// (kernel_init_unwrap_2d + circ_shift by (-W/2, -H/2)).
+__device__ __forceinline__ float kfreq(int i, int n) { return (float)(i < n / 2 ? i : i - n); }
+
+size_t unwrap_scratch_bytes(int width, int height) { return (size_t)width * height * sizeof(float2) * 3; }
+
+__global__ void k_unwrap_init(float2* z, float2* g, const float* phase, size_t P) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float2 v = make_float2(cosf(phase[i]), sinf(phase[i]));
+    z[i] = v;
+    g[i] = v;
+}
+
+// gradient_unwrap_2d (:298-322): Gx = FFT(z) fx, Gy = FFT(z) fy
+__global__ void k_unwrap_grad(float2* gx, float2* gy, int width, int height) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= width) return;
+    const size_t i = (size_t)y * width + x;
+    const float2 v = gx[i];
+    const float fx = kfreq(x, width), fy = kfreq(y, height);
+    gx[i] = make_float2(v.x * fx, v.y * fx);
+    gy[i] = make_float2(v.x * fy, v.y * fy);
+}
+
+// (gx, gy) * 2 pi i, then eq_unwrap_2d (:324-351): z <- conj(z i), g *= z
+__global__ void k_unwrap_eq(float2* gx, float2* gy, const float2* z, size_t P) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float2 tpi = make_float2(0.f, (float)(2.0 * M_PI));
+    const float2 zz = z[i];
+    const float2 zi = make_float2(zz.x * 0.f - zz.y * 1.f, -(zz.x * 1.f + zz.y * 0.f));
+    gx[i] = cmul(cmul(gx[i], tpi), zi);
+    gy[i] = cmul(cmul(gy[i], tpi), zi);
+}
+
+// kernel_norm_ratio + kernel_add_complex_frames: G = Gx fx/|f|^2 + Gy fy/|f|^2, 0 at DC
+__global__ void k_unwrap_ratio(float2* gx, const float2* gy, int width, int height) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= width) return;
+    const size_t i = (size_t)y * width + x;
+    const float fx = kfreq(x, width), fy = kfreq(y, height);
+    const float norm = fx * fx + fy * fy;
+    float2 a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+    if (norm != 0) {
+        const float cx = fx / norm, cy = fy / norm;
+        a = make_float2(gx[i].x * cx, gx[i].y * cx);
+        b = make_float2(gy[i].x * cy, gy[i].y * cy);
+    }
+    gx[i] = cadd(a, b);
+}
+
+// kernel_unwrap2d_last_step (:197-203)
+__global__ void k_unwrap_last(float* out, const float2* g, size_t P) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P) out[i] = (float)((double)g[i].y / -(2.0 * M_PI));
+}
+
+hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st) {
+    const size_t P = (size_t)width * height;
+    float2* z = reinterpret_cast<float2*>(scratch);
+    float2* gx = z + P;
+    float2* gy = gx + P;  // gx, gy adjacent: batched FFT2 of 2 frames
+    const unsigned g1 = (unsigned)((P + 255) / 256);
+    const dim3 g2((width + 255) / 256, height);
+    k_unwrap_init<<<g1, 256, 0, st>>>(z, gx, phase, P);
+    HV_CHECK_LAUNCH("k_unwrap_init");
+    HV_TRY(launch_fft2(gx, 1, width, height, -1, st));
+    k_unwrap_grad<<<g2, 256, 0, st>>>(gx, gy, width, height);
+    HV_CHECK_LAUNCH("k_unwrap_grad");
+    HV_TRY(launch_fft2(gx, 2, width, height, +1, st));
+    k_unwrap_eq<<<g1, 256, 0, st>>>(gx, gy, z, P);
+    HV_CHECK_LAUNCH("k_unwrap_eq");
+    HV_TRY(launch_fft2(gx, 2, width, height, -1, st));
+    k_unwrap_ratio<<<g2, 256, 0, st>>>(gx, gy, width, height);
+    HV_CHECK_LAUNCH("k_unwrap_ratio");
+    HV_TRY(launch_fft2(gx, 1, width, height, +1, st));
+    k_unwrap_last<<<g1, 256, 0, st>>>(out, gx, P);
+    HV_CHECK_LAUNCH("k_unwrap_last");
+    return HV_OK;
+}
+
+}  // namespace hv

@@ -1,0 +1,388 @@
+// hv_spatial.hip -- input conversion, lenses and the space transforms.
+//
+// A batched FFT2 is two HBM passes: a row pass (transforms along x, each
+// workgroup owns M whole rows) and a column pass (transforms along y, each
+// workgroup owns a CW-column tile; lanes run along x so every load/store is a
+// run of CW consecutive complex values).  The angular spectrum
+// (angular_spectrum.cu:33-63) is three passes:
+//   1. row FFT fused with the u8/u16/f32 -> complex conversion
+//      (tools_conversion.cu:107-171), written straight into the frame's
+//      time-window ring slot;
+//   2. column FFT -> x lens/(W H) -> inverse column FFT, in LDS;
+//   3. inverse row FFT, in place in the ring slot.
+// The Fresnel transform (fresnel_transform.cu:65-83) is two: the chirp is
+// applied in the row pass's loader.  No pass needs a separate buffer.
+#include "hv_fft.hpp"
+#include "hv_kernels.hpp"
+
+namespace hv {
+
+// ---------------------------------------------------------------------------
+// conversion (tools_conversion.cu:107-171)
+// ---------------------------------------------------------------------------
+template <typename TIn>
+__device__ __forceinline__ float convert_px(TIn v);
+template <>
+__device__ __forceinline__ float convert_px<uint8_t>(uint8_t v) { return static_cast<float>(v) * 257; }
+template <>
+__device__ __forceinline__ float convert_px<uint16_t>(uint16_t v) { return static_cast<float>(v); }
+template <>
+__device__ __forceinline__ float convert_px<float>(float v) { return v; }
+
+template <typename TIn>
+__global__ void __launch_bounds__(256) k_convert(float2* __restrict__ out, const TIn* __restrict__ in, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = make_float2(convert_px<TIn>(in[i]), 0.0f);
+}
+
+static int grid_for(size_t n, int threads, int max_blocks = 8192) {
+    size_t b = (n + threads - 1) / threads;
+    return (int)(b < (size_t)max_blocks ? (b > 0 ? b : 1) : max_blocks);
+}
+
+hv_status launch_convert(float2* out, const void* in, size_t n, int depth, hipStream_t st) {
+    const int g = grid_for(n, 256);
+    if (depth == HV_DEPTH_8) k_convert<uint8_t><<<g, 256, 0, st>>>(out, (const uint8_t*)in, n);
+    else if (depth == HV_DEPTH_16) k_convert<uint16_t><<<g, 256, 0, st>>>(out, (const uint16_t*)in, n);
+    else if (depth == HV_DEPTH_32F) k_convert<float><<<g, 256, 0, st>>>(out, (const float*)in, n);
+    else return fail(HV_ERR_INVALID_ARG, "depth must be 1, 2 or 4");
+    HV_CHECK_LAUNCH("k_convert");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// lenses (masks.cu:7-59, fresnel_transform.cu:14-63)
+// ---------------------------------------------------------------------------
+// Angular-spectrum transfer function.  `shift` writes element (x, y) at its
+// corner-shifted position; `scale` multiplies the result (1/(W H) folds the
+// inverse-FFT normalisation of angular_spectrum.cu:61 into the lens; the
+// scale is a power of two, so the product is exact).
+__global__ void k_lens_as(float2* __restrict__ out, int nx, int ny, float z, float lambda, float x_step,
+                          float y_step, int precision, int shift, float scale) {
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= nx || y >= ny) return;
+    float c, s;
+    if (precision == HV_LENS_REFERENCE_FP32) {
+        // the reference's float expression order (masks.cu:45-57)
+        float u_step = 1.0f / (nx * x_step);
+        float v_step = 1.0f / (ny * y_step);
+        float u = (x - (nx >> 1)) * u_step;
+        float v = (y - (ny >> 1)) * v_step;
+        float tmp = 1.0f - (lambda * lambda * (u * u + v * v));
+        if (tmp < 0.0f) tmp = 0.0f;
+        float phase = 2.0f * M_PI * z / lambda * sqrtf(tmp);
+        c = cosf(phase);
+        s = sinf(phase);
+    } else {
+        double u = (double)(x - (nx >> 1)) / ((double)nx * (double)x_step);
+        double v = (double)(y - (ny >> 1)) / ((double)ny * (double)y_step);
+        double lam = (double)lambda;
+        double tmp = 1.0 - lam * lam * (u * u + v * v);
+        if (tmp < 0.0) tmp = 0.0;
+        double phase = 2.0 * M_PI * (double)z / lam * sqrt(tmp);
+        c = (float)cos(phase);
+        s = (float)sin(phase);
+    }
+    int ox = x, oy = y;
+    if (shift) {  // kernel_shift_corners for even sizes: (x, y) <-> (x +- nx/2, y +- ny/2)
+        ox = (x + nx / 2) % nx;
+        oy = (y + ny / 2) % ny;
+    }
+    out[(size_t)oy * nx + ox] = make_float2(c * scale, s * scale);
+}
+
+// Fresnel chirp on a square of side `side`, written cropped to width x height.
+__global__ void k_lens_fresnel(float2* __restrict__ out, int side, int width, int height, float lambda, float z,
+                               float pixel_size, int precision) {
+#pragma clang fp contract(off)
+    const int xo = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yo = blockIdx.y;
+    if (xo >= width || yo >= height) return;
+    const int i = xo + (side - width) / 2, j = yo + (side - height) / 2;
+    float c, s;
+    if (precision == HV_LENS_REFERENCE_FP32) {
+        const float cc = M_PI / (lambda * z);
+        const float dx = pixel_size * 1.0e-6f;
+        const float dy = dx;
+        float x = (i - static_cast<float>(side >> 1)) * dx;
+        float y = (j - static_cast<float>(side >> 1)) * dy;
+        float csq = cc * (x * x + y * y);
+        c = cosf(csq);
+        s = sinf(csq);
+    } else {
+        double cc = M_PI / ((double)lambda * (double)z);
+        double dx = (double)pixel_size * 1.0e-6;
+        double x = (i - (double)(side >> 1)) * dx, y = (j - (double)(side >> 1)) * dx;
+        double ph = cc * (x * x + y * y);
+        c = (float)cos(ph);
+        s = (float)sin(ph);
+    }
+    out[(size_t)yo * width + xo] = make_float2(c, s);
+}
+
+hv_status launch_lens_as(float2* out, int nx, int ny, float z, float lambda, float xs, float ys, int precision,
+                         int shift, float scale, hipStream_t st) {
+    dim3 grid((nx + 255) / 256, ny);
+    k_lens_as<<<grid, 256, 0, st>>>(out, nx, ny, z, lambda, xs, ys, precision, shift, scale);
+    HV_CHECK_LAUNCH("k_lens_as");
+    return HV_OK;
+}
+
+hv_status launch_lens_fresnel(float2* out, int side, int width, int height, float lambda, float z, float px,
+                              int precision, hipStream_t st) {
+    dim3 grid((width + 255) / 256, height);
+    k_lens_fresnel<<<grid, 256, 0, st>>>(out, side, width, height, lambda, z, px, precision);
+    HV_CHECK_LAUNCH("k_lens_fresnel");
+    return HV_OK;
+}
+
+// kernel_shift_corners (shift_corners.cu:8-44): in-place swap of (i, j),
+// j < sy/2, with (i +- sx/2, j + sy/2).
+template <typename T>
+__global__ void k_shift_corners(T* buf, int batch, int sx, int sy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= sx || j >= sy / 2) return;
+    const int sx2 = sx / 2, sy2 = sy / 2;
+    const int ni = i < sx2 ? i + sx2 : i - sx2, nj = j + sy2;
+    const size_t P = (size_t)sx * sy;
+    for (int b = 0; b < batch; ++b) {
+        T* f = buf + b * P;
+        T a = f[(size_t)j * sx + i];
+        f[(size_t)j * sx + i] = f[(size_t)nj * sx + ni];
+        f[(size_t)nj * sx + ni] = a;
+    }
+}
+
+hv_status launch_shift_corners_c64(float2* buf, int batch, int sx, int sy, hipStream_t st) {
+    dim3 grid((sx + 255) / 256, sy / 2);
+    k_shift_corners<float2><<<grid, 256, 0, st>>>(buf, batch, sx, sy);
+    HV_CHECK_LAUNCH("k_shift_corners");
+    return HV_OK;
+}
+hv_status launch_shift_corners_f32(float* buf, int batch, int sx, int sy, hipStream_t st) {
+    dim3 grid((sx + 255) / 256, sy / 2);
+    k_shift_corners<float><<<grid, 256, 0, st>>>(buf, batch, sx, sy);
+    HV_CHECK_LAUNCH("k_shift_corners");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// row pass
+// ---------------------------------------------------------------------------
+// Input kinds of the row pass.
+enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
+
+template <int N>
+struct RowCfg {
+    static constexpr int E = N >= 1024 ? 16 : 8;
+    static constexpr int NT = FftShape<N, E>::NT;
+    static constexpr int M = (256 / NT) > 0 ? (256 / NT) : 1;  // rows per workgroup
+    static constexpr int THREADS = M * NT;
+    static constexpr int PITCH = N + N / 8;                      // padded LDS row
+};
+
+// Frames of the batch map to output frame (slot0 + b) mod ring (ring > 0), or
+// to frame b of `out` (ring == 0).  Input frame b is frame b of `in`.
+struct FrameMap {
+    int ring;
+    int slot0;
+    __device__ __forceinline__ size_t out_frame(int b) const {
+        return ring > 0 ? (size_t)((slot0 + b) % ring) : (size_t)b;
+    }
+};
+
+template <int N, int DIR, int KIN>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+    k_row_fft(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
+              FrameMap fin, FrameMap fm, float scale) {
+    using C = RowCfg<N>;
+    __shared__ float2 lds[C::M * C::PITCH];
+    const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    const int y = blockIdx.x * C::M + m;
+    const int b = blockIdx.y;
+    const size_t P = (size_t)N * height;
+    const size_t in_off = fin.out_frame(b) * P + (size_t)y * N;
+    float2* dst = out + fm.out_frame(b) * P + (size_t)y * N;
+    const float2* crow = chirp ? chirp + (size_t)y * N : nullptr;
+    float2* my = lds + m * C::PITCH;
+    auto load = [&](int n) -> float2 {
+        float2 v;
+        if constexpr (KIN == IN_U8) v = make_float2(convert_px<uint8_t>(((const uint8_t*)in)[in_off + n]), 0.f);
+        else if constexpr (KIN == IN_U16) v = make_float2(convert_px<uint16_t>(((const uint16_t*)in)[in_off + n]), 0.f);
+        else if constexpr (KIN == IN_F32) v = make_float2(((const float*)in)[in_off + n], 0.f);
+        else v = ((const float2*)in)[in_off + n];
+        if (crow) v = cmul(v, crow[n]);
+        return v;
+    };
+    auto store = [&](int n, float2 v) { dst[n] = scale == 1.0f ? v : cscale(v, scale); };
+    auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
+    fft_run<N, C::E, DIR>(t, load, store, lds_at);
+}
+
+// ---------------------------------------------------------------------------
+// column pass
+// ---------------------------------------------------------------------------
+enum { COL_FWD = 0, COL_INV = 1, COL_LENS = 2 };
+
+template <int N>
+struct ColCfg {
+    static constexpr int E = N >= 1024 ? 16 : 8;
+    static constexpr int NT = FftShape<N, E>::NT;
+    static constexpr int CW = N <= 512 ? 16 : (N == 1024 ? 8 : 4);  // columns per workgroup
+    static constexpr int THREADS = NT * CW;
+};
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+    k_col_fft(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ lens, int width,
+              FrameMap fin, FrameMap fm) {
+    using C = ColCfg<N>;
+    __shared__ float2 lds[N * C::CW];
+    const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
+    const int x = blockIdx.x * C::CW + c;
+    const int b = blockIdx.y;
+    const size_t P = (size_t)N * width;
+    const float2* src = in + fin.out_frame(b) * P + x;
+    float2* dst = out + fm.out_frame(b) * P + x;
+    auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
+    auto load = [&](int n) -> float2 { return src[(size_t)n * width]; };
+    if constexpr (MODE == COL_LENS) {
+        // forward column FFT, x lens (already / (W H)), inverse column FFT
+        auto store_lds = [&](int n, float2 v) { lds_at(n) = cmul(v, lens[(size_t)n * width + x]); };
+        auto load_lds = [&](int n) -> float2 { return lds_at(n); };
+        auto store = [&](int n, float2 v) { dst[(size_t)n * width] = v; };
+        fft_run<N, C::E, -1, true>(t, load, store_lds, lds_at);
+        __syncthreads();
+        fft_run<N, C::E, +1>(t, load_lds, store, lds_at);
+    } else {
+        auto store = [&](int n, float2 v) { dst[(size_t)n * width] = v; };
+        fft_run<N, C::E, MODE == COL_FWD ? -1 : +1>(t, load, store, lds_at);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <int N, int DIR>
+static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* chirp, int height, int batch,
+                            FrameMap fin, FrameMap fm, float scale, hipStream_t st) {
+    using C = RowCfg<N>;
+    dim3 grid(height / C::M, batch);
+    if (height % C::M) return fail(HV_ERR_UNSUPPORTED, "height not a multiple of the rows per workgroup");
+    switch (kin) {
+        case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
+        case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
+        case IN_F32: k_row_fft<N, DIR, IN_F32><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
+        default: k_row_fft<N, DIR, IN_C64><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
+    }
+    HV_CHECK_LAUNCH("k_row_fft");
+    return HV_OK;
+}
+
+template <int DIR>
+static hv_status row_pass(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
+                          int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st) {
+    switch (width) {
+        case 16: return row_pass_n<16, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 32: return row_pass_n<32, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 64: return row_pass_n<64, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 128: return row_pass_n<128, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 256: return row_pass_n<256, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 512: return row_pass_n<512, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 1024: return row_pass_n<1024, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 2048: return row_pass_n<2048, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 4096: return row_pass_n<4096, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "width must be a power of two in [16, 4096]");
+    }
+}
+
+template <int N, int MODE>
+static hv_status col_pass_n(const float2* in, float2* out, const float2* lens, int width, int batch,
+                            FrameMap fin, FrameMap fm, hipStream_t st) {
+    using C = ColCfg<N>;
+    if (width % C::CW) return fail(HV_ERR_UNSUPPORTED, "width not a multiple of the column tile");
+    dim3 grid(width / C::CW, batch);
+    k_col_fft<N, MODE><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm);
+    HV_CHECK_LAUNCH("k_col_fft");
+    return HV_OK;
+}
+
+template <int MODE>
+static hv_status col_pass(const float2* in, float2* out, const float2* lens, int width, int height, int batch,
+                          FrameMap fin, FrameMap fm, hipStream_t st) {
+    switch (height) {
+        case 16: return col_pass_n<16, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 32: return col_pass_n<32, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 64: return col_pass_n<64, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 128: return col_pass_n<128, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 256: return col_pass_n<256, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 512: return col_pass_n<512, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 1024: return col_pass_n<1024, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 2048: return col_pass_n<2048, MODE>(in, out, lens, width, batch, fin, fm, st);
+        case 4096: return col_pass_n<4096, MODE>(in, out, lens, width, batch, fin, fm, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "height must be a power of two in [16, 4096]");
+    }
+}
+
+static int depth_to_kin(int depth) {
+    return depth == HV_DEPTH_8 ? IN_U8 : depth == HV_DEPTH_16 ? IN_U16 : depth == HV_DEPTH_32F ? IN_F32 : -1;
+}
+
+hv_status launch_fft2(float2* buf, int batch, int width, int height, int direction, hipStream_t st) {
+    FrameMap fm{0, 0};
+    if (direction < 0) {
+        HV_TRY(row_pass<-1>(buf, IN_C64, buf, nullptr, width, height, batch, fm, fm, 1.0f, st));
+        return col_pass<COL_FWD>(buf, buf, nullptr, width, height, batch, fm, fm, st);
+    }
+    HV_TRY(row_pass<+1>(buf, IN_C64, buf, nullptr, width, height, batch, fm, fm, 1.0f, st));
+    return col_pass<COL_INV>(buf, buf, nullptr, width, height, batch, fm, fm, st);
+}
+
+hv_status launch_angular_spectrum(float2* buf, int batch, const float2* lens, int width, int height,
+                                  float post_scale, hipStream_t st) {
+    FrameMap fm{0, 0};
+    HV_TRY(row_pass<-1>(buf, IN_C64, buf, nullptr, width, height, batch, fm, fm, 1.0f, st));
+    HV_TRY(col_pass<COL_LENS>(buf, buf, lens, width, height, batch, fm, fm, st));
+    return row_pass<+1>(buf, IN_C64, buf, nullptr, width, height, batch, fm, fm, post_scale, st);
+}
+
+hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width, int height, hipStream_t st) {
+    FrameMap fm{0, 0};
+    HV_TRY(row_pass<-1>(buf, IN_C64, buf, chirp, width, height, batch, fm, fm, 1.0f, st));
+    return col_pass<COL_FWD>(buf, buf, nullptr, width, height, batch, fm, fm, st);
+}
+
+hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
+                                   const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st) {
+    const int kin = depth_to_kin(depth);
+    if (kin < 0) return fail(HV_ERR_INVALID_ARG, "depth must be 1, 2 or 4");
+    const size_t P = (size_t)width * height;
+    // Only the last ring_size frames of an oversized batch survive the enqueue
+    // (queue.cc:337-343); the earlier ones are skipped.
+    const int skip = batch > ring_size ? batch - ring_size : 0;
+    const char* src = (const char*)raw + (size_t)skip * P * depth;
+    const int nb = batch - skip;
+    const FrameMap plain{0, 0};
+    const FrameMap slots{ring_size, (slot0 + skip) % ring_size};
+    if (space == HV_SPACE_ANGULARSP) {
+        // pass 1 writes the frames into their ring slots; passes 2-3 run in place there
+        HV_TRY(row_pass<-1>(src, kin, ring, nullptr, width, height, nb, plain, slots, 1.0f, st));
+        HV_TRY(col_pass<COL_LENS>(ring, ring, lens, width, height, nb, slots, slots, st));
+        return row_pass<+1>(ring, IN_C64, ring, nullptr, width, height, nb, slots, slots, 1.0f, st);
+    }
+    if (space == HV_SPACE_FRESNELTR) {
+        HV_TRY(row_pass<-1>(src, kin, ring, lens, width, height, nb, plain, slots, 1.0f, st));
+        return col_pass<COL_FWD>(ring, ring, nullptr, width, height, nb, slots, slots, st);
+    }
+    // no space transform: conversion only, frame by frame into the slots
+    for (int b = 0; b < nb; ++b) {
+        float2* dst = ring + (size_t)((slots.slot0 + b) % ring_size) * P;
+        HV_TRY(launch_convert(dst, src + (size_t)b * P * depth, P, depth, st));
+    }
+    return HV_OK;
+}
+
+}  // namespace hv

@@ -1,0 +1,258 @@
+// hv_time.hip -- time transforms (STFT) fused with the image-type epilogue.
+//
+// The time window is the ring of T frames (T x P complex, frame-major, slots
+// in physical order as the reference's Queue stores them, queue.cc:332-376).
+// The STFT (stft.cu:9-15: a batched length-T FFT with stride P) runs per
+// pixel: lanes take consecutive pixels, so every load of slot t is a
+// contiguous run.  For T <= 64 one thread holds the pixel's whole window in
+// registers; for T = 128, 256 a group of threads shares it through LDS.
+// The epilogue turns the spectrum into the image without writing the T x P
+// spectrum to HBM: modulus / squared modulus / argument mean over [pmin, pmax]
+// (tools_conversion.cu:11-30, 84-103, 199-232) or the moments
+// (tools_conversion.cu:33-66, tools_compute.cu:38-97), plus the image's
+// min/max for the contrast stage.
+#include "hv_fft.hpp"
+#include "hv_kernels.hpp"
+
+namespace hv {
+
+enum { TM_IMAGE = 0, TM_FULL = 1, TM_MOMENTS = 2 };
+
+struct TimeParams {
+    float* out;        // image (TM_IMAGE) or M0 (TM_MOMENTS)
+    float* m1;         // TM_MOMENTS
+    float* m2;         // TM_MOMENTS
+    float2* spec;      // TM_FULL
+    int img_type;
+    int pmin, pmax;    // inclusive range (image) or [f_start, f_end] (moments)
+    float fps;         // moments frequency scale (CameraFps)
+    uint32_t* mm;      // encoded min/max of the image, may be null
+    int mm_moment;     // which moment feeds mm (0, 1, 2)
+};
+
+// Image-type contribution of one spectrum value (tools_conversion.cu:20-24, 96-100, 222-226).
+__device__ __forceinline__ float image_term(float2 z, int img_type) {
+    if (img_type == HV_IMG_MODULUS) return hypotf(z.x, z.y);
+    if (img_type == HV_IMG_SQUARED_MODULUS) {
+        const float h = hypotf(z.x, z.y);
+        return h * h;
+    }
+    return atan2f(z.y, z.x);
+}
+
+// fft_freqs (icompute.cc:26-69): f1[i] = i d for i < T/2, -(T - i) d above
+// (even T), d = fps / T, in float as the reference computes it.
+template <int T>
+__device__ __forceinline__ float freq1(int i, float d) {
+    if constexpr (T % 2 == 0) return i < T / 2 ? i * d : -((float)T - i) * d;
+    else return i < (T + 1) / 2 ? i * d : (i - (float)T) * d;
+}
+
+// Moments of one pixel from its spectrum (tools_compute.cu:38-97 applied to
+// |S_f|^2 from tools_conversion.cu:33-66).
+template <int T, class Spec>
+__device__ __forceinline__ void pixel_moments(Spec&& spec, int fs, int fe, float fps, float& a0, float& a1,
+                                              float& a2) {
+    const float d = fps / T;
+    a0 = a1 = a2 = 0.f;
+    for (int i = fs; i <= fe; ++i) {
+        const float2 z = spec(i);
+        const float s = z.x * z.x + z.y * z.y;
+        const float f = freq1<T>(i, d);
+        a0 += s * 1.0f;
+        a1 += s * f;
+        a2 += s * (f * f);
+    }
+    constexpr int nyq = T / 2;
+    if (T % 2 == 0 && nyq >= fs && nyq <= fe) {
+        const float2 z = spec(nyq);
+        const float s = z.x * z.x + z.y * z.y;
+        const float f = freq1<T>(nyq, d);
+        a0 += s * 1.0f;
+        a1 -= s * f;
+        a2 += s * (f * f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// T <= 64: one thread per pixel, whole window in registers
+// ---------------------------------------------------------------------------
+template <int T, int MODE>
+__global__ void __launch_bounds__(256) k_time_reg(const float2* __restrict__ ring, size_t P, TimeParams prm) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < P;
+    float2 v[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) v[t] = valid ? ring[(size_t)t * P + p] : make_float2(0.f, 0.f);
+    if constexpr (T > 1) fft_reg<T, -1>(v);
+    float vmin = INFINITY, vmax = -INFINITY;
+    if constexpr (MODE == TM_FULL) {
+        if (valid) {
+#pragma unroll
+            for (int t = 0; t < T; ++t) prm.spec[(size_t)t * P + p] = v[t];
+        }
+    } else if constexpr (MODE == TM_IMAGE) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < T; ++k)
+            if (k >= prm.pmin && k <= prm.pmax) acc += image_term(v[k], prm.img_type);
+        const float val = acc / (prm.pmax - prm.pmin + 1);
+        if (valid) {
+            prm.out[p] = val;
+            vmin = vmax = val;
+        }
+    } else {
+        // registers are indexed with compile-time k only: select from the unrolled loop
+        auto spec = [&](int i) -> float2 {
+            float2 r = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < T; ++k)
+                if (k == i) r = v[k];
+            return r;
+        };
+        float a0, a1, a2;
+        pixel_moments<T>(spec, prm.pmin, prm.pmax, prm.fps, a0, a1, a2);
+        if (valid) {
+            if (prm.out) prm.out[p] = a0;
+            if (prm.m1) prm.m1[p] = a1;
+            if (prm.m2) prm.m2[p] = a2;
+            const float sel = prm.mm_moment == 0 ? a0 : prm.mm_moment == 1 ? a1 : a2;
+            vmin = vmax = sel;
+        }
+    }
+    if (MODE != TM_FULL && prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
+}
+
+// ---------------------------------------------------------------------------
+// T = 128, 256: CW pixels per workgroup, NT threads per pixel, LDS exchange
+// ---------------------------------------------------------------------------
+template <int T>
+struct TimeLdsCfg {
+    static constexpr int E = 16;
+    static constexpr int NT = FftShape<T, E>::NT;
+    static constexpr int CW = 64;
+    static constexpr int THREADS = NT * CW;
+};
+
+template <int T, int MODE>
+__global__ void __launch_bounds__(TimeLdsCfg<T>::THREADS)
+    k_time_lds(const float2* __restrict__ ring, size_t P, TimeParams prm) {
+    using C = TimeLdsCfg<T>;
+    __shared__ float2 lds[T * C::CW];
+    const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
+    const size_t p = (size_t)blockIdx.x * C::CW + c;
+    const bool valid = p < P;
+    auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
+    auto load = [&](int n) -> float2 { return valid ? ring[(size_t)n * P + p] : make_float2(0.f, 0.f); };
+    float vmin = INFINITY, vmax = -INFINITY;
+    if constexpr (MODE == TM_FULL) {
+        auto store = [&](int n, float2 z) {
+            if (valid) prm.spec[(size_t)n * P + p] = z;
+        };
+        fft_run<T, C::E, -1>(t, load, store, lds_at);
+    } else {
+        auto store = [&](int n, float2 z) { lds_at(n) = z; };
+        fft_run<T, C::E, -1, true>(t, load, store, lds_at);
+        __syncthreads();
+        if (t == 0 && valid) {
+            if constexpr (MODE == TM_IMAGE) {
+                float acc = 0.f;
+                for (int k = prm.pmin; k <= prm.pmax; ++k) acc += image_term(lds_at(k), prm.img_type);
+                const float val = acc / (prm.pmax - prm.pmin + 1);
+                prm.out[p] = val;
+                vmin = vmax = val;
+            } else {
+                float a0, a1, a2;
+                pixel_moments<T>([&](int i) { return lds_at(i); }, prm.pmin, prm.pmax, prm.fps, a0, a1, a2);
+                if (prm.out) prm.out[p] = a0;
+                if (prm.m1) prm.m1[p] = a1;
+                if (prm.m2) prm.m2[p] = a2;
+                const float sel = prm.mm_moment == 0 ? a0 : prm.mm_moment == 1 ? a1 : a2;
+                vmin = vmax = sel;
+            }
+        }
+        if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
+    }
+}
+
+template <int T, int MODE>
+static hv_status time_launch_t(const float2* ring, size_t P, const TimeParams& prm, hipStream_t st) {
+    if constexpr (T <= 64) {
+        const unsigned g = (unsigned)((P + 255) / 256);
+        k_time_reg<T, MODE><<<g, 256, 0, st>>>(ring, P, prm);
+    } else {
+        using C = TimeLdsCfg<T>;
+        const unsigned g = (unsigned)((P + C::CW - 1) / C::CW);
+        k_time_lds<T, MODE><<<g, C::THREADS, 0, st>>>(ring, P, prm);
+    }
+    HV_CHECK_LAUNCH("k_time");
+    return HV_OK;
+}
+
+template <int MODE>
+static hv_status time_launch(const float2* ring, int T, size_t P, const TimeParams& prm, hipStream_t st) {
+    switch (T) {
+        case 1: return time_launch_t<1, MODE>(ring, P, prm, st);
+        case 2: return time_launch_t<2, MODE>(ring, P, prm, st);
+        case 4: return time_launch_t<4, MODE>(ring, P, prm, st);
+        case 8: return time_launch_t<8, MODE>(ring, P, prm, st);
+        case 16: return time_launch_t<16, MODE>(ring, P, prm, st);
+        case 32: return time_launch_t<32, MODE>(ring, P, prm, st);
+        case 64: return time_launch_t<64, MODE>(ring, P, prm, st);
+        case 128: return time_launch_t<128, MODE>(ring, P, prm, st);
+        case 256: return time_launch_t<256, MODE>(ring, P, prm, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
+    }
+}
+
+hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_t st) {
+    TimeParams prm{};
+    prm.spec = out;
+    return time_launch<TM_FULL>(in, T, P, prm, st);
+}
+
+hv_status launch_stft_image(float* out, const float2* ring, int T, size_t P, int img_type, int pmin, int pmax,
+                            uint32_t* mm, hipStream_t st) {
+    TimeParams prm{};
+    prm.out = out;
+    prm.img_type = img_type;
+    prm.pmin = pmin;
+    prm.pmax = pmax;
+    prm.mm = mm;
+    return time_launch<TM_IMAGE>(ring, T, P, prm, st);
+}
+
+hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* ring, int T, size_t P, int f_start,
+                              int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st) {
+    TimeParams prm{};
+    prm.out = m0;
+    prm.m1 = m1;
+    prm.m2 = m2;
+    prm.pmin = f_start;
+    prm.pmax = f_end;
+    prm.fps = fps;
+    prm.mm = mm;
+    prm.mm_moment = mm_moment;
+    return time_launch<TM_MOMENTS>(ring, T, P, prm, st);
+}
+
+// complex_to_modulus / squared_modulus / argument on a stored spectrum
+// (tools_conversion.cu:11-30, 84-103, 199-232).
+__global__ void __launch_bounds__(256)
+    k_complex_to_image(float* __restrict__ out, const float2* __restrict__ in, int img_type, int pmin, int pmax,
+                       size_t P) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float acc = 0.f;
+    for (int k = pmin; k <= pmax; ++k) acc += image_term(in[(size_t)k * P + p], img_type);
+    out[p] = acc / (pmax - pmin + 1);
+}
+
+hv_status launch_complex_to_image(float* out, const float2* in, int img_type, int pmin, int pmax, size_t P,
+                                  hipStream_t st) {
+    k_complex_to_image<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(out, in, img_type, pmin, pmax, P);
+    HV_CHECK_LAUNCH("k_complex_to_image");
+    return HV_OK;
+}
+
+}  // namespace hv

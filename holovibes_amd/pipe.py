@@ -1,0 +1,195 @@
+"""Python face of the per-batch Pipe (Backend/sources/core/pipe.cc) and of
+the multi-source server.  The orchestration itself is C++ (hv_pipe.cpp);
+this module only owns handles and copies results into torch tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ComputeSettings
+
+_hip = None
+_D2D, _D2H, _H2D = 3, 2, 1
+
+
+def hip():
+    """The HIP runtime (already loaded by torch) for plain copies."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                        ctypes.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    return _hip
+
+
+def copy_d2d(dst: torch.Tensor, src_ptr: int, nbytes: int, stream) -> None:
+    e = hip().hipMemcpyAsync(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src_ptr), nbytes, _D2D, stream)
+    if e != 0:
+        raise RuntimeError(f"hipMemcpyAsync failed ({e})")
+
+
+class Pipe:
+    """One source's compute pipe.  `settings` is a ComputeSettings; the pipe
+    runs on torch's current stream unless `own_stream`."""
+
+    def __init__(self, settings: ComputeSettings, own_stream: bool = False, graph: bool = False):
+        self.settings = settings
+        self.h = ctypes.c_void_p()
+        stream = None if own_stream else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.call("hv_pipe_create", ctypes.byref(settings), stream, ctypes.byref(self.h))
+        st = ctypes.c_void_p()
+        _lib.call("hv_pipe_stream", self.h, ctypes.byref(st))
+        self.stream = st
+        if graph:
+            _lib.call("hv_pipe_set_graph", self.h, 1)
+        self.P = settings.width * settings.height
+
+    @staticmethod
+    def memory_bytes(settings: ComputeSettings) -> int:
+        n = ctypes.c_size_t()
+        _lib.call("hv_pipe_memory_bytes", ctypes.byref(settings), ctypes.byref(n))
+        return n.value
+
+    def input_buffer(self) -> int:
+        p = ctypes.c_void_p()
+        _lib.call("hv_pipe_input_buffer", self.h, ctypes.byref(p))
+        return p.value
+
+    def push(self, frames) -> bool:
+        """frames: device tensor (batch_size, H, W) or a raw device pointer."""
+        ptr = frames.data_ptr() if isinstance(frames, torch.Tensor) else int(frames)
+        produced = ctypes.c_int()
+        _lib.call("hv_pipe_push_batch", self.h, ctypes.c_void_p(ptr), ctypes.byref(produced))
+        return bool(produced.value)
+
+    def sync(self) -> None:
+        _lib.call("hv_pipe_sync", self.h)
+
+    def _outputs(self):
+        u, c, i = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        st = _lib.status("hv_pipe_output", self.h, ctypes.byref(u), ctypes.byref(c), ctypes.byref(i))
+        if st not in (_lib.HV_OK, _lib.HV_ERR_NOT_READY):
+            _lib.call("hv_pipe_output", self.h, ctypes.byref(u), ctypes.byref(c), ctypes.byref(i))
+        return u.value, c.value, i.value
+
+    def output_u16(self) -> np.ndarray:
+        u, _, _ = self._outputs()
+        t = torch.empty((self.settings.height, self.settings.width), dtype=torch.int16, device="cuda")
+        copy_d2d(t, u, self.P * 2, self.stream)
+        self.sync()
+        return t.cpu().numpy().view(np.uint16)
+
+    def image(self) -> torch.Tensor:
+        """Float image before contrast (gpu_postprocess_frame)."""
+        _, _, i = self._outputs()
+        t = torch.empty((self.settings.height, self.settings.width), dtype=torch.float32, device="cuda")
+        copy_d2d(t, i, self.P * 4, self.stream)
+        self.sync()
+        return t
+
+    def contrast_image(self) -> torch.Tensor:
+        _, c, _ = self._outputs()
+        t = torch.empty((self.settings.height, self.settings.width), dtype=torch.float32, device="cuda")
+        copy_d2d(t, c, self.P * 4, self.stream)
+        self.sync()
+        return t
+
+    def contrast_range(self) -> np.ndarray:
+        r = ctypes.c_void_p()
+        _lib.call("hv_pipe_contrast_range", self.h, ctypes.byref(r))
+        t = torch.empty(2, dtype=torch.float32, device="cuda")
+        copy_d2d(t, r.value, 8, self.stream)
+        self.sync()
+        return t.cpu().numpy()
+
+    def ring(self):
+        """(ring copy (T, H, W) complex64, start_index, size)."""
+        r, a, b = ctypes.c_void_p(), ctypes.c_uint32(), ctypes.c_uint32()
+        _lib.call("hv_pipe_ring", self.h, ctypes.byref(r), ctypes.byref(a), ctypes.byref(b))
+        T = self.settings.time_transformation_size
+        t = torch.empty((T, self.settings.height, self.settings.width), dtype=torch.complex64, device="cuda")
+        copy_d2d(t, r.value, T * self.P * 8, self.stream)
+        self.sync()
+        return t, a.value, b.value
+
+    def close(self) -> None:
+        if self.h:
+            _lib.call("hv_pipe_destroy", self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Several sources served concurrently; admission by an HBM budget."""
+
+    def __init__(self, budget_bytes: int):
+        self.h = ctypes.c_void_p()
+        _lib.call("hv_server_create", budget_bytes, ctypes.byref(self.h))
+
+    def add_source(self, settings: ComputeSettings):
+        """Returns the source id, or None when the budget refuses it."""
+        sid = ctypes.c_int()
+        st = _lib.status("hv_server_add_source", self.h, ctypes.byref(settings), ctypes.byref(sid))
+        if st == _lib.HV_ERR_BUDGET:
+            return None
+        if st != _lib.HV_OK:
+            _lib.call("hv_server_add_source", self.h, ctypes.byref(settings), ctypes.byref(sid))
+        return sid.value
+
+    def remove_source(self, sid: int) -> None:
+        _lib.call("hv_server_remove_source", self.h, sid)
+
+    def submit(self, sid: int, frames_ptr: int, t_arrival_ns: int) -> None:
+        _lib.call("hv_server_submit", self.h, sid, ctypes.c_void_p(frames_ptr), t_arrival_ns)
+
+    def poll(self, max_n: int = 4096):
+        srcs = (ctypes.c_int * max_n)()
+        lat = (ctypes.c_int64 * max_n)()
+        n = ctypes.c_int()
+        _lib.call("hv_server_poll", self.h, srcs, lat, max_n, ctypes.byref(n))
+        return [(srcs[i], lat[i]) for i in range(n.value)]
+
+    def drain(self) -> None:
+        _lib.call("hv_server_drain", self.h)
+
+    def output_u16(self, sid: int, height: int, width: int) -> np.ndarray:
+        u, f = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.call("hv_server_output", self.h, sid, ctypes.byref(u), ctypes.byref(f))
+        t = torch.empty((height, width), dtype=torch.int16, device="cuda")
+        copy_d2d(t, u.value, height * width * 2, None)
+        hip().hipStreamSynchronize(None)
+        return t.cpu().numpy().view(np.uint16)
+
+    def memory_bytes(self) -> int:
+        n = ctypes.c_size_t()
+        _lib.call("hv_server_memory_bytes", self.h, ctypes.byref(n))
+        return n.value
+
+    def close(self) -> None:
+        if self.h:
+            _lib.call("hv_server_destroy", self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_frames(params, first_frame: int, count: int) -> np.ndarray:
+    """Host frames from the seeded generator (count, H, W) uint8/uint16."""
+    dt = np.uint8 if params.depth == 1 else np.uint16
+    out = np.empty((count, params.height, params.width), dt)
+    _lib.call("hv_synth_frames", ctypes.byref(params), first_frame, count, out.ctypes.data_as(ctypes.c_void_p))
+    return out

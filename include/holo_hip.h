@@ -1,0 +1,423 @@
+/*
+ * holo_hip.h -- C-ABI of the MI355X-native Holovibes per-batch compute path.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an hv_status.  Device pointers are caller-owned (hipMalloc'd, or a
+ * torch tensor's data_ptr()); `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream).  All step functions are asynchronous with
+ * respect to the host unless stated otherwise; none of them allocates device
+ * memory unless stated otherwise.
+ *
+ * Complex buffers are interleaved float2 (re, im) -- the layout of cuComplex
+ * in the reference -- and hold frames frame-major, rows row-major:
+ * element (frame f, row y, column x) is at f*W*H + y*W + x.
+ *
+ * Supported geometry: W and H powers of two in [16, 4096]; time-window sizes
+ * T powers of two in [1, 256].  Other sizes return HV_ERR_UNSUPPORTED.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the Holovibes 14.8.2 source tree).
+ */
+#ifndef HOLO_HIP_H
+#define HOLO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  The reference returns nothing and exit()s on CUDA errors   */
+/* (Backend/cuda_includes/common.cuh:53-64); this boundary reports instead.  */
+/* ------------------------------------------------------------------------ */
+typedef int hv_status;
+#define HV_OK 0
+#define HV_ERR_INVALID_ARG 1  /* null pointer, zero size, range out of bounds */
+#define HV_ERR_UNSUPPORTED 2  /* valid but not implemented (e.g. non power-of-two) */
+#define HV_ERR_OUT_OF_MEMORY 3
+#define HV_ERR_HIP 4          /* a HIP runtime call failed */
+#define HV_ERR_BUDGET 5       /* admission refused: HBM budget exhausted */
+#define HV_ERR_NOT_READY 6    /* no result available yet */
+
+/* camera::PixelDepth (Camera/include/frame_desc.hh:13-24), subset on the path */
+#define HV_DEPTH_8 1
+#define HV_DEPTH_16 2
+#define HV_DEPTH_32F 4
+
+/* holovibes::SpaceTransformation (Backend/includes/enum/enum_space_transformation.hh:17-22) */
+#define HV_SPACE_NONE 0
+#define HV_SPACE_FRESNELTR 1
+#define HV_SPACE_ANGULARSP 2
+
+/* holovibes::TimeTransformation (Backend/includes/enum/enum_time_transformation.hh:16-22) */
+#define HV_TIME_NONE 0
+#define HV_TIME_STFT 1
+#define HV_TIME_PCA 2
+
+/* holovibes::ImgType (Backend/includes/enum/enum_img_type.hh:15-25), subset on the path */
+#define HV_IMG_MODULUS 0
+#define HV_IMG_SQUARED_MODULUS 1
+#define HV_IMG_ARGUMENT 2
+#define HV_IMG_MOMENTS_0 5
+#define HV_IMG_MOMENTS_1 6
+#define HV_IMG_MOMENTS_2 7
+
+/* Lens (transfer function / chirp) arithmetic.  REFERENCE_FP32 evaluates the
+ * reference kernels' float expressions in the same order (masks.cu:7-59);
+ * FP64 evaluates the phase in double and rounds the lens to float once. */
+#define HV_LENS_FP64 0
+#define HV_LENS_REFERENCE_FP32 1
+
+/* PCA covariance arithmetic. */
+#define HV_PCA_FP32 0          /* plain FP32 FMA reduction */
+#define HV_PCA_BF16_SPLIT 1    /* MFMA bf16, x = hi + lo, hi*hi + hi*lo + lo*hi, FP32 accumulate */
+#define HV_PCA_FP16_SPLIT 2    /* MFMA fp16 (scaled), same split */
+#define HV_PCA_BF16 3          /* MFMA bf16, single product (no split) */
+#define HV_PCA_FP16 4          /* MFMA fp16 (scaled), single product */
+
+/* Contrast range policy. */
+#define HV_CONTRAST_REFERENCE 0   /* auto-contrast by percentiles on the first full window, then frozen (rendering.cc:239-258) */
+#define HV_CONTRAST_PERCENTILE 1  /* percentiles recomputed on every image, device-resident */
+#define HV_CONTRAST_MINMAX 2      /* min/max of every image, device-resident */
+#define HV_CONTRAST_MANUAL 3      /* fixed contrast_min / contrast_max */
+
+/* 2-D phase unwrapping variant. */
+#define HV_UNWRAP_REFERENCE 0     /* periodic least squares with 7 FFT2 (tools_unwrap.cu:269-369) */
+
+/* ------------------------------------------------------------------------ */
+/* Library information                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart: version string of this library. */
+const char* hv_version(void);
+
+/* No reference counterpart: last error text of the calling thread. */
+const char* hv_last_error(void);
+
+/* No reference counterpart: number of visible GPUs (0 without a GPU). */
+hv_status hv_device_count(int* count);
+
+/* ------------------------------------------------------------------------ */
+/* Step functions (one per reference step)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces input_queue_to_input_buffer (Backend/cuda_sources/tools_conversion.cu:120-171):
+ * u8 -> float(x)*257, u16 -> float(x), f32 -> x; imaginary part 0.
+ * in: batch*frame_res pixels of `depth` bytes; out: batch*frame_res complex. */
+hv_status hv_convert_to_complex(void* out_c64, const void* in, size_t frame_res, int batch, int depth,
+                                void* stream);
+
+/* Replaces angular_spectrum_lens (Backend/cuda_sources/angular_spectrum.cu:14-31,
+ * kernel_spectral_lens masks.cu:31-59): centred transfer function
+ * H = exp(i*2*pi*z/lambda*sqrt(max(0, 1 - lambda^2 (u^2+v^2)))), Nx*Ny complex.
+ * x_step, y_step in metres (the caller passes PixelSize*1e-6). */
+hv_status hv_angular_spectrum_lens(void* lens_c64, int nx, int ny, float z, float lambda, float x_step,
+                                   float y_step, int precision, void* stream);
+
+/* Replaces shift_corners (Backend/cuda_sources/shift_corners.cu:8-117): in-place
+ * quadrant swap of `batch` complex frames (fftshift for even sizes). */
+hv_status hv_shift_corners_c64(void* buf_c64, int batch, int size_x, int size_y, void* stream);
+
+/* Same as hv_shift_corners_c64 for float frames (rendering.cc:21-40). */
+hv_status hv_shift_corners_f32(float* buf, int batch, int size_x, int size_y, void* stream);
+
+/* Replaces fresnel_transform_lens (Backend/cuda_sources/fresnel_transform.cu:14-63,
+ * kernel_quadratic_lens masks.cu:7-29): chirp exp(i*pi/(lambda z)*(x^2+y^2)),
+ * x = (i - side/2)*pixel_size*1e-6, built on a square of side lens_side and
+ * cropped centrally to width x height.  pixel_size in micrometres. */
+hv_status hv_fresnel_transform_lens(void* lens_c64, int lens_side, int height, int width, float lambda,
+                                    float z, float pixel_size, int precision, void* stream);
+
+/* Replaces angular_spectrum (Backend/cuda_sources/angular_spectrum.cu:33-63):
+ * in place on `batch` frames: x = IFFT2(FFT2(x) * lens) / (W*H).
+ * lens_shifted is the corner-shifted transfer function. */
+hv_status hv_angular_spectrum(void* buf_c64, int batch, const void* lens_shifted, int width, int height,
+                              void* stream);
+
+/* Replaces fresnel_transform (Backend/cuda_sources/fresnel_transform.cu:65-83):
+ * in place on `batch` frames: x = FFT2(x * lens), unnormalised. */
+hv_status hv_fresnel_transform(void* buf_c64, int batch, const void* lens, int width, int height,
+                               void* stream);
+
+/* Batched 2-D FFT of `batch` frames, in place, unnormalised; direction -1
+ * forward, +1 inverse.  Replaces the cufftXtExec calls on the spatial plan
+ * (Backend/sources/core/icompute.cc:151-175). */
+hv_status hv_fft2(void* buf_c64, int batch, int width, int height, int direction, void* stream);
+
+/* Fused form of steps a2-a8 (conversion, space transform, transfer into the
+ * time-transformation queue; converts.cc:369-396, fourier_transform.cc:30-137,
+ * pipe.cc:457-467, queue.cc:332-376): raw frames -> complex -> space transform,
+ * computed in place in ring slots (slot0 + b) mod ring_size, b < batch (of an
+ * oversized batch only the last ring_size frames, as the queue keeps them).
+ * `lens` is the shifted angular-spectrum lens pre-multiplied by 1/(W*H) (see
+ * hv_prepare_lens) or the Fresnel chirp; ignored for HV_SPACE_NONE. */
+hv_status hv_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
+                               const void* lens, void* ring_c64, int ring_size, int slot0, void* stream);
+
+/* Builds the lens hv_propagate_to_ring expects (refresh time, not per batch):
+ * angular spectrum -> corner-shifted H / (W*H); Fresnel -> chirp; NONE -> no-op. */
+hv_status hv_prepare_lens(void* lens_c64, int space, int width, int height, float lambda, float z,
+                          float pixel_size_um, int precision, void* stream);
+
+/* Replaces stft (Backend/cuda_sources/stft.cu:9-15): for each pixel p,
+ * out[k*P+p] = sum_t in[t*P+p] exp(-2 pi i k t / T), unnormalised. */
+hv_status hv_stft(void* out_c64, const void* in_c64, int T, size_t frame_res, void* stream);
+
+/* Replace complex_to_modulus / complex_to_squared_modulus / complex_to_argument
+ * (Backend/cuda_sources/tools_conversion.cu:11-30, 84-103, 199-232): mean over
+ * the inclusive range [pmin, pmax] of |x|, |x|^2 or atan2(im, re). */
+hv_status hv_complex_to_modulus(float* out, const void* in_c64, int pmin, int pmax, size_t frame_res,
+                                void* stream);
+hv_status hv_complex_to_squared_modulus(float* out, const void* in_c64, int pmin, int pmax,
+                                        size_t frame_res, void* stream);
+hv_status hv_complex_to_argument(float* out, const void* in_c64, int pmin, int pmax, size_t frame_res,
+                                 void* stream);
+
+/* Fused STFT + image type: the ring (T frames, physical slot order) ->
+ * mean over k in [pmin, pmax] of the chosen image type of S[k].  Equivalent
+ * to hv_stft followed by hv_complex_to_* without the T*P spectrum in HBM.
+ * If minmax (device, 2 x uint32) is non-NULL the image's min and max are
+ * folded into it (see hv_minmax_init). */
+hv_status hv_stft_image(float* out, const void* ring_c64, int T, size_t frame_res, int img_type, int pmin,
+                        int pmax, uint32_t* minmax, void* stream);
+
+/* Replaces the moments path (tools_conversion.cu:33-66, tools_compute.cu:38-97,
+ * icompute.cc:26-69, pipe.cc:429-455): from the ring, S = STFT, then
+ * M_k = sum_{f=f_start}^{f_end} freq_k[f] |S_f|^2 with the Nyquist bin added
+ * again for k = 0, 2 and subtracted for k = 1 when T is even and it is in range.
+ * freq_1 = fftfreq(T) * camera_fps.  Any of m0/m1/m2 may be NULL. */
+hv_status hv_stft_moments(float* m0, float* m1, float* m2, const void* ring_c64, int T, size_t frame_res,
+                          int f_start, int f_end, float camera_fps, void* stream);
+
+/* Replaces cov_matrix (Backend/sources/compute/matrix_operations.cc:9-26):
+ * cov = H^H H, H = ring as P x T column-major; cov is T x T complex
+ * column-major.  Partial sums go to `workspace` (hv_pca_workspace_size bytes). */
+hv_status hv_pca_workspace_size(int T, size_t frame_res, int precision, size_t* bytes);
+hv_status hv_pca_cov(void* cov_c64, const void* ring_c64, int T, size_t frame_res, int precision,
+                     void* workspace, void* stream);
+
+/* Replaces eigen_values_vectors (matrix_operations.cc:28-65, cusolverDnCheevd
+ * lower, ascending): cyclic Jacobi in FP64 on the GPU.  cov: T x T complex
+ * float (column-major); evals: T doubles ascending; evecs: T x T complex
+ * double column-major (column j = eigenvector j, phase normalised so its
+ * largest-magnitude component is real positive).  Only the lower triangle
+ * of cov is read.  T <= 64 runs in LDS; larger T needs `workspace`
+ * (hv_eig_workspace_size bytes, may be NULL when that size is 0). */
+hv_status hv_eig_workspace_size(int T, size_t* bytes);
+hv_status hv_eig_hermitian_f64(double* evals, void* evecs_c128, const void* cov_c64, int T, void* workspace,
+                               void* stream);
+
+/* Replaces matrix_multiply_complex (matrix_operations.cc:77-102) followed by the
+ * image-type conversion and the PCA x131071 scale (converts.cc:22-63):
+ * C = H V, image = 131071 * mean over t in [pmin, pmax] of type(C[:, t]). */
+hv_status hv_pca_image(float* out, const void* ring_c64, const void* evecs_c128, int T, size_t frame_res,
+                       int img_type, int pmin, int pmax, uint32_t* minmax, void* stream);
+
+/* Replaces unwrap_2d (Backend/cuda_sources/tools_unwrap.cu:269-369):
+ * least-squares 2-D phase unwrapping of a W x H phase image (the reference's
+ * FFT2 sequence, computing FFT2(z) once instead of twice);
+ * out = Im(IFFT2(...)) / (-2 pi).  scratch: hv_unwrap_scratch_size bytes. */
+hv_status hv_unwrap_scratch_size(int width, int height, size_t* bytes);
+hv_status hv_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, void* stream);
+
+/* Replaces rescale_float_unwrap2d (tools_conversion.cu:315-335) with a
+ * device-side min/max: out = (in - min) / (max - min) * 65535.
+ * minmax_scratch: 2 x uint32 of device memory. */
+hv_status hv_rescale_unwrap2d(float* out, const float* in, size_t frame_res, uint32_t* minmax_scratch,
+                              void* stream);
+
+/* Replaces subsample_frame (Backend/cuda_sources/frame_reshape.cu:278-330) for
+ * float frames: out(x,y) = in[(y*s + (x odd ? s/2 : 0))*W + x*s + (y odd ? s/2 : 0)]. */
+hv_status hv_subsample_frame_f32(float* out, const float* in, int width, int height, int step, void* stream);
+
+/* Device-side min/max accumulators: two uint32 holding the order-preserving
+ * encoding of a float (sign bit set for non-negative values, all bits
+ * inverted for negative ones), so that one integer atomicMin/atomicMax per
+ * workgroup updates them exactly and in any order.
+ * hv_minmax_init sets them to (+inf, -inf); hv_minmax folds n floats in;
+ * hv_minmax_decode writes the two floats (min, max) to out. */
+hv_status hv_minmax_init(uint32_t* minmax, void* stream);
+hv_status hv_minmax(uint32_t* minmax, const float* in, size_t n, void* stream);
+hv_status hv_minmax_decode(float* out, const uint32_t* minmax, void* stream);
+
+/* Replaces compute_percentile_xy_view + update_contrast
+ * (Backend/cuda_sources/percentile.cu:75-222, contrast_api.cc:137-146):
+ * exact order statistics of the (optionally subsampled, step 0/1 = none)
+ * image by radix select on the float bits; value at index
+ * uint(pct/100 * N) of the ascending order, clamped to >= 1.0 when
+ * clamp_ge_one.  range_out: 2 floats (device).  scratch: hv_percentile_scratch_size. */
+hv_status hv_percentile_scratch_size(size_t n, size_t* bytes);
+hv_status hv_percentile_range(float* range_out, const float* img, int width, int height, int subsample_step,
+                              float lower_pct, float upper_pct, int clamp_ge_one, void* scratch, void* stream);
+
+/* Replaces apply_contrast_correction (Backend/cuda_sources/contrast_correction.cu:5-17):
+ * x <- 65535 / (max - min + FLT_EPSILON) * (x - min), in place, host range. */
+hv_status hv_apply_contrast(float* img, size_t n, float min, float max, void* stream);
+
+/* Replaces float_to_ushort (tools_conversion.cu:350-387): x <= 0 -> 0,
+ * x > 65535 -> 65535, else truncation. */
+hv_status hv_float_to_ushort(uint16_t* out, const float* in, size_t n, void* stream);
+
+/* Fused apply_contrast_correction + float_to_ushort with the range read from
+ * device memory (range_dev[0] = min, range_dev[1] = max); invert swaps them
+ * (rendering.cc:185-237). */
+hv_status hv_contrast_to_ushort(uint16_t* out, const float* in, size_t n, const float* range_dev, int invert,
+                                void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Pipe: the per-batch orchestrator (Backend/sources/core/pipe.cc:255-387)   */
+/* ------------------------------------------------------------------------ */
+
+/* The compute settings the path reads (Backend/includes/settings/settings.hh:94-206). */
+typedef struct hv_compute_settings {
+    uint32_t width, height;          /* FrameDescriptor width/height */
+    uint32_t depth;                  /* HV_DEPTH_8 / HV_DEPTH_16 / HV_DEPTH_32F */
+    uint32_t batch_size;             /* BatchSize */
+    uint32_t time_stride;            /* TimeStride (multiple of batch_size) */
+    uint32_t time_transformation_size; /* TimeTransformationSize, T */
+    int32_t space_transformation;    /* HV_SPACE_* */
+    int32_t time_transformation;     /* HV_TIME_* */
+    int32_t img_type;                /* HV_IMG_* */
+    float lambda;                    /* Lambda, metres */
+    float z_distance;                /* ZDistance, metres */
+    float pixel_size;                /* PixelSize, micrometres */
+    uint32_t p_start, p_width;       /* P view (accumulation range) */
+    int32_t unwrap_2d;               /* Unwrap2d (argument images) */
+    int32_t fft_shift;               /* FftShiftEnabled */
+    int32_t contrast_enabled;        /* XY.contrast.enabled */
+    int32_t contrast_invert;         /* XY.contrast.invert */
+    int32_t contrast_mode;           /* HV_CONTRAST_* */
+    float contrast_min, contrast_max;      /* XYContrastRange (HV_CONTRAST_MANUAL) */
+    float contrast_lower, contrast_upper;  /* ContrastLower/UpperThreshold, percent */
+    uint32_t contrast_subsample;     /* percentile on an s x s strided subsample (0/1 = full) */
+    float camera_fps;                /* CameraFps (moments frequency scale) */
+    int32_t lens_precision;          /* HV_LENS_* */
+    int32_t pca_precision;           /* HV_PCA_* */
+} hv_compute_settings;
+
+/* Fills the reference defaults (Backend/includes/core/holovibes.hh:359-447). */
+void hv_default_settings(hv_compute_settings* s);
+
+/* Validates settings the way the reference setters clamp/refuse them
+ * (Backend/sources/api/transform_api.cc:16-206). */
+hv_status hv_validate_settings(const hv_compute_settings* s);
+
+typedef struct hv_pipe hv_pipe;
+
+/* Replaces Pipe construction + refresh (pipe.hh:70-125, icompute.hh:122-171,
+ * pipe.cc:255-387): allocates every buffer once (ring, lens, scratch, output)
+ * on the current device and builds the lens.  No allocation after this. */
+hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** out);
+
+/* Device bytes a pipe with these settings holds (admission accounting). */
+hv_status hv_pipe_memory_bytes(const hv_compute_settings* s, size_t* bytes);
+
+/* One Pipe::exec cycle (pipe.cc:650-676) for one batch already in device
+ * memory (batch_size raw frames): stride discard, transfer into the ring,
+ * wait for T frames, time transform, image type, contrast, u16.  Sets
+ * *produced to 1 when an output image was written (hv_pipe_output), 0 when
+ * the cycle exited early.  Asynchronous: work is queued on the pipe stream. */
+hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced);
+
+/* Device pointers of the latest output: u16 image, float image after contrast
+ * (before the u16 cast), and the float image before contrast. */
+hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out_f32_contrast,
+                         const float** out_f32_image);
+
+/* Current contrast range (device pointer to 2 floats). */
+hv_status hv_pipe_contrast_range(hv_pipe* p, const float** range_dev);
+
+/* Captures the steady-state per-batch work (ring full, no discard) into a
+ * hipGraph per ring phase and replays it on later pushes; 0 disables. */
+hv_status hv_pipe_set_graph(hv_pipe* p, int enabled);
+
+/* No reference counterpart: the pipe's input staging buffer (batch_size
+ * frames).  A batch written there is pushed without a device copy. */
+hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames);
+
+/* The time-transformation queue: ring base (T frames, complex), its
+ * start index and size (Queue::get_start_index / get_size, queue.hh). */
+hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size);
+
+/* The pipe's stream (hipStream_t). */
+hv_status hv_pipe_stream(hv_pipe* p, void** stream);
+
+/* Synchronises the pipe stream (end-of-pipe cudaStreamSynchronize, pipe.cc:386). */
+hv_status hv_pipe_sync(hv_pipe* p);
+
+hv_status hv_pipe_destroy(hv_pipe* p);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic camera source (host, deterministic)                              */
+/* ------------------------------------------------------------------------ */
+
+/* Seeded interferogram generator standing in for ICamera::get_frames
+ * (Camera/include/icamera.hh:152): I = |R + O_t|^2, R = exp(2 pi i (fx x + fy y)),
+ * O_t = O_0 exp(i q r^2/r_max^2) exp(2 pi i f_d(x,y) t / period), then
+ * Poisson(K I) clipped to the bit depth.  Counter-based: frame t of a source
+ * is the same whatever order frames are generated in. */
+typedef struct hv_synth_params {
+    uint32_t width, height;
+    uint32_t depth;          /* HV_DEPTH_8 or HV_DEPTH_16 */
+    uint32_t bits;           /* significant bits (8 or 12) */
+    uint32_t period;         /* Doppler period in frames (T of the config) */
+    float sigma;             /* speckle amplitude */
+    float tilt_x, tilt_y;    /* reference-wave tilt, cycles/px */
+    float quad_phase;        /* quadratic phase amplitude (rad) at r_max */
+    float mean_counts;       /* mean of K*I */
+    uint32_t n_discs;        /* Doppler discs (<= 4) */
+    float disc_cx[4], disc_cy[4], disc_r[4];   /* pixels */
+    int32_t fd_lo[4], fd_hi[4];                /* integer Doppler range per disc */
+    uint64_t seed;
+} hv_synth_params;
+
+/* Writes frames [first_frame, first_frame + count) to host memory `out`
+ * (count * W * H * depth bytes).  Runs on host threads. */
+hv_status hv_synth_frames(const hv_synth_params* p, uint64_t first_frame, uint32_t count, void* out);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: several sources, admission by HBM budget, latency accounting     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hv_server hv_server;
+
+/* No single reference counterpart (the reference serves one source per
+ * process through BatchInputQueue, batch_input_queue.cc:124-235): a server
+ * owns several pipes, each with its own HBM ring and lens, admitted while
+ * their memory fits in budget_bytes. */
+hv_status hv_server_create(size_t budget_bytes, hv_server** out);
+
+/* Admits a source; HV_ERR_BUDGET when its pipe does not fit the budget. */
+hv_status hv_server_add_source(hv_server* s, const hv_compute_settings* settings, int* source_id);
+
+/* Releases a source and its memory. */
+hv_status hv_server_remove_source(hv_server* s, int source_id);
+
+/* Submits one batch (device memory, batch_size frames, already landed in HBM)
+ * that became available at host time t_arrival_ns (steady clock); the pipe
+ * work is queued on the source's stream and overlaps with other sources'.
+ * Returns immediately; `frames` must stay valid until the batch completes. */
+hv_status hv_server_submit(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns);
+
+/* Completes finished batches; writes up to max_n (source, latency_ns) pairs.
+ * Latency = completion of the batch's u16 image - t_arrival. */
+hv_status hv_server_poll(hv_server* s, int* sources, int64_t* latency_ns, int max_n, int* n_out);
+
+/* Blocks until every submitted batch has completed. */
+hv_status hv_server_drain(hv_server* s);
+
+/* Latest output of a source (device pointers, see hv_pipe_output). */
+hv_status hv_server_output(hv_server* s, int source_id, const uint16_t** out_u16, const float** out_f32_image);
+
+/* Bytes currently admitted. */
+hv_status hv_server_memory_bytes(hv_server* s, size_t* bytes);
+
+hv_status hv_server_destroy(hv_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HOLO_HIP_H */
