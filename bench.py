@@ -1,0 +1,193 @@
+"""Benchmark of the per-batch compute path on one MI355X.
+
+A step is one Pipe cycle (hv_pipe_push_batch) over one batch of synthetic
+frames already resident in HBM: conversion + angular spectrum into the
+time-window ring, STFT + image type, contrast, u16.  Default workload:
+512 x 512 u16, B = S = T = 32 (see holovibes_amd/configs.py "metric").
+
+Prints one JSON line: frames/s (whole job, one GPU), ms per step, p50/p99
+batch -> image latency, the roofline of the dominant kernel (HIP events
+around each stage, on the pipe's stream) and the CPU oracle baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="metric")
+    ap.add_argument("--latency-batches", type=int, default=200)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run warmup + this many steps (for rocprofv3), print the JSON without extras")
+    return ap.parse_args()
+
+
+def stage_bytes(s):
+    """Algorithmic HBM bytes per launch of each stage: every stage reads its
+    input once and writes its output once."""
+    P = s.width * s.height
+    B, T, d = s.batch_size, s.time_transformation_size, s.depth
+    nb = min(B, T)
+    space = s.space_transformation
+    if space == 2:  # angular spectrum: row FFT (+convert), column FFT x lens x IFFT, inverse row FFT
+        st0 = nb * P * (d + 8)
+        st1 = nb * P * 16 + P * 8
+        st2 = nb * P * 16
+    elif space == 1:  # Fresnel: row FFT (+convert, x chirp), column FFT
+        st0 = nb * P * (d + 8) + P * 8
+        st1 = nb * P * 16
+        st2 = 0
+    else:
+        st0, st1, st2 = nb * P * (d + 8), 0, 0
+    if s.time_transformation == 1:
+        st3 = T * P * 8 + P * 4
+    elif s.time_transformation == 2:  # covariance read + projection read + image write
+        st3 = 2 * T * P * 8 + P * 4
+    else:
+        st3 = T * P * 8 + P * 4
+    st4 = P * 4 + P * 2 + P * 4  # image in, u16 out, contrasted float out
+    return [st0, st1, st2, st3, st4]
+
+
+STAGE_NAMES = ["row_fft_convert", "column_fft_lens_ifft", "row_ifft_to_ring", "time_transform_image",
+               "contrast_u16"]
+
+
+def cpu_baseline(name, settings, batches, seconds):
+    from oracle import oracle as o
+    o.build()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    o.set_threads(cores)
+    pipe = o.OraclePipe(settings, 32)
+    B, S, T = settings.batch_size, settings.time_stride, settings.time_transformation_size
+    k = 0
+    while pipe.push(batches[k % len(batches)]) is None:  # fill the window
+        k += 1
+    n, frames = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        pipe.push(batches[k % len(batches)])
+        k += 1
+        n += 1
+        frames += B
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 64:
+            break
+    return {"value": frames / el, "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"{n} batches of {B} frames ({name}) through the FP32 C++ oracle pipe, "
+                      f"{el:.1f} s on {cores} host threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by source, "
+              "see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from holovibes_amd import configs
+    from holovibes_amd.pipe import Pipe, synth_frames
+
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.set_device(0)
+    name = args.workload
+    s = configs.settings_for(name)
+    sp = configs.synth_for(name)
+    B, S, T, P = s.batch_size, s.time_stride, s.time_transformation_size, s.width * s.height
+    nbatches = 4
+    host_batches = [synth_frames(sp, k * B, B) for k in range(nbatches)]
+    to_dev = (lambda a: torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).cuda())
+    dev_batches = [to_dev(b) for b in host_batches]
+    torch.cuda.synchronize()
+
+    pipe = Pipe(s, graph=not args.no_graph)
+    k = 0
+    while not pipe.push(dev_batches[k % nbatches]):  # fill the time window
+        k += 1
+    for _ in range(args.warmup):
+        pipe.push(dev_batches[k % nbatches])
+        k += 1
+    torch.cuda.synchronize()
+    steps = args.profile_steps or args.steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pipe.push(dev_batches[k % nbatches])
+        k += 1
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = el / steps * 1e3
+    frames_per_s = steps * B / el
+    line = {"metric": "frames/s + p99 batch->image latency at 512^2 T=32; % HBM roofline; err vs FP64",
+            "value": frames_per_s, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded interferogram generator)",
+            "config": {"workload": name, "width": s.width, "height": s.height, "depth_bytes": s.depth,
+                       "batch": B, "stride": S, "T": T, "space": ["none", "fresnel", "angular_spectrum"][
+                           s.space_transformation], "time": ["none", "stft", "pca"][s.time_transformation],
+                       "graph": not args.no_graph}}
+    if args.profile_steps:
+        print(json.dumps(line))
+        return
+
+    # per-stage HIP events over a second timed region of the same length
+    pipe.set_timing(True)
+    for _ in range(steps):
+        pipe.push(dev_batches[k % nbatches])
+        k += 1
+    ms, cyc = pipe.stage_times()
+    pipe.set_timing(False)
+    sb = stage_bytes(s)
+    stages = []
+    for nm, t, b in zip(STAGE_NAMES, ms, sb):
+        gbs = b / (t * 1e-3) / 1e9 if t > 0 else 0.0
+        stages.append({"name": nm, "ms": t, "bytes": b, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS})
+    dom = max(range(5), key=lambda i: ms[i])
+    line["roofline"] = {"bound": "hbm", "kernel": STAGE_NAMES[dom], "achieved": stages[dom]["GB/s"],
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["frac"], "traffic": None}
+    tot_b, tot_ms = sum(sb), sum(ms)
+    line["pipeline_roofline"] = {"bytes_per_step": tot_b, "ms_per_step_stages": tot_ms,
+                                 "GB/s": tot_b / (tot_ms * 1e-3) / 1e9, "frac": tot_b / (tot_ms * 1e-3) / 1e9 /
+                                 HBM_PEAK_GBS}
+    line["stages"] = stages
+
+    # batch -> image latency, one batch at a time (unloaded)
+    lat = []
+    for _ in range(args.latency_batches):
+        t1 = time.perf_counter_ns()
+        made = pipe.push(dev_batches[k % nbatches])
+        k += 1
+        pipe.sync()
+        if made:
+            lat.append((time.perf_counter_ns() - t1) / 1e6)
+    lat.sort()
+    if lat:
+        line["p50_latency_ms"] = statistics.median(lat)
+        line["p99_latency_ms"] = lat[min(len(lat) - 1, int(0.99 * len(lat)))]
+        line["batch_period_ms_at_max_rate"] = ms_per_step
+    pipe.close()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(name, s, host_batches, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

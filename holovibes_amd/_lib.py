@@ -110,6 +110,8 @@ SIGNATURES = {
     "hv_pipe_input_buffer": [P, P],
     "hv_pipe_ring": [P, P, P, P],
     "hv_pipe_stream": [P, P],
+    "hv_pipe_set_timing": [P, I],
+    "hv_pipe_stage_times": [P, P, P],
     "hv_pipe_sync": [P],
     "hv_pipe_destroy": [P],
     "hv_synth_frames": [P, U64, U32, P],

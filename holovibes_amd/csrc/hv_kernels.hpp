@@ -20,7 +20,8 @@ hv_status launch_angular_spectrum(float2* buf, int batch, const float2* lens, in
                                   float post_scale, hipStream_t st);
 hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width, int height, hipStream_t st);
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
-                                   const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st);
+                                   const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
+                                   hipEvent_t* marks = nullptr);
 
 // hv_time.hip
 hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_t st);

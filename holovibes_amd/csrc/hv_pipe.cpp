@@ -63,6 +63,9 @@ Layout layout_of(const hv_compute_settings& s) {
 
 }  // namespace
 
+constexpr int kStages = 5;
+constexpr size_t kMaxGraphs = 64;
+
 struct hv_pipe {
     hv_compute_settings s{};
     hipStream_t stream = nullptr;
@@ -92,12 +95,32 @@ struct hv_pipe {
     uint32_t q_start = 0, q_size = 0;
     bool autocontrast_pending = true;
     bool have_output = false;
-    // graphs, one per (ring phase, autocontrast-pending) pair
+    // graphs, one per (ring phase, input buffer) pair
     bool graph_enabled = false;
-    std::map<int, hipGraphExec_t> graphs;
+    std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
+    // stage timer: events between row pass, column pass, inverse row pass,
+    // image kernel(s) and the contrast / u16 stage
+    bool timing = false;
+    hipEvent_t ev[kStages + 1] = {};
+    double stage_ms[kStages] = {};
+    int timed_cycles = 0;
 
     hv_status image_work();
-    hv_status enqueue_cycle(int slot0);
+    hv_status enqueue_cycle(const void* frames, int slot0);
+    hv_status mark(int i) {
+        if (timing) HV_HIP(hipEventRecord(ev[i], stream));
+        return HV_OK;
+    }
+    hv_status collect_times() {
+        HV_HIP(hipEventSynchronize(ev[kStages]));
+        for (int i = 0; i < kStages; ++i) {
+            float t = 0.f;
+            HV_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+            stage_ms[i] += t;
+        }
+        ++timed_cycles;
+        return HV_OK;
+    }
 };
 
 // Device work of one cycle after the ring holds T frames (pipe.cc:333-380).
@@ -138,6 +161,7 @@ hv_status hv_pipe::image_work() {
                                    it - HV_IMG_MOMENTS_0, stream));
     }
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(image, 1, s.width, s.height, stream));
+    HV_TRY(mark(4));
     if (!s.contrast_enabled) return launch_float_to_ushort(out, image, P, stream);
     switch (s.contrast_mode) {
         case HV_CONTRAST_MINMAX:
@@ -163,10 +187,13 @@ hv_status hv_pipe::image_work() {
     return launch_contrast_to_ushort(out, cimage, image, P, range, 0, s.contrast_invert, stream);
 }
 
-hv_status hv_pipe::enqueue_cycle(int slot0) {
-    HV_TRY(launch_propagate_to_ring(staging, (int)s.depth, B, (int)s.width, (int)s.height, s.space_transformation,
-                                    lens, ring, T, slot0, stream));
-    return image_work();
+hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
+    HV_TRY(mark(0));
+    HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height, s.space_transformation,
+                                    lens, ring, T, slot0, stream, timing ? &ev[1] : nullptr));
+    HV_TRY(mark(3));
+    HV_TRY(image_work());
+    return mark(5);
 }
 
 extern "C" {
@@ -333,15 +360,11 @@ hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames) {
 
 hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     if (!p || !frames) return fail(HV_ERR_INVALID_ARG, "null argument");
-    int made = 0;
     if (produced) *produced = 0;
-    // insert_wait_time_stride (pipe.cc:400-416)
+    // insert_wait_time_stride (pipe.cc:400-416): other batches are dropped
     p->batch_index += p->B;
     if (p->batch_index != p->s.time_stride) return HV_OK;
     p->batch_index = 0;
-    if (frames != p->staging)
-        HV_HIP(hipMemcpyAsync(p->staging, frames, (size_t)p->B * p->P * p->s.depth, hipMemcpyDeviceToDevice,
-                              p->stream));
     // Queue::enqueue_multiple bookkeeping (queue.cc:332-376)
     const uint32_t maxq = (uint32_t)p->T;
     const int slot0 = (int)((p->q_start + p->q_size) % maxq);
@@ -358,17 +381,31 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     // insert_wait_time_transformation_size (pipe.cc:418-427)
     const bool full = p->q_size >= maxq;
     if (!full) {
-        return launch_propagate_to_ring(p->staging, (int)p->s.depth, p->B, (int)p->s.width, (int)p->s.height,
+        return launch_propagate_to_ring(frames, (int)p->s.depth, p->B, (int)p->s.width, (int)p->s.height,
                                         p->s.space_transformation, p->lens, p->ring, p->T, slot0, p->stream);
     }
     const bool pending = p->s.contrast_enabled && p->s.contrast_mode == HV_CONTRAST_REFERENCE &&
                          p->autocontrast_pending;
-    if (p->graph_enabled && !pending) {
-        auto it = p->graphs.find(slot0);
+    if (p->timing) {
+        HV_TRY(p->enqueue_cycle(frames, slot0));
+        HV_TRY(p->collect_times());
+    } else if (p->graph_enabled && !pending) {
+        // one graph per (ring phase, input buffer); past the cache bound the
+        // batch is staged into the pipe's own input buffer first
+        const void* src = frames;
+        auto key = std::make_pair(slot0, src);
+        auto it = p->graphs.find(key);
+        if (it == p->graphs.end() && p->graphs.size() >= kMaxGraphs) {
+            HV_HIP(hipMemcpyAsync(p->staging, frames, (size_t)p->B * p->P * p->s.depth, hipMemcpyDeviceToDevice,
+                                  p->stream));
+            src = p->staging;
+            key = std::make_pair(slot0, src);
+            it = p->graphs.find(key);
+        }
         if (it == p->graphs.end()) {
             hipGraph_t g = nullptr;
             HV_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            hv_status st = p->enqueue_cycle(slot0);
+            hv_status st = p->enqueue_cycle(src, slot0);
             hipError_t e = hipStreamEndCapture(p->stream, &g);
             if (st != HV_OK) {
                 if (g) (void)hipGraphDestroy(g);
@@ -379,16 +416,32 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
             e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-            it = p->graphs.emplace(slot0, ex).first;
+            it = p->graphs.emplace(key, ex).first;
         }
         HV_HIP(hipGraphLaunch(it->second, p->stream));
     } else {
-        HV_TRY(p->enqueue_cycle(slot0));
+        HV_TRY(p->enqueue_cycle(frames, slot0));
     }
     if (pending) p->autocontrast_pending = false;  // rendering.cc:251-258
     p->have_output = true;
-    made = 1;
-    if (produced) *produced = made;
+    if (produced) *produced = 1;
+    return HV_OK;
+}
+
+hv_status hv_pipe_set_timing(hv_pipe* p, int enabled) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    if (enabled && !p->ev[0])
+        for (auto& e : p->ev) HV_HIP(hipEventCreate(&e));
+    p->timing = enabled != 0;
+    for (auto& t : p->stage_ms) t = 0.0;
+    p->timed_cycles = 0;
+    return HV_OK;
+}
+
+hv_status hv_pipe_stage_times(hv_pipe* p, float* ms, int* cycles) {
+    if (!p || !ms) return fail(HV_ERR_INVALID_ARG, "null argument");
+    for (int i = 0; i < kStages; ++i) ms[i] = p->timed_cycles ? (float)(p->stage_ms[i] / p->timed_cycles) : 0.f;
+    if (cycles) *cycles = p->timed_cycles;
     return HV_OK;
 }
 
@@ -437,6 +490,8 @@ hv_status hv_pipe_destroy(hv_pipe* p) {
     if (!p) return HV_OK;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto e : p->ev)
+        if (e) (void)hipEventDestroy(e);
     if (p->base) (void)hipFree(p->base);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;

@@ -356,7 +356,9 @@ hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width,
 }
 
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
-                                   const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st) {
+                                   const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
+                                   hipEvent_t* marks) {
+    // marks (optional): events recorded after pass 1 and after pass 2
     const int kin = depth_to_kin(depth);
     if (kin < 0) return fail(HV_ERR_INVALID_ARG, "depth must be 1, 2 or 4");
     const size_t P = (size_t)width * height;
@@ -370,17 +372,27 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
     if (space == HV_SPACE_ANGULARSP) {
         // pass 1 writes the frames into their ring slots; passes 2-3 run in place there
         HV_TRY(row_pass<-1>(src, kin, ring, nullptr, width, height, nb, plain, slots, 1.0f, st));
+        if (marks) HV_HIP(hipEventRecord(marks[0], st));
         HV_TRY(col_pass<COL_LENS>(ring, ring, lens, width, height, nb, slots, slots, st));
+        if (marks) HV_HIP(hipEventRecord(marks[1], st));
         return row_pass<+1>(ring, IN_C64, ring, nullptr, width, height, nb, slots, slots, 1.0f, st);
     }
     if (space == HV_SPACE_FRESNELTR) {
         HV_TRY(row_pass<-1>(src, kin, ring, lens, width, height, nb, plain, slots, 1.0f, st));
+        if (marks) {
+            HV_HIP(hipEventRecord(marks[0], st));
+            HV_HIP(hipEventRecord(marks[1], st));
+        }
         return col_pass<COL_FWD>(ring, ring, nullptr, width, height, nb, slots, slots, st);
     }
     // no space transform: conversion only, frame by frame into the slots
     for (int b = 0; b < nb; ++b) {
         float2* dst = ring + (size_t)((slots.slot0 + b) % ring_size) * P;
         HV_TRY(launch_convert(dst, src + (size_t)b * P * depth, P, depth, st));
+    }
+    if (marks) {
+        HV_HIP(hipEventRecord(marks[0], st));
+        HV_HIP(hipEventRecord(marks[1], st));
     }
     return HV_OK;
 }

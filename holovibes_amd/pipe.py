@@ -35,11 +35,13 @@ def copy_d2d(dst: torch.Tensor, src_ptr: int, nbytes: int, stream) -> None:
 
 class Pipe:
     """One source's compute pipe.  `settings` is a ComputeSettings; the pipe
-    runs on torch's current stream unless `own_stream`."""
+    runs on torch's current stream unless `own_stream` (always with `graph`:
+    the legacy default stream cannot be captured)."""
 
     def __init__(self, settings: ComputeSettings, own_stream: bool = False, graph: bool = False):
         self.settings = settings
         self.h = ctypes.c_void_p()
+        own_stream = own_stream or graph
         stream = None if own_stream else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.call("hv_pipe_create", ctypes.byref(settings), stream, ctypes.byref(self.h))
         st = ctypes.c_void_p()
@@ -66,6 +68,16 @@ class Pipe:
         produced = ctypes.c_int()
         _lib.call("hv_pipe_push_batch", self.h, ctypes.c_void_p(ptr), ctypes.byref(produced))
         return bool(produced.value)
+
+    def set_timing(self, enabled: bool) -> None:
+        _lib.call("hv_pipe_set_timing", self.h, 1 if enabled else 0)
+
+    def stage_times(self):
+        """Mean ms of (row FFT, column pass, inverse row, image, contrast + u16) and the cycle count."""
+        ms = (ctypes.c_float * 5)()
+        n = ctypes.c_int()
+        _lib.call("hv_pipe_stage_times", self.h, ms, ctypes.byref(n))
+        return list(ms), n.value
 
     def sync(self) -> None:
         _lib.call("hv_pipe_sync", self.h)

@@ -333,9 +333,21 @@ hv_status hv_pipe_contrast_range(hv_pipe* p, const float** range_dev);
  * hipGraph per ring phase and replays it on later pushes; 0 disables. */
 hv_status hv_pipe_set_graph(hv_pipe* p, int enabled);
 
-/* No reference counterpart: the pipe's input staging buffer (batch_size
- * frames).  A batch written there is pushed without a device copy. */
+/* No reference counterpart: the pipe's own input buffer (batch_size frames).
+ * Pushed batches are read in place; with graphs enabled each distinct input
+ * pointer gets its own captured graph (up to 64, then batches are copied
+ * into this buffer first), so producers should cycle through a fixed set of
+ * landing buffers -- or write here. */
 hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames);
+
+/* No reference counterpart (the reference's Chrono / benchmark worker time
+ * whole cycles only): with timing enabled every steady-state cycle runs
+ * eagerly with events between its stages and waits for them.
+ * hv_pipe_stage_times writes the mean milliseconds of the 5 stages -- row
+ * FFT pass, column pass, inverse row pass, image kernel(s), contrast + u16 --
+ * over the timed cycles. */
+hv_status hv_pipe_set_timing(hv_pipe* p, int enabled);
+hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
 
 /* The time-transformation queue: ring base (T frames, complex), its
  * start index and size (Queue::get_start_index / get_size, queue.hh). */
