@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprofv3), print the JSON without extras")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also copy this many MiB device-to-device once (a plain copy kernel of known size, "
+                         "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
     return ap.parse_args()
 
 
@@ -118,6 +121,13 @@ def main():
     to_dev = (lambda a: torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).cuda())
     dev_batches = [to_dev(b) for b in host_batches]
     torch.cuda.synchronize()
+    if args.copy_probe_mb:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     pipe = Pipe(s, graph=not args.no_graph)
     k = 0

@@ -21,7 +21,7 @@ hv_status launch_angular_spectrum(float2* buf, int batch, const float2* lens, in
 hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width, int height, hipStream_t st);
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
-                                   hipEvent_t* marks = nullptr);
+                                   hipEvent_t* marks = nullptr, bool spectral_rows = false);
 
 // hv_time.hip
 hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_t st);
@@ -30,6 +30,12 @@ hv_status launch_complex_to_image(float* out, const float2* in, int img_type, in
 // Fused STFT + image type.  mm: encoded min/max (may be null).
 hv_status launch_stft_image(float* out, const float2* ring, int T, size_t P, int img_type, int pmin, int pmax,
                             uint32_t* mm, hipStream_t st);
+// Fused inverse row FFT + STFT + image type for a window whose T frames hold
+// the angular spectrum after its column pass (B = S = T); HV_ERR_UNSUPPORTED
+// when (width, T) has no fused kernel (its LDS tile would not fit).
+bool row_ifft_stft_supported(int width, int T);
+hv_status launch_row_ifft_stft_image(float* out, float2* ring, int width, int height, int T, int img_type, int pmin,
+                                     int pmax, uint32_t* mm, hipStream_t st);
 hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* ring, int T, size_t P, int f_start,
                               int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st);
 

@@ -29,6 +29,14 @@ bool is_moment_kind(int t) { return t == HV_IMG_MOMENTS_0 || t == HV_IMG_MOMENTS
 
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
+// The whole window is replaced by every batch (B = S = T), so the angular
+// spectrum's inverse row pass can be fused with the STFT.
+bool fused_eligible(const hv_compute_settings& s) {
+    return s.space_transformation == HV_SPACE_ANGULARSP && s.time_transformation == HV_TIME_STFT &&
+           is_image_kind(s.img_type) && s.batch_size == s.time_transformation_size &&
+           s.time_stride == s.batch_size && row_ifft_stft_supported((int)s.width, (int)s.time_transformation_size);
+}
+
 struct Layout {
     size_t ring, lens, staging, image, cimage, tmp, out, small, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
     size_t total;
@@ -98,6 +106,8 @@ struct hv_pipe {
     // graphs, one per (ring phase, input buffer) pair
     bool graph_enabled = false;
     std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
+    // B = S = T angular spectrum + STFT: inverse row pass fused with the STFT
+    bool fused = false;
     // stage timer: events between row pass, column pass, inverse row pass,
     // image kernel(s) and the contrast / u16 stage
     bool timing = false;
@@ -136,7 +146,10 @@ hv_status hv_pipe::image_work() {
         // insert_compute_p_accu (converts.cc:76-92)
         const int pmin = (int)s.p_start;
         const int pmax = s.p_width ? std::min<int>(pmin + (int)s.p_width, Tn) : pmin;
-        if (s.time_transformation == HV_TIME_STFT) {
+        if (s.time_transformation == HV_TIME_STFT && fused) {
+            HV_TRY(launch_row_ifft_stft_image(image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
+                                              stream));
+        } else if (s.time_transformation == HV_TIME_STFT) {
             HV_TRY(launch_stft_image(image, ring, Tn, P, it, pmin, pmax, mm_img, stream));
         } else if (s.time_transformation == HV_TIME_PCA) {
             HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, stream));
@@ -190,7 +203,7 @@ hv_status hv_pipe::image_work() {
 hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     HV_TRY(mark(0));
     HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height, s.space_transformation,
-                                    lens, ring, T, slot0, stream, timing ? &ev[1] : nullptr));
+                                    lens, ring, T, slot0, stream, timing ? &ev[1] : nullptr, fused));
     HV_TRY(mark(3));
     HV_TRY(image_work());
     return mark(5);
@@ -297,6 +310,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->T = (int)s->time_transformation_size;
     p->B = (int)s->batch_size;
     p->lay = layout_of(*s);
+    p->fused = fused_eligible(*s);
     if (stream) {
         p->stream = as_stream(stream);
     } else {
@@ -465,6 +479,14 @@ hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t*
     if (ring) *ring = p->ring;
     if (start) *start = p->q_start;
     if (size) *size = p->q_size;
+    return HV_OK;
+}
+
+hv_status hv_pipe_set_fused(hv_pipe* p, int enabled) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    p->fused = enabled && fused_eligible(p->s);
+    for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+    p->graphs.clear();
     return HV_OK;
 }
 

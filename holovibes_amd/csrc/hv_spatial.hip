@@ -201,7 +201,9 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     using C = RowCfg<N>;
     __shared__ float2 lds[C::M * C::PITCH];
     const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
-    const int y = blockIdx.x * C::M + m;
+    const int yr = blockIdx.x * C::M + m;
+    const bool valid = yr < height;  // the last workgroup of a short frame has idle rows
+    const int y = valid ? yr : 0;
     const int b = blockIdx.y;
     const size_t P = (size_t)N * height;
     const size_t in_off = fin.out_frame(b) * P + (size_t)y * N;
@@ -210,6 +212,7 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     float2* my = lds + m * C::PITCH;
     auto load = [&](int n) -> float2 {
         float2 v;
+        if (!valid) return make_float2(0.f, 0.f);
         if constexpr (KIN == IN_U8) v = make_float2(convert_px<uint8_t>(((const uint8_t*)in)[in_off + n]), 0.f);
         else if constexpr (KIN == IN_U16) v = make_float2(convert_px<uint16_t>(((const uint16_t*)in)[in_off + n]), 0.f);
         else if constexpr (KIN == IN_F32) v = make_float2(((const float*)in)[in_off + n], 0.f);
@@ -217,7 +220,9 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS)
         if (crow) v = cmul(v, crow[n]);
         return v;
     };
-    auto store = [&](int n, float2 v) { dst[n] = scale == 1.0f ? v : cscale(v, scale); };
+    auto store = [&](int n, float2 v) {
+        if (valid) dst[n] = scale == 1.0f ? v : cscale(v, scale);
+    };
     auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
     fft_run<N, C::E, DIR>(t, load, store, lds_at);
 }
@@ -270,8 +275,7 @@ template <int N, int DIR>
 static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* chirp, int height, int batch,
                             FrameMap fin, FrameMap fm, float scale, hipStream_t st) {
     using C = RowCfg<N>;
-    dim3 grid(height / C::M, batch);
-    if (height % C::M) return fail(HV_ERR_UNSUPPORTED, "height not a multiple of the rows per workgroup");
+    dim3 grid((height + C::M - 1) / C::M, batch);
     switch (kin) {
         case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
         case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
@@ -357,8 +361,10 @@ hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width,
 
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
-                                   hipEvent_t* marks) {
-    // marks (optional): events recorded after pass 1 and after pass 2
+                                   hipEvent_t* marks, bool spectral_rows) {
+    // marks (optional): events recorded after pass 1 and after pass 2.
+    // spectral_rows: stop the angular spectrum before its inverse row pass
+    // (the fused row-IFFT + STFT kernel finishes it).
     const int kin = depth_to_kin(depth);
     if (kin < 0) return fail(HV_ERR_INVALID_ARG, "depth must be 1, 2 or 4");
     const size_t P = (size_t)width * height;
@@ -375,6 +381,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
         if (marks) HV_HIP(hipEventRecord(marks[0], st));
         HV_TRY(col_pass<COL_LENS>(ring, ring, lens, width, height, nb, slots, slots, st));
         if (marks) HV_HIP(hipEventRecord(marks[1], st));
+        if (spectral_rows) return HV_OK;
         return row_pass<+1>(ring, IN_C64, ring, nullptr, width, height, nb, slots, slots, 1.0f, st);
     }
     if (space == HV_SPACE_FRESNELTR) {

@@ -236,6 +236,121 @@ hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* rin
     return time_launch<TM_MOMENTS>(ring, T, P, prm, st);
 }
 
+// ---------------------------------------------------------------------------
+// B = S = T: inverse row FFT of every frame of one image row, then the STFT
+// of that row's pixels, in one workgroup.  The window is replaced whole by
+// every batch, so its frames never need to reach HBM in the spatial domain:
+// this kernel reads the column pass's output rows and writes one image row,
+// instead of writing the T frames back and reading them again.
+// ---------------------------------------------------------------------------
+template <int W, int T>
+struct RowStftCfg {
+    static constexpr int E = W >= 1024 ? 16 : 8;
+    static constexpr int NT = FftShape<W, E>::NT;                  // threads per row transform
+    static constexpr int THREADS = 512;                             // 256 VGPRs for the T-point FFT
+    static constexpr int G = THREADS / NT < T ? THREADS / NT : T;   // rows transformed at once
+    static constexpr bool PAD = (size_t)T * (W + W / 8) * 8 <= 160 * 1024;
+    static constexpr int PITCH = PAD ? W + W / 8 : W;
+    static constexpr bool FITS = (size_t)T * PITCH * 8 <= 160 * 1024 && T <= 32 && THREADS % NT == 0 &&
+                                 T % G == 0;
+};
+
+template <int W, int T>
+__global__ void __launch_bounds__(512)
+    k_row_ifft_stft(const float2* __restrict__ ring, size_t P, TimeParams prm) {
+    using C = RowStftCfg<W, T>;
+    __shared__ float2 tile[T * C::PITCH];
+    const int y = blockIdx.x;
+    const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    auto pad = [](int n) { return C::PAD ? n + (n >> 3) : n; };
+    // 1. inverse row FFTs, G frames at a time, results in the tile
+#pragma unroll 1
+    for (int f0 = 0; f0 < T; f0 += C::G) {
+        const int f = f0 + g;
+        const bool active = g < C::G;
+        float2* row = tile + (active ? f : 0) * C::PITCH;
+        const float2* src = ring + (size_t)(active ? f : 0) * P + (size_t)y * W;
+        auto load = [&](int n) -> float2 { return active ? src[n] : make_float2(0.f, 0.f); };
+        auto store = [&](int n, float2 v) {
+            if (active) row[pad(n)] = v;
+        };
+        float2 dummy;
+        auto lds_at = [&](int n) -> float2& { return active ? row[pad(n)] : dummy; };
+        fft_run<W, C::E, +1, true>(t, load, store, lds_at);
+        __syncthreads();
+    }
+    // 2. STFT of each pixel of the row across the T frames + image epilogue
+    float vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll 1
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        float2 v[T];
+#pragma unroll
+        for (int f = 0; f < T; ++f) v[f] = tile[f * C::PITCH + pad(x)];
+        fft_reg<T, -1>(v);
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < T; ++k)
+            if (k >= prm.pmin && k <= prm.pmax) acc += image_term(v[k], prm.img_type);
+        const float val = acc / (prm.pmax - prm.pmin + 1);
+        prm.out[(size_t)y * W + x] = val;
+        vmin = fminf(vmin, val);
+        vmax = fmaxf(vmax, val);
+    }
+    if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
+}
+
+template <int W, int T>
+static hv_status row_stft_wt(float* out, float2* ring, int height, size_t P, const TimeParams& prm, hipStream_t st) {
+    if constexpr (RowStftCfg<W, T>::FITS) {
+        k_row_ifft_stft<W, T><<<height, RowStftCfg<W, T>::THREADS, 0, st>>>(ring, P, prm);
+        HV_CHECK_LAUNCH("k_row_ifft_stft");
+        return HV_OK;
+    } else {
+        return fail(HV_ERR_UNSUPPORTED, "no fused row-IFFT + STFT kernel for this width and T");
+    }
+}
+
+template <int W>
+static hv_status row_stft_w(float* out, float2* ring, int height, int T, size_t P, const TimeParams& prm,
+                            hipStream_t st, bool query) {
+    switch (T) {
+        case 8: return query ? (RowStftCfg<W, 8>::FITS ? HV_OK : HV_ERR_UNSUPPORTED) : row_stft_wt<W, 8>(out, ring, height, P, prm, st);
+        case 16: return query ? (RowStftCfg<W, 16>::FITS ? HV_OK : HV_ERR_UNSUPPORTED) : row_stft_wt<W, 16>(out, ring, height, P, prm, st);
+        case 32: return query ? (RowStftCfg<W, 32>::FITS ? HV_OK : HV_ERR_UNSUPPORTED) : row_stft_wt<W, 32>(out, ring, height, P, prm, st);
+        case 64: return query ? (RowStftCfg<W, 64>::FITS ? HV_OK : HV_ERR_UNSUPPORTED) : row_stft_wt<W, 64>(out, ring, height, P, prm, st);
+        default: return HV_ERR_UNSUPPORTED;
+    }
+}
+
+static hv_status row_stft_dispatch(float* out, float2* ring, int width, int height, int T, const TimeParams& prm,
+                                   hipStream_t st, bool query) {
+    const size_t P = (size_t)width * height;
+    switch (width) {
+        case 64: return row_stft_w<64>(out, ring, height, T, P, prm, st, query);
+        case 128: return row_stft_w<128>(out, ring, height, T, P, prm, st, query);
+        case 256: return row_stft_w<256>(out, ring, height, T, P, prm, st, query);
+        case 512: return row_stft_w<512>(out, ring, height, T, P, prm, st, query);
+        case 1024: return row_stft_w<1024>(out, ring, height, T, P, prm, st, query);
+        default: return HV_ERR_UNSUPPORTED;
+    }
+}
+
+bool row_ifft_stft_supported(int width, int T) {
+    TimeParams prm{};
+    return row_stft_dispatch(nullptr, nullptr, width, 1, T, prm, nullptr, true) == HV_OK;
+}
+
+hv_status launch_row_ifft_stft_image(float* out, float2* ring, int width, int height, int T, int img_type, int pmin,
+                                     int pmax, uint32_t* mm, hipStream_t st) {
+    TimeParams prm{};
+    prm.out = out;
+    prm.img_type = img_type;
+    prm.pmin = pmin;
+    prm.pmax = pmax;
+    prm.mm = mm;
+    return row_stft_dispatch(out, ring, width, height, T, prm, st, false);
+}
+
 // complex_to_modulus / squared_modulus / argument on a stored spectrum
 // (tools_conversion.cu:11-30, 84-103, 199-232).
 __global__ void __launch_bounds__(256)

@@ -38,7 +38,8 @@ class Pipe:
     runs on torch's current stream unless `own_stream` (always with `graph`:
     the legacy default stream cannot be captured)."""
 
-    def __init__(self, settings: ComputeSettings, own_stream: bool = False, graph: bool = False):
+    def __init__(self, settings: ComputeSettings, own_stream: bool = False, graph: bool = False,
+                 fused: bool = True):
         self.settings = settings
         self.h = ctypes.c_void_p()
         own_stream = own_stream or graph
@@ -49,6 +50,8 @@ class Pipe:
         self.stream = st
         if graph:
             _lib.call("hv_pipe_set_graph", self.h, 1)
+        if not fused:
+            _lib.call("hv_pipe_set_fused", self.h, 0)
         self.P = settings.width * settings.height
 
     @staticmethod

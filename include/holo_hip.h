@@ -349,6 +349,14 @@ hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames);
 hv_status hv_pipe_set_timing(hv_pipe* p, int enabled);
 hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
 
+/* No reference counterpart: with B = S = T every batch replaces the whole
+ * window, so (angular spectrum + STFT, T <= 32, W <= 1024 and a T x W tile
+ * within LDS) the inverse row FFT pass is fused with the STFT and image
+ * epilogue; the ring then holds each frame after the column pass instead of
+ * the propagated frame.  On by default where eligible; 0 forces the
+ * three-pass + STFT sequence. */
+hv_status hv_pipe_set_fused(hv_pipe* p, int enabled);
+
 /* The time-transformation queue: ring base (T frames, complex), its
  * start index and size (Queue::get_start_index / get_size, queue.hh). */
 hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size);

@@ -17,9 +17,9 @@ def rel_l2(a, b):
     return float(np.linalg.norm((np.asarray(a, np.float64) - b).ravel()) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
-def run_both(oracle, settings, frames_list, graph=False, u16_exact_frac=0.99):
+def run_both(oracle, settings, frames_list, graph=False, u16_exact_frac=0.99, fused=True):
     from holovibes_amd.pipe import Pipe
-    p = Pipe(settings, graph=graph)
+    p = Pipe(settings, graph=graph, fused=fused)
     o = oracle.OraclePipe(settings, 64)
     results = []
     for fr in frames_list:
@@ -61,8 +61,18 @@ def frames(name, n_batches, **over):
     return s, [synth_frames(sp, k * B, B) for k in range(n_batches)]
 
 
-def test_pipe_c1_small(oracle):
+@pytest.mark.parametrize("fused", [True, False])
+def test_pipe_c1_small(oracle, fused):
+    """B = S = T: with the fused row-IFFT + STFT kernel and with the
+    three-pass + STFT sequence."""
     s, fl = frames("c1", 3, width=64, height=64)
+    check(run_both(oracle, s, fl, fused=fused), 1e-5)
+
+
+@pytest.mark.parametrize("W,H,T", [(128, 64, 8), (256, 32, 32), (512, 16, 16), (1024, 16, 8)])
+def test_pipe_fused_shapes(oracle, W, H, T):
+    s, fl = frames("c1", 1, width=W, height=H, batch_size=T, time_stride=T, time_transformation_size=T,
+                   p_start=1, p_width=T // 2 - 1)
     check(run_both(oracle, s, fl), 1e-5)
 
 
@@ -145,10 +155,11 @@ def test_pipe_graph_replay_equals_eager(oracle):
             assert torch.equal(eager.image(), graph.image())
 
 
-def test_pipe_metric_full_size(oracle):
+@pytest.mark.parametrize("fused", [True, False])
+def test_pipe_metric_full_size(oracle, fused):
     """The benchmark workload at full size (512 x 512, T = B = S = 32)."""
     s, fl = frames("metric", 1)
-    check(run_both(oracle, s, fl, graph=True), 1e-5)
+    check(run_both(oracle, s, fl, graph=True, fused=fused), 1e-5)
 
 
 def test_server_alone_vs_mix():
