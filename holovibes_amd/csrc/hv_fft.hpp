@@ -12,6 +12,8 @@
 // from double (hv_twiddles.inc); compile-time indices fold to constants.
 #pragma once
 
+#include <type_traits>
+
 #include "hv_common.hpp"
 #include "hv_twiddles.inc"  // generated into _build/gen by build.py
 
@@ -119,21 +121,36 @@ struct FftShape {
 // Every thread of the workgroup must call fft_run the same number of times
 // (it contains __syncthreads()).  STORE_SYNC puts a barrier before the last
 // stage's stores, for storers that write into the same LDS staging area.
+// Stage-0 inputs of thread t: v[k R + r] = x[t + k NT + r N / R].  Split
+// from fft_compute so a kernel can issue the loads of its next transform
+// before computing the current one.
+template <int N, int E, class Load>
+__device__ __forceinline__ void fft_load(int t, Load&& load, float2* v) {
+    using S = FftShape<N, E>;
+    constexpr int kE = S::kE, NT = S::NT, R = S::radix(0), NB = kE / R;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int j = t + k * NT;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[k * R + r] = load(j + r * (N / R));
+    }
+}
+
+template <int N, int E, int DIR, bool STORE_SYNC = false, class Store, class Lds>
+__device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds&& lds);
+
 template <int N, int E, int DIR, bool STORE_SYNC = false, class Load, class Store, class Lds>
 __device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&& lds) {
+    float2 v[FftShape<N, E>::kE];
+    fft_load<N, E>(t, load, v);
+    fft_compute<N, E, DIR, STORE_SYNC>(t, v, store, lds);
+}
+
+template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
+__device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds&& lds) {
     using S = FftShape<N, E>;
     constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
     constexpr int LOGN = clog2(N);
-    float2 v[kE];
-    {
-        constexpr int R = S::radix(0), NB = kE / R;
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            const int j = t + k * NT;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[k * R + r] = load(j + r * (N / R));
-        }
-    }
     static_for<0, NS>([&](auto s_) {
         constexpr int s = decltype(s_)::value;
         constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R;
@@ -149,10 +166,28 @@ __device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&&
         for (int k = 0; k < NB; ++k) {
             const int j = t + k * NT;
             if constexpr (Ns > 1) {
+                // twiddles w^r, w = exp(dir 2 pi i kk / (Ns R)): one table load,
+                // powers by a product tree of depth <= 4 (w^2, w^4, w^8, then products)
                 const int kk = j & (Ns - 1);
+                float2 wp[R];
+                wp[1] = twiddle<DIR>(kk * (N / (Ns * R)), LOGN);
+                if constexpr (R > 2) {
+                    wp[2] = cmul(wp[1], wp[1]);
+                    wp[3] = cmul(wp[2], wp[1]);
+                }
+                if constexpr (R > 4) {
+                    wp[4] = cmul(wp[2], wp[2]);
+                    wp[5] = cmul(wp[4], wp[1]);
+                    wp[6] = cmul(wp[4], wp[2]);
+                    wp[7] = cmul(wp[4], wp[3]);
+                }
+                if constexpr (R > 8) {
+                    wp[8] = cmul(wp[4], wp[4]);
 #pragma unroll
-                for (int r = 1; r < R; ++r)
-                    v[k * R + r] = cmul(v[k * R + r], twiddle<DIR>(r * kk * (N / (Ns * R)), LOGN));
+                    for (int r = 9; r < R; ++r) wp[r] = cmul(wp[8], wp[r - 8]);
+                }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[k * R + r] = cmul(v[k * R + r], wp[r]);
             }
             dft_reg<R, DIR>(&v[k * R]);
         }
@@ -163,7 +198,11 @@ __device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&&
                 const int j = t + k * NT;
                 const int d = (j / Ns) * Ns * R + (j & (Ns - 1));
 #pragma unroll
-                for (int r = 0; r < R; ++r) store(d + r * Ns, v[k * R + r]);
+                for (int r = 0; r < R; ++r) {
+                    // a storer may also take the register slot (k R + r) of the value
+                    if constexpr (std::is_invocable_v<Store, int, float2, int>) store(d + r * Ns, v[k * R + r], k * R + r);
+                    else store(d + r * Ns, v[k * R + r]);
+                }
             }
         } else {
             __syncthreads();
@@ -177,6 +216,17 @@ __device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&&
             __syncthreads();
         }
     });
+}
+
+// Output index of register slot (k R + r) of thread t in the last stage
+// (the n a storer receives for that value).
+template <int N, int E>
+__device__ __forceinline__ int fft_out_index(int t, int slot) {
+    using S = FftShape<N, E>;
+    constexpr int NS = S::stages(), R = S::radix(NS - 1), Ns = S::ns(NS - 1), NT = S::NT;
+    const int k = slot / R, r = slot % R;
+    const int j = t + k * NT;
+    return (j / Ns) * Ns * R + (j & (Ns - 1)) + r * Ns;
 }
 
 // Whole transform in one thread's registers (N <= 64): radix-2 DIF with

@@ -21,7 +21,8 @@ hv_status launch_angular_spectrum(float2* buf, int batch, const float2* lens, in
 hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width, int height, hipStream_t st);
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
-                                   hipEvent_t* marks = nullptr, bool spectral_rows = false);
+                                   hipEvent_t* marks = nullptr, bool spectral_rows = false,
+                                   uint32_t* mm_reset = nullptr);
 
 // hv_time.hip
 hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_t st);
@@ -47,8 +48,9 @@ hv_status launch_subsample_f32(float* out, const float* in, int width, int heigh
 hv_status launch_apply_contrast(float* img, size_t n, float mn, float mx, hipStream_t st);
 hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st);
 // range: 2 values, floats (encoded = 0) or ordered-uint min/max (encoded = 1).
+// range_out (optional): the decoded range actually applied (before invert).
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
-                                    int encoded, int invert, hipStream_t st);
+                                    int encoded, int invert, hipStream_t st, float* range_out = nullptr);
 hv_status launch_scale(float* img, size_t n, float factor, hipStream_t st);
 size_t percentile_scratch_bytes();
 // Writes range[0..1] = values of rank k0, k1 (ascending) of the image (or its

@@ -117,6 +117,13 @@ struct hv_pipe {
 
     hv_status image_work();
     hv_status enqueue_cycle(const void* frames, int slot0);
+    // min/max folded into the image kernel, accumulators re-armed by the
+    // cycle's first kernel (the row pass of a space transform)
+    bool minmax_fused() const {
+        return s.contrast_enabled && s.contrast_mode == HV_CONTRAST_MINMAX &&
+               !(s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d) && !s.fft_shift &&
+               s.space_transformation != HV_SPACE_NONE;
+    }
     hv_status mark(int i) {
         if (timing) HV_HIP(hipEventRecord(ev[i], stream));
         return HV_OK;
@@ -139,9 +146,9 @@ hv_status hv_pipe::image_work() {
     const int Tn = T;
     const bool minmax_mode = s.contrast_enabled && s.contrast_mode == HV_CONTRAST_MINMAX;
     const bool unwrap = it == HV_IMG_ARGUMENT && s.unwrap_2d;
-    // the image's min/max is folded in by the producing kernel when it is the final image
-    uint32_t* mm_img = (minmax_mode && !unwrap) ? mm : nullptr;
-    if (mm_img) HV_TRY(launch_minmax_init(mm, stream));
+    // the image's min/max is folded in by the producing kernel when it is the
+    // final image; the cycle's first kernel has re-armed the accumulators
+    uint32_t* mm_img = minmax_fused() ? mm : nullptr;
     if (is_image_kind(it)) {
         // insert_compute_p_accu (converts.cc:76-92)
         const int pmin = (int)s.p_start;
@@ -178,13 +185,13 @@ hv_status hv_pipe::image_work() {
     if (!s.contrast_enabled) return launch_float_to_ushort(out, image, P, stream);
     switch (s.contrast_mode) {
         case HV_CONTRAST_MINMAX:
-            if (unwrap || s.fft_shift) {
-                // recompute on the final image (unwrap rescales; shift keeps min/max but is cheap to cover)
+            if (!minmax_fused()) {
+                // min/max of the final image (unwrap rescales it; no first-kernel reset without a space transform)
                 HV_TRY(launch_minmax_init(mm, stream));
                 HV_TRY(launch_minmax(mm, image, P, stream));
             }
-            HV_TRY(launch_minmax_decode(range, mm, stream));
-            break;
+            // the contrast kernel decodes the accumulators and records the range it applied
+            return launch_contrast_to_ushort(out, cimage, image, P, mm, 1, s.contrast_invert, stream, range);
         case HV_CONTRAST_PERCENTILE:
             HV_TRY(launch_percentile(range, image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
                                      s.contrast_upper, 1, pct, stream));
@@ -203,7 +210,8 @@ hv_status hv_pipe::image_work() {
 hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     HV_TRY(mark(0));
     HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height, s.space_transformation,
-                                    lens, ring, T, slot0, stream, timing ? &ev[1] : nullptr, fused));
+                                    lens, ring, T, slot0, stream, timing ? &ev[1] : nullptr, fused,
+                                    minmax_fused() ? mm : nullptr));
     HV_TRY(mark(3));
     HV_TRY(image_work());
     return mark(5);

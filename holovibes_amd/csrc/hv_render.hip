@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) k_float_to_ushort(uint16_t* out, const fl
 
 __global__ void __launch_bounds__(256)
     k_contrast_u16(uint16_t* __restrict__ out, float* __restrict__ out_f32, const float* __restrict__ in, size_t n,
-                   const void* range, int encoded, int invert) {
+                   const void* range, int encoded, int invert, float* range_out) {
     float mn, mx;
     if (encoded) {
         mn = ordered_to_float(((const uint32_t*)range)[0]);
@@ -118,6 +118,10 @@ __global__ void __launch_bounds__(256)
     } else {
         mn = ((const float*)range)[0];
         mx = ((const float*)range)[1];
+    }
+    if (range_out && blockIdx.x == 0 && threadIdx.x == 0) {
+        range_out[0] = mn;
+        range_out[1] = mx;
     }
     if (invert) {
         const float t = mn;
@@ -164,8 +168,9 @@ hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipSt
     return HV_OK;
 }
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
-                                    int encoded, int invert, hipStream_t st) {
-    k_contrast_u16<<<blocks_for((n + 3) / 4, 256), 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert);
+                                    int encoded, int invert, hipStream_t st, float* range_out) {
+    k_contrast_u16<<<blocks_for((n + 3) / 4, 256), 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert,
+                                                                   range_out);
     HV_CHECK_LAUNCH("k_contrast_u16");
     return HV_OK;
 }

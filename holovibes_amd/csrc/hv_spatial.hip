@@ -194,37 +194,56 @@ struct FrameMap {
     }
 };
 
+// Each workgroup transforms `iters` groups of M rows of one frame; the loads
+// of group i + 1 are issued before group i is computed, so a workgroup keeps
+// two groups of input in flight.
 template <int N, int DIR, int KIN>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     k_row_fft(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
-              FrameMap fin, FrameMap fm, float scale) {
+              FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int iters) {
     using C = RowCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
     __shared__ float2 lds[C::M * C::PITCH];
+    // the cycle's first kernel re-arms the image min/max accumulators of its last kernels
+    if (mm_reset && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        mm_reset[0] = float_to_ordered(INFINITY);
+        mm_reset[1] = float_to_ordered(-INFINITY);
+    }
     const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
-    const int yr = blockIdx.x * C::M + m;
-    const bool valid = yr < height;  // the last workgroup of a short frame has idle rows
-    const int y = valid ? yr : 0;
     const int b = blockIdx.y;
     const size_t P = (size_t)N * height;
-    const size_t in_off = fin.out_frame(b) * P + (size_t)y * N;
-    float2* dst = out + fm.out_frame(b) * P + (size_t)y * N;
-    const float2* crow = chirp ? chirp + (size_t)y * N : nullptr;
+    const size_t in_frame = fin.out_frame(b) * P, out_frame = fm.out_frame(b) * P;
     float2* my = lds + m * C::PITCH;
-    auto load = [&](int n) -> float2 {
-        float2 v;
-        if (!valid) return make_float2(0.f, 0.f);
-        if constexpr (KIN == IN_U8) v = make_float2(convert_px<uint8_t>(((const uint8_t*)in)[in_off + n]), 0.f);
-        else if constexpr (KIN == IN_U16) v = make_float2(convert_px<uint16_t>(((const uint16_t*)in)[in_off + n]), 0.f);
-        else if constexpr (KIN == IN_F32) v = make_float2(((const float*)in)[in_off + n], 0.f);
-        else v = ((const float2*)in)[in_off + n];
-        if (crow) v = cmul(v, crow[n]);
-        return v;
-    };
-    auto store = [&](int n, float2 v) {
-        if (valid) dst[n] = scale == 1.0f ? v : cscale(v, scale);
-    };
     auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
-    fft_run<N, C::E, DIR>(t, load, store, lds_at);
+    auto row_of = [&](int it) { return (blockIdx.x * iters + it) * C::M + m; };
+    auto load_row = [&](int yr, float2* v) {
+        const bool valid = yr < height;  // idle rows of the last workgroup
+        const size_t in_off = in_frame + (size_t)(valid ? yr : 0) * N;
+        const float2* crow = chirp ? chirp + (size_t)(valid ? yr : 0) * N : nullptr;
+        fft_load<N, C::E>(t, [&](int n) -> float2 {
+            float2 x;
+            if (!valid) return make_float2(0.f, 0.f);
+            if constexpr (KIN == IN_U8) x = make_float2(convert_px<uint8_t>(((const uint8_t*)in)[in_off + n]), 0.f);
+            else if constexpr (KIN == IN_U16) x = make_float2(convert_px<uint16_t>(((const uint16_t*)in)[in_off + n]), 0.f);
+            else if constexpr (KIN == IN_F32) x = make_float2(((const float*)in)[in_off + n], 0.f);
+            else x = ((const float2*)in)[in_off + n];
+            if (crow) x = cmul(x, crow[n]);
+            return x;
+        }, v);
+    };
+    float2 cur[kE], nxt[kE];
+    load_row(row_of(0), cur);
+    for (int it = 0; it < iters; ++it) {
+        const int yr = row_of(it);
+        if (it + 1 < iters) load_row(row_of(it + 1), nxt);
+        float2* dst = out + out_frame + (size_t)(yr < height ? yr : 0) * N;
+        const bool valid = yr < height;
+        fft_compute<N, C::E, DIR>(t, cur, [&](int n, float2 v) {
+            if (valid) dst[n] = scale == 1.0f ? v : cscale(v, scale);
+        }, lds_at);
+#pragma unroll
+        for (int i = 0; i < kE; ++i) cur[i] = nxt[i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -240,31 +259,51 @@ struct ColCfg {
     static constexpr int THREADS = NT * CW;
 };
 
+// Each workgroup owns a CW-column tile and walks `fpb` frames of the batch,
+// loading the next frame's tile while the current one is transformed; the
+// lens values of its output positions stay in registers for all frames.
 template <int N, int MODE>
 __global__ void __launch_bounds__(ColCfg<N>::THREADS)
     k_col_fft(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ lens, int width,
-              FrameMap fin, FrameMap fm) {
+              FrameMap fin, FrameMap fm, int batch, int fpb) {
     using C = ColCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
     __shared__ float2 lds[N * C::CW];
     const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
     const int x = blockIdx.x * C::CW + c;
-    const int b = blockIdx.y;
+    const int b0 = blockIdx.y * fpb;
     const size_t P = (size_t)N * width;
-    const float2* src = in + fin.out_frame(b) * P + x;
-    float2* dst = out + fm.out_frame(b) * P + x;
     auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
-    auto load = [&](int n) -> float2 { return src[(size_t)n * width]; };
+    auto load_frame = [&](int b, float2* v) {
+        const float2* src = in + fin.out_frame(b < batch ? b : 0) * P + x;
+        fft_load<N, C::E>(t, [&](int n) -> float2 { return src[(size_t)n * width]; }, v);
+    };
+    float2 lr[kE];
     if constexpr (MODE == COL_LENS) {
-        // forward column FFT, x lens (already / (W H)), inverse column FFT
-        auto store_lds = [&](int n, float2 v) { lds_at(n) = cmul(v, lens[(size_t)n * width + x]); };
-        auto load_lds = [&](int n) -> float2 { return lds_at(n); };
+#pragma unroll
+        for (int i = 0; i < kE; ++i) lr[i] = lens[(size_t)fft_out_index<N, C::E>(t, i) * width + x];
+    }
+    float2 cur[kE], nxt[kE];
+    load_frame(b0, cur);
+    for (int i = 0; i < fpb; ++i) {
+        const int b = b0 + i;
+        if (b >= batch) break;  // uniform across the workgroup
+        if (i + 1 < fpb) load_frame(b + 1, nxt);
+        float2* dst = out + fm.out_frame(b) * P + x;
         auto store = [&](int n, float2 v) { dst[(size_t)n * width] = v; };
-        fft_run<N, C::E, -1, true>(t, load, store_lds, lds_at);
+        if constexpr (MODE == COL_LENS) {
+            // forward column FFT, x lens (already / (W H)), inverse column FFT
+            auto store_lds = [&](int n, float2 v, int slot) { lds_at(n) = cmul(v, lr[slot]); };
+            auto load_lds = [&](int n) -> float2 { return lds_at(n); };
+            fft_compute<N, C::E, -1, true>(t, cur, store_lds, lds_at);
+            __syncthreads();
+            fft_run<N, C::E, +1>(t, load_lds, store, lds_at);
+        } else {
+            fft_compute<N, C::E, MODE == COL_FWD ? -1 : +1>(t, cur, store, lds_at);
+        }
         __syncthreads();
-        fft_run<N, C::E, +1>(t, load_lds, store, lds_at);
-    } else {
-        auto store = [&](int n, float2 v) { dst[(size_t)n * width] = v; };
-        fft_run<N, C::E, MODE == COL_FWD ? -1 : +1>(t, load, store, lds_at);
+#pragma unroll
+        for (int k = 0; k < kE; ++k) cur[k] = nxt[k];
     }
 }
 
@@ -273,14 +312,18 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
 // ---------------------------------------------------------------------------
 template <int N, int DIR>
 static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* chirp, int height, int batch,
-                            FrameMap fin, FrameMap fm, float scale, hipStream_t st) {
+                            FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset) {
     using C = RowCfg<N>;
-    dim3 grid((height + C::M - 1) / C::M, batch);
+    // enough workgroups to fill the chip, each with a few row groups to pipeline
+    const int groups = (height + C::M - 1) / C::M;
+    int iters = 1;
+    while (iters < 8 && groups % (iters * 2) == 0 && (size_t)(groups / (iters * 2)) * batch >= 1024) iters *= 2;
+    dim3 grid(groups / iters, batch);
     switch (kin) {
-        case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
-        case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
-        case IN_F32: k_row_fft<N, DIR, IN_F32><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
-        default: k_row_fft<N, DIR, IN_C64><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale); break;
+        case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
+        case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
+        case IN_F32: k_row_fft<N, DIR, IN_F32><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
+        default: k_row_fft<N, DIR, IN_C64><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
     }
     HV_CHECK_LAUNCH("k_row_fft");
     return HV_OK;
@@ -288,17 +331,18 @@ static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* 
 
 template <int DIR>
 static hv_status row_pass(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
-                          int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st) {
+                          int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st,
+                          uint32_t* mm_reset = nullptr) {
     switch (width) {
-        case 16: return row_pass_n<16, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 32: return row_pass_n<32, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 64: return row_pass_n<64, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 128: return row_pass_n<128, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 256: return row_pass_n<256, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 512: return row_pass_n<512, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 1024: return row_pass_n<1024, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 2048: return row_pass_n<2048, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
-        case 4096: return row_pass_n<4096, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st);
+        case 16: return row_pass_n<16, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 32: return row_pass_n<32, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 64: return row_pass_n<64, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 128: return row_pass_n<128, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 256: return row_pass_n<256, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 512: return row_pass_n<512, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 1024: return row_pass_n<1024, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 2048: return row_pass_n<2048, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 4096: return row_pass_n<4096, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
         default: return fail(HV_ERR_UNSUPPORTED, "width must be a power of two in [16, 4096]");
     }
 }
@@ -308,8 +352,12 @@ static hv_status col_pass_n(const float2* in, float2* out, const float2* lens, i
                             FrameMap fin, FrameMap fm, hipStream_t st) {
     using C = ColCfg<N>;
     if (width % C::CW) return fail(HV_ERR_UNSUPPORTED, "width not a multiple of the column tile");
-    dim3 grid(width / C::CW, batch);
-    k_col_fft<N, MODE><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm);
+    // frames per workgroup: pipeline several while keeping >= 512 workgroups
+    const int tiles = width / C::CW;
+    int fpb = 1;
+    while (fpb < 8 && (size_t)tiles * ((batch + fpb * 2 - 1) / (fpb * 2)) >= 512) fpb *= 2;
+    dim3 grid(tiles, (batch + fpb - 1) / fpb);
+    k_col_fft<N, MODE><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm, batch, fpb);
     HV_CHECK_LAUNCH("k_col_fft");
     return HV_OK;
 }
@@ -361,7 +409,7 @@ hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width,
 
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
-                                   hipEvent_t* marks, bool spectral_rows) {
+                                   hipEvent_t* marks, bool spectral_rows, uint32_t* mm_reset) {
     // marks (optional): events recorded after pass 1 and after pass 2.
     // spectral_rows: stop the angular spectrum before its inverse row pass
     // (the fused row-IFFT + STFT kernel finishes it).
@@ -377,7 +425,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
     const FrameMap slots{ring_size, (slot0 + skip) % ring_size};
     if (space == HV_SPACE_ANGULARSP) {
         // pass 1 writes the frames into their ring slots; passes 2-3 run in place there
-        HV_TRY(row_pass<-1>(src, kin, ring, nullptr, width, height, nb, plain, slots, 1.0f, st));
+        HV_TRY(row_pass<-1>(src, kin, ring, nullptr, width, height, nb, plain, slots, 1.0f, st, mm_reset));
         if (marks) HV_HIP(hipEventRecord(marks[0], st));
         HV_TRY(col_pass<COL_LENS>(ring, ring, lens, width, height, nb, slots, slots, st));
         if (marks) HV_HIP(hipEventRecord(marks[1], st));
@@ -385,7 +433,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
         return row_pass<+1>(ring, IN_C64, ring, nullptr, width, height, nb, slots, slots, 1.0f, st);
     }
     if (space == HV_SPACE_FRESNELTR) {
-        HV_TRY(row_pass<-1>(src, kin, ring, lens, width, height, nb, plain, slots, 1.0f, st));
+        HV_TRY(row_pass<-1>(src, kin, ring, lens, width, height, nb, plain, slots, 1.0f, st, mm_reset));
         if (marks) {
             HV_HIP(hipEventRecord(marks[0], st));
             HV_HIP(hipEventRecord(marks[1], st));
@@ -393,6 +441,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
         return col_pass<COL_FWD>(ring, ring, nullptr, width, height, nb, slots, slots, st);
     }
     // no space transform: conversion only, frame by frame into the slots
+    if (mm_reset) HV_TRY(launch_minmax_init(mm_reset, st));
     for (int b = 0; b < nb; ++b) {
         float2* dst = ring + (size_t)((slots.slot0 + b) % ring_size) * P;
         HV_TRY(launch_convert(dst, src + (size_t)b * P * depth, P, depth, st));

@@ -251,7 +251,9 @@ struct RowStftCfg {
     static constexpr int G = THREADS / NT < T ? THREADS / NT : T;   // rows transformed at once
     static constexpr bool PAD = (size_t)T * (W + W / 8) * 8 <= 160 * 1024;
     static constexpr int PITCH = PAD ? W + W / 8 : W;
-    static constexpr bool FITS = (size_t)T * PITCH * 8 <= 160 * 1024 && T <= 32 && THREADS % NT == 0 &&
+    // threads beyond G row transforms work on a spare LDS row
+    static constexpr int ROWS = G * NT == THREADS ? T : T + 1;
+    static constexpr bool FITS = (size_t)ROWS * PITCH * 8 <= 160 * 1024 && T <= 32 && THREADS % NT == 0 &&
                                  T % G == 0;
 };
 
@@ -259,38 +261,41 @@ template <int W, int T>
 __global__ void __launch_bounds__(512)
     k_row_ifft_stft(const float2* __restrict__ ring, size_t P, TimeParams prm) {
     using C = RowStftCfg<W, T>;
-    __shared__ float2 tile[T * C::PITCH];
-    const int y = blockIdx.x;
+    constexpr int kE = FftShape<W, C::E>::kE;
+    constexpr int ROUNDS = T / C::G;
+    __shared__ float2 tile[C::ROWS * C::PITCH];
     const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    const bool active = g < C::G;
     auto pad = [](int n) { return C::PAD ? n + (n >> 3) : n; };
+    const int y = blockIdx.x;
+    // all ROUNDS x kE inputs of the row are loaded before any is transformed
+    float2 v[ROUNDS][kE];
+#pragma unroll
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+        const float2* src = ring + (size_t)(active ? rd * C::G + g : 0) * P + (size_t)y * W;
+        fft_load<W, C::E>(t, [&](int n) -> float2 { return active ? src[n] : make_float2(0.f, 0.f); }, v[rd]);
+    }
     // 1. inverse row FFTs, G frames at a time, results in the tile
-#pragma unroll 1
-    for (int f0 = 0; f0 < T; f0 += C::G) {
-        const int f = f0 + g;
-        const bool active = g < C::G;
-        float2* row = tile + (active ? f : 0) * C::PITCH;
-        const float2* src = ring + (size_t)(active ? f : 0) * P + (size_t)y * W;
-        auto load = [&](int n) -> float2 { return active ? src[n] : make_float2(0.f, 0.f); };
-        auto store = [&](int n, float2 v) {
-            if (active) row[pad(n)] = v;
-        };
-        float2 dummy;
-        auto lds_at = [&](int n) -> float2& { return active ? row[pad(n)] : dummy; };
-        fft_run<W, C::E, +1, true>(t, load, store, lds_at);
+#pragma unroll
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+        float2* row = tile + (active ? rd * C::G + g : T) * C::PITCH;
+        auto lds_at = [&](int n) -> float2& { return row[pad(n)]; };
+        auto store = [&](int n, float2 x) { row[pad(n)] = x; };
+        fft_compute<W, C::E, +1, true>(t, v[rd], store, lds_at);
         __syncthreads();
     }
     // 2. STFT of each pixel of the row across the T frames + image epilogue
     float vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll 1
     for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        float2 v[T];
+        float2 s[T];
 #pragma unroll
-        for (int f = 0; f < T; ++f) v[f] = tile[f * C::PITCH + pad(x)];
-        fft_reg<T, -1>(v);
+        for (int f = 0; f < T; ++f) s[f] = tile[f * C::PITCH + pad(x)];
+        fft_reg<T, -1>(s);
         float acc = 0.f;
 #pragma unroll
         for (int k = 0; k < T; ++k)
-            if (k >= prm.pmin && k <= prm.pmax) acc += image_term(v[k], prm.img_type);
+            if (k >= prm.pmin && k <= prm.pmax) acc += image_term(s[k], prm.img_type);
         const float val = acc / (prm.pmax - prm.pmin + 1);
         prm.out[(size_t)y * W + x] = val;
         vmin = fminf(vmin, val);
@@ -298,6 +303,7 @@ __global__ void __launch_bounds__(512)
     }
     if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
 }
+
 
 template <int W, int T>
 static hv_status row_stft_wt(float* out, float2* ring, int height, size_t P, const TimeParams& prm, hipStream_t st) {
