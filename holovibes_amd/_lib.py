@@ -9,7 +9,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libholo_hip.so")
+# HOLOVIBES_AMD_LIB selects another build of the library (e.g. an experiment variant)
+LIB_PATH = os.environ.get("HOLOVIBES_AMD_LIB", os.path.join(HERE, "libholo_hip.so"))
 
 HV_OK = 0
 HV_ERR_INVALID_ARG = 1

@@ -16,6 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 GEN = os.path.join(BUILD, "gen")
 LIB = os.path.join(HERE, "libholo_hip.so")
+# extra compiler flags (e.g. -D tile-shape overrides) for experiment builds
+EXTRA = os.environ.get("HV_EXTRA_FLAGS", "").split()
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HV_OFFLOAD_ARCH", "gfx950")
 
@@ -45,7 +47,7 @@ def _compile(src: str, force: bool) -> str:
     deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in HEADERS]
     if not force and os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps):
         return obj
-    cmd = [HIPCC] + CFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj + ".tmp"]
+    cmd = [HIPCC] + CFLAGS + EXTRA + ["-c", os.path.join(CSRC, src), "-o", obj + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -53,22 +55,32 @@ def _compile(src: str, force: bool) -> str:
     return obj
 
 
-def build(force: bool = True, jobs: int | None = None) -> str:
+def build(force: bool = True, jobs: int | None = None, out: str | None = None, build_dir: str | None = None) -> str:
     """Compiles every source (force: even when objects are newer) and links
-    libholo_hip.so.  Returns the library path."""
+    libholo_hip.so (or `out`).  Returns the library path."""
+    global BUILD
+    if build_dir:
+        BUILD = build_dir
     os.makedirs(BUILD, exist_ok=True)
     _gen_twiddles()
     jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    print(build(force="--incremental" not in sys.argv))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--incremental", action="store_true")
+    ap.add_argument("--out", default=None, help="library path (default holovibes_amd/libholo_hip.so)")
+    ap.add_argument("--build-dir", default=None, help="object directory (default holovibes_amd/_build)")
+    a = ap.parse_args()
+    print(build(force=not a.incremental, out=a.out, build_dir=a.build_dir))

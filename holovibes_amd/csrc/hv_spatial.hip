@@ -175,11 +175,31 @@ hv_status launch_shift_corners_f32(float* buf, int batch, int sx, int sy, hipStr
 // Input kinds of the row pass.
 enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 
+// Tile shapes (compile-time; the defaults were chosen by measurement on MI355X)
+#ifndef HV_ROW_E_SMALL
+#define HV_ROW_E_SMALL 8        // values per thread, row lengths <= 512
+#endif
+#ifndef HV_ROW_THREADS
+#define HV_ROW_THREADS 256      // threads per row workgroup
+#endif
+#ifndef HV_ROW_MIN_WG
+#define HV_ROW_MIN_WG 1024      // keep at least this many row workgroups when pipelining
+#endif
+#ifndef HV_COL_E_SMALL
+#define HV_COL_E_SMALL 8        // values per thread, column lengths <= 512
+#endif
+#ifndef HV_COL_CW_SMALL
+#define HV_COL_CW_SMALL 16      // columns per workgroup, column lengths <= 512
+#endif
+#ifndef HV_COL_MIN_WG
+#define HV_COL_MIN_WG 512       // keep at least this many column workgroups when pipelining
+#endif
+
 template <int N>
 struct RowCfg {
-    static constexpr int E = N >= 1024 ? 16 : 8;
+    static constexpr int E = N >= 1024 ? 16 : HV_ROW_E_SMALL;
     static constexpr int NT = FftShape<N, E>::NT;
-    static constexpr int M = (256 / NT) > 0 ? (256 / NT) : 1;  // rows per workgroup
+    static constexpr int M = (HV_ROW_THREADS / NT) > 0 ? (HV_ROW_THREADS / NT) : 1;  // rows per workgroup
     static constexpr int THREADS = M * NT;
     static constexpr int PITCH = N + N / 8;                      // padded LDS row
 };
@@ -253,9 +273,9 @@ enum { COL_FWD = 0, COL_INV = 1, COL_LENS = 2 };
 
 template <int N>
 struct ColCfg {
-    static constexpr int E = N >= 1024 ? 16 : 8;
+    static constexpr int E = N >= 1024 ? 16 : HV_COL_E_SMALL;
     static constexpr int NT = FftShape<N, E>::NT;
-    static constexpr int CW = N <= 512 ? 16 : (N == 1024 ? 8 : 4);  // columns per workgroup
+    static constexpr int CW = N <= 512 ? HV_COL_CW_SMALL : (N == 1024 ? 8 : 4);  // columns per workgroup
     static constexpr int THREADS = NT * CW;
 };
 
@@ -317,7 +337,7 @@ static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* 
     // enough workgroups to fill the chip, each with a few row groups to pipeline
     const int groups = (height + C::M - 1) / C::M;
     int iters = 1;
-    while (iters < 8 && groups % (iters * 2) == 0 && (size_t)(groups / (iters * 2)) * batch >= 1024) iters *= 2;
+    while (iters < 8 && groups % (iters * 2) == 0 && (size_t)(groups / (iters * 2)) * batch >= HV_ROW_MIN_WG) iters *= 2;
     dim3 grid(groups / iters, batch);
     switch (kin) {
         case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
@@ -355,7 +375,7 @@ static hv_status col_pass_n(const float2* in, float2* out, const float2* lens, i
     // frames per workgroup: pipeline several while keeping >= 512 workgroups
     const int tiles = width / C::CW;
     int fpb = 1;
-    while (fpb < 8 && (size_t)tiles * ((batch + fpb * 2 - 1) / (fpb * 2)) >= 512) fpb *= 2;
+    while (fpb < 8 && (size_t)tiles * ((batch + fpb * 2 - 1) / (fpb * 2)) >= HV_COL_MIN_WG) fpb *= 2;
     dim3 grid(tiles, (batch + fpb - 1) / fpb);
     k_col_fft<N, MODE><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm, batch, fpb);
     HV_CHECK_LAUNCH("k_col_fft");

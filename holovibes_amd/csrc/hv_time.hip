@@ -243,9 +243,13 @@ hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* rin
 // this kernel reads the column pass's output rows and writes one image row,
 // instead of writing the T frames back and reading them again.
 // ---------------------------------------------------------------------------
+#ifndef HV_FUSED_E_SMALL
+#define HV_FUSED_E_SMALL 8      // values per thread of the row transforms, W <= 512
+#endif
+
 template <int W, int T>
 struct RowStftCfg {
-    static constexpr int E = W >= 1024 ? 16 : 8;
+    static constexpr int E = W >= 1024 ? 16 : HV_FUSED_E_SMALL;
     static constexpr int NT = FftShape<W, E>::NT;                  // threads per row transform
     static constexpr int THREADS = 512;                             // 256 VGPRs for the T-point FFT
     static constexpr int G = THREADS / NT < T ? THREADS / NT : T;   // rows transformed at once
