@@ -29,6 +29,8 @@ def parse():
     ap.add_argument("--workload", default="metric")
     ap.add_argument("--latency-batches", type=int, default=200)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--path", type=int, default=2,
+                    help="highest cycle path allowed: 0 three passes + STFT, 1 fused row-IFFT + STFT, 2 bins-first")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -39,35 +41,58 @@ def parse():
     return ap.parse_args()
 
 
-def stage_bytes(s):
-    """Algorithmic HBM bytes per launch of each stage: every stage reads its
-    input once and writes its output once."""
+def stage_bytes(s, path):
+    """Algorithmic HBM bytes per launch of each stage (names, bytes): every
+    stage reads its input once and writes its output once.  `path` is the
+    pipe's cycle path (0 three passes + STFT, 1 inverse row pass fused with
+    the STFT, 2 bins-first)."""
     P = s.width * s.height
     B, T, d = s.batch_size, s.time_transformation_size, s.depth
     nb = min(B, T)
     space = s.space_transformation
+    contrast = ("contrast_u16", P * 4 + P * 2 + P * 4)  # image in, u16 out, contrasted float out
+    if path == 2:
+        pmin = s.p_start
+        pmax = min(pmin + s.p_width, T) if s.p_width else pmin
+        K = pmax - pmin + 1
+        rows = T * P * d + K * P * 8 + (P * 8 if space == 1 else 0)  # raw window in, K planes out (+ chirp)
+        cols = K * P * 16 + (P * 8 if space == 2 else 0)
+        image = K * P * 8 + P * 4
+        return [("time_dft_row_fft", rows), ("column_fft_lens_ifft" if space == 2 else "column_fft", cols),
+                ("empty", 0), ("row_ifft_image" if space == 2 else "bins_image", image), contrast]
     if space == 2:  # angular spectrum: row FFT (+convert), column FFT x lens x IFFT, inverse row FFT
         st0 = nb * P * (d + 8)
         st1 = nb * P * 16 + P * 8
-        st2 = nb * P * 16
+        st2 = 0 if path == 1 else nb * P * 16
     elif space == 1:  # Fresnel: row FFT (+convert, x chirp), column FFT
         st0 = nb * P * (d + 8) + P * 8
         st1 = nb * P * 16
         st2 = 0
     else:
         st0, st1, st2 = nb * P * (d + 8), 0, 0
-    if s.time_transformation == 1:
-        st3 = T * P * 8 + P * 4
-    elif s.time_transformation == 2:  # covariance read + projection read + image write
+    if s.time_transformation == 2:  # covariance read + projection read + image write
         st3 = 2 * T * P * 8 + P * 4
     else:
         st3 = T * P * 8 + P * 4
-    st4 = P * 4 + P * 2 + P * 4  # image in, u16 out, contrasted float out
-    return [st0, st1, st2, st3, st4]
+    t3 = "row_ifft_stft_image" if path == 1 else ["image", "stft_image", "pca_image"][s.time_transformation]
+    return [("row_fft_convert", st0), ("column_fft_lens_ifft" if space == 2 else "column_fft", st1),
+            ("empty" if path == 1 else "row_ifft_to_ring", st2), (t3, st3), contrast]
 
 
-STAGE_NAMES = ["row_fft_convert", "column_fft_lens_ifft", "row_ifft_to_ring", "time_transform_image",
-               "contrast_u16"]
+def image_error(s, batch, gpu_image):
+    """Relative L2 error of the GPU image against the FP64 oracle pipe on the
+    same batch (the window is full after one batch when B = S = T)."""
+    import numpy as np
+    from oracle import oracle as o
+    o.build()
+    pipe = o.OraclePipe(s, 64)
+    ref = None
+    for _ in range(max(1, s.time_transformation_size // s.batch_size)):
+        ref = pipe.push(batch)
+    if ref is None:
+        return None
+    r = ref["image"]
+    return float(np.linalg.norm((gpu_image.astype(np.float64) - r).ravel()) / np.linalg.norm(r.ravel()))
 
 
 def cpu_baseline(name, settings, batches, seconds):
@@ -101,8 +126,8 @@ def cpu_baseline(name, settings, batches, seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by source, "
-              "see DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the pipe shards by source, "
+              "one pipe per GPU", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -129,7 +154,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    pipe = Pipe(s, graph=not args.no_graph)
+    pipe_level = args.path
+    pipe = Pipe(s, graph=not args.no_graph, fused=pipe_level)
     k = 0
     while not pipe.push(dev_batches[k % nbatches]):  # fill the time window
         k += 1
@@ -165,15 +191,17 @@ def main():
         k += 1
     ms, cyc = pipe.stage_times()
     pipe.set_timing(False)
-    sb = stage_bytes(s)
+    path = pipe.path
+    line["config"]["path"] = ["three_pass", "row_ifft_stft_fused", "bins_first"][path]
+    sb = stage_bytes(s, path)
     stages = []
-    for nm, t, b in zip(STAGE_NAMES, ms, sb):
-        gbs = b / (t * 1e-3) / 1e9 if t > 0 else 0.0
+    for (nm, b), t in zip(sb, ms):
+        gbs = b / (t * 1e-3) / 1e9 if t > 0 and b > 0 else 0.0
         stages.append({"name": nm, "ms": t, "bytes": b, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS})
-    dom = max(range(5), key=lambda i: ms[i])
-    line["roofline"] = {"bound": "hbm", "kernel": STAGE_NAMES[dom], "achieved": stages[dom]["GB/s"],
+    dom = max((i for i in range(5) if sb[i][1] > 0), key=lambda i: ms[i])
+    line["roofline"] = {"bound": "hbm", "kernel": sb[dom][0], "achieved": stages[dom]["GB/s"],
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["frac"], "traffic": None}
-    tot_b, tot_ms = sum(sb), sum(ms)
+    tot_b, tot_ms = sum(b for _, b in sb), sum(ms)
     line["pipeline_roofline"] = {"bytes_per_step": tot_b, "ms_per_step_stages": tot_ms,
                                  "GB/s": tot_b / (tot_ms * 1e-3) / 1e9, "frac": tot_b / (tot_ms * 1e-3) / 1e9 /
                                  HBM_PEAK_GBS}
@@ -193,6 +221,16 @@ def main():
         line["p50_latency_ms"] = statistics.median(lat)
         line["p99_latency_ms"] = lat[min(len(lat) - 1, int(0.99 * len(lat)))]
         line["batch_period_ms_at_max_rate"] = ms_per_step
+    # image error against the FP64 oracle on one batch through a fresh pipe
+    check = Pipe(s, fused=pipe_level)
+    made = False
+    for _ in range(max(1, T // B)):
+        made = check.push(dev_batches[0])
+    if made:
+        err = image_error(s, host_batches[0], check.image().cpu().numpy())
+        if err is not None:
+            line["err_vs_fp64"] = {"image_rel_l2": err, "oracle": "C++ FP64 restatement of the reference pipe"}
+    check.close()
     pipe.close()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(name, s, host_batches, args.cpu_seconds)

@@ -1,0 +1,275 @@
+// hv_bins.hip -- the window-replacing STFT path computed bins-first.
+//
+// When B = S = T every batch replaces the whole time window
+// (pipe.cc:400-427, queue.cc:332-376), and the image reads only the STFT bins
+// k in [pmin, pmax] (converts.cc:76-92, tools_conversion.cu:11-30, 84-103,
+// 199-232).  The propagation L (angular spectrum, angular_spectrum.cu:33-63,
+// or Fresnel, fresnel_transform.cu:65-83) is one linear operator applied to
+// every frame, and the STFT (stft.cu:9-15) is a linear combination of frames:
+//     S_k = sum_t w^{k t} L(x_t) = L( sum_t w^{k t} x_t ),  w = exp(-2 pi i / T).
+// So this path takes the K = pmax - pmin + 1 needed bins of the converted
+// frames first and propagates K planes instead of T frames:
+//   1. k_bins_rows: per image row, the length-T DFT of every pixel of the T
+//      raw frames (physical ring-slot order), bins [pmin, pmax] kept in LDS,
+//      x chirp (Fresnel), then their forward row FFTs -> plane rows in HBM;
+//   2. the column pass over the K planes (x lens, inverse column FFT for the
+//      angular spectrum; forward only for Fresnel);
+//   3. k_bins_row_ifft_image (angular spectrum): inverse row FFT of each
+//      plane row + image term, summed over the planes in registers and LDS;
+//      k_bins_image (Fresnel): the image term summed over the planes.
+// The image equals the reference's up to FP32 rounding; bytes moved per batch
+// drop from about 3 x 16 B x T P (three FFT passes over the window) to
+// 2 B T P + 3 x 16 B K P.
+#include <algorithm>
+
+#include "hv_fft.hpp"
+#include "hv_kernels.hpp"
+
+namespace hv {
+
+namespace {
+
+constexpr int kBinThreads = 512;
+
+template <int W>
+struct BinRowCfg {
+    static constexpr int E = W >= 1024 ? 16 : 8;
+    static constexpr int NT = FftShape<W, E>::NT;          // threads per row transform
+    static constexpr int G = kBinThreads / NT;              // row transforms at once
+    static constexpr int PITCH = W + W / 8;                  // padded LDS row
+    static __device__ __forceinline__ int pad(int n) { return n + (n >> 3); }
+};
+
+// LDS bytes of k_bins_rows: K plane rows + one spare row for idle groups.
+template <int W>
+constexpr size_t bins_rows_lds(int K) {
+    return (size_t)(K + 1) * BinRowCfg<W>::PITCH * sizeof(float2);
+}
+
+template <int W, int T, typename TIn>
+__global__ void __launch_bounds__(kBinThreads)
+    k_bins_rows(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
+                int height, int slot0, int pmin, int K, uint32_t* mm_reset) {
+    using C = BinRowCfg<W>;
+    constexpr int kE = FftShape<W, C::E>::kE;
+    extern __shared__ float2 rows[];
+    if (mm_reset && blockIdx.x == 0 && threadIdx.x == 0) {
+        mm_reset[0] = float_to_ordered(INFINITY);
+        mm_reset[1] = float_to_ordered(-INFINITY);
+    }
+    const int y = blockIdx.x;
+    const size_t P = (size_t)W * height;
+    // 1. DFT over the window of each pixel.  Slot f of the ring holds frame
+    //    (f - slot0) mod T of the batch.
+#pragma unroll 1
+    for (int x = threadIdx.x; x < W; x += kBinThreads) {
+        float2 s[T];
+#pragma unroll
+        for (int f = 0; f < T; ++f) {
+            const int j = (f - slot0) & (T - 1);
+            s[f] = make_float2(convert_px<TIn>(raw[(size_t)j * P + (size_t)y * W + x]), 0.f);
+        }
+        fft_reg<T, -1>(s);
+        const float2 c = chirp ? chirp[(size_t)y * W + x] : make_float2(1.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < T; ++k)
+            if (k >= pmin && k < pmin + K) rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(s[k], c) : s[k];
+    }
+    __syncthreads();
+    // 2. forward row FFT of each plane row, G rows at a time, in place in LDS
+    const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    float2 tw[FftTw<W, C::E>::COUNT];
+    fft_twiddles<W, C::E>(t, tw);
+    const int rounds = (K + C::G - 1) / C::G;
+#pragma unroll 1
+    for (int r = 0; r < rounds; ++r) {
+        const int row = r * C::G + g;
+        const bool active = row < K;
+        float2* lrow = rows + (active ? row : K) * C::PITCH;
+        auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
+        float2 v[kE];
+        fft_load<W, C::E>(t, [&](int n) -> float2 { return lds_at(n); }, v);
+        float2* dst = planes + (size_t)(active ? row : 0) * P + (size_t)y * W;
+        fft_compute_tw<W, C::E, -1>(t, v, tw, [&](int n, float2 z) {
+            if (active) dst[n] = z;
+        }, lds_at);
+    }
+}
+
+// LDS bytes of k_bins_row_ifft_image: one exchange row per group.
+template <int W>
+constexpr size_t bins_ifft_lds() {
+    return (size_t)BinRowCfg<W>::G * BinRowCfg<W>::PITCH * sizeof(float2);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBinThreads)
+    k_bins_row_ifft_image(const float2* __restrict__ planes, float* __restrict__ out, int height, int K,
+                          int img_type, float count, uint32_t* mm) {
+    using C = BinRowCfg<W>;
+    constexpr int kE = FftShape<W, C::E>::kE;
+    __shared__ float2 xch[C::G * C::PITCH];
+    const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    const int y = blockIdx.x;
+    const size_t P = (size_t)W * height;
+    float2* lrow = xch + g * C::PITCH;
+    auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
+    auto load_row = [&](int row, float2* v) {
+        const float2* src = planes + (size_t)(row < K ? row : 0) * P + (size_t)y * W;
+        fft_load<W, C::E>(t, [&](int n) -> float2 { return src[n]; }, v);
+    };
+    float2 tw[FftTw<W, C::E>::COUNT];
+    fft_twiddles<W, C::E>(t, tw);
+    float acc[kE];
+#pragma unroll
+    for (int i = 0; i < kE; ++i) acc[i] = 0.f;
+    const int rounds = (K + C::G - 1) / C::G;
+    float2 cur[kE], nxt[kE];
+    load_row(g, cur);
+#pragma unroll 1
+    for (int r = 0; r < rounds; ++r) {
+        const bool active = r * C::G + g < K;
+        if (r + 1 < rounds) load_row((r + 1) * C::G + g, nxt);
+        fft_compute_tw<W, C::E, +1>(t, cur, tw, [&](int, float2 z, int slot) {
+            if (active) acc[slot] += image_term(z, img_type);
+        }, lds_at);
+#pragma unroll
+        for (int i = 0; i < kE; ++i) cur[i] = nxt[i];
+    }
+    // sum the G groups' partial sums of each pixel
+    __syncthreads();
+    float* part = reinterpret_cast<float*>(xch);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) part[g * W + fft_out_index<W, C::E>(t, i)] = acc[i];
+    __syncthreads();
+    float vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll 1
+    for (int x = threadIdx.x; x < W; x += kBinThreads) {
+        float a = 0.f;
+#pragma unroll
+        for (int q = 0; q < C::G; ++q) a += part[q * W + x];
+        const float val = a / count;
+        out[(size_t)y * W + x] = val;
+        vmin = fminf(vmin, val);
+        vmax = fmaxf(vmax, val);
+    }
+    if (mm) block_minmax_atomic(vmin, vmax, mm);
+}
+
+// Fresnel: the planes already hold the propagated bins.
+__global__ void __launch_bounds__(256)
+    k_bins_image(const float2* __restrict__ planes, float* __restrict__ out, size_t P, int K, int img_type,
+                 float count, uint32_t* mm) {
+    float vmin = INFINITY, vmax = -INFINITY;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+        float a = 0.f;
+        for (int k = 0; k < K; ++k) a += image_term(planes[(size_t)k * P + p], img_type);
+        const float val = a / count;
+        out[p] = val;
+        vmin = fminf(vmin, val);
+        vmax = fmaxf(vmax, val);
+    }
+    if (mm) block_minmax_atomic(vmin, vmax, mm);
+}
+
+template <int W, int T, typename TIn>
+hv_status rows_launch(const void* raw, float2* planes, const float2* chirp, int height, int slot0, int pmin, int K,
+                      uint32_t* mm_reset, hipStream_t st) {
+    // the plane rows may take more than the default 64 KiB of dynamic LDS
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bins_rows<W, T, TIn>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_bins_rows)");
+    k_bins_rows<W, T, TIn><<<height, kBinThreads, bins_rows_lds<W>(K), st>>>((const TIn*)raw, planes, chirp, height,
+                                                                            slot0, pmin, K, mm_reset);
+    HV_CHECK_LAUNCH("k_bins_rows");
+    return HV_OK;
+}
+
+template <int W, int T>
+hv_status rows_wt(const void* raw, int depth, float2* planes, const float2* chirp, int height, int slot0, int pmin,
+                  int K, uint32_t* mm_reset, hipStream_t st) {
+    switch (depth) {
+        case HV_DEPTH_8: return rows_launch<W, T, uint8_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case HV_DEPTH_16: return rows_launch<W, T, uint16_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        default: return rows_launch<W, T, float>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+    }
+}
+
+template <int W>
+hv_status rows_w(const void* raw, int depth, float2* planes, const float2* chirp, int height, int T, int slot0,
+                 int pmin, int K, uint32_t* mm_reset, hipStream_t st) {
+    switch (T) {
+        case 2: return rows_wt<W, 2>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case 4: return rows_wt<W, 4>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case 8: return rows_wt<W, 8>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case 16: return rows_wt<W, 16>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case 32: return rows_wt<W, 32>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case 64: return rows_wt<W, 64>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: T must be a power of two in [2, 64]");
+    }
+}
+
+size_t rows_lds_for(int width, int K) {
+    switch (width) {
+        case 64: return bins_rows_lds<64>(K);
+        case 128: return bins_rows_lds<128>(K);
+        case 256: return bins_rows_lds<256>(K);
+        case 512: return bins_rows_lds<512>(K);
+        case 1024: return bins_rows_lds<1024>(K);
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+bool bins_first_supported(int width, int height, int T, int K, int depth) {
+    if (!(width == 64 || width == 128 || width == 256 || width == 512 || width == 1024)) return false;
+    if (!is_pow2(height) || height < 16 || height > 4096) return false;
+    if (!is_pow2(T) || T < 2 || T > 64 || K < 1 || K > T) return false;
+    if (depth != HV_DEPTH_8 && depth != HV_DEPTH_16 && depth != HV_DEPTH_32F) return false;
+    return rows_lds_for(width, K) <= 160 * 1024;
+}
+
+hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
+                                int K, int space, const float2* lens, float2* planes, hipStream_t st,
+                                hipEvent_t mark, uint32_t* mm_reset) {
+    if (!bins_first_supported(width, height, T, K, depth))
+        return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
+    if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
+        return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
+    const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
+    switch (width) {
+        case 64: HV_TRY(rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
+        case 128: HV_TRY(rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
+        case 256: HV_TRY(rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
+        case 512: HV_TRY(rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
+        default: HV_TRY(rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
+    }
+    if (mark) HV_HIP(hipEventRecord(mark, st));
+    return launch_col_pass(planes, space == HV_SPACE_ANGULARSP ? lens : nullptr, width, height, K,
+                           space == HV_SPACE_ANGULARSP ? COLPASS_LENS : COLPASS_FWD, st);
+}
+
+hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st) {
+    const float count = (float)(pmax - pmin + 1);
+    if (space == HV_SPACE_FRESNELTR) {
+        const size_t P = (size_t)width * height;
+        int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
+        k_bins_image<<<blocks, 256, 0, st>>>(planes, out, P, K, img_type, count, mm);
+        HV_CHECK_LAUNCH("k_bins_image");
+        return HV_OK;
+    }
+    switch (width) {
+        case 64: k_bins_row_ifft_image<64><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
+        case 128: k_bins_row_ifft_image<128><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
+        case 256: k_bins_row_ifft_image<256><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
+        case 512: k_bins_row_ifft_image<512><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
+        case 1024: k_bins_row_ifft_image<1024><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
+        default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported width");
+    }
+    HV_CHECK_LAUNCH("k_bins_row_ifft_image");
+    return HV_OK;
+}
+
+}  // namespace hv

@@ -84,6 +84,26 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Pixel conversion of input_queue_to_input_buffer (tools_conversion.cu:134-141).
+template <typename TIn>
+__device__ __forceinline__ float convert_px(TIn v);
+template <>
+__device__ __forceinline__ float convert_px<uint8_t>(uint8_t v) { return static_cast<float>(v) * 257; }
+template <>
+__device__ __forceinline__ float convert_px<uint16_t>(uint16_t v) { return static_cast<float>(v); }
+template <>
+__device__ __forceinline__ float convert_px<float>(float v) { return v; }
+
+// Image-type contribution of one spectrum value (tools_conversion.cu:20-24, 96-100, 222-226).
+__device__ __forceinline__ float image_term(float2 z, int img_type) {
+    if (img_type == HV_IMG_MODULUS) return hypotf(z.x, z.y);
+    if (img_type == HV_IMG_SQUARED_MODULUS) {
+        const float h = hypotf(z.x, z.y);
+        return h * h;
+    }
+    return atan2f(z.y, z.x);
+}
+
 // Block min/max of v, then one atomic per block on the ordered encodings in
 // mm[0] (min) and mm[1] (max).  Every thread of the block must call it.
 // Deterministic: min and max do not depend on the order of updates.

@@ -146,11 +146,71 @@ __device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&&
     fft_compute<N, E, DIR, STORE_SYNC>(t, v, store, lds);
 }
 
+// Per-thread stage twiddles.  Thread t's twiddles depend only on t, so a
+// kernel that runs many transforms (frames, rows, forward + inverse) builds
+// them once: tw holds, for every stage s >= 1 and butterfly k of the thread,
+// the forward twiddles w^r, r = 1 .. R_s - 1 (the inverse uses conjugates).
+template <int N, int E>
+struct FftTw {
+    using S = FftShape<N, E>;
+    static constexpr int base(int s) {  // offset of stage s
+        int b = 0;
+        for (int i = 1; i < s; ++i) b += (S::kE / S::radix(i)) * (S::radix(i) - 1);
+        return b;
+    }
+    static constexpr int COUNT = base(S::stages()) > 0 ? base(S::stages()) : 1;
+};
+
+template <int N, int E>
+__device__ __forceinline__ void fft_twiddles(int t, float2* tw) {
+    using S = FftShape<N, E>;
+    constexpr int kE = S::kE, NT = S::NT, NS = S::stages(), LOGN = clog2(N);
+    static_for<1, NS>([&](auto s_) {
+        constexpr int s = decltype(s_)::value;
+        constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R, B0 = FftTw<N, E>::base(s);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int kk = (t + k * NT) & (Ns - 1);
+            float2* w = tw + B0 + k * (R - 1) - 1;  // w[r], r >= 1
+            w[1] = twiddle<-1>(kk * (N / (Ns * R)), LOGN);
+            if constexpr (R > 2) {
+                w[2] = cmul(w[1], w[1]);
+                w[3] = cmul(w[2], w[1]);
+            }
+            if constexpr (R > 4) {
+                w[4] = cmul(w[2], w[2]);
+                w[5] = cmul(w[4], w[1]);
+                w[6] = cmul(w[4], w[2]);
+                w[7] = cmul(w[4], w[3]);
+            }
+            if constexpr (R > 8) {
+                w[8] = cmul(w[4], w[4]);
+#pragma unroll
+                for (int r = 9; r < R; ++r) w[r] = cmul(w[8], w[r - 8]);
+            }
+        }
+    });
+}
+
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+template <int N, int E, int DIR, bool STORE_SYNC = false, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds);
+
 template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
 __device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds&& lds) {
+    float2 tw[FftTw<N, E>::COUNT];
+    fft_twiddles<N, E>(t, tw);
+    fft_compute_tw<N, E, DIR, STORE_SYNC>(t, v, tw, store, lds);
+}
+
+template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds) {
     using S = FftShape<N, E>;
     constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
-    constexpr int LOGN = clog2(N);
     static_for<0, NS>([&](auto s_) {
         constexpr int s = decltype(s_)::value;
         constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R;
@@ -166,28 +226,11 @@ __device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds
         for (int k = 0; k < NB; ++k) {
             const int j = t + k * NT;
             if constexpr (Ns > 1) {
-                // twiddles w^r, w = exp(dir 2 pi i kk / (Ns R)): one table load,
-                // powers by a product tree of depth <= 4 (w^2, w^4, w^8, then products)
-                const int kk = j & (Ns - 1);
-                float2 wp[R];
-                wp[1] = twiddle<DIR>(kk * (N / (Ns * R)), LOGN);
-                if constexpr (R > 2) {
-                    wp[2] = cmul(wp[1], wp[1]);
-                    wp[3] = cmul(wp[2], wp[1]);
-                }
-                if constexpr (R > 4) {
-                    wp[4] = cmul(wp[2], wp[2]);
-                    wp[5] = cmul(wp[4], wp[1]);
-                    wp[6] = cmul(wp[4], wp[2]);
-                    wp[7] = cmul(wp[4], wp[3]);
-                }
-                if constexpr (R > 8) {
-                    wp[8] = cmul(wp[4], wp[4]);
+                // w^r, w = exp(dir 2 pi i (j mod Ns) / (Ns R)), from the thread's table
+                const float2* w = tw + FftTw<N, E>::base(s) + k * (R - 1) - 1;
 #pragma unroll
-                    for (int r = 9; r < R; ++r) wp[r] = cmul(wp[8], wp[r - 8]);
-                }
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[k * R + r] = cmul(v[k * R + r], wp[r]);
+                for (int r = 1; r < R; ++r)
+                    v[k * R + r] = DIR < 0 ? cmul(v[k * R + r], w[r]) : cmulc(v[k * R + r], w[r]);
             }
             dft_reg<R, DIR>(&v[k * R]);
         }

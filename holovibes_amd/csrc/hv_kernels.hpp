@@ -19,6 +19,11 @@ hv_status launch_fft2(float2* buf, int batch, int width, int height, int directi
 hv_status launch_angular_spectrum(float2* buf, int batch, const float2* lens, int width, int height,
                                   float post_scale, hipStream_t st);
 hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width, int height, hipStream_t st);
+// One column pass over `planes` frames of `buf` (in place): forward, inverse,
+// or forward x lens then inverse.
+enum { COLPASS_FWD = 0, COLPASS_INV = 1, COLPASS_LENS = 2 };
+hv_status launch_col_pass(float2* buf, const float2* lens, int width, int height, int planes, int mode,
+                          hipStream_t st);
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
                                    hipEvent_t* marks = nullptr, bool spectral_rows = false,
@@ -39,6 +44,19 @@ hv_status launch_row_ifft_stft_image(float* out, float2* ring, int width, int he
                                      int pmax, uint32_t* mm, hipStream_t st);
 hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* ring, int T, size_t P, int f_start,
                               int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st);
+
+// hv_bins.hip: B = S = T STFT images computed bins-first (the STFT bins
+// [pmin, pmin + K) of the raw window are propagated instead of its T frames).
+bool bins_first_supported(int width, int height, int T, int K, int depth);
+// Bins, row FFTs and the column pass into planes[0 .. K) (K x P complex);
+// `mark` (optional) is recorded between the row and column passes.
+hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
+                                int K, int space, const float2* lens, float2* planes, hipStream_t st,
+                                hipEvent_t mark, uint32_t* mm_reset);
+// Image of the propagated planes: inverse row FFT (angular spectrum) + image
+// term, mean over [pmin, pmax]; mm: encoded min/max (may be null).
+hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st);
 
 // hv_render.hip
 hv_status launch_minmax_init(uint32_t* mm, hipStream_t st);

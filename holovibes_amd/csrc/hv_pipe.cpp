@@ -29,12 +29,43 @@ bool is_moment_kind(int t) { return t == HV_IMG_MOMENTS_0 || t == HV_IMG_MOMENTS
 
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 
+// insert_compute_p_accu (converts.cc:76-92): inclusive bin range of the image
+void p_range(const hv_compute_settings& s, int& pmin, int& pmax) {
+    const int T = (int)s.time_transformation_size;
+    pmin = (int)s.p_start;
+    pmax = s.p_width ? std::min<int>(pmin + (int)s.p_width, T) : pmin;
+}
+
+bool window_replaced(const hv_compute_settings& s) {
+    return s.batch_size == s.time_transformation_size && s.time_stride == s.batch_size;
+}
+
 // The whole window is replaced by every batch (B = S = T), so the angular
 // spectrum's inverse row pass can be fused with the STFT.
 bool fused_eligible(const hv_compute_settings& s) {
     return s.space_transformation == HV_SPACE_ANGULARSP && s.time_transformation == HV_TIME_STFT &&
-           is_image_kind(s.img_type) && s.batch_size == s.time_transformation_size &&
-           s.time_stride == s.batch_size && row_ifft_stft_supported((int)s.width, (int)s.time_transformation_size);
+           is_image_kind(s.img_type) && window_replaced(s) &&
+           row_ifft_stft_supported((int)s.width, (int)s.time_transformation_size);
+}
+
+// B = S = T STFT image with a space transform: the needed bins can be taken
+// before the propagation (the two are linear and commute).
+bool bins_eligible(const hv_compute_settings& s) {
+    if (!(s.space_transformation == HV_SPACE_ANGULARSP || s.space_transformation == HV_SPACE_FRESNELTR)) return false;
+    if (s.time_transformation != HV_TIME_STFT || !is_image_kind(s.img_type) || !window_replaced(s)) return false;
+    int pmin, pmax;
+    p_range(s, pmin, pmax);
+    return bins_first_supported((int)s.width, (int)s.height, (int)s.time_transformation_size, pmax - pmin + 1,
+                                (int)s.depth);
+}
+
+// Execution path of a steady-state cycle.
+enum { PATH_THREE_PASS = 0, PATH_ROW_STFT = 1, PATH_BINS = 2 };
+
+int choose_path(const hv_compute_settings& s, int level) {
+    if (level >= PATH_BINS && bins_eligible(s)) return PATH_BINS;
+    if (level >= PATH_ROW_STFT && fused_eligible(s)) return PATH_ROW_STFT;
+    return PATH_THREE_PASS;
 }
 
 struct Layout {
@@ -106,8 +137,9 @@ struct hv_pipe {
     // graphs, one per (ring phase, input buffer) pair
     bool graph_enabled = false;
     std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
-    // B = S = T angular spectrum + STFT: inverse row pass fused with the STFT
-    bool fused = false;
+    // cycle path (PATH_*): three passes + STFT, inverse row pass fused with
+    // the STFT, or bins-first
+    int path = PATH_THREE_PASS;
     // stage timer: events between row pass, column pass, inverse row pass,
     // image kernel(s) and the contrast / u16 stage
     bool timing = false;
@@ -150,10 +182,12 @@ hv_status hv_pipe::image_work() {
     // final image; the cycle's first kernel has re-armed the accumulators
     uint32_t* mm_img = minmax_fused() ? mm : nullptr;
     if (is_image_kind(it)) {
-        // insert_compute_p_accu (converts.cc:76-92)
-        const int pmin = (int)s.p_start;
-        const int pmax = s.p_width ? std::min<int>(pmin + (int)s.p_width, Tn) : pmin;
-        if (s.time_transformation == HV_TIME_STFT && fused) {
+        int pmin, pmax;
+        p_range(s, pmin, pmax);
+        if (path == PATH_BINS) {
+            HV_TRY(launch_bins_image(image, ring, (int)s.width, (int)s.height, pmax - pmin + 1,
+                                     s.space_transformation, it, pmin, pmax, mm_img, stream));
+        } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
             HV_TRY(launch_row_ifft_stft_image(image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
                                               stream));
         } else if (s.time_transformation == HV_TIME_STFT) {
@@ -209,9 +243,20 @@ hv_status hv_pipe::image_work() {
 
 hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     HV_TRY(mark(0));
-    HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height, s.space_transformation,
-                                    lens, ring, T, slot0, stream, timing ? &ev[1] : nullptr, fused,
-                                    minmax_fused() ? mm : nullptr));
+    if (path == PATH_BINS) {
+        // the K bin planes take the first K slots of the ring's memory
+        int pmin, pmax;
+        p_range(s, pmin, pmax);
+        HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, pmin,
+                                     pmax - pmin + 1, s.space_transformation, lens, ring, stream,
+                                     timing ? ev[1] : nullptr, minmax_fused() ? mm : nullptr));
+        HV_TRY(mark(2));
+    } else {
+        HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
+                                        s.space_transformation, lens, ring, T, slot0, stream,
+                                        timing ? &ev[1] : nullptr, path == PATH_ROW_STFT,
+                                        minmax_fused() ? mm : nullptr));
+    }
     HV_TRY(mark(3));
     HV_TRY(image_work());
     return mark(5);
@@ -318,7 +363,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->T = (int)s->time_transformation_size;
     p->B = (int)s->batch_size;
     p->lay = layout_of(*s);
-    p->fused = fused_eligible(*s);
+    p->path = choose_path(*s, PATH_BINS);
     if (stream) {
         p->stream = as_stream(stream);
     } else {
@@ -490,11 +535,18 @@ hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t*
     return HV_OK;
 }
 
-hv_status hv_pipe_set_fused(hv_pipe* p, int enabled) {
+hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
-    p->fused = enabled && fused_eligible(p->s);
+    if (level < 0) return fail(HV_ERR_INVALID_ARG, "fusion level must be >= 0");
+    p->path = choose_path(p->s, level);
     for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
     p->graphs.clear();
+    return HV_OK;
+}
+
+hv_status hv_pipe_path(const hv_pipe* p, int* path) {
+    if (!p || !path) return fail(HV_ERR_INVALID_ARG, "null argument");
+    *path = p->path;
     return HV_OK;
 }
 

@@ -21,15 +21,6 @@ namespace hv {
 // conversion (tools_conversion.cu:107-171)
 // ---------------------------------------------------------------------------
 template <typename TIn>
-__device__ __forceinline__ float convert_px(TIn v);
-template <>
-__device__ __forceinline__ float convert_px<uint8_t>(uint8_t v) { return static_cast<float>(v) * 257; }
-template <>
-__device__ __forceinline__ float convert_px<uint16_t>(uint16_t v) { return static_cast<float>(v); }
-template <>
-__device__ __forceinline__ float convert_px<float>(float v) { return v; }
-
-template <typename TIn>
 __global__ void __launch_bounds__(256) k_convert(float2* __restrict__ out, const TIn* __restrict__ in, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = make_float2(convert_px<TIn>(in[i]), 0.0f);
@@ -253,12 +244,14 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     };
     float2 cur[kE], nxt[kE];
     load_row(row_of(0), cur);
+    float2 tw[FftTw<N, C::E>::COUNT];
+    fft_twiddles<N, C::E>(t, tw);
     for (int it = 0; it < iters; ++it) {
         const int yr = row_of(it);
         if (it + 1 < iters) load_row(row_of(it + 1), nxt);
         float2* dst = out + out_frame + (size_t)(yr < height ? yr : 0) * N;
         const bool valid = yr < height;
-        fft_compute<N, C::E, DIR>(t, cur, [&](int n, float2 v) {
+        fft_compute_tw<N, C::E, DIR>(t, cur, tw, [&](int n, float2 v) {
             if (valid) dst[n] = scale == 1.0f ? v : cscale(v, scale);
         }, lds_at);
 #pragma unroll
@@ -305,6 +298,8 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
     }
     float2 cur[kE], nxt[kE];
     load_frame(b0, cur);
+    float2 tw[FftTw<N, C::E>::COUNT];
+    fft_twiddles<N, C::E>(t, tw);
     for (int i = 0; i < fpb; ++i) {
         const int b = b0 + i;
         if (b >= batch) break;  // uniform across the workgroup
@@ -315,11 +310,12 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
             // forward column FFT, x lens (already / (W H)), inverse column FFT
             auto store_lds = [&](int n, float2 v, int slot) { lds_at(n) = cmul(v, lr[slot]); };
             auto load_lds = [&](int n) -> float2 { return lds_at(n); };
-            fft_compute<N, C::E, -1, true>(t, cur, store_lds, lds_at);
+            fft_compute_tw<N, C::E, -1, true>(t, cur, tw, store_lds, lds_at);
             __syncthreads();
-            fft_run<N, C::E, +1>(t, load_lds, store, lds_at);
+            fft_load<N, C::E>(t, load_lds, cur);
+            fft_compute_tw<N, C::E, +1>(t, cur, tw, store, lds_at);
         } else {
-            fft_compute<N, C::E, MODE == COL_FWD ? -1 : +1>(t, cur, store, lds_at);
+            fft_compute_tw<N, C::E, MODE == COL_FWD ? -1 : +1>(t, cur, tw, store, lds_at);
         }
         __syncthreads();
 #pragma unroll
@@ -401,6 +397,16 @@ static hv_status col_pass(const float2* in, float2* out, const float2* lens, int
 
 static int depth_to_kin(int depth) {
     return depth == HV_DEPTH_8 ? IN_U8 : depth == HV_DEPTH_16 ? IN_U16 : depth == HV_DEPTH_32F ? IN_F32 : -1;
+}
+
+hv_status launch_col_pass(float2* buf, const float2* lens, int width, int height, int planes, int mode,
+                          hipStream_t st) {
+    const FrameMap fm{0, 0};
+    switch (mode) {
+        case COLPASS_FWD: return col_pass<COL_FWD>(buf, buf, nullptr, width, height, planes, fm, fm, st);
+        case COLPASS_INV: return col_pass<COL_INV>(buf, buf, nullptr, width, height, planes, fm, fm, st);
+        default: return col_pass<COL_LENS>(buf, buf, lens, width, height, planes, fm, fm, st);
+    }
 }
 
 hv_status launch_fft2(float2* buf, int batch, int width, int height, int direction, hipStream_t st) {

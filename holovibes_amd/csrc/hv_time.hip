@@ -30,16 +30,6 @@ struct TimeParams {
     int mm_moment;     // which moment feeds mm (0, 1, 2)
 };
 
-// Image-type contribution of one spectrum value (tools_conversion.cu:20-24, 96-100, 222-226).
-__device__ __forceinline__ float image_term(float2 z, int img_type) {
-    if (img_type == HV_IMG_MODULUS) return hypotf(z.x, z.y);
-    if (img_type == HV_IMG_SQUARED_MODULUS) {
-        const float h = hypotf(z.x, z.y);
-        return h * h;
-    }
-    return atan2f(z.y, z.x);
-}
-
 // fft_freqs (icompute.cc:26-69): f1[i] = i d for i < T/2, -(T - i) d above
 // (even T), d = fps / T, in float as the reference computes it.
 template <int T>
@@ -280,12 +270,14 @@ __global__ void __launch_bounds__(512)
         fft_load<W, C::E>(t, [&](int n) -> float2 { return active ? src[n] : make_float2(0.f, 0.f); }, v[rd]);
     }
     // 1. inverse row FFTs, G frames at a time, results in the tile
+    float2 tw[FftTw<W, C::E>::COUNT];
+    fft_twiddles<W, C::E>(t, tw);
 #pragma unroll
     for (int rd = 0; rd < ROUNDS; ++rd) {
         float2* row = tile + (active ? rd * C::G + g : T) * C::PITCH;
         auto lds_at = [&](int n) -> float2& { return row[pad(n)]; };
         auto store = [&](int n, float2 x) { row[pad(n)] = x; };
-        fft_compute<W, C::E, +1, true>(t, v[rd], store, lds_at);
+        fft_compute_tw<W, C::E, +1, true>(t, v[rd], tw, store, lds_at);
         __syncthreads();
     }
     // 2. STFT of each pixel of the row across the T frames + image epilogue

@@ -38,8 +38,11 @@ class Pipe:
     runs on torch's current stream unless `own_stream` (always with `graph`:
     the legacy default stream cannot be captured)."""
 
+    # cycle paths (hv_pipe_set_fused levels)
+    THREE_PASS, ROW_STFT, BINS = 0, 1, 2
+
     def __init__(self, settings: ComputeSettings, own_stream: bool = False, graph: bool = False,
-                 fused: bool = True):
+                 fused: int = 2):
         self.settings = settings
         self.h = ctypes.c_void_p()
         own_stream = own_stream or graph
@@ -50,8 +53,10 @@ class Pipe:
         self.stream = st
         if graph:
             _lib.call("hv_pipe_set_graph", self.h, 1)
-        if not fused:
-            _lib.call("hv_pipe_set_fused", self.h, 0)
+        # fused: True (every eligible path, the default), False (three passes) or a level
+        level = self.BINS if fused is True else int(fused)
+        if level != self.BINS:
+            _lib.call("hv_pipe_set_fused", self.h, level)
         self.P = settings.width * settings.height
 
     @staticmethod
@@ -71,6 +76,13 @@ class Pipe:
         produced = ctypes.c_int()
         _lib.call("hv_pipe_push_batch", self.h, ctypes.c_void_p(ptr), ctypes.byref(produced))
         return bool(produced.value)
+
+    @property
+    def path(self) -> int:
+        """Cycle path in use: THREE_PASS, ROW_STFT or BINS."""
+        v = ctypes.c_int()
+        _lib.call("hv_pipe_path", self.h, ctypes.byref(v))
+        return v.value
 
     def set_timing(self, enabled: bool) -> None:
         _lib.call("hv_pipe_set_timing", self.h, 1 if enabled else 0)

@@ -349,13 +349,22 @@ hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames);
 hv_status hv_pipe_set_timing(hv_pipe* p, int enabled);
 hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
 
-/* No reference counterpart: with B = S = T every batch replaces the whole
- * window, so (angular spectrum + STFT, T <= 32, W <= 1024 and a T x W tile
- * within LDS) the inverse row FFT pass is fused with the STFT and image
- * epilogue; the ring then holds each frame after the column pass instead of
- * the propagated frame.  On by default where eligible; 0 forces the
- * three-pass + STFT sequence. */
-hv_status hv_pipe_set_fused(hv_pipe* p, int enabled);
+/* No reference counterpart.  With B = S = T every batch replaces the whole
+ * window, which allows two shorter cycle paths for STFT images:
+ *   level 1 -- angular spectrum, T <= 32, a T x W tile within LDS: the
+ *              inverse row FFT pass is fused with the STFT and image
+ *              epilogue; the ring holds each frame after the column pass
+ *              instead of the propagated frame;
+ *   level 2 -- angular spectrum or Fresnel, T <= 64, W in 64..1024: the
+ *              STFT bins [pmin, pmax] of the raw window are taken first and
+ *              only those K planes are propagated (the STFT and the
+ *              propagation are linear and commute); the ring's first K
+ *              frames hold the planes.
+ * hv_pipe_set_fused picks the highest eligible path at or below `level`
+ * (default: 2); 0 forces the three-pass + STFT sequence.  hv_pipe_path
+ * reports the path in use (0, 1 or 2). */
+hv_status hv_pipe_set_fused(hv_pipe* p, int level);
+hv_status hv_pipe_path(const hv_pipe* p, int* path);
 
 /* The time-transformation queue: ring base (T frames, complex), its
  * start index and size (Queue::get_start_index / get_size, queue.hh). */

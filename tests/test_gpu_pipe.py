@@ -18,9 +18,13 @@ def rel_l2(a, b):
     return float(np.linalg.norm((np.asarray(a, np.float64) - b).ravel()) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
-def run_both(oracle, settings, frames_list, graph=False, fused=True):
+def run_both(oracle, settings, frames_list, graph=False, fused=True, path=None):
+    """fused: highest cycle path allowed (True = all); path: the path the
+    pipe must have chosen (None: not checked)."""
     from holovibes_amd.pipe import Pipe
     p = Pipe(settings, graph=graph, fused=fused)
+    if path is not None:
+        assert p.path == path, (p.path, path)
     o = oracle.OraclePipe(settings, 64)
     results = []
     for fr in frames_list:
@@ -62,12 +66,35 @@ def frames(name, n_batches, **over):
     return s, [synth_frames(sp, k * B, B) for k in range(n_batches)]
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_pipe_c1_small(oracle, fused):
-    """B = S = T: with the fused row-IFFT + STFT kernel and with the
-    three-pass + STFT sequence."""
+@pytest.mark.parametrize("level", [2, 1, 0])
+def test_pipe_c1_small(oracle, level):
+    """B = S = T on each cycle path: bins-first (2), fused row-IFFT + STFT
+    (1) and the three-pass + STFT sequence (0)."""
     s, fl = frames("c1", 3, width=64, height=64)
-    check(run_both(oracle, s, fl, fused=fused), 1e-5)
+    check(run_both(oracle, s, fl, fused=level, path=level), 1e-5)
+
+
+@pytest.mark.parametrize("W,H,T,p0,pw", [(64, 16, 2, 0, 1), (128, 64, 4, 1, 2), (256, 32, 8, 0, 7),
+                                         (512, 16, 16, 2, 13), (1024, 16, 8, 3, 4), (64, 64, 64, 5, 40),
+                                         (512, 32, 32, 0, 31), (1024, 32, 16, 15, 0)])
+def test_pipe_bins_shapes(oracle, W, H, T, p0, pw):
+    """Bins-first path over widths, window sizes and bin ranges (single bin,
+    all bins, a range reaching T - 1)."""
+    s, fl = frames("c1", 2, width=W, height=H, batch_size=T, time_stride=T, time_transformation_size=T,
+                   p_start=p0, p_width=pw)
+    check(run_both(oracle, s, fl, path=2), 1e-5)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4])
+@pytest.mark.parametrize("space", [1, 2])
+def test_pipe_bins_depths(oracle, depth, space):
+    """Bins-first path from u8 / u16 / f32 frames, Fresnel and angular spectrum."""
+    syn = dict(syn_depth=1, syn_bits=8, syn_mean_counts=64.0) if depth == 1 else {}
+    s, fl = frames("c1", 2, width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16,
+                   depth=depth, space_transformation=space, z_distance=0.25, **syn)
+    if depth == 4:
+        fl = [f.astype(np.float32) * 0.5 for f in fl]
+    check(run_both(oracle, s, fl, path=2), 1e-5)
 
 
 @pytest.mark.parametrize("W,H,T", [(128, 64, 8), (256, 32, 32), (512, 16, 16), (1024, 16, 8)])
@@ -156,11 +183,11 @@ def test_pipe_graph_replay_equals_eager(oracle):
             assert torch.equal(eager.image(), graph.image())
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_pipe_metric_full_size(oracle, fused):
-    """The benchmark workload at full size (512 x 512, T = B = S = 32)."""
+@pytest.mark.parametrize("level", [2, 1, 0])
+def test_pipe_metric_full_size(oracle, level):
+    """The benchmark workload at full size (512 x 512, T = B = S = 32) on each cycle path."""
     s, fl = frames("metric", 1)
-    check(run_both(oracle, s, fl, graph=True, fused=fused), 1e-5)
+    check(run_both(oracle, s, fl, graph=True, fused=level, path=level), 1e-5)
 
 
 def test_server_alone_vs_mix():
