@@ -29,6 +29,15 @@ namespace hv {
 
 namespace {
 
+// image rows per workgroup of the two row kernels (pipelined: the next
+// row's loads overlap this row's transforms)
+#ifndef HV_BINS_RPW_ROWS
+#define HV_BINS_RPW_ROWS 1
+#endif
+#ifndef HV_BINS_RPW_IMAGE
+#define HV_BINS_RPW_IMAGE 2
+#endif
+
 constexpr int kBinThreads = 512;
 
 template <int W>
@@ -37,6 +46,7 @@ struct BinRowCfg {
     static constexpr int NT = FftShape<W, E>::NT;          // threads per row transform
     static constexpr int G = kBinThreads / NT;              // row transforms at once
     static constexpr int PITCH = W + W / 8;                  // padded LDS row
+    static constexpr int PPT = (W + kBinThreads - 1) / kBinThreads;  // pixels per thread in the time DFT
     static __device__ __forceinline__ int pad(int n) { return n + (n >> 3); }
 };
 
@@ -46,124 +56,168 @@ constexpr size_t bins_rows_lds(int K) {
     return (size_t)(K + 1) * BinRowCfg<W>::PITCH * sizeof(float2);
 }
 
+// The next row's window is prefetched into registers while the current
+// row's plane rows are transformed, when it fits beside the DFT registers.
+template <int W, int T>
+constexpr bool bins_rows_prefetch() {
+    return HV_BINS_RPW_ROWS > 1 && T * BinRowCfg<W>::PPT <= 32;
+}
+
 template <int W, int T, typename TIn>
 __global__ void __launch_bounds__(kBinThreads)
     k_bins_rows(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
                 int height, int slot0, int pmin, int K, uint32_t* mm_reset) {
+    constexpr int rpw = HV_BINS_RPW_ROWS;
     using C = BinRowCfg<W>;
     constexpr int kE = FftShape<W, C::E>::kE;
+    constexpr bool PF = bins_rows_prefetch<W, T>();
     extern __shared__ float2 rows[];
     if (mm_reset && blockIdx.x == 0 && threadIdx.x == 0) {
         mm_reset[0] = float_to_ordered(INFINITY);
         mm_reset[1] = float_to_ordered(-INFINITY);
     }
-    const int y = blockIdx.x;
     const size_t P = (size_t)W * height;
-    // 1. DFT over the window of each pixel.  Slot f of the ring holds frame
-    //    (f - slot0) mod T of the batch.
-#pragma unroll 1
-    for (int x = threadIdx.x; x < W; x += kBinThreads) {
-        float2 s[T];
+    const int y0 = blockIdx.x * rpw;
+    const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    // Slot f of the ring holds frame (f - slot0) mod T of the batch.
+    auto load_window = [&](int y, int x, float* xr) {
 #pragma unroll
         for (int f = 0; f < T; ++f) {
             const int j = (f - slot0) & (T - 1);
-            s[f] = make_float2(convert_px<TIn>(raw[(size_t)j * P + (size_t)y * W + x]), 0.f);
+            xr[f] = convert_px<TIn>(raw[(size_t)j * P + (size_t)y * W + x]);
         }
-        fft_reg<T, -1>(s);
-        const float2 c = chirp ? chirp[(size_t)y * W + x] : make_float2(1.f, 0.f);
+    };
+    float xr[PF ? C::PPT : 1][T];
+    if constexpr (PF) {
 #pragma unroll
-        for (int k = 0; k < T; ++k)
-            if (k >= pmin && k < pmin + K) rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(s[k], c) : s[k];
+        for (int q = 0; q < C::PPT; ++q)
+            if (threadIdx.x + q * kBinThreads < W) load_window(y0, threadIdx.x + q * kBinThreads, xr[q]);
     }
-    __syncthreads();
-    // 2. forward row FFT of each plane row, G rows at a time, in place in LDS
-    const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
-    float2 tw[FftTw<W, C::E>::COUNT];
-    fft_twiddles<W, C::E>(t, tw);
-    const int rounds = (K + C::G - 1) / C::G;
+#pragma unroll
+    for (int i = 0; i < rpw; ++i) {
+        const int y = y0 + i;
+        if (y >= height) break;  // uniform across the workgroup
+        // 1. DFT over the window of each pixel; bins [pmin, pmin + K) to LDS
+#pragma unroll
+        for (int q = 0; q < C::PPT; ++q) {
+            const int x = threadIdx.x + q * kBinThreads;
+            if (x >= W) continue;
+            if constexpr (!PF) load_window(y, x, xr[0]);
+            float2 s[T];
+#pragma unroll
+            for (int f = 0; f < T; ++f) s[f] = make_float2(xr[PF ? q : 0][f], 0.f);
+            fft_reg<T, -1>(s);
+            const float2 c = chirp ? chirp[(size_t)y * W + x] : make_float2(1.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < T; ++k)
+                if (k >= pmin && k < pmin + K) rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(s[k], c) : s[k];
+        }
+        if constexpr (PF) {
+            if (i + 1 < rpw && y + 1 < height) {
+#pragma unroll
+                for (int q = 0; q < C::PPT; ++q)
+                    if (threadIdx.x + q * kBinThreads < W) load_window(y + 1, threadIdx.x + q * kBinThreads, xr[q]);
+            }
+        }
+        __syncthreads();
+        // 2. forward row FFT of each plane row, G rows at a time, in place in LDS
+        float2 tw[FftTw<W, C::E>::COUNT];
+        fft_twiddles<W, C::E>(t, tw);
+        const int rounds = (K + C::G - 1) / C::G;
 #pragma unroll 1
-    for (int r = 0; r < rounds; ++r) {
-        const int row = r * C::G + g;
-        const bool active = row < K;
-        float2* lrow = rows + (active ? row : K) * C::PITCH;
-        auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
-        float2 v[kE];
-        fft_load<W, C::E>(t, [&](int n) -> float2 { return lds_at(n); }, v);
-        float2* dst = planes + (size_t)(active ? row : 0) * P + (size_t)y * W;
-        fft_compute_tw<W, C::E, -1>(t, v, tw, [&](int n, float2 z) {
-            if (active) dst[n] = z;
-        }, lds_at);
+        for (int r = 0; r < rounds; ++r) {
+            const int row = r * C::G + g;
+            const bool active = row < K;
+            float2* lrow = rows + (active ? row : K) * C::PITCH;
+            auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
+            float2 v[kE];
+            fft_load<W, C::E>(t, [&](int n) -> float2 { return lds_at(n); }, v);
+            float2* dst = planes + (size_t)(active ? row : 0) * P + (size_t)y * W;
+            fft_compute_tw<W, C::E, -1, false, (C::NT <= 64)>(t, v, tw, [&](int n, float2 z) {
+                if (active) dst[n] = z;
+            }, lds_at);
+        }
+        __syncthreads();  // the plane rows are rewritten by the next row's DFT
     }
 }
 
-// LDS bytes of k_bins_row_ifft_image: one exchange row per group.
-template <int W>
-constexpr size_t bins_ifft_lds() {
-    return (size_t)BinRowCfg<W>::G * BinRowCfg<W>::PITCH * sizeof(float2);
-}
-
-template <int W>
+template <int W, int IMG>
 __global__ void __launch_bounds__(kBinThreads)
-    k_bins_row_ifft_image(const float2* __restrict__ planes, float* __restrict__ out, int height, int K,
-                          int img_type, float count, uint32_t* mm) {
+    k_bins_row_ifft_image(const float2* __restrict__ planes, float* __restrict__ out, int height, int K, int rpw,
+                          float count, uint32_t* mm) {
     using C = BinRowCfg<W>;
     constexpr int kE = FftShape<W, C::E>::kE;
     __shared__ float2 xch[C::G * C::PITCH];
     const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
-    const int y = blockIdx.x;
     const size_t P = (size_t)W * height;
+    const int y0 = blockIdx.x * rpw;
+    const int nrows = min(rpw, height - y0);
+    const int rounds = (K + C::G - 1) / C::G;
+    const int items = nrows * rounds;  // (row, round) pairs, loads one item ahead
     float2* lrow = xch + g * C::PITCH;
     auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
-    auto load_row = [&](int row, float2* v) {
-        const float2* src = planes + (size_t)(row < K ? row : 0) * P + (size_t)y * W;
+    auto load_item = [&](int it, float2* v) {
+        const int row = (it % rounds) * C::G + g;
+        const float2* src = planes + (size_t)(row < K ? row : 0) * P + (size_t)(y0 + it / rounds) * W;
         fft_load<W, C::E>(t, [&](int n) -> float2 { return src[n]; }, v);
     };
     float2 tw[FftTw<W, C::E>::COUNT];
     fft_twiddles<W, C::E>(t, tw);
     float acc[kE];
-#pragma unroll
-    for (int i = 0; i < kE; ++i) acc[i] = 0.f;
-    const int rounds = (K + C::G - 1) / C::G;
     float2 cur[kE], nxt[kE];
-    load_row(g, cur);
-#pragma unroll 1
-    for (int r = 0; r < rounds; ++r) {
-        const bool active = r * C::G + g < K;
-        if (r + 1 < rounds) load_row((r + 1) * C::G + g, nxt);
-        fft_compute_tw<W, C::E, +1>(t, cur, tw, [&](int, float2 z, int slot) {
-            if (active) acc[slot] += image_term(z, img_type);
-        }, lds_at);
-#pragma unroll
-        for (int i = 0; i < kE; ++i) cur[i] = nxt[i];
-    }
-    // sum the G groups' partial sums of each pixel
-    __syncthreads();
+    load_item(0, cur);
     float* part = reinterpret_cast<float*>(xch);
-#pragma unroll
-    for (int i = 0; i < kE; ++i) part[g * W + fft_out_index<W, C::E>(t, i)] = acc[i];
-    __syncthreads();
     float vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll 1
-    for (int x = threadIdx.x; x < W; x += kBinThreads) {
-        float a = 0.f;
+    for (int it = 0; it < items; ++it) {
+        const int r = it % rounds;
+        const bool active = r * C::G + g < K;
+        if (it + 1 < items) load_item(it + 1, nxt);
+        if (r == 0) {
 #pragma unroll
-        for (int q = 0; q < C::G; ++q) a += part[q * W + x];
-        const float val = a / count;
-        out[(size_t)y * W + x] = val;
-        vmin = fminf(vmin, val);
-        vmax = fmaxf(vmax, val);
+            for (int i = 0; i < kE; ++i) acc[i] = 0.f;
+        }
+        fft_compute_tw<W, C::E, +1, false, (C::NT <= 64)>(t, cur, tw, [&](int, float2 z, int slot) {
+            if (active) acc[slot] += image_term_fast<IMG>(z);
+        }, lds_at);
+        if (r == rounds - 1) {
+            // sum the G groups' partial sums of each pixel of this row
+            const int y = y0 + it / rounds;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kE; ++i) part[g * W + fft_out_index<W, C::E>(t, i)] = acc[i];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < C::PPT; ++q) {
+                const int x = threadIdx.x + q * kBinThreads;
+                if (x >= W) continue;
+                float a = 0.f;
+#pragma unroll
+                for (int j = 0; j < C::G; ++j) a += part[j * W + x];
+                const float val = a / count;
+                out[(size_t)y * W + x] = val;
+                vmin = fminf(vmin, val);
+                vmax = fmaxf(vmax, val);
+            }
+            // the exchange rows alias `part`: the next row's transforms start
+            // once every wave has read its sums
+            if (it + 1 < items) __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < kE; ++i) cur[i] = nxt[i];
     }
     if (mm) block_minmax_atomic(vmin, vmax, mm);
 }
 
 // Fresnel: the planes already hold the propagated bins.
+template <int IMG>
 __global__ void __launch_bounds__(256)
-    k_bins_image(const float2* __restrict__ planes, float* __restrict__ out, size_t P, int K, int img_type,
-                 float count, uint32_t* mm) {
+    k_bins_image(const float2* __restrict__ planes, float* __restrict__ out, size_t P, int K, float count,
+                 uint32_t* mm) {
     float vmin = INFINITY, vmax = -INFINITY;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
         float a = 0.f;
-        for (int k = 0; k < K; ++k) a += image_term(planes[(size_t)k * P + p], img_type);
+        for (int k = 0; k < K; ++k) a += image_term_fast<IMG>(planes[(size_t)k * P + p]);
         const float val = a / count;
         out[p] = val;
         vmin = fminf(vmin, val);
@@ -179,8 +233,9 @@ hv_status rows_launch(const void* raw, float2* planes, const float2* chirp, int 
     static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bins_rows<W, T, TIn>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_bins_rows)");
-    k_bins_rows<W, T, TIn><<<height, kBinThreads, bins_rows_lds<W>(K), st>>>((const TIn*)raw, planes, chirp, height,
-                                                                            slot0, pmin, K, mm_reset);
+    const int rpw = HV_BINS_RPW_ROWS;
+    k_bins_rows<W, T, TIn><<<(height + rpw - 1) / rpw, kBinThreads, bins_rows_lds<W>(K), st>>>(
+        (const TIn*)raw, planes, chirp, height, slot0, pmin, K, mm_reset);
     HV_CHECK_LAUNCH("k_bins_rows");
     return HV_OK;
 }
@@ -250,26 +305,40 @@ hv_status launch_bins_propagate(const void* raw, int depth, int width, int heigh
                            space == HV_SPACE_ANGULARSP ? COLPASS_LENS : COLPASS_FWD, st);
 }
 
-hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st) {
-    const float count = (float)(pmax - pmin + 1);
+template <int IMG>
+hv_status bins_image_t(float* out, const float2* planes, int width, int height, int K, int space, float count,
+                       uint32_t* mm, hipStream_t st) {
     if (space == HV_SPACE_FRESNELTR) {
         const size_t P = (size_t)width * height;
-        int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
-        k_bins_image<<<blocks, 256, 0, st>>>(planes, out, P, K, img_type, count, mm);
+        const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
+        k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm);
         HV_CHECK_LAUNCH("k_bins_image");
         return HV_OK;
     }
+    const int rpw = HV_BINS_RPW_IMAGE;
+    const dim3 grid((height + rpw - 1) / rpw);
     switch (width) {
-        case 64: k_bins_row_ifft_image<64><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
-        case 128: k_bins_row_ifft_image<128><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
-        case 256: k_bins_row_ifft_image<256><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
-        case 512: k_bins_row_ifft_image<512><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
-        case 1024: k_bins_row_ifft_image<1024><<<height, kBinThreads, 0, st>>>(planes, out, height, K, img_type, count, mm); break;
+        case 64: k_bins_row_ifft_image<64, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 128: k_bins_row_ifft_image<128, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 256: k_bins_row_ifft_image<256, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 512: k_bins_row_ifft_image<512, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported width");
     }
     HV_CHECK_LAUNCH("k_bins_row_ifft_image");
     return HV_OK;
+}
+
+hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st) {
+    const float count = (float)(pmax - pmin + 1);
+    switch (img_type) {
+        case HV_IMG_MODULUS: return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, st);
+        case HV_IMG_SQUARED_MODULUS:
+            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, st);
+        case HV_IMG_ARGUMENT: return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: image type must be modulus, squared or argument");
+    }
 }
 
 }  // namespace hv

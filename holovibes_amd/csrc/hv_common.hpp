@@ -104,6 +104,16 @@ __device__ __forceinline__ float image_term(float2 z, int img_type) {
     return atan2f(z.y, z.x);
 }
 
+// Compile-time image type with a short modulus: sqrt(x^2 + y^2) with the
+// hardware square root (1 ulp) instead of hypotf's scaled evaluation; the
+// operands here stay far from overflow (|z| < 1e19).
+template <int IMG>
+__device__ __forceinline__ float image_term_fast(float2 z) {
+    if constexpr (IMG == HV_IMG_MODULUS) return __builtin_amdgcn_sqrtf(fmaf(z.x, z.x, z.y * z.y));
+    else if constexpr (IMG == HV_IMG_SQUARED_MODULUS) return fmaf(z.x, z.x, z.y * z.y);
+    else return atan2f(z.y, z.x);
+}
+
 // Block min/max of v, then one atomic per block on the ordered encodings in
 // mm[0] (min) and mm[1] (max).  Every thread of the block must call it.
 // Deterministic: min and max do not depend on the order of updates.

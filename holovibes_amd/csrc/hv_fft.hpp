@@ -197,7 +197,26 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-template <int N, int E, int DIR, bool STORE_SYNC = false, class Store, class Lds>
+#ifndef HV_FFT_WAVE_SYNC
+#define HV_FFT_WAVE_SYNC 1
+#endif
+
+// Exchange barrier of one transform.  WAVE: the transform's threads are one
+// wavefront (or part of one); LDS operations of a wavefront are performed in
+// order, so the exchange needs no workgroup barrier, only a fence that keeps
+// the compiler from moving LDS accesses across it.
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync() {
+    if constexpr (WAVE && HV_FFT_WAVE_SYNC) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <int N, int E, int DIR, bool STORE_SYNC = false, bool WAVE = false, class Store, class Lds>
 __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds);
 
 template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
@@ -207,8 +226,9 @@ __device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds
     fft_compute_tw<N, E, DIR, STORE_SYNC>(t, v, tw, store, lds);
 }
 
-template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
+template <int N, int E, int DIR, bool STORE_SYNC, bool WAVE, class Store, class Lds>
 __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds) {
+    static_assert(!WAVE || FftShape<N, E>::NT <= 64, "a wave-local transform has at most 64 threads");
     using S = FftShape<N, E>;
     constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
     static_for<0, NS>([&](auto s_) {
@@ -235,7 +255,7 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
             dft_reg<R, DIR>(&v[k * R]);
         }
         if constexpr (s == NS - 1) {
-            if constexpr (STORE_SYNC) __syncthreads();
+            if constexpr (STORE_SYNC) fft_sync<WAVE>();
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
                 const int j = t + k * NT;
@@ -248,7 +268,7 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
                 }
             }
         } else {
-            __syncthreads();
+            fft_sync<WAVE>();
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
                 const int j = t + k * NT;
@@ -256,7 +276,7 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
 #pragma unroll
                 for (int r = 0; r < R; ++r) lds(d + r * Ns) = v[k * R + r];
             }
-            __syncthreads();
+            fft_sync<WAVE>();
         }
     });
 }

@@ -183,7 +183,7 @@ enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 #define HV_COL_CW_SMALL 16      // columns per workgroup, column lengths <= 512
 #endif
 #ifndef HV_COL_MIN_WG
-#define HV_COL_MIN_WG 512       // keep at least this many column workgroups when pipelining
+#define HV_COL_MIN_WG 216       // keep at least this many column workgroups when pipelining (~ one per CU)
 #endif
 
 template <int N>
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS)
         if (it + 1 < iters) load_row(row_of(it + 1), nxt);
         float2* dst = out + out_frame + (size_t)(yr < height ? yr : 0) * N;
         const bool valid = yr < height;
-        fft_compute_tw<N, C::E, DIR>(t, cur, tw, [&](int n, float2 v) {
+        fft_compute_tw<N, C::E, DIR, false, (C::NT <= 64)>(t, cur, tw, [&](int n, float2 v) {
             if (valid) dst[n] = scale == 1.0f ? v : cscale(v, scale);
         }, lds_at);
 #pragma unroll

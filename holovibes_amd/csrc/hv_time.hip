@@ -277,9 +277,10 @@ __global__ void __launch_bounds__(512)
         float2* row = tile + (active ? rd * C::G + g : T) * C::PITCH;
         auto lds_at = [&](int n) -> float2& { return row[pad(n)]; };
         auto store = [&](int n, float2 x) { row[pad(n)] = x; };
-        fft_compute_tw<W, C::E, +1, true>(t, v[rd], tw, store, lds_at);
-        __syncthreads();
+        fft_compute_tw<W, C::E, +1, true, (C::NT <= 64)>(t, v[rd], tw, store, lds_at);
+        if constexpr (C::NT > 64) __syncthreads();
     }
+    __syncthreads();
     // 2. STFT of each pixel of the row across the T frames + image epilogue
     float vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll 1
