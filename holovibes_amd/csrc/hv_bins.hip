@@ -38,15 +38,29 @@ namespace {
 #define HV_BINS_RPW_IMAGE 2
 #endif
 
-constexpr int kBinThreads = 512;
+#ifndef HV_BINS_IMG_THREADS
+#define HV_BINS_IMG_THREADS 512
+#endif
+#ifndef HV_BINS_IMG_DEPTH
+#define HV_BINS_IMG_DEPTH 1     // plane rows loaded ahead by the image kernel
+#endif
 
+constexpr int kBinThreads = 512;  // time DFT + row FFT kernel
+
+// inverse row FFT + image kernel: its exchange rows and partial sums stay
+// within LDS
 template <int W>
+constexpr int img_threads() {
+    return W >= 1024 ? 512 : HV_BINS_IMG_THREADS;
+}
+
+template <int W, int THREADS = kBinThreads>
 struct BinRowCfg {
     static constexpr int E = W >= 1024 ? 16 : 8;
     static constexpr int NT = FftShape<W, E>::NT;          // threads per row transform
-    static constexpr int G = kBinThreads / NT;              // row transforms at once
+    static constexpr int G = THREADS / NT;                  // row transforms at once
     static constexpr int PITCH = W + W / 8;                  // padded LDS row
-    static constexpr int PPT = (W + kBinThreads - 1) / kBinThreads;  // pixels per thread in the time DFT
+    static constexpr int PPT = (W + THREADS - 1) / THREADS;  // pixels per thread
     static __device__ __forceinline__ int pad(int n) { return n + (n >> 3); }
 };
 
@@ -142,21 +156,24 @@ __global__ void __launch_bounds__(kBinThreads)
 }
 
 template <int W, int IMG>
-__global__ void __launch_bounds__(kBinThreads)
+__global__ void __launch_bounds__(img_threads<W>())
     k_bins_row_ifft_image(const float2* __restrict__ planes, float* __restrict__ out, int height, int K, int rpw,
                           float count, uint32_t* mm) {
-    using C = BinRowCfg<W>;
+    constexpr int kImgThreads = img_threads<W>();
+    using C = BinRowCfg<W, kImgThreads>;
     constexpr int kE = FftShape<W, C::E>::kE;
     __shared__ float2 xch[C::G * C::PITCH];
+    __shared__ float part[C::G * W];
     const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
     const size_t P = (size_t)W * height;
     const int y0 = blockIdx.x * rpw;
     const int nrows = min(rpw, height - y0);
     const int rounds = (K + C::G - 1) / C::G;
-    const int items = nrows * rounds;  // (row, round) pairs, loads one item ahead
+    const int items = nrows * rounds;  // (row, round) pairs, loaded ahead
     float2* lrow = xch + g * C::PITCH;
     auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
     auto load_item = [&](int it, float2* v) {
+        if (it >= items) return;
         const int row = (it % rounds) * C::G + g;
         const float2* src = planes + (size_t)(row < K ? row : 0) * P + (size_t)(y0 + it / rounds) * W;
         fft_load<W, C::E>(t, [&](int n) -> float2 { return src[n]; }, v);
@@ -164,32 +181,34 @@ __global__ void __launch_bounds__(kBinThreads)
     float2 tw[FftTw<W, C::E>::COUNT];
     fft_twiddles<W, C::E>(t, tw);
     float acc[kE];
-    float2 cur[kE], nxt[kE];
-    load_item(0, cur);
-    float* part = reinterpret_cast<float*>(xch);
+    // D items in flight beyond the one being transformed
+    constexpr int D = HV_BINS_IMG_DEPTH;
+    float2 buf[D + 1][kE];
+#pragma unroll
+    for (int d = 0; d < D; ++d) load_item(d, buf[d]);
     float vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll 1
     for (int it = 0; it < items; ++it) {
         const int r = it % rounds;
         const bool active = r * C::G + g < K;
-        if (it + 1 < items) load_item(it + 1, nxt);
+        load_item(it + D, buf[D]);
         if (r == 0) {
 #pragma unroll
             for (int i = 0; i < kE; ++i) acc[i] = 0.f;
         }
-        fft_compute_tw<W, C::E, +1, false, (C::NT <= 64)>(t, cur, tw, [&](int, float2 z, int slot) {
+        fft_compute_tw<W, C::E, +1, false, (C::NT <= 64)>(t, buf[0], tw, [&](int, float2 z, int slot) {
             if (active) acc[slot] += image_term_fast<IMG>(z);
         }, lds_at);
         if (r == rounds - 1) {
             // sum the G groups' partial sums of each pixel of this row
             const int y = y0 + it / rounds;
-            __syncthreads();
+            __syncthreads();  // the previous row's sums have been read
 #pragma unroll
             for (int i = 0; i < kE; ++i) part[g * W + fft_out_index<W, C::E>(t, i)] = acc[i];
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < C::PPT; ++q) {
-                const int x = threadIdx.x + q * kBinThreads;
+                const int x = threadIdx.x + q * kImgThreads;
                 if (x >= W) continue;
                 float a = 0.f;
 #pragma unroll
@@ -199,12 +218,11 @@ __global__ void __launch_bounds__(kBinThreads)
                 vmin = fminf(vmin, val);
                 vmax = fmaxf(vmax, val);
             }
-            // the exchange rows alias `part`: the next row's transforms start
-            // once every wave has read its sums
-            if (it + 1 < items) __syncthreads();
         }
 #pragma unroll
-        for (int i = 0; i < kE; ++i) cur[i] = nxt[i];
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int i = 0; i < kE; ++i) buf[d][i] = buf[d + 1][i];
     }
     if (mm) block_minmax_atomic(vmin, vmax, mm);
 }
@@ -318,11 +336,11 @@ hv_status bins_image_t(float* out, const float2* planes, int width, int height, 
     const int rpw = HV_BINS_RPW_IMAGE;
     const dim3 grid((height + rpw - 1) / rpw);
     switch (width) {
-        case 64: k_bins_row_ifft_image<64, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 128: k_bins_row_ifft_image<128, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 256: k_bins_row_ifft_image<256, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 512: k_bins_row_ifft_image<512, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, kBinThreads, 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 64: k_bins_row_ifft_image<64, IMG><<<grid, img_threads<64>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 128: k_bins_row_ifft_image<128, IMG><<<grid, img_threads<128>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 256: k_bins_row_ifft_image<256, IMG><<<grid, img_threads<256>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 512: k_bins_row_ifft_image<512, IMG><<<grid, img_threads<512>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, img_threads<1024>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported width");
     }
     HV_CHECK_LAUNCH("k_bins_row_ifft_image");

@@ -216,7 +216,12 @@ __device__ __forceinline__ void fft_sync() {
     }
 }
 
-template <int N, int E, int DIR, bool STORE_SYNC = false, bool WAVE = false, class Store, class Lds>
+// Stages [FIRST, LAST) only (LAST = 0: to the end).  A run that starts past
+// stage 0 reads its input from LDS; a run that stops early leaves its output
+// in LDS (exchange order, no barrier) for a later run, possibly with another
+// thread mapping after a workgroup barrier.
+template <int N, int E, int DIR, bool STORE_SYNC = false, bool WAVE = false, int FIRST = 0, int LAST = 0,
+          class Store, class Lds>
 __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds);
 
 template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
@@ -226,12 +231,13 @@ __device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds
     fft_compute_tw<N, E, DIR, STORE_SYNC>(t, v, tw, store, lds);
 }
 
-template <int N, int E, int DIR, bool STORE_SYNC, bool WAVE, class Store, class Lds>
+template <int N, int E, int DIR, bool STORE_SYNC, bool WAVE, int FIRST, int LAST, class Store, class Lds>
 __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds) {
     static_assert(!WAVE || FftShape<N, E>::NT <= 64, "a wave-local transform has at most 64 threads");
     using S = FftShape<N, E>;
     constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
-    static_for<0, NS>([&](auto s_) {
+    constexpr int END = LAST > 0 ? LAST : NS;
+    static_for<FIRST, END>([&](auto s_) {
         constexpr int s = decltype(s_)::value;
         constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R;
         if constexpr (s > 0) {
@@ -254,7 +260,16 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
             }
             dft_reg<R, DIR>(&v[k * R]);
         }
-        if constexpr (s == NS - 1) {
+        if constexpr (s == END - 1 && END < NS) {
+            // partial run: leave the exchange in LDS for the next run
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int j = t + k * NT;
+                const int d = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds(d + r * Ns) = v[k * R + r];
+            }
+        } else if constexpr (s == NS - 1) {
             if constexpr (STORE_SYNC) fft_sync<WAVE>();
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
@@ -277,6 +292,62 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
                 for (int r = 0; r < R; ++r) lds(d + r * Ns) = v[k * R + r];
             }
             fft_sync<WAVE>();
+        }
+    });
+}
+
+// Adjoint run: the stages of the DIR_FWD transform applied backwards, each as
+// its adjoint (DFT of the opposite direction, then conjugate twiddles,
+// exchanges with read and write positions swapped).  Input: v in the slot
+// order a forward run leaves it (slot k R + r of thread t holds element
+// fft_out_index(t, k R + r)); output: the opposite-direction transform
+// (unnormalised), written through store(n, value) at the forward's load
+// positions.  A forward run whose results stay in registers, a pointwise
+// product and this adjoint run make a filter with one exchange fewer than a
+// forward and an inverse transform in natural order.
+template <int N, int E, int DIR_FWD, bool WAVE = false, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_adj_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds) {
+    using S = FftShape<N, E>;
+    constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
+    constexpr int DIR = -DIR_FWD;
+    static_for<0, NS>([&](auto i_) {
+        constexpr int s = NS - 1 - decltype(i_)::value;
+        constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            dft_reg<R, DIR>(&v[k * R]);
+            if constexpr (Ns > 1) {
+                const float2* w = tw + FftTw<N, E>::base(s) + k * (R - 1) - 1;
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[k * R + r] = DIR_FWD < 0 ? cmulc(v[k * R + r], w[r]) : cmul(v[k * R + r], w[r]);
+            }
+        }
+        if constexpr (s == 0) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int j = t + k * NT;
+#pragma unroll
+                for (int r = 0; r < R; ++r) store(j + r * (N / R), v[k * R + r]);
+            }
+        } else {
+            // the forward stage s read positions j + r N/R; stage s - 1 wrote d + r' Ns'
+            fft_sync<WAVE>();
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int j = t + k * NT;
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds(j + r * (N / R)) = v[k * R + r];
+            }
+            fft_sync<WAVE>();
+            constexpr int Rp = S::radix(s - 1), Nsp = S::ns(s - 1), NBp = kE / Rp;
+#pragma unroll
+            for (int k = 0; k < NBp; ++k) {
+                const int j = t + k * NT;
+                const int d = (j / Nsp) * Nsp * Rp + (j & (Nsp - 1));
+#pragma unroll
+                for (int r = 0; r < Rp; ++r) v[k * Rp + r] = lds(d + r * Nsp);
+            }
         }
     });
 }

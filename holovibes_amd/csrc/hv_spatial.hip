@@ -275,6 +275,8 @@ struct ColCfg {
 // Each workgroup owns a CW-column tile and walks `fpb` frames of the batch,
 // loading the next frame's tile while the current one is transformed; the
 // lens values of its output positions stay in registers for all frames.
+// Successive frames need no barrier between them: a transform's first LDS
+// write follows a barrier inside the engine.
 template <int N, int MODE>
 __global__ void __launch_bounds__(ColCfg<N>::THREADS)
     k_col_fft(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ lens, int width,
@@ -307,17 +309,15 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
         float2* dst = out + fm.out_frame(b) * P + x;
         auto store = [&](int n, float2 v) { dst[(size_t)n * width] = v; };
         if constexpr (MODE == COL_LENS) {
-            // forward column FFT, x lens (already / (W H)), inverse column FFT
-            auto store_lds = [&](int n, float2 v, int slot) { lds_at(n) = cmul(v, lr[slot]); };
-            auto load_lds = [&](int n) -> float2 { return lds_at(n); };
-            fft_compute_tw<N, C::E, -1, true>(t, cur, tw, store_lds, lds_at);
-            __syncthreads();
-            fft_load<N, C::E>(t, load_lds, cur);
-            fft_compute_tw<N, C::E, +1>(t, cur, tw, store, lds_at);
+            // forward column FFT with its results left in registers, x lens
+            // (already / (W H)), inverse column FFT as the forward's adjoint
+            fft_compute_tw<N, C::E, -1>(t, cur, tw, [](int, float2) {}, lds_at);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) cur[k] = cmul(cur[k], lr[k]);
+            fft_compute_adj_tw<N, C::E, -1>(t, cur, tw, store, lds_at);
         } else {
             fft_compute_tw<N, C::E, MODE == COL_FWD ? -1 : +1>(t, cur, tw, store, lds_at);
         }
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < kE; ++k) cur[k] = nxt[k];
     }
