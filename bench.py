@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--input-mib", type=int, default=512,
                     help="device memory spread over distinct input batches cycled through by the timed steps")
+    ap.add_argument("--lanes", type=int, default=3,
+                    help="cycles in flight on the GPU at once (bins-first path only; 1 elsewhere)")
     ap.add_argument("--path", type=int, default=2,
                     help="highest cycle path allowed: 0 three passes + STFT, 1 fused row-IFFT + STFT, 2 bins-first")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -161,6 +163,10 @@ def main():
 
     pipe_level = args.path
     pipe = Pipe(s, graph=not args.no_graph, fused=pipe_level)
+    lanes = args.lanes if pipe.path == Pipe.BINS else 1
+    if lanes > 1:
+        pipe.close()
+        pipe = Pipe(s, graph=not args.no_graph, fused=pipe_level, lanes=lanes)
     k = 0
     while not pipe.push(dev_batches[k % nbatches]):  # fill the time window
         k += 1
@@ -187,7 +193,7 @@ def main():
             "config": {"workload": name, "width": s.width, "height": s.height, "depth_bytes": s.depth,
                        "batch": B, "stride": S, "T": T, "space": ["none", "fresnel", "angular_spectrum"][
                            s.space_transformation], "time": ["none", "stft", "pca"][s.time_transformation],
-                       "graph": not args.no_graph}}
+                       "graph": not args.no_graph, "lanes": lanes}}
     if args.profile_steps:
         print(json.dumps(line))
         return

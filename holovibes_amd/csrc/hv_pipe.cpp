@@ -105,17 +105,11 @@ Layout layout_of(const hv_compute_settings& s) {
 constexpr int kStages = 5;
 constexpr size_t kMaxGraphs = 64;
 
-struct hv_pipe {
-    hv_compute_settings s{};
+// Buffers of one in-flight cycle.  Lane 0 lives in the pipe's main
+// allocation; further lanes (bins-first path only) have their own.
+struct Lane {
     hipStream_t stream = nullptr;
-    bool own_stream = false;
-    size_t P = 0;
-    int T = 0, B = 0;
-    Layout lay{};
-    char* base = nullptr;
-    float2* ring = nullptr;
-    float2* lens = nullptr;
-    void* staging = nullptr;
+    float2* planes = nullptr;  // bins-first planes (lane 0: the ring's first K frames)
     float* image = nullptr;
     float* cimage = nullptr;
     float* tmp = nullptr;
@@ -124,6 +118,23 @@ struct hv_pipe {
     float* range = nullptr;
     void* pct = nullptr;
     void* unwrap_ws = nullptr;
+    void* mem = nullptr;  // own allocation (lanes >= 1)
+    // graphs, one per (ring phase, input buffer) pair
+    std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
+};
+
+constexpr int kMaxLanes = 4;
+
+struct hv_pipe {
+    hv_compute_settings s{};
+    bool own_stream = false;
+    size_t P = 0;
+    int T = 0, B = 0;
+    Layout lay{};
+    char* base = nullptr;
+    float2* ring = nullptr;
+    float2* lens = nullptr;
+    void* staging = nullptr;
     float2* cov = nullptr;
     double* evals = nullptr;
     double2* evecs = nullptr;
@@ -134,9 +145,12 @@ struct hv_pipe {
     uint32_t q_start = 0, q_size = 0;
     bool autocontrast_pending = true;
     bool have_output = false;
-    // graphs, one per (ring phase, input buffer) pair
     bool graph_enabled = false;
-    std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
+    // in-flight cycles: cycle n runs on lane n mod nlanes (1 unless set by
+    // hv_pipe_set_lanes); `cur` is the lane being enqueued, `last` the lane
+    // of the latest output
+    std::vector<Lane> lane = std::vector<Lane>(1);
+    int nlanes = 1, cur = 0, last = 0, next = 0;
     // cycle path (PATH_*): three passes + STFT, inverse row pass fused with
     // the STFT, or bins-first
     int path = PATH_THREE_PASS;
@@ -157,8 +171,14 @@ struct hv_pipe {
                s.space_transformation != HV_SPACE_NONE;
     }
     hv_status mark(int i) {
-        if (timing) HV_HIP(hipEventRecord(ev[i], stream));
+        if (timing) HV_HIP(hipEventRecord(ev[i], lane[cur].stream));
         return HV_OK;
+    }
+    void drop_graphs() {
+        for (auto& l : lane) {
+            for (auto& kv : l.graphs) (void)hipGraphExecDestroy(kv.second);
+            l.graphs.clear();
+        }
     }
     hv_status collect_times() {
         HV_HIP(hipEventSynchronize(ev[kStages]));
@@ -174,88 +194,89 @@ struct hv_pipe {
 
 // Device work of one cycle after the ring holds T frames (pipe.cc:333-380).
 hv_status hv_pipe::image_work() {
+    Lane& L = lane[cur];
     const int it = s.img_type;
     const int Tn = T;
     const bool minmax_mode = s.contrast_enabled && s.contrast_mode == HV_CONTRAST_MINMAX;
     const bool unwrap = it == HV_IMG_ARGUMENT && s.unwrap_2d;
     // the image's min/max is folded in by the producing kernel when it is the
     // final image; the cycle's first kernel has re-armed the accumulators
-    uint32_t* mm_img = minmax_fused() ? mm : nullptr;
+    uint32_t* mm_img = minmax_fused() ? L.mm : nullptr;
     if (is_image_kind(it)) {
         int pmin, pmax;
         p_range(s, pmin, pmax);
         if (path == PATH_BINS) {
-            HV_TRY(launch_bins_image(image, ring, (int)s.width, (int)s.height, pmax - pmin + 1,
-                                     s.space_transformation, it, pmin, pmax, mm_img, stream));
+            HV_TRY(launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, pmax - pmin + 1,
+                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream));
         } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
-            HV_TRY(launch_row_ifft_stft_image(image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
-                                              stream));
+            HV_TRY(launch_row_ifft_stft_image(L.image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
+                                              L.stream));
         } else if (s.time_transformation == HV_TIME_STFT) {
-            HV_TRY(launch_stft_image(image, ring, Tn, P, it, pmin, pmax, mm_img, stream));
+            HV_TRY(launch_stft_image(L.image, ring, Tn, P, it, pmin, pmax, mm_img, L.stream));
         } else if (s.time_transformation == HV_TIME_PCA) {
-            HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, stream));
-            HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, stream));
-            HV_TRY(launch_pca_image(image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, stream));
+            HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, L.stream));
+            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, L.stream));
         } else {
-            HV_TRY(launch_complex_to_image(image, ring, it, pmin, pmax, P, stream));
-            if (mm_img) HV_TRY(launch_minmax(mm, image, P, stream));
+            HV_TRY(launch_complex_to_image(L.image, ring, it, pmin, pmax, P, L.stream));
+            if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream));
         }
         if (unwrap) {
-            HV_TRY(launch_unwrap_2d(tmp, image, s.width, s.height, unwrap_ws, stream));
-            HV_TRY(launch_rescale_unwrap(image, tmp, P, mm, stream));
+            HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream));
+            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream));
         }
     } else {
         // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
         const int fs = (int)s.p_start;
         const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
-        float* m0 = it == HV_IMG_MOMENTS_0 ? image : nullptr;
-        float* m1 = it == HV_IMG_MOMENTS_1 ? image : nullptr;
-        float* m2 = it == HV_IMG_MOMENTS_2 ? image : nullptr;
+        float* m0 = it == HV_IMG_MOMENTS_0 ? L.image : nullptr;
+        float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
+        float* m2 = it == HV_IMG_MOMENTS_2 ? L.image : nullptr;
         HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
-                                   it - HV_IMG_MOMENTS_0, stream));
+                                   it - HV_IMG_MOMENTS_0, L.stream));
     }
-    if (s.fft_shift) HV_TRY(launch_shift_corners_f32(image, 1, s.width, s.height, stream));
+    if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
     HV_TRY(mark(4));
-    if (!s.contrast_enabled) return launch_float_to_ushort(out, image, P, stream);
+    if (!s.contrast_enabled) return launch_float_to_ushort(L.out, L.image, P, L.stream);
     switch (s.contrast_mode) {
         case HV_CONTRAST_MINMAX:
             if (!minmax_fused()) {
                 // min/max of the final image (unwrap rescales it; no first-kernel reset without a space transform)
-                HV_TRY(launch_minmax_init(mm, stream));
-                HV_TRY(launch_minmax(mm, image, P, stream));
+                HV_TRY(launch_minmax_init(L.mm, L.stream));
+                HV_TRY(launch_minmax(L.mm, L.image, P, L.stream));
             }
             // the contrast kernel decodes the accumulators and records the range it applied
-            return launch_contrast_to_ushort(out, cimage, image, P, mm, 1, s.contrast_invert, stream, range);
+            return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.mm, 1, s.contrast_invert, L.stream, L.range);
         case HV_CONTRAST_PERCENTILE:
-            HV_TRY(launch_percentile(range, image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
-                                     s.contrast_upper, 1, pct, stream));
+            HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
+                                     s.contrast_upper, 1, L.pct, L.stream));
             break;
         case HV_CONTRAST_REFERENCE:
             if (autocontrast_pending)
-                HV_TRY(launch_percentile(range, image, s.width, s.height, (int)s.contrast_subsample,
-                                         s.contrast_lower, s.contrast_upper, 1, pct, stream));
+                HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample,
+                                         s.contrast_lower, s.contrast_upper, 1, L.pct, L.stream));
             break;
         default:  // MANUAL: range uploaded at create
             break;
     }
-    return launch_contrast_to_ushort(out, cimage, image, P, range, 0, s.contrast_invert, stream);
+    return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.range, 0, s.contrast_invert, L.stream);
 }
 
 hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
+    Lane& L = lane[cur];
     HV_TRY(mark(0));
     if (path == PATH_BINS) {
-        // the K bin planes take the first K slots of the ring's memory
         int pmin, pmax;
         p_range(s, pmin, pmax);
         HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, pmin,
-                                     pmax - pmin + 1, s.space_transformation, lens, ring, stream,
-                                     timing ? ev[1] : nullptr, minmax_fused() ? mm : nullptr));
+                                     pmax - pmin + 1, s.space_transformation, lens, L.planes, L.stream,
+                                     timing ? ev[1] : nullptr, minmax_fused() ? L.mm : nullptr));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
-                                        s.space_transformation, lens, ring, T, slot0, stream,
+                                        s.space_transformation, lens, ring, T, slot0, L.stream,
                                         timing ? &ev[1] : nullptr, path == PATH_ROW_STFT,
-                                        minmax_fused() ? mm : nullptr));
+                                        minmax_fused() ? L.mm : nullptr));
     }
     HV_TRY(mark(3));
     HV_TRY(image_work());
@@ -364,16 +385,17 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->B = (int)s->batch_size;
     p->lay = layout_of(*s);
     p->path = choose_path(*s, PATH_BINS);
+    Lane& L0 = p->lane[0];
     if (stream) {
-        p->stream = as_stream(stream);
+        L0.stream = as_stream(stream);
     } else {
-        HV_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        HV_HIP(hipStreamCreateWithFlags(&L0.stream, hipStreamNonBlocking));
         p->own_stream = true;
     }
     void* mem = nullptr;
     hipError_t e = hipMalloc(&mem, p->lay.total);
     if (e != hipSuccess) {
-        if (p->own_stream) (void)hipStreamDestroy(p->stream);
+        if (p->own_stream) (void)hipStreamDestroy(L0.stream);
         return hip_fail(e, "hipMalloc(pipe)");
     }
     p->base = (char*)mem;
@@ -387,15 +409,16 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->ring = (float2*)take(l.ring);
     p->lens = (float2*)take(l.lens);
     p->staging = take(l.staging);
-    p->image = (float*)take(l.image);
-    p->cimage = (float*)take(l.cimage);
-    p->tmp = (float*)take(l.tmp);
-    p->out = (uint16_t*)take(l.out);
+    L0.planes = p->ring;
+    L0.image = (float*)take(l.image);
+    L0.cimage = (float*)take(l.cimage);
+    L0.tmp = (float*)take(l.tmp);
+    L0.out = (uint16_t*)take(l.out);
     char* small = take(l.small);
-    p->mm = (uint32_t*)small;
-    p->range = (float*)(small + 16);
-    p->pct = take(l.pct);
-    p->unwrap_ws = take(l.unwrap);
+    L0.mm = (uint32_t*)small;
+    L0.range = (float*)(small + 16);
+    L0.pct = take(l.pct);
+    L0.unwrap_ws = take(l.unwrap);
     p->cov = (float2*)take(l.cov);
     p->evals = (double*)take(l.evals);
     p->evecs = (double2*)take(l.evecs);
@@ -403,12 +426,12 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->eig_ws = take(l.eig_ws);
     // refresh-time work: lens, manual contrast range, zeroed ring
     hv_status st = hv_prepare_lens(p->lens, s->space_transformation, s->width, s->height, s->lambda, s->z_distance,
-                                   s->pixel_size, s->lens_precision, p->stream);
+                                   s->pixel_size, s->lens_precision, L0.stream);
     if (st == HV_OK) {
         const float r[2] = {s->contrast_min, s->contrast_max};
-        e = hipMemcpyAsync(p->range, r, sizeof(r), hipMemcpyHostToDevice, p->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(p->ring, 0, l.ring, p->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+        e = hipMemcpyAsync(L0.range, r, sizeof(r), hipMemcpyHostToDevice, L0.stream);
+        if (e == hipSuccess) e = hipMemsetAsync(p->ring, 0, l.ring, L0.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(L0.stream);
         if (e != hipSuccess) st = hip_fail(e, "pipe init");
     }
     if (st != HV_OK) {
@@ -449,47 +472,78 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     const bool full = p->q_size >= maxq;
     if (!full) {
         return launch_propagate_to_ring(frames, (int)p->s.depth, p->B, (int)p->s.width, (int)p->s.height,
-                                        p->s.space_transformation, p->lens, p->ring, p->T, slot0, p->stream);
+                                        p->s.space_transformation, p->lens, p->ring, p->T, slot0,
+                                        p->lane[0].stream);
     }
     const bool pending = p->s.contrast_enabled && p->s.contrast_mode == HV_CONTRAST_REFERENCE &&
                          p->autocontrast_pending;
+    // the cycle's lane: round robin over the in-flight lanes (lane 0 while timing)
+    p->cur = p->timing ? 0 : p->next;
+    p->next = (p->cur + 1) % p->nlanes;
+    Lane& L = p->lane[p->cur];
     if (p->timing) {
         HV_TRY(p->enqueue_cycle(frames, slot0));
         HV_TRY(p->collect_times());
     } else if (p->graph_enabled && !pending) {
         // one graph per (ring phase, input buffer); past the cache bound the
-        // batch is staged into the pipe's own input buffer first
+        // batch is staged into the pipe's own input buffer first (one lane)
+        // or run without a graph (several lanes share no staging buffer)
         const void* src = frames;
         auto key = std::make_pair(slot0, src);
-        auto it = p->graphs.find(key);
-        if (it == p->graphs.end() && p->graphs.size() >= kMaxGraphs) {
-            HV_HIP(hipMemcpyAsync(p->staging, frames, (size_t)p->B * p->P * p->s.depth, hipMemcpyDeviceToDevice,
-                                  p->stream));
-            src = p->staging;
-            key = std::make_pair(slot0, src);
-            it = p->graphs.find(key);
-        }
-        if (it == p->graphs.end()) {
-            hipGraph_t g = nullptr;
-            HV_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-            hv_status st = p->enqueue_cycle(src, slot0);
-            hipError_t e = hipStreamEndCapture(p->stream, &g);
-            if (st != HV_OK) {
-                if (g) (void)hipGraphDestroy(g);
-                return st;
+        auto it = L.graphs.find(key);
+        bool eager = false;
+        if (it == L.graphs.end() && L.graphs.size() >= kMaxGraphs) {
+            if (p->nlanes > 1) {
+                eager = true;
+            } else {
+                HV_HIP(hipMemcpyAsync(p->staging, frames, (size_t)p->B * p->P * p->s.depth,
+                                      hipMemcpyDeviceToDevice, L.stream));
+                src = p->staging;
+                key = std::make_pair(slot0, src);
+                it = L.graphs.find(key);
             }
-            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-            hipGraphExec_t ex = nullptr;
-            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-            it = p->graphs.emplace(key, ex).first;
         }
-        HV_HIP(hipGraphLaunch(it->second, p->stream));
+        if (eager) {
+            HV_TRY(p->enqueue_cycle(frames, slot0));
+        } else {
+            if (it == L.graphs.end()) {
+                hipGraph_t g = nullptr;
+                HV_HIP(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+                hv_status st = p->enqueue_cycle(src, slot0);
+                hipError_t e = hipStreamEndCapture(L.stream, &g);
+                if (st != HV_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return st;
+                }
+                if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+                hipGraphExec_t ex = nullptr;
+                e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+                it = L.graphs.emplace(key, ex).first;
+            }
+            HV_HIP(hipGraphLaunch(it->second, L.stream));
+        }
     } else {
         HV_TRY(p->enqueue_cycle(frames, slot0));
     }
-    if (pending) p->autocontrast_pending = false;  // rendering.cc:251-258
+    if (pending) {
+        p->autocontrast_pending = false;  // rendering.cc:251-258
+        // the frozen range reaches the other lanes in their stream order
+        if (p->nlanes > 1) {
+            hipEvent_t e = nullptr;
+            HV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            HV_HIP(hipEventRecord(e, L.stream));
+            for (int i = 0; i < p->nlanes; ++i) {
+                if (i == p->cur) continue;
+                HV_HIP(hipStreamWaitEvent(p->lane[i].stream, e, 0));
+                HV_HIP(hipMemcpyAsync(p->lane[i].range, L.range, 2 * sizeof(float), hipMemcpyDeviceToDevice,
+                                      p->lane[i].stream));
+            }
+            HV_HIP(hipEventDestroy(e));
+        }
+    }
+    p->last = p->cur;
     p->have_output = true;
     if (produced) *produced = 1;
     return HV_OK;
@@ -515,15 +569,16 @@ hv_status hv_pipe_stage_times(hv_pipe* p, float* ms, int* cycles) {
 hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out_f32_contrast,
                          const float** out_f32_image) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
-    if (out_u16) *out_u16 = p->out;
-    if (out_f32_contrast) *out_f32_contrast = p->s.contrast_enabled ? p->cimage : p->image;
-    if (out_f32_image) *out_f32_image = p->image;
+    const Lane& L = p->lane[p->last];
+    if (out_u16) *out_u16 = L.out;
+    if (out_f32_contrast) *out_f32_contrast = p->s.contrast_enabled ? L.cimage : L.image;
+    if (out_f32_image) *out_f32_image = L.image;
     return p->have_output ? HV_OK : HV_ERR_NOT_READY;
 }
 
 hv_status hv_pipe_contrast_range(hv_pipe* p, const float** range_dev) {
     if (!p || !range_dev) return fail(HV_ERR_INVALID_ARG, "null argument");
-    *range_dev = p->range;
+    *range_dev = p->lane[p->last].range;
     return HV_OK;
 }
 
@@ -539,8 +594,63 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (level < 0) return fail(HV_ERR_INVALID_ARG, "fusion level must be >= 0");
     p->path = choose_path(p->s, level);
-    for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
-    p->graphs.clear();
+    HV_TRY(hv_pipe_sync(p));
+    p->drop_graphs();
+    if (p->path != PATH_BINS) p->nlanes = 1;  // only the bins-first path runs cycles side by side
+    p->next = 0;
+    return HV_OK;
+}
+
+hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    if (lanes < 1 || lanes > kMaxLanes) return fail(HV_ERR_INVALID_ARG, "lanes must be in [1, 4]");
+    if (lanes > 1 && p->path != PATH_BINS)
+        return fail(HV_ERR_UNSUPPORTED, "only the bins-first path (B = S = T, STFT image, space transform) runs "
+                                        "cycles side by side");
+    if (lanes > 1 && !p->own_stream)
+        return fail(HV_ERR_UNSUPPORTED, "several lanes need the pipe's own streams (create it without a stream)");
+    HV_TRY(hv_pipe_sync(p));
+    p->drop_graphs();
+    int pmin, pmax;
+    p_range(p->s, pmin, pmax);
+    const size_t K = (size_t)(pmax - pmin + 1);
+    const Layout& l = p->lay;
+    for (int i = (int)p->lane.size(); i < lanes; ++i) {
+        Lane n;
+        const size_t planes = align256(K * p->P * sizeof(float2));
+        const size_t total = planes + l.image + l.cimage + l.tmp + l.out + l.small + l.pct + l.unwrap;
+        hipError_t e = hipMalloc(&n.mem, total);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(lane)");
+        e = hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            (void)hipFree(n.mem);
+            return hip_fail(e, "hipStreamCreateWithFlags(lane)");
+        }
+        char* c = (char*)n.mem;
+        auto take = [&](size_t b) {
+            char* r = b ? c : nullptr;
+            c += b;
+            return r;
+        };
+        n.planes = (float2*)take(planes);
+        n.image = (float*)take(l.image);
+        n.cimage = (float*)take(l.cimage);
+        n.tmp = (float*)take(l.tmp);
+        n.out = (uint16_t*)take(l.out);
+        char* small = take(l.small);
+        n.mm = (uint32_t*)small;
+        n.range = (float*)(small + 16);
+        n.pct = take(l.pct);
+        n.unwrap_ws = take(l.unwrap);
+        p->lane.push_back(n);
+    }
+    // every lane starts from lane 0's contrast range (manual or already frozen)
+    for (int i = 1; i < lanes; ++i)
+        HV_HIP(hipMemcpyAsync(p->lane[i].range, p->lane[0].range, 2 * sizeof(float), hipMemcpyDeviceToDevice,
+                              p->lane[0].stream));
+    HV_HIP(hipStreamSynchronize(p->lane[0].stream));
+    p->nlanes = lanes;
+    p->next = 0;
     return HV_OK;
 }
 
@@ -558,24 +668,30 @@ hv_status hv_pipe_set_graph(hv_pipe* p, int enabled) {
 
 hv_status hv_pipe_stream(hv_pipe* p, void** stream) {
     if (!p || !stream) return fail(HV_ERR_INVALID_ARG, "null argument");
-    *stream = p->stream;
+    *stream = p->lane[0].stream;
     return HV_OK;
 }
 
 hv_status hv_pipe_sync(hv_pipe* p) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
-    HV_HIP(hipStreamSynchronize(p->stream));
+    for (int i = 0; i < (int)p->lane.size(); ++i)
+        if (p->lane[i].stream) HV_HIP(hipStreamSynchronize(p->lane[i].stream));
     return HV_OK;
 }
 
 hv_status hv_pipe_destroy(hv_pipe* p) {
     if (!p) return HV_OK;
-    if (p->stream) (void)hipStreamSynchronize(p->stream);
-    for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& l : p->lane)
+        if (l.stream) (void)hipStreamSynchronize(l.stream);
+    p->drop_graphs();
+    for (size_t i = 1; i < p->lane.size(); ++i) {
+        if (p->lane[i].mem) (void)hipFree(p->lane[i].mem);
+        if (p->lane[i].stream) (void)hipStreamDestroy(p->lane[i].stream);
+    }
     for (auto e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->base) (void)hipFree(p->base);
-    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->own_stream && p->lane[0].stream) (void)hipStreamDestroy(p->lane[0].stream);
     delete p;
     return HV_OK;
 }
@@ -661,7 +777,7 @@ hv_status hv_server_submit(hv_server* s, int source_id, const void* frames, int6
     } else {
         HV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
-    HV_HIP(hipEventRecord(ev, p->stream));
+    HV_HIP(hipEventRecord(ev, p->lane[p->last].stream));
     s->inflight.push_back({source_id, ev, t_arrival_ns, produced != 0});
     return HV_OK;
 }

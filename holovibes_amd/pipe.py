@@ -42,10 +42,10 @@ class Pipe:
     THREE_PASS, ROW_STFT, BINS = 0, 1, 2
 
     def __init__(self, settings: ComputeSettings, own_stream: bool = False, graph: bool = False,
-                 fused: int = 2):
+                 fused: int = 2, lanes: int = 1):
         self.settings = settings
         self.h = ctypes.c_void_p()
-        own_stream = own_stream or graph
+        own_stream = own_stream or graph or lanes > 1
         stream = None if own_stream else ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.call("hv_pipe_create", ctypes.byref(settings), stream, ctypes.byref(self.h))
         st = ctypes.c_void_p()
@@ -57,6 +57,8 @@ class Pipe:
         level = self.BINS if fused is True else int(fused)
         if level != self.BINS:
             _lib.call("hv_pipe_set_fused", self.h, level)
+        if lanes > 1:
+            _lib.call("hv_pipe_set_lanes", self.h, lanes)
         self.P = settings.width * settings.height
 
     @staticmethod
@@ -105,6 +107,7 @@ class Pipe:
         return u.value, c.value, i.value
 
     def output_u16(self) -> np.ndarray:
+        self.sync()  # the latest cycle may run on another lane's stream
         u, _, _ = self._outputs()
         t = torch.empty((self.settings.height, self.settings.width), dtype=torch.int16, device="cuda")
         copy_d2d(t, u, self.P * 2, self.stream)
@@ -113,6 +116,7 @@ class Pipe:
 
     def image(self) -> torch.Tensor:
         """Float image before contrast (gpu_postprocess_frame)."""
+        self.sync()
         _, _, i = self._outputs()
         t = torch.empty((self.settings.height, self.settings.width), dtype=torch.float32, device="cuda")
         copy_d2d(t, i, self.P * 4, self.stream)
@@ -120,6 +124,7 @@ class Pipe:
         return t
 
     def contrast_image(self) -> torch.Tensor:
+        self.sync()
         _, c, _ = self._outputs()
         t = torch.empty((self.settings.height, self.settings.width), dtype=torch.float32, device="cuda")
         copy_d2d(t, c, self.P * 4, self.stream)
@@ -127,6 +132,7 @@ class Pipe:
         return t
 
     def contrast_range(self) -> np.ndarray:
+        self.sync()
         r = ctypes.c_void_p()
         _lib.call("hv_pipe_contrast_range", self.h, ctypes.byref(r))
         t = torch.empty(2, dtype=torch.float32, device="cuda")

@@ -366,6 +366,17 @@ hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
 hv_status hv_pipe_set_fused(hv_pipe* p, int level);
 hv_status hv_pipe_path(const hv_pipe* p, int* path);
 
+/* No reference counterpart.  Lets up to `lanes` (1..4) consecutive cycles
+ * run on the GPU at once, each on its own stream with its own plane,
+ * image, contrast and output buffers (allocated here, outside the per-batch
+ * path); cycle n runs on lane n mod lanes.  Only for the bins-first path,
+ * whose cycles share no state, and only for a pipe that owns its streams
+ * (created without a stream).  Outputs (hv_pipe_output,
+ * hv_pipe_contrast_range) are the latest cycle's; call hv_pipe_sync before
+ * reading them.  A frozen autocontrast range reaches every lane in stream
+ * order.  Switching the path back (hv_pipe_set_fused) returns to one lane. */
+hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
+
 /* The time-transformation queue: ring base (T frames, complex), its
  * start index and size (Queue::get_start_index / get_size, queue.hh). */
 hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size);

@@ -190,6 +190,42 @@ def test_pipe_metric_full_size(oracle, level):
     check(run_both(oracle, s, fl, graph=True, fused=level, path=level), 1e-5)
 
 
+@pytest.mark.parametrize("lanes", [2, 3])
+@pytest.mark.parametrize("contrast_mode", [0, 2])
+def test_pipe_lanes_match_one_lane(lanes, contrast_mode):
+    """Cycles side by side on several lanes give the one-lane outputs,
+    batch by batch and after a burst of pushes without synchronisation
+    (min/max contrast per cycle; autocontrast frozen after the first)."""
+    from holovibes_amd.pipe import Pipe
+    s, fl = frames("c1", 6, width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16,
+                   contrast_mode=contrast_mode, contrast_lower=0.5, contrast_upper=99.5)
+    dev = [torch.from_numpy(f.view(np.int16)).cuda() for f in fl]
+    torch.cuda.synchronize()
+    one, many = Pipe(s, graph=True), Pipe(s, graph=True, lanes=lanes)
+    assert one.path == Pipe.BINS and many.path == Pipe.BINS
+    for d in dev:
+        assert one.push(d) and many.push(d)
+        assert np.array_equal(one.output_u16(), many.output_u16())
+        assert torch.equal(one.image(), many.image())
+    for d in dev + dev[::-1]:
+        one.push(d)
+        many.push(d)
+    assert np.array_equal(one.output_u16(), many.output_u16())
+    assert np.array_equal(one.contrast_range(), many.contrast_range())
+    one.close()
+    many.close()
+
+
+def test_pipe_lanes_refused_off_bins_path():
+    from holovibes_amd import _lib
+    from holovibes_amd.pipe import Pipe
+    s, _ = frames("c1", 1, width=64, height=64, batch_size=4, time_stride=4, time_transformation_size=8)
+    p = Pipe(s, graph=True)
+    assert p.path != Pipe.BINS
+    assert _lib.status("hv_pipe_set_lanes", p.h, 2) == _lib.HV_ERR_UNSUPPORTED
+    p.close()
+
+
 def test_server_alone_vs_mix():
     """Continuous batching: a source's outputs are identical alone and when
     its batches interleave with other sources' on the device."""
