@@ -28,7 +28,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="metric")
     ap.add_argument("--latency-batches", type=int, default=200)
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each cycle from a hipGraph capture (eager launches are faster here)")
+    ap.add_argument("--no-graph", action="store_true", help="(default) eager launches")
     ap.add_argument("--input-mib", type=int, default=512,
                     help="device memory spread over distinct input batches cycled through by the timed steps")
     ap.add_argument("--lanes", type=int, default=3,
@@ -39,6 +41,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprofv3), print the JSON without extras")
+    ap.add_argument("--stage-iters", type=int, default=50)
+    ap.add_argument("--profile-stages", action="store_true",
+                    help="with --profile-steps: after the steps, time the stages and print (for rocprofv3)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also copy this many MiB device-to-device once (a plain copy kernel of known size, "
                          "the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE scaling)")
@@ -162,11 +167,11 @@ def main():
         del src, dst
 
     pipe_level = args.path
-    pipe = Pipe(s, graph=not args.no_graph, fused=pipe_level)
+    pipe = Pipe(s, graph=args.graph and not args.no_graph, fused=pipe_level)
     lanes = args.lanes if pipe.path == Pipe.BINS else 1
     if lanes > 1:
         pipe.close()
-        pipe = Pipe(s, graph=not args.no_graph, fused=pipe_level, lanes=lanes)
+        pipe = Pipe(s, graph=args.graph and not args.no_graph, fused=pipe_level, lanes=lanes)
     k = 0
     while not pipe.push(dev_batches[k % nbatches]):  # fill the time window
         k += 1
@@ -182,44 +187,58 @@ def main():
     for _ in range(steps):
         pipe.push(dev_batches[k % nbatches])
         k += 1
+    t_host = time.perf_counter() - t0
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_per_step = el / steps * 1e3
     frames_per_s = steps * B / el
     line = {"metric": "frames/s + p99 batch->image latency at 512^2 T=32; % HBM roofline; err vs FP64",
             "value": frames_per_s, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "ms_per_step": ms_per_step, "host_submit_ms_per_step": t_host / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic (seeded interferogram generator)",
             "config": {"workload": name, "width": s.width, "height": s.height, "depth_bytes": s.depth,
                        "batch": B, "stride": S, "T": T, "space": ["none", "fresnel", "angular_spectrum"][
                            s.space_transformation], "time": ["none", "stft", "pca"][s.time_transformation],
-                       "graph": not args.no_graph, "lanes": lanes}}
-    if args.profile_steps:
+                       "graph": bool(args.graph and not args.no_graph), "lanes": lanes}}
+    if args.profile_steps and not args.profile_stages:
         print(json.dumps(line))
         return
 
-    # per-stage HIP events over a second timed region of the same length
-    pipe.set_timing(True)
-    for _ in range(steps):
-        pipe.push(dev_batches[k % nbatches])
-        k += 1
-    ms, cyc = pipe.stage_times()
-    pipe.set_timing(False)
     path = pipe.path
     line["config"]["path"] = ["three_pass", "row_ifft_stft_fused", "bins_first"][path]
     sb = stage_bytes(s, path)
+    if path == Pipe.BINS:
+        # each stage launched `stage_iters` times back to back between two HIP
+        # events on the pipe's stream (stage 0 cycling over the input batches)
+        ms = [pipe.time_stage(i, dev_batches, args.stage_iters) if sb[i][1] > 0 else 0.0 for i in range(5)]
+        timing = f"mean of {args.stage_iters} back-to-back launches per stage, HIP events on the pipe's stream"
+    else:
+        # HIP events between the stages of every cycle over a second timed region
+        pipe.set_timing(True)
+        for _ in range(steps):
+            pipe.push(dev_batches[k % nbatches])
+            k += 1
+        ms, _ = pipe.stage_times()
+        pipe.set_timing(False)
+        timing = "HIP events between the stages of each cycle (includes the event overhead)"
     stages = []
     for (nm, b), t in zip(sb, ms):
         gbs = b / (t * 1e-3) / 1e9 if t > 0 and b > 0 else 0.0
         stages.append({"name": nm, "ms": t, "bytes": b, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS})
     dom = max((i for i in range(5) if sb[i][1] > 0), key=lambda i: ms[i])
     line["roofline"] = {"bound": "hbm", "kernel": sb[dom][0], "achieved": stages[dom]["GB/s"],
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["frac"], "traffic": None}
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["frac"], "traffic": None,
+                        "timing": timing}
     tot_b, tot_ms = sum(b for _, b in sb), sum(ms)
     line["pipeline_roofline"] = {"bytes_per_step": tot_b, "ms_per_step_stages": tot_ms,
                                  "GB/s": tot_b / (tot_ms * 1e-3) / 1e9, "frac": tot_b / (tot_ms * 1e-3) / 1e9 /
-                                 HBM_PEAK_GBS}
+                                 HBM_PEAK_GBS,
+                                 "step_GB/s": tot_b / (ms_per_step * 1e-3) / 1e9,
+                                 "step_frac": tot_b / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS}
     line["stages"] = stages
+    if args.profile_stages:
+        print(json.dumps(line))
+        return
 
     # batch -> image latency, one batch at a time (unloaded)
     lat = []

@@ -111,6 +111,7 @@ SIGNATURES = {
     "hv_pipe_set_fused": [P, I],
     "hv_pipe_path": [P, P],
     "hv_pipe_set_lanes": [P, I],
+    "hv_pipe_time_stage": [P, I, P, I, I, P],
     "hv_pipe_input_buffer": [P, P],
     "hv_pipe_ring": [P, P, P, P],
     "hv_pipe_stream": [P, P],

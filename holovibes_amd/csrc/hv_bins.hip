@@ -303,24 +303,34 @@ bool bins_first_supported(int width, int height, int T, int K, int depth) {
     return rows_lds_for(width, K) <= 160 * 1024;
 }
 
-hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
-                                int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset) {
+hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
+                           int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset) {
     if (!bins_first_supported(width, height, T, K, depth))
         return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
     if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
         return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
     const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
     switch (width) {
-        case 64: HV_TRY(rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
-        case 128: HV_TRY(rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
-        case 256: HV_TRY(rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
-        case 512: HV_TRY(rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
-        default: HV_TRY(rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st)); break;
+        case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
+        case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
+        case 256: return rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
+        case 512: return rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
+        default: return rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
     }
-    if (mark) HV_HIP(hipEventRecord(mark, st));
+}
+
+hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
+                              hipStream_t st) {
     return launch_col_pass(planes, space == HV_SPACE_ANGULARSP ? lens : nullptr, width, height, K,
                            space == HV_SPACE_ANGULARSP ? COLPASS_LENS : COLPASS_FWD, st);
+}
+
+hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
+                                int K, int space, const float2* lens, float2* planes, hipStream_t st,
+                                hipEvent_t mark, uint32_t* mm_reset) {
+    HV_TRY(launch_bins_rows(raw, depth, width, height, T, slot0, pmin, K, space, lens, planes, st, mm_reset));
+    if (mark) HV_HIP(hipEventRecord(mark, st));
+    return launch_bins_columns(planes, width, height, K, space, lens, st);
 }
 
 template <int IMG>

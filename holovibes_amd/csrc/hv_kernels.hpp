@@ -48,8 +48,13 @@ hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* rin
 // hv_bins.hip: B = S = T STFT images computed bins-first (the STFT bins
 // [pmin, pmin + K) of the raw window are propagated instead of its T frames).
 bool bins_first_supported(int width, int height, int T, int K, int depth);
-// Bins, row FFTs and the column pass into planes[0 .. K) (K x P complex);
-// `mark` (optional) is recorded between the row and column passes.
+// Bins + row FFTs (x chirp for Fresnel) into planes[0 .. K) (K x P complex),
+// then the column pass over them; launch_bins_propagate runs both, with
+// `mark` (optional) recorded in between.
+hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
+                           int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset);
+hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
+                              hipStream_t st);
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
                                 hipEvent_t mark, uint32_t* mm_reset);

@@ -520,6 +520,11 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
                 e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+                e = hipGraphUpload(ex, L.stream);  // device-side resources before the first launch
+                if (e != hipSuccess) {
+                    (void)hipGraphExecDestroy(ex);
+                    return hip_fail(e, "hipGraphUpload");
+                }
                 it = L.graphs.emplace(key, ex).first;
             }
             HV_HIP(hipGraphLaunch(it->second, L.stream));
@@ -651,6 +656,54 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     HV_HIP(hipStreamSynchronize(p->lane[0].stream));
     p->nlanes = lanes;
     p->next = 0;
+    return HV_OK;
+}
+
+hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, int n_frames, int iters, float* ms) {
+    if (!p || !ms || iters < 1) return fail(HV_ERR_INVALID_ARG, "null argument or iters < 1");
+    if (p->path != PATH_BINS) return fail(HV_ERR_UNSUPPORTED, "stage timing covers the bins-first path");
+    if (stage == 0 && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 0 needs input frames");
+    if (stage < 0 || stage > 4 || stage == 2) return fail(HV_ERR_INVALID_ARG, "stage must be 0, 1, 3 or 4");
+    HV_TRY(hv_pipe_sync(p));
+    Lane& L = p->lane[0];
+    const hv_compute_settings& s = p->s;
+    int pmin, pmax;
+    p_range(s, pmin, pmax);
+    const int K = pmax - pmin + 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HV_HIP(hipEventCreate(&e0));
+    HV_HIP(hipEventCreate(&e1));
+    hv_status st = HV_OK;
+    hipError_t e = hipEventRecord(e0, L.stream);
+    for (int i = 0; i < iters && st == HV_OK && e == hipSuccess; ++i) {
+        switch (stage) {
+            case 0:
+                st = launch_bins_rows(frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
+                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm);
+                break;
+            case 1:
+                st = launch_bins_columns(L.planes, (int)s.width, (int)s.height, K, s.space_transformation, p->lens,
+                                         L.stream);
+                break;
+            case 3:
+                st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
+                                       s.img_type, pmin, pmax, L.mm, L.stream);
+                break;
+            default:
+                st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, 1, s.contrast_invert, L.stream,
+                                               L.range);
+                break;
+        }
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, L.stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != HV_OK) return st;
+    if (e != hipSuccess) return hip_fail(e, "stage timing");
+    *ms = t / iters;
     return HV_OK;
 }
 

@@ -96,6 +96,16 @@ class Pipe:
         _lib.call("hv_pipe_stage_times", self.h, ms, ctypes.byref(n))
         return list(ms), n.value
 
+    def time_stage(self, stage: int, frames, iters: int = 50) -> float:
+        """Mean ms of one launch of a bins-first stage (0 rows, 1 columns,
+        3 image, 4 contrast), `iters` launches back to back; stage 0 cycles
+        over `frames` (device tensors or pointers)."""
+        ptrs = [f.data_ptr() if isinstance(f, torch.Tensor) else int(f) for f in frames]
+        arr = (ctypes.c_void_p * max(1, len(ptrs)))(*ptrs)
+        ms = ctypes.c_float()
+        _lib.call("hv_pipe_time_stage", self.h, stage, arr, len(ptrs), iters, ctypes.byref(ms))
+        return ms.value
+
     def sync(self) -> None:
         _lib.call("hv_pipe_sync", self.h)
 

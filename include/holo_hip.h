@@ -377,6 +377,13 @@ hv_status hv_pipe_path(const hv_pipe* p, int* path);
  * order.  Switching the path back (hv_pipe_set_fused) returns to one lane. */
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
 
+/* No reference counterpart (measurement).  Mean milliseconds of one launch of
+ * a bins-first stage -- 0 bins + row FFTs (cycling over the n_frames input
+ * batches), 1 column pass, 3 inverse row FFT + image, 4 contrast + u16 --
+ * over `iters` launches back to back on lane 0's stream, between two HIP
+ * events.  The pipe's buffers are overwritten; the next cycle rewrites them. */
+hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, int n_frames, int iters, float* ms);
+
 /* The time-transformation queue: ring base (T frames, complex), its
  * start index and size (Queue::get_start_index / get_size, queue.hh). */
 hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size);
