@@ -27,6 +27,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="metric")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="override a compute setting of the workload (e.g. pca_precision=0)")
     ap.add_argument("--latency-batches", type=int, default=200)
     ap.add_argument("--graph", action="store_true",
                     help="replay each cycle from a hipGraph capture (eager launches are faster here)")
@@ -101,6 +103,7 @@ def image_error(s, batch, gpu_image):
     if ref is None:
         return None
     r = ref["image"]
+    r = r.astype(np.float64)  # squared norms of PCA images overflow float32
     return float(np.linalg.norm((gpu_image.astype(np.float64) - r).ravel()) / np.linalg.norm(r.ravel()))
 
 
@@ -147,7 +150,11 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
     name = args.workload
-    s = configs.settings_for(name)
+    overrides = {}
+    for kv in args.set:
+        key, val = kv.split("=", 1)
+        overrides[key] = float(val) if "." in val or "e" in val.lower() else int(val)
+    s = configs.settings_for(name, **overrides)
     sp = configs.synth_for(name)
     B, S, T, P = s.batch_size, s.time_stride, s.time_transformation_size, s.width * s.height
     host_batches = [synth_frames(sp, k * B, B) for k in range(4)]
@@ -196,7 +203,7 @@ def main():
             "value": frames_per_s, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
             "ms_per_step": ms_per_step, "host_submit_ms_per_step": t_host / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic (seeded interferogram generator)",
-            "config": {"workload": name, "width": s.width, "height": s.height, "depth_bytes": s.depth,
+            "config": {"workload": name, "overrides": overrides, "width": s.width, "height": s.height, "depth_bytes": s.depth,
                        "batch": B, "stride": S, "T": T, "space": ["none", "fresnel", "angular_spectrum"][
                            s.space_transformation], "time": ["none", "stft", "pca"][s.time_transformation],
                        "graph": bool(args.graph and not args.no_graph), "lanes": lanes}}

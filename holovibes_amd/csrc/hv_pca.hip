@@ -46,10 +46,12 @@ static int pca_pairs(int T) {
 static constexpr int kCovWaves = 4;
 static constexpr int kTile = 2 * 32 * 32;  // floats of one wave's two accumulators
 
+// Pixel slices per frame-group pair: about 512 workgroups in all, each
+// wave covering at least 256 pixels.
 static int pca_slices(int T, size_t P) {
     const int np = pca_pairs(T);
     int ns = 1;
-    while (ns * 2 * np <= 2048 && (size_t)(ns * 2) * kCovWaves * 64 <= P) ns *= 2;
+    while (np * ns * 2 <= 512 && (size_t)(ns * 2) * kCovWaves * 256 <= P) ns *= 2;
     return ns;
 }
 
@@ -215,34 +217,42 @@ __global__ void __launch_bounds__(64 * kCovWaves)
 
 // Sum the slices and write the full Hermitian matrix, column-major.
 // Tile element (acc, r, lane) is row 8 (r / 4) + 4 (lane / 32) + r % 4 and
-// column lane % 32 of the 32 x 32 MFMA output.
+// column lane % 32 of the 32 x 32 MFMA output.  Each workgroup owns 64
+// elements of one pair's tile; its four waves sum every fourth slice, then
+// one wave adds the four (a fixed order: deterministic).
 __global__ void __launch_bounds__(256)
     k_pca_cov_reduce(float2* __restrict__ cov, const float* __restrict__ partial, int T, int slices, int npairs,
                      const uint32_t* __restrict__ scale_mm, int use_scale) {
     const int pair = blockIdx.x;
     int F, G;
     pair_of(pair, F, G);
-    float inv = 1.0f;
-    if (use_scale) {
-        const float s = fp16_scale(scale_mm);
-        inv = 1.0f / (s * s);
+    const int g = threadIdx.x >> 6, el = threadIdx.x & 63;
+    const int e = blockIdx.y * 64 + el;  // 0 .. 1023
+    float re = 0.f, im = 0.f;
+    for (int sl = g; sl < slices; sl += 4) {
+        const float* t = partial + ((size_t)sl * npairs + pair) * kTile;
+        re += t[e];
+        im += t[1024 + e];
     }
-    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
-        const int r = e / 64, lane = e % 64;
-        const int i = 8 * (r / 4) + 4 * (lane / 32) + (r % 4), j = lane % 32;
-        const int fi = 32 * F + i, fj = 32 * G + j;
-        if (fi >= T || fj >= T) continue;
-        float re = 0.f, im = 0.f;
-        for (int s = 0; s < slices; ++s) {
-            const float* t = partial + ((size_t)s * npairs + pair) * kTile;
-            re += t[e];
-            im += t[1024 + e];
-        }
+    __shared__ float sr[4][64], si[4][64];
+    sr[g][el] = re;
+    si[g][el] = im;
+    __syncthreads();
+    if (g != 0) return;
+    re = (sr[0][el] + sr[1][el]) + (sr[2][el] + sr[3][el]);
+    im = (si[0][el] + si[1][el]) + (si[2][el] + si[3][el]);
+    if (use_scale) {
+        const float sc = fp16_scale(scale_mm);
+        const float inv = 1.0f / (sc * sc);
         re *= inv;
         im *= inv;
-        cov[fi + (size_t)fj * T] = make_float2(re, im);
-        if (F != G) cov[fj + (size_t)fi * T] = make_float2(re, -im);
     }
+    const int r = e / 64, lane = e % 64;
+    const int i = 8 * (r / 4) + 4 * (lane / 32) + (r % 4), jj = lane % 32;
+    const int fi = 32 * F + i, fj = 32 * G + jj;
+    if (fi >= T || fj >= T) return;
+    cov[fi + (size_t)fj * T] = make_float2(re, im);
+    if (F != G) cov[fj + (size_t)fi * T] = make_float2(re, -im);
 }
 
 hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int precision, void* ws,
@@ -269,7 +279,7 @@ hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int p
         default: return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
     }
     HV_CHECK_LAUNCH("k_pca_cov");
-    k_pca_cov_reduce<<<np, 256, 0, st>>>(cov, partial, T, slices, np, scale_mm, fp16 ? 1 : 0);
+    k_pca_cov_reduce<<<dim3(np, 1024 / 64), 256, 0, st>>>(cov, partial, T, slices, np, scale_mm, fp16 ? 1 : 0);
     HV_CHECK_LAUNCH("k_pca_cov_reduce");
     return HV_OK;
 }
@@ -297,8 +307,27 @@ __device__ __forceinline__ void rr_pair(int n, int r, int k, int& p, int& q) {
     q = a < b ? b : a;
 }
 
+#ifndef HV_JACOBI_MAX_SWEEPS
+#define HV_JACOBI_MAX_SWEEPS 40
+#endif
+#ifndef HV_JACOBI_NT32
+#define HV_JACOBI_NT32 512      // threads of the T <= 32 eigensolver
+#endif
+#ifndef HV_JACOBI_NT64
+#define HV_JACOBI_NT64 512      // threads of the T <= 64 eigensolver
+#endif
+
+// One sweep round applies n/2 disjoint rotations J_k (pair k = (p_k, q_k)).
+// A' = J^H A J acts on the 2 x 2 blocks A[k, k'] = rows (p_k, q_k) x columns
+// (p_k', q_k') independently: A'[k, k'] = J_k^H A[k, k'] J_k'.  So a round is
+// one rotation phase (n/2 threads) and one update phase in which each thread
+// owns whole blocks of A and pairs of rows of V -- two barriers per round.
+// A and V (n x n, column-major, leading dimension ld) live in LDS or global
+// memory; n is a compile-time constant (the pairs of a round are computed
+// with constant divisors).
 struct Rot {
-    double2 jpp, jpq, jqp, jqq;
+    double c, s;   // J = [[c, s], [-s conj(e), c conj(e)]]
+    double2 ec;    // conj(e), e = a_pq / |a_pq|
 };
 
 template <int NT>
@@ -313,162 +342,193 @@ __device__ double block_sum(double v, double* red) {
     return s;
 }
 
-// A (n x n, column-major) and V live in `A`/`V` (LDS or global).
-template <int NT>
-__device__ void jacobi_run(double2* A, double2* V, int n, double* evals, double2* evecs, Rot* rot, double* red,
+// x J restricted to one pair: (x_p, x_q) -> (x_p c - x_q s conj(e), x_p s + x_q c conj(e))
+__device__ __forceinline__ void rot_right(const Rot& R, double2& xp, double2& xq) {
+    const double2 t = zmul(xq, R.ec);
+    const double2 np_ = make_double2(xp.x * R.c - t.x * R.s, xp.y * R.c - t.y * R.s);
+    const double2 nq_ = make_double2(xp.x * R.s + t.x * R.c, xp.y * R.s + t.y * R.c);
+    xp = np_;
+    xq = nq_;
+}
+// J^H y restricted to one pair: (y_p, y_q) -> (c y_p - s e y_q, s y_p + c e y_q)
+__device__ __forceinline__ void rot_left(const Rot& R, double2& yp, double2& yq) {
+    const double2 t = zmul(yq, zconj(R.ec));
+    const double2 np_ = make_double2(R.c * yp.x - R.s * t.x, R.c * yp.y - R.s * t.y);
+    const double2 nq_ = make_double2(R.s * yp.x + R.c * t.x, R.s * yp.y + R.c * t.y);
+    yp = np_;
+    yq = nq_;
+}
+
+template <int NT, int N>
+__device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double2* evecs, Rot* rot, double* red,
                            int* order) {
+    constexpr int n = N, half = N / 2;
     const int tid = threadIdx.x;
-    auto a = [&](int i, int j) -> double2& { return A[i + (size_t)j * n]; };
-    auto v = [&](int i, int j) -> double2& { return V[i + (size_t)j * n]; };
+    auto a = [&](int i, int j) -> double2& { return A[i + (size_t)j * ld]; };
+    auto v = [&](int i, int j) -> double2& { return V[i + (size_t)j * ld]; };
     double fro = 0.0;
-    for (int e = tid; e < n * n; e += NT) fro += A[e].x * A[e].x + A[e].y * A[e].y;
+    for (int e = tid; e < n * n; e += NT) {
+        const double2 x = a(e % n, e / n);
+        fro += x.x * x.x + x.y * x.y;
+    }
     fro = block_sum<NT>(fro, red);
-    const int half = n / 2;
-    for (int sweep = 0; sweep < 40 && n > 1; ++sweep) {
+    for (int sweep = 0; sweep < HV_JACOBI_MAX_SWEEPS && n > 1; ++sweep) {
         double off = 0.0;
         for (int e = tid; e < n * n; e += NT) {
             const int i = e % n, j = e / n;
-            if (i != j) off += A[e].x * A[e].x + A[e].y * A[e].y;
+            if (i != j) {
+                const double2 x = a(i, j);
+                off += x.x * x.x + x.y * x.y;
+            }
         }
         off = block_sum<NT>(off, red);
-        if (off <= 1e-30 * fro || off == 0.0) break;
+        // relative off-diagonal norm below 1e-13: far under the FP32 input's own precision
+        if (off <= 1e-26 * fro || off == 0.0) break;
         for (int r = 0; r < n - 1; ++r) {
             for (int k = tid; k < half; k += NT) {
                 int p, q;
                 rr_pair(n, r, k, p, q);
                 const double2 apq = a(p, q);
-                const double mag = hypot(apq.x, apq.y);
+                const double mag2 = apq.x * apq.x + apq.y * apq.y;
                 Rot R;
-                if (mag == 0.0) {
-                    R.jpp = make_double2(1, 0);
-                    R.jpq = make_double2(0, 0);
-                    R.jqp = make_double2(0, 0);
-                    R.jqq = make_double2(1, 0);
+                if (mag2 == 0.0) {
+                    R.c = 1.0;
+                    R.s = 0.0;
+                    R.ec = make_double2(1.0, 0.0);
                 } else {
-                    const double2 e = make_double2(apq.x / mag, apq.y / mag);
-                    const double tau = (a(q, q).x - a(p, p).x) / (2.0 * mag);
+                    // t = sgn(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c
+                    const double inv = 1.0 / sqrt(mag2);
+                    R.ec = make_double2(apq.x * inv, -apq.y * inv);
+                    const double tau = (a(q, q).x - a(p, p).x) * (0.5 * inv);
                     const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-                    R.jpp = make_double2(c, 0);
-                    R.jpq = make_double2(s, 0);
-                    R.jqp = zmul(make_double2(-s, 0), zconj(e));
-                    R.jqq = zmul(make_double2(c, 0), zconj(e));
+                    R.c = 1.0 / sqrt(1.0 + t * t);
+                    R.s = t * R.c;
                 }
                 rot[k] = R;
             }
             __syncthreads();
-            // A <- A J, V <- V J (columns p, q of every pair)
-            for (int task = tid; task < half * n; task += NT) {
-                const int k = task / n, m = task % n;
-                int p, q;
+            // A <- J^H A J, block by block; V <- V J, row pair by row pair
+            for (int task = tid; task < half * half; task += NT) {
+                const int k = task % half, kk = task / half;  // row pair k, column pair kk
+                int p, q, pp, qq;
                 rr_pair(n, r, k, p, q);
-                const Rot R = rot[k];
-                const double2 amp = a(m, p), amq = a(m, q);
-                a(m, p) = zadd(zmul(amp, R.jpp), zmul(amq, R.jqp));
-                a(m, q) = zadd(zmul(amp, R.jpq), zmul(amq, R.jqq));
-                const double2 vmp = v(m, p), vmq = v(m, q);
-                v(m, p) = zadd(zmul(vmp, R.jpp), zmul(vmq, R.jqp));
-                v(m, q) = zadd(zmul(vmp, R.jpq), zmul(vmq, R.jqq));
+                rr_pair(n, r, kk, pp, qq);
+                const Rot Rr = rot[k], Rc = rot[kk];
+                double2 b00 = a(p, pp), b01 = a(p, qq), b10 = a(q, pp), b11 = a(q, qq);
+                rot_right(Rc, b00, b01);
+                rot_right(Rc, b10, b11);
+                rot_left(Rr, b00, b10);
+                rot_left(Rr, b01, b11);
+                if (k == kk) {  // the annihilated pair: exact zeros, real diagonal
+                    b01 = make_double2(0.0, 0.0);
+                    b10 = make_double2(0.0, 0.0);
+                    b00.y = 0.0;
+                    b11.y = 0.0;
+                }
+                a(p, pp) = b00;
+                a(p, qq) = b01;
+                a(q, pp) = b10;
+                a(q, qq) = b11;
             }
-            __syncthreads();
-            // A <- J^H A (rows p, q of every pair); zero the annihilated pair
             for (int task = tid; task < half * n; task += NT) {
-                const int k = task / n, m = task % n;
-                int p, q;
-                rr_pair(n, r, k, p, q);
-                const Rot R = rot[k];
-                const double2 apm = a(p, m), aqm = a(q, m);
-                double2 np_ = zadd(zmul(zconj(R.jpp), apm), zmul(zconj(R.jqp), aqm));
-                double2 nq_ = zadd(zmul(zconj(R.jpq), apm), zmul(zconj(R.jqq), aqm));
-                if (m == q) np_ = make_double2(0, 0);
-                if (m == p) nq_ = make_double2(0, 0);
-                if (m == p) np_.y = 0;
-                if (m == q) nq_.y = 0;
-                a(p, m) = np_;
-                a(q, m) = nq_;
+                const int m = task % n, kk = task / n;
+                int pp, qq;
+                rr_pair(n, r, kk, pp, qq);
+                double2 x = v(m, pp), y = v(m, qq);
+                rot_right(rot[kk], x, y);
+                v(m, pp) = x;
+                v(m, qq) = y;
             }
             __syncthreads();
         }
     }
-    // ascending order (stable), phase-normalised eigenvectors
-    if (tid == 0) {
-        for (int i = 0; i < n; ++i) order[i] = i;
-        for (int i = 1; i < n; ++i) {
-            const int key = order[i];
-            const double kv = a(key, key).x;
-            int j = i - 1;
-            while (j >= 0 && a(order[j], order[j]).x > kv) {
-                order[j + 1] = order[j];
-                --j;
-            }
-            order[j + 1] = key;
+    // ascending order (stable: ties keep their index order), by parallel ranks
+    for (int j = tid; j < n; j += NT) {
+        const double dj = a(j, j).x;
+        int rank = 0;
+        for (int i = 0; i < n; ++i) {
+            const double di = a(i, i).x;
+            rank += (di < dj || (di == dj && i < j)) ? 1 : 0;
         }
+        order[rank] = j;
     }
     __syncthreads();
+    // phase-normalised eigenvectors: the largest component real positive
+    // (the first of equal magnitudes within a relative 1e-12)
     for (int jj = tid; jj < n; jj += NT) {
         const int j = order[jj];
         evals[jj] = a(j, j).x;
         int kmax = 0;
         double best = -1.0;
         for (int k = 0; k < n; ++k) {
-            const double m = hypot(v(k, j).x, v(k, j).y);
-            if (m > best * (1 + 1e-12)) {
-                best = m;
+            const double2 x = v(k, j);
+            const double m2 = x.x * x.x + x.y * x.y;
+            if (m2 > best * (1 + 2e-12)) {
+                best = m2;
                 kmax = k;
             }
         }
         const double2 ph = v(kmax, j);
-        const double m = hypot(ph.x, ph.y);
-        const double2 rotp = make_double2(ph.x / m, -ph.y / m);
+        const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
+        const double2 rotp = make_double2(ph.x * inv, -ph.y * inv);
         for (int k = 0; k < n; ++k) evecs[k + (size_t)jj * n] = zmul(v(k, j), rotp);
     }
 }
 
-template <int NT, int NMAX>
-__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const float2* cov, int n) {
-    __shared__ double2 A[NMAX * NMAX], V[NMAX * NMAX];
-    __shared__ Rot rot[NMAX / 2];
+template <int NT, int N>
+__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const float2* cov) {
+    constexpr int LD = N + 1;  // padded columns: row and column walks spread over the banks
+    __shared__ double2 A[N * LD], V[N * LD];
+    __shared__ Rot rot[N / 2 > 0 ? N / 2 : 1];
     __shared__ double red[NT / 64];
-    __shared__ int order[NMAX];
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e % n, j = e / n;
+    __shared__ int order[N];
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
         // Hermitian from the lower triangle (CUBLAS_FILL_MODE_LOWER)
-        const float2 c = i >= j ? cov[i + (size_t)j * n] : cov[j + (size_t)i * n];
-        A[e] = make_double2(c.x, i >= j ? c.y : -c.y);
-        if (i == j) A[e].y = 0.0;
-        V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        const float2 c = i >= j ? cov[i + (size_t)j * N] : cov[j + (size_t)i * N];
+        A[i + j * LD] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
+        V[i + j * LD] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    jacobi_run<NT>(A, V, n, evals, evecs, rot, red, order);
+    jacobi_run<NT, N>(A, V, LD, evals, evecs, rot, red, order);
 }
 
-template <int NT, int NMAX>
-__global__ void __launch_bounds__(NT)
-    k_jacobi_global(double* evals, double2* evecs, const float2* cov, int n, double2* work) {
-    __shared__ Rot rot[NMAX / 2];
+template <int NT, int N>
+__global__ void __launch_bounds__(NT) k_jacobi_global(double* evals, double2* evecs, const float2* cov,
+                                                     double2* work) {
+    __shared__ Rot rot[N / 2];
     __shared__ double red[NT / 64];
-    __shared__ int order[NMAX];
+    __shared__ int order[N];
     double2* A = work;
-    double2* V = work + (size_t)n * n;
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e % n, j = e / n;
-        const float2 c = i >= j ? cov[i + (size_t)j * n] : cov[j + (size_t)i * n];
-        A[e] = make_double2(c.x, i >= j ? c.y : -c.y);
-        if (i == j) A[e].y = 0.0;
+    double2* V = work + (size_t)N * N;
+    for (int e = threadIdx.x; e < N * N; e += NT) {
+        const int i = e % N, j = e / N;
+        const float2 c = i >= j ? cov[i + (size_t)j * N] : cov[j + (size_t)i * N];
+        A[e] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
         V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    jacobi_run<NT>(A, V, n, evals, evecs, rot, red, order);
+    jacobi_run<NT, N>(A, V, N, evals, evecs, rot, red, order);
 }
 
 size_t eig_workspace_bytes(int T) { return T <= 64 ? 0 : (size_t)2 * T * T * sizeof(double2); }
 
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st) {
-    if (T <= 32) k_jacobi_lds<256, 32><<<1, 256, 0, st>>>(evals, evecs, cov, T);
-    else if (T <= 64) k_jacobi_lds<512, 64><<<1, 512, 0, st>>>(evals, evecs, cov, T);
-    else {
-        if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
-        k_jacobi_global<1024, 256><<<1, 1024, 0, st>>>(evals, evecs, cov, T, (double2*)work);
+    switch (T) {
+        case 1: k_jacobi_lds<64, 1><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 2: k_jacobi_lds<64, 2><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 4: k_jacobi_lds<64, 4><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 8: k_jacobi_lds<64, 8><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 16: k_jacobi_lds<256, 16><<<1, 256, 0, st>>>(evals, evecs, cov); break;
+        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov); break;
+        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov); break;
+        case 128:
+        case 256:
+            if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
+            if (T == 128) k_jacobi_global<1024, 128><<<1, 1024, 0, st>>>(evals, evecs, cov, (double2*)work);
+            else k_jacobi_global<1024, 256><<<1, 1024, 0, st>>>(evals, evecs, cov, (double2*)work);
+            break;
+        default: return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
     }
     HV_CHECK_LAUNCH("k_jacobi");
     return HV_OK;

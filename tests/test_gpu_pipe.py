@@ -15,7 +15,9 @@ pytestmark = pytest.mark.gpu
 
 
 def rel_l2(a, b):
-    return float(np.linalg.norm((np.asarray(a, np.float64) - b).ravel()) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+    # in float64: squared norms of PCA images (~1e20 per pixel) overflow float32
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
 def run_both(oracle, settings, frames_list, graph=False, fused=True, path=None):

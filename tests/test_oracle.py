@@ -24,6 +24,8 @@ def crand(*shape):
 
 
 def rel_l2(a, b):
+    cplx = np.iscomplexobj(a) or np.iscomplexobj(b)
+    a, b = np.asarray(a, np.complex128 if cplx else np.float64), np.asarray(b, np.complex128 if cplx else np.float64)
     return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
