@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kBinThreads)
         const int y = y0 + i;
         if (y >= height) break;  // uniform across the workgroup
         // 1. DFT over the window of each pixel; bins [pmin, pmin + K) to LDS
-#pragma unroll
+#pragma unroll 1
         for (int q = 0; q < C::PPT; ++q) {
             const int x = threadIdx.x + q * kBinThreads;
             if (x >= W) continue;
@@ -299,6 +299,7 @@ bool bins_first_supported(int width, int height, int T, int K, int depth) {
     if (!(width == 64 || width == 128 || width == 256 || width == 512 || width == 1024)) return false;
     if (!is_pow2(height) || height < 16 || height > 4096) return false;
     if (!is_pow2(T) || T < 2 || T > 64 || K < 1 || K > T) return false;
+    if (width == 1024 && T > 32) return false;  // two pixels' windows per thread exceed the registers
     if (depth != HV_DEPTH_8 && depth != HV_DEPTH_16 && depth != HV_DEPTH_32F) return false;
     return rows_lds_for(width, K) <= 160 * 1024;
 }

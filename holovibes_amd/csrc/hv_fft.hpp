@@ -161,35 +161,59 @@ struct FftTw {
     static constexpr int COUNT = base(S::stages()) > 0 ? base(S::stages()) : 1;
 };
 
+// Twiddles w^r, r = 1 .. R - 1, of butterfly k of thread t at stage S
+// (w = exp(-2 pi i (j mod Ns) / (Ns R)), j = t + k NT): one table load, the
+// powers by a product tree of depth <= 4.
+template <int N, int E, int S>
+__device__ __forceinline__ void stage_twiddles_gen(int t, int k, float2* w) {
+    using FS = FftShape<N, E>;
+    constexpr int R = FS::radix(S), Ns = FS::ns(S), NT = FS::NT, LOGN = clog2(N);
+    const int kk = (t + k * NT) & (Ns - 1);
+    w[1] = twiddle<-1>(kk * (N / (Ns * R)), LOGN);
+    if constexpr (R > 2) {
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+    }
+    if constexpr (R > 4) {
+        w[4] = cmul(w[2], w[2]);
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+    }
+    if constexpr (R > 8) {
+        w[8] = cmul(w[4], w[4]);
+#pragma unroll
+        for (int r = 9; r < R; ++r) w[r] = cmul(w[8], w[r - 8]);
+    }
+}
+
 template <int N, int E>
 __device__ __forceinline__ void fft_twiddles(int t, float2* tw) {
     using S = FftShape<N, E>;
-    constexpr int kE = S::kE, NT = S::NT, NS = S::stages(), LOGN = clog2(N);
+    constexpr int kE = S::kE, NS = S::stages();
     static_for<1, NS>([&](auto s_) {
         constexpr int s = decltype(s_)::value;
-        constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R, B0 = FftTw<N, E>::base(s);
+        constexpr int R = S::radix(s), NB = kE / R;
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            const int kk = (t + k * NT) & (Ns - 1);
-            float2* w = tw + B0 + k * (R - 1) - 1;  // w[r], r >= 1
-            w[1] = twiddle<-1>(kk * (N / (Ns * R)), LOGN);
-            if constexpr (R > 2) {
-                w[2] = cmul(w[1], w[1]);
-                w[3] = cmul(w[2], w[1]);
-            }
-            if constexpr (R > 4) {
-                w[4] = cmul(w[2], w[2]);
-                w[5] = cmul(w[4], w[1]);
-                w[6] = cmul(w[4], w[2]);
-                w[7] = cmul(w[4], w[3]);
-            }
-            if constexpr (R > 8) {
-                w[8] = cmul(w[4], w[4]);
-#pragma unroll
-                for (int r = 9; r < R; ++r) w[r] = cmul(w[8], w[r - 8]);
-            }
-        }
+        for (int k = 0; k < NB; ++k) stage_twiddles_gen<N, E, s>(t, k, tw + FftTw<N, E>::base(s) + k * (R - 1) - 1);
     });
+}
+
+// Twiddle source of a run: a thread's table from fft_twiddles (const
+// float2*), or TwOnTheFly -- each stage's twiddles generated where they are
+// used (fewer registers held across the run, for long transforms).
+struct TwOnTheFly {};
+
+template <int N, int E, int S>
+__device__ __forceinline__ void stage_twiddles(const float2* tw, int, int k, float2* w) {
+    constexpr int R = FftShape<N, E>::radix(S);
+    const float2* src = tw + FftTw<N, E>::base(S) + k * (R - 1) - 1;
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[r] = src[r];
+}
+template <int N, int E, int S>
+__device__ __forceinline__ void stage_twiddles(TwOnTheFly, int t, int k, float2* w) {
+    stage_twiddles_gen<N, E, S>(t, k, w);
 }
 
 // a * conj(b)
@@ -221,8 +245,8 @@ __device__ __forceinline__ void fft_sync() {
 // in LDS (exchange order, no barrier) for a later run, possibly with another
 // thread mapping after a workgroup barrier.
 template <int N, int E, int DIR, bool STORE_SYNC = false, bool WAVE = false, int FIRST = 0, int LAST = 0,
-          class Store, class Lds>
-__device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds);
+          class TW, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_tw(int t, float2* v, const TW& tw, Store&& store, Lds&& lds);
 
 template <int N, int E, int DIR, bool STORE_SYNC, class Store, class Lds>
 __device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds&& lds) {
@@ -231,8 +255,8 @@ __device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds
     fft_compute_tw<N, E, DIR, STORE_SYNC>(t, v, tw, store, lds);
 }
 
-template <int N, int E, int DIR, bool STORE_SYNC, bool WAVE, int FIRST, int LAST, class Store, class Lds>
-__device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds) {
+template <int N, int E, int DIR, bool STORE_SYNC, bool WAVE, int FIRST, int LAST, class TW, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_tw(int t, float2* v, const TW& tw, Store&& store, Lds&& lds) {
     static_assert(!WAVE || FftShape<N, E>::NT <= 64, "a wave-local transform has at most 64 threads");
     using S = FftShape<N, E>;
     constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
@@ -252,8 +276,9 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
         for (int k = 0; k < NB; ++k) {
             const int j = t + k * NT;
             if constexpr (Ns > 1) {
-                // w^r, w = exp(dir 2 pi i (j mod Ns) / (Ns R)), from the thread's table
-                const float2* w = tw + FftTw<N, E>::base(s) + k * (R - 1) - 1;
+                // w^r, w = exp(dir 2 pi i (j mod Ns) / (Ns R))
+                float2 w[R];
+                stage_twiddles<N, E, s>(tw, t, k, w);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
                     v[k * R + r] = DIR < 0 ? cmul(v[k * R + r], w[r]) : cmulc(v[k * R + r], w[r]);
@@ -305,8 +330,8 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const float2* t
 // positions.  A forward run whose results stay in registers, a pointwise
 // product and this adjoint run make a filter with one exchange fewer than a
 // forward and an inverse transform in natural order.
-template <int N, int E, int DIR_FWD, bool WAVE = false, class Store, class Lds>
-__device__ __forceinline__ void fft_compute_adj_tw(int t, float2* v, const float2* tw, Store&& store, Lds&& lds) {
+template <int N, int E, int DIR_FWD, bool WAVE = false, class TW, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_adj_tw(int t, float2* v, const TW& tw, Store&& store, Lds&& lds) {
     using S = FftShape<N, E>;
     constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
     constexpr int DIR = -DIR_FWD;
@@ -317,7 +342,8 @@ __device__ __forceinline__ void fft_compute_adj_tw(int t, float2* v, const float
         for (int k = 0; k < NB; ++k) {
             dft_reg<R, DIR>(&v[k * R]);
             if constexpr (Ns > 1) {
-                const float2* w = tw + FftTw<N, E>::base(s) + k * (R - 1) - 1;
+                float2 w[R];
+                stage_twiddles<N, E, s>(tw, t, k, w);
 #pragma unroll
                 for (int r = 1; r < R; ++r)
                     v[k * R + r] = DIR_FWD < 0 ? cmulc(v[k * R + r], w[r]) : cmul(v[k * R + r], w[r]);

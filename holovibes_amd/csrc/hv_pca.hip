@@ -446,6 +446,7 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
     for (int j = tid; j < n; j += NT) {
         const double dj = a(j, j).x;
         int rank = 0;
+#pragma unroll 4
         for (int i = 0; i < n; ++i) {
             const double di = a(i, i).x;
             rank += (di < dj || (di == dj && i < j)) ? 1 : 0;
@@ -460,6 +461,7 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
         evals[jj] = a(j, j).x;
         int kmax = 0;
         double best = -1.0;
+#pragma unroll 4
         for (int k = 0; k < n; ++k) {
             const double2 x = v(k, j);
             const double m2 = x.x * x.x + x.y * x.y;
@@ -471,6 +473,7 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
         const double2 ph = v(kmax, j);
         const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
         const double2 rotp = make_double2(ph.x * inv, -ph.y * inv);
+#pragma unroll 4
         for (int k = 0; k < n; ++k) evecs[k + (size_t)jj * n] = zmul(v(k, j), rotp);
     }
 }

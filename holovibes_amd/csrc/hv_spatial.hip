@@ -182,6 +182,12 @@ enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 #ifndef HV_COL_CW_SMALL
 #define HV_COL_CW_SMALL 16      // columns per workgroup, column lengths <= 512
 #endif
+#ifndef HV_COL_CW_1024
+#define HV_COL_CW_1024 8        // columns per workgroup, column length 1024
+#endif
+#ifndef HV_COL_CW_BIG
+#define HV_COL_CW_BIG 4         // columns per workgroup, column length 2048 (4096: 2)
+#endif
 #ifndef HV_COL_MIN_WG
 #define HV_COL_MIN_WG 216       // keep at least this many column workgroups when pipelining (~ one per CU)
 #endif
@@ -268,7 +274,9 @@ template <int N>
 struct ColCfg {
     static constexpr int E = N >= 1024 ? 16 : HV_COL_E_SMALL;
     static constexpr int NT = FftShape<N, E>::NT;
-    static constexpr int CW = N <= 512 ? HV_COL_CW_SMALL : (N == 1024 ? 8 : 4);  // columns per workgroup
+    // columns per workgroup: 128-byte rows up to N = 512; longer columns
+    // narrow the tile (the N x CW tile and 1024 threads' registers bound it)
+    static constexpr int CW = N <= 512 ? HV_COL_CW_SMALL : (N == 1024 ? HV_COL_CW_1024 : (N == 2048 ? HV_COL_CW_BIG : 2));
     static constexpr int THREADS = NT * CW;
 };
 
@@ -311,12 +319,13 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
         if constexpr (MODE == COL_LENS) {
             // forward column FFT with its results left in registers, x lens
             // (already / (W H)), inverse column FFT as the forward's adjoint
-            fft_compute_tw<N, C::E, -1>(t, cur, tw, [](int, float2) {}, lds_at);
+            fft_compute_tw<N, C::E, -1>(t, cur, static_cast<const float2*>(tw), [](int, float2) {}, lds_at);
 #pragma unroll
             for (int k = 0; k < kE; ++k) cur[k] = cmul(cur[k], lr[k]);
-            fft_compute_adj_tw<N, C::E, -1>(t, cur, tw, store, lds_at);
+            fft_compute_adj_tw<N, C::E, -1>(t, cur, static_cast<const float2*>(tw), store, lds_at);
         } else {
-            fft_compute_tw<N, C::E, MODE == COL_FWD ? -1 : +1>(t, cur, tw, store, lds_at);
+            fft_compute_tw<N, C::E, MODE == COL_FWD ? -1 : +1>(t, cur, static_cast<const float2*>(tw), store,
+                                                                lds_at);
         }
 #pragma unroll
         for (int k = 0; k < kE; ++k) cur[k] = nxt[k];
