@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprofv3), print the JSON without extras")
     ap.add_argument("--stage-iters", type=int, default=50)
+    ap.add_argument("--serving", action="store_true",
+                    help="serving mix instead: four sources, Poisson batch arrivals, HBM-budget admission")
+    ap.add_argument("--seconds", type=float, default=20.0, help="serving run length")
+    ap.add_argument("--rate-scale", type=float, default=1.0, help="serving: multiply every source's batch rate")
     ap.add_argument("--profile-stages", action="store_true",
                     help="with --profile-steps: after the steps, time the stages and print (for rocprofv3)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -135,8 +139,99 @@ def cpu_baseline(name, settings, batches, seconds):
                       f"{el:.1f} s on {cores} host threads"}
 
 
+def serving(args):
+    """Four sources with Poisson batch arrivals served side by side by one
+    hv_server (per-source pipes on their own streams, hipGraph replay);
+    batch -> image latency from arrival to completion."""
+    import numpy as np
+    import torch
+
+    from holovibes_amd import configs
+    from holovibes_amd.pipe import Pipe, Server, synth_frames
+
+    torch.cuda.set_device(0)
+    srv = Server(configs.SERVING_BUDGET_BYTES)
+    sources = []
+    for i, (name, over, rate) in enumerate(configs.SERVING_MIX):
+        s = configs.settings_for(name, **over)
+        sp = configs.synth_for(name, seed_offset=i, depth=s.depth,
+                               bits=12 if s.depth == 2 else 8, mean_counts=1024.0 if s.depth == 2 else 64.0)
+        B = s.batch_size
+        host = [synth_frames(sp, k * B, B) for k in range(2)]
+        dev = [torch.from_numpy(h.view(np.int16) if h.dtype == np.uint16 else h).cuda() for h in host]
+        sid = srv.add_source(s)
+        if sid is None:
+            raise RuntimeError(f"source {name} refused by the HBM budget")
+        sources.append({"name": name, "sid": sid, "rate": rate * args.rate_scale, "B": B, "dev": dev,
+                        "P": s.width * s.height, "T": s.time_transformation_size, "k": 0,
+                        "bytes": Pipe.memory_bytes(s)})
+    torch.cuda.synchronize()
+    by_sid = {src["sid"]: src for src in sources}
+    # warm-up: fill every window and capture each source's graphs
+    for src in sources:
+        for _ in range(4 * max(1, src["T"] // src["B"])):
+            srv.submit(src["sid"], src["dev"][src["k"] % 2].data_ptr(), time.perf_counter_ns())
+            src["k"] += 1
+    srv.drain()
+    srv.poll()
+    # Poisson arrivals for every source over the run
+    rng = np.random.default_rng(4000)
+    events = []
+    for src in sources:
+        t = 0.0
+        while True:
+            t += rng.exponential(1.0 / src["rate"])
+            if t >= args.seconds:
+                break
+            events.append((t, src["sid"]))
+    events.sort()
+    lat = {src["sid"]: [] for src in sources}
+    frames = {src["sid"]: 0 for src in sources}
+    t0 = time.perf_counter_ns()
+    for t_ev, sid in events:
+        target = t0 + int(t_ev * 1e9)
+        while True:
+            now = time.perf_counter_ns()
+            if now >= target:
+                break
+            for s_id, l in srv.poll():
+                lat[s_id].append(l)
+        src = by_sid[sid]
+        srv.submit(sid, src["dev"][src["k"] % 2].data_ptr(), target)
+        src["k"] += 1
+        frames[sid] += src["B"]
+        for s_id, l in srv.poll():
+            lat[s_id].append(l)
+    srv.drain()
+    for s_id, l in srv.poll():
+        lat[s_id].append(l)
+    el = (time.perf_counter_ns() - t0) / 1e9
+    per = []
+    all_lat = []
+    for src in sources:
+        l = sorted(lat[src["sid"]])
+        all_lat += l
+        per.append({"source": src["name"], "batches_per_s": src["rate"], "batch": src["B"],
+                    "images": len(l), "p50_latency_ms": (l[len(l) // 2] / 1e6) if l else None,
+                    "p99_latency_ms": (l[min(len(l) - 1, int(0.99 * len(l)))] / 1e6) if l else None,
+                    "hbm_bytes": src["bytes"]})
+    all_lat.sort()
+    line = {"metric": "serving mix: aggregate frames/s + p50/p99 batch->image latency",
+            "value": sum(frames.values()) / el, "unit": "frames/s", "n_gpus": 1, "seconds": el,
+            "higher_is_better": True, "data": "synthetic (seeded interferogram generator)",
+            "p50_latency_ms": all_lat[len(all_lat) // 2] / 1e6 if all_lat else None,
+            "p99_latency_ms": all_lat[min(len(all_lat) - 1, int(0.99 * len(all_lat)))] / 1e6 if all_lat else None,
+            "budget_bytes": configs.SERVING_BUDGET_BYTES, "used_bytes": srv.memory_bytes(),
+            "sources": per, "config": {"workload": "serving_mix", "rate_scale": args.rate_scale}}
+    srv.close()
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
+    if args.serving:
+        serving(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the pipe shards by source, "
               "one pipe per GPU", file=sys.stderr)

@@ -108,3 +108,14 @@ def synth_for(name: str, seed_offset: int = 0, **overrides) -> SynthParams:
         setattr(p, k, v)
     return p
 
+
+
+# Serving mix: (workload, settings overrides, batches per second).  All
+# sources deliver uint16 frames from the config-1 style generator.
+SERVING_MIX = [
+    ("c1", {}, 1500.0),                        # 256^2, B = S = T = 16
+    ("metric", {}, 600.0),                     # 512^2, B = S = T = 32
+    ("c2", {"depth": HV_DEPTH_16}, 300.0),     # 512^2 PCA, T = 32
+    ("c3", {}, 80.0),                          # 1024^2, T = 64, B = S = 32
+]
+SERVING_BUDGET_BYTES = 64 * 10**9
