@@ -77,6 +77,34 @@ constexpr bool bins_rows_prefetch() {
     return HV_BINS_RPW_ROWS > 1 && T * BinRowCfg<W>::PPT <= 32;
 }
 
+// Bins k in [pmin, pmin + K) of the length-T DFT of a real sequence x, from
+// one complex FFT of half the length: z_n = x_2n + i x_2n+1, Z = FFT_M(z)
+// (M = T/2), then X_k = E_k + w^k O_k with E_k = (Z_k + conj Z_{M-k}) / 2,
+// O_k = (Z_k - conj Z_{M-k}) / 2i, w = exp(-2 pi i / T) (k <= M), and
+// X_k = conj X_{T-k} above M.  emit(k, X_k) for each wanted bin.
+template <int T, class Emit>
+__device__ __forceinline__ void real_dft_bins(const float* x, int pmin, int K, Emit&& emit) {
+    constexpr int M = T / 2;
+    float2 z[M];
+#pragma unroll
+    for (int n = 0; n < M; ++n) z[n] = make_float2(x[2 * n], x[2 * n + 1]);
+    if constexpr (M > 1) fft_reg<M, -1>(z);
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+        if (k < pmin || k >= pmin + K) continue;  // uniform: the bin range is a kernel argument
+        const int kk = k <= M ? k : T - k;
+        const float2 a = z[kk % M], bc = z[(M - kk) % M];
+        const float2 b = make_float2(bc.x, -bc.y);
+        const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+        const float2 d = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y - b.y));
+        const float2 o = make_float2(d.y, -d.x);  // d / i
+        const float2 w = twiddle<-1>(kk, clog2(T));
+        float2 v = make_float2(e.x + (w.x * o.x - w.y * o.y), e.y + (w.x * o.y + w.y * o.x));
+        if (k > M) v.y = -v.y;
+        emit(k, v);
+    }
+}
+
 template <int W, int T, typename TIn>
 __global__ void __launch_bounds__(kBinThreads)
     k_bins_rows(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
@@ -117,14 +145,11 @@ __global__ void __launch_bounds__(kBinThreads)
             const int x = threadIdx.x + q * kBinThreads;
             if (x >= W) continue;
             if constexpr (!PF) load_window(y, x, xr[0]);
-            float2 s[T];
-#pragma unroll
-            for (int f = 0; f < T; ++f) s[f] = make_float2(xr[PF ? q : 0][f], 0.f);
-            fft_reg<T, -1>(s);
+            const float* xs = xr[PF ? q : 0];
             const float2 c = chirp ? chirp[(size_t)y * W + x] : make_float2(1.f, 0.f);
-#pragma unroll
-            for (int k = 0; k < T; ++k)
-                if (k >= pmin && k < pmin + K) rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(s[k], c) : s[k];
+            real_dft_bins<T>(xs, pmin, K, [&](int k, float2 v) {
+                rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(v, c) : v;
+            });
         }
         if constexpr (PF) {
             if (i + 1 < rpw && y + 1 < height) {

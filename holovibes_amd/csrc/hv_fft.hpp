@@ -85,28 +85,30 @@ __device__ __forceinline__ void dft_reg(float2* v) {
     for (int i = 0; i < R; ++i) v[i] = t[i];
 }
 
-// Stage plan for length N with E values per thread.
+// Stage plan for length N with E values per thread: radix R_s = min(E, 16,
+// what is left of N), each thread running E / R_s butterflies per stage.
 template <int N, int E>
 struct FftShape {
     static constexpr int kE = E < N ? E : N;
     static constexpr int NT = N / kE;
+    static constexpr int RMAX = kE < 16 ? kE : 16;
     static constexpr int stages() {
         int s = 0, rem = N;
         while (rem > 1) {
-            rem /= (rem >= kE ? kE : rem);
+            rem /= (rem >= RMAX ? RMAX : rem);
             ++s;
         }
         return s;
     }
     static constexpr int radix(int s) {
         int rem = N;
-        for (int i = 0; i < s; ++i) rem /= (rem >= kE ? kE : rem);
-        return rem >= kE ? kE : rem;
+        for (int i = 0; i < s; ++i) rem /= (rem >= RMAX ? RMAX : rem);
+        return rem >= RMAX ? RMAX : rem;
     }
     static constexpr int ns(int s) {
         int n = 1, rem = N;
         for (int i = 0; i < s; ++i) {
-            const int r = rem >= kE ? kE : rem;
+            const int r = rem >= RMAX ? RMAX : rem;
             n *= r;
             rem /= r;
         }
