@@ -41,6 +41,9 @@ namespace {
 #ifndef HV_BINS_IMG_THREADS
 #define HV_BINS_IMG_THREADS 512
 #endif
+#ifndef HV_BINS_IMG_MT
+#define HV_BINS_IMG_MT 2        // plane rows each wavefront transforms side by side in the image kernel
+#endif
 #ifndef HV_BINS_IMG_DEPTH
 #define HV_BINS_IMG_DEPTH 1     // plane rows loaded ahead by the image kernel
 #endif
@@ -108,16 +111,13 @@ __device__ __forceinline__ void real_dft_bins(const float* x, int pmin, int K, E
 template <int W, int T, typename TIn>
 __global__ void __launch_bounds__(kBinThreads)
     k_bins_rows(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
-                int height, int slot0, int pmin, int K, uint32_t* mm_reset) {
+                int height, int slot0, int pmin, int K, uint32_t* mm_reset, int mm_slots) {
     constexpr int rpw = HV_BINS_RPW_ROWS;
     using C = BinRowCfg<W>;
     constexpr int kE = FftShape<W, C::E>::kE;
     constexpr bool PF = bins_rows_prefetch<W, T>();
     extern __shared__ float2 rows[];
-    if (mm_reset && blockIdx.x == 0 && threadIdx.x == 0) {
-        mm_reset[0] = float_to_ordered(INFINITY);
-        mm_reset[1] = float_to_ordered(-INFINITY);
-    }
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
     const size_t P = (size_t)W * height;
     const int y0 = blockIdx.x * rpw;
     const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
@@ -183,46 +183,50 @@ __global__ void __launch_bounds__(kBinThreads)
 template <int W, int IMG>
 __global__ void __launch_bounds__(img_threads<W>())
     k_bins_row_ifft_image(const float2* __restrict__ planes, float* __restrict__ out, int height, int K, int rpw,
-                          float count, uint32_t* mm) {
+                          float count, uint32_t* mm, int mm_slots) {
     constexpr int kImgThreads = img_threads<W>();
     using C = BinRowCfg<W, kImgThreads>;
     constexpr int kE = FftShape<W, C::E>::kE;
-    __shared__ float2 xch[C::G * C::PITCH];
+    constexpr int MT = W >= 1024 ? 1 : HV_BINS_IMG_MT;  // plane rows per group per round, interleaved
+    constexpr int GR = C::G * MT;      // plane rows per round
+    __shared__ float2 xch[GR * C::PITCH];
     __shared__ float part[C::G * W];
     const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
     const size_t P = (size_t)W * height;
     const int y0 = blockIdx.x * rpw;
     const int nrows = min(rpw, height - y0);
-    const int rounds = (K + C::G - 1) / C::G;
+    const int rounds = (K + GR - 1) / GR;
     const int items = nrows * rounds;  // (row, round) pairs, loaded ahead
-    float2* lrow = xch + g * C::PITCH;
-    auto lds_at = [&](int n) -> float2& { return lrow[C::pad(n)]; };
+    // plane row of transform m of this group in round r: r GR + m G + g
+    auto lds_at = [&](int m, int n) -> float2& { return xch[(m * C::G + g) * C::PITCH + C::pad(n)]; };
     auto load_item = [&](int it, float2* v) {
         if (it >= items) return;
-        const int row = (it % rounds) * C::G + g;
-        const float2* src = planes + (size_t)(row < K ? row : 0) * P + (size_t)(y0 + it / rounds) * W;
-        fft_load<W, C::E>(t, [&](int n) -> float2 { return src[n]; }, v);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int row = (it % rounds) * GR + m * C::G + g;
+            const float2* src = planes + (size_t)(row < K ? row : 0) * P + (size_t)(y0 + it / rounds) * W;
+            fft_load<W, C::E>(t, [&](int n) -> float2 { return src[n]; }, v + m * kE);
+        }
     };
     float2 tw[FftTw<W, C::E>::COUNT];
     fft_twiddles<W, C::E>(t, tw);
     float acc[kE];
     // D items in flight beyond the one being transformed
     constexpr int D = HV_BINS_IMG_DEPTH;
-    float2 buf[D + 1][kE];
+    float2 buf[D + 1][MT * kE];
 #pragma unroll
     for (int d = 0; d < D; ++d) load_item(d, buf[d]);
     float vmin = INFINITY, vmax = -INFINITY;
 #pragma unroll 1
     for (int it = 0; it < items; ++it) {
         const int r = it % rounds;
-        const bool active = r * C::G + g < K;
         load_item(it + D, buf[D]);
         if (r == 0) {
 #pragma unroll
             for (int i = 0; i < kE; ++i) acc[i] = 0.f;
         }
-        fft_compute_tw<W, C::E, +1, false, (C::NT <= 64)>(t, buf[0], tw, [&](int, float2 z, int slot) {
-            if (active) acc[slot] += image_term_fast<IMG>(z);
+        fft_compute_multi<W, C::E, +1, MT, (C::NT <= 64)>(t, buf[0], tw, [&](int m, int, float2 z, int slot) {
+            if (r * GR + m * C::G + g < K) acc[slot] += image_term_fast<IMG>(z);
         }, lds_at);
         if (r == rounds - 1) {
             // sum the G groups' partial sums of each pixel of this row
@@ -237,7 +241,7 @@ __global__ void __launch_bounds__(img_threads<W>())
                 if (x >= W) continue;
                 float a = 0.f;
 #pragma unroll
-                for (int j = 0; j < C::G; ++j) a += part[j * W + x];
+                for (int j2 = 0; j2 < C::G; ++j2) a += part[j2 * W + x];
                 const float val = a / count;
                 out[(size_t)y * W + x] = val;
                 vmin = fminf(vmin, val);
@@ -247,16 +251,16 @@ __global__ void __launch_bounds__(img_threads<W>())
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
-            for (int i = 0; i < kE; ++i) buf[d][i] = buf[d + 1][i];
+            for (int i = 0; i < MT * kE; ++i) buf[d][i] = buf[d + 1][i];
     }
-    if (mm) block_minmax_atomic(vmin, vmax, mm);
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
 // Fresnel: the planes already hold the propagated bins.
 template <int IMG>
 __global__ void __launch_bounds__(256)
     k_bins_image(const float2* __restrict__ planes, float* __restrict__ out, size_t P, int K, float count,
-                 uint32_t* mm) {
+                 uint32_t* mm, int mm_slots) {
     float vmin = INFINITY, vmax = -INFINITY;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
         float a = 0.f;
@@ -266,43 +270,43 @@ __global__ void __launch_bounds__(256)
         vmin = fminf(vmin, val);
         vmax = fmaxf(vmax, val);
     }
-    if (mm) block_minmax_atomic(vmin, vmax, mm);
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
 template <int W, int T, typename TIn>
 hv_status rows_launch(const void* raw, float2* planes, const float2* chirp, int height, int slot0, int pmin, int K,
-                      uint32_t* mm_reset, hipStream_t st) {
+                      uint32_t* mm_reset, int mm_slots, hipStream_t st) {
     // the plane rows may take more than the default 64 KiB of dynamic LDS
     static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bins_rows<W, T, TIn>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_bins_rows)");
     const int rpw = HV_BINS_RPW_ROWS;
     k_bins_rows<W, T, TIn><<<(height + rpw - 1) / rpw, kBinThreads, bins_rows_lds<W>(K), st>>>(
-        (const TIn*)raw, planes, chirp, height, slot0, pmin, K, mm_reset);
+        (const TIn*)raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots);
     HV_CHECK_LAUNCH("k_bins_rows");
     return HV_OK;
 }
 
 template <int W, int T>
 hv_status rows_wt(const void* raw, int depth, float2* planes, const float2* chirp, int height, int slot0, int pmin,
-                  int K, uint32_t* mm_reset, hipStream_t st) {
+                  int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
     switch (depth) {
-        case HV_DEPTH_8: return rows_launch<W, T, uint8_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        case HV_DEPTH_16: return rows_launch<W, T, uint16_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        default: return rows_launch<W, T, float>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case HV_DEPTH_8: return rows_launch<W, T, uint8_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case HV_DEPTH_16: return rows_launch<W, T, uint16_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        default: return rows_launch<W, T, float>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
     }
 }
 
 template <int W>
 hv_status rows_w(const void* raw, int depth, float2* planes, const float2* chirp, int height, int T, int slot0,
-                 int pmin, int K, uint32_t* mm_reset, hipStream_t st) {
+                 int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
     switch (T) {
-        case 2: return rows_wt<W, 2>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        case 4: return rows_wt<W, 4>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        case 8: return rows_wt<W, 8>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        case 16: return rows_wt<W, 16>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        case 32: return rows_wt<W, 32>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
-        case 64: return rows_wt<W, 64>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, st);
+        case 2: return rows_wt<W, 2>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 4: return rows_wt<W, 4>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 8: return rows_wt<W, 8>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 16: return rows_wt<W, 16>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 32: return rows_wt<W, 32>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 64: return rows_wt<W, 64>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: T must be a power of two in [2, 64]");
     }
 }
@@ -330,18 +334,19 @@ bool bins_first_supported(int width, int height, int T, int K, int depth) {
 }
 
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
-                           int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset) {
+                           int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
+                           int mm_slots) {
     if (!bins_first_supported(width, height, T, K, depth))
         return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
     if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
         return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
     const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
     switch (width) {
-        case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
-        case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
-        case 256: return rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
-        case 512: return rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
-        default: return rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, st);
+        case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 256: return rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 512: return rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
+        default: return rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
     }
 }
 
@@ -353,30 +358,31 @@ hv_status launch_bins_columns(float2* planes, int width, int height, int K, int 
 
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset) {
-    HV_TRY(launch_bins_rows(raw, depth, width, height, T, slot0, pmin, K, space, lens, planes, st, mm_reset));
+                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots) {
+    HV_TRY(launch_bins_rows(raw, depth, width, height, T, slot0, pmin, K, space, lens, planes, st, mm_reset,
+                            mm_slots));
     if (mark) HV_HIP(hipEventRecord(mark, st));
     return launch_bins_columns(planes, width, height, K, space, lens, st);
 }
 
 template <int IMG>
 hv_status bins_image_t(float* out, const float2* planes, int width, int height, int K, int space, float count,
-                       uint32_t* mm, hipStream_t st) {
+                       uint32_t* mm, int mm_slots, hipStream_t st) {
     if (space == HV_SPACE_FRESNELTR) {
         const size_t P = (size_t)width * height;
         const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
-        k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm);
+        k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm, mm_slots);
         HV_CHECK_LAUNCH("k_bins_image");
         return HV_OK;
     }
     const int rpw = HV_BINS_RPW_IMAGE;
     const dim3 grid((height + rpw - 1) / rpw);
     switch (width) {
-        case 64: k_bins_row_ifft_image<64, IMG><<<grid, img_threads<64>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 128: k_bins_row_ifft_image<128, IMG><<<grid, img_threads<128>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 256: k_bins_row_ifft_image<256, IMG><<<grid, img_threads<256>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 512: k_bins_row_ifft_image<512, IMG><<<grid, img_threads<512>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
-        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, img_threads<1024>(), 0, st>>>(planes, out, height, K, rpw, count, mm); break;
+        case 64: k_bins_row_ifft_image<64, IMG><<<grid, img_threads<64>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
+        case 128: k_bins_row_ifft_image<128, IMG><<<grid, img_threads<128>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
+        case 256: k_bins_row_ifft_image<256, IMG><<<grid, img_threads<256>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
+        case 512: k_bins_row_ifft_image<512, IMG><<<grid, img_threads<512>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
+        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, img_threads<1024>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported width");
     }
     HV_CHECK_LAUNCH("k_bins_row_ifft_image");
@@ -384,13 +390,13 @@ hv_status bins_image_t(float* out, const float2* planes, int width, int height, 
 }
 
 hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st) {
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
     const float count = (float)(pmax - pmin + 1);
     switch (img_type) {
-        case HV_IMG_MODULUS: return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, st);
+        case HV_IMG_MODULUS: return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, st);
         case HV_IMG_SQUARED_MODULUS:
-            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, st);
-        case HV_IMG_ARGUMENT: return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, st);
+            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, st);
+        case HV_IMG_ARGUMENT: return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, mm_slots, st);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: image type must be modulus, squared or argument");
     }
 }

@@ -114,10 +114,22 @@ __device__ __forceinline__ float image_term_fast(float2 z) {
     else return atan2f(z.y, z.x);
 }
 
-// Block min/max of v, then one atomic per block on the ordered encodings in
-// mm[0] (min) and mm[1] (max).  Every thread of the block must call it.
-// Deterministic: min and max do not depend on the order of updates.
-__device__ __forceinline__ void block_minmax_atomic(float vmin, float vmax, uint32_t* mm) {
+// Image min/max accumulation, by the `slots` argument of the producers:
+//   slots >= 1   -- `slots` pairs of ordered encodings, mm[2 s] (min) and
+//                   mm[2 s + 1] (max), updated with atomics (the C-ABI step
+//                   functions use one pair);
+//   kMmPartials  -- no atomics: block b writes its (min, max) as floats at
+//                   ((float2*) mm)[1 + b] and block 0 the block count at mm[0];
+//                   the consumer reduces the partials.  Device-scope atomics
+//                   from hundreds of blocks finishing together serialise at
+//                   the memory side for microseconds; the pipe uses this mode.
+constexpr int kMmPartials = -1;
+
+// Block min/max of v, then its contribution to mm (see above).  Every thread
+// of the block must call it.  Deterministic: the result does not depend on
+// the order of the blocks.  In atomic mode a block first reads its pair and
+// skips an update that cannot change it (a stale read only costs an atomic).
+__device__ __forceinline__ void block_minmax_atomic(float vmin, float vmax, uint32_t* mm, int slots = 1) {
     __shared__ float s_min[16], s_max[16];
     vmin = wave_min(vmin);
     vmax = wave_max(vmax);
@@ -134,8 +146,64 @@ __device__ __forceinline__ void block_minmax_atomic(float vmin, float vmax, uint
             a = fminf(a, s_min[w]);
             b = fmaxf(b, s_max[w]);
         }
-        atomicMin(&mm[0], float_to_ordered(a));
-        atomicMax(&mm[1], float_to_ordered(b));
+        const unsigned bid = blockIdx.x + blockIdx.y * gridDim.x;
+        if (slots == kMmPartials) {
+            reinterpret_cast<float2*>(mm)[1 + bid] = make_float2(a, b);
+            if (bid == 0) mm[0] = gridDim.x * gridDim.y;
+            return;
+        }
+        const uint32_t ea = float_to_ordered(a), eb = float_to_ordered(b);
+        const int slot = slots > 1 ? (int)(bid % (unsigned)slots) : 0;
+        uint32_t* p = mm + 2 * slot;
+        if (ea < __hip_atomic_load(&p[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&p[0], ea);
+        if (eb > __hip_atomic_load(&p[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&p[1], eb);
+    }
+}
+
+// Min and max of the accumulators, for every thread of the block (all must
+// call it: the partials are reduced by the whole block).
+__device__ __forceinline__ void mm_decode_block(const uint32_t* mm, int slots, float& mn, float& mx) {
+    __shared__ float d_min[16], d_max[16];
+    float a = INFINITY, b = -INFINITY;
+    if (slots == kMmPartials) {
+        const unsigned n = mm[0];
+        const float2* parts = reinterpret_cast<const float2*>(mm) + 1;
+        for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+            const float2 v = parts[i];
+            a = fminf(a, v.x);
+            b = fmaxf(b, v.y);
+        }
+    } else {
+        for (int i = threadIdx.x; i < slots; i += blockDim.x) {
+            a = fminf(a, ordered_to_float(mm[2 * i]));
+            b = fmaxf(b, ordered_to_float(mm[2 * i + 1]));
+        }
+    }
+    a = wave_min(a);
+    b = wave_max(b);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nwaves = (blockDim.x + 63) >> 6;
+    if (lane == 0) {
+        d_min[wave] = a;
+        d_max[wave] = b;
+    }
+    __syncthreads();
+    a = d_min[0];
+    b = d_max[0];
+    for (int w = 1; w < nwaves; ++w) {
+        a = fminf(a, d_min[w]);
+        b = fmaxf(b, d_max[w]);
+    }
+    mn = a;
+    mx = b;
+}
+
+// Re-arms atomic accumulator pairs (threads of one block, any count); the
+// partials mode needs no re-arming.
+__device__ __forceinline__ void mm_reset_slots(uint32_t* mm, int slots) {
+    for (int s = threadIdx.x; s < slots; s += blockDim.x) {
+        mm[2 * s] = float_to_ordered(INFINITY);
+        mm[2 * s + 1] = float_to_ordered(-INFINITY);
     }
 }
 

@@ -323,6 +323,69 @@ __device__ __forceinline__ void fft_compute_tw(int t, float2* v, const TW& tw, S
     });
 }
 
+// MT independent transforms of one length on the same thread mapping,
+// interleaved stage by stage: the instruction-level parallelism hides LDS
+// and ALU latency in kernels with few wavefronts per SIMD, and the twiddles
+// are shared.  v[m kE + i] holds transform m; lds(m, n) is transform m's
+// staging slot n; store(m, n, value, slot) receives the outputs.
+template <int N, int E, int DIR, int MT, bool WAVE, class TW, class Store, class Lds>
+__device__ __forceinline__ void fft_compute_multi(int t, float2* v, const TW& tw, Store&& store, Lds&& lds) {
+    static_assert(!WAVE || FftShape<N, E>::NT <= 64, "a wave-local transform has at most 64 threads");
+    using S = FftShape<N, E>;
+    constexpr int kE = S::kE, NT = S::NT, NS = S::stages();
+    static_for<0, NS>([&](auto s_) {
+        constexpr int s = decltype(s_)::value;
+        constexpr int R = S::radix(s), Ns = S::ns(s), NB = kE / R;
+        if constexpr (s > 0) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const int j = t + k * NT;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[m * kE + k * R + r] = lds(m, j + r * (N / R));
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            float2 w[R];
+            if constexpr (Ns > 1) stage_twiddles<N, E, s>(tw, t, k, w);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) {
+                float2* x = v + m * kE + k * R;
+                if constexpr (Ns > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) x[r] = DIR < 0 ? cmul(x[r], w[r]) : cmulc(x[r], w[r]);
+                }
+                dft_reg<R, DIR>(x);
+            }
+        }
+        if constexpr (s == NS - 1) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const int j = t + k * NT;
+                    const int d = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+                    for (int r = 0; r < R; ++r) store(m, d + r * Ns, v[m * kE + k * R + r], k * R + r);
+                }
+        } else {
+            fft_sync<WAVE>();
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const int j = t + k * NT;
+                    const int d = (j / Ns) * Ns * R + (j & (Ns - 1));
+#pragma unroll
+                    for (int r = 0; r < R; ++r) lds(m, d + r * Ns) = v[m * kE + k * R + r];
+                }
+            fft_sync<WAVE>();
+        }
+    });
+}
+
 // Adjoint run: the stages of the DIR_FWD transform applied backwards, each as
 // its adjoint (DFT of the opposite direction, then conjugate twiddles,
 // exchanges with read and write positions swapped).  Input: v in the slot

@@ -27,7 +27,7 @@ hv_status launch_col_pass(float2* buf, const float2* lens, int width, int height
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
                                    hipEvent_t* marks = nullptr, bool spectral_rows = false,
-                                   uint32_t* mm_reset = nullptr);
+                                   uint32_t* mm_reset = nullptr, int mm_slots = 1);
 
 // hv_time.hip
 hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_t st);
@@ -35,15 +35,15 @@ hv_status launch_complex_to_image(float* out, const float2* in, int img_type, in
                                   hipStream_t st);
 // Fused STFT + image type.  mm: encoded min/max (may be null).
 hv_status launch_stft_image(float* out, const float2* ring, int T, size_t P, int img_type, int pmin, int pmax,
-                            uint32_t* mm, hipStream_t st);
+                            uint32_t* mm, hipStream_t st, int mm_slots = 1);
 // Fused inverse row FFT + STFT + image type for a window whose T frames hold
 // the angular spectrum after its column pass (B = S = T); HV_ERR_UNSUPPORTED
 // when (width, T) has no fused kernel (its LDS tile would not fit).
 bool row_ifft_stft_supported(int width, int T);
 hv_status launch_row_ifft_stft_image(float* out, float2* ring, int width, int height, int T, int img_type, int pmin,
-                                     int pmax, uint32_t* mm, hipStream_t st);
+                                     int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1);
 hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* ring, int T, size_t P, int f_start,
-                              int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st);
+                              int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st, int mm_slots = 1);
 
 // hv_bins.hip: B = S = T STFT images computed bins-first (the STFT bins
 // [pmin, pmin + K) of the raw window are propagated instead of its T frames).
@@ -52,25 +52,28 @@ bool bins_first_supported(int width, int height, int T, int K, int depth);
 // then the column pass over them; launch_bins_propagate runs both, with
 // `mark` (optional) recorded in between.
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
-                           int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset);
+                           int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
+                           int mm_slots = 1);
 hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
                               hipStream_t st);
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset);
+                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots = 1);
 // Image of the propagated planes: inverse row FFT (angular spectrum) + image
 // term, mean over [pmin, pmax]; mm: encoded min/max (may be null).
 hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st);
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1);
 
 // hv_render.hip
-hv_status launch_minmax_init(uint32_t* mm, hipStream_t st);
-hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st);
-hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st);
+// mm: `slots` accumulator pairs (block_minmax_atomic); the C-ABI uses one.
+hv_status launch_minmax_init(uint32_t* mm, hipStream_t st, int slots = 1);
+hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st, int slots = 1);
+hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st, int slots = 1);
 hv_status launch_subsample_f32(float* out, const float* in, int width, int height, int step, hipStream_t st);
 hv_status launch_apply_contrast(float* img, size_t n, float mn, float mx, hipStream_t st);
 hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st);
-// range: 2 values, floats (encoded = 0) or ordered-uint min/max (encoded = 1).
+// range: 2 floats (encoded = 0) or `encoded` pairs of ordered-uint min/max
+// accumulators (encoded >= 1).
 // range_out (optional): the decoded range actually applied (before invert).
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
                                     int encoded, int invert, hipStream_t st, float* range_out = nullptr);
@@ -80,7 +83,7 @@ size_t percentile_scratch_bytes();
 // subsample), optionally clamped to >= 1.
 hv_status launch_percentile(float* range, const float* img, int width, int height, int step, float lo, float hi,
                             int clamp, void* scratch, hipStream_t st);
-hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st);
+hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots = 1);
 hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st);
 size_t unwrap_scratch_bytes(int width, int height);
 
@@ -91,6 +94,6 @@ hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int p
 size_t eig_workspace_bytes(int T);
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st);
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
-                           int pmin, int pmax, uint32_t* mm, hipStream_t st);
+                           int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1);
 
 }  // namespace hv

@@ -552,7 +552,7 @@ __device__ __forceinline__ float pca_term(float2 z, int img_type) {
 template <int T>
 __global__ void __launch_bounds__(256)
     k_pca_image(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs, size_t P,
-                int img_type, int pmin, int pmax, uint32_t* mm) {
+                int img_type, int pmin, int pmax, uint32_t* mm, int mm_slots) {
     extern __shared__ float2 vs[];  // T x nsel, s fastest
     const int nsel = pmax - pmin + 1;
     for (int e = threadIdx.x; e < T * nsel; e += blockDim.x) {
@@ -584,13 +584,13 @@ __global__ void __launch_bounds__(256)
         out[p] = val;
         vmin = vmax = val;
     }
-    if (mm) block_minmax_atomic(vmin, vmax, mm);
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
 // T > 64: the pixel's window does not fit in registers; stream it per column.
 __global__ void __launch_bounds__(256)
     k_pca_image_big(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs, int T,
-                    size_t P, int img_type, int pmin, int pmax, uint32_t* mm) {
+                    size_t P, int img_type, int pmin, int pmax, uint32_t* mm, int mm_slots) {
     extern __shared__ float2 vs[];
     const int nsel = pmax - pmin + 1;
     for (int e = threadIdx.x; e < T * nsel; e += blockDim.x) {
@@ -619,23 +619,23 @@ __global__ void __launch_bounds__(256)
         out[p] = val;
         vmin = vmax = val;
     }
-    if (mm) block_minmax_atomic(vmin, vmax, mm);
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
-                           int pmin, int pmax, uint32_t* mm, hipStream_t st) {
+                           int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
     const unsigned g = (unsigned)((P + 255) / 256);
     const size_t shm = (size_t)T * (pmax - pmin + 1) * sizeof(float2);
     if (shm > 64 * 1024) return fail(HV_ERR_UNSUPPORTED, "PCA eigenvector range too large");
     switch (T) {
-        case 1: k_pca_image<1><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        case 2: k_pca_image<2><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        case 4: k_pca_image<4><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        case 8: k_pca_image<8><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        case 16: k_pca_image<16><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        case 32: k_pca_image<32><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        case 64: k_pca_image<64><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm); break;
-        default: k_pca_image_big<<<g, 256, shm, st>>>(out, ring, evecs, T, P, img_type, pmin, pmax, mm); break;
+        case 1: k_pca_image<1><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        case 2: k_pca_image<2><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        case 4: k_pca_image<4><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        case 8: k_pca_image<8><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        case 16: k_pca_image<16><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        case 32: k_pca_image<32><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        case 64: k_pca_image<64><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
+        default: k_pca_image_big<<<g, 256, shm, st>>>(out, ring, evecs, T, P, img_type, pmin, pmax, mm, mm_slots); break;
     }
     HV_CHECK_LAUNCH("k_pca_image");
     return HV_OK;

@@ -69,7 +69,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 }
 
 struct Layout {
-    size_t ring, lens, staging, image, cimage, tmp, out, small, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
+    size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
     size_t total;
 };
 
@@ -85,7 +85,10 @@ Layout layout_of(const hv_compute_settings& s) {
     const bool unwrap = s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d;
     l.tmp = unwrap ? align256(P * sizeof(float)) : 0;
     l.out = align256(P * sizeof(uint16_t));
-    l.small = 256;  // min/max (2 x uint32) + range (2 floats)
+    l.small = 256;   // contrast range (2 floats)
+    // per-block image min/max partials: a header pair + one float pair per
+    // block of the widest image kernel (P / 64 blocks)
+    l.mm = align256(sizeof(float2) * (1 + (P + 63) / 64 + 4096));
     l.pct = align256(percentile_scratch_bytes());
     l.unwrap = unwrap ? align256(unwrap_scratch_bytes(s.width, s.height)) : 0;
     if (s.time_transformation == HV_TIME_PCA) {
@@ -95,7 +98,7 @@ Layout layout_of(const hv_compute_settings& s) {
         l.pca_ws = align256(pca_workspace_bytes((int)T, P, s.pca_precision));
         l.eig_ws = align256(eig_workspace_bytes((int)T));
     }
-    l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.pct + l.unwrap + l.cov +
+    l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
               l.evals + l.evecs + l.pca_ws + l.eig_ws;
     return l;
 }
@@ -207,23 +210,23 @@ hv_status hv_pipe::image_work() {
         p_range(s, pmin, pmax);
         if (path == PATH_BINS) {
             HV_TRY(launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, pmax - pmin + 1,
-                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream));
+                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kMmPartials));
         } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
             HV_TRY(launch_row_ifft_stft_image(L.image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
-                                              L.stream));
+                                              L.stream, kMmPartials));
         } else if (s.time_transformation == HV_TIME_STFT) {
-            HV_TRY(launch_stft_image(L.image, ring, Tn, P, it, pmin, pmax, mm_img, L.stream));
+            HV_TRY(launch_stft_image(L.image, ring, Tn, P, it, pmin, pmax, mm_img, L.stream, kMmPartials));
         } else if (s.time_transformation == HV_TIME_PCA) {
             HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, L.stream));
             HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, L.stream));
-            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, L.stream));
+            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, L.stream, kMmPartials));
         } else {
             HV_TRY(launch_complex_to_image(L.image, ring, it, pmin, pmax, P, L.stream));
-            if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream));
+            if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kMmPartials));
         }
         if (unwrap) {
             HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream));
-            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream));
+            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kMmPartials));
         }
     } else {
         // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
@@ -233,7 +236,7 @@ hv_status hv_pipe::image_work() {
         float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
         float* m2 = it == HV_IMG_MOMENTS_2 ? L.image : nullptr;
         HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
-                                   it - HV_IMG_MOMENTS_0, L.stream));
+                                   it - HV_IMG_MOMENTS_0, L.stream, kMmPartials));
     }
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
     HV_TRY(mark(4));
@@ -242,11 +245,12 @@ hv_status hv_pipe::image_work() {
         case HV_CONTRAST_MINMAX:
             if (!minmax_fused()) {
                 // min/max of the final image (unwrap rescales it; no first-kernel reset without a space transform)
-                HV_TRY(launch_minmax_init(L.mm, L.stream));
-                HV_TRY(launch_minmax(L.mm, L.image, P, L.stream));
+                HV_TRY(launch_minmax_init(L.mm, L.stream, kMmPartials));
+                HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kMmPartials));
             }
             // the contrast kernel decodes the accumulators and records the range it applied
-            return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.mm, 1, s.contrast_invert, L.stream, L.range);
+            return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.mm, kMmPartials, s.contrast_invert, L.stream,
+                                             L.range);
         case HV_CONTRAST_PERCENTILE:
             HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
                                      s.contrast_upper, 1, L.pct, L.stream));
@@ -270,13 +274,13 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
         p_range(s, pmin, pmax);
         HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, pmin,
                                      pmax - pmin + 1, s.space_transformation, lens, L.planes, L.stream,
-                                     timing ? ev[1] : nullptr, minmax_fused() ? L.mm : nullptr));
+                                     timing ? ev[1] : nullptr, minmax_fused() ? L.mm : nullptr, kMmPartials));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
                                         s.space_transformation, lens, ring, T, slot0, L.stream,
                                         timing ? &ev[1] : nullptr, path == PATH_ROW_STFT,
-                                        minmax_fused() ? L.mm : nullptr));
+                                        minmax_fused() ? L.mm : nullptr, kMmPartials));
     }
     HV_TRY(mark(3));
     HV_TRY(image_work());
@@ -414,9 +418,8 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     L0.cimage = (float*)take(l.cimage);
     L0.tmp = (float*)take(l.tmp);
     L0.out = (uint16_t*)take(l.out);
-    char* small = take(l.small);
-    L0.mm = (uint32_t*)small;
-    L0.range = (float*)(small + 16);
+    L0.range = (float*)take(l.small);
+    L0.mm = (uint32_t*)take(l.mm);
     L0.pct = take(l.pct);
     L0.unwrap_ws = take(l.unwrap);
     p->cov = (float2*)take(l.cov);
@@ -623,7 +626,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     for (int i = (int)p->lane.size(); i < lanes; ++i) {
         Lane n;
         const size_t planes = align256(K * p->P * sizeof(float2));
-        const size_t total = planes + l.image + l.cimage + l.tmp + l.out + l.small + l.pct + l.unwrap;
+        const size_t total = planes + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap;
         hipError_t e = hipMalloc(&n.mem, total);
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(lane)");
         e = hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking);
@@ -642,9 +645,8 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         n.cimage = (float*)take(l.cimage);
         n.tmp = (float*)take(l.tmp);
         n.out = (uint16_t*)take(l.out);
-        char* small = take(l.small);
-        n.mm = (uint32_t*)small;
-        n.range = (float*)(small + 16);
+        n.range = (float*)take(l.small);
+        n.mm = (uint32_t*)take(l.mm);
         n.pct = take(l.pct);
         n.unwrap_ws = take(l.unwrap);
         p->lane.push_back(n);
@@ -679,7 +681,7 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
         switch (stage) {
             case 0:
                 st = launch_bins_rows(frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
-                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm);
+                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kMmPartials);
                 break;
             case 1:
                 st = launch_bins_columns(L.planes, (int)s.width, (int)s.height, K, s.space_transformation, p->lens,
@@ -687,10 +689,10 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
                 break;
             case 3:
                 st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
-                                       s.img_type, pmin, pmax, L.mm, L.stream);
+                                       s.img_type, pmin, pmax, L.mm, L.stream, kMmPartials);
                 break;
             default:
-                st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, 1, s.contrast_invert, L.stream,
+                st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, kMmPartials, s.contrast_invert, L.stream,
                                                L.range);
                 break;
         }

@@ -24,43 +24,39 @@ static unsigned blocks_for(size_t n, int threads, size_t cap = 16384) {
 // ---------------------------------------------------------------------------
 // min / max
 // ---------------------------------------------------------------------------
-__global__ void k_minmax_init(uint32_t* mm) {
-    if (threadIdx.x == 0) {
-        mm[0] = float_to_ordered(INFINITY);
-        mm[1] = float_to_ordered(-INFINITY);
-    }
-}
+__global__ void k_minmax_init(uint32_t* mm, int slots) { mm_reset_slots(mm, slots); }
 
-__global__ void __launch_bounds__(256) k_minmax(uint32_t* mm, const float* __restrict__ in, size_t n) {
+__global__ void __launch_bounds__(256) k_minmax(uint32_t* mm, const float* __restrict__ in, size_t n, int slots) {
     float a = INFINITY, b = -INFINITY;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float v = in[i];
         a = fminf(a, v);
         b = fmaxf(b, v);
     }
-    block_minmax_atomic(a, b, mm);
+    block_minmax_atomic(a, b, mm, slots);
 }
 
-__global__ void k_minmax_decode(float* out, const uint32_t* mm) {
+__global__ void k_minmax_decode(float* out, const uint32_t* mm, int slots) {
+    float a, b;
+    mm_decode_block(mm, slots, a, b);
     if (threadIdx.x == 0) {
-        const float a = ordered_to_float(mm[0]), b = ordered_to_float(mm[1]);
         out[0] = a;
         out[1] = b;
     }
 }
 
-hv_status launch_minmax_init(uint32_t* mm, hipStream_t st) {
-    k_minmax_init<<<1, 64, 0, st>>>(mm);
+hv_status launch_minmax_init(uint32_t* mm, hipStream_t st, int slots) {
+    k_minmax_init<<<1, 64, 0, st>>>(mm, slots);
     HV_CHECK_LAUNCH("k_minmax_init");
     return HV_OK;
 }
-hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st) {
-    k_minmax<<<blocks_for(n, 256, 2048), 256, 0, st>>>(mm, in, n);
+hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st, int slots) {
+    k_minmax<<<blocks_for(n, 256, 2048), 256, 0, st>>>(mm, in, n, slots);
     HV_CHECK_LAUNCH("k_minmax");
     return HV_OK;
 }
-hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st) {
-    k_minmax_decode<<<1, 64, 0, st>>>(out, mm);
+hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st, int slots) {
+    k_minmax_decode<<<1, 64, 0, st>>>(out, mm, slots);
     HV_CHECK_LAUNCH("k_minmax_decode");
     return HV_OK;
 }
@@ -113,8 +109,7 @@ __global__ void __launch_bounds__(256)
                    const void* range, int encoded, int invert, float* range_out) {
     float mn, mx;
     if (encoded) {
-        mn = ordered_to_float(((const uint32_t*)range)[0]);
-        mx = ordered_to_float(((const uint32_t*)range)[1]);
+        mm_decode_block((const uint32_t*)range, encoded, mn, mx);
     } else {
         mn = ((const float*)range)[0];
         mx = ((const float*)range)[1];
@@ -308,8 +303,10 @@ hv_status launch_percentile(float* range, const float* img, int width, int heigh
 // ---------------------------------------------------------------------------
 // rescale_float_unwrap2d (tools_conversion.cu:315-335), device min/max
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float* in, size_t n, const uint32_t* mm) {
-    const float mn = ordered_to_float(mm[0]), mx = ordered_to_float(mm[1]);
+__global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float* in, size_t n, const uint32_t* mm,
+                                                        int slots) {
+    float mn, mx;
+    mm_decode_block(mm, slots, mn, mx);
     const float m = (unsigned short)65535;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float v = in[i];
@@ -317,10 +314,10 @@ __global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float*
     }
 }
 
-hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st) {
-    HV_TRY(launch_minmax_init(mm, st));
-    HV_TRY(launch_minmax(mm, in, n, st));
-    k_rescale_unwrap<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n, mm);
+hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots) {
+    HV_TRY(launch_minmax_init(mm, st, slots));
+    HV_TRY(launch_minmax(mm, in, n, st, slots));
+    k_rescale_unwrap<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n, mm, slots);
     HV_CHECK_LAUNCH("k_rescale_unwrap");
     return HV_OK;
 }

@@ -217,15 +217,12 @@ struct FrameMap {
 template <int N, int DIR, int KIN>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     k_row_fft(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
-              FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int iters) {
+              FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int mm_slots, int iters) {
     using C = RowCfg<N>;
     constexpr int kE = FftShape<N, C::E>::kE;
     __shared__ float2 lds[C::M * C::PITCH];
     // the cycle's first kernel re-arms the image min/max accumulators of its last kernels
-    if (mm_reset && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        mm_reset[0] = float_to_ordered(INFINITY);
-        mm_reset[1] = float_to_ordered(-INFINITY);
-    }
+    if (mm_reset && blockIdx.x == 0 && blockIdx.y == 0) mm_reset_slots(mm_reset, mm_slots);
     const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
     const int b = blockIdx.y;
     const size_t P = (size_t)N * height;
@@ -337,7 +334,8 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
 // ---------------------------------------------------------------------------
 template <int N, int DIR>
 static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* chirp, int height, int batch,
-                            FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset) {
+                            FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset,
+                            int mm_slots) {
     using C = RowCfg<N>;
     // enough workgroups to fill the chip, each with a few row groups to pipeline
     const int groups = (height + C::M - 1) / C::M;
@@ -345,10 +343,10 @@ static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* 
     while (iters < 8 && groups % (iters * 2) == 0 && (size_t)(groups / (iters * 2)) * batch >= HV_ROW_MIN_WG) iters *= 2;
     dim3 grid(groups / iters, batch);
     switch (kin) {
-        case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
-        case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
-        case IN_F32: k_row_fft<N, DIR, IN_F32><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
-        default: k_row_fft<N, DIR, IN_C64><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, iters); break;
+        case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
+        case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
+        case IN_F32: k_row_fft<N, DIR, IN_F32><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
+        default: k_row_fft<N, DIR, IN_C64><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
     }
     HV_CHECK_LAUNCH("k_row_fft");
     return HV_OK;
@@ -357,17 +355,17 @@ static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* 
 template <int DIR>
 static hv_status row_pass(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
                           int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st,
-                          uint32_t* mm_reset = nullptr) {
+                          uint32_t* mm_reset = nullptr, int mm_slots = 1) {
     switch (width) {
-        case 16: return row_pass_n<16, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 32: return row_pass_n<32, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 64: return row_pass_n<64, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 128: return row_pass_n<128, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 256: return row_pass_n<256, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 512: return row_pass_n<512, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 1024: return row_pass_n<1024, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 2048: return row_pass_n<2048, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
-        case 4096: return row_pass_n<4096, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset);
+        case 16: return row_pass_n<16, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 32: return row_pass_n<32, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 64: return row_pass_n<64, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 128: return row_pass_n<128, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 256: return row_pass_n<256, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 512: return row_pass_n<512, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 1024: return row_pass_n<1024, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 2048: return row_pass_n<2048, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
+        case 4096: return row_pass_n<4096, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
         default: return fail(HV_ERR_UNSUPPORTED, "width must be a power of two in [16, 4096]");
     }
 }
@@ -444,7 +442,7 @@ hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width,
 
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
-                                   hipEvent_t* marks, bool spectral_rows, uint32_t* mm_reset) {
+                                   hipEvent_t* marks, bool spectral_rows, uint32_t* mm_reset, int mm_slots) {
     // marks (optional): events recorded after pass 1 and after pass 2.
     // spectral_rows: stop the angular spectrum before its inverse row pass
     // (the fused row-IFFT + STFT kernel finishes it).
@@ -460,7 +458,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
     const FrameMap slots{ring_size, (slot0 + skip) % ring_size};
     if (space == HV_SPACE_ANGULARSP) {
         // pass 1 writes the frames into their ring slots; passes 2-3 run in place there
-        HV_TRY(row_pass<-1>(src, kin, ring, nullptr, width, height, nb, plain, slots, 1.0f, st, mm_reset));
+        HV_TRY(row_pass<-1>(src, kin, ring, nullptr, width, height, nb, plain, slots, 1.0f, st, mm_reset, mm_slots));
         if (marks) HV_HIP(hipEventRecord(marks[0], st));
         HV_TRY(col_pass<COL_LENS>(ring, ring, lens, width, height, nb, slots, slots, st));
         if (marks) HV_HIP(hipEventRecord(marks[1], st));
@@ -468,7 +466,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
         return row_pass<+1>(ring, IN_C64, ring, nullptr, width, height, nb, slots, slots, 1.0f, st);
     }
     if (space == HV_SPACE_FRESNELTR) {
-        HV_TRY(row_pass<-1>(src, kin, ring, lens, width, height, nb, plain, slots, 1.0f, st, mm_reset));
+        HV_TRY(row_pass<-1>(src, kin, ring, lens, width, height, nb, plain, slots, 1.0f, st, mm_reset, mm_slots));
         if (marks) {
             HV_HIP(hipEventRecord(marks[0], st));
             HV_HIP(hipEventRecord(marks[1], st));
@@ -476,7 +474,7 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
         return col_pass<COL_FWD>(ring, ring, nullptr, width, height, nb, slots, slots, st);
     }
     // no space transform: conversion only, frame by frame into the slots
-    if (mm_reset) HV_TRY(launch_minmax_init(mm_reset, st));
+    if (mm_reset) HV_TRY(launch_minmax_init(mm_reset, st, mm_slots));
     for (int b = 0; b < nb; ++b) {
         float2* dst = ring + (size_t)((slots.slot0 + b) % ring_size) * P;
         HV_TRY(launch_convert(dst, src + (size_t)b * P * depth, P, depth, st));

@@ -28,6 +28,7 @@ struct TimeParams {
     float fps;         // moments frequency scale (CameraFps)
     uint32_t* mm;      // encoded min/max of the image, may be null
     int mm_moment;     // which moment feeds mm (0, 1, 2)
+    int mm_slots;      // accumulation mode of mm (block_minmax_atomic; 0 counts as 1)
 };
 
 // fft_freqs (icompute.cc:26-69): f1[i] = i d for i < T/2, -(T - i) d above
@@ -110,7 +111,7 @@ __global__ void __launch_bounds__(256) k_time_reg(const float2* __restrict__ rin
             vmin = vmax = sel;
         }
     }
-    if (MODE != TM_FULL && prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
+    if (MODE != TM_FULL && prm.mm) block_minmax_atomic(vmin, vmax, prm.mm, prm.mm_slots != 0 ? prm.mm_slots : 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -161,7 +162,7 @@ __global__ void __launch_bounds__(TimeLdsCfg<T>::THREADS)
                 vmin = vmax = sel;
             }
         }
-        if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
+        if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm, prm.mm_slots != 0 ? prm.mm_slots : 1);
     }
 }
 
@@ -202,18 +203,19 @@ hv_status launch_stft(float2* out, const float2* in, int T, size_t P, hipStream_
 }
 
 hv_status launch_stft_image(float* out, const float2* ring, int T, size_t P, int img_type, int pmin, int pmax,
-                            uint32_t* mm, hipStream_t st) {
+                            uint32_t* mm, hipStream_t st, int mm_slots) {
     TimeParams prm{};
     prm.out = out;
     prm.img_type = img_type;
     prm.pmin = pmin;
     prm.pmax = pmax;
     prm.mm = mm;
+    prm.mm_slots = mm_slots;
     return time_launch<TM_IMAGE>(ring, T, P, prm, st);
 }
 
 hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* ring, int T, size_t P, int f_start,
-                              int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st) {
+                              int f_end, float fps, uint32_t* mm, int mm_moment, hipStream_t st, int mm_slots) {
     TimeParams prm{};
     prm.out = m0;
     prm.m1 = m1;
@@ -222,6 +224,7 @@ hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* rin
     prm.pmax = f_end;
     prm.fps = fps;
     prm.mm = mm;
+    prm.mm_slots = mm_slots;
     prm.mm_moment = mm_moment;
     return time_launch<TM_MOMENTS>(ring, T, P, prm, st);
 }
@@ -298,7 +301,7 @@ __global__ void __launch_bounds__(512)
         vmin = fminf(vmin, val);
         vmax = fmaxf(vmax, val);
     }
-    if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm);
+    if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm, prm.mm_slots != 0 ? prm.mm_slots : 1);
 }
 
 
@@ -344,13 +347,14 @@ bool row_ifft_stft_supported(int width, int T) {
 }
 
 hv_status launch_row_ifft_stft_image(float* out, float2* ring, int width, int height, int T, int img_type, int pmin,
-                                     int pmax, uint32_t* mm, hipStream_t st) {
+                                     int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
     TimeParams prm{};
     prm.out = out;
     prm.img_type = img_type;
     prm.pmin = pmin;
     prm.pmax = pmax;
     prm.mm = mm;
+    prm.mm_slots = mm_slots;
     return row_stft_dispatch(out, ring, width, height, T, prm, st, false);
 }
 
