@@ -35,14 +35,14 @@ namespace {
 #define HV_BINS_RPW_ROWS 1
 #endif
 #ifndef HV_BINS_RPW_IMAGE
-#define HV_BINS_RPW_IMAGE 2
+#define HV_BINS_RPW_IMAGE 1
 #endif
 
 #ifndef HV_BINS_IMG_THREADS
 #define HV_BINS_IMG_THREADS 512
 #endif
 #ifndef HV_BINS_IMG_MT
-#define HV_BINS_IMG_MT 2        // plane rows each wavefront transforms side by side in the image kernel
+#define HV_BINS_IMG_MT 1        // plane rows each wavefront transforms side by side in the image kernel
 #endif
 #ifndef HV_BINS_IMG_DEPTH
 #define HV_BINS_IMG_DEPTH 1     // plane rows loaded ahead by the image kernel

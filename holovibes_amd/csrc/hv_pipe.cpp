@@ -108,6 +108,13 @@ Layout layout_of(const hv_compute_settings& s) {
 constexpr int kStages = 5;
 constexpr size_t kMaxGraphs = 64;
 
+// Image min/max accumulation of a cycle's producers: per-block partials, or
+// (measurement builds, -DHV_PIPE_MM_MODE=n) n atomic accumulator pairs.
+#ifndef HV_PIPE_MM_MODE
+#define HV_PIPE_MM_MODE kMmPartials
+#endif
+constexpr int kPipeMm = HV_PIPE_MM_MODE;
+
 // Buffers of one in-flight cycle.  Lane 0 lives in the pipe's main
 // allocation; further lanes (bins-first path only) have their own.
 struct Lane {
@@ -210,23 +217,23 @@ hv_status hv_pipe::image_work() {
         p_range(s, pmin, pmax);
         if (path == PATH_BINS) {
             HV_TRY(launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, pmax - pmin + 1,
-                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kMmPartials));
+                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
             HV_TRY(launch_row_ifft_stft_image(L.image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
-                                              L.stream, kMmPartials));
+                                              L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_STFT) {
-            HV_TRY(launch_stft_image(L.image, ring, Tn, P, it, pmin, pmax, mm_img, L.stream, kMmPartials));
+            HV_TRY(launch_stft_image(L.image, ring, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_PCA) {
             HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, L.stream));
             HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, L.stream));
-            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, L.stream, kMmPartials));
+            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else {
             HV_TRY(launch_complex_to_image(L.image, ring, it, pmin, pmax, P, L.stream));
-            if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kMmPartials));
+            if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kPipeMm));
         }
         if (unwrap) {
             HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream));
-            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kMmPartials));
+            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kPipeMm));
         }
     } else {
         // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
@@ -236,7 +243,7 @@ hv_status hv_pipe::image_work() {
         float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
         float* m2 = it == HV_IMG_MOMENTS_2 ? L.image : nullptr;
         HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
-                                   it - HV_IMG_MOMENTS_0, L.stream, kMmPartials));
+                                   it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
     }
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
     HV_TRY(mark(4));
@@ -245,11 +252,11 @@ hv_status hv_pipe::image_work() {
         case HV_CONTRAST_MINMAX:
             if (!minmax_fused()) {
                 // min/max of the final image (unwrap rescales it; no first-kernel reset without a space transform)
-                HV_TRY(launch_minmax_init(L.mm, L.stream, kMmPartials));
-                HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kMmPartials));
+                HV_TRY(launch_minmax_init(L.mm, L.stream, kPipeMm));
+                HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kPipeMm));
             }
             // the contrast kernel decodes the accumulators and records the range it applied
-            return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.mm, kMmPartials, s.contrast_invert, L.stream,
+            return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.mm, kPipeMm, s.contrast_invert, L.stream,
                                              L.range);
         case HV_CONTRAST_PERCENTILE:
             HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
@@ -274,13 +281,13 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
         p_range(s, pmin, pmax);
         HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, pmin,
                                      pmax - pmin + 1, s.space_transformation, lens, L.planes, L.stream,
-                                     timing ? ev[1] : nullptr, minmax_fused() ? L.mm : nullptr, kMmPartials));
+                                     timing ? ev[1] : nullptr, minmax_fused() ? L.mm : nullptr, kPipeMm));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
                                         s.space_transformation, lens, ring, T, slot0, L.stream,
                                         timing ? &ev[1] : nullptr, path == PATH_ROW_STFT,
-                                        minmax_fused() ? L.mm : nullptr, kMmPartials));
+                                        minmax_fused() ? L.mm : nullptr, kPipeMm));
     }
     HV_TRY(mark(3));
     HV_TRY(image_work());
@@ -681,7 +688,7 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
         switch (stage) {
             case 0:
                 st = launch_bins_rows(frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
-                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kMmPartials);
+                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kPipeMm);
                 break;
             case 1:
                 st = launch_bins_columns(L.planes, (int)s.width, (int)s.height, K, s.space_transformation, p->lens,
@@ -689,10 +696,10 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
                 break;
             case 3:
                 st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
-                                       s.img_type, pmin, pmax, L.mm, L.stream, kMmPartials);
+                                       s.img_type, pmin, pmax, L.mm, L.stream, kPipeMm);
                 break;
             default:
-                st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, kMmPartials, s.contrast_invert, L.stream,
+                st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, kPipeMm, s.contrast_invert, L.stream,
                                                L.range);
                 break;
         }
