@@ -116,6 +116,30 @@ struct FftShape {
     }
 };
 
+// f(i, n) for every value slot i of thread t and the input index n that the
+// first stage expects there: slot k R + r holds x[t + k NT + r N / R].
+template <int N, int E, class F>
+__device__ __forceinline__ void fft_load_slots(int t, F&& f) {
+    using S = FftShape<N, E>;
+    constexpr int kE = S::kE, NT = S::NT, R = S::radix(0), NB = kE / R;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int j = t + k * NT;
+#pragma unroll
+        for (int r = 0; r < R; ++r) f(k * R + r, j + r * (N / R));
+    }
+}
+
+// Stage-0 inputs of thread t.  Split from fft_compute so a kernel can issue
+// the loads of its next transform before computing the current one.
+template <int N, int E, class Load>
+__device__ __forceinline__ void fft_load(int t, Load&& load, float2* v) {
+    fft_load_slots<N, E>(t, [&](int i, int n) { v[i] = load(n); });
+}
+
+template <int N, int E, int DIR, bool STORE_SYNC = false, class Store, class Lds>
+__device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds&& lds);
+
 // Runs one length-N transform on thread t (0 <= t < NT) of its group.
 //   load(n)      -> float2 : element n of the input (stage 0)
 //   store(n, v)            : element n of the output (last stage)
@@ -123,24 +147,6 @@ struct FftShape {
 // Every thread of the workgroup must call fft_run the same number of times
 // (it contains __syncthreads()).  STORE_SYNC puts a barrier before the last
 // stage's stores, for storers that write into the same LDS staging area.
-// Stage-0 inputs of thread t: v[k R + r] = x[t + k NT + r N / R].  Split
-// from fft_compute so a kernel can issue the loads of its next transform
-// before computing the current one.
-template <int N, int E, class Load>
-__device__ __forceinline__ void fft_load(int t, Load&& load, float2* v) {
-    using S = FftShape<N, E>;
-    constexpr int kE = S::kE, NT = S::NT, R = S::radix(0), NB = kE / R;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const int j = t + k * NT;
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[k * R + r] = load(j + r * (N / R));
-    }
-}
-
-template <int N, int E, int DIR, bool STORE_SYNC = false, class Store, class Lds>
-__device__ __forceinline__ void fft_compute(int t, float2* v, Store&& store, Lds&& lds);
-
 template <int N, int E, int DIR, bool STORE_SYNC = false, class Load, class Store, class Lds>
 __device__ __forceinline__ void fft_run(int t, Load&& load, Store&& store, Lds&& lds) {
     float2 v[FftShape<N, E>::kE];

@@ -45,6 +45,27 @@ __global__ void k_minmax_decode(float* out, const uint32_t* mm, int slots) {
     }
 }
 
+// Reduces per-block partials in place to one entry (count 1).  Consumers
+// with many blocks each reduce every partial; past a few hundred consumer
+// blocks one reduction up front is cheaper than that quadratic read traffic.
+__global__ void __launch_bounds__(1024) k_mm_collapse(uint32_t* mm) {
+    float a, b;
+    mm_decode_block(mm, kMmPartials, a, b);  // ends with a barrier after every read of the partials
+    if (threadIdx.x == 0) {
+        reinterpret_cast<float2*>(mm)[1] = make_float2(a, b);
+        mm[0] = 1;
+    }
+}
+
+constexpr unsigned kDecodeBlocksDirect = 256;
+
+static hv_status collapse_partials(uint32_t* mm, unsigned consumer_blocks, int slots, hipStream_t st) {
+    if (slots != kMmPartials || consumer_blocks <= kDecodeBlocksDirect) return HV_OK;
+    k_mm_collapse<<<1, 1024, 0, st>>>(mm);
+    HV_CHECK_LAUNCH("k_mm_collapse");
+    return HV_OK;
+}
+
 hv_status launch_minmax_init(uint32_t* mm, hipStream_t st, int slots) {
     k_minmax_init<<<1, 64, 0, st>>>(mm, slots);
     HV_CHECK_LAUNCH("k_minmax_init");
@@ -164,8 +185,9 @@ hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipSt
 }
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
                                     int encoded, int invert, hipStream_t st, float* range_out) {
-    k_contrast_u16<<<blocks_for((n + 3) / 4, 256), 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert,
-                                                                   range_out);
+    const unsigned blocks = blocks_for((n + 3) / 4, 256);
+    if (encoded == kMmPartials) HV_TRY(collapse_partials((uint32_t*)range, blocks, encoded, st));
+    k_contrast_u16<<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out);
     HV_CHECK_LAUNCH("k_contrast_u16");
     return HV_OK;
 }
@@ -317,7 +339,9 @@ __global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float*
 hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots) {
     HV_TRY(launch_minmax_init(mm, st, slots));
     HV_TRY(launch_minmax(mm, in, n, st, slots));
-    k_rescale_unwrap<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n, mm, slots);
+    const unsigned blocks = blocks_for(n, 256);
+    HV_TRY(collapse_partials(mm, blocks, slots, st));
+    k_rescale_unwrap<<<blocks, 256, 0, st>>>(out, in, n, mm, slots);
     HV_CHECK_LAUNCH("k_rescale_unwrap");
     return HV_OK;
 }

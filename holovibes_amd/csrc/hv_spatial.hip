@@ -12,6 +12,8 @@
 //   3. inverse row FFT, in place in the ring slot.
 // The Fresnel transform (fresnel_transform.cu:65-83) is two: the chirp is
 // applied in the row pass's loader.  No pass needs a separate buffer.
+#include <type_traits>
+
 #include "hv_fft.hpp"
 #include "hv_kernels.hpp"
 
@@ -182,6 +184,9 @@ enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 #ifndef HV_COL_CW_SMALL
 #define HV_COL_CW_SMALL 16      // columns per workgroup, column lengths <= 512
 #endif
+#ifndef HV_COL_E_1024
+#define HV_COL_E_1024 16        // values per thread, column length 1024
+#endif
 #ifndef HV_COL_CW_1024
 #define HV_COL_CW_1024 8        // columns per workgroup, column length 1024
 #endif
@@ -211,10 +216,31 @@ struct FrameMap {
     }
 };
 
+// Input pixel as loaded: integer pixels stay unconverted until the row is
+// transformed, so that the loads of the next row stay in flight meanwhile.
+template <int KIN>
+using RawPx = std::conditional_t<KIN == IN_C64, float2, std::conditional_t<KIN == IN_F32, float, uint32_t>>;
+
+template <int KIN>
+__device__ __forceinline__ RawPx<KIN> load_px(const void* in, size_t i) {
+    if constexpr (KIN == IN_U8) return static_cast<const uint8_t*>(in)[i];
+    else if constexpr (KIN == IN_U16) return static_cast<const uint16_t*>(in)[i];
+    else if constexpr (KIN == IN_F32) return static_cast<const float*>(in)[i];
+    else return static_cast<const float2*>(in)[i];
+}
+
+template <int KIN>
+__device__ __forceinline__ float2 px_complex(RawPx<KIN> r) {
+    if constexpr (KIN == IN_U8) return make_float2(convert_px<uint8_t>(static_cast<uint8_t>(r)), 0.f);
+    else if constexpr (KIN == IN_U16) return make_float2(convert_px<uint16_t>(static_cast<uint16_t>(r)), 0.f);
+    else if constexpr (KIN == IN_F32) return make_float2(r, 0.f);
+    else return r;
+}
+
 // Each workgroup transforms `iters` groups of M rows of one frame; the loads
-// of group i + 1 are issued before group i is computed, so a workgroup keeps
-// two groups of input in flight.
-template <int N, int DIR, int KIN>
+// of group i + 1 (pixels, and chirp factors with CHIRP) are issued before
+// group i is computed, so a workgroup keeps two groups of input in flight.
+template <int N, int DIR, int KIN, bool CHIRP>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     k_row_fft(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
               FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int mm_slots, int iters) {
@@ -230,35 +256,33 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS)
     float2* my = lds + m * C::PITCH;
     auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
     auto row_of = [&](int it) { return (blockIdx.x * iters + it) * C::M + m; };
-    auto load_row = [&](int yr, float2* v) {
-        const bool valid = yr < height;  // idle rows of the last workgroup
-        const size_t in_off = in_frame + (size_t)(valid ? yr : 0) * N;
-        const float2* crow = chirp ? chirp + (size_t)(valid ? yr : 0) * N : nullptr;
-        fft_load<N, C::E>(t, [&](int n) -> float2 {
-            float2 x;
-            if (!valid) return make_float2(0.f, 0.f);
-            if constexpr (KIN == IN_U8) x = make_float2(convert_px<uint8_t>(((const uint8_t*)in)[in_off + n]), 0.f);
-            else if constexpr (KIN == IN_U16) x = make_float2(convert_px<uint16_t>(((const uint16_t*)in)[in_off + n]), 0.f);
-            else if constexpr (KIN == IN_F32) x = make_float2(((const float*)in)[in_off + n], 0.f);
-            else x = ((const float2*)in)[in_off + n];
-            if (crow) x = cmul(x, crow[n]);
-            return x;
-        }, v);
+    RawPx<KIN> px[kE];
+    float2 cf[CHIRP ? kE : 1];
+    auto load_row = [&](int yr) {
+        // idle rows of the last workgroup read row 0; their results are not stored
+        const size_t row = (size_t)(yr < height ? yr : 0) * N;
+        fft_load_slots<N, C::E>(t, [&](int i, int n) {
+            px[i] = load_px<KIN>(in, in_frame + row + n);
+            if constexpr (CHIRP) cf[i] = chirp[row + n];
+        });
     };
-    float2 cur[kE], nxt[kE];
-    load_row(row_of(0), cur);
+    load_row(row_of(0));
     float2 tw[FftTw<N, C::E>::COUNT];
     fft_twiddles<N, C::E>(t, tw);
     for (int it = 0; it < iters; ++it) {
         const int yr = row_of(it);
-        if (it + 1 < iters) load_row(row_of(it + 1), nxt);
+        float2 v[kE];
+#pragma unroll
+        for (int i = 0; i < kE; ++i) {
+            v[i] = px_complex<KIN>(px[i]);
+            if constexpr (CHIRP) v[i] = cmul(v[i], cf[i]);
+        }
+        if (it + 1 < iters) load_row(row_of(it + 1));
         float2* dst = out + out_frame + (size_t)(yr < height ? yr : 0) * N;
         const bool valid = yr < height;
-        fft_compute_tw<N, C::E, DIR, false, (C::NT <= 64)>(t, cur, tw, [&](int n, float2 v) {
-            if (valid) dst[n] = scale == 1.0f ? v : cscale(v, scale);
+        fft_compute_tw<N, C::E, DIR, false, (C::NT <= 64)>(t, v, tw, [&](int n, float2 z) {
+            if (valid) dst[n] = scale == 1.0f ? z : cscale(z, scale);
         }, lds_at);
-#pragma unroll
-        for (int i = 0; i < kE; ++i) cur[i] = nxt[i];
     }
 }
 
@@ -269,13 +293,29 @@ enum { COL_FWD = 0, COL_INV = 1, COL_LENS = 2 };
 
 template <int N>
 struct ColCfg {
-    static constexpr int E = N >= 1024 ? 16 : HV_COL_E_SMALL;
+    static constexpr int E = N >= 2048 ? 16 : (N == 1024 ? HV_COL_E_1024 : HV_COL_E_SMALL);
     static constexpr int NT = FftShape<N, E>::NT;
     // columns per workgroup: 128-byte rows up to N = 512; longer columns
     // narrow the tile (the N x CW tile and 1024 threads' registers bound it)
     static constexpr int CW = N <= 512 ? HV_COL_CW_SMALL : (N == 1024 ? HV_COL_CW_1024 : (N == 2048 ? HV_COL_CW_BIG : 2));
     static constexpr int THREADS = NT * CW;
 };
+
+// Column tile of workgroup bx of nx.  Workgroups go round-robin to the 8
+// XCDs (linear id mod 8), each with its own L2; tiles narrower than a
+// 128-byte line share lines with their neighbours, so the LG tiles of one
+// line group are given to consecutive workgroups of one XCD and that XCD's L2
+// fetches each line once (instead of once per XCD touching it).
+template <int LG>
+__device__ __forceinline__ int xcd_tile(int bx, int nx) {
+    if constexpr (LG <= 1) {
+        return bx;
+    } else {
+        if (nx % (8 * LG)) return bx;
+        const int q = bx >> 3;
+        return ((q / LG) * 8 + (bx & 7)) * LG + q % LG;
+    }
+}
 
 // Each workgroup owns a CW-column tile and walks `fpb` frames of the batch,
 // loading the next frame's tile while the current one is transformed; the
@@ -290,7 +330,7 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
     constexpr int kE = FftShape<N, C::E>::kE;
     __shared__ float2 lds[N * C::CW];
     const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
-    const int x = blockIdx.x * C::CW + c;
+    const int x = xcd_tile<(16 / C::CW)>(blockIdx.x, gridDim.x) * C::CW + c;
     const int b0 = blockIdx.y * fpb;
     const size_t P = (size_t)N * width;
     auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
@@ -342,12 +382,29 @@ static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* 
     int iters = 1;
     while (iters < 8 && groups % (iters * 2) == 0 && (size_t)(groups / (iters * 2)) * batch >= HV_ROW_MIN_WG) iters *= 2;
     dim3 grid(groups / iters, batch);
-    switch (kin) {
-        case IN_U8: k_row_fft<N, DIR, IN_U8><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
-        case IN_U16: k_row_fft<N, DIR, IN_U16><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
-        case IN_F32: k_row_fft<N, DIR, IN_F32><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
-        default: k_row_fft<N, DIR, IN_C64><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters); break;
+#define HV_ROW_LAUNCH(KIN, CH) \
+    k_row_fft<N, DIR, KIN, CH><<<grid, C::THREADS, 0, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, mm_slots, iters)
+    if (chirp) {
+        // chirp factors only multiply the forward pass's input (Fresnel)
+        if constexpr (DIR < 0) {
+            switch (kin) {
+                case IN_U8: HV_ROW_LAUNCH(IN_U8, true); break;
+                case IN_U16: HV_ROW_LAUNCH(IN_U16, true); break;
+                case IN_F32: HV_ROW_LAUNCH(IN_F32, true); break;
+                default: HV_ROW_LAUNCH(IN_C64, true); break;
+            }
+        } else {
+            return fail(HV_ERR_UNSUPPORTED, "chirp factors on an inverse row pass");
+        }
+    } else {
+        switch (kin) {
+            case IN_U8: HV_ROW_LAUNCH(IN_U8, false); break;
+            case IN_U16: HV_ROW_LAUNCH(IN_U16, false); break;
+            case IN_F32: HV_ROW_LAUNCH(IN_F32, false); break;
+            default: HV_ROW_LAUNCH(IN_C64, false); break;
+        }
     }
+#undef HV_ROW_LAUNCH
     HV_CHECK_LAUNCH("k_row_fft");
     return HV_OK;
 }
