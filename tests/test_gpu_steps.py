@@ -92,7 +92,9 @@ def test_shift_corners_exact(st):
 
 
 FFT_SHAPES = [(2, 16, 16), (1, 16, 4096), (1, 4096, 16), (2, 32, 64), (3, 64, 128), (1, 256, 256), (2, 512, 512),
-              (1, 1024, 1024), (1, 2048, 32), (1, 32, 2048)]
+              (1, 1024, 1024), (1, 2048, 32), (1, 32, 2048),
+              # long columns in tiles narrower than a cache line, with the XCD-aware tile order active
+              (1, 2048, 128), (1, 4096, 128)]
 
 
 @pytest.mark.parametrize("shape", FFT_SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -106,7 +108,8 @@ def test_fft2(st, oracle, shape):
         assert err < 1e-6, (shape, d, err)
 
 
-@pytest.mark.parametrize("shape", [(4, 64, 64), (2, 512, 512), (1, 256, 1024)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", [(4, 64, 64), (2, 512, 512), (1, 256, 1024), (2, 1024, 128), (1, 2048, 128),
+                                   (1, 4096, 128)], ids=lambda s: "x".join(map(str, s)))
 def test_angular_spectrum(st, oracle, shape):
     rng = np.random.default_rng(4)
     B, H, W = shape
