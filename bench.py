@@ -19,6 +19,19 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+
+
+def measured_traffic(workload: str, path: str, stage: str):
+    """HBM bytes per launch of a stage's kernel (FETCH_SIZE + WRITE_SIZE from
+    rocprofv3, calibrated on a device copy) as the committed profile holds
+    them for this workload and path, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)[f"{workload}/{path}"]["stages"][stage]
+    except (OSError, KeyError, ValueError):
+        return None
+    return rec["fetch_bytes"] + rec["write_bytes"]
 
 
 def parse():
@@ -329,8 +342,11 @@ def main():
         stages.append({"name": nm, "ms": t, "bytes": b, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS})
     dom = max((i for i in range(5) if sb[i][1] > 0), key=lambda i: ms[i])
     line["roofline"] = {"bound": "hbm", "kernel": sb[dom][0], "achieved": stages[dom]["GB/s"],
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["frac"], "traffic": None,
-                        "timing": timing}
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stages[dom]["frac"],
+                        "traffic": measured_traffic(name, line["config"]["path"], sb[dom][0]),
+                        "algorithmic_bytes": sb[dom][1], "timing": timing}
+    if line["roofline"]["traffic"] is not None:
+        line["roofline"]["traffic_source"] = "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch)"
     tot_b, tot_ms = sum(b for _, b in sb), sum(ms)
     line["pipeline_roofline"] = {"bytes_per_step": tot_b, "ms_per_step_stages": tot_ms,
                                  "GB/s": tot_b / (tot_ms * 1e-3) / 1e9, "frac": tot_b / (tot_ms * 1e-3) / 1e9 /
