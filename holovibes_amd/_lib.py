@@ -88,6 +88,7 @@ SIGNATURES = {
     "hv_eig_workspace_size": [I, P],
     "hv_eig_hermitian_f64": [P, P, P, I, P, P],
     "hv_pca_image": [P, P, P, I, SZ, I, I, I, P, P],
+    "hv_pca_moments": [P, P, P, I, SZ, I, I, I, F, P, P],
     "hv_unwrap_scratch_size": [I, I, P],
     "hv_unwrap_2d": [P, P, I, I, P, P],
     "hv_rescale_unwrap2d": [P, P, SZ, P, P],

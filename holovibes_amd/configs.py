@@ -13,7 +13,8 @@ does not fix them; S = B keeps every batch in the window):
           angular spectrum (z=0.3 m), STFT T=64 with B=S=32, argument of bin
           2 then 2-D unwrap, percentile 0.5/99.5 % on an 8x8 subsample.
   c5      2048x2048 u16, 3 Doppler discs (f_d 4..60), angular spectrum
-          (z=0.35 m), T=B=S=128, STFT or PCA, moment 0 or 1 over bins 4..60.
+          (z=0.35 m), T=B=S=128, moment 0 (or 1) over bins / components
+          4..60: c5_stft (STFT) and c5_pca (PCA, split-FP16 covariance).
 """
 from __future__ import annotations
 
@@ -23,7 +24,7 @@ import ctypes
 from ._lib import ComputeSettings, SynthParams, load
 from .constants import (HV_CONTRAST_MINMAX, HV_CONTRAST_PERCENTILE, HV_CONTRAST_REFERENCE, HV_DEPTH_8, HV_DEPTH_16,
                         HV_IMG_ARGUMENT, HV_IMG_MODULUS, HV_IMG_MOMENTS_0, HV_IMG_MOMENTS_1, HV_IMG_SQUARED_MODULUS,
-                        HV_LENS_FP64, HV_PCA_BF16_SPLIT, HV_SPACE_ANGULARSP, HV_SPACE_FRESNELTR, HV_TIME_PCA,
+                        HV_LENS_FP64, HV_PCA_BF16_SPLIT, HV_PCA_FP16_SPLIT, HV_SPACE_ANGULARSP, HV_SPACE_FRESNELTR, HV_TIME_PCA,
                         HV_TIME_STFT)
 
 # settings fields (hv_compute_settings) and synthetic-source fields per workload
@@ -63,6 +64,16 @@ WORKLOADS = {
                       time_transformation_size=128, space_transformation=HV_SPACE_ANGULARSP,
                       time_transformation=HV_TIME_STFT, img_type=HV_IMG_MOMENTS_0, lambda_=852e-9, z_distance=0.35,
                       pixel_size=20.0, p_start=4, p_width=56, camera_fps=128.0, contrast_mode=HV_CONTRAST_MINMAX),
+        synth=dict(bits=12, sigma=0.3, mean_counts=1024.0,
+                   discs=[(300.0, 4, 20, 600.0, 700.0), (300.0, 21, 40, 1400.0, 800.0),
+                          (300.0, 41, 60, 1000.0, 1500.0)]),
+        seed=5000),
+    "c5_pca": dict(
+        settings=dict(width=2048, height=2048, depth=HV_DEPTH_16, batch_size=128, time_stride=128,
+                      time_transformation_size=128, space_transformation=HV_SPACE_ANGULARSP,
+                      time_transformation=HV_TIME_PCA, img_type=HV_IMG_MOMENTS_0, lambda_=852e-9, z_distance=0.35,
+                      pixel_size=20.0, p_start=4, p_width=56, camera_fps=128.0, contrast_mode=HV_CONTRAST_MINMAX,
+                      pca_precision=HV_PCA_FP16_SPLIT),
         synth=dict(bits=12, sigma=0.3, mean_counts=1024.0,
                    discs=[(300.0, 4, 20, 600.0, 700.0), (300.0, 21, 40, 1400.0, 800.0),
                           (300.0, 41, 60, 1000.0, 1500.0)]),

@@ -294,6 +294,19 @@ hv_status hv_pca_image(float* out, const void* ring_c64, const void* evecs_c128,
                             pmax, minmax, as_stream(stream));
 }
 
+hv_status hv_pca_moments(float* out, const void* ring_c64, const void* evecs_c128, int T, size_t frame_res,
+                         int moment, int f_start, int f_end, float camera_fps, uint32_t* minmax, void* stream) {
+    HV_NOT_NULL(out);
+    HV_NOT_NULL(ring_c64);
+    HV_NOT_NULL(evecs_c128);
+    HV_TRY(check_T(T));
+    HV_TRY(check_res(frame_res));
+    if (moment < 0 || moment > 2) return fail(HV_ERR_INVALID_ARG, "moment must be 0, 1 or 2");
+    HV_TRY(check_range(T, f_start, f_end));
+    return launch_pca_image(out, (const float2*)ring_c64, (const double2*)evecs_c128, T, frame_res,
+                            HV_IMG_MOMENTS_0 + moment, f_start, f_end, minmax, as_stream(stream), 1, camera_fps);
+}
+
 hv_status hv_unwrap_scratch_size(int width, int height, size_t* bytes) {
     HV_NOT_NULL(bytes);
     HV_TRY(check_frame(width, height));

@@ -93,7 +93,9 @@ hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int p
                          hipStream_t st);
 size_t eig_workspace_bytes(int T);
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st);
+// img_type: an image type (mean over [pmin, pmax], x131071) or HV_IMG_MOMENTS_k
+// (weights from fps over [pmin, pmax] = [f_start, f_end])
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
-                           int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1);
+                           int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1, float fps = 0.f);
 
 }  // namespace hv

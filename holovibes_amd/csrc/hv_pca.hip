@@ -538,21 +538,51 @@ hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, in
 }
 
 // ---------------------------------------------------------------------------
-// projection C[:, t] = H v_t, t in [pmin, pmax], fused with the image type
+// projection C[:, t] = H v_t, t in [pmin, pmax], fused with the image type or
+// a moment
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float pca_term(float2 z, int img_type) {
-    if (img_type == HV_IMG_MODULUS) return hypotf(z.x, z.y);
-    if (img_type == HV_IMG_SQUARED_MODULUS) {
-        const float h = hypotf(z.x, z.y);
-        return h * h;
+// What a pixel's image accumulates from component t of its projection.
+//   image types (converts.cc:22-63): modulus / squared modulus / argument,
+//     then the mean over [pmin, pmax] and x131071 (converts.cc:50-61);
+//   moments (pipe.cc:429-455 over the PCA output, tools_conversion.cu:33-66,
+//     tools_compute.cu:38-97): |C_t|^2 weighted by 1, f_t, f_t^2 with
+//     f = fftfreq(T) * fps / T, the Nyquist component (T even) counted twice
+//     in M0 and M2 and cancelled in M1; no scale (the reference's x131071
+//     acts on the frame the moment copy then overwrites).
+struct PcaOut {
+    int kind;   // HV_IMG_*
+    int T;
+    float fps;
+    __device__ __forceinline__ bool moment() const { return kind >= HV_IMG_MOMENTS_0; }
+    __device__ __forceinline__ float weight(int t) const {
+        const float d = fps / T;
+        const float f = (T % 2 == 0) ? (t < T / 2 ? t * d : -((float)T - t) * d)
+                                     : (t < (T + 1) / 2 ? t * d : (t - (float)T) * d);
+        const bool nyq = (T % 2 == 0) && t == T / 2;
+        if (kind == HV_IMG_MOMENTS_0) return nyq ? 2.f : 1.f;
+        if (kind == HV_IMG_MOMENTS_1) return nyq ? 0.f : f;
+        return nyq ? 2.f * (f * f) : f * f;
     }
-    return atan2f(z.y, z.x);
-}
+    __device__ __forceinline__ float term(float2 z, int t) const {
+        if (moment()) return weight(t) * (z.x * z.x + z.y * z.y);
+        if (kind == HV_IMG_MODULUS) return hypotf(z.x, z.y);
+        if (kind == HV_IMG_SQUARED_MODULUS) {
+            const float h = hypotf(z.x, z.y);
+            return h * h;
+        }
+        return atan2f(z.y, z.x);
+    }
+    __device__ __forceinline__ float finish(float acc, int nsel) const {
+        if (moment()) return acc;
+        return acc / nsel * (float)((2 << 16) - 1);
+    }
+};
 
+// T <= 64: the pixel's window in registers, every selected eigenvector in LDS.
 template <int T>
 __global__ void __launch_bounds__(256)
     k_pca_image(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs, size_t P,
-                int img_type, int pmin, int pmax, uint32_t* mm, int mm_slots) {
+                PcaOut po, int pmin, int pmax, uint32_t* mm, int mm_slots) {
     extern __shared__ float2 vs[];  // T x nsel, s fastest
     const int nsel = pmax - pmin + 1;
     for (int e = threadIdx.x; e < T * nsel; e += blockDim.x) {
@@ -575,10 +605,9 @@ __global__ void __launch_bounds__(256)
             re += x[s].x * w.x - x[s].y * w.y;
             im += x[s].x * w.y + x[s].y * w.x;
         }
-        acc += pca_term(make_float2(re, im), img_type);
+        acc += po.term(make_float2(re, im), pmin + t);
     }
-    float val = acc / (pmax - pmin + 1);
-    val = val * (float)((2 << 16) - 1);  // converts.cc:50-61
+    const float val = po.finish(acc, nsel);
     float vmin = INFINITY, vmax = -INFINITY;
     if (valid) {
         out[p] = val;
@@ -587,33 +616,50 @@ __global__ void __launch_bounds__(256)
     if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
-// T > 64: the pixel's window does not fit in registers; stream it per column.
+// T > 64: the window does not fit in registers.  The selected components go
+// in chunks of kPcaChunk; per chunk the block stages those eigenvector
+// columns in LDS and each thread streams its pixel's window once, keeping
+// the chunk's accumulators in registers (the ring is read once per chunk,
+// not once per component).
+constexpr int kPcaChunk = 32;
+
 __global__ void __launch_bounds__(256)
     k_pca_image_big(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs, int T,
-                    size_t P, int img_type, int pmin, int pmax, uint32_t* mm, int mm_slots) {
-    extern __shared__ float2 vs[];
+                    size_t P, PcaOut po, int pmin, int pmax, uint32_t* mm, int mm_slots) {
+    extern __shared__ float2 vs[];  // T x kPcaChunk, component fastest
     const int nsel = pmax - pmin + 1;
-    for (int e = threadIdx.x; e < T * nsel; e += blockDim.x) {
-        const int s = e % T, t = pmin + e / T;
-        const double2 d = evecs[s + (size_t)t * T];
-        vs[e] = make_float2((float)d.x, (float)d.y);
-    }
-    __syncthreads();
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = p < P;
+    const float2* xp = ring + (valid ? p : 0);
     float acc = 0.f;
-    for (int t = 0; t < nsel; ++t) {
-        float re = 0.f, im = 0.f;
-        for (int s = 0; s < T; ++s) {
-            const float2 xs = valid ? ring[(size_t)s * P + p] : make_float2(0.f, 0.f);
-            const float2 w = vs[t * T + s];
-            re += xs.x * w.x - xs.y * w.y;
-            im += xs.x * w.y + xs.y * w.x;
+    for (int c0 = 0; c0 < nsel; c0 += kPcaChunk) {
+        const int nc = min(kPcaChunk, nsel - c0);
+        __syncthreads();  // the previous chunk's columns have been read
+        for (int e = threadIdx.x; e < T * kPcaChunk; e += blockDim.x) {
+            const int j = e % kPcaChunk, sidx = e / kPcaChunk;
+            const double2 d = j < nc ? evecs[sidx + (size_t)(pmin + c0 + j) * T] : make_double2(0.0, 0.0);
+            vs[e] = make_float2((float)d.x, (float)d.y);
         }
-        acc += pca_term(make_float2(re, im), img_type);
+        __syncthreads();
+        float re[kPcaChunk], im[kPcaChunk];
+#pragma unroll
+        for (int j = 0; j < kPcaChunk; ++j) re[j] = im[j] = 0.f;
+#pragma unroll 2
+        for (int sidx = 0; sidx < T; ++sidx) {
+            const float2 xs = xp[(size_t)sidx * P];
+            const float2* w = vs + sidx * kPcaChunk;
+#pragma unroll
+            for (int j = 0; j < kPcaChunk; ++j) {
+                const float2 v = w[j];
+                re[j] += xs.x * v.x - xs.y * v.y;
+                im[j] += xs.x * v.y + xs.y * v.x;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPcaChunk; ++j)
+            if (j < nc) acc += po.term(make_float2(re[j], im[j]), pmin + c0 + j);
     }
-    float val = acc / (pmax - pmin + 1);
-    val = val * (float)((2 << 16) - 1);
+    const float val = po.finish(acc, nsel);
     float vmin = INFINITY, vmax = -INFINITY;
     if (valid) {
         out[p] = val;
@@ -623,19 +669,25 @@ __global__ void __launch_bounds__(256)
 }
 
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
-                           int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
+                           int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots, float fps) {
     const unsigned g = (unsigned)((P + 255) / 256);
-    const size_t shm = (size_t)T * (pmax - pmin + 1) * sizeof(float2);
-    if (shm > 64 * 1024) return fail(HV_ERR_UNSUPPORTED, "PCA eigenvector range too large");
-    switch (T) {
-        case 1: k_pca_image<1><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        case 2: k_pca_image<2><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        case 4: k_pca_image<4><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        case 8: k_pca_image<8><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        case 16: k_pca_image<16><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        case 32: k_pca_image<32><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        case 64: k_pca_image<64><<<g, 256, shm, st>>>(out, ring, evecs, P, img_type, pmin, pmax, mm, mm_slots); break;
-        default: k_pca_image_big<<<g, 256, shm, st>>>(out, ring, evecs, T, P, img_type, pmin, pmax, mm, mm_slots); break;
+    const PcaOut po{img_type, T, fps};
+    if (T <= 64) {
+        const size_t shm = (size_t)T * (pmax - pmin + 1) * sizeof(float2);
+        if (shm > 64 * 1024) return fail(HV_ERR_UNSUPPORTED, "PCA eigenvector range too large");
+        switch (T) {
+            case 1: k_pca_image<1><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+            case 2: k_pca_image<2><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+            case 4: k_pca_image<4><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+            case 8: k_pca_image<8><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+            case 16: k_pca_image<16><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+            case 32: k_pca_image<32><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+            default: k_pca_image<64><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
+        }
+    } else {
+        // T x kPcaChunk float2 = 64 KiB at T = 256
+        const size_t shm = (size_t)T * kPcaChunk * sizeof(float2);
+        k_pca_image_big<<<g, 256, shm, st>>>(out, ring, evecs, T, P, po, pmin, pmax, mm, mm_slots);
     }
     HV_CHECK_LAUNCH("k_pca_image");
     return HV_OK;

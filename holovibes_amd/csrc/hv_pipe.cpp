@@ -239,11 +239,19 @@ hv_status hv_pipe::image_work() {
         // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
         const int fs = (int)s.p_start;
         const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
-        float* m0 = it == HV_IMG_MOMENTS_0 ? L.image : nullptr;
-        float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
-        float* m2 = it == HV_IMG_MOMENTS_2 ? L.image : nullptr;
-        HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
-                                   it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
+        if (s.time_transformation == HV_TIME_PCA) {
+            // moments of the PCA output (gpu_p_acc_buffer = H V, fourier_transform.cc:340-377)
+            HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, L.stream));
+            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, fs, fe, mm_img, L.stream, kPipeMm,
+                                    s.camera_fps));
+        } else {
+            float* m0 = it == HV_IMG_MOMENTS_0 ? L.image : nullptr;
+            float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
+            float* m2 = it == HV_IMG_MOMENTS_2 ? L.image : nullptr;
+            HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
+                                       it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
+        }
     }
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
     HV_TRY(mark(4));
@@ -363,8 +371,8 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
             return fail(HV_ERR_INVALID_ARG, "P range must satisfy p_start + p_width < T");
     } else if (is_moment_kind(s->img_type)) {
         if (s->p_start >= T) return fail(HV_ERR_INVALID_ARG, "p_start must be < T");
-        if (s->time_transformation != HV_TIME_STFT)
-            return fail(HV_ERR_UNSUPPORTED, "moments are computed from the STFT");
+        if (s->time_transformation != HV_TIME_STFT && s->time_transformation != HV_TIME_PCA)
+            return fail(HV_ERR_UNSUPPORTED, "moments are computed from the STFT or PCA output");
     } else {
         return fail(HV_ERR_UNSUPPORTED, "image type not on this path");
     }

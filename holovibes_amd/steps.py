@@ -177,6 +177,18 @@ def pca_image(ring: torch.Tensor, V: torch.Tensor, img_type: int, pmin: int, pma
     return out
 
 
+def pca_moments(ring: torch.Tensor, V: torch.Tensor, moment: int, f_start: int, f_end: int,
+                fps: float) -> torch.Tensor:
+    """C = H V (matrix_operations.cc:77-102), then moment `moment` of |C_f|^2 over
+    [f_start, f_end] (pipe.cc:429-455, tools_compute.cu:38-97)."""
+    T = ring.shape[0]
+    colmajor = V.t().contiguous().to(torch.complex128)
+    out = torch.empty(ring.shape[1:], dtype=torch.float32, device=ring.device)
+    _lib.call("hv_pca_moments", _p(out), _p(ring), _p(colmajor), T, ring[0].numel(), moment, f_start, f_end, fps,
+              None, _s())
+    return out
+
+
 def unwrap_2d(phase: torch.Tensor) -> torch.Tensor:
     """unwrap_2d (tools_unwrap.cu:269-369), raw output."""
     H, W = phase.shape

@@ -215,6 +215,16 @@ hv_status hv_eig_hermitian_f64(double* evals, void* evecs_c128, const void* cov_
 hv_status hv_pca_image(float* out, const void* ring_c64, const void* evecs_c128, int T, size_t frame_res,
                        int img_type, int pmin, int pmax, uint32_t* minmax, void* stream);
 
+/* Replaces, for a PCA time transform, matrix_multiply_complex (matrix_operations.cc:77-102)
+ * followed by the moments path over its output gpu_p_acc_buffer (pipe.cc:429-455,
+ * tools_conversion.cu:33-66, tools_compute.cu:38-97): C = H V, then
+ * out = sum_{f=f_start}^{f_end} freq_k[f] |C[:, f]|^2 for k = moment (0, 1, 2), with the
+ * Nyquist component counted again for k = 0, 2 and cancelled for k = 1 when T is even;
+ * freq_1 = fftfreq(T) * camera_fps.  No x131071 scale (the reference's scale acts on the
+ * frame that the moment copy then overwrites).  minmax as in hv_stft_image. */
+hv_status hv_pca_moments(float* out, const void* ring_c64, const void* evecs_c128, int T, size_t frame_res,
+                         int moment, int f_start, int f_end, float camera_fps, uint32_t* minmax, void* stream);
+
 /* Replaces unwrap_2d (Backend/cuda_sources/tools_unwrap.cu:269-369):
  * least-squares 2-D phase unwrapping of a W x H phase image (the reference's
  * FFT2 sequence, computing FFT2(z) once instead of twice);

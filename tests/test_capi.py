@@ -50,7 +50,7 @@ def test_settings_struct_layout(lib):
 
 def test_validate_settings(lib):
     from holovibes_amd import configs
-    from holovibes_amd.constants import HV_IMG_MOMENTS_0, HV_TIME_PCA
+    from holovibes_amd.constants import HV_IMG_MOMENTS_0, HV_TIME_NONE
     ok = configs.settings_for("metric")
     assert lib.hv_validate_settings(ctypes.byref(ok)) == 0
     for name in configs.WORKLOADS:
@@ -58,7 +58,7 @@ def test_validate_settings(lib):
     bad = [dict(time_stride=48), dict(time_stride=16), dict(batch_size=0), dict(width=500),
            dict(time_transformation_size=24), dict(p_start=2, p_width=30), dict(depth=3), dict(lambda_=0.0),
            dict(contrast_mode=9), dict(contrast_subsample=3),
-           dict(img_type=HV_IMG_MOMENTS_0, time_transformation=HV_TIME_PCA)]
+           dict(img_type=HV_IMG_MOMENTS_0, time_transformation=HV_TIME_NONE)]
     for over in bad:
         s = configs.settings_for("metric", **over)
         assert lib.hv_validate_settings(ctypes.byref(s)) != 0, over

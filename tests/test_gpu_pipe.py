@@ -165,6 +165,36 @@ def test_pipe_moments(oracle, img_type):
     check(run_both(oracle, s, fl), 1e-5)
 
 
+def designed_frames(T, W, H, seed):
+    """Float frames whose time covariance has well-separated eigenvalues:
+    x_t = sum_k sqrt(l_k) u_k[t] b_k with u orthonormal over time, b
+    orthonormal over pixels and l_k = 1 + k / T (relative gaps 1 / (2T)).
+    The angular spectrum is unitary per frame, so the ring keeps them."""
+    rng = np.random.default_rng(seed)
+    U = np.linalg.qr(rng.standard_normal((T, T)))[0]
+    B = np.linalg.qr(rng.standard_normal((W * H, T)))[0]
+    lam = 1.0 + np.arange(T) / T
+    X = (U * np.sqrt(lam)) @ B.T
+    return (X * 1000).astype(np.float32).reshape(T, H, W)
+
+
+@pytest.mark.parametrize("img_type", [5, 6, 7])
+@pytest.mark.parametrize("T,precision,tol", [(32, 0, 1e-5), (128, 0, 1e-5), (128, 2, 1e-4)])
+def test_pipe_pca_moments(oracle, img_type, T, precision, tol):
+    """c5 PCA variant at 32 x 32: moments of the projections onto
+    eigenvectors f_start..f_end (register kernel at T = 32, chunked projection
+    over two component chunks at T = 128).  The synthetic interferograms'
+    eigenvalues lie 1e-8 x lambda_max apart at T = 128, below FP32 covariance
+    precision, so their eigenvectors are not determined; these frames have a
+    designed, well-separated spectrum."""
+    from holovibes_amd.configs import settings_for
+    from holovibes_amd.constants import HV_DEPTH_32F
+    s = settings_for("c5_pca", width=32, height=32, depth=HV_DEPTH_32F, batch_size=T, time_stride=T,
+                     time_transformation_size=T, img_type=img_type, p_start=4, p_width=min(56, T - 5),
+                     camera_fps=float(T), pca_precision=precision)
+    check(run_both(oracle, s, [designed_frames(T, 32, 32, 17)]), tol)
+
+
 def test_pipe_fft_shift_and_invert(oracle):
     s, fl = frames("c1", 1, width=64, height=64, fft_shift=1, contrast_invert=1)
     check(run_both(oracle, s, fl), 1e-5)

@@ -222,7 +222,8 @@ def test_eig_hermitian(st, oracle, T):
     assert rel_l2(A64 @ V, V * ev[None, :]) < 1e-12
 
 
-@pytest.mark.parametrize("T,pmin,pmax", [(8, 0, 7), (32, 4, 31), (16, 0, 0), (128, 100, 127)])
+@pytest.mark.parametrize("T,pmin,pmax", [(8, 0, 7), (32, 4, 31), (16, 0, 0), (128, 100, 127), (128, 4, 60),
+                                         (256, 0, 255)])
 @pytest.mark.parametrize("img_type", [0, 1])
 def test_pca_image(st, oracle, T, pmin, pmax, img_type):
     rng = np.random.default_rng(12)
@@ -232,6 +233,23 @@ def test_pca_image(st, oracle, T, pmin, pmax, img_type):
     C = oracle.pca_project(ring.astype(np.complex128), V)
     want = oracle.image_type(C, img_type, pmin, pmax) * 131071
     assert rel_l2(got, want) < 2e-6
+
+
+@pytest.mark.parametrize("T,fs,fe", [(16, 0, 15), (32, 4, 31), (128, 4, 60), (256, 3, 200)])
+@pytest.mark.parametrize("moment", [0, 1, 2])
+def test_pca_moments(st, oracle, T, fs, fe, moment):
+    """Moments of the PCA output (well-conditioned random unitary V): projection
+    pinned against the oracle's matmul, moments against its fftfreq/Nyquist rule."""
+    rng = np.random.default_rng(13)
+    ring = crand(rng, T, 16, 32) * 50
+    V = np.linalg.qr(rng.standard_normal((T, T)) + 1j * rng.standard_normal((T, T)))[0]
+    fps = 1.5 * T
+    got = host(st.pca_moments(dev(ring), dev(V.astype(np.complex128)), moment, fs, fe, fps))
+    C = oracle.pca_project(ring.astype(np.complex128), V)
+    w = oracle.moments(C, fs, fe, fps)
+    # M1 mixes signs (negative frequencies): bound the error by M0's scale
+    scale = np.abs(w[0]).max() * fps ** moment
+    assert np.abs(got - w[moment]).max() < 2e-6 * scale
 
 
 def test_unwrap_2d(st, oracle):
