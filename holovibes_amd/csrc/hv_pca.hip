@@ -27,6 +27,7 @@
 // (converts.cc:50-61) and the image min/max.
 #include <cfloat>
 
+#include "hv_fft.hpp"
 #include "hv_kernels.hpp"
 
 namespace hv {
@@ -38,21 +39,49 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 // covariance
 // ---------------------------------------------------------------------------
-static int pca_groups(int T) { return T < 32 ? 1 : T / 32; }
-static int pca_pairs(int T) {
+__host__ __device__ inline int pca_groups(int T) { return T < 32 ? 1 : T / 32; }
+__host__ __device__ inline int pca_pairs(int T) {
     const int g = pca_groups(T);
     return g * (g + 1) / 2;
 }
 static constexpr int kCovWaves = 4;
 static constexpr int kTile = 2 * 32 * 32;  // floats of one wave's two accumulators
 
-// Pixel slices per frame-group pair: about 512 workgroups in all, each
-// wave covering at least 256 pixels.
-static int pca_slices(int T, size_t P) {
+#ifndef HV_PCA_COV_GROUPED
+#define HV_PCA_COV_GROUPED 1    // T >= 64, half precision: every pair per workgroup, fragments shared in LDS
+#endif
+#ifndef HV_PCA_COV_WG
+#define HV_PCA_COV_WG 2048      // workgroups to aim for over all pairs and slices
+#endif
+
+// Pixel slices per frame-group pair: about HV_PCA_COV_WG workgroups in all
+// (several per CU: a wave waits on its next pixels' loads), each wave
+// covering at least 256 pixels.
+static int pca_slices_pairs(int T, size_t P) {
     const int np = pca_pairs(T);
     int ns = 1;
-    while (np * ns * 2 <= 512 && (size_t)(ns * 2) * kCovWaves * 256 <= P) ns *= 2;
+    while (np * ns * 2 <= HV_PCA_COV_WG && (size_t)(ns * 2) * kCovWaves * 256 <= P) ns *= 2;
     return ns;
+}
+
+// Half-precision covariance with T >= 64 (two or more frame groups): one
+// workgroup per pixel slice computes every pair (k_pca_cov_groups).
+static bool pca_cov_grouped(int T, int precision) {
+    return T >= 64 && precision != HV_PCA_FP32 && HV_PCA_COV_GROUPED;
+}
+constexpr int kCovGroupWaves = 8;
+constexpr int kCovGroupSlices = 512;  // about two workgroups per CU
+
+// Slices of the grouped kernel: at most kCovGroupSlices, at least four
+// 16-pixel steps per workgroup.
+static int pca_slices_grouped(size_t P) {
+    int ns = 1;
+    while (ns * 2 <= kCovGroupSlices && (size_t)(ns * 2) * 64 <= P && P % ((size_t)(ns * 2) * 16) == 0) ns *= 2;
+    return ns;
+}
+
+static int pca_slices(int T, size_t P, int precision) {
+    return pca_cov_grouped(T, precision) ? pca_slices_grouped(P) : pca_slices_pairs(T, P);
 }
 
 struct CovLayout {
@@ -60,18 +89,15 @@ struct CovLayout {
     size_t offset_scale;    // byte offset of the FP16 scale word
     size_t bytes;
 };
-static CovLayout cov_layout(int T, size_t P) {
+static CovLayout cov_layout(int T, size_t P, int precision) {
     CovLayout l;
-    l.partial_floats = (size_t)pca_slices(T, P) * pca_pairs(T) * kTile;
+    l.partial_floats = (size_t)pca_slices(T, P, precision) * pca_pairs(T) * kTile;
     l.offset_scale = l.partial_floats * sizeof(float);
     l.bytes = l.offset_scale + 256;
     return l;
 }
 
-size_t pca_workspace_bytes(int T, size_t P, int precision) {
-    (void)precision;
-    return cov_layout(T, P).bytes;
-}
+size_t pca_workspace_bytes(int T, size_t P, int precision) { return cov_layout(T, P, precision).bytes; }
 
 __device__ __forceinline__ void pair_of(int idx, int& F, int& G) {
     // pairs enumerated (0,0), (1,0), (1,1), (2,0), ... with F >= G
@@ -115,8 +141,20 @@ __global__ void __launch_bounds__(64 * kCovWaves)
     k_pca_cov(float* __restrict__ partial, const float2* __restrict__ ring, int T, size_t P, int slices,
               const uint32_t* __restrict__ scale_mm) {
     using S = CovStep<PREC>;
-    const int pair = blockIdx.x, slice = blockIdx.y;
-    const int npairs = gridDim.x;
+    // Workgroups go round-robin over the 8 XCDs (linear id mod 8), each with
+    // its own L2.  With slices a multiple of 8, the pairs of one slice run as
+    // consecutive workgroups of one XCD and read the same frame groups at
+    // about the same time, so that XCD's L2 fetches them once.
+    const int npairs = pca_pairs(T);
+    int pair, slice;
+    if (slices % 8 == 0) {
+        const int idx = blockIdx.x >> 3;
+        pair = idx % npairs;
+        slice = (idx / npairs) * 8 + (blockIdx.x & 7);
+    } else {
+        pair = blockIdx.x % npairs;
+        slice = blockIdx.x / npairs;
+    }
     int F, G;
     pair_of(pair, F, G);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -132,29 +170,60 @@ __global__ void __launch_bounds__(64 * kCovWaves)
     if constexpr (PREC == HV_PCA_FP16 || PREC == HV_PCA_FP16_SPLIT) scale = fp16_scale(scale_mm);
 
     v16f acc_re = {}, acc_im = {};
-    for (size_t p0 = p_begin; p0 < p_end; p0 += S::K) {
-        if constexpr (PREC == HV_PCA_FP32) {
-            const float2 a = okF ? rowF[p0 + koff] : make_float2(0.f, 0.f);
-            const float2 b = okG ? rowG[p0 + koff] : make_float2(0.f, 0.f);
-            acc_re = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc_re, 0, 0, 0);
-            acc_re = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc_re, 0, 0, 0);
-            acc_im = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acc_im, 0, 0, 0);
-            acc_im = __builtin_amdgcn_mfma_f32_32x32x2f32(-a.y, b.x, acc_im, 0, 0, 0);
-        } else {
-            float2 xa[8], xb[8];
-            {
-                const float4* pa = reinterpret_cast<const float4*>(rowF + p0 + koff);
-                const float4* pb = reinterpret_cast<const float4*>(rowG + p0 + koff);
+    if constexpr (PREC == HV_PCA_FP32) {
+        // 4 MFMA steps (2 pixels each) per iteration, the next iteration's
+        // pixels loaded before this one's products
+        constexpr int U = 4;
+        float2 na[U], nb[U];
+        auto load = [&](size_t p0) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float4 va = okF ? pa[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    float4 vb = okG ? pb[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    xa[2 * q] = make_float2(va.x * scale, va.y * scale);
-                    xa[2 * q + 1] = make_float2(va.z * scale, va.w * scale);
-                    xb[2 * q] = make_float2(vb.x * scale, vb.y * scale);
-                    xb[2 * q + 1] = make_float2(vb.z * scale, vb.w * scale);
-                }
+            for (int u = 0; u < U; ++u) {
+                na[u] = okF ? rowF[p0 + u * S::K + koff] : make_float2(0.f, 0.f);
+                nb[u] = okG ? rowG[p0 + u * S::K + koff] : make_float2(0.f, 0.f);
             }
+        };
+        load(p_begin);
+        for (size_t p0 = p_begin; p0 < p_end; p0 += U * S::K) {
+            float2 ca[U], cb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ca[u] = na[u];
+                cb[u] = nb[u];
+            }
+            if (p0 + U * S::K < p_end) load(p0 + U * S::K);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float2 a = ca[u], b = cb[u];
+                acc_re = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, acc_re, 0, 0, 0);
+                acc_re = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, acc_re, 0, 0, 0);
+                acc_im = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acc_im, 0, 0, 0);
+                acc_im = __builtin_amdgcn_mfma_f32_32x32x2f32(-a.y, b.x, acc_im, 0, 0, 0);
+            }
+        }
+    } else {
+        // the next 16 pixels' loads are issued before this step's conversions and products
+        float4 na[4], nb[4];
+        auto load = [&](size_t p0) {
+            const float4* pa = reinterpret_cast<const float4*>(rowF + p0 + koff);
+            const float4* pb = reinterpret_cast<const float4*>(rowG + p0 + koff);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                na[q] = okF ? pa[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                nb[q] = okG ? pb[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        };
+        load(p_begin);
+        for (size_t p0 = p_begin; p0 < p_end; p0 += S::K) {
+            float2 xa[8], xb[8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 va = na[q], vb = nb[q];
+                xa[2 * q] = make_float2(va.x * scale, va.y * scale);
+                xa[2 * q + 1] = make_float2(va.z * scale, va.w * scale);
+                xb[2 * q] = make_float2(vb.x * scale, vb.y * scale);
+                xb[2 * q + 1] = make_float2(vb.z * scale, vb.w * scale);
+            }
+            if (p0 + S::K < p_end) load(p0 + S::K);
             constexpr bool split = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_FP16_SPLIT);
             constexpr bool bf = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_BF16);
             using V8 = typename std::conditional<bf, v8bf, v8h>::type;
@@ -215,6 +284,138 @@ __global__ void __launch_bounds__(64 * kCovWaves)
     }
 }
 
+// T = 32 G frames (G >= 2), half precision.  Per 16-pixel step, wave g < G
+// loads group g's 32 frames x 16 pixels (lane l: frame 32 g + l % 32,
+// pixels 8 (l / 32) .. + 7, the MFMA operand layout of both A and B), converts
+// them once to the operand fragments -- real, imaginary and negated imaginary
+// parts, hi and (split) lo -- and stores them in LDS (double-buffered: one
+// barrier per step); then every wave runs the products of its pairs
+// (w, w + 8, ...) from the fragments.  The ring is read once; the
+// pair-per-workgroup kernel reads each group once per pair it belongs to.
+template <int PREC, int G>
+__global__ void __launch_bounds__(64 * kCovGroupWaves)
+    k_pca_cov_groups(float* __restrict__ partial, const float2* __restrict__ ring, size_t P, int slices,
+                     const uint32_t* __restrict__ scale_mm) {
+    constexpr int NP = G * (G + 1) / 2;
+    constexpr int W = kCovGroupWaves;
+    constexpr int PPW = (NP + W - 1) / W;
+    constexpr bool split = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_FP16_SPLIT);
+    constexpr bool bf = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_BF16);
+    using V8 = typename std::conditional<bf, v8bf, v8h>::type;
+    using E = typename std::conditional<bf, __bf16, _Float16>::type;
+    constexpr int NF = split ? 6 : 3;  // re, im, -im (hi), then the same (lo)
+    __shared__ V8 frag[2][G][NF][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t per_slice = P / slices;
+    const size_t p_begin = (size_t)blockIdx.x * per_slice, p_end = p_begin + per_slice;
+    float scale = 1.0f;
+    if constexpr (PREC == HV_PCA_FP16 || PREC == HV_PCA_FP16_SPLIT) scale = fp16_scale(scale_mm);
+    // converting waves: group `wave`, frame 32 wave + lane % 32, pixel offset 8 (lane / 32)
+    const bool conv = wave < G;
+    const float2* src = ring + (size_t)(32 * (conv ? wave : 0) + (lane & 31)) * P + (lane >> 5) * 8;
+    float4 nx[4];
+    auto load = [&](size_t p0) {
+        const float4* q = reinterpret_cast<const float4*>(src + p0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nx[k] = q[k];
+    };
+    if (conv) load(p_begin);
+    v16f acc_re[PPW], acc_im[PPW];
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) acc_re[j] = acc_im[j] = v16f{};
+    int buf = 0;
+    for (size_t p0 = p_begin; p0 < p_end; p0 += 16, buf ^= 1) {
+        if (conv) {
+            float2 x[8];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                x[2 * k] = make_float2(nx[k].x * scale, nx[k].y * scale);
+                x[2 * k + 1] = make_float2(nx[k].z * scale, nx[k].w * scale);
+            }
+            if (p0 + 16 < p_end) load(p0 + 16);
+            V8 rh, ih, nh, rl, il, nl;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const E r = (E)x[e].x, i = (E)x[e].y;
+                rh[e] = r;
+                ih[e] = i;
+                nh[e] = -i;
+                if constexpr (split) {
+                    const E rlo = (E)(x[e].x - (float)r), ilo = (E)(x[e].y - (float)i);
+                    rl[e] = rlo;
+                    il[e] = ilo;
+                    nl[e] = -ilo;
+                }
+            }
+            frag[buf][wave][0][lane] = rh;
+            frag[buf][wave][1][lane] = ih;
+            frag[buf][wave][2][lane] = nh;
+            if constexpr (split) {
+                frag[buf][wave][3][lane] = rl;
+                frag[buf][wave][4][lane] = il;
+                frag[buf][wave][5][lane] = nl;
+            }
+        }
+        __syncthreads();
+        auto mfma = [](V8 a, V8 b, v16f c) {
+            if constexpr (bf) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+            else return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+        };
+#pragma unroll
+        for (int j = 0; j < PPW; ++j) {
+            const int pair = wave + j * W;
+            if (pair >= NP) break;
+            int F, Gp;
+            pair_of(pair, F, Gp);
+            const V8 ar = frag[buf][F][0][lane], ai = frag[buf][F][1][lane], an = frag[buf][F][2][lane];
+            const V8 br = frag[buf][Gp][0][lane], bi = frag[buf][Gp][1][lane];
+            if constexpr (split) {
+                const V8 arl = frag[buf][F][3][lane], ail = frag[buf][F][4][lane], anl = frag[buf][F][5][lane];
+                const V8 brl = frag[buf][Gp][3][lane], bil = frag[buf][Gp][4][lane];
+                // small cross terms first, then hi*hi
+                acc_re[j] = mfma(arl, br, acc_re[j]);
+                acc_re[j] = mfma(ar, brl, acc_re[j]);
+                acc_re[j] = mfma(ail, bi, acc_re[j]);
+                acc_re[j] = mfma(ai, bil, acc_re[j]);
+                acc_im[j] = mfma(arl, bi, acc_im[j]);
+                acc_im[j] = mfma(ar, bil, acc_im[j]);
+                acc_im[j] = mfma(anl, br, acc_im[j]);
+                acc_im[j] = mfma(an, brl, acc_im[j]);
+            }
+            acc_re[j] = mfma(ar, br, acc_re[j]);
+            acc_re[j] = mfma(ai, bi, acc_re[j]);
+            acc_im[j] = mfma(ar, bi, acc_im[j]);
+            acc_im[j] = mfma(an, br, acc_im[j]);
+        }
+    }
+    // each pair's tile straight to its partial slot (one wave per pair: no cross-wave sum)
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) {
+        const int pair = wave + j * W;
+        if (pair >= NP) break;
+        float* out = partial + ((size_t)blockIdx.x * NP + pair) * kTile;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            out[r * 64 + lane] = acc_re[j][r];
+            out[1024 + r * 64 + lane] = acc_im[j][r];
+        }
+    }
+}
+
+template <int PREC>
+static hv_status cov_groups(float* partial, const float2* ring, int T, size_t P, int slices,
+                            const uint32_t* scale_mm, hipStream_t st) {
+    constexpr int threads = 64 * kCovGroupWaves;
+    switch (T) {
+        case 64: k_pca_cov_groups<PREC, 2><<<slices, threads, 0, st>>>(partial, ring, P, slices, scale_mm); break;
+        case 128: k_pca_cov_groups<PREC, 4><<<slices, threads, 0, st>>>(partial, ring, P, slices, scale_mm); break;
+        case 256: k_pca_cov_groups<PREC, 8><<<slices, threads, 0, st>>>(partial, ring, P, slices, scale_mm); break;
+        default: return fail(HV_ERR_UNSUPPORTED, "grouped covariance needs T in {64, 128, 256}");
+    }
+    HV_CHECK_LAUNCH("k_pca_cov_groups");
+    return HV_OK;
+}
+
 // Sum the slices and write the full Hermitian matrix, column-major.
 // Tile element (acc, r, lane) is row 8 (r / 4) + 4 (lane / 32) + r % 4 and
 // column lane % 32 of the 32 x 32 MFMA output.  Each workgroup owns 64
@@ -257,19 +458,34 @@ __global__ void __launch_bounds__(256)
 
 hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int precision, void* ws,
                          hipStream_t st) {
-    const CovLayout l = cov_layout(T, P);
-    const int slices = pca_slices(T, P), np = pca_pairs(T);
+    const CovLayout l = cov_layout(T, P, precision);
+    const int slices = pca_slices(T, P, precision), np = pca_pairs(T);
+    const bool grouped = pca_cov_grouped(T, precision);
     float* partial = reinterpret_cast<float*>(ws);
     uint32_t* scale_mm = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + l.offset_scale);
     const bool fp16 = precision == HV_PCA_FP16 || precision == HV_PCA_FP16_SPLIT;
-    if (P % ((size_t)slices * kCovWaves * 16)) return fail(HV_ERR_UNSUPPORTED, "frame_res not a multiple of the PCA tile");
+    // every wave's pixel range: a multiple of the 8 pixels per FP32 iteration and the 16 per half-precision step
+    if (P % ((size_t)slices * (grouped ? 1 : kCovWaves) * 16))
+        return fail(HV_ERR_UNSUPPORTED, "frame_res not a multiple of the PCA tile");
     if (fp16) {
         k_scale_init<<<1, 64, 0, st>>>(scale_mm);
         const size_t n = (size_t)T * P * 2;
         k_ring_absmax<<<(unsigned)std::min<size_t>((n + 255) / 256, 4096), 256, 0, st>>>(scale_mm, (const float*)ring, n);
         HV_CHECK_LAUNCH("k_ring_absmax");
     }
-    dim3 grid(np, slices);
+    const unsigned grid = (unsigned)(np * slices);
+    if (grouped) {
+        switch (precision) {
+            case HV_PCA_BF16_SPLIT: HV_TRY(cov_groups<HV_PCA_BF16_SPLIT>(partial, ring, T, P, slices, scale_mm, st)); break;
+            case HV_PCA_FP16_SPLIT: HV_TRY(cov_groups<HV_PCA_FP16_SPLIT>(partial, ring, T, P, slices, scale_mm, st)); break;
+            case HV_PCA_BF16: HV_TRY(cov_groups<HV_PCA_BF16>(partial, ring, T, P, slices, scale_mm, st)); break;
+            case HV_PCA_FP16: HV_TRY(cov_groups<HV_PCA_FP16>(partial, ring, T, P, slices, scale_mm, st)); break;
+            default: return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
+        }
+        k_pca_cov_reduce<<<dim3(np, 1024 / 64), 256, 0, st>>>(cov, partial, T, slices, np, scale_mm, fp16 ? 1 : 0);
+        HV_CHECK_LAUNCH("k_pca_cov_reduce");
+        return HV_OK;
+    }
     switch (precision) {
         case HV_PCA_FP32: k_pca_cov<HV_PCA_FP32><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
         case HV_PCA_BF16_SPLIT: k_pca_cov<HV_PCA_BF16_SPLIT><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
@@ -330,6 +546,26 @@ struct Rot {
     double2 ec;    // conj(e), e = a_pq / |a_pq|
 };
 
+// Rotation annihilating a_pq of the 2 x 2 Hermitian block [[app, apq], [conj apq, aqq]].
+__device__ __forceinline__ Rot make_rot(double app, double aqq, double2 apq) {
+    const double mag2 = apq.x * apq.x + apq.y * apq.y;
+    Rot R;
+    if (mag2 == 0.0) {
+        R.c = 1.0;
+        R.s = 0.0;
+        R.ec = make_double2(1.0, 0.0);
+    } else {
+        // t = sgn(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c
+        const double inv = 1.0 / sqrt(mag2);
+        R.ec = make_double2(apq.x * inv, -apq.y * inv);
+        const double tau = (aqq - app) * (0.5 * inv);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        R.c = 1.0 / sqrt(1.0 + t * t);
+        R.s = t * R.c;
+    }
+    return R;
+}
+
 template <int NT>
 __device__ double block_sum(double v, double* red) {
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -359,11 +595,62 @@ __device__ __forceinline__ void rot_left(const Rot& R, double2& yp, double2& yq)
     yq = nq_;
 }
 
-template <int NT, int N>
+// Eigenvalues ascending (stable ranks: ties keep their index order) and
+// phase-normalised eigenvectors -- the largest component real positive, the
+// first of equal magnitudes within a relative 1e-12 -- from the converged
+// diagonal d(j) and the accumulated rotations v(k, j).
+template <int NT, int N, class D, class VA>
+__device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int* order) {
+    constexpr int n = N;
+    const int tid = threadIdx.x;
+    for (int j = tid; j < n; j += NT) {
+        const double dj = d(j);
+        int rank = 0;
+#pragma unroll 4
+        for (int i = 0; i < n; ++i) {
+            const double di = d(i);
+            rank += (di < dj || (di == dj && i < j)) ? 1 : 0;
+        }
+        order[rank] = j;
+    }
+    __syncthreads();
+    for (int jj = tid; jj < n; jj += NT) {
+        const int j = order[jj];
+        evals[jj] = d(j);
+        int kmax = 0;
+        double best = -1.0;
+#pragma unroll 4
+        for (int k = 0; k < n; ++k) {
+            const double2 x = v(k, j);
+            const double m2 = x.x * x.x + x.y * x.y;
+            if (m2 > best * (1 + 2e-12)) {
+                best = m2;
+                kmax = k;
+            }
+        }
+        const double2 ph = v(kmax, j);
+        const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
+        const double2 rotp = make_double2(ph.x * inv, -ph.y * inv);
+#pragma unroll 4
+        for (int k = 0; k < n; ++k) evecs[k + (size_t)jj * n] = zmul(v(k, j), rotp);
+    }
+}
+
+// Rotation log of a deferred run: every round's n/2 rotations in order, then
+// the round count; the eigenvectors are accumulated afterwards row by row
+// (k_jacobi_apply), which many workgroups do in parallel.
+struct JacobiLog {
+    Rot* rot;       // HV_JACOBI_MAX_SWEEPS (n - 1) rounds of n / 2
+    int* rounds;
+    double* diag;   // converged diagonal, unordered
+};
+
+template <int NT, int N, bool DEFER = false>
 __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double2* evecs, Rot* rot, double* red,
-                           int* order) {
+                           int* order, JacobiLog log = JacobiLog{}) {
     constexpr int n = N, half = N / 2;
     const int tid = threadIdx.x;
+    int g = 0;  // rounds done
     auto a = [&](int i, int j) -> double2& { return A[i + (size_t)j * ld]; };
     auto v = [&](int i, int j) -> double2& { return V[i + (size_t)j * ld]; };
     double fro = 0.0;
@@ -388,24 +675,11 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
             for (int k = tid; k < half; k += NT) {
                 int p, q;
                 rr_pair(n, r, k, p, q);
-                const double2 apq = a(p, q);
-                const double mag2 = apq.x * apq.x + apq.y * apq.y;
-                Rot R;
-                if (mag2 == 0.0) {
-                    R.c = 1.0;
-                    R.s = 0.0;
-                    R.ec = make_double2(1.0, 0.0);
-                } else {
-                    // t = sgn(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c
-                    const double inv = 1.0 / sqrt(mag2);
-                    R.ec = make_double2(apq.x * inv, -apq.y * inv);
-                    const double tau = (a(q, q).x - a(p, p).x) * (0.5 * inv);
-                    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    R.c = 1.0 / sqrt(1.0 + t * t);
-                    R.s = t * R.c;
-                }
+                const Rot R = make_rot(a(p, p).x, a(q, q).x, a(p, q));
                 rot[k] = R;
+                if constexpr (DEFER) log.rot[(size_t)g * half + k] = R;
             }
+            ++g;
             __syncthreads();
             // A <- J^H A J, block by block; V <- V J, row pair by row pair
             for (int task = tid; task < half * half; task += NT) {
@@ -430,52 +704,26 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
                 a(q, pp) = b10;
                 a(q, qq) = b11;
             }
-            for (int task = tid; task < half * n; task += NT) {
-                const int m = task % n, kk = task / n;
-                int pp, qq;
-                rr_pair(n, r, kk, pp, qq);
-                double2 x = v(m, pp), y = v(m, qq);
-                rot_right(rot[kk], x, y);
-                v(m, pp) = x;
-                v(m, qq) = y;
+            if constexpr (!DEFER) {
+                for (int task = tid; task < half * n; task += NT) {
+                    const int m = task % n, kk = task / n;
+                    int pp, qq;
+                    rr_pair(n, r, kk, pp, qq);
+                    double2 x = v(m, pp), y = v(m, qq);
+                    rot_right(rot[kk], x, y);
+                    v(m, pp) = x;
+                    v(m, qq) = y;
+                }
             }
             __syncthreads();
         }
     }
-    // ascending order (stable: ties keep their index order), by parallel ranks
-    for (int j = tid; j < n; j += NT) {
-        const double dj = a(j, j).x;
-        int rank = 0;
-#pragma unroll 4
-        for (int i = 0; i < n; ++i) {
-            const double di = a(i, i).x;
-            rank += (di < dj || (di == dj && i < j)) ? 1 : 0;
-        }
-        order[rank] = j;
+    if constexpr (DEFER) {
+        for (int j = tid; j < n; j += NT) log.diag[j] = a(j, j).x;
+        if (tid == 0) *log.rounds = g;
+        return;
     }
-    __syncthreads();
-    // phase-normalised eigenvectors: the largest component real positive
-    // (the first of equal magnitudes within a relative 1e-12)
-    for (int jj = tid; jj < n; jj += NT) {
-        const int j = order[jj];
-        evals[jj] = a(j, j).x;
-        int kmax = 0;
-        double best = -1.0;
-#pragma unroll 4
-        for (int k = 0; k < n; ++k) {
-            const double2 x = v(k, j);
-            const double m2 = x.x * x.x + x.y * x.y;
-            if (m2 > best * (1 + 2e-12)) {
-                best = m2;
-                kmax = k;
-            }
-        }
-        const double2 ph = v(kmax, j);
-        const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
-        const double2 rotp = make_double2(ph.x * inv, -ph.y * inv);
-#pragma unroll 4
-        for (int k = 0; k < n; ++k) evecs[k + (size_t)jj * n] = zmul(v(k, j), rotp);
-    }
+    jacobi_finish<NT, N>([&](int j) { return a(j, j).x; }, v, evals, evecs, order);
 }
 
 template <int NT, int N>
@@ -496,25 +744,229 @@ __global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs
     jacobi_run<NT, N>(A, V, LD, evals, evecs, rot, red, order);
 }
 
+// T >= 128: A in the workspace, the rotations logged (JacobiLog) instead of
+// applied to V in the same workgroup: one workgroup streaming both matrices
+// through L2 every round was bound by that one CU's L2 bandwidth.
+template <int N>
+struct JacobiWs {
+    static constexpr size_t kLogRounds = (size_t)HV_JACOBI_MAX_SWEEPS * (N - 1);
+    static constexpr size_t a_off = 0;                                             // N x N double2
+    static constexpr size_t v_off = a_off + (size_t)N * N * sizeof(double2);       // N x N double2
+    static constexpr size_t log_off = v_off + (size_t)N * N * sizeof(double2);     // kLogRounds x N/2 Rot
+    static constexpr size_t diag_off = log_off + kLogRounds * (N / 2) * sizeof(Rot);
+    static constexpr size_t rounds_off = diag_off + (size_t)N * sizeof(double);
+    static constexpr size_t bytes = rounds_off + 256;
+};
+
 template <int NT, int N>
-__global__ void __launch_bounds__(NT) k_jacobi_global(double* evals, double2* evecs, const float2* cov,
-                                                     double2* work) {
+__global__ void __launch_bounds__(NT) k_jacobi_global(const float2* cov, char* ws) {
+    using L = JacobiWs<N>;
     __shared__ Rot rot[N / 2];
     __shared__ double red[NT / 64];
     __shared__ int order[N];
-    double2* A = work;
-    double2* V = work + (size_t)N * N;
+    double2* A = reinterpret_cast<double2*>(ws + L::a_off);
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         const float2 c = i >= j ? cov[i + (size_t)j * N] : cov[j + (size_t)i * N];
         A[e] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
-        V[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    jacobi_run<NT, N>(A, V, N, evals, evecs, rot, red, order);
+    const JacobiLog log{reinterpret_cast<Rot*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
+                        reinterpret_cast<double*>(ws + L::diag_off)};
+    jacobi_run<NT, N, true>(A, nullptr, N, nullptr, nullptr, rot, red, order, log);
 }
 
-size_t eig_workspace_bytes(int T) { return T <= 64 ? 0 : (size_t)2 * T * T * sizeof(double2); }
+// T = 128: the lower triangle of A packed in LDS (132 KB; the full matrix
+// does not fit).  A round's 2 x 2 blocks (k, kk) and (kk, k) hold conjugate
+// elements, so only blocks with k >= kk are updated: every stored element
+// once per round.  Rotations go to the log as in k_jacobi_global.
+template <int NT, int N>
+__global__ void __launch_bounds__(NT) k_jacobi_packed(const float2* cov, char* ws) {
+    using L = JacobiWs<N>;
+    constexpr int n = N, half = N / 2, nblocks = half * (half + 1) / 2;
+    extern __shared__ double2 Lt[];  // element (i, j), i >= j, at i (i + 1) / 2 + j
+    __shared__ Rot rot[half];
+    __shared__ double red[NT / 64];
+    const int tid = threadIdx.x;
+    auto at = [&](int i, int j) -> double2& { return Lt[i * (i + 1) / 2 + j]; };
+    auto get = [&](int i, int j) -> double2 {
+        if (i >= j) return at(i, j);
+        const double2 x = at(j, i);
+        return make_double2(x.x, -x.y);
+    };
+    auto set = [&](int i, int j, double2 v) {
+        if (i >= j) at(i, j) = v;
+        else at(j, i) = make_double2(v.x, -v.y);
+    };
+    // lower triangle of cov (CUBLAS_FILL_MODE_LOWER), real diagonal
+    for (int i = 0; i < n; ++i)
+        for (int j = tid; j <= i; j += NT) {
+            const float2 c = cov[i + (size_t)j * n];
+            at(i, j) = make_double2(c.x, i == j ? 0.0 : c.y);
+        }
+    __syncthreads();
+    auto tri_sum = [&](bool with_diag) {
+        double acc = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = tid; j <= i; j += NT) {
+                const double2 x = at(i, j);
+                const double m2 = x.x * x.x + x.y * x.y;
+                acc += i == j ? (with_diag ? m2 : 0.0) : 2.0 * m2;
+            }
+        return block_sum<NT>(acc, red);
+    };
+    const double fro = tri_sum(true);
+    const JacobiLog log{reinterpret_cast<Rot*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
+                        reinterpret_cast<double*>(ws + L::diag_off)};
+    // this thread's blocks (k, kk), k >= kk, with task = k (k + 1) / 2 + kk (the same every round)
+    constexpr int kTasks = (nblocks + NT - 1) / NT;
+    int tk[kTasks], tkk[kTasks];
+#pragma unroll
+    for (int u = 0; u < kTasks; ++u) {
+        const int task = tid + u * NT;
+        tk[u] = tkk[u] = -1;
+        if (task < nblocks) {
+            int k = (int)((sqrtf(8.0f * task + 1.0f) - 1.0f) * 0.5f);
+            while (k * (k + 1) / 2 > task) --k;
+            while ((k + 1) * (k + 2) / 2 <= task) ++k;
+            tk[u] = k;
+            tkk[u] = task - k * (k + 1) / 2;
+        }
+    }
+    int g = 0;
+    for (int sweep = 0; sweep < HV_JACOBI_MAX_SWEEPS; ++sweep) {
+        const double off = tri_sum(false);
+        if (off <= 1e-26 * fro || off == 0.0) break;
+        for (int r = 0; r < n - 1; ++r) {
+            for (int k = tid; k < half; k += NT) {
+                int p, q;
+                rr_pair(n, r, k, p, q);
+                const Rot R = make_rot(at(p, p).x, at(q, q).x, get(p, q));
+                rot[k] = R;
+                log.rot[(size_t)g * half + k] = R;
+            }
+            ++g;
+            __syncthreads();
+            // a round's blocks touch disjoint elements: all loads, then all
+            // rotations, then all stores, so that the LDS latencies overlap
+            int p[kTasks], q[kTasks], pp[kTasks], qq[kTasks];
+            double2 b[kTasks][4];
+#pragma unroll
+            for (int u = 0; u < kTasks; ++u) {
+                if (tk[u] < 0) continue;
+                rr_pair(n, r, tk[u], p[u], q[u]);
+                rr_pair(n, r, tkk[u], pp[u], qq[u]);
+                b[u][0] = get(p[u], pp[u]);
+                b[u][1] = get(p[u], qq[u]);
+                b[u][2] = get(q[u], pp[u]);
+                b[u][3] = get(q[u], qq[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kTasks; ++u) {
+                if (tk[u] < 0) continue;
+                const Rot Rr = rot[tk[u]], Rc = rot[tkk[u]];
+                rot_right(Rc, b[u][0], b[u][1]);
+                rot_right(Rc, b[u][2], b[u][3]);
+                rot_left(Rr, b[u][0], b[u][2]);
+                rot_left(Rr, b[u][1], b[u][3]);
+                if (tk[u] == tkk[u]) {  // the annihilated pair: exact zeros, real diagonal
+                    b[u][1] = make_double2(0.0, 0.0);
+                    b[u][2] = make_double2(0.0, 0.0);
+                    b[u][0].y = 0.0;
+                    b[u][3].y = 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kTasks; ++u) {
+                if (tk[u] < 0) continue;
+                set(p[u], pp[u], b[u][0]);
+                set(p[u], qq[u], b[u][1]);
+                set(q[u], pp[u], b[u][2]);
+                set(q[u], qq[u], b[u][3]);
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = tid; j < n; j += NT) log.diag[j] = at(j, j).x;
+    if (tid == 0) *log.rounds = g;
+}
+
+constexpr size_t jacobi_packed_lds(int n) { return (size_t)n * (n + 1) / 2 * sizeof(double2); }
+
+// V = J_1 J_2 ... J_G row by row: row m of the identity times the logged
+// rotations, one wavefront per row in LDS (a round's rotations touch
+// disjoint pairs; the wavefront's LDS accesses complete in order, so rounds
+// need only a fence).
+template <int N>
+__global__ void __launch_bounds__(256) k_jacobi_apply(const char* __restrict__ ws_in, char* ws) {
+    using L = JacobiWs<N>;
+    constexpr int half = N / 2;
+    __shared__ double2 rows[4][N];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = blockIdx.x * 4 + w;
+    double2* row = rows[w];
+    const Rot* log = reinterpret_cast<const Rot*>(ws_in + L::log_off);
+    const int G = *reinterpret_cast<const int*>(ws_in + L::rounds_off);
+    for (int j = lane; j < N; j += 64) row[j] = make_double2(j == m ? 1.0 : 0.0, 0.0);
+    fft_sync<true>();
+    for (int g = 0; g < G; ++g) {
+        const int r = g % (N - 1);
+#pragma unroll
+        for (int k = lane; k < half; k += 64) {
+            int p, q;
+            rr_pair(N, r, k, p, q);
+            const Rot R = log[(size_t)g * half + k];
+            double2 x = row[p], y = row[q];
+            rot_right(R, x, y);
+            row[p] = x;
+            row[q] = y;
+        }
+        fft_sync<true>();
+    }
+    double2* V = reinterpret_cast<double2*>(ws + L::v_off);
+    for (int j = lane; j < N; j += 64) V[m + (size_t)j * N] = row[j];
+}
+
+template <int NT, int N>
+__global__ void __launch_bounds__(NT) k_jacobi_finish(double* evals, double2* evecs, const char* __restrict__ ws) {
+    using L = JacobiWs<N>;
+    __shared__ int order[N];
+    const double* diag = reinterpret_cast<const double*>(ws + L::diag_off);
+    const double2* V = reinterpret_cast<const double2*>(ws + L::v_off);
+    jacobi_finish<NT, N>([&](int j) { return diag[j]; }, [&](int k, int j) { return V[k + (size_t)j * N]; }, evals,
+                         evecs, order);
+}
+
+size_t eig_workspace_bytes(int T) {
+    return T <= 64 ? 0 : (T == 128 ? JacobiWs<128>::bytes : JacobiWs<256>::bytes);
+}
+
+#ifndef HV_JACOBI_PACKED
+#define HV_JACOBI_PACKED 1      // T = 128: A as a packed triangle in LDS
+#endif
+#ifndef HV_JACOBI_NT128
+#define HV_JACOBI_NT128 1024    // threads of the packed T = 128 eigensolver
+#endif
+
+template <int N>
+static hv_status eig_deferred(double* evals, double2* evecs, const float2* cov, char* ws, hipStream_t st) {
+    if constexpr (N == 128 && HV_JACOBI_PACKED) {
+        constexpr size_t lds = jacobi_packed_lds(N);
+        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jacobi_packed<HV_JACOBI_NT128, N>),
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_jacobi_packed)");
+        k_jacobi_packed<HV_JACOBI_NT128, N><<<1, HV_JACOBI_NT128, lds, st>>>(cov, ws);
+        HV_CHECK_LAUNCH("k_jacobi_packed");
+    } else {
+        k_jacobi_global<1024, N><<<1, 1024, 0, st>>>(cov, ws);
+        HV_CHECK_LAUNCH("k_jacobi_global");
+    }
+    k_jacobi_apply<N><<<N / 4, 256, 0, st>>>(ws, ws);
+    HV_CHECK_LAUNCH("k_jacobi_apply");
+    k_jacobi_finish<256, N><<<1, 256, 0, st>>>(evals, evecs, ws);
+    HV_CHECK_LAUNCH("k_jacobi_finish");
+    return HV_OK;
+}
 
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st) {
     switch (T) {
@@ -528,9 +980,8 @@ hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, in
         case 128:
         case 256:
             if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
-            if (T == 128) k_jacobi_global<1024, 128><<<1, 1024, 0, st>>>(evals, evecs, cov, (double2*)work);
-            else k_jacobi_global<1024, 256><<<1, 1024, 0, st>>>(evals, evecs, cov, (double2*)work);
-            break;
+            return T == 128 ? eig_deferred<128>(evals, evecs, cov, (char*)work, st)
+                            : eig_deferred<256>(evals, evecs, cov, (char*)work, st);
         default: return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
     }
     HV_CHECK_LAUNCH("k_jacobi");

@@ -196,10 +196,13 @@ def test_stft_moments(st, oracle, T, fs, fe):
 
 
 @pytest.mark.parametrize("precision,tol", [(0, 2e-6), (1, 2e-5), (2, 2e-6), (3, 1e-2), (4, 2e-3)])
-@pytest.mark.parametrize("T", [8, 16, 32, 64, 128])
-def test_pca_cov(st, oracle, T, precision, tol):
+@pytest.mark.parametrize("T,H,W", [(8, 64, 64), (16, 64, 64), (32, 64, 64), (64, 64, 64), (128, 64, 64),
+                                   (256, 32, 64), (128, 256, 512)])
+def test_pca_cov(st, oracle, T, H, W, precision, tol):
+    """(T >= 64 in half precision runs the grouped kernel: every pair per
+    workgroup; 128 x 256 x 512 gives it 512 slices of 256 pixels.)"""
     rng = np.random.default_rng(10)
-    ring = crand(rng, T, 64, 64) * 300
+    ring = crand(rng, T, H, W) * 300
     got = host(st.pca_cov(dev(ring), precision))
     want = oracle.pca_cov(ring.astype(np.complex128))
     err = rel_l2(got, want)
@@ -207,7 +210,7 @@ def test_pca_cov(st, oracle, T, precision, tol):
     assert np.allclose(got, got.conj().T, atol=1e-5 * np.abs(want).max())
 
 
-@pytest.mark.parametrize("T", [2, 8, 32, 64, 128])
+@pytest.mark.parametrize("T", [2, 8, 32, 64, 128, 256])
 def test_eig_hermitian(st, oracle, T):
     rng = np.random.default_rng(11)
     A = crand(rng, T, 3 * T).astype(np.complex128)
