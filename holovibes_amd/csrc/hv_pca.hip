@@ -292,8 +292,12 @@ __global__ void __launch_bounds__(64 * kCovWaves)
 // barrier per step); then every wave runs the products of its pairs
 // (w, w + 8, ...) from the fragments.  The ring is read once; the
 // pair-per-workgroup kernel reads each group once per pair it belongs to.
+#ifndef HV_PCA_COV_GROUP_WPE
+#define HV_PCA_COV_GROUP_WPE 1  // minimum waves per SIMD the grouped kernel's registers must allow
+#endif
+
 template <int PREC, int G>
-__global__ void __launch_bounds__(64 * kCovGroupWaves)
+__global__ void __launch_bounds__(64 * kCovGroupWaves, HV_PCA_COV_GROUP_WPE)
     k_pca_cov_groups(float* __restrict__ partial, const float2* __restrict__ ring, size_t P, int slices,
                      const uint32_t* __restrict__ scale_mm) {
     constexpr int NP = G * (G + 1) / 2;
@@ -1072,8 +1076,7 @@ __global__ void __launch_bounds__(256)
 // columns in LDS and each thread streams its pixel's window once, keeping
 // the chunk's accumulators in registers (the ring is read once per chunk,
 // not once per component).
-constexpr int kPcaChunk = 32;
-
+template <int kPcaChunk>
 __global__ void __launch_bounds__(256)
     k_pca_image_big(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs, int T,
                     size_t P, PcaOut po, int pmin, int pmax, uint32_t* mm, int mm_slots) {
@@ -1136,9 +1139,10 @@ hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs,
             default: k_pca_image<64><<<g, 256, shm, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots); break;
         }
     } else {
-        // T x kPcaChunk float2 = 64 KiB at T = 256
-        const size_t shm = (size_t)T * kPcaChunk * sizeof(float2);
-        k_pca_image_big<<<g, 256, shm, st>>>(out, ring, evecs, T, P, po, pmin, pmax, mm, mm_slots);
+        // 32 components per chunk (64 held 151 registers and ran slower despite one pass less)
+        // T x 32 float2 = 64 KiB at T = 256
+        const size_t shm = (size_t)T * 32 * sizeof(float2);
+        k_pca_image_big<32><<<g, 256, shm, st>>>(out, ring, evecs, T, P, po, pmin, pmax, mm, mm_slots);
     }
     HV_CHECK_LAUNCH("k_pca_image");
     return HV_OK;
