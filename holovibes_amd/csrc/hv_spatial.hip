@@ -187,6 +187,9 @@ enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 #ifndef HV_COL_E_1024
 #define HV_COL_E_1024 16        // values per thread, column length 1024
 #endif
+#ifndef HV_COL_E_2048
+#define HV_COL_E_2048 16        // values per thread, column length 2048
+#endif
 #ifndef HV_COL_CW_1024
 #define HV_COL_CW_1024 8        // columns per workgroup, column length 1024
 #endif
@@ -293,7 +296,7 @@ enum { COL_FWD = 0, COL_INV = 1, COL_LENS = 2 };
 
 template <int N>
 struct ColCfg {
-    static constexpr int E = N >= 2048 ? 16 : (N == 1024 ? HV_COL_E_1024 : HV_COL_E_SMALL);
+    static constexpr int E = N >= 4096 ? 16 : (N == 2048 ? HV_COL_E_2048 : (N == 1024 ? HV_COL_E_1024 : HV_COL_E_SMALL));
     static constexpr int NT = FftShape<N, E>::NT;
     // columns per workgroup: 128-byte rows up to N = 512; longer columns
     // narrow the tile (the N x CW tile and 1024 threads' registers bound it)
