@@ -182,7 +182,7 @@ enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 #define HV_COL_E_SMALL 8        // values per thread, column lengths <= 512
 #endif
 #ifndef HV_COL_CW_SMALL
-#define HV_COL_CW_SMALL 16      // columns per workgroup, column lengths <= 512
+#define HV_COL_CW_SMALL 8       // columns per workgroup, column lengths <= 512
 #endif
 #ifndef HV_COL_E_1024
 #define HV_COL_E_1024 16        // values per thread, column length 1024
@@ -197,7 +197,7 @@ enum { IN_U8 = 0, IN_U16 = 1, IN_F32 = 2, IN_C64 = 3 };
 #define HV_COL_CW_BIG 4         // columns per workgroup, column length 2048 (4096: 2)
 #endif
 #ifndef HV_COL_MIN_WG
-#define HV_COL_MIN_WG 216       // keep at least this many column workgroups when pipelining (~ one per CU)
+#define HV_COL_MIN_WG 400       // keep at least this many column workgroups when pipelining (~ two per CU)
 #endif
 
 template <int N>
@@ -298,8 +298,11 @@ template <int N>
 struct ColCfg {
     static constexpr int E = N >= 4096 ? 16 : (N == 2048 ? HV_COL_E_2048 : (N == 1024 ? HV_COL_E_1024 : HV_COL_E_SMALL));
     static constexpr int NT = FftShape<N, E>::NT;
-    // columns per workgroup: 128-byte rows up to N = 512; longer columns
-    // narrow the tile (the N x CW tile and 1024 threads' registers bound it)
+    // columns per workgroup: 64-byte rows up to N = 512 (two 512-thread
+    // workgroups per CU; xcd_tile gives the two tiles of a 128-byte line to
+    // one XCD -- 16-column tiles of 1024 threads left one workgroup per CU
+    // and were 5 % slower alone, 10 % with the lanes' kernels beside them);
+    // longer columns narrow the tile (the N x CW tile and the registers bound it)
     static constexpr int CW = N <= 512 ? HV_COL_CW_SMALL : (N == 1024 ? HV_COL_CW_1024 : (N == 2048 ? HV_COL_CW_BIG : 2));
     static constexpr int THREADS = NT * CW;
 };
