@@ -18,6 +18,10 @@ namespace hv {
 
 enum { TM_IMAGE = 0, TM_FULL = 1, TM_MOMENTS = 2 };
 
+#ifndef HV_TIME_DIRECT_BINS
+#define HV_TIME_DIRECT_BINS 1   // STFT images of <= 4 bins by direct accumulation
+#endif
+
 struct TimeParams {
     float* out;        // image (TM_IMAGE) or M0 (TM_MOMENTS)
     float* m1;         // TM_MOMENTS
@@ -180,8 +184,66 @@ static hv_status time_launch_t(const float2* ring, size_t P, const TimeParams& p
     return HV_OK;
 }
 
+// STFT image of a few bins (K = pmax - pmin + 1 <= kDirectBins): the bins
+// accumulated directly as the window streams in, X_k = sum_t x_t w^{k t}
+// (w = exp(-2 pi i / T), the ring's physical slot order as in the FFT
+// kernels) -- K T complex products and 2 K accumulators per pixel instead of
+// the whole window in registers and a length-T FFT, so the pass runs at the
+// ring's read rate.
+constexpr int kDirectBins = 4;
+
+template <int T>
+__global__ void __launch_bounds__(256) k_time_bins(const float2* __restrict__ ring, size_t P, TimeParams prm) {
+    constexpr int LOGT = clog2(T);
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < P;
+    const float2* xp = ring + (valid ? p : 0);
+    const int K = prm.pmax - prm.pmin + 1;
+    float2 acc[kDirectBins];
+#pragma unroll
+    for (int j = 0; j < kDirectBins; ++j) acc[j] = make_float2(0.f, 0.f);
+#pragma unroll 8
+    for (int t = 0; t < T; ++t) {
+        const float2 x = xp[(size_t)t * P];
+#pragma unroll
+        for (int j = 0; j < kDirectBins; ++j) {
+            if (j >= K) break;  // uniform
+            const float2 w = twiddle<-1>(((prm.pmin + j) * t) & (T - 1), LOGT);
+            acc[j].x = fmaf(x.x, w.x, fmaf(-x.y, w.y, acc[j].x));
+            acc[j].y = fmaf(x.x, w.y, fmaf(x.y, w.x, acc[j].y));
+        }
+    }
+    float a = 0.f;
+#pragma unroll
+    for (int j = 0; j < kDirectBins; ++j)
+        if (j < K) a += image_term(acc[j], prm.img_type);
+    const float val = a / K;
+    float vmin = INFINITY, vmax = -INFINITY;
+    if (valid) {
+        prm.out[p] = val;
+        vmin = vmax = val;
+    }
+    if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm, prm.mm_slots != 0 ? prm.mm_slots : 1);
+}
+
+template <int T>
+static hv_status bins_launch_t(const float2* ring, size_t P, const TimeParams& prm, hipStream_t st) {
+    k_time_bins<T><<<(unsigned)((P + 255) / 256), 256, 0, st>>>(ring, P, prm);
+    HV_CHECK_LAUNCH("k_time_bins");
+    return HV_OK;
+}
+
 template <int MODE>
 static hv_status time_launch(const float2* ring, int T, size_t P, const TimeParams& prm, hipStream_t st) {
+    if (MODE == TM_IMAGE && T >= 16 && prm.pmax - prm.pmin + 1 <= kDirectBins && HV_TIME_DIRECT_BINS) {
+        switch (T) {
+            case 16: return bins_launch_t<16>(ring, P, prm, st);
+            case 32: return bins_launch_t<32>(ring, P, prm, st);
+            case 64: return bins_launch_t<64>(ring, P, prm, st);
+            case 128: return bins_launch_t<128>(ring, P, prm, st);
+            default: return bins_launch_t<256>(ring, P, prm, st);
+        }
+    }
     switch (T) {
         case 1: return time_launch_t<1, MODE>(ring, P, prm, st);
         case 2: return time_launch_t<2, MODE>(ring, P, prm, st);

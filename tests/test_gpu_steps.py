@@ -166,7 +166,9 @@ def test_stft(st, oracle, T):
 
 
 @pytest.mark.parametrize("T,pmin,pmax", [(1, 0, 0), (8, 0, 7), (16, 2, 7), (32, 2, 15), (64, 5, 5), (128, 4, 60),
-                                         (256, 0, 255)])
+                                         (256, 0, 255),
+                                         # <= 4 bins: direct accumulation instead of the FFT
+                                         (16, 0, 3), (64, 2, 2), (128, 61, 64), (256, 250, 253)])
 @pytest.mark.parametrize("img_type", [0, 1, 2])
 def test_stft_image(st, oracle, T, pmin, pmax, img_type):
     rng = np.random.default_rng(8)
