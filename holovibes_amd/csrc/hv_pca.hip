@@ -434,6 +434,8 @@ __global__ void __launch_bounds__(256)
     const int g = threadIdx.x >> 6, el = threadIdx.x & 63;
     const int e = blockIdx.y * 64 + el;  // 0 .. 1023
     float re = 0.f, im = 0.f;
+    // unrolled so that several slices' loads are in flight (same summation order)
+#pragma unroll 8
     for (int sl = g; sl < slices; sl += 4) {
         const float* t = partial + ((size_t)sl * npairs + pair) * kTile;
         re += t[e];
@@ -527,6 +529,9 @@ __device__ __forceinline__ void rr_pair(int n, int r, int k, int& p, int& q) {
     q = a < b ? b : a;
 }
 
+#ifndef HV_JACOBI_RSQRT
+#define HV_JACOBI_RSQRT 1       // rotation cosines by reciprocal square root
+#endif
 #ifndef HV_JACOBI_MAX_SWEEPS
 #define HV_JACOBI_MAX_SWEEPS 40
 #endif
@@ -534,7 +539,7 @@ __device__ __forceinline__ void rr_pair(int n, int r, int k, int& p, int& q) {
 #define HV_JACOBI_NT32 512      // threads of the T <= 32 eigensolver
 #endif
 #ifndef HV_JACOBI_NT64
-#define HV_JACOBI_NT64 512      // threads of the T <= 64 eigensolver
+#define HV_JACOBI_NT64 1024     // threads of the T <= 64 eigensolver
 #endif
 
 // One sweep round applies n/2 disjoint rotations J_k (pair k = (p_k, q_k)).
@@ -560,11 +565,12 @@ __device__ __forceinline__ Rot make_rot(double app, double aqq, double2 apq) {
         R.ec = make_double2(1.0, 0.0);
     } else {
         // t = sgn(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c
-        const double inv = 1.0 / sqrt(mag2);
+        // (reciprocal square roots: one dependent chain each instead of sqrt + divide)
+        const double inv = HV_JACOBI_RSQRT ? rsqrt(mag2) : 1.0 / sqrt(mag2);
         R.ec = make_double2(apq.x * inv, -apq.y * inv);
         const double tau = (aqq - app) * (0.5 * inv);
         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        R.c = 1.0 / sqrt(1.0 + t * t);
+        R.c = HV_JACOBI_RSQRT ? rsqrt(1.0 + t * t) : 1.0 / sqrt(1.0 + t * t);
         R.s = t * R.c;
     }
     return R;
