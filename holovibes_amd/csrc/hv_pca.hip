@@ -1128,10 +1128,111 @@ __global__ void __launch_bounds__(256)
     if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
+// T >= 32: the projection on MFMA (v_mfma_f32_32x32x2_f32, FP32 products and
+// sums as on the VALU).  A wavefront owns 32 pixels; per chunk of 32
+// components C (32 px x 32 components) = X (32 x T) V (T x 32) in steps of 2
+// frames: Re C = Xr Vr - Xi Vi, Im C = Xr Vi + Xi Vr.  Lane l feeds
+// X[pixel l % 32][frame 2 s + l / 32] as the A operand and V[frame 2 s + l / 32]
+// [component l % 32] (the chunk's columns staged in LDS) as the B operand; its
+// 16 outputs are C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  Each output's term
+// is summed over the components (the 32 lanes of a half-wavefront) by
+// shuffles.  NC chunks are held at once so that the window is read once for up
+// to 32 NC components.
+template <int T, int NC>
+__global__ void __launch_bounds__(256)
+    k_pca_image_mfma(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs,
+                     size_t P, PcaOut po, int pmin, int pmax, uint32_t* mm, int mm_slots) {
+    constexpr int W = 32 * NC;  // components staged per pass
+    __shared__ float2 vs[T * W];   // [frame][component]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const size_t px0 = ((size_t)blockIdx.x * 4 + wave) * 32;
+    const bool tile_ok = px0 + 32 <= P;
+    const float2* xp = ring + (size_t)half * P + (tile_ok ? px0 : 0) + col;
+    const int nsel = pmax - pmin + 1;
+    float rowacc[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) rowacc[r] = 0.f;
+    for (int c0 = 0; c0 < nsel; c0 += W) {
+        __syncthreads();  // the previous pass's columns have been read
+        for (int e = threadIdx.x; e < T * W; e += blockDim.x) {
+            const int j = e % W, fr = e / W, comp = pmin + c0 + j;
+            const double2 d = comp <= pmax ? evecs[fr + (size_t)comp * T] : make_double2(0.0, 0.0);
+            vs[e] = make_float2((float)d.x, (float)d.y);
+        }
+        __syncthreads();
+        v16f cr[NC], ci[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cr[c] = ci[c] = v16f{};
+#pragma unroll 4
+        for (int st = 0; st < T / 2; ++st) {
+            const float2 x = xp[(size_t)(2 * st) * P];
+            const float2* vrow = vs + (2 * st + half) * W + col;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const float2 v = vrow[32 * c];
+                cr[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, v.x, cr[c], 0, 0, 0);
+                cr[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(-x.y, v.y, cr[c], 0, 0, 0);
+                ci[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.x, v.y, ci[c], 0, 0, 0);
+                ci[c] = __builtin_amdgcn_mfma_f32_32x32x2f32(x.y, v.x, ci[c], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = c0 + 32 * c + col;  // selected component of this lane's outputs
+            const bool ok = j < nsel;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                float tv = ok ? po.term(make_float2(cr[c][r], ci[c][r]), pmin + j) : 0.f;
+#pragma unroll
+                for (int o = 16; o >= 1; o >>= 1) tv += __shfl_xor(tv, o, 64);
+                rowacc[r] += tv;
+            }
+        }
+    }
+    // lane (half, col < 16) writes pixel row 8 (col / 4) + 4 half + col % 4
+    float v = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        if (r == col) v = rowacc[r];
+    float vmin = INFINITY, vmax = -INFINITY;
+    if (tile_ok && col < 16) {
+        const float val = po.finish(v, nsel);
+        out[px0 + 8 * (col / 4) + 4 * half + (col % 4)] = val;
+        vmin = vmax = val;
+    }
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
+}
+
+#ifndef HV_PCA_PROJ_MFMA
+#define HV_PCA_PROJ_MFMA 1      // T >= 32: projection on MFMA
+#endif
+
+template <int T>
+static hv_status pca_image_mfma(float* out, const float2* ring, const double2* evecs, size_t P, const PcaOut& po,
+                                int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
+    const unsigned g = (unsigned)((P + 127) / 128);
+    // two chunks at once (64 components, T x 64 float2 of LDS) up to T = 128
+    if (T <= 128 && pmax - pmin + 1 > 32)
+        k_pca_image_mfma<T, (T <= 128 ? 2 : 1)><<<g, 256, 0, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots);
+    else
+        k_pca_image_mfma<T, 1><<<g, 256, 0, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots);
+    HV_CHECK_LAUNCH("k_pca_image_mfma");
+    return HV_OK;
+}
+
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots, float fps) {
     const unsigned g = (unsigned)((P + 255) / 256);
     const PcaOut po{img_type, T, fps};
+    if (HV_PCA_PROJ_MFMA && T >= 32 && P % 128 == 0) {
+        switch (T) {
+            case 32: return pca_image_mfma<32>(out, ring, evecs, P, po, pmin, pmax, mm, st, mm_slots);
+            case 64: return pca_image_mfma<64>(out, ring, evecs, P, po, pmin, pmax, mm, st, mm_slots);
+            case 128: return pca_image_mfma<128>(out, ring, evecs, P, po, pmin, pmax, mm, st, mm_slots);
+            default: return pca_image_mfma<256>(out, ring, evecs, P, po, pmin, pmax, mm, st, mm_slots);
+        }
+    }
     if (T <= 64) {
         const size_t shm = (size_t)T * (pmax - pmin + 1) * sizeof(float2);
         if (shm > 64 * 1024) return fail(HV_ERR_UNSUPPORTED, "PCA eigenvector range too large");
