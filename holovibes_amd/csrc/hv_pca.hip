@@ -106,11 +106,21 @@ __device__ __forceinline__ void pair_of(int idx, int& F, int& G) {
     G = idx - F * (F + 1) / 2;
 }
 
-// Max |component| of the ring, as an ordered uint (FP16 scaling).
-__global__ void __launch_bounds__(256) k_ring_absmax(uint32_t* out, const float* __restrict__ ring, size_t n) {
+// Max |component| of the ring, as an ordered uint (FP16 scaling); 16-byte
+// loads, four in flight per thread.
+__global__ void __launch_bounds__(256) k_ring_absmax(uint32_t* out, const float* __restrict__ ring, size_t n,
+                                                     bool vec) {
     float m = 0.f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        m = fmaxf(m, fabsf(ring[i]));
+    const size_t n4 = vec ? n / 4 : 0, stride = (size_t)gridDim.x * blockDim.x;
+    const float4* r4 = reinterpret_cast<const float4*>(ring);
+    auto amax = [](float4 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))); };
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        const float4 a = r4[i], b = r4[i + stride], c = r4[i + 2 * stride], d = r4[i + 3 * stride];
+        m = fmaxf(m, fmaxf(fmaxf(amax(a), amax(b)), fmaxf(amax(c), amax(d))));
+    }
+    for (; i < n4; i += stride) m = fmaxf(m, amax(r4[i]));
+    for (size_t j = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) m = fmaxf(m, fabsf(ring[j]));
     block_minmax_atomic(m, m, out);
 }
 
@@ -476,7 +486,9 @@ hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int p
     if (fp16) {
         k_scale_init<<<1, 64, 0, st>>>(scale_mm);
         const size_t n = (size_t)T * P * 2;
-        k_ring_absmax<<<(unsigned)std::min<size_t>((n + 255) / 256, 4096), 256, 0, st>>>(scale_mm, (const float*)ring, n);
+        const bool vec = (reinterpret_cast<uintptr_t>(ring) & 15) == 0;  // 16-byte loads need 16-byte alignment
+        k_ring_absmax<<<(unsigned)std::min<size_t>((n / 4 + 255) / 256 + 1, 8192), 256, 0, st>>>(
+            scale_mm, (const float*)ring, n, vec);
         HV_CHECK_LAUNCH("k_ring_absmax");
     }
     const unsigned grid = (unsigned)(np * slices);
