@@ -302,12 +302,8 @@ __global__ void __launch_bounds__(64 * kCovWaves)
 // barrier per step); then every wave runs the products of its pairs
 // (w, w + 8, ...) from the fragments.  The ring is read once; the
 // pair-per-workgroup kernel reads each group once per pair it belongs to.
-#ifndef HV_PCA_COV_GROUP_WPE
-#define HV_PCA_COV_GROUP_WPE 1  // minimum waves per SIMD the grouped kernel's registers must allow
-#endif
-
 template <int PREC, int G>
-__global__ void __launch_bounds__(64 * kCovGroupWaves, HV_PCA_COV_GROUP_WPE)
+__global__ void __launch_bounds__(64 * kCovGroupWaves)
     k_pca_cov_groups(float* __restrict__ partial, const float2* __restrict__ ring, size_t P, int slices,
                      const uint32_t* __restrict__ scale_mm) {
     constexpr int NP = G * (G + 1) / 2;
