@@ -158,6 +158,24 @@ def test_pipe_unwrap_percentile_subsample(oracle):
     check(res, 1e-3, u16_frac=0.95)
 
 
+@pytest.mark.parametrize("path", [2, 0])
+def test_pipe_minmax_partials_collapse(oracle, path):
+    """1024 x 512 image: more than 256 contrast workgroups, so the per-block
+    min/max partials are first reduced to one pair (k_mm_collapse); both the
+    bins-first and the three-pass producers."""
+    s, fl = frames("c1", 2, width=1024, height=512, batch_size=8, time_stride=8, time_transformation_size=8,
+                   p_start=1, p_width=3)
+    check(run_both(oracle, s, fl, fused=path, path=path), 1e-5)
+
+
+def test_pipe_unwrap_rescale_collapse(oracle):
+    """Unwrap at 512 x 256: the rescale's min/max has more than 256 consumer
+    workgroups (partials collapsed first)."""
+    s, fl = frames("c3", 2, width=512, height=256, time_transformation_size=8, batch_size=8, time_stride=8,
+                   syn_period=8)
+    check(run_both(oracle, s, fl), 1e-3, u16_frac=0.95)
+
+
 @pytest.mark.parametrize("img_type", [5, 6, 7])
 def test_pipe_moments(oracle, img_type):
     s, fl = frames("c5_stft", 1, width=64, height=64, batch_size=32, time_stride=32, time_transformation_size=32,
