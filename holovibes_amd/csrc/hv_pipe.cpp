@@ -120,6 +120,14 @@ constexpr int kPipeMm = HV_PIPE_MM_MODE;
 struct Lane {
     hipStream_t stream = nullptr;
     float2* planes = nullptr;  // bins-first planes (lane 0: the ring's first K frames)
+    // time-window ring and PCA buffers (lane 0: the pipe's; further lanes of a
+    // window-replaced three-pass pipe: their own)
+    float2* ring = nullptr;
+    float2* cov = nullptr;
+    double* evals = nullptr;
+    double2* evecs = nullptr;
+    void* pca_ws = nullptr;
+    void* eig_ws = nullptr;
     float* image = nullptr;
     float* cimage = nullptr;
     float* tmp = nullptr;
@@ -129,6 +137,7 @@ struct Lane {
     void* pct = nullptr;
     void* unwrap_ws = nullptr;
     void* mem = nullptr;  // own allocation (lanes >= 1)
+    bool bins_kind = false;  // own allocation holds bins-first planes (else a ring and PCA buffers)
     // graphs, one per (ring phase, input buffer) pair
     std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
 };
@@ -219,16 +228,16 @@ hv_status hv_pipe::image_work() {
             HV_TRY(launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, pmax - pmin + 1,
                                      s.space_transformation, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
-            HV_TRY(launch_row_ifft_stft_image(L.image, ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
+            HV_TRY(launch_row_ifft_stft_image(L.image, L.ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
                                               L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_STFT) {
-            HV_TRY(launch_stft_image(L.image, ring, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
+            HV_TRY(launch_stft_image(L.image, L.ring, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_PCA) {
-            HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, L.stream));
-            HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, L.stream));
-            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
+            HV_TRY(launch_pca_cov(L.cov, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, Tn, L.eig_ws, L.stream));
+            HV_TRY(launch_pca_image(L.image, L.ring, L.evecs, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else {
-            HV_TRY(launch_complex_to_image(L.image, ring, it, pmin, pmax, P, L.stream));
+            HV_TRY(launch_complex_to_image(L.image, L.ring, it, pmin, pmax, P, L.stream));
             if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kPipeMm));
         }
         if (unwrap) {
@@ -241,15 +250,15 @@ hv_status hv_pipe::image_work() {
         const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
         if (s.time_transformation == HV_TIME_PCA) {
             // moments of the PCA output (gpu_p_acc_buffer = H V, fourier_transform.cc:340-377)
-            HV_TRY(launch_pca_cov(cov, ring, Tn, P, s.pca_precision, pca_ws, L.stream));
-            HV_TRY(launch_eig_jacobi(evals, evecs, cov, Tn, eig_ws, L.stream));
-            HV_TRY(launch_pca_image(L.image, ring, evecs, Tn, P, it, fs, fe, mm_img, L.stream, kPipeMm,
+            HV_TRY(launch_pca_cov(L.cov, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, Tn, L.eig_ws, L.stream));
+            HV_TRY(launch_pca_image(L.image, L.ring, L.evecs, Tn, P, it, fs, fe, mm_img, L.stream, kPipeMm,
                                     s.camera_fps));
         } else {
             float* m0 = it == HV_IMG_MOMENTS_0 ? L.image : nullptr;
             float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
             float* m2 = it == HV_IMG_MOMENTS_2 ? L.image : nullptr;
-            HV_TRY(launch_stft_moments(m0, m1, m2, ring, Tn, P, fs, fe, s.camera_fps, mm_img,
+            HV_TRY(launch_stft_moments(m0, m1, m2, L.ring, Tn, P, fs, fe, s.camera_fps, mm_img,
                                        it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
         }
     }
@@ -293,7 +302,7 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
-                                        s.space_transformation, lens, ring, T, slot0, L.stream,
+                                        s.space_transformation, lens, L.ring, T, slot0, L.stream,
                                         timing ? &ev[1] : nullptr, path == PATH_ROW_STFT,
                                         minmax_fused() ? L.mm : nullptr, kPipeMm));
     }
@@ -442,6 +451,12 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->evecs = (double2*)take(l.evecs);
     p->pca_ws = take(l.pca_ws);
     p->eig_ws = take(l.eig_ws);
+    L0.ring = p->ring;
+    L0.cov = p->cov;
+    L0.evals = p->evals;
+    L0.evecs = p->evecs;
+    L0.pca_ws = p->pca_ws;
+    L0.eig_ws = p->eig_ws;
     // refresh-time work: lens, manual contrast range, zeroed ring
     hv_status st = hv_prepare_lens(p->lens, s->space_transformation, s->width, s->height, s->lambda, s->z_distance,
                                    s->pixel_size, s->lens_precision, L0.stream);
@@ -619,7 +634,7 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     p->path = choose_path(p->s, level);
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
-    if (p->path != PATH_BINS) p->nlanes = 1;  // only the bins-first path runs cycles side by side
+    p->nlanes = 1;  // extra lanes hold one path's buffers: hv_pipe_set_lanes again after a path change
     p->next = 0;
     return HV_OK;
 }
@@ -627,9 +642,12 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (lanes < 1 || lanes > kMaxLanes) return fail(HV_ERR_INVALID_ARG, "lanes must be in [1, 4]");
-    if (lanes > 1 && p->path != PATH_BINS)
-        return fail(HV_ERR_UNSUPPORTED, "only the bins-first path (B = S = T, STFT image, space transform) runs "
-                                        "cycles side by side");
+    // a cycle shares no device state with the next one when every batch
+    // replaces the whole window (B = S = T): then each lane has its own ring
+    // (or bins-first planes), PCA buffers, image and min/max
+    if (lanes > 1 && p->path != PATH_BINS && !window_replaced(p->s))
+        return fail(HV_ERR_UNSUPPORTED, "cycles run side by side only when every batch replaces the whole time "
+                                        "window (B = S = T)");
     if (lanes > 1 && !p->own_stream)
         return fail(HV_ERR_UNSUPPORTED, "several lanes need the pipe's own streams (create it without a stream)");
     HV_TRY(hv_pipe_sync(p));
@@ -638,10 +656,22 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     p_range(p->s, pmin, pmax);
     const size_t K = (size_t)(pmax - pmin + 1);
     const Layout& l = p->lay;
+    const bool bins = p->path == PATH_BINS;
+    // extra lanes allocated for the other kind of path (all of them: one call
+    // allocates one kind) are released first
+    while (p->lane.size() > 1 && p->lane.back().bins_kind != bins) {
+        Lane& x = p->lane.back();
+        if (x.mem) (void)hipFree(x.mem);
+        if (x.stream) (void)hipStreamDestroy(x.stream);
+        p->lane.pop_back();
+    }
     for (int i = (int)p->lane.size(); i < lanes; ++i) {
         Lane n;
-        const size_t planes = align256(K * p->P * sizeof(float2));
-        const size_t total = planes + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap;
+        const size_t planes = bins ? align256(K * p->P * sizeof(float2)) : 0;
+        const size_t ring = bins ? 0 : l.ring;
+        const size_t pca = bins ? 0 : l.cov + l.evals + l.evecs + l.pca_ws + l.eig_ws;
+        const size_t total = planes + ring + pca + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct +
+                             l.unwrap;
         hipError_t e = hipMalloc(&n.mem, total);
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(lane)");
         e = hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking);
@@ -655,7 +685,16 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
             c += b;
             return r;
         };
+        n.bins_kind = bins;
         n.planes = (float2*)take(planes);
+        n.ring = (float2*)take(ring);
+        if (!bins) {
+            n.cov = (float2*)take(l.cov);
+            n.evals = (double*)take(l.evals);
+            n.evecs = (double2*)take(l.evecs);
+            n.pca_ws = take(l.pca_ws);
+            n.eig_ws = take(l.eig_ws);
+        }
         n.image = (float*)take(l.image);
         n.cimage = (float*)take(l.cimage);
         n.tmp = (float*)take(l.tmp);

@@ -266,7 +266,42 @@ def test_pipe_lanes_match_one_lane(lanes, contrast_mode):
     many.close()
 
 
-def test_pipe_lanes_refused_off_bins_path():
+@pytest.mark.parametrize("case", ["stft", "row_stft", "pca", "pca_moments", "unwrap"])
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
+    """B = S = T on the other cycle paths: every batch replaces the window, so
+    lanes with their own rings and PCA buffers give the one-lane outputs."""
+    from holovibes_amd.pipe import Pipe
+    cfg = {"stft": ("c1", 0, dict(width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16)),
+           "row_stft": ("c1", 1, dict(width=128, height=128, batch_size=16, time_stride=16,
+                                      time_transformation_size=16)),
+           "pca": ("c2", 2, dict(width=128, height=128)),
+           "pca_moments": ("c5_pca", 2, dict(width=64, height=64, batch_size=32, time_stride=32,
+                                             time_transformation_size=32, p_start=4, p_width=20, camera_fps=32.0,
+                                             syn_period=32)),
+           "unwrap": ("c3", 0, dict(width=128, height=128, batch_size=16, time_stride=16,
+                                    time_transformation_size=16, syn_period=16))}[case]
+    name, level, over = cfg
+    s, fl = frames(name, 4, **over)
+    conv = (lambda f: torch.from_numpy(f.view(np.int16) if f.dtype == np.uint16 else f).cuda())
+    dev = [conv(f) for f in fl]
+    torch.cuda.synchronize()
+    one, many = Pipe(s, graph=True, fused=level), Pipe(s, graph=True, fused=level, lanes=lanes)
+    assert one.path == many.path and one.path != Pipe.BINS
+    for d in dev:
+        assert one.push(d) and many.push(d)
+        assert np.array_equal(one.output_u16(), many.output_u16())
+        assert torch.equal(one.image(), many.image())
+    for d in dev + dev[::-1]:
+        one.push(d)
+        many.push(d)
+    assert np.array_equal(one.output_u16(), many.output_u16())
+    assert np.array_equal(one.contrast_range(), many.contrast_range())
+    one.close()
+    many.close()
+
+
+def test_pipe_lanes_refused_when_window_not_replaced():
     from holovibes_amd import _lib
     from holovibes_amd.pipe import Pipe
     s, _ = frames("c1", 1, width=64, height=64, batch_size=4, time_stride=4, time_transformation_size=8)
