@@ -377,14 +377,16 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level);
 hv_status hv_pipe_path(const hv_pipe* p, int* path);
 
 /* No reference counterpart.  Lets up to `lanes` (1..4) consecutive cycles
- * run on the GPU at once, each on its own stream with its own plane,
- * image, contrast and output buffers (allocated here, outside the per-batch
- * path); cycle n runs on lane n mod lanes.  Only for the bins-first path,
- * whose cycles share no state, and only for a pipe that owns its streams
- * (created without a stream).  Outputs (hv_pipe_output,
+ * run on the GPU at once, each on its own stream with its own planes (or
+ * time-window ring and PCA buffers), image, contrast and output buffers
+ * (allocated here, outside the per-batch path); cycle n runs on lane n mod
+ * lanes.  Only when every batch replaces the whole window (B = S = T, so
+ * that cycles share no device state) and only for a pipe that owns its
+ * streams (created without a stream).  Outputs (hv_pipe_output,
  * hv_pipe_contrast_range) are the latest cycle's; call hv_pipe_sync before
  * reading them.  A frozen autocontrast range reaches every lane in stream
- * order.  Switching the path back (hv_pipe_set_fused) returns to one lane. */
+ * order.  Changing the path (hv_pipe_set_fused) returns to one lane; the
+ * frames a lane reads must stay unchanged until hv_pipe_sync. */
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
 
 /* No reference counterpart (measurement).  Mean milliseconds of one launch of
