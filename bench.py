@@ -283,9 +283,8 @@ def main():
 
     pipe_level = args.path
     pipe = Pipe(s, graph=args.graph and not args.no_graph, fused=pipe_level)
-    # cycles side by side: the bins-first path, or any path when every batch replaces the window (B = S = T)
-    window_replaced = s.batch_size == s.time_stride == s.time_transformation_size
-    lanes = args.lanes if (pipe.path == Pipe.BINS or window_replaced) else 1
+    # cycles side by side (own buffers when every batch replaces the window, else a shared ring)
+    lanes = args.lanes
     if lanes > 1:
         pipe.close()
         pipe = Pipe(s, graph=args.graph and not args.no_graph, fused=pipe_level, lanes=lanes)

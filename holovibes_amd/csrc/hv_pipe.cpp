@@ -137,12 +137,22 @@ struct Lane {
     void* pct = nullptr;
     void* unwrap_ws = nullptr;
     void* mem = nullptr;  // own allocation (lanes >= 1)
-    bool bins_kind = false;  // own allocation holds bins-first planes (else a ring and PCA buffers)
+    int kind = 0;         // LANE_*: what the own allocation holds besides the image buffers
+    // shared-ring lanes: recorded once the cycle's time transform has read the ring
+    hipEvent_t tt_done = nullptr;
     // graphs, one per (ring phase, input buffer) pair
     std::map<std::pair<int, const void*>, hipGraphExec_t> graphs;
 };
 
 constexpr int kMaxLanes = 4;
+
+// Lane kinds.  B = S = T: a cycle shares no device state with the next, so a
+// lane holds its own bins-first planes (LANE_BINS) or ring and PCA buffers
+// (LANE_OWN_RING).  Otherwise (LANE_SHARED_RING) the lanes share lane 0's ring
+// and PCA buffers: cycle n + 1's propagation waits until cycle n's time
+// transform has read the ring, and cycle n's image post-processing (unwrap,
+// percentile, contrast) runs beside cycle n + 1's propagation.
+enum { LANE_BINS = 0, LANE_OWN_RING = 1, LANE_SHARED_RING = 2 };
 
 struct hv_pipe {
     hv_compute_settings s{};
@@ -182,6 +192,7 @@ struct hv_pipe {
 
     hv_status image_work();
     hv_status enqueue_cycle(const void* frames, int slot0);
+    bool shared_ring() const { return nlanes > 1 && lane[1].kind == LANE_SHARED_RING; }
     // min/max folded into the image kernel, accumulators re-armed by the
     // cycle's first kernel (the row pass of a space transform)
     bool minmax_fused() const {
@@ -240,6 +251,7 @@ hv_status hv_pipe::image_work() {
             HV_TRY(launch_complex_to_image(L.image, L.ring, it, pmin, pmax, P, L.stream));
             if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kPipeMm));
         }
+        if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
         if (unwrap) {
             HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream));
             HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kPipeMm));
@@ -261,6 +273,7 @@ hv_status hv_pipe::image_work() {
             HV_TRY(launch_stft_moments(m0, m1, m2, L.ring, Tn, P, fs, fe, s.camera_fps, mm_img,
                                        it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
         }
+        if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
     }
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
     HV_TRY(mark(4));
@@ -292,6 +305,9 @@ hv_status hv_pipe::image_work() {
 
 hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     Lane& L = lane[cur];
+    // shared ring: this cycle overwrites slots the previous cycle's time
+    // transform reads (and reads the slots it keeps)
+    if (shared_ring() && last != cur && have_output) HV_HIP(hipStreamWaitEvent(L.stream, lane[last].tt_done, 0));
     HV_TRY(mark(0));
     if (path == PATH_BINS) {
         int pmin, pmax;
@@ -517,7 +533,7 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     if (p->timing) {
         HV_TRY(p->enqueue_cycle(frames, slot0));
         HV_TRY(p->collect_times());
-    } else if (p->graph_enabled && !pending) {
+    } else if (p->graph_enabled && !pending && !p->shared_ring()) {
         // one graph per (ring phase, input buffer); past the cache bound the
         // batch is staged into the pipe's own input buffer first (one lane)
         // or run without a graph (several lanes share no staging buffer)
@@ -642,12 +658,6 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (lanes < 1 || lanes > kMaxLanes) return fail(HV_ERR_INVALID_ARG, "lanes must be in [1, 4]");
-    // a cycle shares no device state with the next one when every batch
-    // replaces the whole window (B = S = T): then each lane has its own ring
-    // (or bins-first planes), PCA buffers, image and min/max
-    if (lanes > 1 && p->path != PATH_BINS && !window_replaced(p->s))
-        return fail(HV_ERR_UNSUPPORTED, "cycles run side by side only when every batch replaces the whole time "
-                                        "window (B = S = T)");
     if (lanes > 1 && !p->own_stream)
         return fail(HV_ERR_UNSUPPORTED, "several lanes need the pipe's own streams (create it without a stream)");
     HV_TRY(hv_pipe_sync(p));
@@ -656,20 +666,24 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     p_range(p->s, pmin, pmax);
     const size_t K = (size_t)(pmax - pmin + 1);
     const Layout& l = p->lay;
-    const bool bins = p->path == PATH_BINS;
-    // extra lanes allocated for the other kind of path (all of them: one call
-    // allocates one kind) are released first
-    while (p->lane.size() > 1 && p->lane.back().bins_kind != bins) {
+    const int kind = p->path == PATH_BINS ? LANE_BINS : window_replaced(p->s) ? LANE_OWN_RING : LANE_SHARED_RING;
+    const bool bins = kind == LANE_BINS, own_ring = kind == LANE_OWN_RING;
+    // extra lanes allocated for another kind (all of them: one call allocates
+    // one kind) are released first
+    while (p->lane.size() > 1 && p->lane.back().kind != kind) {
         Lane& x = p->lane.back();
         if (x.mem) (void)hipFree(x.mem);
         if (x.stream) (void)hipStreamDestroy(x.stream);
+        if (x.tt_done) (void)hipEventDestroy(x.tt_done);
         p->lane.pop_back();
     }
+    if (!p->lane[0].tt_done)
+        HV_HIP(hipEventCreateWithFlags(&p->lane[0].tt_done, hipEventDisableTiming));
     for (int i = (int)p->lane.size(); i < lanes; ++i) {
         Lane n;
         const size_t planes = bins ? align256(K * p->P * sizeof(float2)) : 0;
-        const size_t ring = bins ? 0 : l.ring;
-        const size_t pca = bins ? 0 : l.cov + l.evals + l.evecs + l.pca_ws + l.eig_ws;
+        const size_t ring = own_ring ? l.ring : 0;
+        const size_t pca = own_ring ? l.cov + l.evals + l.evecs + l.pca_ws + l.eig_ws : 0;
         const size_t total = planes + ring + pca + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct +
                              l.unwrap;
         hipError_t e = hipMalloc(&n.mem, total);
@@ -679,21 +693,35 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
             (void)hipFree(n.mem);
             return hip_fail(e, "hipStreamCreateWithFlags(lane)");
         }
+        e = hipEventCreateWithFlags(&n.tt_done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipFree(n.mem);
+            (void)hipStreamDestroy(n.stream);
+            return hip_fail(e, "hipEventCreateWithFlags(lane)");
+        }
         char* c = (char*)n.mem;
         auto take = [&](size_t b) {
             char* r = b ? c : nullptr;
             c += b;
             return r;
         };
-        n.bins_kind = bins;
+        n.kind = kind;
         n.planes = (float2*)take(planes);
-        n.ring = (float2*)take(ring);
-        if (!bins) {
+        if (own_ring) {
+            n.ring = (float2*)take(ring);
             n.cov = (float2*)take(l.cov);
             n.evals = (double*)take(l.evals);
             n.evecs = (double2*)take(l.evecs);
             n.pca_ws = take(l.pca_ws);
             n.eig_ws = take(l.eig_ws);
+        } else if (!bins) {
+            const Lane& z = p->lane[0];
+            n.ring = z.ring;
+            n.cov = z.cov;
+            n.evals = z.evals;
+            n.evecs = z.evecs;
+            n.pca_ws = z.pca_ws;
+            n.eig_ws = z.eig_ws;
         }
         n.image = (float*)take(l.image);
         n.cimage = (float*)take(l.cimage);
@@ -797,6 +825,8 @@ hv_status hv_pipe_destroy(hv_pipe* p) {
         if (p->lane[i].mem) (void)hipFree(p->lane[i].mem);
         if (p->lane[i].stream) (void)hipStreamDestroy(p->lane[i].stream);
     }
+    for (auto& l : p->lane)
+        if (l.tt_done) (void)hipEventDestroy(l.tt_done);
     for (auto e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->base) (void)hipFree(p->base);

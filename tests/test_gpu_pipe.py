@@ -266,11 +266,13 @@ def test_pipe_lanes_match_one_lane(lanes, contrast_mode):
     many.close()
 
 
-@pytest.mark.parametrize("case", ["stft", "row_stft", "pca", "pca_moments", "unwrap"])
+@pytest.mark.parametrize("case", ["stft", "row_stft", "pca", "pca_moments", "unwrap", "shared_stft",
+                                  "shared_unwrap", "shared_pca"])
 @pytest.mark.parametrize("lanes", [2, 3])
 def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
-    """B = S = T on the other cycle paths: every batch replaces the window, so
-    lanes with their own rings and PCA buffers give the one-lane outputs."""
+    """The other cycle paths on several lanes give the one-lane outputs: with
+    B = S = T every batch replaces the window and each lane has its own ring
+    and PCA buffers; with B < T the lanes share the ring."""
     from holovibes_amd.pipe import Pipe
     cfg = {"stft": ("c1", 0, dict(width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16)),
            "row_stft": ("c1", 1, dict(width=128, height=128, batch_size=16, time_stride=16,
@@ -280,18 +282,27 @@ def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
                                              time_transformation_size=32, p_start=4, p_width=20, camera_fps=32.0,
                                              syn_period=32)),
            "unwrap": ("c3", 0, dict(width=128, height=128, batch_size=16, time_stride=16,
-                                    time_transformation_size=16, syn_period=16))}[case]
+                                    time_transformation_size=16, syn_period=16)),
+           # B < T: the lanes share the ring (each cycle waits for the previous cycle's time transform)
+           "shared_stft": ("c1", 0, dict(width=128, height=64, batch_size=4, time_stride=4,
+                                         time_transformation_size=16)),
+           "shared_unwrap": ("c3", 0, dict(width=128, height=128, batch_size=8, time_stride=8,
+                                           time_transformation_size=16, syn_period=16)),
+           "shared_pca": ("c2", 0, dict(width=128, height=128, batch_size=8, time_stride=8,
+                                        time_transformation_size=32))}[case]
     name, level, over = cfg
-    s, fl = frames(name, 4, **over)
+    s, fl = frames(name, 8, **over)
     conv = (lambda f: torch.from_numpy(f.view(np.int16) if f.dtype == np.uint16 else f).cuda())
     dev = [conv(f) for f in fl]
     torch.cuda.synchronize()
     one, many = Pipe(s, graph=True, fused=level), Pipe(s, graph=True, fused=level, lanes=lanes)
     assert one.path == many.path and one.path != Pipe.BINS
     for d in dev:
-        assert one.push(d) and many.push(d)
-        assert np.array_equal(one.output_u16(), many.output_u16())
-        assert torch.equal(one.image(), many.image())
+        a, b = one.push(d), many.push(d)
+        assert a == b
+        if a:
+            assert np.array_equal(one.output_u16(), many.output_u16())
+            assert torch.equal(one.image(), many.image())
     for d in dev + dev[::-1]:
         one.push(d)
         many.push(d)
@@ -301,12 +312,12 @@ def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
     many.close()
 
 
-def test_pipe_lanes_refused_when_window_not_replaced():
+def test_pipe_lanes_need_own_streams():
     from holovibes_amd import _lib
     from holovibes_amd.pipe import Pipe
     s, _ = frames("c1", 1, width=64, height=64, batch_size=4, time_stride=4, time_transformation_size=8)
-    p = Pipe(s, graph=True)
-    assert p.path != Pipe.BINS
+    with torch.cuda.stream(torch.cuda.Stream()):
+        p = Pipe(s)  # on the caller's (non-default) stream
     assert _lib.status("hv_pipe_set_lanes", p.h, 2) == _lib.HV_ERR_UNSUPPORTED
     p.close()
 
