@@ -377,16 +377,20 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level);
 hv_status hv_pipe_path(const hv_pipe* p, int* path);
 
 /* No reference counterpart.  Lets up to `lanes` (1..4) consecutive cycles
- * run on the GPU at once, each on its own stream with its own planes (or
- * time-window ring and PCA buffers), image, contrast and output buffers
- * (allocated here, outside the per-batch path); cycle n runs on lane n mod
- * lanes.  Only when every batch replaces the whole window (B = S = T, so
- * that cycles share no device state) and only for a pipe that owns its
- * streams (created without a stream).  Outputs (hv_pipe_output,
- * hv_pipe_contrast_range) are the latest cycle's; call hv_pipe_sync before
- * reading them.  A frozen autocontrast range reaches every lane in stream
- * order.  Changing the path (hv_pipe_set_fused) returns to one lane; the
- * frames a lane reads must stay unchanged until hv_pipe_sync. */
+ * run on the GPU at once, each on its own stream with its own image,
+ * contrast and output buffers (allocated here, outside the per-batch path);
+ * cycle n runs on lane n mod lanes.  When every batch replaces the whole
+ * window (B = S = T) cycles share no device state and each lane also has its
+ * own bins-first planes or time-window ring and PCA buffers; otherwise the
+ * lanes share the ring: a cycle's propagation waits until the previous
+ * cycle's time transform has read it, and only the image post-processing
+ * (unwrap, percentile, contrast) overlaps the next cycle (eager launches,
+ * no graphs).  Only for a pipe that owns its streams (created without a
+ * stream).  Outputs (hv_pipe_output, hv_pipe_contrast_range) are the latest
+ * cycle's; call hv_pipe_sync before reading them.  A frozen autocontrast
+ * range reaches every lane in stream order.  Changing the path
+ * (hv_pipe_set_fused) returns to one lane; the frames a lane reads must stay
+ * unchanged until hv_pipe_sync. */
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
 
 /* No reference counterpart (measurement).  Mean milliseconds of one launch of
