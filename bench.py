@@ -60,6 +60,10 @@ def parse():
     ap.add_argument("--serving", action="store_true",
                     help="serving mix instead: four sources, Poisson batch arrivals, HBM-budget admission")
     ap.add_argument("--seconds", type=float, default=20.0, help="serving run length")
+    ap.add_argument("--holo-io", action="store_true",
+                    help=".holo recording I/O: file (page cache) -> HBM and HBM -> file throughput")
+    ap.add_argument("--holo-frames", type=int, default=256, help="--holo-io: 2048x2048 u16 frames in the file")
+    ap.add_argument("--holo-dir", default="/tmp", help="--holo-io: directory of the recording")
     ap.add_argument("--rate-scale", type=float, default=1.0, help="serving: multiply every source's batch rate")
     ap.add_argument("--profile-stages", action="store_true",
                     help="with --profile-steps: after the steps, time the stages and print (for rocprofv3)")
@@ -240,10 +244,111 @@ def serving(args):
     print(json.dumps(line))
 
 
+def holo_io(args):
+    """A 2048x2048 u16 recording (c5 frame size) streamed from the page cache
+    into HBM (hv_holo_load_frames) and recorded from HBM into a file
+    (hv_holo_store_frames); the bound is the host<->device link, measured
+    here as a pinned hipMemcpy of the same size."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from holovibes_amd import holo_file as hfm
+    from holovibes_amd.pipe import hip
+
+    torch.cuda.set_device(0)
+    n, H, W = args.holo_frames, 2048, 2048
+    fsz = H * W * 2
+    total = n * fsz
+    path = os.path.join(args.holo_dir, f"hv_bench_{os.getpid()}.holo")
+    out_path = path + ".out"
+    rng = np.random.default_rng(7)
+    batch = rng.integers(0, 4096, (16, H, W), dtype=np.uint16)
+    with hfm.OutputHoloFile(path, hfm.FrameDescriptor(W, H, 2), n, hfm.RAW) as out:
+        out.write_header()
+        for k in range(0, n, 16):
+            out.write_frame(batch[: min(16, n - k)])
+    dev = torch.empty((n, fsz), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    reps = max(1, args.steps // 40)
+
+    def timed(fn):
+        fn()  # warm: page cache, pinned staging
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    f = hfm.InputHoloFile(path)
+
+    def load():
+        f.set_pos_to_frame(0)
+        assert f.read_frames_to_device(dev, n, stream) == n
+
+    def store():
+        # a fresh file each time (freeing the previous one's pages is not timed)
+        if os.path.exists(out_path):
+            os.remove(out_path)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with hfm.OutputHoloFile(out_path, hfm.FrameDescriptor(W, H, 2), n, hfm.PROCESSED) as o:
+            o.write_header()
+            o.write_frames_from_device(dev, n, stream)
+        return time.perf_counter() - t0
+
+    t_load = timed(load)
+    store()
+    t_store = sum(store() for _ in range(reps)) / reps
+    assert open(out_path, "rb").read(64 + 4096)[64:] == open(path, "rb").read(64 + 4096)[64:]
+    f.close()
+    # the link: one pinned copy of the same size each way
+    pinned = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+    h = hip()
+
+    def copy(kind):
+        src, dst = (pinned, dev) if kind == 1 else (dev, pinned)
+        e = h.hipMemcpyAsync(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), total, kind,
+                             ctypes.c_void_p(stream.cuda_stream))
+        assert e == 0
+
+    t_h2d = timed(lambda: copy(1))
+    t_d2h = timed(lambda: copy(2))
+    del pinned
+    # the oracle's read of the same file: one thread, Python buffered I/O
+    t0 = time.perf_counter()
+    with open(path, "rb") as fh:
+        fh.seek(64)
+        data = fh.read(total)
+    t_cpu = time.perf_counter() - t0
+    assert len(data) == total
+    del data
+    os.remove(path)
+    os.remove(out_path)
+    gb = total / 1e9
+    line = {"metric": ".holo ingest: recording (page cache) -> HBM", "value": gb / t_load, "unit": "GB/s",
+            "n_gpus": 1, "steps": reps, "higher_is_better": True, "data": "synthetic 12-bit frames",
+            "ms_per_step": t_load * 1e3,
+            "record_GBps": gb / t_store,
+            "roofline": {"bound": "host-device link", "achieved": gb / t_load, "peak": gb / t_h2d, "unit": "GB/s",
+                         "frac": t_h2d / t_load, "peak_source": "pinned hipMemcpy H2D of the same bytes",
+                         "record_peak": gb / t_d2h, "record_frac": t_d2h / t_store},
+            "cpu_baseline": {"value": gb / t_cpu, "unit": "GB/s", "cores": 1, "kind": "port",
+                             "sample": f"one buffered read of the {gb:.2f} GB of frames into host memory"},
+            "config": {"workload": "holo_io", "frames": n, "width": W, "height": H, "depth_bytes": 2,
+                       "bytes": total}}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.serving:
         serving(args)
+        return
+    if args.holo_io:
+        holo_io(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the pipe shards by source, "

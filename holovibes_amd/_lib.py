@@ -19,15 +19,36 @@ HV_ERR_OUT_OF_MEMORY = 3
 HV_ERR_HIP = 4
 HV_ERR_BUDGET = 5
 HV_ERR_NOT_READY = 6
+HV_ERR_IO = 7
+HV_ERR_FORMAT = 8
 
 _STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "UNSUPPORTED", 3: "OUT_OF_MEMORY", 4: "HIP", 5: "BUDGET",
-                 6: "NOT_READY"}
+                 6: "NOT_READY", 7: "IO", 8: "FORMAT"}
 
 
 class HoloError(RuntimeError):
     def __init__(self, status: int, fn: str, msg: str):
         super().__init__(f"{fn} -> HV_ERR_{_STATUS_NAMES.get(status, status)}: {msg}")
         self.status = status
+
+
+class HoloHeader(ctypes.Structure):
+    """hv_holo_header: the 64-byte .holo header, packed on 2 bytes."""
+    _pack_ = 2
+    _fields_ = [("magic_number", ctypes.c_char * 4), ("version", ctypes.c_uint16), ("bits_per_pixel", ctypes.c_uint16),
+                ("img_width", ctypes.c_uint32), ("img_height", ctypes.c_uint32), ("img_nb", ctypes.c_uint32),
+                ("total_data_size", ctypes.c_uint64), ("endianness", ctypes.c_uint8), ("data_type", ctypes.c_uint8),
+                ("padding", ctypes.c_char * 34)]
+
+
+class HoloInfo(ctypes.Structure):
+    """hv_holo_info: what the reader derives from a .holo header."""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("img_nb", ctypes.c_uint32), ("version", ctypes.c_uint16), ("bits_per_pixel", ctypes.c_uint16),
+                ("endianness", ctypes.c_uint8), ("data_type", ctypes.c_uint8), ("truncated", ctypes.c_uint8),
+                ("reserved", ctypes.c_uint8), ("frame_size", ctypes.c_uint64), ("total_data_size", ctypes.c_uint64),
+                ("frames_in_file", ctypes.c_uint64), ("footer_offset", ctypes.c_uint64),
+                ("footer_size", ctypes.c_uint64)]
 
 
 class SynthParams(ctypes.Structure):
@@ -130,6 +151,21 @@ SIGNATURES = {
     "hv_server_output": [P, I, P, P],
     "hv_server_memory_bytes": [P, P],
     "hv_server_destroy": [P],
+    "hv_holo_make_header": [U32, U32, U32, U64, I, P],
+    "hv_holo_open": [ctypes.c_char_p, P],
+    "hv_holo_get_info": [P, P],
+    "hv_holo_get_header": [P, P],
+    "hv_holo_read_frames": [P, U64, U64, P, P],
+    "hv_holo_load_frames": [P, U64, U64, P, P, P],
+    "hv_holo_read_footer": [P, P, SZ, P],
+    "hv_holo_close": [P],
+    "hv_holo_create": [ctypes.c_char_p, U32, U32, U32, U64, I, P],
+    "hv_holo_write_header": [P],
+    "hv_holo_write_frames": [P, P, U64],
+    "hv_holo_store_frames": [P, P, U64, P],
+    "hv_holo_write_footer": [P, P, SZ],
+    "hv_holo_correct_number_of_frames": [P, U64],
+    "hv_holo_writer_close": [P],
 }
 _RESTYPES = {"hv_version": ctypes.c_char_p, "hv_last_error": ctypes.c_char_p, "hv_default_settings": None}
 

@@ -40,6 +40,8 @@ typedef int hv_status;
 #define HV_ERR_HIP 4          /* a HIP runtime call failed */
 #define HV_ERR_BUDGET 5       /* admission refused: HBM budget exhausted */
 #define HV_ERR_NOT_READY 6    /* no result available yet */
+#define HV_ERR_IO 7           /* a file could not be opened, read or written */
+#define HV_ERR_FORMAT 8       /* not a valid .holo file (header checks of the reference) */
 
 /* camera::PixelDepth (Camera/include/frame_desc.hh:13-24), subset on the path */
 #define HV_DEPTH_8 1
@@ -439,6 +441,105 @@ typedef struct hv_synth_params {
 /* Writes frames [first_frame, first_frame + count) to host memory `out`
  * (count * W * H * depth bytes).  Runs on host threads. */
 hv_status hv_synth_frames(const hv_synth_params* p, uint64_t first_frame, uint32_t count, void* out);
+
+/* ------------------------------------------------------------------------ */
+/* .holo recordings (Backend/includes/io/holo_file.hh:19-102,               */
+/* Backend/sources/io/input_file/input_holo_file.cc:19-167,                 */
+/* Backend/sources/io/output_file/output_holo_file.cc:10-122)               */
+/* ------------------------------------------------------------------------ */
+
+/* holovibes::RecordedDataType (Backend/includes/enum/enum_recorded_data_type.hh:15-21) */
+#define HV_HOLO_RAW 0
+#define HV_HOLO_PROCESSED 1
+#define HV_HOLO_MOMENTS 2
+
+/* HoloFile::HoloFileHeader (holo_file.hh:58-83): 64 bytes, packed on 2
+ * bytes, little-endian; the frames follow it, then the JSON footer. */
+#pragma pack(push, 2)
+typedef struct hv_holo_header {
+    char magic_number[4];     /* "HOLO" */
+    uint16_t version;         /* 7 when written here */
+    uint16_t bits_per_pixel;  /* 8 * bytes per pixel */
+    uint32_t img_width;
+    uint32_t img_height;
+    uint32_t img_nb;
+    uint64_t total_data_size; /* img_width * img_height * img_nb * bits_per_pixel / 8 */
+    uint8_t endianness;       /* 0 little, 1 big (recorded, not applied, as in the reference) */
+    uint8_t data_type;        /* HV_HOLO_* (version 7) */
+    char padding[34];
+} hv_holo_header;
+#pragma pack(pop)
+
+/* What InputHoloFile derives from the header (input_holo_file.cc:19-43, 82-90). */
+typedef struct hv_holo_info {
+    uint32_t width, height;
+    uint32_t depth;            /* bytes per pixel = bits_per_pixel / 8 */
+    uint32_t img_nb;
+    uint16_t version, bits_per_pixel;
+    uint8_t endianness;
+    uint8_t data_type;         /* header data_type for version 7, HV_HOLO_RAW before */
+    uint8_t truncated;         /* the file ends before the frames the header announces */
+    uint8_t reserved;
+    uint64_t frame_size;       /* width * height * depth */
+    uint64_t total_data_size;
+    uint64_t frames_in_file;   /* whole frames present (img_nb unless truncated) */
+    uint64_t footer_offset;    /* 64 + total_data_size */
+    uint64_t footer_size;      /* bytes after the frames (0: no footer) */
+} hv_holo_info;
+
+typedef struct hv_holo_reader hv_holo_reader;
+typedef struct hv_holo_writer hv_holo_writer;
+
+/* Fills a version-7 header as OutputHoloFile's constructor does
+ * (output_holo_file.cc:10-34): depth in bytes, data_type HV_HOLO_*. */
+hv_status hv_holo_make_header(uint32_t width, uint32_t height, uint32_t depth, uint64_t img_nb, int data_type,
+                              hv_holo_header* out);
+
+/* InputHoloFile::InputHoloFile + load_header + load_fd (input_holo_file.cc:19-90).
+ * HV_ERR_IO when the file cannot be opened; HV_ERR_FORMAT with the
+ * reference's messages for a short header or a wrong magic ("invalid
+ * bytecode size") and for total_data_size != frame_size * img_nb ("invalid
+ * total data size"). */
+hv_status hv_holo_open(const char* path, hv_holo_reader** out);
+hv_status hv_holo_get_info(const hv_holo_reader* r, hv_holo_info* info);
+hv_status hv_holo_get_header(const hv_holo_reader* r, hv_holo_header* h);
+
+/* set_pos_to_frame + read_frames (input_holo_file.cc:45-51,
+ * Backend/sources/io/input_file/input_frame_file.cc:7-17) into host memory:
+ * reads min(count, frames_in_file - first) frames, never footer bytes;
+ * first > frames_in_file is HV_ERR_INVALID_ARG.  Blocking. */
+hv_status hv_holo_read_frames(hv_holo_reader* r, uint64_t first, uint64_t count, void* host_dst,
+                              uint64_t* frames_read);
+
+/* FileFrameReadWorker::read_copy_file (Backend/sources/thread/file_frame_read_worker.cc:197-271):
+ * the same frames straight into device memory through two pinned 32 MiB
+ * staging chunks (file reads overlap the DMA of the previous chunk).
+ * Returns when the frames are in device memory.  Allocates the reader's
+ * pinned staging on first use; no device allocation. */
+hv_status hv_holo_load_frames(hv_holo_reader* r, uint64_t first, uint64_t count, void* dev_dst, void* stream,
+                              uint64_t* frames_read);
+
+/* The raw footer bytes (input_holo_file.cc:92-133; JSON parsing is the
+ * caller's).  dst NULL: *size = footer size only.  HV_ERR_FORMAT "invalid
+ * footer" for a truncated file. */
+hv_status hv_holo_read_footer(hv_holo_reader* r, char* dst, size_t capacity, size_t* size);
+hv_status hv_holo_close(hv_holo_reader* r);
+
+/* OutputHoloFile (output_holo_file.cc:10-122).  create truncates the file and
+ * fills the header (not yet written); each write_* appends at the cursor. */
+hv_status hv_holo_create(const char* path, uint32_t width, uint32_t height, uint32_t depth, uint64_t img_nb,
+                         int data_type, hv_holo_writer** out);
+hv_status hv_holo_write_header(hv_holo_writer* w);
+/* write_frame for count frames in host memory. */
+hv_status hv_holo_write_frames(hv_holo_writer* w, const void* host_src, uint64_t count);
+/* The record path's device -> file copy: count frames in device memory,
+ * through two pinned staging chunks (the file write of one chunk overlaps
+ * the DMA of the next).  Returns when the bytes are in the file. */
+hv_status hv_holo_store_frames(hv_holo_writer* w, const void* dev_src, uint64_t count, void* stream);
+hv_status hv_holo_write_footer(hv_holo_writer* w, const char* json, size_t size);
+/* Rewrites img_nb and total_data_size in the header at offset 0; the cursor stays. */
+hv_status hv_holo_correct_number_of_frames(hv_holo_writer* w, uint64_t frames_written);
+hv_status hv_holo_writer_close(hv_holo_writer* w);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: several sources, admission by HBM budget, latency accounting     */
