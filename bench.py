@@ -64,6 +64,8 @@ def parse():
                     help=".holo recording I/O: file (page cache) -> HBM and HBM -> file throughput")
     ap.add_argument("--holo-frames", type=int, default=256, help="--holo-io: 2048x2048 u16 frames in the file")
     ap.add_argument("--holo-dir", default="/tmp", help="--holo-io: directory of the recording")
+    ap.add_argument("--input-side", action="store_true",
+                    help="packed 12/10-bit unpack (HBM roofline) and input-queue enqueue throughput")
     ap.add_argument("--rate-scale", type=float, default=1.0, help="serving: multiply every source's batch rate")
     ap.add_argument("--profile-stages", action="store_true",
                     help="with --profile-steps: after the steps, time the stages and print (for rocprofv3)")
@@ -342,10 +344,89 @@ def holo_io(args):
     print(json.dumps(line))
 
 
+def input_side(args):
+    """Unpack of a 128-frame 2048x2048 batch packed on 12 and 10 bits
+    (algorithmic HBM bytes: bits/8 read + 2 written per pixel) and the batch
+    input queue filled from pinned host memory with 16-bit frames and with
+    12-bit packed frames (unpacked on arrival)."""
+    import numpy as np
+    import torch
+
+    from holovibes_amd import steps
+    from holovibes_amd.input_queue import BatchInputQueue
+    from oracle import input_side as ins
+
+    torch.cuda.set_device(0)
+    F, H, W = 128, 2048, 2048
+    P = F * H * W
+    reps = max(5, args.steps // 10)
+    res = {}
+    for bits in (12, 10):
+        packed = torch.randint(0, 256, (P * bits // 8,), dtype=torch.uint8, device="cuda")
+        out = torch.empty(P, dtype=torch.int16, device="cuda")
+        for _ in range(3):
+            steps.unpack(packed, P, bits, out=out)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            steps.unpack(packed, P, bits, out=out)
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / reps
+        algo = P * (bits / 8 + 2)
+        res[bits] = {"us_per_launch": t * 1e6, "GBps": algo / t / 1e9, "frac": algo / t / 8e12,
+                     "bytes": int(algo), "frames_per_s": F / t}
+        del packed, out
+    # oracle unpack of a bounded sample, one host thread
+    sample = np.random.default_rng(1).integers(0, 256, 8 * H * W * 3 // 2, dtype=np.uint8)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < min(args.cpu_seconds, 5.0):
+        ins.unpack12(sample, 8 * H * W)
+        n += 8
+    cpu_fps = n / (time.perf_counter() - t0)
+    # queue fill from pinned host memory: 16-bit frames, then 12-bit packed frames
+    B = 32
+    q = BatchInputQueue(4 * B, B, W, H, 2)
+    host16 = torch.empty((B, H, W), dtype=torch.int16, pin_memory=True)
+    host12 = torch.empty((B * H * W * 3 // 2,), dtype=torch.uint8, pin_memory=True)
+    stream = torch.cuda.current_stream()
+
+    def fill(fn):
+        for _ in range(4):
+            fn()
+            q.pop()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+            q.pop()
+        torch.cuda.synchronize()
+        return B * reps / (time.perf_counter() - t0)
+
+    fps16 = fill(lambda: q.enqueue(host16.numpy(), B, stream))
+    fps12 = fill(lambda: q.enqueue_packed(host12.numpy(), B, 12, stream))
+    q.close()
+    line = {"metric": "packed 12-bit unpack: 2048x2048 frames/s (128-frame batch)", "value": res[12]["frames_per_s"],
+            "unit": "frames/s", "n_gpus": 1, "steps": reps, "higher_is_better": True, "dtype": "u16",
+            "data": "synthetic packed bytes",
+            "roofline": {"bound": "hbm", "achieved": res[12]["GBps"], "peak": 8000.0, "unit": "GB/s",
+                         "frac": res[12]["frac"], "algorithmic_bytes": res[12]["bytes"]},
+            "unpack10": res[10], "unpack12": res[12],
+            "queue_fill_frames_per_s": {"u16_from_pinned_host": fps16, "packed12_from_pinned_host": fps12},
+            "cpu_baseline": {"value": cpu_fps, "unit": "frames/s", "cores": 1, "kind": "port",
+                             "sample": "numpy restatement of the 12-bit unpack on 8-frame chunks, 5 s"},
+            "config": {"workload": "input_side", "frames": F, "width": W, "height": H}}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.serving:
         serving(args)
+        return
+    if args.input_side:
+        input_side(args)
         return
     if args.holo_io:
         holo_io(args)

@@ -151,6 +151,19 @@ SIGNATURES = {
     "hv_server_output": [P, I, P, P],
     "hv_server_memory_bytes": [P, P],
     "hv_server_destroy": [P],
+    "hv_unpack_12_to_16bit": [P, SZ, P, SZ, P],
+    "hv_unpack_10_to_16bit": [P, SZ, P, SZ, P],
+    "hv_input_queue_create": [U32, U32, U32, U32, U32, P],
+    "hv_input_queue_enqueue": [P, P, I, U32, P],
+    "hv_input_queue_enqueue_packed": [P, P, I, I, U32, P],
+    "hv_input_queue_front": [P, P, I, P],
+    "hv_input_queue_pop": [P, P],
+    "hv_input_queue_dequeue": [P, P, I, P],
+    "hv_input_queue_state": [P, P, P, P, P, P],
+    "hv_input_queue_data": [P, P],
+    "hv_input_queue_resize": [P, U32],
+    "hv_input_queue_reset": [P],
+    "hv_input_queue_destroy": [P],
     "hv_holo_make_header": [U32, U32, U32, U64, I, P],
     "hv_holo_open": [ctypes.c_char_p, P],
     "hv_holo_get_info": [P, P],

@@ -264,3 +264,13 @@ def contrast_to_ushort(img: torch.Tensor, range_dev: torch.Tensor, invert: bool 
 def u16(t: torch.Tensor):
     """int16-carried u16 tensor -> numpy uint16."""
     return t.cpu().numpy().view("uint16")
+
+
+def unpack(packed: torch.Tensor, pixels: int, bits: int, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """unpack_12_to_16bit / unpack_10_to_16bit (unpack.cu:34-60): `pixels`
+    16-bit values (carried in an int16 tensor) from a uint8 packed stream."""
+    if out is None:
+        out = torch.empty(pixels, dtype=torch.int16, device=packed.device)
+    fn = "hv_unpack_12_to_16bit" if bits == 12 else "hv_unpack_10_to_16bit"
+    _lib.call(fn, _p(out), pixels, _p(packed), packed.numel(), _s(stream))
+    return out

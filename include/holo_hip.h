@@ -443,6 +443,78 @@ typedef struct hv_synth_params {
 hv_status hv_synth_frames(const hv_synth_params* p, uint64_t first_frame, uint32_t count, void* out);
 
 /* ------------------------------------------------------------------------ */
+/* Input side: packed-pixel unpack and the batch input queue                */
+/* ------------------------------------------------------------------------ */
+
+/* unpack_12_to_16bit / unpack_10_to_16bit (Backend/cuda_sources/unpack.cu:34-60):
+ * output_size u16 pixels from a stream packed MSB first; each output holds
+ * the value in its top bits (12-bit: v << 4, 10-bit: v << 6), as the
+ * reference writes it.  output_size must be a multiple of 2 (12-bit) or 4
+ * (10-bit) and input_size must cover output_size * bits / 8 bytes
+ * (HV_ERR_INVALID_ARG otherwise; the reference leaves such pixels
+ * unwritten).  input and output must not overlap.  Any number of frames in
+ * one call (the frames' packed streams are contiguous). */
+hv_status hv_unpack_12_to_16bit(uint16_t* output, size_t output_size, const uint8_t* input, size_t input_size,
+                                void* stream);
+hv_status hv_unpack_10_to_16bit(uint16_t* output, size_t output_size, const uint8_t* input, size_t input_size,
+                                void* stream);
+
+typedef struct hv_input_queue hv_input_queue;
+
+/* BatchInputQueue (Backend/sources/data_struct/batch_input_queue.cc:8-59):
+ * room for frame_capacity - frame_capacity % batch_size frames of
+ * width * height * depth bytes in one HBM allocation (made here). */
+hv_status hv_input_queue_create(uint32_t frame_capacity, uint32_t batch_size, uint32_t width, uint32_t height,
+                                uint32_t depth, hv_input_queue** out);
+
+/* BatchInputQueue::enqueue (batch_input_queue.cc:124-187): appends nb_frames
+ * frames (host memory: memcpy_kind 1 = hipMemcpyHostToDevice, pinned for an
+ * asynchronous copy; device memory: 3 = hipMemcpyDeviceToDevice) on `stream`.
+ * A batch becomes visible when complete; when the queue is full the oldest
+ * batch is overwritten (has_overwritten).  Waits on the host only while the
+ * consumer holds the slot to overwrite (HV_ERR_INVALID_ARG if the caller's
+ * own thread holds it). */
+hv_status hv_input_queue_enqueue(hv_input_queue* q, const void* frames, int memcpy_kind, uint32_t nb_frames,
+                                 void* stream);
+
+/* The packed-file read (Backend/sources/thread/file_frame_read_worker.cc:212-240)
+ * as an enqueue: nb_frames frames packed on `bits` (10 or 12) bits, from the
+ * host (copied into a one-batch device staging buffer, allocated on first
+ * use) or the device, unpacked straight into the queue's 16-bit slots. */
+hv_status hv_input_queue_enqueue_packed(hv_input_queue* q, const void* packed_frames, int memcpy_kind, int bits,
+                                        uint32_t nb_frames, void* stream);
+
+/* The oldest whole batch, in place (batch_size frames in HBM), for
+ * hv_pipe_push_batch: `stream` waits for the producer's copy.  Waits up to
+ * timeout_ms on the host for one (0: no wait); HV_ERR_NOT_READY when there
+ * is none.  The batch stays the consumer's until hv_input_queue_pop. */
+hv_status hv_input_queue_front(hv_input_queue* q, void* stream, int timeout_ms, const void** batch);
+
+/* Releases the batch front returned once the work queued on `stream` so far
+ * has run (the producer's overwrite of the slot waits for it); without a
+ * held batch, drops the oldest one like BatchInputQueue::dequeue()
+ * (batch_input_queue.cc:205-213). */
+hv_status hv_input_queue_pop(hv_input_queue* q, void* stream);
+
+/* BatchInputQueue::dequeue(dest, ...) (batch_input_queue.cc:189-203) with a
+ * plain copy: front + copy (memcpy_kind 3 device, 2 host) + pop, then waits
+ * for `stream`.  HV_ERR_NOT_READY when the queue has no whole batch. */
+hv_status hv_input_queue_dequeue(hv_input_queue* q, void* dst, int memcpy_kind, void* stream);
+
+/* get_size (whole batches), max_size, start index, frames held, has_overwritten. */
+hv_status hv_input_queue_state(hv_input_queue* q, uint32_t* size, uint32_t* max_size, uint32_t* start_index,
+                               uint32_t* curr_nb_frames, int* has_overwritten);
+/* get_data: the HBM allocation (slot k at k * batch_size * frame size). */
+hv_status hv_input_queue_data(hv_input_queue* q, const void** data);
+
+/* BatchInputQueue::resize (batch_input_queue.cc:241-266): new batch size,
+ * same frame capacity, queue emptied. */
+hv_status hv_input_queue_resize(hv_input_queue* q, uint32_t new_batch_size);
+/* make_empty / reset_override (batch_input_queue.cc:88-106). */
+hv_status hv_input_queue_reset(hv_input_queue* q);
+hv_status hv_input_queue_destroy(hv_input_queue* q);
+
+/* ------------------------------------------------------------------------ */
 /* .holo recordings (Backend/includes/io/holo_file.hh:19-102,               */
 /* Backend/sources/io/input_file/input_holo_file.cc:19-167,                 */
 /* Backend/sources/io/output_file/output_holo_file.cc:10-122)               */

@@ -7,6 +7,14 @@ unit tests that pin this path, transcribed as inputs and expected outputs.
   reduce      Backend/tests/test_reduce.cc:13-85, 91-464 (data patterns,
               sizes and the exact expected min / max / sum)
   map         Backend/tests/test_map.cc:72-98 (sizes and factor 5)
+  batch_input_queue
+              Backend/tests/test_batch_input_queue.cc:48-287: frame capacity,
+              batch size, frame width (height 1, 8-bit), the frames (C strings
+              padded with NUL to the frame size), then the operations with
+              the size expected after each: ["enq", frame], ["deq", expected
+              frames or null when the test does not check them], ["resize",
+              new batch size], ["elt", position, expected frame]; the size
+              is null where the test does not check it
 """
 import json
 import os
@@ -38,6 +46,41 @@ kats = {
         + [{"pattern": "mod1000p1", "op": "min", "width": 1024, "height": 1024, "expected": 1},
            {"pattern": "neg_mod1000p1", "op": "max", "width": 1024, "height": 1024, "expected": -1}]),
     "map": [{"size": 103, "value": 5.0}, {"size": 512 * 1024, "value": 5.0}],
+    "batch_input_queue": [
+        {"name": "SimpleInstantiation", "capacity": 3, "batch": 1, "width": 2, "height": 2, "frames": [],
+         "ops": [["size", 0]]},
+        {"name": "SimpleEnqueueOfThreeElements", "capacity": 3, "batch": 1, "width": 2, "height": 1,
+         "frames": ["a", "b", "c", "d", "e"],
+         "ops": [["enq", 0, None], ["enq", 1, None], ["enq", 2, 3], ["elt", 0, "a", 3], ["elt", 1, "b", 3],
+                 ["elt", 2, "c", 3]]},
+        {"name": "SimpleEnqueueAndDequeueOfThreeElements", "capacity": 3, "batch": 1, "width": 2, "height": 1,
+         "frames": ["a", "b", "c", "d", "e"],
+         "ops": [["enq", 0, None], ["enq", 1, None], ["enq", 2, None], ["deq", ["a"], 2], ["deq", ["b"], 1],
+                 ["deq", ["c"], 0]]},
+        {"name": "SimpleOverwriteElements", "capacity": 3, "batch": 1, "width": 2, "height": 1,
+         "frames": ["a", "b", "c", "d", "e"],
+         "ops": [["enq", 0, None], ["enq", 1, None], ["enq", 2, None], ["enq", 3, None], ["enq", 4, 3],
+                 ["deq", ["c"], 2],
+                 ["deq", ["d"], 1], ["deq", ["e"], 0]]},
+        {"name": "SimpleOverwriteMoreElements", "capacity": 4, "batch": 2, "width": 4, "height": 1,
+         "frames": ["abc", "ABC", "def", "DEF", "ghi", "GHI"],
+         "ops": [["enq", 0, 0], ["enq", 1, 1], ["enq", 2, 1], ["enq", 3, 2], ["deq", ["abc", "ABC"], 1],
+                 ["enq", 4, 1], ["enq", 5, 2], ["enq", 0, 2], ["enq", 1, 2], ["deq", ["ghi", "GHI"], 1],
+                 ["deq", ["abc", "ABC"], 0]]},
+        {"name": "SimpleResizeSame", "capacity": 4, "batch": 2, "width": 5, "height": 1,
+         "frames": ["ilan", "nico", "anto", "kaci", "theo"],
+         "ops": [["enq", 0, None], ["enq", 1, None], ["enq", 2, None], ["enq", 3, None], ["deq", None, 1],
+                 ["resize", 2, 0], ["enq", 4, None], ["enq", 0, 1], ["deq", None, 0]]},
+        {"name": "SimpleResizeGreater", "capacity": 4, "batch": 2, "width": 5, "height": 1,
+         "frames": ["ilan", "nico", "anto", "kaci", "theo"],
+         "ops": [["enq", 0, None], ["enq", 1, None], ["enq", 2, None], ["enq", 3, None], ["deq", None, 1],
+                 ["resize", 4, 0], ["enq", 4, None], ["enq", 0, None], ["enq", 2, None], ["enq", 3, 1],
+                 ["deq", None, 0]]},
+        {"name": "SimpleResizeLower", "capacity": 4, "batch": 2, "width": 5, "height": 1,
+         "frames": ["ilan", "nico", "anto", "kaci", "theo"],
+         "ops": [["enq", 0, None], ["enq", 1, None], ["enq", 2, None], ["enq", 3, None], ["deq", None, 1],
+                 ["resize", 1, 0], ["enq", 4, 1], ["deq", None, 0]]},
+    ],
 }
 for c in kats["reduce"]:
     if c["op"] == "sum":
