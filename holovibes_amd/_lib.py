@@ -51,6 +51,26 @@ class HoloInfo(ctypes.Structure):
                 ("footer_size", ctypes.c_uint64)]
 
 
+class CapturedFrames(ctypes.Structure):
+    """hv_captured_frames: up to two regions of frames from a source."""
+    _fields_ = [("region1", ctypes.c_void_p), ("count1", ctypes.c_uint32), ("region2", ctypes.c_void_p),
+                ("count2", ctypes.c_uint32), ("on_gpu", ctypes.c_int)]
+
+
+CAM_STATUS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+CAM_FRAMES_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(CapturedFrames))
+CAM_FD_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32),
+                             ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32))
+CAM_FPS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+
+
+class Camera(ctypes.Structure):
+    """hv_camera: the camera interface as a C function table."""
+    _fields_ = [("self", ctypes.c_void_p), ("init_camera", CAM_STATUS_FN), ("start_acquisition", CAM_STATUS_FN),
+                ("stop_acquisition", CAM_STATUS_FN), ("shutdown_camera", CAM_STATUS_FN),
+                ("get_frames", CAM_FRAMES_FN), ("get_fd", CAM_FD_FN), ("get_camera_fps", CAM_FPS_FN)]
+
+
 class SynthParams(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("depth", ctypes.c_uint32),
                 ("bits", ctypes.c_uint32), ("period", ctypes.c_uint32), ("sigma", ctypes.c_float),
@@ -164,6 +184,11 @@ SIGNATURES = {
     "hv_input_queue_resize": [P, U32],
     "hv_input_queue_reset": [P],
     "hv_input_queue_destroy": [P],
+    "hv_camera_synthetic": [P, U32, U32, P],
+    "hv_camera_holo_file": [ctypes.c_char_p, U64, U64, U32, P],
+    "hv_reader_start": [P, P, F, P],
+    "hv_reader_frames": [P, P],
+    "hv_reader_stop": [P, P],
     "hv_holo_make_header": [U32, U32, U32, U64, I, P],
     "hv_holo_open": [ctypes.c_char_p, P],
     "hv_holo_get_info": [P, P],

@@ -514,6 +514,56 @@ hv_status hv_input_queue_resize(hv_input_queue* q, uint32_t new_batch_size);
 hv_status hv_input_queue_reset(hv_input_queue* q);
 hv_status hv_input_queue_destroy(hv_input_queue* q);
 
+/* CapturedFramesDescriptor (Camera/include/icamera.hh:8-58): up to two
+ * regions of frames (the end and the start of a camera's ring buffer). */
+typedef struct hv_captured_frames {
+    void* region1;
+    uint32_t count1;
+    void* region2;
+    uint32_t count2;
+    int on_gpu;  /* 1: device memory */
+} hv_captured_frames;
+
+/* ICamera (Camera/include/icamera.hh:91-166) as a C function table: any
+ * camera plugin with a C ABI fills one.  get_frames' regions stay valid
+ * until the next get_frames call. */
+typedef struct hv_camera {
+    void* self;
+    hv_status (*init_camera)(void* self);
+    hv_status (*start_acquisition)(void* self);
+    hv_status (*stop_acquisition)(void* self);
+    hv_status (*shutdown_camera)(void* self);  /* also frees the source */
+    hv_status (*get_frames)(void* self, hv_captured_frames* out);
+    hv_status (*get_fd)(void* self, uint32_t* width, uint32_t* height, uint32_t* depth);
+    int (*get_camera_fps)(void* self);         /* -1 when unknown */
+} hv_camera;
+
+/* A synthetic camera: ring_frames frames of hv_synth_frames generated into
+ * pinned memory at init_camera, replayed frames_per_call at a time. */
+hv_status hv_camera_synthetic(const hv_synth_params* p, uint32_t ring_frames, uint32_t frames_per_call,
+                              hv_camera* out);
+
+/* A .holo recording as a source (FileFrameReadWorker,
+ * Backend/sources/thread/file_frame_read_worker.cc:35-312): frames
+ * [first_frame, end_frame) (end 0: to the end), frames_per_call at a time
+ * (the reference's file buffer size), from first_frame again at the end. */
+hv_status hv_camera_holo_file(const char* path, uint64_t first_frame, uint64_t end_frame, uint32_t frames_per_call,
+                              hv_camera* out);
+
+typedef struct hv_reader hv_reader;
+
+/* The read worker (Backend/sources/thread/camera_frame_read_worker.cc:20-72):
+ * init_camera here, then a host thread that starts the acquisition and
+ * enqueues every get_frames into q on its own stream until stopped, at most
+ * fps_limit frames/s when fps_limit > 0.  The consumer must keep popping
+ * (a full queue waits for the batch the consumer holds). */
+hv_status hv_reader_start(const hv_camera* cam, hv_input_queue* q, float fps_limit, hv_reader** out);
+/* Frames enqueued so far. */
+hv_status hv_reader_frames(hv_reader* r, uint64_t* frames);
+/* Stops the thread (stop_acquisition), reports the frames enqueued and the
+ * first error the thread met.  The camera stays the caller's (shutdown_camera). */
+hv_status hv_reader_stop(hv_reader* r, uint64_t* frames);
+
 /* ------------------------------------------------------------------------ */
 /* .holo recordings (Backend/includes/io/holo_file.hh:19-102,               */
 /* Backend/sources/io/input_file/input_holo_file.cc:19-167,                 */
