@@ -404,8 +404,8 @@ def input_side(args):
         torch.cuda.synchronize()
         return B * reps / (time.perf_counter() - t0)
 
-    fps16 = fill(lambda: q.enqueue(host16.numpy(), B, stream))
-    fps12 = fill(lambda: q.enqueue_packed(host12.numpy(), B, 12, stream))
+    fps16 = fill(lambda: q.enqueue(host16.numpy(), B, stream, pinned=True))
+    fps12 = fill(lambda: q.enqueue_packed(host12.numpy(), B, 12, stream, pinned=True))
     q.close()
     line = {"metric": "packed 12-bit unpack: 2048x2048 frames/s (128-frame batch)", "value": res[12]["frames_per_s"],
             "unit": "frames/s", "n_gpus": 1, "steps": reps, "higher_is_better": True, "dtype": "u16",

@@ -92,7 +92,9 @@ class ComputeSettings(ctypes.Structure):
                 ("contrast_mode", ctypes.c_int32), ("contrast_min", ctypes.c_float), ("contrast_max", ctypes.c_float),
                 ("contrast_lower", ctypes.c_float), ("contrast_upper", ctypes.c_float),
                 ("contrast_subsample", ctypes.c_uint32), ("camera_fps", ctypes.c_float),
-                ("lens_precision", ctypes.c_int32), ("pca_precision", ctypes.c_int32)]
+                ("lens_precision", ctypes.c_int32), ("pca_precision", ctypes.c_int32),
+                ("renorm_enabled", ctypes.c_int32), ("renorm_constant", ctypes.c_uint32),
+                ("img_accu_level", ctypes.c_uint32), ("log_enabled", ctypes.c_int32)]
 
 
 P = ctypes.c_void_p

@@ -23,7 +23,7 @@ ARCH = os.environ.get("HV_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["hv_capi.cpp", "hv_pipe.cpp", "hv_synth.cpp", "hv_spatial.hip", "hv_time.hip", "hv_bins.hip", "hv_render.hip",
            "hv_pca.hip", "hv_holo.cpp", "hv_input.hip",
-           "hv_source.cpp"]
+           "hv_source.cpp", "hv_post.hip"]
 HEADERS = ["hv_common.hpp", "hv_fft.hpp", "hv_kernels.hpp", os.path.join("..", "..", "include", "holo_hip.h")]
 
 CFLAGS = [

@@ -98,4 +98,11 @@ hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, in
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1, float fps = 0.f);
 
+// hv_post.hip: post-processing of the float image (P a multiple of 4)
+size_t renorm_scratch_bytes();
+hv_status launch_renormalize(float* img, size_t P, unsigned norm_constant, void* scratch, hipStream_t st);
+// state: {next slot, images held}, zeroed when the ring is (re)started
+hv_status launch_accumulate(float* img, float* ring, uint32_t* state, int level, size_t P, hipStream_t st);
+hv_status launch_log10(float* img, size_t P, hipStream_t st);
+
 }  // namespace hv

@@ -70,6 +70,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 
 struct Layout {
     size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
+    size_t accu, renorm;
     size_t total;
 };
 
@@ -98,8 +99,11 @@ Layout layout_of(const hv_compute_settings& s) {
         l.pca_ws = align256(pca_workspace_bytes((int)T, P, s.pca_precision));
         l.eig_ws = align256(eig_workspace_bytes((int)T));
     }
+    // image accumulation ring (+ its position) and renormalisation sums
+    l.accu = s.img_accu_level > 1 ? align256((size_t)s.img_accu_level * P * sizeof(float)) + 256 : 0;
+    l.renorm = s.renorm_enabled ? align256(renorm_scratch_bytes()) : 0;
     l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
-              l.evals + l.evecs + l.pca_ws + l.eig_ws;
+              l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm;
     return l;
 }
 
@@ -169,6 +173,9 @@ struct hv_pipe {
     double2* evecs = nullptr;
     void* pca_ws = nullptr;
     void* eig_ws = nullptr;
+    float* accu_ring = nullptr;     // img_accu_level images
+    uint32_t* accu_state = nullptr; // {next slot, images held}
+    void* renorm_ws = nullptr;
     // host mirror of the reference's state
     uint32_t batch_index = 0;  // BatchEnv::batch_index
     uint32_t q_start = 0, q_size = 0;
@@ -197,9 +204,11 @@ struct hv_pipe {
     // cycle's first kernel (the row pass of a space transform)
     bool minmax_fused() const {
         return s.contrast_enabled && s.contrast_mode == HV_CONTRAST_MINMAX &&
-               !(s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d) && !s.fft_shift &&
+               !(s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d) && !s.fft_shift && !post_steps() &&
                s.space_transformation != HV_SPACE_NONE;
     }
+    // renormalisation, accumulation or log10 change the image after its kernel
+    bool post_steps() const { return s.renorm_enabled || s.img_accu_level > 1 || s.log_enabled; }
     hv_status mark(int i) {
         if (timing) HV_HIP(hipEventRecord(ev[i], lane[cur].stream));
         return HV_OK;
@@ -275,7 +284,13 @@ hv_status hv_pipe::image_work() {
         }
         if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
     }
+    // post-processing in the reference's order (pipe.cc:353-369): renormalise,
+    // fft shift, accumulate, log10
+    if (s.renorm_enabled) HV_TRY(launch_renormalize(L.image, P, s.renorm_constant, renorm_ws, L.stream));
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
+    if (s.img_accu_level > 1)
+        HV_TRY(launch_accumulate(L.image, accu_ring, accu_state, (int)s.img_accu_level, P, L.stream));
+    if (s.log_enabled) HV_TRY(launch_log10(L.image, P, L.stream));
     HV_TRY(mark(4));
     if (!s.contrast_enabled) return launch_float_to_ushort(L.out, L.image, P, L.stream);
     switch (s.contrast_mode) {
@@ -360,6 +375,12 @@ void hv_default_settings(hv_compute_settings* s) {
     s->camera_fps = 0.0f;
     s->lens_precision = HV_LENS_FP64;
     s->pca_precision = HV_PCA_BF16_SPLIT;
+    // the reference's GUI turns renormalisation on by default (holovibes.hh:399);
+    // the path's configurations do not use it, so it starts off here
+    s->renorm_enabled = 0;
+    s->renorm_constant = 5;
+    s->img_accu_level = 1;
+    s->log_enabled = 0;
 }
 
 hv_status hv_validate_settings(const hv_compute_settings* s) {
@@ -408,6 +429,8 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     if (s->contrast_subsample > 1 && (s->width % s->contrast_subsample || s->height % s->contrast_subsample))
         return fail(HV_ERR_INVALID_ARG, "contrast_subsample must divide the frame sides");
     if (s->fft_shift && (s->width % 2 || s->height % 2)) return fail(HV_ERR_UNSUPPORTED, "fft shift needs even sides");
+    if (s->renorm_enabled && s->renorm_constant > 31) return fail(HV_ERR_INVALID_ARG, "renorm_constant must be <= 31");
+    if (s->img_accu_level > 4096) return fail(HV_ERR_INVALID_ARG, "img_accu_level must be <= 4096");
     return HV_OK;
 }
 
@@ -467,6 +490,11 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->evecs = (double2*)take(l.evecs);
     p->pca_ws = take(l.pca_ws);
     p->eig_ws = take(l.eig_ws);
+    if (l.accu) {
+        p->accu_ring = (float*)take(l.accu - 256);
+        p->accu_state = (uint32_t*)take(256);
+    }
+    p->renorm_ws = take(l.renorm);
     L0.ring = p->ring;
     L0.cov = p->cov;
     L0.evals = p->evals;
@@ -480,6 +508,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
         const float r[2] = {s->contrast_min, s->contrast_max};
         e = hipMemcpyAsync(L0.range, r, sizeof(r), hipMemcpyHostToDevice, L0.stream);
         if (e == hipSuccess) e = hipMemsetAsync(p->ring, 0, l.ring, L0.stream);
+        if (e == hipSuccess && p->accu_state) e = hipMemsetAsync(p->accu_state, 0, 256, L0.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(L0.stream);
         if (e != hipSuccess) st = hip_fail(e, "pipe init");
     }
@@ -660,6 +689,8 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     if (lanes < 1 || lanes > kMaxLanes) return fail(HV_ERR_INVALID_ARG, "lanes must be in [1, 4]");
     if (lanes > 1 && !p->own_stream)
         return fail(HV_ERR_UNSUPPORTED, "several lanes need the pipe's own streams (create it without a stream)");
+    if (lanes > 1 && (p->s.renorm_enabled || p->s.img_accu_level > 1))
+        return fail(HV_ERR_UNSUPPORTED, "renormalisation and image accumulation run one cycle at a time");
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
     int pmin, pmax;

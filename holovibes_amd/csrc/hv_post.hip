@@ -1,0 +1,143 @@
+// hv_post.hip -- optional post-processing of the float image, between the
+// image kernel and the contrast (Backend/sources/core/pipe.cc:353-370):
+// renormalisation, image accumulation, log10.  Each is one HBM pass over
+// the image (float4 per thread); accumulation also reads the level - 1
+// previous images.  The accumulation ring's position lives in device memory,
+// so a captured cycle replays correctly.
+#include "hv_common.hpp"
+#include "hv_kernels.hpp"
+
+namespace {
+
+constexpr int kRenormParts = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    return t;  // valid in thread 0
+}
+
+// per-block double sums of the image (reduce_add, Backend/cuda_includes/reduce.cuh:207-221)
+__global__ void __launch_bounds__(256) k_renorm_partials(const float4* __restrict__ x, size_t n4,
+                                                         double* __restrict__ parts) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)gridDim.x * 256) {
+        float4 v = x[i];
+        acc += (double)v.x + (double)v.y + (double)v.z + (double)v.w;
+    }
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) parts[blockIdx.x] = t;
+}
+
+// total of the partials, then frame_res / total in float (tools_compute.cu:99-123)
+__global__ void __launch_bounds__(256) k_renorm_ratio(double* __restrict__ parts, int nparts, float frame_res) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) acc += parts[i];
+    double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) {
+        float* ratio = (float*)(parts + kRenormParts);
+        ratio[0] = frame_res / (float)t;
+    }
+}
+
+// x = x * 2^c * (frame_res / sum), evaluated left to right in float as the reference does
+__global__ void __launch_bounds__(256) k_renorm_apply(float4* __restrict__ x, size_t n4, const double* __restrict__ parts,
+                                                      float multiplier) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    const float r = ((const float*)(parts + kRenormParts))[0];
+    float4 v = x[i];
+    v.x = (v.x * multiplier) * r;
+    v.y = (v.y * multiplier) * r;
+    v.z = (v.z * multiplier) * r;
+    v.w = (v.w * multiplier) * r;
+    x[i] = v;
+}
+
+// The image joins the accumulation ring (Queue::enqueue, queue.cc:149-189)
+// and becomes the mean of the ring's images, summed from the oldest in float
+// (kernel_accumulate_images, tools_conversion.cu:418-441).  state = {next
+// slot, images held} before this image.
+__global__ void __launch_bounds__(256) k_accumulate(float4* __restrict__ img, float4* __restrict__ ring,
+                                                    const uint32_t* __restrict__ state, int level, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    const int slot = (int)state[0];
+    const int held = min((int)state[1] + 1, level);
+    const float4 cur = img[i];
+    ring[(size_t)slot * n4 + i] = cur;
+    int pos = (slot + 1 - held + level) % level;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int k = 0; k < held; ++k) {
+        const float4 v = pos == slot ? cur : ring[(size_t)pos * n4 + i];
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+        pos = pos + 1 == level ? 0 : pos + 1;
+    }
+    const float d = (float)held;
+    img[i] = make_float4(s.x / d, s.y / d, s.z / d, s.w / d);
+}
+
+__global__ void k_accumulate_advance(uint32_t* state, int level) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        state[0] = (state[0] + 1) % (uint32_t)level;
+        state[1] = min(state[1] + 1, (uint32_t)level);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_log10(float4* __restrict__ x, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    float4 v = x[i];
+    x[i] = make_float4(log10f(v.x), log10f(v.y), log10f(v.z), log10f(v.w));
+}
+
+inline unsigned grid_for(size_t n4) { return (unsigned)((n4 + 255) / 256); }
+
+}  // namespace
+
+namespace hv {
+
+size_t renorm_scratch_bytes() { return kRenormParts * sizeof(double) + 256; }
+
+hv_status launch_renormalize(float* img, size_t P, unsigned norm_constant, void* scratch, hipStream_t st) {
+    if (P % 4) return fail(HV_ERR_UNSUPPORTED, "renormalize: frame_res must be a multiple of 4");
+    const size_t n4 = P / 4;
+    double* parts = (double*)scratch;
+    const int nparts = (int)std::min<size_t>(kRenormParts, std::max<size_t>(1, (n4 + 2047) / 2048));
+    hipLaunchKernelGGL(k_renorm_partials, dim3(nparts), dim3(256), 0, st, (const float4*)img, n4, parts);
+    hipLaunchKernelGGL(k_renorm_ratio, dim3(1), dim3(256), 0, st, parts, nparts, (float)P);
+    hipLaunchKernelGGL(k_renorm_apply, dim3(grid_for(n4)), dim3(256), 0, st, (float4*)img, n4, parts,
+                       (float)(1u << norm_constant));
+    HV_CHECK_LAUNCH("renormalize");
+    return HV_OK;
+}
+
+hv_status launch_accumulate(float* img, float* ring, uint32_t* state, int level, size_t P, hipStream_t st) {
+    if (P % 4) return fail(HV_ERR_UNSUPPORTED, "accumulation: frame_res must be a multiple of 4");
+    const size_t n4 = P / 4;
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(n4)), dim3(256), 0, st, (float4*)img, (float4*)ring, state, level,
+                       n4);
+    hipLaunchKernelGGL(k_accumulate_advance, dim3(1), dim3(64), 0, st, state, level);
+    HV_CHECK_LAUNCH("accumulate");
+    return HV_OK;
+}
+
+hv_status launch_log10(float* img, size_t P, hipStream_t st) {
+    if (P % 4) return fail(HV_ERR_UNSUPPORTED, "log10: frame_res must be a multiple of 4");
+    hipLaunchKernelGGL(k_log10, dim3(grid_for(P / 4)), dim3(256), 0, st, (float4*)img, P / 4);
+    HV_CHECK_LAUNCH("log10");
+    return HV_OK;
+}
+
+}  // namespace hv

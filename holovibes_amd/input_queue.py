@@ -47,21 +47,23 @@ class BatchInputQueue:
         self.batch_size = batch_size
         self._held = None
 
-    def enqueue(self, frames, nb_frames: int | None = None, stream=None) -> None:
-        """Appends frames: a host numpy array (kept alive by the caller until
-        the stream's copy has run; pinned for an asynchronous copy) or a
-        contiguous device tensor."""
+    def enqueue(self, frames, nb_frames: int | None = None, stream=None, pinned: bool = False) -> None:
+        """Appends frames: a contiguous device tensor, or a host numpy array.
+        A host copy may still be reading the array when the call returns, so
+        the stream is synchronised unless `pinned` says the array is pinned
+        memory that the caller keeps alive until the stream's copy has run."""
         ptr, kind, tmp = _src(frames)
         n = nb_frames if nb_frames is not None else _nbytes(frames) // self.frame_size
         _lib.call("hv_input_queue_enqueue", self.h, ptr, kind, n, _stream(stream))
-        if tmp is not None:  # the copy reads a temporary: let it finish first
+        if kind == H2D and (tmp is not None or not pinned):
             _sync(stream)
 
-    def enqueue_packed(self, packed, nb_frames: int, bits: int, stream=None) -> None:
-        """Appends nb_frames frames packed on `bits` bits, unpacked into the queue."""
+    def enqueue_packed(self, packed, nb_frames: int, bits: int, stream=None, pinned: bool = False) -> None:
+        """Appends nb_frames frames packed on `bits` bits, unpacked into the
+        queue (host arrays as in enqueue)."""
         ptr, kind, tmp = _src(packed)
         _lib.call("hv_input_queue_enqueue_packed", self.h, ptr, kind, bits, nb_frames, _stream(stream))
-        if tmp is not None:
+        if kind == H2D and (tmp is not None or not pinned):
             _sync(stream)
 
     def front(self, stream=None, timeout_ms: int = 0):

@@ -307,6 +307,13 @@ typedef struct hv_compute_settings {
     float camera_fps;                /* CameraFps (moments frequency scale) */
     int32_t lens_precision;          /* HV_LENS_* */
     int32_t pca_precision;           /* HV_PCA_* */
+    /* optional post-processing of the float image, in the reference's order
+     * (Backend/sources/core/pipe.cc:353-370): renormalise, fft shift,
+     * accumulate, log10, then contrast */
+    int32_t renorm_enabled;          /* RenormEnabled: x * 2^renorm_constant / mean(x) */
+    uint32_t renorm_constant;        /* RenormConstant, 0..31 */
+    uint32_t img_accu_level;         /* XY.output_image_accumulation: mean of the last n images (0, 1: off) */
+    int32_t log_enabled;             /* XY.log_enabled: log10 */
 } hv_compute_settings;
 
 /* Fills the reference defaults (Backend/includes/core/holovibes.hh:359-447). */
@@ -470,6 +477,8 @@ hv_status hv_input_queue_create(uint32_t frame_capacity, uint32_t batch_size, ui
 /* BatchInputQueue::enqueue (batch_input_queue.cc:124-187): appends nb_frames
  * frames (host memory: memcpy_kind 1 = hipMemcpyHostToDevice, pinned for an
  * asynchronous copy; device memory: 3 = hipMemcpyDeviceToDevice) on `stream`.
+ * The frames must stay valid until the copy has run on `stream`, host
+ * memory included (a pageable copy may still be reading it on return).
  * A batch becomes visible when complete; when the queue is full the oldest
  * batch is overwritten (has_overwritten).  Waits on the host only while the
  * consumer holds the slot to overwrite (HV_ERR_INVALID_ARG if the caller's

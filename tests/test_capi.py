@@ -45,7 +45,9 @@ def test_settings_struct_layout(lib):
     assert abs(s.lambda_ - 852e-9) < 1e-12 and s.pixel_size == 12.0 and s.z_distance == 0.0
     assert abs(s.contrast_lower - 0.02) < 1e-7 and abs(s.contrast_upper - 99.98) < 1e-4
     assert s.camera_fps == 0.0 and s.fft_shift == 0 and s.unwrap_2d == 0
-    assert ctypes.sizeof(ComputeSettings) == 27 * 4
+    # the last fields, written by the library at their C offsets
+    assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (0, 5, 1, 0)
+    assert ctypes.sizeof(ComputeSettings) == 31 * 4
 
 
 def test_validate_settings(lib):

@@ -195,7 +195,7 @@ def test_producer_thread_and_consumer():
     def producer():
         try:
             for k in range(N):
-                q.enqueue(host[k * B:(k + 1) * B].numpy(), B, prod_stream)
+                q.enqueue(host[k * B:(k + 1) * B].numpy(), B, prod_stream, pinned=True)
         except Exception as e:  # surfaced by the assertion below
             errors.append(e)
 

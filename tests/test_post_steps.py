@@ -1,0 +1,55 @@
+"""The post-processing restatement (oracle/post_steps.py) against what the
+mathematics fixes: renormalised images have mean 2^c, a ramp of one image
+averages to itself and constant images to their constant, means of
+exactly representable values are exact, log10 of powers of ten is exact;
+and the settings layout carries the new fields."""
+import numpy as np
+
+from oracle import post_steps as ps
+
+
+def test_renormalize_mean_is_power_of_two():
+    x = np.random.default_rng(0).random((64, 64)).astype(np.float32) * 37.0 + 1.0
+    for c in (0, 5, 12):
+        y = ps.renormalize(x, c)
+        assert abs(float(y.mean(dtype=np.float64)) / 2 ** c - 1.0) < 1e-6
+        # a scale of the input leaves the output unchanged up to rounding
+        assert np.allclose(ps.renormalize(x * 4.0, c), y, rtol=1e-6)
+
+
+def test_accumulator_means():
+    acc = ps.Accumulator(3)
+    imgs = [np.full((4, 4), v, np.float32) for v in (1.0, 2.0, 4.0, 8.0, 16.0)]
+    got = [float(acc.push(im)[0, 0]) for im in imgs]
+    # held images: 1, 2, 3, 3, 3 -> sums 1, 3, 7, 14, 28 (exact) over min(n, 3)
+    third = [float(np.float32(v) / np.float32(3.0)) for v in (7.0, 14.0, 28.0)]
+    assert got == [1.0, 1.5] + third
+    one = ps.Accumulator(1)
+    r = np.random.default_rng(1).random((8, 8)).astype(np.float32)
+    assert np.array_equal(one.push(r), r)
+
+
+def ulps(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, -(a & 0x7FFFFFFF), a)
+    b = np.where(b < 0, -(b & 0x7FFFFFFF), b)
+    return np.abs(a - b)
+
+
+def test_log10_correctly_rounded():
+    """The restatement is log10 rounded once to float: exact on powers of
+    ten, within 2 ulp of numpy's float32 log10, monotone."""
+    p10 = np.float32(10.0) ** np.arange(0, 8, dtype=np.float32)
+    assert np.array_equal(ps.log10(p10), np.arange(0, 8, dtype=np.float32))
+    x = np.sort(np.random.default_rng(2).random(4096).astype(np.float32) * 1e4 + 1e-3)
+    assert int(ulps(ps.log10(x), np.log10(x)).max()) <= 2
+    assert np.all(np.diff(ps.log10(x)) >= 0)
+
+
+def test_settings_carry_post_steps():
+    from holovibes_amd.configs import settings_for
+    s = settings_for("c1", renorm_enabled=1, renorm_constant=7, img_accu_level=4, log_enabled=1)
+    assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (1, 7, 4, 1)
+    d = settings_for("c1")
+    assert (d.renorm_enabled, d.renorm_constant, d.img_accu_level, d.log_enabled) == (0, 5, 1, 0)
