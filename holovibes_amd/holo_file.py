@@ -378,12 +378,14 @@ def compute_settings_to_json(s: ComputeSettings) -> dict:
     }
     xy = _view_xyz({"enabled": bool(s.contrast_enabled), "auto_refresh": auto, "invert": bool(s.contrast_invert),
                     "min": _f32(s.contrast_min), "max": _f32(s.contrast_max)})
+    xy["log_enabled"] = bool(s.log_enabled)
+    xy["output_image_accumulation"] = max(1, int(s.img_accu_level))
     view = {
         "image_type": _IMG_WRITE[s.img_type], "fft_shift": bool(s.fft_shift),
         "x": {"width": 0, "start": 0}, "y": {"width": 0, "start": 0},
         "z": {"width": int(s.p_width), "start": int(s.p_start)}, "z2": {"width": 0, "start": 0},
         "window": {"xy": xy, "yz": _view_xyz(), "xz": _view_xyz(), "filter2d": _view_window()},
-        "renorm": False, "reticle": {"display_enabled": False, "scale": 0.5},
+        "renorm": bool(s.renorm_enabled), "reticle": {"display_enabled": False, "scale": 0.5},
         "registration": {"registration_enabled": False, "registration_zone": _f32(0.7)},
     }
     composite = {"mode": "RGB", "auto_weight": False,
@@ -394,7 +396,7 @@ def compute_settings_to_json(s: ComputeSettings) -> dict:
                 "filter2d_smooth": {"low": 0, "high": 0},
                 "contrast": {"lower": _f32(s.contrast_lower), "upper": _f32(s.contrast_upper),
                              "frame_index_offset": 2},
-                "renorm_constant": 5, "raw_bitshift": 0, "nb_frames_to_record": 0}
+                "renorm_constant": int(s.renorm_constant), "raw_bitshift": 0, "nb_frames_to_record": 0}
     return {"version": "v1", "image_rendering": rendering, "view": view, "color_composite_image": composite,
             "advanced": advanced}
 
@@ -407,8 +409,9 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     the footer info.  Settings the JSON does not carry (unwrap, arithmetic
     precisions, contrast subsample, the automatic contrast policy) come from
     `base` (default: hv_default_settings).  Returns (settings, ignored): `ignored` names the
-    enabled post-steps the path does not run (renormalisation, log, image
-    accumulation, registration, band-pass filters, output flip/rotation).
+    enabled steps the path does not run (registration, band-pass filters,
+    output flip/rotation); renormalisation, log10 and image accumulation of
+    the xy view are mapped.
     Raises UnsupportedSettings for computations outside the path: raw mode,
     SSA_STFT, phase increase and composite images."""
     if base is None:
@@ -464,13 +467,12 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
             s.pixel_size = float(info["pixel_size"])
         if "camera_fps" in info:
             s.camera_fps = float(info["camera_fps"])
-    # post-steps outside the path that the footer switches on
-    if v.get("renorm"):
-        ignored.append("view.renorm")
-    if xy.get("log_enabled"):
-        ignored.append("view.window.xy.log_enabled")
-    if int(xy.get("output_image_accumulation", 1) or 1) > 1:
-        ignored.append("view.window.xy.output_image_accumulation")
+    # post-processing of the xy image (pipe.cc:353-370)
+    s.renorm_enabled = int(bool(v.get("renorm", False)))
+    s.renorm_constant = int(adv.get("renorm_constant", s.renorm_constant))
+    s.log_enabled = int(bool(xy.get("log_enabled", False)))
+    s.img_accu_level = max(1, int(xy.get("output_image_accumulation", 1) or 1))
+    # steps outside the path that the footer switches on
     if xy.get("horizontal_flip") or float(xy.get("rotation", 0.0) or 0.0) != 0.0:
         ignored.append("view.window.xy.horizontal_flip/rotation")
     if v.get("registration", {}).get("registration_enabled"):

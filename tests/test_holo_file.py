@@ -258,8 +258,10 @@ def test_settings_files_map_as_their_names_say(holo):
             assert s.time_stride == 3
         if "batch_size_30" in name:
             assert s.batch_size == 30
-        if name.endswith("_g"):
-            assert "view.renorm" in ignored
+        # the files set renormalisation ("renorm") and its constant explicitly
+        assert s.renorm_enabled == int(cs["view"]["renorm"])
+        assert s.renorm_constant == cs["advanced"]["renorm_constant"]
+        assert ignored == []
         seen += 1
     assert seen == 16
 
