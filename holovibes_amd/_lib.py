@@ -162,6 +162,8 @@ SIGNATURES = {
     "hv_pipe_set_timing": [P, I],
     "hv_pipe_stage_times": [P, P, P],
     "hv_pipe_sync": [P],
+    "hv_pipe_set_record_moments": [P, I],
+    "hv_pipe_record_frames": [P, P],
     "hv_pipe_destroy": [P],
     "hv_synth_frames": [P, U64, U32, P],
     "hv_server_create": [SZ, P],

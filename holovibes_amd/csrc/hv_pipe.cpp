@@ -176,6 +176,7 @@ struct hv_pipe {
     float* accu_ring = nullptr;     // img_accu_level images
     uint32_t* accu_state = nullptr; // {next slot, images held}
     void* renorm_ws = nullptr;
+    float* rec = nullptr;           // moments record: M0, M1, M2 of the latest cycle (own allocation)
     // host mirror of the reference's state
     uint32_t batch_index = 0;  // BatchEnv::batch_index
     uint32_t q_start = 0, q_size = 0;
@@ -283,6 +284,20 @@ hv_status hv_pipe::image_work() {
                                        it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
         }
         if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
+    }
+    // moments record (insert_moments + insert_moments_record, pipe.cc:429-455,
+    // 583-598): the three moment frames of the window, whatever the image type
+    if (rec) {
+        const int fs = (int)s.p_start;
+        const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
+        if (s.time_transformation == HV_TIME_PCA) {
+            for (int k = 0; k < 3; ++k)
+                HV_TRY(launch_pca_image(rec + (size_t)k * P, L.ring, L.evecs, Tn, P, HV_IMG_MOMENTS_0 + k, fs, fe,
+                                        nullptr, L.stream, kPipeMm, s.camera_fps));
+        } else {
+            HV_TRY(launch_stft_moments(rec, rec + P, rec + 2 * P, L.ring, Tn, P, fs, fe, s.camera_fps, nullptr, 0,
+                                       L.stream, kPipeMm));
+        }
     }
     // post-processing in the reference's order (pipe.cc:353-369): renormalise,
     // fft shift, accumulate, log10
@@ -659,6 +674,32 @@ hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out
     return p->have_output ? HV_OK : HV_ERR_NOT_READY;
 }
 
+hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    HV_TRY(hv_pipe_sync(p));
+    if (!enabled) {
+        if (p->rec) (void)hipFree(p->rec);
+        p->rec = nullptr;
+        return HV_OK;
+    }
+    if (p->s.time_transformation != HV_TIME_STFT && p->s.time_transformation != HV_TIME_PCA)
+        return fail(HV_ERR_UNSUPPORTED, "moments are recorded from the STFT or PCA output");
+    if (p->nlanes > 1) return fail(HV_ERR_UNSUPPORTED, "the moments record runs one cycle at a time");
+    if (!p->rec) HV_HIP(hipMalloc((void**)&p->rec, 3 * p->P * sizeof(float)));
+    // the moments read the whole time window, which only the three-pass path keeps
+    p->path = PATH_THREE_PASS;
+    p->drop_graphs();
+    p->next = 0;
+    return HV_OK;
+}
+
+hv_status hv_pipe_record_frames(hv_pipe* p, const float** frames) {
+    if (!p || !frames) return fail(HV_ERR_INVALID_ARG, "null argument");
+    if (!p->rec) return fail(HV_ERR_INVALID_ARG, "moments record not enabled");
+    *frames = p->rec;
+    return p->have_output ? HV_OK : HV_ERR_NOT_READY;
+}
+
 hv_status hv_pipe_contrast_range(hv_pipe* p, const float** range_dev) {
     if (!p || !range_dev) return fail(HV_ERR_INVALID_ARG, "null argument");
     *range_dev = p->lane[p->last].range;
@@ -676,7 +717,7 @@ hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t*
 hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (level < 0) return fail(HV_ERR_INVALID_ARG, "fusion level must be >= 0");
-    p->path = choose_path(p->s, level);
+    p->path = p->rec ? PATH_THREE_PASS : choose_path(p->s, level);
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
     p->nlanes = 1;  // extra lanes hold one path's buffers: hv_pipe_set_lanes again after a path change
@@ -691,6 +732,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         return fail(HV_ERR_UNSUPPORTED, "several lanes need the pipe's own streams (create it without a stream)");
     if (lanes > 1 && (p->s.renorm_enabled || p->s.img_accu_level > 1))
         return fail(HV_ERR_UNSUPPORTED, "renormalisation and image accumulation run one cycle at a time");
+    if (lanes > 1 && p->rec) return fail(HV_ERR_UNSUPPORTED, "the moments record runs one cycle at a time");
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
     int pmin, pmax;
@@ -861,6 +903,7 @@ hv_status hv_pipe_destroy(hv_pipe* p) {
     for (auto e : p->ev)
         if (e) (void)hipEventDestroy(e);
     if (p->base) (void)hipFree(p->base);
+    if (p->rec) (void)hipFree(p->rec);
     if (p->own_stream && p->lane[0].stream) (void)hipStreamDestroy(p->lane[0].stream);
     delete p;
     return HV_OK;

@@ -419,6 +419,16 @@ hv_status hv_pipe_stream(hv_pipe* p, void** stream);
 /* Synchronises the pipe stream (end-of-pipe cudaStreamSynchronize, pipe.cc:386). */
 hv_status hv_pipe_sync(hv_pipe* p);
 
+/* RecordMode::MOMENTS (Backend/sources/core/pipe.cc:429-455, 583-598): every
+ * cycle also writes the three moment frames M0, M1, M2 of the window (the
+ * STFT's, or the PCA output's) as float frames, whatever the image type.
+ * Allocates 3 frames of floats here and keeps the three-pass path (the
+ * moments read the whole window).  One lane only.  enabled 0 releases. */
+hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled);
+/* The latest cycle's M0, M1, M2 (device, 3 * W * H floats, valid until the
+ * next push), for hv_holo_store_frames; HV_ERR_NOT_READY before a cycle. */
+hv_status hv_pipe_record_frames(hv_pipe* p, const float** frames);
+
 hv_status hv_pipe_destroy(hv_pipe* p);
 
 /* ------------------------------------------------------------------------ */
