@@ -94,7 +94,9 @@ class ComputeSettings(ctypes.Structure):
                 ("contrast_subsample", ctypes.c_uint32), ("camera_fps", ctypes.c_float),
                 ("lens_precision", ctypes.c_int32), ("pca_precision", ctypes.c_int32),
                 ("renorm_enabled", ctypes.c_int32), ("renorm_constant", ctypes.c_uint32),
-                ("img_accu_level", ctypes.c_uint32), ("log_enabled", ctypes.c_int32)]
+                ("img_accu_level", ctypes.c_uint32), ("log_enabled", ctypes.c_int32),
+                ("cuts_enabled", ctypes.c_int32), ("x_start", ctypes.c_uint32), ("x_width", ctypes.c_uint32),
+                ("y_start", ctypes.c_uint32), ("y_width", ctypes.c_uint32)]
 
 
 P = ctypes.c_void_p
@@ -126,6 +128,7 @@ SIGNATURES = {
     "hv_complex_to_argument": [P, P, I, I, SZ, P],
     "hv_stft_image": [P, P, I, SZ, I, I, I, P, P],
     "hv_stft_moments": [P, P, P, P, I, SZ, I, I, F, P],
+    "hv_stft_cuts": [P, P, P, I, I, I, I, I, I, I, I, P],
     "hv_pca_workspace_size": [I, SZ, I, P],
     "hv_pca_cov": [P, P, I, SZ, I, P, P],
     "hv_eig_workspace_size": [I, P],
@@ -162,6 +165,7 @@ SIGNATURES = {
     "hv_pipe_set_timing": [P, I],
     "hv_pipe_stage_times": [P, P, P],
     "hv_pipe_sync": [P],
+    "hv_pipe_cuts": [P, P, P],
     "hv_pipe_set_record_moments": [P, I],
     "hv_pipe_record_frames": [P, P],
     "hv_pipe_destroy": [P],

@@ -244,6 +244,19 @@ hv_status hv_stft_moments(float* m0, float* m1, float* m2, const void* ring_c64,
                                nullptr, 0, as_stream(stream));
 }
 
+hv_status hv_stft_cuts(float* out_xz, float* out_yz, const void* stft_c64, int width, int height, int T, int xmin,
+                       int ymin, int xmax, int ymax, int img_type, void* stream) {
+    HV_NOT_NULL(out_xz);
+    HV_NOT_NULL(out_yz);
+    HV_NOT_NULL(stft_c64);
+    HV_TRY(check_frame(width, height));
+    HV_TRY(check_T(T));
+    if (xmin < 0 || ymin < 0 || xmin > xmax || ymin > ymax || xmax >= width || ymax >= height)
+        return fail(HV_ERR_INVALID_ARG, "cuts: need 0 <= min <= max < frame side");
+    return launch_cuts(out_xz, out_yz, (const float2*)stft_c64, width, height, T, xmin, ymin, xmax, ymax, img_type,
+                       as_stream(stream));
+}
+
 hv_status hv_pca_workspace_size(int T, size_t frame_res, int precision, size_t* bytes) {
     HV_NOT_NULL(bytes);
     HV_TRY(check_T(T));

@@ -104,5 +104,8 @@ hv_status launch_renormalize(float* img, size_t P, unsigned norm_constant, void*
 // state: {next slot, images held}, zeroed when the ring is (re)started
 hv_status launch_accumulate(float* img, float* ring, uint32_t* state, int level, size_t P, hipStream_t st);
 hv_status launch_log10(float* img, size_t P, hipStream_t st);
+// XZ (T x W) and YZ (H x T) cuts of a time-transformed window (bin-major)
+hv_status launch_cuts(float* xz, float* yz, const float2* stft, int W, int H, int T, int xmin, int ymin, int xmax,
+                      int ymax, int img_type, hipStream_t st);
 
 }  // namespace hv

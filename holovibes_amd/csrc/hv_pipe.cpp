@@ -63,6 +63,7 @@ bool bins_eligible(const hv_compute_settings& s) {
 enum { PATH_THREE_PASS = 0, PATH_ROW_STFT = 1, PATH_BINS = 2 };
 
 int choose_path(const hv_compute_settings& s, int level) {
+    if (s.cuts_enabled) return PATH_THREE_PASS;  // the cuts read the whole propagated window
     if (level >= PATH_BINS && bins_eligible(s)) return PATH_BINS;
     if (level >= PATH_ROW_STFT && fused_eligible(s)) return PATH_ROW_STFT;
     return PATH_THREE_PASS;
@@ -70,7 +71,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 
 struct Layout {
     size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
-    size_t accu, renorm;
+    size_t accu, renorm, cuts;
     size_t total;
 };
 
@@ -102,8 +103,10 @@ Layout layout_of(const hv_compute_settings& s) {
     // image accumulation ring (+ its position) and renormalisation sums
     l.accu = s.img_accu_level > 1 ? align256((size_t)s.img_accu_level * P * sizeof(float)) + 256 : 0;
     l.renorm = s.renorm_enabled ? align256(renorm_scratch_bytes()) : 0;
+    // cuts: the window's STFT, then the XZ and YZ images
+    l.cuts = s.cuts_enabled ? align256(T * P * sizeof(float2)) + 2 * align256(T * std::max(s.width, s.height) * 4) : 0;
     l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
-              l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm;
+              l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts;
     return l;
 }
 
@@ -177,6 +180,9 @@ struct hv_pipe {
     uint32_t* accu_state = nullptr; // {next slot, images held}
     void* renorm_ws = nullptr;
     float* rec = nullptr;           // moments record: M0, M1, M2 of the latest cycle (own allocation)
+    float2* cuts_stft = nullptr;    // cuts: the window's STFT, XZ and YZ images
+    float* cut_xz = nullptr;
+    float* cut_yz = nullptr;
     // host mirror of the reference's state
     uint32_t batch_index = 0;  // BatchEnv::batch_index
     uint32_t q_start = 0, q_size = 0;
@@ -284,6 +290,12 @@ hv_status hv_pipe::image_work() {
                                        it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
         }
         if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
+    }
+    // XZ / YZ cuts of the STFT (insert_time_transformation_cuts_view, fourier_transform.cc:466-511)
+    if (s.cuts_enabled) {
+        HV_TRY(launch_stft(cuts_stft, L.ring, Tn, P, L.stream));
+        HV_TRY(launch_cuts(cut_xz, cut_yz, cuts_stft, (int)s.width, (int)s.height, Tn, (int)s.x_start, (int)s.y_start,
+                           (int)(s.x_start + s.x_width), (int)(s.y_start + s.y_width), it, L.stream));
     }
     // moments record (insert_moments + insert_moments_record, pipe.cc:429-455,
     // 583-598): the three moment frames of the window, whatever the image type
@@ -446,6 +458,12 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     if (s->fft_shift && (s->width % 2 || s->height % 2)) return fail(HV_ERR_UNSUPPORTED, "fft shift needs even sides");
     if (s->renorm_enabled && s->renorm_constant > 31) return fail(HV_ERR_INVALID_ARG, "renorm_constant must be <= 31");
     if (s->img_accu_level > 4096) return fail(HV_ERR_INVALID_ARG, "img_accu_level must be <= 4096");
+    if (s->cuts_enabled) {
+        if (s->time_transformation != HV_TIME_STFT) return fail(HV_ERR_UNSUPPORTED, "cuts are taken of the STFT");
+        // the reference reads past the frame when a view runs over its edge
+        if ((uint64_t)s->x_start + s->x_width >= s->width || (uint64_t)s->y_start + s->y_width >= s->height)
+            return fail(HV_ERR_INVALID_ARG, "cuts: x_start + x_width < width and y_start + y_width < height");
+    }
     return HV_OK;
 }
 
@@ -510,6 +528,12 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
         p->accu_state = (uint32_t*)take(256);
     }
     p->renorm_ws = take(l.renorm);
+    if (l.cuts) {
+        const size_t cut = (l.cuts - align256((size_t)p->T * p->P * sizeof(float2))) / 2;
+        p->cuts_stft = (float2*)take(l.cuts - 2 * cut);
+        p->cut_xz = (float*)take(cut);
+        p->cut_yz = (float*)take(cut);
+    }
     L0.ring = p->ring;
     L0.cov = p->cov;
     L0.evals = p->evals;
@@ -674,6 +698,14 @@ hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out
     return p->have_output ? HV_OK : HV_ERR_NOT_READY;
 }
 
+hv_status hv_pipe_cuts(hv_pipe* p, const float** xz, const float** yz) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    if (!p->cut_xz) return fail(HV_ERR_INVALID_ARG, "cuts not enabled");
+    if (xz) *xz = p->cut_xz;
+    if (yz) *yz = p->cut_yz;
+    return p->have_output ? HV_OK : HV_ERR_NOT_READY;
+}
+
 hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     HV_TRY(hv_pipe_sync(p));
@@ -733,6 +765,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     if (lanes > 1 && (p->s.renorm_enabled || p->s.img_accu_level > 1))
         return fail(HV_ERR_UNSUPPORTED, "renormalisation and image accumulation run one cycle at a time");
     if (lanes > 1 && p->rec) return fail(HV_ERR_UNSUPPORTED, "the moments record runs one cycle at a time");
+    if (lanes > 1 && p->s.cuts_enabled) return fail(HV_ERR_UNSUPPORTED, "the cuts run one cycle at a time");
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
     int pmin, pmax;

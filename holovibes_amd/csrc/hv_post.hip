@@ -1,9 +1,10 @@
 // hv_post.hip -- optional post-processing of the float image, between the
 // image kernel and the contrast (Backend/sources/core/pipe.cc:353-370):
-// renormalisation, image accumulation, log10.  Each is one HBM pass over
-// the image (float4 per thread); accumulation also reads the level - 1
-// previous images.  The accumulation ring's position lives in device memory,
-// so a captured cycle replays correctly.
+// renormalisation, image accumulation, log10; and the XZ / YZ cuts of the
+// time-transformed window.  Each is one HBM pass over the image (float4 per
+// thread); accumulation also reads the level - 1 previous images.  The
+// accumulation ring's position lives in device memory, so a captured cycle
+// replays correctly.
 #include "hv_common.hpp"
 #include "hv_kernels.hpp"
 
@@ -137,6 +138,66 @@ hv_status launch_log10(float* img, size_t P, hipStream_t st) {
     if (P % 4) return fail(HV_ERR_UNSUPPORTED, "log10: frame_res must be a multiple of 4");
     hipLaunchKernelGGL(k_log10, dim3(grid_for(P / 4)), dim3(256), 0, st, (float4*)img, P / 4);
     HV_CHECK_LAUNCH("log10");
+    return HV_OK;
+}
+
+}  // namespace hv
+
+// ---------------------------------------------------------------------------
+// XZ / YZ cuts (fill_32bit_slices, Backend/cuda_sources/stft.cu:17-115): the
+// time-transformed window seen from the side.  XZ (T rows x W) holds, per
+// frequency bin and column, the mean over rows [ymin, ymax]; YZ (H rows x T)
+// per row and bin the mean over columns [xmin, xmax].  Values as the
+// reference computes them: hypotf, its square, or atanf(im / re) + pi/2
+// (atan, not atan2), summed in float in index order.
+// ---------------------------------------------------------------------------
+namespace {
+
+// other image types (the moments) leave the value at 0, as in the reference
+__device__ __forceinline__ float cut_value(float2 z, int img_type) {
+    if (img_type == HV_IMG_MODULUS) return hypotf(z.x, z.y);
+    if (img_type == HV_IMG_SQUARED_MODULUS) {
+        const float m = hypotf(z.x, z.y);
+        return m * m;
+    }
+    if (img_type == HV_IMG_ARGUMENT) return (float)((double)atanf(z.y / z.x) + 1.5707963267948966);
+    return 0.f;
+}
+
+// one thread per (bin, x): rows ymin..ymax of the bin's plane, coalesced over x
+__global__ void __launch_bounds__(256) k_cut_xz(float* __restrict__ xz, const float2* __restrict__ stft, int W,
+                                                size_t P, int T, int ymin, int ymax, int img_type) {
+    const int x = blockIdx.x * 256 + threadIdx.x, bin = blockIdx.y;
+    if (x >= W || bin >= T) return;
+    const float2* plane = stft + (size_t)bin * P;
+    float sum = 0.f;
+    for (int y = ymin; y <= ymax; ++y) sum += cut_value(plane[(size_t)y * W + x], img_type);
+    xz[(size_t)bin * W + x] = sum / (float)(ymax - ymin + 1);
+}
+
+// one thread per (y, bin): columns xmin..xmax of row y in the bin's plane
+__global__ void __launch_bounds__(256) k_cut_yz(float* __restrict__ yz, const float2* __restrict__ stft, int W, int H,
+                                                size_t P, int T, int xmin, int xmax, int img_type) {
+    const size_t id = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (id >= (size_t)H * T) return;
+    const int y = (int)(id / T), bin = (int)(id % T);
+    const float2* row = stft + (size_t)bin * P + (size_t)y * W;
+    float sum = 0.f;
+    for (int x = xmin; x <= xmax; ++x) sum += cut_value(row[x], img_type);
+    yz[id] = sum / (float)(xmax - xmin + 1);
+}
+
+}  // namespace
+
+namespace hv {
+
+hv_status launch_cuts(float* xz, float* yz, const float2* stft, int W, int H, int T, int xmin, int ymin, int xmax,
+                      int ymax, int img_type, hipStream_t st) {
+    const size_t P = (size_t)W * H;
+    hipLaunchKernelGGL(k_cut_xz, dim3((W + 255) / 256, T), dim3(256), 0, st, xz, stft, W, P, T, ymin, ymax, img_type);
+    hipLaunchKernelGGL(k_cut_yz, dim3((unsigned)(((size_t)H * T + 255) / 256)), dim3(256), 0, st, yz, stft, W, H, P, T,
+                       xmin, xmax, img_type);
+    HV_CHECK_LAUNCH("cuts");
     return HV_OK;
 }
 

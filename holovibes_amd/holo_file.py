@@ -345,6 +345,12 @@ def _view_xyz(contrast: dict | None = None) -> dict:
     return w
 
 
+def _cut_view(s: ComputeSettings) -> dict:
+    w = _view_xyz()
+    w["enabled"] = bool(s.cuts_enabled)
+    return w
+
+
 def _channel(shift: bool = False) -> dict:
     # Backend/includes/struct/composite_struct.hh CompositeChannel / CompositeH defaults
     c = {"frame_index": {"min": 0, "max": 0, "activated": False},
@@ -382,9 +388,10 @@ def compute_settings_to_json(s: ComputeSettings) -> dict:
     xy["output_image_accumulation"] = max(1, int(s.img_accu_level))
     view = {
         "image_type": _IMG_WRITE[s.img_type], "fft_shift": bool(s.fft_shift),
-        "x": {"width": 0, "start": 0}, "y": {"width": 0, "start": 0},
+        "x": {"width": int(s.x_width), "start": int(s.x_start)},
+        "y": {"width": int(s.y_width), "start": int(s.y_start)},
         "z": {"width": int(s.p_width), "start": int(s.p_start)}, "z2": {"width": 0, "start": 0},
-        "window": {"xy": xy, "yz": _view_xyz(), "xz": _view_xyz(), "filter2d": _view_window()},
+        "window": {"xy": xy, "yz": _cut_view(s), "xz": _cut_view(s), "filter2d": _view_window()},
         "renorm": bool(s.renorm_enabled), "reticle": {"display_enabled": False, "scale": 0.5},
         "registration": {"registration_enabled": False, "registration_zone": _f32(0.7)},
     }
@@ -472,6 +479,11 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     s.renorm_constant = int(adv.get("renorm_constant", s.renorm_constant))
     s.log_enabled = int(bool(xy.get("log_enabled", False)))
     s.img_accu_level = max(1, int(xy.get("output_image_accumulation", 1) or 1))
+    # XZ / YZ cuts at the x / y views
+    win = v.get("window", {})
+    s.cuts_enabled = int(bool(win.get("xz", {}).get("enabled") or win.get("yz", {}).get("enabled")))
+    s.x_start, s.x_width = int(v.get("x", {}).get("start", 0)), int(v.get("x", {}).get("width", 0))
+    s.y_start, s.y_width = int(v.get("y", {}).get("start", 0)), int(v.get("y", {}).get("width", 0))
     # steps outside the path that the footer switches on
     if xy.get("horizontal_flip") or float(xy.get("rotation", 0.0) or 0.0) != 0.0:
         ignored.append("view.window.xy.horizontal_flip/rotation")

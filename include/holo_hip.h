@@ -193,6 +193,15 @@ hv_status hv_stft_image(float* out, const void* ring_c64, int T, size_t frame_re
 hv_status hv_stft_moments(float* m0, float* m1, float* m2, const void* ring_c64, int T, size_t frame_res,
                           int f_start, int f_end, float camera_fps, void* stream);
 
+/* time_transformation_cuts_begin (Backend/cuda_sources/stft.cu:75-115): from a
+ * time-transformed window (T frames of W x H, bin-major), XZ (T x W) = mean
+ * over rows [ymin, ymax] and YZ (H x T) = mean over columns [xmin, xmax] of
+ * the image value (modulus, squared modulus, atan(im/re) + pi/2; 0 for other
+ * types).  xmax < W and ymax < H (the reference reads past the frame
+ * otherwise: HV_ERR_INVALID_ARG here). */
+hv_status hv_stft_cuts(float* out_xz, float* out_yz, const void* stft_c64, int width, int height, int T, int xmin,
+                       int ymin, int xmax, int ymax, int img_type, void* stream);
+
 /* Replaces cov_matrix (Backend/sources/compute/matrix_operations.cc:9-26):
  * cov = H^H H, H = ring as P x T column-major; cov is T x T complex
  * column-major.  Partial sums go to `workspace` (hv_pca_workspace_size bytes). */
@@ -314,6 +323,11 @@ typedef struct hv_compute_settings {
     uint32_t renorm_constant;        /* RenormConstant, 0..31 */
     uint32_t img_accu_level;         /* XY.output_image_accumulation: mean of the last n images (0, 1: off) */
     int32_t log_enabled;             /* XY.log_enabled: log10 */
+    /* CutsViewEnabled with the X / Y accumulation views (fourier_transform.cc:466-511):
+     * XZ (T x W) = mean over rows [y_start, y_start + y_width], YZ (H x T) =
+     * mean over columns [x_start, x_start + x_width] of the STFT image values */
+    int32_t cuts_enabled;
+    uint32_t x_start, x_width, y_start, y_width;
 } hv_compute_settings;
 
 /* Fills the reference defaults (Backend/includes/core/holovibes.hh:359-447). */
@@ -418,6 +432,10 @@ hv_status hv_pipe_stream(hv_pipe* p, void** stream);
 
 /* Synchronises the pipe stream (end-of-pipe cudaStreamSynchronize, pipe.cc:386). */
 hv_status hv_pipe_sync(hv_pipe* p);
+
+/* The latest cycle's XZ (T x W) and YZ (H x T) cut images (device floats,
+ * valid until the next push); settings.cuts_enabled. */
+hv_status hv_pipe_cuts(hv_pipe* p, const float** xz, const float** yz);
 
 /* RecordMode::MOMENTS (Backend/sources/core/pipe.cc:429-455, 583-598): every
  * cycle also writes the three moment frames M0, M1, M2 of the window (the

@@ -53,3 +53,28 @@ def test_settings_carry_post_steps():
     assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (1, 7, 4, 1)
     d = settings_for("c1")
     assert (d.renorm_enabled, d.renorm_constant, d.img_accu_level, d.log_enabled) == (0, 5, 1, 0)
+
+
+def test_cuts_closed_forms_and_brute_force():
+    T, H, W = 4, 6, 5
+    # a constant field: every cut value is |c|, its square, or the argument
+    c = np.full((T, H, W), 3 + 4j, np.complex64)
+    xz, yz = ps.cuts(c, 1, 2, 3, 4, 0)
+    assert np.all(xz == 5.0) and np.all(yz == 5.0) and xz.shape == (T, W) and yz.shape == (H, T)
+    assert np.all(ps.cuts(c, 0, 0, 4, 5, 1)[0] == 25.0)
+    pos = np.full((T, H, W), 2 + 0j, np.complex64)
+    assert np.all(ps.cuts(pos, 0, 0, 0, 0, 2)[1] == np.float32(np.pi / 2))
+    assert np.all(ps.cuts(c, 0, 0, 1, 1, 5)[0] == 0.0)  # moments: no value in the cuts
+    # brute force on a small random window
+    rng = np.random.default_rng(4)
+    z = (rng.standard_normal((T, H, W)) + 1j * rng.standard_normal((T, H, W))).astype(np.complex64)
+    xz, yz = ps.cuts(z, 1, 2, 3, 4, 0)
+    for b in range(T):
+        for x in range(W):
+            acc = np.float32(0)
+            for y in range(2, 5):
+                acc = np.float32(acc + np.float32(abs(complex(z[b, y, x]))))
+            assert abs(xz[b, x] - acc / np.float32(3)) <= 1e-6 * abs(acc)
+        for y in range(H):
+            acc = sum(abs(complex(z[b, y, x])) for x in range(1, 4)) / 3
+            assert abs(yz[y, b] - acc) <= 1e-6 * acc
