@@ -96,7 +96,8 @@ class ComputeSettings(ctypes.Structure):
                 ("renorm_enabled", ctypes.c_int32), ("renorm_constant", ctypes.c_uint32),
                 ("img_accu_level", ctypes.c_uint32), ("log_enabled", ctypes.c_int32),
                 ("cuts_enabled", ctypes.c_int32), ("x_start", ctypes.c_uint32), ("x_width", ctypes.c_uint32),
-                ("y_start", ctypes.c_uint32), ("y_width", ctypes.c_uint32)]
+                ("y_start", ctypes.c_uint32), ("y_width", ctypes.c_uint32),
+                ("registration_enabled", ctypes.c_int32), ("registration_zone", ctypes.c_float)]
 
 
 P = ctypes.c_void_p
@@ -166,6 +167,7 @@ SIGNATURES = {
     "hv_pipe_stage_times": [P, P, P],
     "hv_pipe_sync": [P],
     "hv_pipe_cuts": [P, P, P],
+    "hv_pipe_registration_shift": [P, P, P],
     "hv_pipe_set_record_moments": [P, I],
     "hv_pipe_record_frames": [P, P],
     "hv_pipe_destroy": [P],

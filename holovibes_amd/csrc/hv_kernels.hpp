@@ -104,6 +104,13 @@ hv_status launch_renormalize(float* img, size_t P, unsigned norm_constant, void*
 // state: {next slot, images held}, zeroed when the ring is (re)started
 hv_status launch_accumulate(float* img, float* ring, uint32_t* state, int level, size_t P, hipStream_t st);
 hv_status launch_log10(float* img, size_t P, hipStream_t st);
+// registration: scratch of registration_scratch_bytes(P); mask once, reference
+// once, then one shift per cycle (img shifted in place through tmp)
+size_t registration_scratch_bytes(size_t P);
+hv_status launch_registration_mask(void* scratch, int W, int H, float zone, int swap, hipStream_t st);
+hv_status launch_registration_set_reference(void* scratch, const float* img, int W, int H, hipStream_t st);
+hv_status launch_registration(void* scratch, float* img, float* tmp, int W, int H, hipStream_t st);
+hv_status registration_shift(void* scratch, size_t P, int* shift_host);
 // XZ (T x W) and YZ (H x T) cuts of a time-transformed window (bin-major)
 hv_status launch_cuts(float* xz, float* yz, const float2* stft, int W, int H, int T, int xmin, int ymin, int xmax,
                       int ymax, int img_type, hipStream_t st);

@@ -71,7 +71,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 
 struct Layout {
     size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
-    size_t accu, renorm, cuts;
+    size_t accu, renorm, cuts, reg;
     size_t total;
 };
 
@@ -105,8 +105,10 @@ Layout layout_of(const hv_compute_settings& s) {
     l.renorm = s.renorm_enabled ? align256(renorm_scratch_bytes()) : 0;
     // cuts: the window's STFT, then the XZ and YZ images
     l.cuts = s.cuts_enabled ? align256(T * P * sizeof(float2)) + 2 * align256(T * std::max(s.width, s.height) * 4) : 0;
+    // registration: mask, reference spectrum, work plane, and a shifted copy
+    l.reg = s.registration_enabled ? align256(registration_scratch_bytes(P)) + align256(P * sizeof(float)) : 0;
     l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
-              l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts;
+              l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts + l.reg;
     return l;
 }
 
@@ -183,6 +185,10 @@ struct hv_pipe {
     float2* cuts_stft = nullptr;    // cuts: the window's STFT, XZ and YZ images
     float* cut_xz = nullptr;
     float* cut_yz = nullptr;
+    void* reg = nullptr;            // registration scratch
+    float* reg_tmp = nullptr;
+    bool reg_ref_set = false;       // the reference image has been taken
+    uint32_t accu_held = 0;         // host mirror of the accumulation queue's size
     // host mirror of the reference's state
     uint32_t batch_index = 0;  // BatchEnv::batch_index
     uint32_t q_start = 0, q_size = 0;
@@ -215,7 +221,9 @@ struct hv_pipe {
                s.space_transformation != HV_SPACE_NONE;
     }
     // renormalisation, accumulation or log10 change the image after its kernel
-    bool post_steps() const { return s.renorm_enabled || s.img_accu_level > 1 || s.log_enabled; }
+    bool post_steps() const {
+        return s.renorm_enabled || s.img_accu_level > 1 || s.log_enabled || s.registration_enabled;
+    }
     hv_status mark(int i) {
         if (timing) HV_HIP(hipEventRecord(ev[i], lane[cur].stream));
         return HV_OK;
@@ -315,8 +323,20 @@ hv_status hv_pipe::image_work() {
     // fft shift, accumulate, log10
     if (s.renorm_enabled) HV_TRY(launch_renormalize(L.image, P, s.renorm_constant, renorm_ws, L.stream));
     if (s.fft_shift) HV_TRY(launch_shift_corners_f32(L.image, 1, s.width, s.height, L.stream));
+    // registration runs once the accumulation queue is full (or without one)
+    // against the reference taken after the accumulation of the first such
+    // cycle (registration.cc:11-20, 71-90)
+    const uint32_t level = std::max<uint32_t>(1, s.img_accu_level);
+    const bool queue_full_before = level <= 1 || accu_held == level;
+    if (s.registration_enabled && queue_full_before && reg_ref_set)
+        HV_TRY(launch_registration(reg, L.image, reg_tmp, (int)s.width, (int)s.height, L.stream));
     if (s.img_accu_level > 1)
         HV_TRY(launch_accumulate(L.image, accu_ring, accu_state, (int)s.img_accu_level, P, L.stream));
+    if (level > 1) accu_held = std::min(accu_held + 1, level);
+    if (s.registration_enabled && !reg_ref_set && (level <= 1 || accu_held == level)) {
+        HV_TRY(launch_registration_set_reference(reg, L.image, (int)s.width, (int)s.height, L.stream));
+        reg_ref_set = true;
+    }
     if (s.log_enabled) HV_TRY(launch_log10(L.image, P, L.stream));
     HV_TRY(mark(4));
     if (!s.contrast_enabled) return launch_float_to_ushort(L.out, L.image, P, L.stream);
@@ -408,6 +428,8 @@ void hv_default_settings(hv_compute_settings* s) {
     s->renorm_constant = 5;
     s->img_accu_level = 1;
     s->log_enabled = 0;
+    s->registration_enabled = 0;
+    s->registration_zone = 0.7f;  // Registration (view_struct.hh)
 }
 
 hv_status hv_validate_settings(const hv_compute_settings* s) {
@@ -458,6 +480,8 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     if (s->fft_shift && (s->width % 2 || s->height % 2)) return fail(HV_ERR_UNSUPPORTED, "fft shift needs even sides");
     if (s->renorm_enabled && s->renorm_constant > 31) return fail(HV_ERR_INVALID_ARG, "renorm_constant must be <= 31");
     if (s->img_accu_level > 4096) return fail(HV_ERR_INVALID_ARG, "img_accu_level must be <= 4096");
+    if (s->registration_enabled && !(s->registration_zone > 0.f && s->registration_zone <= 1.f))
+        return fail(HV_ERR_INVALID_ARG, "registration_zone must be in (0, 1]");
     if (s->cuts_enabled) {
         if (s->time_transformation != HV_TIME_STFT) return fail(HV_ERR_UNSUPPORTED, "cuts are taken of the STFT");
         // the reference reads past the frame when a view runs over its edge
@@ -528,6 +552,10 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
         p->accu_state = (uint32_t*)take(256);
     }
     p->renorm_ws = take(l.renorm);
+    if (l.reg) {
+        p->reg = take(l.reg - align256(p->P * sizeof(float)));
+        p->reg_tmp = (float*)take(align256(p->P * sizeof(float)));
+    }
     if (l.cuts) {
         const size_t cut = (l.cuts - align256((size_t)p->T * p->P * sizeof(float2))) / 2;
         p->cuts_stft = (float2*)take(l.cuts - 2 * cut);
@@ -548,6 +576,14 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
         e = hipMemcpyAsync(L0.range, r, sizeof(r), hipMemcpyHostToDevice, L0.stream);
         if (e == hipSuccess) e = hipMemsetAsync(p->ring, 0, l.ring, L0.stream);
         if (e == hipSuccess && p->accu_state) e = hipMemsetAsync(p->accu_state, 0, 256, L0.stream);
+        if (e == hipSuccess && p->reg)
+            e = hipMemsetAsync(p->reg, 0, l.reg - align256(p->P * sizeof(float)), L0.stream);
+        // the disc is quadrant-swapped when exactly one of Fresnel and the fft shift holds (registration.cc:100-110)
+        if (e == hipSuccess && p->reg &&
+            launch_registration_mask(p->reg, (int)s->width, (int)s->height, s->registration_zone,
+                                     (s->space_transformation == HV_SPACE_FRESNELTR) != (s->fft_shift != 0),
+                                     L0.stream) != HV_OK)
+            e = hipErrorLaunchFailure;
         if (e == hipSuccess) e = hipStreamSynchronize(L0.stream);
         if (e != hipSuccess) st = hip_fail(e, "pipe init");
     }
@@ -601,7 +637,7 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     if (p->timing) {
         HV_TRY(p->enqueue_cycle(frames, slot0));
         HV_TRY(p->collect_times());
-    } else if (p->graph_enabled && !pending && !p->shared_ring()) {
+    } else if (p->graph_enabled && !pending && !p->shared_ring() && !p->s.registration_enabled) {
         // one graph per (ring phase, input buffer); past the cache bound the
         // batch is staged into the pipe's own input buffer first (one lane)
         // or run without a graph (several lanes share no staging buffer)
@@ -698,6 +734,17 @@ hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out
     return p->have_output ? HV_OK : HV_ERR_NOT_READY;
 }
 
+hv_status hv_pipe_registration_shift(hv_pipe* p, int* shift_x, int* shift_y) {
+    if (!p || !shift_x || !shift_y) return fail(HV_ERR_INVALID_ARG, "null argument");
+    if (!p->reg) return fail(HV_ERR_INVALID_ARG, "registration not enabled");
+    HV_TRY(hv_pipe_sync(p));
+    int sh[2] = {0, 0};
+    if (p->reg_ref_set) HV_TRY(registration_shift(p->reg, p->P, sh));
+    *shift_x = sh[0];
+    *shift_y = sh[1];
+    return HV_OK;
+}
+
 hv_status hv_pipe_cuts(hv_pipe* p, const float** xz, const float** yz) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (!p->cut_xz) return fail(HV_ERR_INVALID_ARG, "cuts not enabled");
@@ -766,6 +813,8 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         return fail(HV_ERR_UNSUPPORTED, "renormalisation and image accumulation run one cycle at a time");
     if (lanes > 1 && p->rec) return fail(HV_ERR_UNSUPPORTED, "the moments record runs one cycle at a time");
     if (lanes > 1 && p->s.cuts_enabled) return fail(HV_ERR_UNSUPPORTED, "the cuts run one cycle at a time");
+    if (lanes > 1 && p->s.registration_enabled)
+        return fail(HV_ERR_UNSUPPORTED, "registration runs one cycle at a time");
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
     int pmin, pmax;

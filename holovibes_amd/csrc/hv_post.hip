@@ -202,3 +202,232 @@ hv_status launch_cuts(float* xz, float* yz, const float2* stft, int W, int H, in
 }
 
 }  // namespace hv
+
+// ---------------------------------------------------------------------------
+// Registration (Backend/sources/compute/registration.cc:9-122): the image is
+// shifted circularly so that its cross-correlation with a reference image
+// peaks at the origin.  Both images are prepared the same way: the mean over
+// a centred disc is removed inside the disc and everything outside is set to
+// 0.  The correlation is IFFT2(FFT2(current) * conj(FFT2(reference))) (real
+// part; the reference's spectrum is kept from the cycle that set it), its
+// peak is the first index of the largest |value| (cublasIsamax), and the
+// shift is computed and applied on the device (the reference reads the peak
+// back to the host).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kRegParts = 1024;
+
+// get_circular_mask (masks.cu:61-77) with the quadrant swap of registration.cc:100-110
+__global__ void k_reg_mask(float* mask, int W, int H, float cx, float cy, float radius, int swap) {
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    const float d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
+    const float v = d2 <= radius * radius ? 1.f : 0.f;
+    int ox = x, oy = y;
+    if (swap) {
+        ox = (x + W / 2) % W;
+        oy = (y + H / 2) % H;
+    }
+    mask[(size_t)oy * W + ox] = v;
+}
+
+// per-block sums of input * mask and of mask (get_mean_in_mask, apply_mask.cu:45-202), in double
+__global__ void __launch_bounds__(256) k_reg_mean_parts(const float* __restrict__ in, const float* __restrict__ mask,
+                                                        size_t n, double* __restrict__ parts) {
+    __shared__ double sh[4];
+    double s = 0.0, c = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        s += (double)(in[i] * mask[i]);
+        c += (double)mask[i];
+    }
+    const double ts = block_sum(s, sh);
+    __syncthreads();
+    const double tc = block_sum(c, sh);
+    if (threadIdx.x == 0) {
+        parts[2 * blockIdx.x] = ts;
+        parts[2 * blockIdx.x + 1] = tc;
+    }
+}
+
+// the mean inside the mask: sum / count in float, 0 for an empty mask (apply_mask.cu:198-202)
+__global__ void k_reg_mean(const double* __restrict__ parts, int nparts, float* __restrict__ mean) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < nparts; ++k) {
+        s += parts[2 * k];
+        c += parts[2 * k + 1];
+    }
+    mean[0] = c > 0.0 ? (float)s / (float)c : 0.f;
+}
+
+// out = (in - mean) * mask (rescale_in_mask + apply_mask), as the real part of a complex plane
+__global__ void __launch_bounds__(256) k_reg_prepare(float2* __restrict__ out, const float* __restrict__ in,
+                                                     const float* __restrict__ mask, size_t n,
+                                                     const float* __restrict__ mean) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[i] = make_float2((in[i] - mean[0]) * mask[i], 0.f);
+}
+
+// cur *= conj(ref)
+__global__ void __launch_bounds__(256) k_reg_cross(float2* __restrict__ cur, const float2* __restrict__ ref, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float2 a = cur[i], b = ref[i];
+    cur[i] = make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// (|value|, index) maxima per block, first index on ties
+__global__ void __launch_bounds__(256) k_reg_peak_parts(const float2* __restrict__ xc, size_t n,
+                                                        float* __restrict__ pv, uint32_t* __restrict__ pi) {
+    __shared__ float sv[256];
+    __shared__ uint32_t si[256];
+    float best = -1.f;
+    uint32_t bi = 0xFFFFFFFFu;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const float v = fabsf(xc[i].x);
+        if (v > best) {  // increasing i per thread: the first maximum stays
+            best = v;
+            bi = (uint32_t)i;
+        }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const float v2 = sv[threadIdx.x + o];
+            const uint32_t i2 = si[threadIdx.x + o];
+            if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+                sv[threadIdx.x] = v2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pv[blockIdx.x] = sv[0];
+        pi[blockIdx.x] = si[0];
+    }
+}
+
+// the peak's (x, y), wrapped to the signed shift and negated (registration.cc:46-55)
+__global__ void k_reg_peak_final(const float* __restrict__ pv, const uint32_t* __restrict__ pi, int nparts, int W,
+                                 int H, int* __restrict__ shift) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float best = -1.f;
+    uint32_t bi = 0;
+    for (int k = 0; k < nparts; ++k)
+        if (pv[k] > best || (pv[k] == best && pi[k] < bi)) {
+            best = pv[k];
+            bi = pi[k];
+        }
+    int x = (int)(bi % (uint32_t)W), y = (int)(bi / (uint32_t)W);
+    if (x > W / 2) x -= W;
+    if (y > H / 2) y -= H;
+    shift[0] = -x;
+    shift[1] = -y;
+}
+
+// circ_shift (tools.cu:40-79): out[(y + sy) mod H][(x + sx) mod W] = in[y][x]
+__global__ void __launch_bounds__(256) k_reg_shift(float* __restrict__ out, const float* __restrict__ in, int W, int H,
+                                                   const int* __restrict__ shift) {
+    const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    int nx = (x + shift[0]) % W, ny = (y + shift[1]) % H;
+    nx += W * (nx < 0);
+    ny += H * (ny < 0);
+    out[(size_t)ny * W + nx] = in[(size_t)y * W + x];
+}
+
+}  // namespace
+
+namespace hv {
+
+// mask (P floats) + reference spectrum and work plane (P complex each) + partials + shift
+size_t registration_scratch_bytes(size_t P) {
+    return P * sizeof(float) + 2 * P * sizeof(float2) + 2 * kRegParts * sizeof(double) + kRegParts * 8 + 256;
+}
+
+struct RegScratch {
+    float* mask;
+    float2* ref;
+    float2* work;
+    double* parts;
+    float* pv;
+    uint32_t* pi;
+    int* shift;
+    float* mean;
+};
+
+static RegScratch reg_layout(void* scratch, size_t P) {
+    char* c = (char*)scratch;
+    RegScratch r;
+    r.mask = (float*)c;
+    c += P * sizeof(float);
+    r.ref = (float2*)c;
+    c += P * sizeof(float2);
+    r.work = (float2*)c;
+    c += P * sizeof(float2);
+    r.parts = (double*)c;
+    c += 2 * kRegParts * sizeof(double);
+    r.pv = (float*)c;
+    c += kRegParts * sizeof(float);
+    r.pi = (uint32_t*)c;
+    c += kRegParts * sizeof(uint32_t);
+    r.shift = (int*)c;
+    r.mean = (float*)(c + 16);
+    return r;
+}
+
+hv_status launch_registration_mask(void* scratch, int W, int H, float zone, int swap, hipStream_t st) {
+    RegScratch r = reg_layout(scratch, (size_t)W * H);
+    const float radius = (float)(std::min(W, H) / 2) * zone;
+    hipLaunchKernelGGL(k_reg_mask, dim3((W + 255) / 256, H), dim3(256), 0, st, r.mask, W, H, W / 2.0f, H / 2.0f, radius,
+                       swap);
+    HV_CHECK_LAUNCH("registration mask");
+    return HV_OK;
+}
+
+static hv_status reg_prepare(const RegScratch& r, float2* out, const float* img, size_t P, hipStream_t st) {
+    const int nparts = (int)std::min<size_t>(kRegParts, std::max<size_t>(1, (P + 8191) / 8192));
+    hipLaunchKernelGGL(k_reg_mean_parts, dim3(nparts), dim3(256), 0, st, img, r.mask, P, r.parts);
+    hipLaunchKernelGGL(k_reg_mean, dim3(1), dim3(64), 0, st, r.parts, nparts, r.mean);
+    hipLaunchKernelGGL(k_reg_prepare, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, out, img, r.mask, P, r.mean);
+    HV_CHECK_LAUNCH("registration prepare");
+    return HV_OK;
+}
+
+// the reference image: prepared, its spectrum kept
+hv_status launch_registration_set_reference(void* scratch, const float* img, int W, int H, hipStream_t st) {
+    const size_t P = (size_t)W * H;
+    RegScratch r = reg_layout(scratch, P);
+    HV_TRY(reg_prepare(r, r.ref, img, P, st));
+    return launch_fft2(r.ref, 1, W, H, -1, st);
+}
+
+// shifts img (in place, through tmp) onto the reference
+hv_status launch_registration(void* scratch, float* img, float* tmp, int W, int H, hipStream_t st) {
+    const size_t P = (size_t)W * H;
+    RegScratch r = reg_layout(scratch, P);
+    HV_TRY(reg_prepare(r, r.work, img, P, st));
+    HV_TRY(launch_fft2(r.work, 1, W, H, -1, st));
+    hipLaunchKernelGGL(k_reg_cross, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, r.work, r.ref, P);
+    HV_TRY(launch_fft2(r.work, 1, W, H, 1, st));
+    const int nparts = (int)std::min<size_t>(kRegParts, std::max<size_t>(1, (P + 4095) / 4096));
+    hipLaunchKernelGGL(k_reg_peak_parts, dim3(nparts), dim3(256), 0, st, r.work, P, r.pv, r.pi);
+    hipLaunchKernelGGL(k_reg_peak_final, dim3(1), dim3(64), 0, st, r.pv, r.pi, nparts, W, H, r.shift);
+    hipLaunchKernelGGL(k_reg_shift, dim3((W + 255) / 256, H), dim3(256), 0, st, tmp, img, W, H, r.shift);
+    HV_CHECK_LAUNCH("registration");
+    HV_HIP(hipMemcpyAsync(img, tmp, P * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return HV_OK;
+}
+
+hv_status registration_shift(void* scratch, size_t P, int* shift_host) {
+    RegScratch r = reg_layout(scratch, P);
+    HV_HIP(hipMemcpy(shift_host, r.shift, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    return HV_OK;
+}
+
+}  // namespace hv

@@ -393,7 +393,8 @@ def compute_settings_to_json(s: ComputeSettings) -> dict:
         "z": {"width": int(s.p_width), "start": int(s.p_start)}, "z2": {"width": 0, "start": 0},
         "window": {"xy": xy, "yz": _cut_view(s), "xz": _cut_view(s), "filter2d": _view_window()},
         "renorm": bool(s.renorm_enabled), "reticle": {"display_enabled": False, "scale": 0.5},
-        "registration": {"registration_enabled": False, "registration_zone": _f32(0.7)},
+        "registration": {"registration_enabled": bool(s.registration_enabled),
+                         "registration_zone": _f32(s.registration_zone)},
     }
     composite = {"mode": "RGB", "auto_weight": False,
                  "rgb": {"frame_index": {"min": 0, "max": 0}, "weight": {"r": 1.0, "g": 1.0, "b": 1.0}},
@@ -416,9 +417,9 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     the footer info.  Settings the JSON does not carry (unwrap, arithmetic
     precisions, contrast subsample, the automatic contrast policy) come from
     `base` (default: hv_default_settings).  Returns (settings, ignored): `ignored` names the
-    enabled steps the path does not run (registration, band-pass filters,
-    output flip/rotation); renormalisation, log10 and image accumulation of
-    the xy view are mapped.
+    enabled steps the path does not run (band-pass filters, output
+    flip/rotation); renormalisation, registration, log10, image accumulation
+    and the cuts are mapped.
     Raises UnsupportedSettings for computations outside the path: raw mode,
     SSA_STFT, phase increase and composite images."""
     if base is None:
@@ -487,8 +488,9 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     # steps outside the path that the footer switches on
     if xy.get("horizontal_flip") or float(xy.get("rotation", 0.0) or 0.0) != 0.0:
         ignored.append("view.window.xy.horizontal_flip/rotation")
-    if v.get("registration", {}).get("registration_enabled"):
-        ignored.append("view.registration")
+    reg = v.get("registration", {})
+    s.registration_enabled = int(bool(reg.get("registration_enabled", False)))
+    s.registration_zone = float(reg.get("registration_zone", s.registration_zone))
     if r.get("filter2d", {}).get("enabled"):
         ignored.append("image_rendering.filter2d")
     if r.get("input_filter", {}).get("type", "None") not in ("None", ""):

@@ -328,6 +328,11 @@ typedef struct hv_compute_settings {
      * mean over columns [x_start, x_start + x_width] of the STFT image values */
     int32_t cuts_enabled;
     uint32_t x_start, x_width, y_start, y_width;
+    /* RegistrationEnabled / RegistrationZone (registration.cc:9-122): shift
+     * each image onto the first one by the peak of their cross-correlation
+     * over a centred disc of radius zone * min(W, H) / 2 */
+    int32_t registration_enabled;
+    float registration_zone;
 } hv_compute_settings;
 
 /* Fills the reference defaults (Backend/includes/core/holovibes.hh:359-447). */
@@ -436,6 +441,10 @@ hv_status hv_pipe_sync(hv_pipe* p);
 /* The latest cycle's XZ (T x W) and YZ (H x T) cut images (device floats,
  * valid until the next push); settings.cuts_enabled. */
 hv_status hv_pipe_cuts(hv_pipe* p, const float** xz, const float** yz);
+
+/* The (x, y) circular shift registration applied to the latest image
+ * (blocking read; settings.registration_enabled). */
+hv_status hv_pipe_registration_shift(hv_pipe* p, int* shift_x, int* shift_y);
 
 /* RecordMode::MOMENTS (Backend/sources/core/pipe.cc:429-455, 583-598): every
  * cycle also writes the three moment frames M0, M1, M2 of the window (the

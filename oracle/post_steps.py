@@ -81,3 +81,41 @@ def cuts(stft: np.ndarray, xmin: int, ymin: int, xmax: int, ymax: int, img_type:
         yz = yz + v[:, :, x].T
     yz = yz / np.float32(xmax - xmin + 1)
     return xz, yz
+
+
+def registration_mask(W: int, H: int, zone: float, swap: bool) -> np.ndarray:
+    """get_circular_mask (Backend/cuda_sources/masks.cu:61-77): 1 inside the
+    disc of radius (min(W, H) / 2) * zone around (W/2, H/2), quadrant-swapped
+    when exactly one of Fresnel and the fft shift holds (registration.cc:100-110)."""
+    y, x = np.mgrid[0:H, 0:W].astype(np.float32)
+    r = np.float32(min(W, H) // 2) * np.float32(zone)
+    m = (((x - np.float32(W / 2)) ** 2 + (y - np.float32(H / 2)) ** 2) <= r * r).astype(np.float32)
+    return np.fft.fftshift(m) if swap else m
+
+
+def registration_prepare(img: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """image_preprocess (registration.cc:58-63): the mean inside the mask
+    removed inside it, 0 outside."""
+    img = np.asarray(img, np.float32)
+    count = float(mask.sum(dtype=np.float64))
+    mean = np.float32(np.float32((img * mask).sum(dtype=np.float64)) / np.float32(count)) if count > 0 else np.float32(0)
+    return (img - mean) * mask
+
+
+def register(img: np.ndarray, ref: np.ndarray, mask: np.ndarray):
+    """One registration (registration.cc:11-56): the peak (first largest
+    |value|) of the cross-correlation of the prepared images, wrapped to a
+    signed shift and negated; the image shifted circularly by it.  Returns
+    (shifted image, (shift_x, shift_y))."""
+    a = np.fft.fft2(registration_prepare(img, mask).astype(np.float64))
+    b = np.fft.fft2(registration_prepare(ref, mask).astype(np.float64))
+    xc = np.fft.ifft2(a * np.conj(b)).real
+    H, W = img.shape
+    idx = int(np.argmax(np.abs(xc)))
+    x, y = idx % W, idx // W
+    if x > W // 2:
+        x -= W
+    if y > H // 2:
+        y -= H
+    sx, sy = -x, -y
+    return np.roll(np.asarray(img, np.float32), (sy, sx), axis=(0, 1)), (sx, sy)

@@ -47,7 +47,7 @@ def test_settings_struct_layout(lib):
     assert s.camera_fps == 0.0 and s.fft_shift == 0 and s.unwrap_2d == 0
     # the last fields, written by the library at their C offsets
     assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (0, 5, 1, 0)
-    assert ctypes.sizeof(ComputeSettings) == 36 * 4
+    assert ctypes.sizeof(ComputeSettings) == 38 * 4
 
 
 def test_validate_settings(lib):

@@ -78,3 +78,23 @@ def test_cuts_closed_forms_and_brute_force():
         for y in range(H):
             acc = sum(abs(complex(z[b, y, x])) for x in range(1, 4)) / 3
             assert abs(yz[y, b] - acc) <= 1e-6 * acc
+
+
+def test_registration_recovers_circular_shifts():
+    """A smooth blob moved by a circular shift is moved back exactly, for
+    shifts either side of the origin; the mask is a disc (swapped: its
+    quadrants exchanged)."""
+    H, W = 64, 96
+    y, x = np.mgrid[0:H, 0:W].astype(np.float32)
+    img = (np.exp(-((x - 40) ** 2 + (y - 30) ** 2) / 60.0) + 0.5 * np.exp(-((x - 55) ** 2 + (y - 26) ** 2) / 20.0)
+           ).astype(np.float32)
+    mask = ps.registration_mask(W, H, 0.7, False)
+    r = np.float32(32) * np.float32(0.7)
+    assert mask[32, 48] == 1 and mask[0, 0] == 0 and int(mask.sum()) == int(
+        (((x - 48) ** 2 + (y - 32) ** 2) <= r * r).sum())
+    assert np.array_equal(ps.registration_mask(W, H, 0.7, True), np.fft.fftshift(mask))
+    for dy, dx in [(3, -5), (-7, 2), (0, 0), (5, 9)]:
+        moved = np.roll(img, (dy, dx), axis=(0, 1))
+        out, (sx, sy) = ps.register(moved, img, mask)
+        assert (sx, sy) == (-dx, -dy)
+        assert np.array_equal(out, img)
