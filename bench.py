@@ -64,6 +64,8 @@ def parse():
                     help=".holo recording I/O: file (page cache) -> HBM and HBM -> file throughput")
     ap.add_argument("--holo-frames", type=int, default=256, help="--holo-io: 2048x2048 u16 frames in the file")
     ap.add_argument("--holo-dir", default="/tmp", help="--holo-io: directory of the recording")
+    ap.add_argument("--record", choices=["moments", "hologram", "raw"], default=None,
+                    help="frames/s of the workload at one lane with and without recording into a .holo file")
     ap.add_argument("--input-side", action="store_true",
                     help="packed 12/10-bit unpack (HBM roofline) and input-queue enqueue throughput")
     ap.add_argument("--rate-scale", type=float, default=1.0, help="serving: multiply every source's batch rate")
@@ -422,10 +424,67 @@ def input_side(args):
     print(json.dumps(line))
 
 
+def record(args):
+    """The workload at one lane, eager, with and without the record path
+    (every cycle's frames written from HBM into a .holo file under
+    --holo-dir): frames/s of both and the record's share of a cycle."""
+    import numpy as np
+    import torch
+
+    from holovibes_amd import configs
+    from holovibes_amd.pipe import Pipe, synth_frames
+    from holovibes_amd.record import Recorder
+
+    torch.cuda.set_device(0)
+    s = configs.settings_for(args.workload)
+    sp = configs.synth_for(args.workload)
+    B = s.batch_size
+    dev = [torch.from_numpy(h.view(np.int16) if h.dtype == np.uint16 else h).cuda()
+           for h in (synth_frames(sp, k * B, B) for k in range(2))]
+    steps = max(8, args.steps // 10)
+    path = os.path.join(args.holo_dir, f"hv_record_{os.getpid()}.holo")
+
+    def run(rec_mode):
+        p = Pipe(s)
+        rec = Recorder(path, p, rec_mode, 1 << 30) if rec_mode else None
+        for k in range(args.warmup // 4 + 2):
+            made = p.push(dev[k % 2])
+            if rec:
+                rec.after_push(dev[k % 2], made)
+        p.sync()
+        n0 = rec.recorded if rec else 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            made = p.push(dev[k % 2])
+            if rec:
+                rec.after_push(dev[k % 2], made)
+        p.sync()
+        el = time.perf_counter() - t0
+        n = rec.recorded - n0 if rec else 0
+        if rec:
+            rec.close(input_fps=1)
+            os.remove(path)
+        p.close()
+        return B * steps / el, n, el
+
+    base, _, _ = run(None)
+    with_rec, frames, el = run(args.record)
+    frame_bytes = {"moments": 4, "hologram": 2, "raw": s.depth}[args.record] * s.width * s.height
+    line = {"metric": f"{args.workload} frames/s with the {args.record} record", "value": with_rec, "unit": "frames/s",
+            "n_gpus": 1, "steps": steps, "higher_is_better": True, "data": "synthetic (seeded interferogram generator)",
+            "without_record": base, "record_share": 1.0 - with_rec / base, "frames_recorded": frames,
+            "record_GBps": frames * frame_bytes / el / 1e9,
+            "config": {"workload": args.workload, "record": args.record, "lanes": 1}}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.serving:
         serving(args)
+        return
+    if args.record:
+        record(args)
         return
     if args.input_side:
         input_side(args)

@@ -50,6 +50,12 @@ namespace {
 #ifndef HV_HOLO_WRITE_MMAP
 #define HV_HOLO_WRITE_MMAP 1
 #endif
+// writes below this size use pwrite: a mapping costs ~5 ms per call to set up
+// and fault in, so it pays only for large writes (measured: 17 MB 2.9 ms with
+// pwrite vs 7.3 ms mapped; 2 GiB 9.6 GB/s with pwrite vs 14.2 GB/s mapped)
+#ifndef HV_HOLO_MAP_MIN_MIB
+#define HV_HOLO_MAP_MIN_MIB 256
+#endif
 
 // chunk of one staging buffer; two are used per reader or writer
 constexpr size_t kChunk = size_t(HV_HOLO_CHUNK_MIB) << 20;
@@ -195,7 +201,7 @@ using namespace hv;
 // systems without fallocate take the pwrite path.
 static hv_status reserve_frames(hv_holo_writer* w, uint64_t total, bool* use_map) {
     *use_map = false;
-    if (!HV_HOLO_WRITE_MMAP || total < kParallelMin) return HV_OK;
+    if (!HV_HOLO_WRITE_MMAP || total < (size_t(HV_HOLO_MAP_MIN_MIB) << 20)) return HV_OK;
     if (::fallocate(w->fd, 0, (off_t)w->pos, (off_t)total) == 0) {
         *use_map = true;
         return HV_OK;

@@ -309,16 +309,19 @@ def test_footer_dump_is_compact_sorted(holo, tmp_path):
 
 
 def test_large_writes_through_mapping(holo, tmp_path):
-    """Writes of 4 MiB and more reserve the file's blocks and go through a
-    shared mapping; the file is byte for byte the one the oracle writes, the
-    footer and the frame-count correction included."""
-    frames = _frames(5, 1024, 1024, 2, seed=11)
+    """A write of 256 MiB or more reserves the file's blocks and goes
+    through a shared mapping, smaller ones through pwrite; the file is byte
+    for byte the one the oracle writes, footer and frame-count correction
+    included."""
+    rng = np.random.default_rng(11)
+    frames = rng.integers(0, 65536, (132, 1024, 1024), dtype=np.uint16)  # 129 frames = 258 MiB in one call
     path = str(tmp_path / "big.holo")
-    with holo.OutputHoloFile(path, holo.FrameDescriptor(1024, 1024, 2), 8) as out:
-        out.meta_data = {"info": {"contiguous": 5}}
+    with holo.OutputHoloFile(path, holo.FrameDescriptor(1024, 1024, 2), 140) as out:
+        out.meta_data = {"info": {"contiguous": 132}}
         out.write_header()
-        out.write_frame(frames[:2])
-        out.write_frame(frames[2:])
+        out.write_frame(frames[:3])
+        out.write_frame(frames[3:])
         out.write_footer()
-        out.correct_number_of_frames(5)
-    assert open(path, "rb").read() == hf.holo_bytes(frames, 2, footer=holo.dump_footer({"info": {"contiguous": 5}}))
+        out.correct_number_of_frames(132)
+    data = open(path, "rb").read()
+    assert data == hf.holo_bytes(frames, 2, footer=holo.dump_footer({"info": {"contiguous": 132}}))
