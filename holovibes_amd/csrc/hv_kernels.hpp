@@ -98,6 +98,10 @@ hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, in
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,
                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1, float fps = 0.f);
 
+// M0, M1, M2 of the PCA output in one projection (MFMA path; three otherwise)
+hv_status launch_pca_moments3(float* m0, float* m1, float* m2, const float2* ring, const double2* evecs, int T,
+                              size_t P, int fs, int fe, float fps, hipStream_t st);
+
 // hv_post.hip: post-processing of the float image (P a multiple of 4)
 size_t renorm_scratch_bytes();
 hv_status launch_renormalize(float* img, size_t P, unsigned norm_constant, void* scratch, hipStream_t st);

@@ -1146,10 +1146,13 @@ __global__ void __launch_bounds__(256)
 // is summed over the components (the 32 lanes of a half-wavefront) by
 // shuffles.  NC chunks are held at once so that the window is read once for up
 // to 32 NC components.
-template <int T, int NC>
+// M3: the three moments at once (out, out1, out2), each weighted |projection|^2
+// summed exactly as the single-moment kernel sums it.
+template <int T, int NC, bool M3 = false>
 __global__ void __launch_bounds__(256)
     k_pca_image_mfma(float* __restrict__ out, const float2* __restrict__ ring, const double2* __restrict__ evecs,
-                     size_t P, PcaOut po, int pmin, int pmax, uint32_t* mm, int mm_slots) {
+                     size_t P, PcaOut po, int pmin, int pmax, uint32_t* mm, int mm_slots, float* __restrict__ out1 = nullptr,
+                     float* __restrict__ out2 = nullptr) {
     constexpr int W = 32 * NC;  // components staged per pass
     __shared__ float2 vs[T * W];   // [frame][component]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1159,8 +1162,14 @@ __global__ void __launch_bounds__(256)
     const float2* xp = ring + (size_t)half * P + (tile_ok ? px0 : 0) + col;
     const int nsel = pmax - pmin + 1;
     float rowacc[16];
+    float rowacc1[M3 ? 16 : 1], rowacc2[M3 ? 16 : 1];
 #pragma unroll
     for (int r = 0; r < 16; ++r) rowacc[r] = 0.f;
+    if constexpr (M3) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) rowacc1[r] = rowacc2[r] = 0.f;
+    }
+    const PcaOut po1{HV_IMG_MOMENTS_1, po.T, po.fps}, po2{HV_IMG_MOMENTS_2, po.T, po.fps};
     for (int c0 = 0; c0 < nsel; c0 += W) {
         __syncthreads();  // the previous pass's columns have been read
         for (int e = threadIdx.x; e < T * W; e += blockDim.x) {
@@ -1191,22 +1200,45 @@ __global__ void __launch_bounds__(256)
             const bool ok = j < nsel;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                float tv = ok ? po.term(make_float2(cr[c][r], ci[c][r]), pmin + j) : 0.f;
+                const float2 z = make_float2(cr[c][r], ci[c][r]);
+                float tv = ok ? po.term(z, pmin + j) : 0.f;
 #pragma unroll
                 for (int o = 16; o >= 1; o >>= 1) tv += __shfl_xor(tv, o, 64);
                 rowacc[r] += tv;
+                if constexpr (M3) {
+                    float t1 = ok ? po1.term(z, pmin + j) : 0.f;
+                    float t2 = ok ? po2.term(z, pmin + j) : 0.f;
+#pragma unroll
+                    for (int o = 16; o >= 1; o >>= 1) {
+                        t1 += __shfl_xor(t1, o, 64);
+                        t2 += __shfl_xor(t2, o, 64);
+                    }
+                    rowacc1[r] += t1;
+                    rowacc2[r] += t2;
+                }
             }
         }
     }
     // lane (half, col < 16) writes pixel row 8 (col / 4) + 4 half + col % 4
-    float v = 0.f;
+    float v = 0.f, v1 = 0.f, v2 = 0.f;
 #pragma unroll
     for (int r = 0; r < 16; ++r)
-        if (r == col) v = rowacc[r];
+        if (r == col) {
+            v = rowacc[r];
+            if constexpr (M3) {
+                v1 = rowacc1[r];
+                v2 = rowacc2[r];
+            }
+        }
     float vmin = INFINITY, vmax = -INFINITY;
     if (tile_ok && col < 16) {
         const float val = po.finish(v, nsel);
-        out[px0 + 8 * (col / 4) + 4 * half + (col % 4)] = val;
+        const size_t o = px0 + 8 * (col / 4) + 4 * half + (col % 4);
+        out[o] = val;
+        if constexpr (M3) {
+            out1[o] = v1;
+            out2[o] = v2;
+        }
         vmin = vmax = val;
     }
     if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
@@ -1226,6 +1258,37 @@ static hv_status pca_image_mfma(float* out, const float2* ring, const double2* e
     else
         k_pca_image_mfma<T, 1><<<g, 256, 0, st>>>(out, ring, evecs, P, po, pmin, pmax, mm, mm_slots);
     HV_CHECK_LAUNCH("k_pca_image_mfma");
+    return HV_OK;
+}
+
+template <int T>
+static hv_status pca_moments3_mfma(float* m0, float* m1, float* m2, const float2* ring, const double2* evecs, size_t P,
+                                   const PcaOut& po, int fs, int fe, hipStream_t st) {
+    const unsigned g = (unsigned)((P + 127) / 128);
+    if (T <= 128 && fe - fs + 1 > 32)
+        k_pca_image_mfma<T, (T <= 128 ? 2 : 1), true>
+            <<<g, 256, 0, st>>>(m0, ring, evecs, P, po, fs, fe, nullptr, 1, m1, m2);
+    else
+        k_pca_image_mfma<T, 1, true><<<g, 256, 0, st>>>(m0, ring, evecs, P, po, fs, fe, nullptr, 1, m1, m2);
+    HV_CHECK_LAUNCH("k_pca_image_mfma<moments>");
+    return HV_OK;
+}
+
+hv_status launch_pca_moments3(float* m0, float* m1, float* m2, const float2* ring, const double2* evecs, int T,
+                              size_t P, int fs, int fe, float fps, hipStream_t st) {
+    const PcaOut po{HV_IMG_MOMENTS_0, T, fps};
+    if (HV_PCA_PROJ_MFMA && T >= 32 && P % 128 == 0) {
+        switch (T) {
+            case 32: return pca_moments3_mfma<32>(m0, m1, m2, ring, evecs, P, po, fs, fe, st);
+            case 64: return pca_moments3_mfma<64>(m0, m1, m2, ring, evecs, P, po, fs, fe, st);
+            case 128: return pca_moments3_mfma<128>(m0, m1, m2, ring, evecs, P, po, fs, fe, st);
+            default: break;
+        }
+    }
+    // one projection per moment elsewhere
+    for (int k = 0; k < 3; ++k)
+        HV_TRY(launch_pca_image(k == 0 ? m0 : (k == 1 ? m1 : m2), ring, evecs, T, P, HV_IMG_MOMENTS_0 + k, fs, fe,
+                                nullptr, st, 1, fps));
     return HV_OK;
 }
 

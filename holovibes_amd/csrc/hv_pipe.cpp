@@ -311,9 +311,8 @@ hv_status hv_pipe::image_work() {
         const int fs = (int)s.p_start;
         const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
         if (s.time_transformation == HV_TIME_PCA) {
-            for (int k = 0; k < 3; ++k)
-                HV_TRY(launch_pca_image(rec + (size_t)k * P, L.ring, L.evecs, Tn, P, HV_IMG_MOMENTS_0 + k, fs, fe,
-                                        nullptr, L.stream, kPipeMm, s.camera_fps));
+            HV_TRY(launch_pca_moments3(rec, rec + P, rec + 2 * P, L.ring, L.evecs, Tn, P, fs, fe, s.camera_fps,
+                                       L.stream));
         } else {
             HV_TRY(launch_stft_moments(rec, rec + P, rec + 2 * P, L.ring, Tn, P, fs, fe, s.camera_fps, nullptr, 0,
                                        L.stream, kPipeMm));
