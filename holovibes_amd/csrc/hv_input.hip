@@ -350,6 +350,8 @@ hv_status hv_input_queue_enqueue_packed(hv_input_queue* q, const void* packed_fr
     if (memcpy_kind == (int)hipMemcpyHostToDevice && q->packed_bytes < pfs * q->batch_size) {
         // one batch of packed frames; reused chunk after chunk on the producer stream
         if (q->packed) {
+            // the old staging may still feed an unpack of a batch in progress
+            HV_HIP(hipStreamSynchronize(st));
             HV_TRY(sync_all(q));
             HV_HIP(hipFree(q->packed));
             q->packed = nullptr;
