@@ -121,6 +121,12 @@ __global__ void __launch_bounds__(256) k_time_reg(const float2* __restrict__ rin
 // ---------------------------------------------------------------------------
 // T = 128, 256: CW pixels per workgroup, NT threads per pixel, LDS exchange
 // ---------------------------------------------------------------------------
+// T = 128, 256 moments: the bins' sums shared by the pixel's threads (1) or
+// taken by one thread per pixel (0)
+#ifndef HV_TIME_MOMENTS_SPLIT
+#define HV_TIME_MOMENTS_SPLIT 1
+#endif
+
 template <int T>
 struct TimeLdsCfg {
     static constexpr int E = 16;
@@ -145,6 +151,51 @@ __global__ void __launch_bounds__(TimeLdsCfg<T>::THREADS)
             if (valid) prm.spec[(size_t)n * P + p] = z;
         };
         fft_run<T, C::E, -1>(t, load, store, lds_at);
+    } else if constexpr (MODE == TM_MOMENTS && HV_TIME_MOMENTS_SPLIT) {
+        // the pixel's NT threads each sum the bins fs + t, fs + t + NT, ...;
+        // thread 0 adds the NT partial sums in order (and the Nyquist rule)
+        __shared__ float red[3][C::NT][C::CW];
+        auto store = [&](int n, float2 z) { lds_at(n) = z; };
+        fft_run<T, C::E, -1, true>(t, load, store, lds_at);
+        __syncthreads();
+        const float d = prm.fps / T;
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int i = prm.pmin + t; i <= prm.pmax; i += C::NT) {
+            const float2 z = lds_at(i);
+            const float sq = z.x * z.x + z.y * z.y;
+            const float f = freq1<T>(i, d);
+            a0 += sq * 1.0f;
+            a1 += sq * f;
+            a2 += sq * (f * f);
+        }
+        red[0][t][c] = a0;
+        red[1][t][c] = a1;
+        red[2][t][c] = a2;
+        __syncthreads();
+        if (t == 0 && valid) {
+            a0 = a1 = a2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < C::NT; ++k) {
+                a0 += red[0][k][c];
+                a1 += red[1][k][c];
+                a2 += red[2][k][c];
+            }
+            constexpr int nyq = T / 2;
+            if (nyq >= prm.pmin && nyq <= prm.pmax) {
+                const float2 z = lds_at(nyq);
+                const float sq = z.x * z.x + z.y * z.y;
+                const float f = freq1<T>(nyq, d);
+                a0 += sq * 1.0f;
+                a1 -= sq * f;
+                a2 += sq * (f * f);
+            }
+            if (prm.out) prm.out[p] = a0;
+            if (prm.m1) prm.m1[p] = a1;
+            if (prm.m2) prm.m2[p] = a2;
+            const float sel = prm.mm_moment == 0 ? a0 : prm.mm_moment == 1 ? a1 : a2;
+            vmin = vmax = sel;
+        }
+        if (prm.mm) block_minmax_atomic(vmin, vmax, prm.mm, prm.mm_slots != 0 ? prm.mm_slots : 1);
     } else {
         auto store = [&](int n, float2 z) { lds_at(n) = z; };
         fft_run<T, C::E, -1, true>(t, load, store, lds_at);
