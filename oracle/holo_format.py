@@ -1,8 +1,7 @@
 """CPU restatement of the .holo file format (test infrastructure only).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ and by the CPU-baseline leg of
-bench.py as the checker of the native reader/writer (hv_holo_*); the
-product package never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the native
+reader/writer (hv_holo_*); the product package never imports it.
 
 Format (Backend/includes/io/holo_file.hh:58-83, #pragma pack(2), 64 bytes):
   magic "HOLO" | u16 version | u16 bits_per_pixel | u32 width | u32 height |

@@ -2,7 +2,7 @@
 Pipe (pipe.cc:295-455 restated), batch by batch, on seeded synthetic frames.
 
 Bars: float image (before contrast) within relative L2 1e-5 of the FP64
-oracle for FP32 paths (PCA split-BF16: 1e-4); u16
+oracle for FP32 paths (PCA split-BF16: 1e-4, the tolerance stated for the path); u16
 output within 1 count on every pixel and identical on >= 95 % (truncation
 to an integer flips a pixel whenever the ~1e-6 float error straddles an
 integer); produced/discarded cycles identical.
