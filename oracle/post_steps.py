@@ -2,8 +2,9 @@
 (test infrastructure only).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the pipe's
-renormalisation, image accumulation and log10; the product package never
-imports it.  float32 arithmetic in the reference's operation order.
+renormalisation, image accumulation, log10, registration and XZ/YZ cuts;
+the product package never imports it.  float32 arithmetic in the
+reference's operation order.
 """
 from __future__ import annotations
 
