@@ -288,8 +288,23 @@ hv_status hv_pipe::image_work() {
             // moments of the PCA output (gpu_p_acc_buffer = H V, fourier_transform.cc:340-377)
             HV_TRY(launch_pca_cov(L.cov, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
             HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, Tn, L.eig_ws, L.stream));
-            HV_TRY(launch_pca_image(L.image, L.ring, L.evecs, Tn, P, it, fs, fe, mm_img, L.stream, kPipeMm,
-                                    s.camera_fps));
+            if (rec) {
+                // the moments record's projection gives the image too
+                HV_TRY(launch_pca_moments3(rec, rec + P, rec + 2 * P, L.ring, L.evecs, Tn, P, fs, fe, s.camera_fps,
+                                           L.stream));
+                HV_HIP(hipMemcpyAsync(L.image, rec + (size_t)(it - HV_IMG_MOMENTS_0) * P, P * sizeof(float),
+                                      hipMemcpyDeviceToDevice, L.stream));
+                if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kPipeMm));
+            } else {
+                HV_TRY(launch_pca_image(L.image, L.ring, L.evecs, Tn, P, it, fs, fe, mm_img, L.stream, kPipeMm,
+                                        s.camera_fps));
+            }
+        } else if (rec) {
+            // the moments record's pass gives the image too
+            HV_TRY(launch_stft_moments(rec, rec + P, rec + 2 * P, L.ring, Tn, P, fs, fe, s.camera_fps, mm_img,
+                                       it - HV_IMG_MOMENTS_0, L.stream, kPipeMm));
+            HV_HIP(hipMemcpyAsync(L.image, rec + (size_t)(it - HV_IMG_MOMENTS_0) * P, P * sizeof(float),
+                                  hipMemcpyDeviceToDevice, L.stream));
         } else {
             float* m0 = it == HV_IMG_MOMENTS_0 ? L.image : nullptr;
             float* m1 = it == HV_IMG_MOMENTS_1 ? L.image : nullptr;
@@ -307,7 +322,7 @@ hv_status hv_pipe::image_work() {
     }
     // moments record (insert_moments + insert_moments_record, pipe.cc:429-455,
     // 583-598): the three moment frames of the window, whatever the image type
-    if (rec) {
+    if (rec && !is_moment_kind(it)) {  // a moment image's pass has written the record already
         const int fs = (int)s.p_start;
         const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
         if (s.time_transformation == HV_TIME_PCA) {
