@@ -220,6 +220,12 @@ struct hv_pipe {
                !(s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d) && !s.fft_shift && !post_steps() &&
                s.space_transformation != HV_SPACE_NONE;
     }
+    // registration's launch sequence is fixed once its reference is taken and
+    // the accumulation queue is full: until then its cycles run eagerly
+    bool registration_steady() const {
+        const uint32_t level = std::max<uint32_t>(1, s.img_accu_level);
+        return !s.registration_enabled || (reg_ref_set && (level <= 1 || accu_held == level));
+    }
     // renormalisation, accumulation or log10 change the image after its kernel
     bool post_steps() const {
         return s.renorm_enabled || s.img_accu_level > 1 || s.log_enabled || s.registration_enabled;
@@ -651,7 +657,7 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     if (p->timing) {
         HV_TRY(p->enqueue_cycle(frames, slot0));
         HV_TRY(p->collect_times());
-    } else if (p->graph_enabled && !pending && !p->shared_ring() && !p->s.registration_enabled) {
+    } else if (p->graph_enabled && !pending && !p->shared_ring() && p->registration_steady()) {
         // one graph per (ring phase, input buffer); past the cache bound the
         // batch is staged into the pipe's own input buffer first (one lane)
         // or run without a graph (several lanes share no staging buffer)

@@ -15,13 +15,18 @@ pytestmark = pytest.mark.gpu
 SHIFTS = [(0, 0), (6, -9), (-4, 13), (11, 3)]  # (dy, dx) of each batch's frames
 
 
-def _run(s, batches):
+def _run(s, batches, graph=False):
     from holovibes_amd import _lib
     from holovibes_amd.pipe import Pipe
-    p = Pipe(s)
+    p = Pipe(s, graph=graph)
     imgs, shifts = [], []
-    for fr in batches:
-        assert p.push(torch.from_numpy(fr.view(np.int16)).cuda())
+    # two input buffers reused in turn: from the second steady cycle on, graphs replay
+    bufs = [torch.empty(batches[0].shape, dtype=torch.int16, device="cuda") for _ in range(2)]
+    for k, fr in enumerate(batches):
+        buf = bufs[k % 2]
+        buf.copy_(torch.from_numpy(fr.view(np.int16)))
+        torch.cuda.synchronize()
+        assert p.push(buf)
         imgs.append(p.image().cpu().numpy())
         if s.registration_enabled:
             sx, sy = ctypes.c_int(), ctypes.c_int()
@@ -52,3 +57,20 @@ def test_registration_moves_images_back(fft_shift):
         assert np.array_equal(got[k], np.roll(plain[k], (-dy, -dx), axis=(0, 1)))
         err = float(np.abs(got[k] - plain[0]).max()) / float(np.abs(plain[0]).max())
         assert err < 1e-4, (k, err)
+
+
+def test_registration_graph_replay_matches_eager():
+    """With graphs on, the cycles before the reference is taken and the
+    accumulation queue is full run eagerly, the later ones are replayed:
+    images identical to the eager pipe's, cycle by cycle."""
+    from holovibes_amd.configs import settings_for, synth_for
+    from holovibes_amd.pipe import synth_frames
+    s = settings_for("c1", registration_enabled=1, img_accu_level=2)
+    first = synth_frames(synth_for("c1"), 0, s.batch_size)
+    shifts = SHIFTS + [(2, 2), (-3, 5), (0, 0), (6, -9)]
+    batches = [np.roll(first, (dy, dx), axis=(1, 2)).copy() for dy, dx in shifts]
+    eager, se = _run(s, batches)
+    replay, sr = _run(s, batches, graph=True)
+    assert se == sr
+    for a, b in zip(eager, replay):
+        assert np.array_equal(a, b)
