@@ -503,6 +503,7 @@ hv_status hv_unpack_12_to_16bit(uint16_t* output, size_t output_size, const uint
 hv_status hv_unpack_10_to_16bit(uint16_t* output, size_t output_size, const uint8_t* input, size_t input_size,
                                 void* stream);
 
+/* A queue outlives the readers (hv_reader_*) that feed it: stop them first. */
 typedef struct hv_input_queue hv_input_queue;
 
 /* BatchInputQueue (Backend/sources/data_struct/batch_input_queue.cc:8-59):
@@ -702,7 +703,9 @@ hv_status hv_holo_write_header(hv_holo_writer* w);
 hv_status hv_holo_write_frames(hv_holo_writer* w, const void* host_src, uint64_t count);
 /* The record path's device -> file copy: count frames in device memory,
  * through two pinned staging chunks (the file write of one chunk overlaps
- * the DMA of the next).  Returns when the bytes are in the file. */
+ * the DMA of the next); writes of 256 MiB or more reserve the file's blocks
+ * and copy through a shared mapping from several threads.  Returns when the
+ * bytes are in the file (page cache). */
 hv_status hv_holo_store_frames(hv_holo_writer* w, const void* dev_src, uint64_t count, void* stream);
 hv_status hv_holo_write_footer(hv_holo_writer* w, const char* json, size_t size);
 /* Rewrites img_nb and total_data_size in the header at offset 0; the cursor stays. */
