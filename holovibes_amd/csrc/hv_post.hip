@@ -37,24 +37,21 @@ __global__ void __launch_bounds__(256) k_renorm_partials(const float4* __restric
     if (threadIdx.x == 0) parts[blockIdx.x] = t;
 }
 
-// total of the partials, then frame_res / total in float (tools_compute.cu:99-123)
-__global__ void __launch_bounds__(256) k_renorm_ratio(double* __restrict__ parts, int nparts, float frame_res) {
-    __shared__ double sh[4];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 256) acc += parts[i];
-    double t = block_sum(acc, sh);
-    if (threadIdx.x == 0) {
-        float* ratio = (float*)(parts + kRenormParts);
-        ratio[0] = frame_res / (float)t;
-    }
-}
-
-// x = x * 2^c * (frame_res / sum), evaluated left to right in float as the reference does
+// x = x * 2^c * (frame_res / sum), evaluated left to right in float as the
+// reference does (tools_compute.cu:99-123); every block adds the partials in
+// the same order, so all get the same total
 __global__ void __launch_bounds__(256) k_renorm_apply(float4* __restrict__ x, size_t n4, const double* __restrict__ parts,
-                                                      float multiplier) {
+                                                      int nparts, float frame_res, float multiplier) {
+    __shared__ double sh[4];
+    __shared__ float ratio;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += 256) acc += parts[k];
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) ratio = frame_res / (float)t;
+    __syncthreads();
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n4) return;
-    const float r = ((const float*)(parts + kRenormParts))[0];
+    const float r = ratio;
     float4 v = x[i];
     v.x = (v.x * multiplier) * r;
     v.y = (v.y * multiplier) * r;
@@ -117,8 +114,7 @@ hv_status launch_renormalize(float* img, size_t P, unsigned norm_constant, void*
     double* parts = (double*)scratch;
     const int nparts = (int)std::min<size_t>(kRenormParts, std::max<size_t>(1, (n4 + 2047) / 2048));
     hipLaunchKernelGGL(k_renorm_partials, dim3(nparts), dim3(256), 0, st, (const float4*)img, n4, parts);
-    hipLaunchKernelGGL(k_renorm_ratio, dim3(1), dim3(256), 0, st, parts, nparts, (float)P);
-    hipLaunchKernelGGL(k_renorm_apply, dim3(grid_for(n4)), dim3(256), 0, st, (float4*)img, n4, parts,
+    hipLaunchKernelGGL(k_renorm_apply, dim3(grid_for(n4)), dim3(256), 0, st, (float4*)img, n4, parts, nparts, (float)P,
                        (float)(1u << norm_constant));
     HV_CHECK_LAUNCH("renormalize");
     return HV_OK;
