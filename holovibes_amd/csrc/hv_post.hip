@@ -246,24 +246,28 @@ __global__ void __launch_bounds__(256) k_reg_mean_parts(const float* __restrict_
     }
 }
 
-// the mean inside the mask: sum / count in float, 0 for an empty mask (apply_mask.cu:198-202)
-__global__ void k_reg_mean(const double* __restrict__ parts, int nparts, float* __restrict__ mean) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// out = (in - mean) * mask (rescale_in_mask + apply_mask), as the real part
+// of a complex plane; the mean inside the mask is sum / count in float, 0 for
+// an empty mask (apply_mask.cu:198-202), every block adding the partials in
+// the same order
+__global__ void __launch_bounds__(256) k_reg_prepare(float2* __restrict__ out, const float* __restrict__ in,
+                                                     const float* __restrict__ mask, size_t n,
+                                                     const double* __restrict__ parts, int nparts) {
+    __shared__ double sh[4];
+    __shared__ float mean_s;
     double s = 0.0, c = 0.0;
-    for (int k = 0; k < nparts; ++k) {
+    for (int k = threadIdx.x; k < nparts; k += 256) {
         s += parts[2 * k];
         c += parts[2 * k + 1];
     }
-    mean[0] = c > 0.0 ? (float)s / (float)c : 0.f;
-}
-
-// out = (in - mean) * mask (rescale_in_mask + apply_mask), as the real part of a complex plane
-__global__ void __launch_bounds__(256) k_reg_prepare(float2* __restrict__ out, const float* __restrict__ in,
-                                                     const float* __restrict__ mask, size_t n,
-                                                     const float* __restrict__ mean) {
+    const double ts = block_sum(s, sh);
+    __syncthreads();
+    const double tc = block_sum(c, sh);
+    if (threadIdx.x == 0) mean_s = tc > 0.0 ? (float)ts / (float)tc : 0.f;
+    __syncthreads();
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    out[i] = make_float2((in[i] - mean[0]) * mask[i], 0.f);
+    out[i] = make_float2((in[i] - mean_s) * mask[i], 0.f);
 }
 
 // cur *= conj(ref)
@@ -354,7 +358,6 @@ struct RegScratch {
     float* pv;
     uint32_t* pi;
     int* shift;
-    float* mean;
 };
 
 static RegScratch reg_layout(void* scratch, size_t P) {
@@ -373,7 +376,6 @@ static RegScratch reg_layout(void* scratch, size_t P) {
     r.pi = (uint32_t*)c;
     c += kRegParts * sizeof(uint32_t);
     r.shift = (int*)c;
-    r.mean = (float*)(c + 16);
     return r;
 }
 
@@ -389,8 +391,8 @@ hv_status launch_registration_mask(void* scratch, int W, int H, float zone, int 
 static hv_status reg_prepare(const RegScratch& r, float2* out, const float* img, size_t P, hipStream_t st) {
     const int nparts = (int)std::min<size_t>(kRegParts, std::max<size_t>(1, (P + 8191) / 8192));
     hipLaunchKernelGGL(k_reg_mean_parts, dim3(nparts), dim3(256), 0, st, img, r.mask, P, r.parts);
-    hipLaunchKernelGGL(k_reg_mean, dim3(1), dim3(64), 0, st, r.parts, nparts, r.mean);
-    hipLaunchKernelGGL(k_reg_prepare, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, out, img, r.mask, P, r.mean);
+    hipLaunchKernelGGL(k_reg_prepare, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, out, img, r.mask, P, r.parts,
+                       nparts);
     HV_CHECK_LAUNCH("registration prepare");
     return HV_OK;
 }
