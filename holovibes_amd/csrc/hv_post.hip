@@ -312,22 +312,41 @@ __global__ void __launch_bounds__(256) k_reg_peak_parts(const float2* __restrict
     }
 }
 
-// the peak's (x, y), wrapped to the signed shift and negated (registration.cc:46-55)
-__global__ void k_reg_peak_final(const float* __restrict__ pv, const uint32_t* __restrict__ pi, int nparts, int W,
-                                 int H, int* __restrict__ shift) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// the peak's (x, y), wrapped to the signed shift and negated (registration.cc:46-55);
+// one block reduces the per-block maxima, first index on ties
+__global__ void __launch_bounds__(256) k_reg_peak_final(const float* __restrict__ pv, const uint32_t* __restrict__ pi,
+                                                        int nparts, int W, int H, int* __restrict__ shift) {
+    __shared__ float sv[256];
+    __shared__ uint32_t si[256];
     float best = -1.f;
-    uint32_t bi = 0;
-    for (int k = 0; k < nparts; ++k)
+    uint32_t bi = 0xFFFFFFFFu;
+    for (int k = threadIdx.x; k < nparts; k += 256)
         if (pv[k] > best || (pv[k] == best && pi[k] < bi)) {
             best = pv[k];
             bi = pi[k];
         }
-    int x = (int)(bi % (uint32_t)W), y = (int)(bi / (uint32_t)W);
-    if (x > W / 2) x -= W;
-    if (y > H / 2) y -= H;
-    shift[0] = -x;
-    shift[1] = -y;
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const float v2 = sv[threadIdx.x + o];
+            const uint32_t i2 = si[threadIdx.x + o];
+            if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+                sv[threadIdx.x] = v2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t b = si[0] == 0xFFFFFFFFu ? 0u : si[0];
+        int x = (int)(b % (uint32_t)W), y = (int)(b / (uint32_t)W);
+        if (x > W / 2) x -= W;
+        if (y > H / 2) y -= H;
+        shift[0] = -x;
+        shift[1] = -y;
+    }
 }
 
 // circ_shift (tools.cu:40-79): out[(y + sy) mod H][(x + sx) mod W] = in[y][x]
@@ -415,7 +434,7 @@ hv_status launch_registration(void* scratch, float* img, float* tmp, int W, int 
     HV_TRY(launch_fft2(r.work, 1, W, H, 1, st));
     const int nparts = (int)std::min<size_t>(kRegParts, std::max<size_t>(1, (P + 4095) / 4096));
     hipLaunchKernelGGL(k_reg_peak_parts, dim3(nparts), dim3(256), 0, st, r.work, P, r.pv, r.pi);
-    hipLaunchKernelGGL(k_reg_peak_final, dim3(1), dim3(64), 0, st, r.pv, r.pi, nparts, W, H, r.shift);
+    hipLaunchKernelGGL(k_reg_peak_final, dim3(1), dim3(256), 0, st, r.pv, r.pi, nparts, W, H, r.shift);
     hipLaunchKernelGGL(k_reg_shift, dim3((W + 255) / 256, H), dim3(256), 0, st, tmp, img, W, H, r.shift);
     HV_CHECK_LAUNCH("registration");
     HV_HIP(hipMemcpyAsync(img, tmp, P * sizeof(float), hipMemcpyDeviceToDevice, st));
