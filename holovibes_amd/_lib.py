@@ -97,7 +97,8 @@ class ComputeSettings(ctypes.Structure):
                 ("img_accu_level", ctypes.c_uint32), ("log_enabled", ctypes.c_int32),
                 ("cuts_enabled", ctypes.c_int32), ("x_start", ctypes.c_uint32), ("x_width", ctypes.c_uint32),
                 ("y_start", ctypes.c_uint32), ("y_width", ctypes.c_uint32),
-                ("registration_enabled", ctypes.c_int32), ("registration_zone", ctypes.c_float)]
+                ("registration_enabled", ctypes.c_int32), ("registration_zone", ctypes.c_float),
+                ("contrast_reticle_enabled", ctypes.c_int32), ("contrast_reticle_scale", ctypes.c_float)]
 
 
 P = ctypes.c_void_p
@@ -145,6 +146,7 @@ SIGNATURES = {
     "hv_minmax_decode": [P, P, P],
     "hv_percentile_scratch_size": [SZ, P],
     "hv_percentile_range": [P, P, I, I, I, F, F, I, P, P],
+    "hv_percentile_range_reticle": [P, P, I, I, F, F, F, I, P, P],
     "hv_apply_contrast": [P, SZ, F, F, P],
     "hv_float_to_ushort": [P, P, SZ, P],
     "hv_contrast_to_ushort": [P, P, SZ, P, I, P],

@@ -82,12 +82,16 @@ WORKLOADS = {
 
 
 def settings_for(name: str, **overrides) -> ComputeSettings:
-    """hv_compute_settings of a workload, starting from the reference defaults."""
+    """hv_compute_settings of a workload, starting from the reference defaults.
+    The workloads name their steps; renormalisation (on by default, as in the
+    reference) is not among them -- it scales the image by a constant that the
+    contrast step removes again -- so it is off unless overridden."""
     s = ComputeSettings()
     load().hv_default_settings(ctypes.byref(s))
     for k, v in WORKLOADS[name]["settings"].items():
         setattr(s, k, v)
     s.lens_precision = HV_LENS_FP64
+    s.renorm_enabled = 0
     for k, v in overrides.items():
         setattr(s, k, v)
     return s

@@ -391,6 +391,22 @@ hv_status hv_percentile_range(float* range_out, const float* img, int width, int
                              as_stream(stream));
 }
 
+hv_status hv_percentile_range_reticle(float* range_out, const float* img, int width, int height, float reticle_scale,
+                                      float lower_pct, float upper_pct, int clamp_ge_one, void* scratch,
+                                      void* stream) {
+    HV_NOT_NULL(range_out);
+    HV_NOT_NULL(img);
+    HV_NOT_NULL(scratch);
+    if (width <= 0 || height <= 0) return fail(HV_ERR_INVALID_ARG, "sizes must be positive");
+    // ContrastApi::set_contrast_reticle_scale (contrast_api.cc:278-297)
+    if (!(reticle_scale >= 0.f && reticle_scale <= 1.f))
+        return fail(HV_ERR_INVALID_ARG, "Reticle scale must be in range [0., 1.]");
+    if (!(lower_pct >= 0.f && lower_pct <= 100.f && upper_pct >= 0.f && upper_pct <= 100.f))
+        return fail(HV_ERR_INVALID_ARG, "percentiles must lie in [0, 100]");
+    return launch_percentile(range_out, img, width, height, 1, lower_pct, upper_pct, clamp_ge_one, scratch,
+                             as_stream(stream), reticle_scale);
+}
+
 hv_status hv_apply_contrast(float* img, size_t n, float min, float max, void* stream) {
     HV_NOT_NULL(img);
     if (n == 0) return fail(HV_ERR_INVALID_ARG, "empty input");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -43,6 +44,13 @@ inline int ilog2(long n) {
     return l;
 }
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Batch input queue, for the read worker (hv_source.cpp): an enqueue whose
+// wait for the slot the consumer holds ends with HV_ERR_NOT_READY once
+// `stop` is set, and a wake-up of such waits.
+hv_status input_queue_enqueue_until(hv_input_queue* q, const void* frames, int memcpy_kind, uint32_t nb_frames,
+                                    void* stream, const std::atomic<bool>* stop);
+void input_queue_wake(hv_input_queue* q);
 
 // Geometry accepted by the FFT kernels.
 inline bool fft_len_ok(long n) { return is_pow2(n) && n >= 16 && n <= 4096; }

@@ -237,8 +237,10 @@ hv_status sync_all(hv_input_queue* q) {
 // Appends n frames through `put(dst, frame_offset, count)`, which enqueues
 // the copy (or unpack) of `count` source frames from `frame_offset` into dst
 // on the producer stream (batch_input_queue.cc enqueue).
+// `stop` (may be null) ends a wait for the consumer's slot.
 template <typename Put>
-hv_status enqueue_frames(hv_input_queue* q, uint32_t nb_frames, hipStream_t st, Put put) {
+hv_status enqueue_frames(hv_input_queue* q, uint32_t nb_frames, hipStream_t st, Put put,
+                         const std::atomic<bool>* stop = nullptr) {
     std::unique_lock<std::mutex> lk(q->m);
     uint32_t done = 0;
     while (done < nb_frames) {
@@ -247,7 +249,9 @@ hv_status enqueue_frames(hv_input_queue* q, uint32_t nb_frames, hipStream_t st, 
             // order the overwrite after the consumer's last use on the GPU
             if (q->held == (int)q->end && q->holder == std::this_thread::get_id())
                 return fail(HV_ERR_INVALID_ARG, "enqueue would overwrite the batch this thread holds: pop it first");
-            q->cv.wait(lk, [&] { return q->held != (int)q->end; });
+            q->cv.wait(lk, [&] { return q->held != (int)q->end || (stop && stop->load()); });
+            if (q->held == (int)q->end)
+                return fail(HV_ERR_NOT_READY, "enqueue stopped while the consumer holds the slot to overwrite");
             if (q->released_rec[q->end]) HV_HIP(hipStreamWaitEvent(st, q->released[q->end], 0));
         }
         const uint32_t n = std::min(nb_frames - done, q->batch_size - q->curr_batch_counter);
@@ -279,6 +283,28 @@ bool producer_holds_start(const hv_input_queue* q) { return q->curr_batch_counte
 }  // namespace
 
 using namespace hv;
+
+hv_status hv::input_queue_enqueue_until(hv_input_queue* q, const void* frames, int memcpy_kind, uint32_t nb_frames,
+                                        void* stream, const std::atomic<bool>* stop) {
+    if (!q || (!frames && nb_frames)) return fail(HV_ERR_INVALID_ARG, "hv_input_queue_enqueue: null argument");
+    if (memcpy_kind != (int)hipMemcpyHostToDevice && memcpy_kind != (int)hipMemcpyDeviceToDevice)
+        return fail(HV_ERR_INVALID_ARG, "Input queue : can't cudaMemcpy to host with the queue on gpu");
+    hipStream_t st = as_stream(stream);
+    const char* src = (const char*)frames;
+    return enqueue_frames(
+        q, nb_frames, st,
+        [&](char* dst, uint32_t off, uint32_t n) -> hv_status {
+            HV_HIP(hipMemcpyAsync(dst, src + (size_t)off * q->frame_size, (size_t)n * q->frame_size,
+                                  (hipMemcpyKind)memcpy_kind, st));
+            return HV_OK;
+        },
+        stop);
+}
+
+void hv::input_queue_wake(hv_input_queue* q) {
+    std::lock_guard<std::mutex> lk(q->m);
+    q->cv.notify_all();
+}
 
 extern "C" {
 
@@ -323,16 +349,7 @@ hv_status hv_input_queue_create(uint32_t frame_capacity, uint32_t batch_size, ui
 
 hv_status hv_input_queue_enqueue(hv_input_queue* q, const void* frames, int memcpy_kind, uint32_t nb_frames,
                                  void* stream) {
-    if (!q || (!frames && nb_frames)) return fail(HV_ERR_INVALID_ARG, "hv_input_queue_enqueue: null argument");
-    if (memcpy_kind != (int)hipMemcpyHostToDevice && memcpy_kind != (int)hipMemcpyDeviceToDevice)
-        return fail(HV_ERR_INVALID_ARG, "Input queue : can't cudaMemcpy to host with the queue on gpu");
-    hipStream_t st = as_stream(stream);
-    const char* src = (const char*)frames;
-    return enqueue_frames(q, nb_frames, st, [&](char* dst, uint32_t off, uint32_t n) -> hv_status {
-        HV_HIP(hipMemcpyAsync(dst, src + (size_t)off * q->frame_size, (size_t)n * q->frame_size,
-                              (hipMemcpyKind)memcpy_kind, st));
-        return HV_OK;
-    });
+    return input_queue_enqueue_until(q, frames, memcpy_kind, nb_frames, stream, nullptr);
 }
 
 hv_status hv_input_queue_enqueue_packed(hv_input_queue* q, const void* packed_frames, int memcpy_kind, int bits,

@@ -81,8 +81,12 @@ hv_status launch_scale(float* img, size_t n, float factor, hipStream_t st);
 size_t percentile_scratch_bytes();
 // Writes range[0..1] = values of rank k0, k1 (ascending) of the image (or its
 // subsample), optionally clamped to >= 1.
+// reticle_scale >= 0 (with step 1): only the pixels of the contrast reticle's
+// centred disc (their count is the N of the index rule); < 0: off.
 hv_status launch_percentile(float* range, const float* img, int width, int height, int step, float lo, float hi,
-                            int clamp, void* scratch, hipStream_t st);
+                            int clamp, void* scratch, hipStream_t st, float reticle_scale = -1.f);
+uint32_t reticle_radius(int width, int height, float scale);
+size_t reticle_pixels(int width, int height, float scale);
 hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots = 1);
 hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st);
 size_t unwrap_scratch_bytes(int width, int height);

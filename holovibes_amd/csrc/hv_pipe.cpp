@@ -203,6 +203,7 @@ struct hv_pipe {
     // cycle path (PATH_*): three passes + STFT, inverse row pass fused with
     // the STFT, or bins-first
     int path = PATH_THREE_PASS;
+    int level = PATH_BINS;  // highest path allowed (hv_pipe_set_fused)
     // stage timer: events between row pass, column pass, inverse row pass,
     // image kernel(s) and the contrast / u16 stage
     bool timing = false;
@@ -226,6 +227,8 @@ struct hv_pipe {
         const uint32_t level = std::max<uint32_t>(1, s.img_accu_level);
         return !s.registration_enabled || (reg_ref_set && (level <= 1 || accu_held == level));
     }
+    // contrast reticle scale for the percentiles (< 0: the whole image)
+    float reticle() const { return s.contrast_reticle_enabled ? s.contrast_reticle_scale : -1.f; }
     // renormalisation, accumulation or log10 change the image after its kernel
     bool post_steps() const {
         return s.renorm_enabled || s.img_accu_level > 1 || s.log_enabled || s.registration_enabled;
@@ -372,12 +375,12 @@ hv_status hv_pipe::image_work() {
                                              L.range);
         case HV_CONTRAST_PERCENTILE:
             HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
-                                     s.contrast_upper, 1, L.pct, L.stream));
+                                     s.contrast_upper, 1, L.pct, L.stream, reticle()));
             break;
         case HV_CONTRAST_REFERENCE:
             if (autocontrast_pending)
                 HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample,
-                                         s.contrast_lower, s.contrast_upper, 1, L.pct, L.stream));
+                                         s.contrast_lower, s.contrast_upper, 1, L.pct, L.stream, reticle()));
             break;
         default:  // MANUAL: range uploaded at create
             break;
@@ -442,14 +445,14 @@ void hv_default_settings(hv_compute_settings* s) {
     s->camera_fps = 0.0f;
     s->lens_precision = HV_LENS_FP64;
     s->pca_precision = HV_PCA_BF16_SPLIT;
-    // the reference's GUI turns renormalisation on by default (holovibes.hh:399);
-    // the path's configurations do not use it, so it starts off here
-    s->renorm_enabled = 0;
+    s->renorm_enabled = 1;  // RenormEnabled{true} (holovibes.hh:399)
     s->renorm_constant = 5;
     s->img_accu_level = 1;
     s->log_enabled = 0;
     s->registration_enabled = 0;
     s->registration_zone = 0.7f;  // Registration (view_struct.hh)
+    s->contrast_reticle_enabled = 0;   // ContrastReticleDisplayEnabled{false} (holovibes.hh:403)
+    s->contrast_reticle_scale = 0.5f;  // ContrastReticleScale{0.5f} (holovibes.hh:400)
 }
 
 hv_status hv_validate_settings(const hv_compute_settings* s) {
@@ -498,6 +501,13 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     if (s->contrast_subsample > 1 && (s->width % s->contrast_subsample || s->height % s->contrast_subsample))
         return fail(HV_ERR_INVALID_ARG, "contrast_subsample must divide the frame sides");
     if (s->fft_shift && (s->width % 2 || s->height % 2)) return fail(HV_ERR_UNSUPPORTED, "fft shift needs even sides");
+    if (s->contrast_reticle_enabled) {
+        // ContrastApi::set_contrast_reticle_scale (contrast_api.cc:278-297)
+        if (!(s->contrast_reticle_scale >= 0.f && s->contrast_reticle_scale <= 1.f))
+            return fail(HV_ERR_INVALID_ARG, "Reticle scale must be in range [0., 1.]");
+        if (s->contrast_subsample > 1)
+            return fail(HV_ERR_INVALID_ARG, "the contrast reticle and contrast_subsample exclude each other");
+    }
     if (s->renorm_enabled && s->renorm_constant > 31) return fail(HV_ERR_INVALID_ARG, "renorm_constant must be <= 31");
     if (s->img_accu_level > 4096) return fail(HV_ERR_INVALID_ARG, "img_accu_level must be <= 4096");
     if (s->registration_enabled && !(s->registration_zone > 0.f && s->registration_zone <= 1.f))
@@ -777,8 +787,14 @@ hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     HV_TRY(hv_pipe_sync(p));
     if (!enabled) {
-        if (p->rec) (void)hipFree(p->rec);
+        if (!p->rec) return HV_OK;
+        // graphs captured with the record write into it: drop them before
+        // the memory goes, and return to the path the record displaced
+        p->drop_graphs();
+        p->next = 0;
+        (void)hipFree(p->rec);
         p->rec = nullptr;
+        p->path = choose_path(p->s, p->level);
         return HV_OK;
     }
     if (p->s.time_transformation != HV_TIME_STFT && p->s.time_transformation != HV_TIME_PCA)
@@ -816,6 +832,7 @@ hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t*
 hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (level < 0) return fail(HV_ERR_INVALID_ARG, "fusion level must be >= 0");
+    p->level = level;
     p->path = p->rec ? PATH_THREE_PASS : choose_path(p->s, level);
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();

@@ -7,7 +7,9 @@
 // rescale_float_unwrap2d and compute_percentile all synchronise the stream).
 // Percentiles are exact order statistics found by a 3-pass radix select
 // (11 + 11 + 10 bits of the ordered key) instead of a full sort.
+#include <algorithm>
 #include <cfloat>
+#include <cmath>
 
 #include "hv_fft.hpp"
 #include "hv_kernels.hpp"
@@ -217,10 +219,19 @@ __global__ void k_pct_init(PctScratch* s, uint32_t r0, uint32_t r1) {
     }
 }
 
+// The pixels the percentiles are taken over: every pixel of the (optionally
+// subsampled) image, or those of the contrast reticle's centred disc.
+struct PctZone {
+    int step = 1;
+    int disc = 0;          // 1: only (x - cx)^2 + (y - cy)^2 <= r2
+    int cx = 0, cy = 0;
+    uint32_t r2 = 0;
+};
+
 // Histogram of digit [shift, shift + width) over keys whose higher bits equal
 // the running prefix of each of the two selections.
 __global__ void __launch_bounds__(256)
-    k_pct_hist(PctScratch* s, const float* __restrict__ img, int in_w, int out_w, int step, size_t n, int shift,
+    k_pct_hist(PctScratch* s, const float* __restrict__ img, int in_w, int out_w, PctZone z, size_t n, int shift,
                int width) {
     __shared__ uint32_t h[2][kPctBins];
     for (int i = threadIdx.x; i < 2 * kPctBins; i += blockDim.x) (&h[0][0])[i] = 0;
@@ -229,7 +240,14 @@ __global__ void __launch_bounds__(256)
     const int hs = shift + width;
     const uint32_t mask = (1u << width) - 1;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const size_t src = step > 1 ? subsample_index(i, in_w, out_w, step) : i;
+        size_t src = i;
+        if (z.step > 1) {
+            src = subsample_index(i, in_w, out_w, z.step);
+        } else if (z.disc) {
+            // copy_circle_kernel (percentile.cu:15-43)
+            const int dx = (int)(i % (size_t)in_w) - z.cx, dy = (int)(i / (size_t)in_w) - z.cy;
+            if ((uint32_t)(dx * dx + dy * dy) > z.r2) continue;
+        }
         const uint32_t key = float_to_ordered(img[src]);
         const uint32_t digit = (key >> shift) & mask;
         const uint32_t high = hs >= 32 ? 0u : (key >> hs);
@@ -295,17 +313,50 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+uint32_t reticle_radius(int width, int height, float scale) {
+    // compute_percentile_xy_view (percentile.cu:166-170): float product, truncated
+    return (uint32_t)(scale * (float)std::min(width, height) / 2);
+}
+
+size_t reticle_pixels(int width, int height, float scale) {
+    const long r = (long)reticle_radius(width, height, scale), r2 = r * r;
+    const long cx = width / 2, cy = height / 2;
+    size_t count = 0;
+    for (long y = 0; y < height; ++y) {
+        const long dy2 = (y - cy) * (y - cy);
+        if (dy2 > r2) continue;
+        // largest m with m^2 <= r2 - dy2
+        long m = (long)std::sqrt((double)(r2 - dy2));
+        while (m * m > r2 - dy2) --m;
+        while ((m + 1) * (m + 1) <= r2 - dy2) ++m;
+        const long x0 = std::max(0L, cx - m), x1 = std::min((long)width - 1, cx + m);
+        if (x1 >= x0) count += (size_t)(x1 - x0 + 1);
+    }
+    return count;
+}
+
 hv_status launch_percentile(float* range, const float* img, int width, int height, int step, float lo, float hi,
-                            int clamp, void* scratch, hipStream_t st) {
+                            int clamp, void* scratch, hipStream_t st, float reticle_scale) {
     if (step < 1) step = 1;
     const int out_w = width / step, out_h = height / step;
-    const size_t n = (size_t)out_w * out_h;
-    if (n == 0) return fail(HV_ERR_INVALID_ARG, "empty image");
+    PctZone z;
+    z.step = step;
+    size_t n = (size_t)out_w * out_h;  // pixels visited
+    size_t count = n;                  // pixels selected (N of the index rule)
+    if (reticle_scale >= 0.f && step == 1) {
+        const uint32_t r = reticle_radius(width, height, reticle_scale);
+        z.disc = 1;
+        z.cx = width / 2;
+        z.cy = height / 2;
+        z.r2 = r * r;
+        count = reticle_pixels(width, height, reticle_scale);
+    }
+    if (count == 0) return fail(HV_ERR_INVALID_ARG, "empty image");
     // compute_percentile (percentile.cu:84-91): index = uint(p / 100 * N), float arithmetic
-    auto rank_of = [n](float pct) {
-        const float f = pct / 100 * (float)n;
+    auto rank_of = [count](float pct) {
+        const float f = pct / 100 * (float)count;
         uint32_t r = f <= 0.f ? 0u : (uint32_t)f;
-        if (r >= n) r = (uint32_t)n - 1;  // p = 100 would read one past the end
+        if (r >= count) r = (uint32_t)count - 1;  // p = 100 would read one past the end
         return r;
     };
     PctScratch* s = reinterpret_cast<PctScratch*>(scratch);
@@ -314,7 +365,7 @@ hv_status launch_percentile(float* range, const float* img, int width, int heigh
     const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
     const unsigned g = blocks_for(n, 256 * 8, 1024);
     for (int d = 0; d < 3; ++d) {
-        k_pct_hist<<<g, 256, 0, st>>>(s, img, width, out_w, step, n, shifts[d], widths[d]);
+        k_pct_hist<<<g, 256, 0, st>>>(s, img, width, out_w, z, n, shifts[d], widths[d]);
         HV_CHECK_LAUNCH("k_pct_hist");
         k_pct_select<<<2, 256, 0, st>>>(s, widths[d], d == 2, range, clamp);
         HV_CHECK_LAUNCH("k_pct_select");

@@ -162,8 +162,14 @@ static void reader_loop(hv_reader* r) {
         s = r->cam.get_frames(r->cam.self, &cf);
         if (s != HV_OK) break;
         const int kind = cf.on_gpu ? (int)hipMemcpyDeviceToDevice : (int)hipMemcpyHostToDevice;
-        if (cf.count1) s = hv_input_queue_enqueue(r->q, cf.region1, kind, cf.count1, r->stream);
-        if (s == HV_OK && cf.count2) s = hv_input_queue_enqueue(r->q, cf.region2, kind, cf.count2, r->stream);
+        // a stop ends a wait for the slot the consumer holds (full queue)
+        if (cf.count1) s = input_queue_enqueue_until(r->q, cf.region1, kind, cf.count1, r->stream, &r->stop);
+        if (s == HV_OK && cf.count2)
+            s = input_queue_enqueue_until(r->q, cf.region2, kind, cf.count2, r->stream, &r->stop);
+        if (s == HV_ERR_NOT_READY && r->stop.load()) {
+            s = HV_OK;  // stopped while waiting: not an error of the source
+            break;
+        }
         // the source may reuse its buffers after this call returns
         if (s == HV_OK && hipStreamSynchronize(r->stream) != hipSuccess) s = fail(HV_ERR_HIP, "reader stream");
         if (s != HV_OK) break;
@@ -242,6 +248,7 @@ hv_status hv_reader_frames(hv_reader* r, uint64_t* frames) {
 hv_status hv_reader_stop(hv_reader* r, uint64_t* frames) {
     if (!r) return HV_OK;
     r->stop.store(true);
+    input_queue_wake(r->q);  // a worker waiting for the consumer's slot sees the stop
     if (r->th.joinable()) r->th.join();
     if (frames) *frames = r->frames.load();
     hv_status s = r->status;

@@ -392,7 +392,8 @@ def compute_settings_to_json(s: ComputeSettings) -> dict:
         "y": {"width": int(s.y_width), "start": int(s.y_start)},
         "z": {"width": int(s.p_width), "start": int(s.p_start)}, "z2": {"width": 0, "start": 0},
         "window": {"xy": xy, "yz": _cut_view(s), "xz": _cut_view(s), "filter2d": _view_window()},
-        "renorm": bool(s.renorm_enabled), "reticle": {"display_enabled": False, "scale": 0.5},
+        "renorm": bool(s.renorm_enabled),
+        "reticle": {"display_enabled": bool(s.contrast_reticle_enabled), "scale": _f32(s.contrast_reticle_scale)},
         "registration": {"registration_enabled": bool(s.registration_enabled),
                          "registration_zone": _f32(s.registration_zone)},
     }
@@ -491,6 +492,12 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     reg = v.get("registration", {})
     s.registration_enabled = int(bool(reg.get("registration_enabled", False)))
     s.registration_zone = float(reg.get("registration_zone", s.registration_zone))
+    # the contrast reticle (Reticle::Load, all_struct.cc:221-225)
+    ret = v.get("reticle", {})
+    s.contrast_reticle_enabled = int(bool(ret.get("display_enabled", False)))
+    s.contrast_reticle_scale = float(ret.get("scale", s.contrast_reticle_scale))
+    if s.contrast_reticle_enabled and s.contrast_subsample > 1:
+        s.contrast_subsample = 1
     if r.get("filter2d", {}).get("enabled"):
         ignored.append("image_rendering.filter2d")
     if r.get("input_filter", {}).get("type", "None") not in ("None", ""):

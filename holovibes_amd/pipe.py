@@ -59,6 +59,7 @@ class Pipe:
             _lib.call("hv_pipe_set_fused", self.h, level)
         if lanes > 1:
             _lib.call("hv_pipe_set_lanes", self.h, lanes)
+        self.lanes = lanes
         self.P = settings.width * settings.height
 
     @staticmethod
@@ -206,6 +207,8 @@ class Server:
         _lib.call("hv_server_drain", self.h)
 
     def output_u16(self, sid: int, height: int, width: int) -> np.ndarray:
+        # the source's streams do not synchronise with the null stream the copy runs on
+        self.drain()
         u, f = ctypes.c_void_p(), ctypes.c_void_p()
         _lib.call("hv_server_output", self.h, sid, ctypes.byref(u), ctypes.byref(f))
         t = torch.empty((height, width), dtype=torch.int16, device="cuda")

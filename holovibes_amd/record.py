@@ -34,6 +34,9 @@ class Recorder:
                  frame_skip: int = 0, slots: int = 4):
         if mode not in MODES:
             raise ValueError(f"record mode {mode!r}: one of {sorted(MODES)}")
+        if getattr(pipe, "lanes", 1) > 1:
+            # slots are filled on the pipe's first stream, in cycle order
+            raise ValueError("the record path needs a pipe with one lane")
         s = pipe.settings
         self.pipe, self.mode = pipe, mode
         depth = {"raw": s.depth, "hologram": 2, "moments": 4}[mode]

@@ -243,6 +243,19 @@ def percentile_range(img: torch.Tensor, lower: float, upper: float, step: int = 
     return out
 
 
+def percentile_range_reticle(img: torch.Tensor, lower: float, upper: float, scale: float, clamp: bool = True):
+    """compute_percentile_xy_view with the contrast reticle on (percentile.cu:15-43, 150-222):
+    percentiles over the centred disc of radius uint(scale * min(W, H) / 2)."""
+    H, W = img.shape
+    nbytes = ctypes.c_size_t()
+    _lib.call("hv_percentile_scratch_size", img.numel(), ctypes.byref(nbytes))
+    ws = torch.empty(nbytes.value, dtype=torch.uint8, device=img.device)
+    out = torch.empty(2, dtype=torch.float32, device=img.device)
+    _lib.call("hv_percentile_range_reticle", _p(out), _p(img), W, H, scale, lower, upper, 1 if clamp else 0, _p(ws),
+              _s())
+    return out
+
+
 def apply_contrast(img: torch.Tensor, mn: float, mx: float) -> None:
     """apply_contrast_correction (contrast_correction.cu:5-17), in place."""
     _lib.call("hv_apply_contrast", _p(img), img.numel(), mn, mx, _s())

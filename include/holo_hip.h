@@ -273,6 +273,15 @@ hv_status hv_percentile_scratch_size(size_t n, size_t* bytes);
 hv_status hv_percentile_range(float* range_out, const float* img, int width, int height, int subsample_step,
                               float lower_pct, float upper_pct, int clamp_ge_one, void* scratch, void* stream);
 
+/* Replaces compute_percentile_xy_view with the contrast reticle on
+ * (percentile.cu:15-43 copy_circle_kernel, 150-222): the same order
+ * statistics over the pixels (x, y) of the W x H image with
+ * (x - W/2)^2 + (y - H/2)^2 <= r^2, r = uint(reticle_scale * min(W, H) / 2);
+ * N = the number of such pixels.  reticle_scale in [0, 1]. */
+hv_status hv_percentile_range_reticle(float* range_out, const float* img, int width, int height, float reticle_scale,
+                                      float lower_pct, float upper_pct, int clamp_ge_one, void* scratch,
+                                      void* stream);
+
 /* Replaces apply_contrast_correction (Backend/cuda_sources/contrast_correction.cu:5-17):
  * x <- 65535 / (max - min + FLT_EPSILON) * (x - min), in place, host range. */
 hv_status hv_apply_contrast(float* img, size_t n, float min, float max, void* stream);
@@ -333,6 +342,13 @@ typedef struct hv_compute_settings {
      * over a centred disc of radius zone * min(W, H) / 2 */
     int32_t registration_enabled;
     float registration_zone;
+    /* ContrastReticleDisplayEnabled / ContrastReticleScale (Backend/includes/core/holovibes.hh:400, 403;
+     * Backend/sources/compute/rendering.cc:310-327): the autocontrast percentiles
+     * are taken over the centred disc of radius uint(scale * min(W, H) / 2)
+     * instead of the whole image (percentile.cu:15-43, 150-190).  scale in [0, 1];
+     * exclusive with contrast_subsample > 1. */
+    int32_t contrast_reticle_enabled;
+    float contrast_reticle_scale;
 } hv_compute_settings;
 
 /* Fills the reference defaults (Backend/includes/core/holovibes.hh:359-447). */

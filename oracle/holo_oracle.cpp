@@ -572,6 +572,20 @@ inline uint16_t f2u16(float x) {
 }
 
 // apply_contrast_correction (contrast_correction.cu:5-17)
+// compute_percentile_xy_view with the contrast reticle (percentile.cu:15-43,
+// 150-190): the pixels with (x - W/2)^2 + (y - H/2)^2 <= r^2,
+// r = uint(scale * min(W, H) / 2), in any order (they are sorted next).
+std::vector<float> reticle_pixels(const float* img, int W, int H, float scale) {
+    const unsigned r = (unsigned)(scale * (float)std::min(W, H) / 2);
+    const long r2 = (long)r * r;
+    const long cx = W / 2, cy = H / 2;
+    std::vector<float> out;
+    for (long y = 0; y < H; ++y)
+        for (long x = 0; x < W; ++x)
+            if ((x - cx) * (x - cx) + (y - cy) * (y - cy) <= r2) out.push_back(img[y * W + x]);
+    return out;
+}
+
 void apply_contrast(float* img, size_t n, float mn, float mx) {
     const float factor = (unsigned short)65535 / (mx - mn + FLT_EPSILON);
     for (size_t i = 0; i < n; ++i) img[i] = factor * (img[i] - mn);
@@ -697,7 +711,10 @@ struct Pipe {
             } else if (s.contrast_mode == HV_CONTRAST_PERCENTILE) recompute = true;
             if (recompute) {
                 const float pct[2] = {s.contrast_lower, s.contrast_upper};
-                if (s.contrast_subsample > 1) {
+                if (s.contrast_reticle_enabled) {
+                    std::vector<float> disc = reticle_pixels(image.data(), s.width, s.height, s.contrast_reticle_scale);
+                    percentile(disc.data(), disc.size(), pct, 2, range);
+                } else if (s.contrast_subsample > 1) {
                     int st = s.contrast_subsample;
                     std::vector<float> sub((size_t)(s.width / st) * (s.height / st));
                     subsample(image.data(), s.width, s.height, st, sub.data());
@@ -796,6 +813,11 @@ void or_subsample_f32(const float* in, int W, int H, int s, float* out) { subsam
 void or_subsample_u8(const uint8_t* in, int W, int H, int s, uint8_t* out) { subsample(in, W, H, s, out); }
 void or_percentile_f32(const float* in, uint64_t n, const float* pct, int npct, float* out) {
     percentile(in, n, pct, npct, out);
+}
+uint64_t or_reticle_pixels_f32(const float* img, int W, int H, float scale, float* out) {
+    std::vector<float> d = reticle_pixels(img, W, H, scale);
+    if (out) std::memcpy(out, d.data(), d.size() * sizeof(float));
+    return d.size();
 }
 void or_apply_contrast_f32(float* img, uint64_t n, float mn, float mx) { apply_contrast(img, n, mn, mx); }
 void or_float_to_u16(const float* in, uint64_t n, uint16_t* out) {

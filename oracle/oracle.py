@@ -71,6 +71,9 @@ def lib():
         _lib.or_percentile_f32.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_void_p]
         _lib.or_float_to_u16.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+        _lib.or_reticle_pixels_f32.restype = ctypes.c_uint64
+        _lib.or_reticle_pixels_f32.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                               ctypes.c_void_p]
         _lib.or_rescale_unwrap_f32.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     return _lib
 
@@ -268,6 +271,16 @@ def percentile(img: np.ndarray, pcts) -> np.ndarray:
     out = np.empty(len(p), np.float32)
     lib().or_percentile_f32(_p(img), img.size, _p(p), len(p), _p(out))
     return out
+
+
+def reticle_pixels(img: np.ndarray, scale: float) -> np.ndarray:
+    """The pixels of the contrast reticle's centred disc (percentile.cu:15-43,
+    150-190), row-major order."""
+    img = np.ascontiguousarray(img, np.float32)
+    H, W = img.shape
+    out = np.empty(img.size, np.float32)
+    n = lib().or_reticle_pixels_f32(_p(img), W, H, scale, _p(out))
+    return out[:n].copy()
 
 
 def apply_contrast(img: np.ndarray, mn: float, mx: float) -> np.ndarray:

@@ -45,9 +45,11 @@ def test_settings_struct_layout(lib):
     assert abs(s.lambda_ - 852e-9) < 1e-12 and s.pixel_size == 12.0 and s.z_distance == 0.0
     assert abs(s.contrast_lower - 0.02) < 1e-7 and abs(s.contrast_upper - 99.98) < 1e-4
     assert s.camera_fps == 0.0 and s.fft_shift == 0 and s.unwrap_2d == 0
-    # the last fields, written by the library at their C offsets
-    assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (0, 5, 1, 0)
-    assert ctypes.sizeof(ComputeSettings) == 38 * 4
+    # the last fields, written by the library at their C offsets: RenormEnabled{true},
+    # RenormConstant 5, ContrastReticleDisplayEnabled{false}, ContrastReticleScale{0.5}
+    assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (1, 5, 1, 0)
+    assert (s.contrast_reticle_enabled, s.contrast_reticle_scale) == (0, 0.5)
+    assert ctypes.sizeof(ComputeSettings) == 40 * 4
 
 
 def test_validate_settings(lib):
@@ -60,6 +62,8 @@ def test_validate_settings(lib):
     bad = [dict(time_stride=48), dict(time_stride=16), dict(batch_size=0), dict(width=500),
            dict(time_transformation_size=24), dict(p_start=2, p_width=30), dict(depth=3), dict(lambda_=0.0),
            dict(contrast_mode=9), dict(contrast_subsample=3),
+           dict(contrast_reticle_enabled=1, contrast_reticle_scale=1.5),
+           dict(contrast_reticle_enabled=1, contrast_subsample=2),
            dict(img_type=HV_IMG_MOMENTS_0, time_transformation=HV_TIME_NONE)]
     for over in bad:
         s = configs.settings_for("metric", **over)

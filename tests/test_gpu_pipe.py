@@ -213,6 +213,19 @@ def test_pipe_pca_moments(oracle, img_type, T, precision, tol):
     check(run_both(oracle, s, [designed_frames(T, 32, 32, 17)]), tol)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pipe_contrast_reticle(oracle, mode):
+    """Autocontrast over the reticle disc (ContrastReticleDisplayEnabled,
+    rendering.cc:310-327): the reference policy (first window, then frozen)
+    and percentiles on every image."""
+    s, fl = frames("c1", 3, width=128, height=64, contrast_mode=mode, contrast_reticle_enabled=1,
+                   contrast_reticle_scale=0.6, contrast_lower=0.5, contrast_upper=99.5)
+    res = run_both(oracle, s, fl)
+    check(res, 1e-5)
+    for _, _, rng, ref in res:
+        assert np.allclose(rng, ref["range"], rtol=1e-5)
+
+
 def test_pipe_fft_shift_and_invert(oracle):
     s, fl = frames("c1", 1, width=64, height=64, fft_shift=1, contrast_invert=1)
     check(run_both(oracle, s, fl), 1e-5)

@@ -121,3 +121,47 @@ def test_raw_record_and_frame_count_correction(tmp_path):
     p = hf.parse(open(path, "rb").read())
     assert (p["img_nb"], p["depth"], p["data_type"]) == (48, 2, hf.RAW)
     assert p["frames"] == frames.tobytes()
+
+
+def test_moments_record_close_restores_graph_pipe(tmp_path):
+    """Closing a moments record on a graph pipe drops the graphs captured
+    with the record (they write into its released frames) and returns to
+    the path the record displaced; later cycles equal an eager pipe's."""
+    from holovibes_amd.configs import settings_for
+    from holovibes_amd.pipe import Pipe
+    from holovibes_amd.record import Recorder
+    s = settings_for("c1", width=128, height=128)
+    n = 6
+    frames = _frames("c1", n * s.batch_size, width=128, height=128)
+    batches = [_dev(frames[b * s.batch_size:(b + 1) * s.batch_size]) for b in range(n)]
+    torch.cuda.synchronize()
+    graph, eager = Pipe(s, graph=True), Pipe(s)
+    assert graph.path == Pipe.BINS
+    rec = Recorder(str(tmp_path / "m.holo"), graph, "moments", 3 * 3)
+    assert graph.path == Pipe.THREE_PASS
+    for b in batches[:3]:
+        made = graph.push(b)
+        rec.after_push(b, made)
+        eager.push(b)
+    assert rec.close(input_fps=100) == 9
+    assert graph.path == Pipe.BINS
+    for b in batches[3:]:
+        assert graph.push(b) and eager.push(b)
+        assert np.array_equal(graph.output_u16(), eager.output_u16())
+        assert torch.equal(graph.image(), eager.image())
+    graph.close()
+    eager.close()
+
+
+def test_recorder_refuses_lanes(tmp_path):
+    """The record copies a cycle's output on the pipe's first stream, so it
+    needs the cycles in order on one lane."""
+    from holovibes_amd.configs import settings_for
+    from holovibes_amd.pipe import Pipe
+    from holovibes_amd.record import Recorder
+    s = settings_for("c1", width=64, height=64)
+    p = Pipe(s, lanes=2)
+    for mode in ("raw", "hologram", "moments"):
+        with pytest.raises(ValueError, match="one lane"):
+            Recorder(str(tmp_path / f"{mode}.holo"), p, mode, 4)
+    p.close()

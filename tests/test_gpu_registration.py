@@ -74,3 +74,30 @@ def test_registration_graph_replay_matches_eager():
     assert se == sr
     for a, b in zip(eager, replay):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("W,H", [(2048, 2048), (2048, 1024)])
+def test_registration_large_frames(W, H):
+    """At 2048 x 2048 the masked-mean and peak searches run past their
+    partial-sum cap (1024 partials, more than 256 per thread in the final
+    reduction); a non-square frame checks the row / column roles.  The
+    shift equals the restatement's on the unregistered images and the image
+    moves back exactly."""
+    from holovibes_amd.configs import settings_for, synth_for
+    from holovibes_amd.pipe import synth_frames
+    over = dict(width=W, height=H, batch_size=2, time_stride=2, time_transformation_size=2, p_start=1, p_width=0)
+    base = settings_for("c1", **over)
+    reg = settings_for("c1", registration_enabled=1, registration_zone=0.7, **over)
+    print(f"registration {W}x{H}", flush=True)
+    first = synth_frames(synth_for("c1", width=W, height=H, period=2), 0, 2)
+    shifts = [(0, 0), (37, -101), (-250, 9)]
+    batches = [np.roll(first, (dy, dx), axis=(1, 2)).copy() for dy, dx in shifts]
+    plain, _ = _run(base, batches)
+    got, found = _run(reg, batches)
+    mask = ps.registration_mask(W, H, 0.7, False)
+    assert np.array_equal(got[0], plain[0])
+    for k in range(1, len(shifts)):
+        dy, dx = shifts[k]
+        _, (osx, osy) = ps.register(plain[k], plain[0], mask)
+        assert found[k] == (osx, osy) == (-dx, -dy), (k, found[k], (osx, osy))
+        assert np.array_equal(got[k], np.roll(plain[k], (-dy, -dx), axis=(0, 1)))

@@ -108,3 +108,30 @@ def test_python_camera_plugin():
     assert np.array_equal(got, np.arange(len(got)))
     assert len(got) == n - n % B
     q.close()
+
+
+def test_reader_stop_while_queue_full_and_front_held():
+    """A full queue whose front batch the consumer holds blocks the read
+    worker's next overwrite; stopping the worker must end that wait (the
+    worker returns without an error) instead of deadlocking."""
+    from holovibes_amd.configs import synth_for
+    from holovibes_amd.input_queue import BatchInputQueue
+    from holovibes_amd.sources import FrameReadWorker, SyntheticCamera
+    p = synth_for("c1", width=32, height=32)
+    B = 4
+    cam = SyntheticCamera(p, 16, B)
+    q = BatchInputQueue(2 * B, B, 32, 32, 2)  # two batch slots
+    w = FrameReadWorker(cam, q)
+    t0 = time.time()
+    while q.state()["size"] < 2:
+        assert time.time() - t0 < 20.0, "queue never filled"
+        time.sleep(0.002)
+    held = q.front(timeout_ms=1000)
+    assert held is not None
+    # the worker keeps overwriting the other slot until it reaches the held one
+    time.sleep(0.2)
+    n = w.stop()
+    assert n >= 2 * B
+    assert q.pop()
+    cam.shutdown()
+    q.close()

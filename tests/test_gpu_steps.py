@@ -309,6 +309,18 @@ def test_percentile_exact(st, oracle, step, lo, hi):
     assert np.array_equal(got_c, np.maximum(want, 1.0))
 
 
+@pytest.mark.parametrize("W,H,scale", [(128, 256, 0.5), (256, 256, 1.0), (64, 32, 0.3), (512, 128, 0.0)])
+def test_percentile_reticle_exact(st, oracle, W, H, scale):
+    """Percentiles over the contrast reticle's centred disc equal the oracle's
+    order statistics of the same pixels (scale 0: the centre pixel alone)."""
+    rng = np.random.default_rng(21)
+    img = (rng.standard_normal((H, W)) * 1000).astype(np.float32)
+    disc = oracle.reticle_pixels(img, scale)
+    for lo, hi in [(0.5, 99.5), (0.02, 99.98), (0.0, 100.0)]:
+        got = host(st.percentile_range_reticle(dev(img), lo, hi, scale, clamp=False))
+        assert np.array_equal(got, oracle.percentile(disc, [lo, hi])), (lo, hi)
+
+
 def test_contrast_and_u16_exact(st, oracle):
     rng = np.random.default_rng(14)
     img = (rng.standard_normal(4099) * 30000 + 20000).astype(np.float32)

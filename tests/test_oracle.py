@@ -399,3 +399,27 @@ def test_oracle_fp32_close_to_fp64(oracle):
     a = oracle.OraclePipe(s, 64).push(fr)
     b = oracle.OraclePipe(s, 32).push(fr)
     assert rel_l2(b["image"], a["image"]) < 1e-5
+
+
+def test_reticle_disc_gauss_circle_counts(oracle):
+    """The contrast reticle's disc (percentile.cu:15-43, 150-190) holds the
+    lattice points of a disc of radius r = uint(scale * min(W, H) / 2) centred
+    on (W/2, H/2): Gauss's circle counts N(r) = 1, 5, 13, 29, 49, 81, 113 for
+    r = 0..6 when the disc lies inside the frame, fewer where the frame clips it."""
+    gauss = [1, 5, 13, 29, 49, 81, 113]
+    W = H = 64
+    img = np.arange(W * H, dtype=np.float32).reshape(H, W)
+    for r, n in enumerate(gauss):
+        sel = oracle.reticle_pixels(img, (2 * r + 0.5) / min(W, H))
+        assert sel.size == n, (r, sel.size)
+        # the centre pixel and the four axis points at distance r
+        ys, xs = np.divmod(sel.astype(np.int64), W)
+        assert ((xs - W // 2) ** 2 + (ys - H // 2) ** 2).max() == r * r
+    # non-square: the radius follows the smaller side, the centre is (W/2, H/2)
+    img = np.arange(32 * 8, dtype=np.float32).reshape(8, 32)
+    sel = oracle.reticle_pixels(img, 1.0)  # r = 4
+    ys, xs = np.divmod(sel.astype(np.int64), 32)
+    assert sel.size == 49 - 1  # the 49-point disc around (16, 4) less its row y = 8 (one point, dy = 4)
+    assert xs.min() == 12 and xs.max() == 20 and ys.min() == 0 and ys.max() == 7
+    # 2 x 2 frame, r = 1 around (1, 1): the corner (0, 0) lies at distance sqrt(2)
+    assert oracle.reticle_pixels(np.arange(4, dtype=np.float32).reshape(2, 2), 1.0).tolist() == [1.0, 2.0, 3.0]
