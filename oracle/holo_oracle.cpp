@@ -340,24 +340,45 @@ void moments(const C<R>* spec, int T, size_t P, int f_start, int f_end, float fp
 // ---------------------------------------------------------------------------
 template <typename R>
 void pca_cov(const C<R>* ring, int T, size_t P, double* cov /* T*T complex col-major */) {
+    // pixel tiles of kTile pixels gathered frame-contiguous; the lower
+    // triangle cov(i, j) = sum_p conj(x_i[p]) x_j[p], i >= j, accumulated in
+    // double, the upper one is its conjugate
+    constexpr size_t kTile = 32;
+    const size_t tiles = (P + kTile - 1) / kTile;
     std::vector<double> acc((size_t)T * T * 2, 0.0);
 #pragma omp parallel
     {
         std::vector<double> loc((size_t)T * T * 2, 0.0);
+        std::vector<double> x(kTile * (size_t)T * 2);
 #pragma omp for schedule(static)
-        for (long p = 0; p < (long)P; ++p)
-            for (int j = 0; j < T; ++j) {
-                C<R> xj = ring[(size_t)j * P + p];
-                for (int i = 0; i < T; ++i) {
-                    C<R> xi = ring[(size_t)i * P + p];
-                    // conj(xi) * xj
-                    loc[2 * (i + (size_t)j * T)] += (double)xi.re * xj.re + (double)xi.im * xj.im;
-                    loc[2 * (i + (size_t)j * T) + 1] += (double)xi.re * xj.im - (double)xi.im * xj.re;
+        for (long b = 0; b < (long)tiles; ++b) {
+            const size_t p0 = (size_t)b * kTile, n = std::min(kTile, P - p0);
+            for (size_t k = 0; k < n; ++k)
+                for (int t = 0; t < T; ++t) {
+                    const C<R> v = ring[(size_t)t * P + p0 + k];
+                    x[2 * (k * T + t)] = (double)v.re;
+                    x[2 * (k * T + t) + 1] = (double)v.im;
+                }
+            for (size_t k = 0; k < n; ++k) {
+                const double* xk = &x[2 * k * T];
+                for (int j = 0; j < T; ++j) {
+                    const double jr = xk[2 * j], ji = xk[2 * j + 1];
+                    double* a = &loc[2 * (size_t)j * T];
+                    for (int i = j; i < T; ++i) {
+                        a[2 * i] += xk[2 * i] * jr + xk[2 * i + 1] * ji;
+                        a[2 * i + 1] += xk[2 * i] * ji - xk[2 * i + 1] * jr;
+                    }
                 }
             }
+        }
 #pragma omp critical
         for (size_t k = 0; k < acc.size(); ++k) acc[k] += loc[k];
     }
+    for (int j = 0; j < T; ++j)
+        for (int i = 0; i < j; ++i) {  // upper triangle: cov(i, j) = conj(cov(j, i))
+            acc[2 * (i + (size_t)j * T)] = acc[2 * (j + (size_t)i * T)];
+            acc[2 * (i + (size_t)j * T) + 1] = -acc[2 * (j + (size_t)i * T) + 1];
+        }
     std::memcpy(cov, acc.data(), acc.size() * sizeof(double));
 }
 
@@ -446,20 +467,42 @@ int eig_jacobi(const double* cov, int n, double* evals, double* evecs) {
     return sweep;
 }
 
+// Components [t0, t1] of C = H V (the others are left untouched).
 template <typename R>
-void pca_project(const C<R>* ring, const double* evecs, int T, size_t P, C<R>* out) {
-#pragma omp parallel for schedule(static)
-    for (long p = 0; p < (long)P; ++p)
-        for (int t = 0; t < T; ++t) {
-            double re = 0, im = 0;
-            for (int s = 0; s < T; ++s) {
-                C<R> x = ring[(size_t)s * P + p];
-                double vr = evecs[2 * (s + (size_t)t * T)], vi = evecs[2 * (s + (size_t)t * T) + 1];
-                re += (double)x.re * vr - (double)x.im * vi;
-                im += (double)x.re * vi + (double)x.im * vr;
-            }
-            out[(size_t)t * P + p] = {(R)re, (R)im};
+void pca_project(const C<R>* ring, const double* evecs, int T, size_t P, C<R>* out, int t0 = 0, int t1 = -1) {
+    if (t1 < 0) t1 = T - 1;
+    const int nt = t1 - t0 + 1;
+    // V rows s with the selected components contiguous
+    std::vector<double> v((size_t)T * nt * 2);
+    for (int s = 0; s < T; ++s)
+        for (int t = t0; t <= t1; ++t) {
+            v[2 * ((size_t)s * nt + (t - t0))] = evecs[2 * (s + (size_t)t * T)];
+            v[2 * ((size_t)s * nt + (t - t0)) + 1] = evecs[2 * (s + (size_t)t * T) + 1];
         }
+    constexpr size_t kTile = 32;
+    const size_t tiles = (P + kTile - 1) / kTile;
+#pragma omp parallel
+    {
+        std::vector<double> acc((size_t)nt * 2);
+#pragma omp for schedule(static)
+        for (long b = 0; b < (long)tiles; ++b) {
+            const size_t p0 = (size_t)b * kTile, n = std::min(kTile, P - p0);
+            for (size_t k = 0; k < n; ++k) {
+                std::fill(acc.begin(), acc.end(), 0.0);
+                for (int s = 0; s < T; ++s) {
+                    const C<R> xv = ring[(size_t)s * P + p0 + k];
+                    const double xr = (double)xv.re, xi = (double)xv.im;
+                    const double* vs = &v[2 * (size_t)s * nt];
+                    for (int t = 0; t < nt; ++t) {
+                        acc[2 * t] += xr * vs[2 * t] - xi * vs[2 * t + 1];
+                        acc[2 * t + 1] += xr * vs[2 * t + 1] + xi * vs[2 * t];
+                    }
+                }
+                for (int t = 0; t < nt; ++t)
+                    out[(size_t)(t0 + t) * P + p0 + k] = {(R)acc[2 * t], (R)acc[2 * t + 1]};
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -670,15 +713,17 @@ struct Pipe {
         if (q_size < (uint32_t)T) return 0;
         // time transform (fourier_transform.cc:195-229)
         if (s.time_transformation == HV_TIME_STFT) stft(ring.data(), T, P, spec.data());
-        else if (s.time_transformation == HV_TIME_PCA) {
-            pca_cov(ring.data(), T, P, cov.data());
-            eig_jacobi(cov.data(), T, evals.data(), evecs.data());
-            pca_project(ring.data(), evecs.data(), T, P, spec.data());
-        } else spec = ring;
         // insert_compute_p_accu (converts.cc:76-92)
         int pmin = s.p_start;
         int pmax = s.p_width ? std::max(0, std::min<int>(pmin + s.p_width, T)) : pmin;
         const int it = s.img_type;
+        if (s.time_transformation == HV_TIME_PCA) {
+            pca_cov(ring.data(), T, P, cov.data());
+            eig_jacobi(cov.data(), T, evals.data(), evecs.data());
+            // the image and the moments read only components [p_start, p_start + p_width]
+            pca_project(ring.data(), evecs.data(), T, P, spec.data(), (int)s.p_start,
+                        std::min<int>((int)(s.p_start + s.p_width), T - 1));
+        } else if (s.time_transformation != HV_TIME_STFT) spec = ring;
         if (it == HV_IMG_MODULUS || it == HV_IMG_SQUARED_MODULUS || it == HV_IMG_ARGUMENT) {
             image_type(spec.data(), P, it, pmin, pmax, image_r.data());
             for (size_t i = 0; i < P; ++i) image[i] = (float)image_r[i];
@@ -865,6 +910,35 @@ void or_pipe_ring_copy(void* h, double* out) {
     auto* p = static_cast<Pipe<double>*>(h);
     std::memcpy(out, p->ring.data(), p->ring.size() * sizeof(C<double>));
 }
+// The latest cycle's moments M0, M1, M2 over the settings' range (the
+// window's STFT or PCA output), and its PCA covariance, eigenvalues and
+// eigenvectors (column-major, complex double).
+void or_pipe_moments(void* h, int precision, double* m0, double* m1, double* m2) {
+    auto run = [&](auto* p) {
+        const auto& s = p->s;
+        const int fe = std::min<int>((int)(s.p_start + s.p_width), p->T - 1);
+        std::vector<decltype(p->spec[0].re)> a(p->P), b(p->P), c(p->P);
+        moments(p->spec.data(), p->T, p->P, (int)s.p_start, fe, s.camera_fps, a.data(), b.data(), c.data());
+        for (size_t i = 0; i < p->P; ++i) {
+            m0[i] = a[i];
+            m1[i] = b[i];
+            m2[i] = c[i];
+        }
+    };
+    if (precision == 64) run(static_cast<Pipe<double>*>(h));
+    else run(static_cast<Pipe<float>*>(h));
+}
+
+void or_pipe_pca_state(void* h, int precision, double* cov, double* evals, double* evecs) {
+    auto run = [&](auto* p) {
+        std::memcpy(cov, p->cov.data(), p->cov.size() * sizeof(double));
+        std::memcpy(evals, p->evals.data(), p->evals.size() * sizeof(double));
+        std::memcpy(evecs, p->evecs.data(), p->evecs.size() * sizeof(double));
+    };
+    if (precision == 64) run(static_cast<Pipe<double>*>(h));
+    else run(static_cast<Pipe<float>*>(h));
+}
+
 void or_pipe_destroy(void* h, int precision) {
     if (precision == 32) delete static_cast<Pipe<float>*>(h);
     else delete static_cast<Pipe<double>*>(h);

@@ -50,6 +50,8 @@ def lib():
         _lib.or_pipe_ring_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_pipe_enqueue_raw.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
         _lib.or_pipe_ring_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.or_pipe_moments.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
+        _lib.or_pipe_pca_state.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3
         for name in ("or_lens_as",):
             getattr(_lib, name).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                             ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_void_p]
@@ -338,6 +340,24 @@ class OraclePipe:
         out = np.empty((s.time_transformation_size, s.height, s.width), np.complex128)
         lib().or_pipe_ring_copy(self.h, _p(out))
         return out
+
+    def moments(self):
+        """M0, M1, M2 (float64, (H, W)) of the latest cycle's time transform over
+        the settings' range [p_start, min(p_start + p_width, T - 1)]."""
+        s = self.settings
+        m = [np.empty((s.height, s.width), np.float64) for _ in range(3)]
+        lib().or_pipe_moments(self.h, self.precision, *[_p(x) for x in m])
+        return m
+
+    def pca_state(self):
+        """(cov[i, j], eigenvalues ascending, V with V[:, j] the j-th
+        eigenvector) of the latest PCA cycle, complex128 / float64."""
+        T = self.settings.time_transformation_size
+        cov = np.empty(T * T, np.complex128)
+        ev = np.empty(T, np.float64)
+        vec = np.empty(T * T, np.complex128)
+        lib().or_pipe_pca_state(self.h, self.precision, _p(cov), _p(ev), _p(vec))
+        return cov.reshape(T, T).T.copy(), ev, vec.reshape(T, T).T.copy()
 
     def close(self):
         if self.h:
