@@ -273,7 +273,7 @@ hv_status hv_pca_cov(void* cov_c64, const void* ring_c64, int T, size_t frame_re
     HV_TRY(check_T(T));
     HV_TRY(check_res(frame_res));
     if (precision < HV_PCA_FP32 || precision > HV_PCA_FP16) return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
-    return launch_pca_cov((float2*)cov_c64, (const float2*)ring_c64, T, frame_res, precision, workspace,
+    return launch_pca_cov(cov_c64, 0, (const float2*)ring_c64, T, frame_res, precision, workspace,
                           as_stream(stream));
 }
 
@@ -291,7 +291,7 @@ hv_status hv_eig_hermitian_f64(double* evals, void* evecs_c128, const void* cov_
     HV_NOT_NULL(cov_c64);
     HV_TRY(check_T(T));
     if (eig_workspace_bytes(T) && !workspace) return fail(HV_ERR_INVALID_ARG, "T > 64 needs a workspace");
-    return launch_eig_jacobi(evals, (double2*)evecs_c128, (const float2*)cov_c64, T, workspace, as_stream(stream));
+    return launch_eig_jacobi(evals, (double2*)evecs_c128, cov_c64, 0, T, workspace, as_stream(stream));
 }
 
 hv_status hv_pca_image(float* out, const void* ring_c64, const void* evecs_c128, int T, size_t frame_res,

@@ -93,10 +93,13 @@ size_t unwrap_scratch_bytes(int width, int height);
 
 // hv_pca.hip
 size_t pca_workspace_bytes(int T, size_t P, int precision);
-hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int precision, void* ws,
+// cov: T x T column-major, double2 when cov_double (the pipe's, for the FP64
+// eigensolver), float2 otherwise (the C-ABI step's).
+hv_status launch_pca_cov(void* cov, int cov_double, const float2* ring, int T, size_t P, int precision, void* ws,
                          hipStream_t st);
 size_t eig_workspace_bytes(int T);
-hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st);
+hv_status launch_eig_jacobi(double* evals, double2* evecs, const void* cov, int cov_double, int T, void* work,
+                            hipStream_t st);
 // img_type: an image type (mean over [pmin, pmax], x131071) or HV_IMG_MOMENTS_k
 // (weights from fps over [pmin, pmax] = [f_start, f_end])
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,

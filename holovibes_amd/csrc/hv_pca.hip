@@ -18,6 +18,19 @@
 // values sit in FP16's normal range.  Partial tiles of every workgroup go to a
 // workspace; a second kernel sums them and writes the Hermitian T x T matrix.
 //
+// Centring.  Every pixel's time series is dominated by its mean (the
+// interferogram's constant term), so every entry of H^H H is about
+// lambda_max / T, and FP32 accumulation leaves errors of ~1e-7 lambda_max on
+// all of them -- far above the 1e-9 lambda_max gaps between the small
+// eigenvalues that decide which eigenvectors a component range selects.  So
+// a first pass takes each pixel's mean mu_p over the window, the MFMA kernels
+// accumulate the centred Y = H - mu 1^T, and
+//   H^H H = Y^H Y + a 1^T + 1 a^H + m 1 1^T,  a_t = sum_p conj(y_pt) mu_p,
+//   m = sum_p |mu_p|^2
+// is assembled in FP64 (the identity holds for any mu; the mean makes Y^H Y
+// small).  Y^H Y's accumulation errors scale with the centred data, ~1e-4
+// of lambda_max here, and the FP64 matrix goes to the FP64 eigensolver.
+//
 // Eigendecomposition.  cusolverDnCheevd (lower, ascending) is replaced by a
 // cyclic Jacobi in FP64 on one workgroup with the round-robin (circle)
 // ordering: T/2 disjoint rotations per round, T - 1 rounds per sweep.
@@ -84,15 +97,27 @@ static int pca_slices(int T, size_t P, int precision) {
     return pca_cov_grouped(T, precision) ? pca_slices_grouped(P) : pca_slices_pairs(T, P);
 }
 
+static size_t align_up(size_t n) { return (n + 255) & ~size_t(255); }
+constexpr int kMeanBlocks = 1024;  // most workgroups of the mean pass
+
+// Workspace: MFMA partial tiles, a_t partials (one row of T per slice), the
+// pixel means, the FP64 partials of sum |mu|^2, the assembled FP64 terms
+// (a_0 .. a_{T-1}, then m) and the scale words.
 struct CovLayout {
     size_t partial_floats;  // slices * pairs * kTile
-    size_t offset_scale;    // byte offset of the FP16 scale word
+    size_t off_apart, off_mean, off_mpart, off_terms;
+    size_t offset_scale;    // byte offset of the scale words
     size_t bytes;
 };
 static CovLayout cov_layout(int T, size_t P, int precision) {
     CovLayout l;
-    l.partial_floats = (size_t)pca_slices(T, P, precision) * pca_pairs(T) * kTile;
-    l.offset_scale = l.partial_floats * sizeof(float);
+    const int slices = pca_slices(T, P, precision);
+    l.partial_floats = (size_t)slices * pca_pairs(T) * kTile;
+    l.off_apart = align_up(l.partial_floats * sizeof(float));
+    l.off_mean = l.off_apart + align_up((size_t)slices * std::max(T, 32) * sizeof(float2));
+    l.off_mpart = l.off_mean + align_up(P * sizeof(float2));
+    l.off_terms = l.off_mpart + align_up(kMeanBlocks * sizeof(double));
+    l.offset_scale = l.off_terms + align_up((size_t)(T + 1) * sizeof(double2));
     l.bytes = l.offset_scale + 256;
     return l;
 }
@@ -106,38 +131,84 @@ __device__ __forceinline__ void pair_of(int idx, int& F, int& G) {
     G = idx - F * (F + 1) / 2;
 }
 
-// Max |component| of the ring, as an ordered uint (FP16 scaling); 16-byte
-// loads, four in flight per thread.
-__global__ void __launch_bounds__(256) k_ring_absmax(uint32_t* out, const float* __restrict__ ring, size_t n,
-                                                     bool vec) {
-    float m = 0.f;
-    const size_t n4 = vec ? n / 4 : 0, stride = (size_t)gridDim.x * blockDim.x;
+// Per-pixel window mean mu_p (float), max |component| of the ring and of mu
+// (ordered uints in mm[1], mm[3]: the FP16 scale bounds |y| <= |x| + |mu|),
+// and one FP64 partial of sum_p |mu_p|^2 per workgroup.  Two pixels (one
+// 16-byte load) per thread and frame, eight frames' loads in flight.
+static int pca_mean_blocks(size_t P) { return (int)std::min<size_t>(kMeanBlocks, (P / 2 + 255) / 256); }
+
+__global__ void __launch_bounds__(256)
+    k_pca_mean(float2* __restrict__ mean, double* __restrict__ mpart, uint32_t* mm, const float2* __restrict__ ring,
+               int T, size_t P) {
+    const size_t P2 = P / 2;
     const float4* r4 = reinterpret_cast<const float4*>(ring);
+    float4* m4 = reinterpret_cast<float4*>(mean);
+    const float inv = 1.0f / (float)T;  // a power of two: exact
+    float mx = 0.f, mmu = 0.f;
+    double msum = 0.0;
     auto amax = [](float4 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))); };
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        const float4 a = r4[i], b = r4[i + stride], c = r4[i + 2 * stride], d = r4[i + 3 * stride];
-        m = fmaxf(m, fmaxf(fmaxf(amax(a), amax(b)), fmaxf(amax(c), amax(d))));
+    for (size_t p2 = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p2 < P2; p2 += (size_t)gridDim.x * blockDim.x) {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        int t = 0;
+        for (; t + 8 <= T; t += 8) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = r4[(size_t)(t + u) * P2 + p2];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc.x += v[u].x;
+                acc.y += v[u].y;
+                acc.z += v[u].z;
+                acc.w += v[u].w;
+                mx = fmaxf(mx, amax(v[u]));
+            }
+        }
+        for (; t < T; ++t) {
+            const float4 v = r4[(size_t)t * P2 + p2];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+            mx = fmaxf(mx, amax(v));
+        }
+        acc = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, acc.w * inv);
+        m4[p2] = acc;
+        mmu = fmaxf(mmu, amax(acc));
+        msum += (double)acc.x * acc.x + (double)acc.y * acc.y + (double)acc.z * acc.z + (double)acc.w * acc.w;
     }
-    for (; i < n4; i += stride) m = fmaxf(m, amax(r4[i]));
-    for (size_t j = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) m = fmaxf(m, fabsf(ring[j]));
-    block_minmax_atomic(m, m, out);
+    block_minmax_atomic(mx, mx, mm);
+    __syncthreads();  // the helper's LDS is reused by the second call
+    block_minmax_atomic(mmu, mmu, mm + 2);
+    __shared__ double red[256];
+    red[threadIdx.x] = msum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mpart[blockIdx.x] = red[0];
 }
 
 __global__ void k_scale_init(uint32_t* mm) {
     if (threadIdx.x == 0) {
-        mm[0] = float_to_ordered(INFINITY);
-        mm[1] = float_to_ordered(0.f);
+        mm[0] = mm[2] = float_to_ordered(INFINITY);
+        mm[1] = mm[3] = float_to_ordered(0.f);
     }
 }
 
-// Power of two bringing max |x| to about 2^12 (FP16 max 65504; products of
-// 2^12-sized values stay far from the FP32 accumulator's limits).
+// Power of two bringing the bound max |x| + max |mu| of the centred values
+// to about 2^12 (FP16 max 65504; products of 2^12-sized values stay far from
+// the FP32 accumulator's limits).
 __device__ __forceinline__ float fp16_scale(const uint32_t* mm) {
-    const float mx = ordered_to_float(mm[1]);
+    const float mx = ordered_to_float(mm[1]) + ordered_to_float(mm[3]);
     if (!(mx > 0.f) || !isfinite(mx)) return 1.0f;
     const int e = ilogbf(mx);
     return ldexpf(1.0f, 12 - e);
+}
+
+// conj(y) mu
+__device__ __forceinline__ float2 conj_mul(float2 y, float2 mu) {
+    return make_float2(y.x * mu.x + y.y * mu.y, y.x * mu.y - y.y * mu.x);
 }
 
 template <int PREC>
@@ -148,8 +219,8 @@ struct CovStep {
 
 template <int PREC>
 __global__ void __launch_bounds__(64 * kCovWaves)
-    k_pca_cov(float* __restrict__ partial, const float2* __restrict__ ring, int T, size_t P, int slices,
-              const uint32_t* __restrict__ scale_mm) {
+    k_pca_cov(float* __restrict__ partial, float2* __restrict__ apart, const float2* __restrict__ ring,
+              const float2* __restrict__ mean, int T, size_t P, int slices, const uint32_t* __restrict__ scale_mm) {
     using S = CovStep<PREC>;
     // Workgroups go round-robin over the 8 XCDs (linear id mod 8), each with
     // its own L2.  With slices a multiple of 8, the pairs of one slice run as
@@ -167,6 +238,7 @@ __global__ void __launch_bounds__(64 * kCovWaves)
     }
     int F, G;
     pair_of(pair, F, G);
+    const bool diag = F == G;  // diagonal pairs also accumulate a_t for their group's frames
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t per_wave = P / ((size_t)slices * kCovWaves);
     const size_t p_begin = ((size_t)slice * kCovWaves + wave) * per_wave;
@@ -178,18 +250,22 @@ __global__ void __launch_bounds__(64 * kCovWaves)
     const int koff = (lane >> 5) * S::KPL;
     float scale = 1.0f;
     if constexpr (PREC == HV_PCA_FP16 || PREC == HV_PCA_FP16_SPLIT) scale = fp16_scale(scale_mm);
+    // centred value of a frame that exists (frames past T stay zero)
+    auto centre = [](float2 x, float2 mu, bool ok) { return ok ? make_float2(x.x - mu.x, x.y - mu.y) : make_float2(0.f, 0.f); };
+    float2 aacc = make_float2(0.f, 0.f);
 
     v16f acc_re = {}, acc_im = {};
     if constexpr (PREC == HV_PCA_FP32) {
         // 4 MFMA steps (2 pixels each) per iteration, the next iteration's
         // pixels loaded before this one's products
         constexpr int U = 4;
-        float2 na[U], nb[U];
+        float2 na[U], nb[U], nm[U];
         auto load = [&](size_t p0) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 na[u] = okF ? rowF[p0 + u * S::K + koff] : make_float2(0.f, 0.f);
                 nb[u] = okG ? rowG[p0 + u * S::K + koff] : make_float2(0.f, 0.f);
+                nm[u] = mean[p0 + u * S::K + koff];
             }
         };
         load(p_begin);
@@ -197,8 +273,13 @@ __global__ void __launch_bounds__(64 * kCovWaves)
             float2 ca[U], cb[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                ca[u] = na[u];
-                cb[u] = nb[u];
+                ca[u] = centre(na[u], nm[u], okF);
+                cb[u] = centre(nb[u], nm[u], okG);
+                if (diag) {
+                    const float2 t = conj_mul(ca[u], nm[u]);
+                    aacc.x += t.x;
+                    aacc.y += t.y;
+                }
             }
             if (p0 + U * S::K < p_end) load(p0 + U * S::K);
 #pragma unroll
@@ -212,14 +293,16 @@ __global__ void __launch_bounds__(64 * kCovWaves)
         }
     } else {
         // the next 16 pixels' loads are issued before this step's conversions and products
-        float4 na[4], nb[4];
+        float4 na[4], nb[4], nm[4];
         auto load = [&](size_t p0) {
             const float4* pa = reinterpret_cast<const float4*>(rowF + p0 + koff);
             const float4* pb = reinterpret_cast<const float4*>(rowG + p0 + koff);
+            const float4* pm = reinterpret_cast<const float4*>(mean + p0 + koff);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 na[q] = okF ? pa[q] : make_float4(0.f, 0.f, 0.f, 0.f);
                 nb[q] = okG ? pb[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                nm[q] = pm[q];
             }
         };
         load(p_begin);
@@ -227,11 +310,19 @@ __global__ void __launch_bounds__(64 * kCovWaves)
             float2 xa[8], xb[8];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float4 va = na[q], vb = nb[q];
-                xa[2 * q] = make_float2(va.x * scale, va.y * scale);
-                xa[2 * q + 1] = make_float2(va.z * scale, va.w * scale);
-                xb[2 * q] = make_float2(vb.x * scale, vb.y * scale);
-                xb[2 * q + 1] = make_float2(vb.z * scale, vb.w * scale);
+                const float4 va = na[q], vb = nb[q], vm = nm[q];
+                const float2 m0 = make_float2(vm.x, vm.y), m1 = make_float2(vm.z, vm.w);
+                const float2 ya0 = centre(make_float2(va.x, va.y), m0, okF), ya1 = centre(make_float2(va.z, va.w), m1, okF);
+                const float2 yb0 = centre(make_float2(vb.x, vb.y), m0, okG), yb1 = centre(make_float2(vb.z, vb.w), m1, okG);
+                if (diag) {
+                    const float2 t0 = conj_mul(ya0, m0), t1 = conj_mul(ya1, m1);
+                    aacc.x += t0.x + t1.x;
+                    aacc.y += t0.y + t1.y;
+                }
+                xa[2 * q] = make_float2(ya0.x * scale, ya0.y * scale);
+                xa[2 * q + 1] = make_float2(ya1.x * scale, ya1.y * scale);
+                xb[2 * q] = make_float2(yb0.x * scale, yb0.y * scale);
+                xb[2 * q + 1] = make_float2(yb1.x * scale, yb1.y * scale);
             }
             if (p0 + S::K < p_end) load(p0 + S::K);
             constexpr bool split = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_FP16_SPLIT);
@@ -279,11 +370,13 @@ __global__ void __launch_bounds__(64 * kCovWaves)
     }
     // reduce the workgroup's waves in LDS, one partial tile per workgroup
     __shared__ float red[kCovWaves][kTile];
+    __shared__ float2 ared[kCovWaves][64];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         red[wave][r * 64 + lane] = acc_re[r];
         red[wave][1024 + r * 64 + lane] = acc_im[r];
     }
+    ared[wave][lane] = aacc;
     __syncthreads();
     float* out = partial + ((size_t)slice * npairs + pair) * kTile;
     for (int i = threadIdx.x; i < kTile; i += blockDim.x) {
@@ -291,6 +384,16 @@ __global__ void __launch_bounds__(64 * kCovWaves)
 #pragma unroll
         for (int w = 0; w < kCovWaves; ++w) s += red[w][i];
         out[i] = s;
+    }
+    // a_t of group F's frames over this slice: lanes l and l + 32 of every wave (a fixed order)
+    if (diag && threadIdx.x < 32 && 32 * F + (int)threadIdx.x < T) {
+        float2 a = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int w = 0; w < kCovWaves; ++w) {
+            a.x += ared[w][threadIdx.x].x + ared[w][threadIdx.x + 32].x;
+            a.y += ared[w][threadIdx.x].y + ared[w][threadIdx.x + 32].y;
+        }
+        apart[(size_t)slice * T + 32 * F + threadIdx.x] = a;
     }
 }
 
@@ -304,8 +407,8 @@ __global__ void __launch_bounds__(64 * kCovWaves)
 // pair-per-workgroup kernel reads each group once per pair it belongs to.
 template <int PREC, int G>
 __global__ void __launch_bounds__(64 * kCovGroupWaves)
-    k_pca_cov_groups(float* __restrict__ partial, const float2* __restrict__ ring, size_t P, int slices,
-                     const uint32_t* __restrict__ scale_mm) {
+    k_pca_cov_groups(float* __restrict__ partial, float2* __restrict__ apart, const float2* __restrict__ ring,
+                     const float2* __restrict__ mean, size_t P, int slices, const uint32_t* __restrict__ scale_mm) {
     constexpr int NP = G * (G + 1) / 2;
     constexpr int W = kCovGroupWaves;
     constexpr int PPW = (NP + W - 1) / W;
@@ -323,11 +426,17 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
     // converting waves: group `wave`, frame 32 wave + lane % 32, pixel offset 8 (lane / 32)
     const bool conv = wave < G;
     const float2* src = ring + (size_t)(32 * (conv ? wave : 0) + (lane & 31)) * P + (lane >> 5) * 8;
-    float4 nx[4];
+    const float2* msrc = mean + (lane >> 5) * 8;
+    float4 nx[4], nm[4];
+    float2 aacc = make_float2(0.f, 0.f);  // a_t of this lane's frame over its pixels
     auto load = [&](size_t p0) {
         const float4* q = reinterpret_cast<const float4*>(src + p0);
+        const float4* qm = reinterpret_cast<const float4*>(msrc + p0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) nx[k] = q[k];
+        for (int k = 0; k < 4; ++k) {
+            nx[k] = q[k];
+            nm[k] = qm[k];
+        }
     };
     if (conv) load(p_begin);
     v16f acc_re[PPW], acc_im[PPW];
@@ -339,8 +448,14 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
             float2 x[8];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                x[2 * k] = make_float2(nx[k].x * scale, nx[k].y * scale);
-                x[2 * k + 1] = make_float2(nx[k].z * scale, nx[k].w * scale);
+                const float2 m0 = make_float2(nm[k].x, nm[k].y), m1 = make_float2(nm[k].z, nm[k].w);
+                const float2 y0 = make_float2(nx[k].x - m0.x, nx[k].y - m0.y);
+                const float2 y1 = make_float2(nx[k].z - m1.x, nx[k].w - m1.y);
+                const float2 t0 = conj_mul(y0, m0), t1 = conj_mul(y1, m1);
+                aacc.x += t0.x + t1.x;
+                aacc.y += t0.y + t1.y;
+                x[2 * k] = make_float2(y0.x * scale, y0.y * scale);
+                x[2 * k + 1] = make_float2(y1.x * scale, y1.y * scale);
             }
             if (p0 + 16 < p_end) load(p0 + 16);
             V8 rh, ih, nh, rl, il, nl;
@@ -398,6 +513,11 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
             acc_im[j] = mfma(an, br, acc_im[j]);
         }
     }
+    // a_t of the converting wave's 32 frames: lanes l and l + 32 hold the same frame
+    if (conv) {
+        const float ax = aacc.x + __shfl_xor(aacc.x, 32), ay = aacc.y + __shfl_xor(aacc.y, 32);
+        if (lane < 32) apart[(size_t)blockIdx.x * (32 * G) + 32 * wave + lane] = make_float2(ax, ay);
+    }
     // each pair's tile straight to its partial slot (one wave per pair: no cross-wave sum)
 #pragma unroll
     for (int j = 0; j < PPW; ++j) {
@@ -413,41 +533,81 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
 }
 
 template <int PREC>
-static hv_status cov_groups(float* partial, const float2* ring, int T, size_t P, int slices,
-                            const uint32_t* scale_mm, hipStream_t st) {
+static hv_status cov_groups(float* partial, float2* apart, const float2* ring, const float2* mean, int T, size_t P,
+                            int slices, const uint32_t* scale_mm, hipStream_t st) {
     constexpr int threads = 64 * kCovGroupWaves;
     switch (T) {
-        case 64: k_pca_cov_groups<PREC, 2><<<slices, threads, 0, st>>>(partial, ring, P, slices, scale_mm); break;
-        case 128: k_pca_cov_groups<PREC, 4><<<slices, threads, 0, st>>>(partial, ring, P, slices, scale_mm); break;
-        case 256: k_pca_cov_groups<PREC, 8><<<slices, threads, 0, st>>>(partial, ring, P, slices, scale_mm); break;
+        case 64: k_pca_cov_groups<PREC, 2><<<slices, threads, 0, st>>>(partial, apart, ring, mean, P, slices, scale_mm); break;
+        case 128: k_pca_cov_groups<PREC, 4><<<slices, threads, 0, st>>>(partial, apart, ring, mean, P, slices, scale_mm); break;
+        case 256: k_pca_cov_groups<PREC, 8><<<slices, threads, 0, st>>>(partial, apart, ring, mean, P, slices, scale_mm); break;
         default: return fail(HV_ERR_UNSUPPORTED, "grouped covariance needs T in {64, 128, 256}");
     }
     HV_CHECK_LAUNCH("k_pca_cov_groups");
     return HV_OK;
 }
 
-// Sum the slices and write the full Hermitian matrix, column-major.
-// Tile element (acc, r, lane) is row 8 (r / 4) + 4 (lane / 32) + r % 4 and
-// column lane % 32 of the 32 x 32 MFMA output.  Each workgroup owns 64
+// a_t = sum of its slice partials and m = sum of the mean pass's partials,
+// in FP64 and a fixed order: one workgroup per frame (workgroup 0 also m).
+__global__ void __launch_bounds__(256)
+    k_pca_mean_terms(double2* __restrict__ terms, const float2* __restrict__ apart, int slices, int T,
+                     const double* __restrict__ mpart, int mblocks) {
+    const int t = blockIdx.x;
+    __shared__ double rr[256], ri[256];
+    double re = 0.0, im = 0.0;
+    for (int sl = threadIdx.x; sl < slices; sl += 256) {
+        const float2 a = apart[(size_t)sl * T + t];
+        re += a.x;
+        im += a.y;
+    }
+    rr[threadIdx.x] = re;
+    ri[threadIdx.x] = im;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            rr[threadIdx.x] += rr[threadIdx.x + o];
+            ri[threadIdx.x] += ri[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) terms[t] = make_double2(rr[0], ri[0]);
+    if (t != 0) return;
+    __syncthreads();
+    double m = 0.0;
+    for (int b = threadIdx.x; b < mblocks; b += 256) m += mpart[b];
+    rr[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) rr[threadIdx.x] += rr[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) terms[T] = make_double2(rr[0], 0.0);
+}
+
+// Sum the slices in FP64, undo the FP16 scale (a power of two), add the mean
+// terms -- cov(i, j) = (Y^H Y)(i, j) + a_i + conj(a_j) + m -- and write the
+// full Hermitian matrix, column-major, as double2 (cov64) and / or float2
+// (cov32).  Tile element (acc, r, lane) is row 8 (r / 4) + 4 (lane / 32) + r % 4
+// and column lane % 32 of the 32 x 32 MFMA output.  Each workgroup owns 64
 // elements of one pair's tile; its four waves sum every fourth slice, then
 // one wave adds the four (a fixed order: deterministic).
 __global__ void __launch_bounds__(256)
-    k_pca_cov_reduce(float2* __restrict__ cov, const float* __restrict__ partial, int T, int slices, int npairs,
+    k_pca_cov_reduce(double2* __restrict__ cov64, float2* __restrict__ cov32, const float* __restrict__ partial,
+                     const double2* __restrict__ terms, int T, int slices, int npairs,
                      const uint32_t* __restrict__ scale_mm, int use_scale) {
     const int pair = blockIdx.x;
     int F, G;
     pair_of(pair, F, G);
     const int g = threadIdx.x >> 6, el = threadIdx.x & 63;
     const int e = blockIdx.y * 64 + el;  // 0 .. 1023
-    float re = 0.f, im = 0.f;
+    double re = 0.0, im = 0.0;
     // unrolled so that several slices' loads are in flight (same summation order)
 #pragma unroll 8
     for (int sl = g; sl < slices; sl += 4) {
         const float* t = partial + ((size_t)sl * npairs + pair) * kTile;
-        re += t[e];
-        im += t[1024 + e];
+        re += (double)t[e];
+        im += (double)t[1024 + e];
     }
-    __shared__ float sr[4][64], si[4][64];
+    __shared__ double sr[4][64], si[4][64];
     sr[g][el] = re;
     si[g][el] = im;
     __syncthreads();
@@ -455,8 +615,8 @@ __global__ void __launch_bounds__(256)
     re = (sr[0][el] + sr[1][el]) + (sr[2][el] + sr[3][el]);
     im = (si[0][el] + si[1][el]) + (si[2][el] + si[3][el]);
     if (use_scale) {
-        const float sc = fp16_scale(scale_mm);
-        const float inv = 1.0f / (sc * sc);
+        const double sc = (double)fp16_scale(scale_mm);
+        const double inv = 1.0 / (sc * sc);  // exact: a power of two
         re *= inv;
         im *= inv;
     }
@@ -464,52 +624,71 @@ __global__ void __launch_bounds__(256)
     const int i = 8 * (r / 4) + 4 * (lane / 32) + (r % 4), jj = lane % 32;
     const int fi = 32 * F + i, fj = 32 * G + jj;
     if (fi >= T || fj >= T) return;
-    cov[fi + (size_t)fj * T] = make_float2(re, im);
-    if (F != G) cov[fj + (size_t)fi * T] = make_float2(re, -im);
+    const double2 ai = terms[fi], aj = terms[fj];
+    const double m = terms[T].x;
+    re += ai.x + aj.x + m;
+    im += ai.y - aj.y;
+    if (cov64) {
+        cov64[fi + (size_t)fj * T] = make_double2(re, im);
+        if (F != G) cov64[fj + (size_t)fi * T] = make_double2(re, -im);
+    }
+    if (cov32) {
+        cov32[fi + (size_t)fj * T] = make_float2((float)re, (float)im);
+        if (F != G) cov32[fj + (size_t)fi * T] = make_float2((float)re, (float)-im);
+    }
 }
 
-hv_status launch_pca_cov(float2* cov, const float2* ring, int T, size_t P, int precision, void* ws,
+hv_status launch_pca_cov(void* cov, int cov_double, const float2* ring, int T, size_t P, int precision, void* ws,
                          hipStream_t st) {
     const CovLayout l = cov_layout(T, P, precision);
     const int slices = pca_slices(T, P, precision), np = pca_pairs(T);
     const bool grouped = pca_cov_grouped(T, precision);
-    float* partial = reinterpret_cast<float*>(ws);
-    uint32_t* scale_mm = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + l.offset_scale);
+    char* w = reinterpret_cast<char*>(ws);
+    float* partial = reinterpret_cast<float*>(w);
+    float2* apart = reinterpret_cast<float2*>(w + l.off_apart);
+    float2* mean = reinterpret_cast<float2*>(w + l.off_mean);
+    double* mpart = reinterpret_cast<double*>(w + l.off_mpart);
+    double2* terms = reinterpret_cast<double2*>(w + l.off_terms);
+    uint32_t* scale_mm = reinterpret_cast<uint32_t*>(w + l.offset_scale);
     const bool fp16 = precision == HV_PCA_FP16 || precision == HV_PCA_FP16_SPLIT;
     // every wave's pixel range: a multiple of the 8 pixels per FP32 iteration and the 16 per half-precision step
     if (P % ((size_t)slices * (grouped ? 1 : kCovWaves) * 16))
         return fail(HV_ERR_UNSUPPORTED, "frame_res not a multiple of the PCA tile");
-    if (fp16) {
-        k_scale_init<<<1, 64, 0, st>>>(scale_mm);
-        const size_t n = (size_t)T * P * 2;
-        const bool vec = (reinterpret_cast<uintptr_t>(ring) & 15) == 0;  // 16-byte loads need 16-byte alignment
-        k_ring_absmax<<<(unsigned)std::min<size_t>((n / 4 + 255) / 256 + 1, 8192), 256, 0, st>>>(
-            scale_mm, (const float*)ring, n, vec);
-        HV_CHECK_LAUNCH("k_ring_absmax");
-    }
-    const unsigned grid = (unsigned)(np * slices);
-    if (grouped) {
-        switch (precision) {
-            case HV_PCA_BF16_SPLIT: HV_TRY(cov_groups<HV_PCA_BF16_SPLIT>(partial, ring, T, P, slices, scale_mm, st)); break;
-            case HV_PCA_FP16_SPLIT: HV_TRY(cov_groups<HV_PCA_FP16_SPLIT>(partial, ring, T, P, slices, scale_mm, st)); break;
-            case HV_PCA_BF16: HV_TRY(cov_groups<HV_PCA_BF16>(partial, ring, T, P, slices, scale_mm, st)); break;
-            case HV_PCA_FP16: HV_TRY(cov_groups<HV_PCA_FP16>(partial, ring, T, P, slices, scale_mm, st)); break;
-            default: return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
-        }
-        k_pca_cov_reduce<<<dim3(np, 1024 / 64), 256, 0, st>>>(cov, partial, T, slices, np, scale_mm, fp16 ? 1 : 0);
-        HV_CHECK_LAUNCH("k_pca_cov_reduce");
-        return HV_OK;
-    }
+    if (reinterpret_cast<uintptr_t>(ring) & 15) return fail(HV_ERR_INVALID_ARG, "the ring must be 16-byte aligned");
     switch (precision) {
-        case HV_PCA_FP32: k_pca_cov<HV_PCA_FP32><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
-        case HV_PCA_BF16_SPLIT: k_pca_cov<HV_PCA_BF16_SPLIT><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
-        case HV_PCA_FP16_SPLIT: k_pca_cov<HV_PCA_FP16_SPLIT><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
-        case HV_PCA_BF16: k_pca_cov<HV_PCA_BF16><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
-        case HV_PCA_FP16: k_pca_cov<HV_PCA_FP16><<<grid, 64 * kCovWaves, 0, st>>>(partial, ring, T, P, slices, scale_mm); break;
+        case HV_PCA_FP32: case HV_PCA_BF16_SPLIT: case HV_PCA_FP16_SPLIT: case HV_PCA_BF16: case HV_PCA_FP16: break;
         default: return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
     }
-    HV_CHECK_LAUNCH("k_pca_cov");
-    k_pca_cov_reduce<<<dim3(np, 1024 / 64), 256, 0, st>>>(cov, partial, T, slices, np, scale_mm, fp16 ? 1 : 0);
+    // window means, max |x|, max |mu| and the partials of m
+    k_scale_init<<<1, 64, 0, st>>>(scale_mm);
+    HV_CHECK_LAUNCH("k_scale_init");
+    const int mblocks = pca_mean_blocks(P);
+    k_pca_mean<<<mblocks, 256, 0, st>>>(mean, mpart, scale_mm, ring, T, P);
+    HV_CHECK_LAUNCH("k_pca_mean");
+    if (grouped) {
+        switch (precision) {
+            case HV_PCA_BF16_SPLIT: HV_TRY(cov_groups<HV_PCA_BF16_SPLIT>(partial, apart, ring, mean, T, P, slices, scale_mm, st)); break;
+            case HV_PCA_FP16_SPLIT: HV_TRY(cov_groups<HV_PCA_FP16_SPLIT>(partial, apart, ring, mean, T, P, slices, scale_mm, st)); break;
+            case HV_PCA_BF16: HV_TRY(cov_groups<HV_PCA_BF16>(partial, apart, ring, mean, T, P, slices, scale_mm, st)); break;
+            default: HV_TRY(cov_groups<HV_PCA_FP16>(partial, apart, ring, mean, T, P, slices, scale_mm, st)); break;
+        }
+    } else {
+        const unsigned grid = (unsigned)(np * slices);
+        const dim3 b(64 * kCovWaves);
+        switch (precision) {
+            case HV_PCA_FP32: k_pca_cov<HV_PCA_FP32><<<grid, b, 0, st>>>(partial, apart, ring, mean, T, P, slices, scale_mm); break;
+            case HV_PCA_BF16_SPLIT: k_pca_cov<HV_PCA_BF16_SPLIT><<<grid, b, 0, st>>>(partial, apart, ring, mean, T, P, slices, scale_mm); break;
+            case HV_PCA_FP16_SPLIT: k_pca_cov<HV_PCA_FP16_SPLIT><<<grid, b, 0, st>>>(partial, apart, ring, mean, T, P, slices, scale_mm); break;
+            case HV_PCA_BF16: k_pca_cov<HV_PCA_BF16><<<grid, b, 0, st>>>(partial, apart, ring, mean, T, P, slices, scale_mm); break;
+            default: k_pca_cov<HV_PCA_FP16><<<grid, b, 0, st>>>(partial, apart, ring, mean, T, P, slices, scale_mm); break;
+        }
+        HV_CHECK_LAUNCH("k_pca_cov");
+    }
+    k_pca_mean_terms<<<T, 256, 0, st>>>(terms, apart, slices, T, mpart, mblocks);
+    HV_CHECK_LAUNCH("k_pca_mean_terms");
+    k_pca_cov_reduce<<<dim3(np, 1024 / 64), 256, 0, st>>>(cov_double ? (double2*)cov : nullptr,
+                                                           cov_double ? nullptr : (float2*)cov, partial, terms, T,
+                                                           slices, np, scale_mm, fp16 ? 1 : 0);
     HV_CHECK_LAUNCH("k_pca_cov_reduce");
     return HV_OK;
 }
@@ -744,8 +923,15 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
     jacobi_finish<NT, N>([&](int j) { return a(j, j).x; }, v, evals, evecs, order);
 }
 
-template <int NT, int N>
-__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const float2* cov) {
+// A covariance element (float2 from the C-ABI step, double2 in the pipe) as double2.
+__device__ __forceinline__ double2 cov_elem(const float2* c, size_t k) {
+    const float2 v = c[k];
+    return make_double2((double)v.x, (double)v.y);
+}
+__device__ __forceinline__ double2 cov_elem(const double2* c, size_t k) { return c[k]; }
+
+template <int NT, int N, class CT>
+__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const CT* cov) {
     constexpr int LD = N + 1;  // padded columns: row and column walks spread over the banks
     __shared__ double2 A[N * LD], V[N * LD];
     __shared__ Rot rot[N / 2 > 0 ? N / 2 : 1];
@@ -754,7 +940,7 @@ __global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         // Hermitian from the lower triangle (CUBLAS_FILL_MODE_LOWER)
-        const float2 c = i >= j ? cov[i + (size_t)j * N] : cov[j + (size_t)i * N];
+        const double2 c = i >= j ? cov_elem(cov, i + (size_t)j * N) : cov_elem(cov, j + (size_t)i * N);
         A[i + j * LD] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
         V[i + j * LD] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
@@ -776,8 +962,8 @@ struct JacobiWs {
     static constexpr size_t bytes = rounds_off + 256;
 };
 
-template <int NT, int N>
-__global__ void __launch_bounds__(NT) k_jacobi_global(const float2* cov, char* ws) {
+template <int NT, int N, class CT>
+__global__ void __launch_bounds__(NT) k_jacobi_global(const CT* cov, char* ws) {
     using L = JacobiWs<N>;
     __shared__ Rot rot[N / 2];
     __shared__ double red[NT / 64];
@@ -785,7 +971,7 @@ __global__ void __launch_bounds__(NT) k_jacobi_global(const float2* cov, char* w
     double2* A = reinterpret_cast<double2*>(ws + L::a_off);
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
-        const float2 c = i >= j ? cov[i + (size_t)j * N] : cov[j + (size_t)i * N];
+        const double2 c = i >= j ? cov_elem(cov, i + (size_t)j * N) : cov_elem(cov, j + (size_t)i * N);
         A[e] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
     }
     __syncthreads();
@@ -798,8 +984,8 @@ __global__ void __launch_bounds__(NT) k_jacobi_global(const float2* cov, char* w
 // does not fit).  A round's 2 x 2 blocks (k, kk) and (kk, k) hold conjugate
 // elements, so only blocks with k >= kk are updated: every stored element
 // once per round.  Rotations go to the log as in k_jacobi_global.
-template <int NT, int N>
-__global__ void __launch_bounds__(NT) k_jacobi_packed(const float2* cov, char* ws) {
+template <int NT, int N, class CT>
+__global__ void __launch_bounds__(NT) k_jacobi_packed(const CT* cov, char* ws) {
     using L = JacobiWs<N>;
     constexpr int n = N, half = N / 2, nblocks = half * (half + 1) / 2;
     extern __shared__ double2 Lt[];  // element (i, j), i >= j, at i (i + 1) / 2 + j
@@ -819,7 +1005,7 @@ __global__ void __launch_bounds__(NT) k_jacobi_packed(const float2* cov, char* w
     // lower triangle of cov (CUBLAS_FILL_MODE_LOWER), real diagonal
     for (int i = 0; i < n; ++i)
         for (int j = tid; j <= i; j += NT) {
-            const float2 c = cov[i + (size_t)j * n];
+            const double2 c = cov_elem(cov, i + (size_t)j * n);
             at(i, j) = make_double2(c.x, i == j ? 0.0 : c.y);
         }
     __syncthreads();
@@ -966,17 +1152,18 @@ size_t eig_workspace_bytes(int T) {
 #define HV_JACOBI_NT128 1024    // threads of the packed T = 128 eigensolver
 #endif
 
-template <int N>
-static hv_status eig_deferred(double* evals, double2* evecs, const float2* cov, char* ws, hipStream_t st) {
+template <int N, class CT>
+static hv_status eig_deferred(double* evals, double2* evecs, const CT* cov, char* ws, hipStream_t st) {
     if constexpr (N == 128 && HV_JACOBI_PACKED) {
         constexpr size_t lds = jacobi_packed_lds(N);
-        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_jacobi_packed<HV_JACOBI_NT128, N>),
-                                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        static const hipError_t attr = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(&k_jacobi_packed<HV_JACOBI_NT128, N, CT>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_jacobi_packed)");
-        k_jacobi_packed<HV_JACOBI_NT128, N><<<1, HV_JACOBI_NT128, lds, st>>>(cov, ws);
+        k_jacobi_packed<HV_JACOBI_NT128, N, CT><<<1, HV_JACOBI_NT128, lds, st>>>(cov, ws);
         HV_CHECK_LAUNCH("k_jacobi_packed");
     } else {
-        k_jacobi_global<1024, N><<<1, 1024, 0, st>>>(cov, ws);
+        k_jacobi_global<1024, N, CT><<<1, 1024, 0, st>>>(cov, ws);
         HV_CHECK_LAUNCH("k_jacobi_global");
     }
     k_jacobi_apply<N><<<N / 4, 256, 0, st>>>(ws, ws);
@@ -986,15 +1173,16 @@ static hv_status eig_deferred(double* evals, double2* evecs, const float2* cov, 
     return HV_OK;
 }
 
-hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, int T, void* work, hipStream_t st) {
+template <class CT>
+static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T, void* work, hipStream_t st) {
     switch (T) {
-        case 1: k_jacobi_lds<64, 1><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 2: k_jacobi_lds<64, 2><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 4: k_jacobi_lds<64, 4><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 8: k_jacobi_lds<64, 8><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 16: k_jacobi_lds<256, 16><<<1, 256, 0, st>>>(evals, evecs, cov); break;
-        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov); break;
-        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov); break;
+        case 1: k_jacobi_lds<64, 1, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 2: k_jacobi_lds<64, 2, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 4: k_jacobi_lds<64, 4, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 8: k_jacobi_lds<64, 8, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
+        case 16: k_jacobi_lds<256, 16, CT><<<1, 256, 0, st>>>(evals, evecs, cov); break;
+        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov); break;
+        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov); break;
         case 128:
         case 256:
             if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
@@ -1004,6 +1192,12 @@ hv_status launch_eig_jacobi(double* evals, double2* evecs, const float2* cov, in
     }
     HV_CHECK_LAUNCH("k_jacobi");
     return HV_OK;
+}
+
+hv_status launch_eig_jacobi(double* evals, double2* evecs, const void* cov, int cov_double, int T, void* work,
+                            hipStream_t st) {
+    return cov_double ? eig_jacobi(evals, evecs, (const double2*)cov, T, work, st)
+                      : eig_jacobi(evals, evecs, (const float2*)cov, T, work, st);
 }
 
 // ---------------------------------------------------------------------------

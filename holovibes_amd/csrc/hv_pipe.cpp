@@ -94,7 +94,7 @@ Layout layout_of(const hv_compute_settings& s) {
     l.pct = align256(percentile_scratch_bytes());
     l.unwrap = unwrap ? align256(unwrap_scratch_bytes(s.width, s.height)) : 0;
     if (s.time_transformation == HV_TIME_PCA) {
-        l.cov = align256(T * T * sizeof(float2));
+        l.cov = align256(T * T * sizeof(double2));  // FP64 covariance for the FP64 eigensolver
         l.evals = align256(T * sizeof(double));
         l.evecs = align256(T * T * sizeof(double2));
         l.pca_ws = align256(pca_workspace_bytes((int)T, P, s.pca_precision));
@@ -132,7 +132,7 @@ struct Lane {
     // time-window ring and PCA buffers (lane 0: the pipe's; further lanes of a
     // window-replaced three-pass pipe: their own)
     float2* ring = nullptr;
-    float2* cov = nullptr;
+    double2* cov = nullptr;
     double* evals = nullptr;
     double2* evecs = nullptr;
     void* pca_ws = nullptr;
@@ -173,7 +173,7 @@ struct hv_pipe {
     float2* ring = nullptr;
     float2* lens = nullptr;
     void* staging = nullptr;
-    float2* cov = nullptr;
+    double2* cov = nullptr;
     double* evals = nullptr;
     double2* evecs = nullptr;
     void* pca_ws = nullptr;
@@ -277,8 +277,8 @@ hv_status hv_pipe::image_work() {
         } else if (s.time_transformation == HV_TIME_STFT) {
             HV_TRY(launch_stft_image(L.image, L.ring, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else if (s.time_transformation == HV_TIME_PCA) {
-            HV_TRY(launch_pca_cov(L.cov, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
-            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, Tn, L.eig_ws, L.stream));
+            HV_TRY(launch_pca_cov(L.cov, 1, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, Tn, L.eig_ws, L.stream));
             HV_TRY(launch_pca_image(L.image, L.ring, L.evecs, Tn, P, it, pmin, pmax, mm_img, L.stream, kPipeMm));
         } else {
             HV_TRY(launch_complex_to_image(L.image, L.ring, it, pmin, pmax, P, L.stream));
@@ -295,8 +295,8 @@ hv_status hv_pipe::image_work() {
         const int fe = std::min<int>((int)(s.p_start + s.p_width), Tn - 1);
         if (s.time_transformation == HV_TIME_PCA) {
             // moments of the PCA output (gpu_p_acc_buffer = H V, fourier_transform.cc:340-377)
-            HV_TRY(launch_pca_cov(L.cov, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
-            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, Tn, L.eig_ws, L.stream));
+            HV_TRY(launch_pca_cov(L.cov, 1, L.ring, Tn, P, s.pca_precision, L.pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, Tn, L.eig_ws, L.stream));
             if (rec) {
                 // the moments record's projection gives the image too
                 HV_TRY(launch_pca_moments3(rec, rec + P, rec + 2 * P, L.ring, L.evecs, Tn, P, fs, fe, s.camera_fps,
@@ -572,7 +572,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     L0.mm = (uint32_t*)take(l.mm);
     L0.pct = take(l.pct);
     L0.unwrap_ws = take(l.unwrap);
-    p->cov = (float2*)take(l.cov);
+    p->cov = (double2*)take(l.cov);
     p->evals = (double*)take(l.evals);
     p->evecs = (double2*)take(l.evecs);
     p->pca_ws = take(l.pca_ws);
@@ -691,7 +691,10 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
         } else {
             if (it == L.graphs.end()) {
                 hipGraph_t g = nullptr;
-                HV_HIP(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+                // relaxed: the cycle enqueues only launches, async copies and
+                // event records, and blocking HIP calls of other host threads
+                // (a record writer, a read worker) must not void the capture
+                HV_HIP(hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed));
                 hv_status st = p->enqueue_cycle(src, slot0);
                 hipError_t e = hipStreamEndCapture(L.stream, &g);
                 if (st != HV_OK) {
@@ -901,7 +904,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         n.planes = (float2*)take(planes);
         if (own_ring) {
             n.ring = (float2*)take(ring);
-            n.cov = (float2*)take(l.cov);
+            n.cov = (double2*)take(l.cov);
             n.evals = (double*)take(l.evals);
             n.evecs = (double2*)take(l.evecs);
             n.pca_ws = take(l.pca_ws);

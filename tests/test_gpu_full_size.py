@@ -53,3 +53,51 @@ def test_c5_stft_full_size(oracle, img_type):
     print(f"c5_stft 2048x2048 T=128 moment {img_type - 5}", flush=True)
     s, fl = frames("c5_stft", 1, img_type=img_type)
     check(run_both(oracle, s, fl), 1e-5)
+
+
+def test_c5_pca_full_size(oracle):
+    """c5 PCA variant: 2048^2 u16, B = S = T = 128, angular spectrum, PCA
+    with the covariance in plain FP32, split BF16 and split FP16 (the
+    configured one), FP64 Jacobi; moment 0 and moment 1 of components 4..60
+    against the FP64 oracle at 1e-4 relative L2.  The window's spectrum has
+    13 near-degenerate noise eigenvalues 1e-9 lambda_max apart and the range
+    starts among them, so this bar needs the covariance centred and
+    assembled in FP64.  The covariance (step function, float output) and the
+    eigenvalues are compared with the oracle's FP64 ones too."""
+    from holovibes_amd import steps as st
+    from holovibes_amd.constants import HV_IMG_MOMENTS_0
+    from holovibes_amd.pipe import Pipe
+    s, fl = frames("c5_pca", 1)
+    print("c5_pca 2048x2048 T=128: FP64 oracle", flush=True)
+    o = oracle.OraclePipe(s, 64)
+    assert o.push(fl[0]) is not None
+    m_ref = o.moments()
+    cov64, ev64, _ = o.pca_state()
+    o.close()
+    gaps = np.diff(ev64) / ev64[-1]
+    print(f"eigen-gaps / lambda_max at the range edges: 3|4 {gaps[3]:.3g}, 60|61 {gaps[60]:.3g}", flush=True)
+    dev = torch.from_numpy(fl[0].view(np.int16)).cuda()
+    torch.cuda.synchronize()
+    errs = {}
+    for precision in (0, 1, 2):
+        for k in (0, 1):
+            print(f"c5_pca pca_precision={precision} moment {k}", flush=True)
+            s2 = type(s).from_buffer_copy(s)
+            s2.pca_precision = precision
+            s2.img_type = HV_IMG_MOMENTS_0 + k
+            p = Pipe(s2)
+            assert p.push(dev)
+            errs[(precision, k)] = rel_l2(p.image().cpu().numpy(), m_ref[k])
+            if k == 0:
+                ring, _, _ = p.ring()
+                cov = st.pca_cov(ring, precision).cpu().numpy()
+                ecov = rel_l2(cov, cov64)
+                ev, _ = st.eig_hermitian(torch.from_numpy(cov).cuda())
+                eev = float(np.abs(ev.cpu().numpy() - ev64).max() / ev64[-1])
+                print(f"  covariance rel L2 {ecov:.3g}, eigenvalues max err / lambda_max {eev:.3g}", flush=True)
+                assert ecov < 2e-6 and eev < 1e-6, (precision, ecov, eev)
+                del ring
+            p.close()
+            print(f"  image rel L2 vs FP64 {errs[(precision, k)]:.3g}", flush=True)
+    for key, e in errs.items():
+        assert e < 1e-4, (key, e)
