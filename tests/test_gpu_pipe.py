@@ -15,8 +15,10 @@ pytestmark = pytest.mark.gpu
 
 
 def rel_l2(a, b):
-    # in float64: squared norms of PCA images (~1e20 per pixel) overflow float32
-    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    # in float64 / complex128: squared norms of PCA images (~1e20 per pixel) overflow float32
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a, b = a.astype(dt), b.astype(dt)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
