@@ -147,6 +147,7 @@ SIGNATURES = {
     "hv_percentile_scratch_size": [SZ, P],
     "hv_percentile_range": [P, P, I, I, I, F, F, I, P, P],
     "hv_percentile_range_reticle": [P, P, I, I, F, F, F, I, P, P],
+    "hv_percentile_range_noalloc": [P, P, I, I, I, F, F, I, P],
     "hv_apply_contrast": [P, SZ, F, F, P],
     "hv_float_to_ushort": [P, P, SZ, P],
     "hv_contrast_to_ushort": [P, P, SZ, P, I, P],

@@ -391,6 +391,19 @@ hv_status hv_percentile_range(float* range_out, const float* img, int width, int
                              as_stream(stream));
 }
 
+hv_status hv_percentile_range_noalloc(float* range_out, const float* img, int width, int height, int subsample_step,
+                                      float lower_pct, float upper_pct, int clamp_ge_one, void* stream) {
+    HV_NOT_NULL(range_out);
+    HV_NOT_NULL(img);
+    if (width <= 0 || height <= 0) return fail(HV_ERR_INVALID_ARG, "sizes must be positive");
+    const int s = subsample_step > 1 ? subsample_step : 1;
+    if (width % s || height % s) return fail(HV_ERR_INVALID_ARG, "step must divide width and height");
+    if (!(lower_pct >= 0.f && lower_pct <= 100.f && upper_pct >= 0.f && upper_pct <= 100.f))
+        return fail(HV_ERR_INVALID_ARG, "percentiles must lie in [0, 100]");
+    return launch_percentile_static(range_out, img, width, height, s, lower_pct, upper_pct, clamp_ge_one,
+                                    as_stream(stream));
+}
+
 hv_status hv_percentile_range_reticle(float* range_out, const float* img, int width, int height, float reticle_scale,
                                       float lower_pct, float upper_pct, int clamp_ge_one, void* scratch,
                                       void* stream) {

@@ -85,6 +85,10 @@ size_t percentile_scratch_bytes();
 // centred disc (their count is the N of the index rule); < 0: off.
 hv_status launch_percentile(float* range, const float* img, int width, int height, int step, float lo, float hi,
                             int clamp, void* scratch, hipStream_t st, float reticle_scale = -1.f);
+// launch_percentile with its scratch in a static device array of the library
+// (no allocation; calls on one GPU share it).
+hv_status launch_percentile_static(float* range, const float* img, int width, int height, int step, float lo, float hi,
+                                   int clamp, hipStream_t st);
 uint32_t reticle_radius(int width, int height, float scale);
 size_t reticle_pixels(int width, int height, float scale);
 hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots = 1);

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 
 #include "hv_fft.hpp"
 #include "hv_kernels.hpp"
@@ -480,6 +481,53 @@ hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height
     k_unwrap_last<<<g1, 256, 0, st>>>(out, gx, P);
     HV_CHECK_LAUNCH("k_unwrap_last");
     return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// percentiles without any allocation: the radix-select histograms in a
+// file-scope device array, one per GPU, resolved once per device
+// ---------------------------------------------------------------------------
+constexpr size_t kPctStaticBytes = 16 * 1024 + 256;  // the array's fixed size
+static_assert(sizeof(PctScratch) <= kPctStaticBytes, "percentile scratch outgrew its static array");
+
+// In host code this name is a host-side stand-in, not a device address:
+// only the per-device pointer from hipGetSymbolAddress is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[kPctStaticBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The current device's copy of `scratch`, looked up on its first use there.
+static hv_status static_scratch(void** out) {
+    int dev = 0;
+    HV_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(HV_ERR_UNSUPPORTED, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(HV_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return HV_OK;
+}
+
+hv_status launch_percentile_static(float* range, const float* img, int width, int height, int step, float lo, float hi,
+                                   int clamp, hipStream_t st) {
+    if (step < 1) step = 1;
+    // the histograms count in 32 bits
+    const size_t n = (size_t)(width / step) * (size_t)(height / step);
+    if (n > (size_t)UINT32_MAX)
+        return fail(HV_ERR_INVALID_ARG, "hv_percentile_range_noalloc: more than 2^32 - 1 pixels to select from");
+    void* ws = nullptr;
+    HV_TRY(static_scratch(&ws));
+    // k_pct_init clears the histograms at the start of every call
+    return launch_percentile(range, img, width, height, step, lo, hi, clamp, ws, st);
 }
 
 }  // namespace hv

@@ -273,6 +273,16 @@ hv_status hv_percentile_scratch_size(size_t n, size_t* bytes);
 hv_status hv_percentile_range(float* range_out, const float* img, int width, int height, int subsample_step,
                               float lower_pct, float upper_pct, int clamp_ge_one, void* scratch, void* stream);
 
+/* No counterpart in the reference: hv_percentile_range without the scratch
+ * argument, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  It allocates nothing: its 16 KiB
+ * of histograms live in a static array of the library, one per GPU, held
+ * until the process ends.  Calls that overlap on one GPU race on that array
+ * (serialise them, e.g. on one stream).  Images of more than 2^32 - 1
+ * selected pixels are refused (HV_ERR_INVALID_ARG). */
+hv_status hv_percentile_range_noalloc(float* range_out, const float* img, int width, int height, int subsample_step,
+                                      float lower_pct, float upper_pct, int clamp_ge_one, void* stream);
+
 /* Replaces compute_percentile_xy_view with the contrast reticle on
  * (percentile.cu:15-43 copy_circle_kernel, 150-222): the same order
  * statistics over the pixels (x, y) of the W x H image with
