@@ -80,6 +80,12 @@ class SynthParams(ctypes.Structure):
                 ("fd_hi", ctypes.c_int32 * 4), ("seed", ctypes.c_uint64)]
 
 
+class RequestResult(ctypes.Structure):
+    """hv_request_result: one finished or shed serving request."""
+    _fields_ = [("request_id", ctypes.c_int64), ("source", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("latency_ns", ctypes.c_int64), ("output_index", ctypes.c_int64)]
+
+
 class ComputeSettings(ctypes.Structure):
     """hv_compute_settings: the reference settings the path reads."""
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("depth", ctypes.c_uint32),
@@ -98,7 +104,8 @@ class ComputeSettings(ctypes.Structure):
                 ("cuts_enabled", ctypes.c_int32), ("x_start", ctypes.c_uint32), ("x_width", ctypes.c_uint32),
                 ("y_start", ctypes.c_uint32), ("y_width", ctypes.c_uint32),
                 ("registration_enabled", ctypes.c_int32), ("registration_zone", ctypes.c_float),
-                ("contrast_reticle_enabled", ctypes.c_int32), ("contrast_reticle_scale", ctypes.c_float)]
+                ("contrast_reticle_enabled", ctypes.c_int32), ("contrast_reticle_scale", ctypes.c_float),
+                ("output_buffer_size", ctypes.c_uint32)]
 
 
 P = ctypes.c_void_p
@@ -184,6 +191,15 @@ SIGNATURES = {
     "hv_server_output": [P, I, P, P],
     "hv_server_memory_bytes": [P, P],
     "hv_server_destroy": [P],
+    "hv_server_configure": [P, I, I],
+    "hv_server_submit_request": [P, I, P, I64, P],
+    "hv_server_poll_requests": [P, P, I, P],
+    "hv_server_output_frame": [P, I, I64, P, P],
+    "hv_server_stats": [P, P, P],
+    "hv_pipe_output_queue": [P, P, P, P, P, P],
+    "hv_pipe_output_dequeue": [P, P, I, U32, P],
+    "hv_pipe_output_frame": [P, U64, P, P],
+    "hv_pipe_output_count": [P, P],
     "hv_unpack_12_to_16bit": [P, SZ, P, SZ, P],
     "hv_unpack_10_to_16bit": [P, SZ, P, SZ, P],
     "hv_input_queue_create": [U32, U32, U32, U32, U32, P],

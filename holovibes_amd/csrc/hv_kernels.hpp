@@ -71,12 +71,21 @@ hv_status launch_minmax(uint32_t* mm, const float* in, size_t n, hipStream_t st,
 hv_status launch_minmax_decode(float* out, const uint32_t* mm, hipStream_t st, int slots = 1);
 hv_status launch_subsample_f32(float* out, const float* in, int width, int height, int step, hipStream_t st);
 hv_status launch_apply_contrast(float* img, size_t n, float mn, float mx, hipStream_t st);
-hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st);
+// Where a cycle's u16 image goes: `out` (ring null), or slot pos[0] of a ring
+// of `cap` images; the kernel's last workgroup to finish advances pos[0] by
+// `step` (the lanes' interleave) modulo cap and clears the count pos[1].
+struct OutSlot {
+    uint16_t* ring = nullptr;
+    uint32_t cap = 0, step = 1;
+    uint32_t* pos = nullptr;
+};
+hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st, OutSlot slot = OutSlot{});
 // range: 2 floats (encoded = 0) or `encoded` pairs of ordered-uint min/max
 // accumulators (encoded >= 1).
 // range_out (optional): the decoded range actually applied (before invert).
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
-                                    int encoded, int invert, hipStream_t st, float* range_out = nullptr);
+                                    int encoded, int invert, hipStream_t st, float* range_out = nullptr,
+                                    OutSlot slot = OutSlot{});
 hv_status launch_scale(float* img, size_t n, float factor, hipStream_t st);
 size_t percentile_scratch_bytes();
 // Writes range[0..1] = values of rank k0, k1 (ascending) of the image (or its

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -71,7 +72,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 
 struct Layout {
     size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
-    size_t accu, renorm, cuts, reg;
+    size_t accu, renorm, cuts, reg, outq;
     size_t total;
 };
 
@@ -87,7 +88,7 @@ Layout layout_of(const hv_compute_settings& s) {
     const bool unwrap = s.img_type == HV_IMG_ARGUMENT && s.unwrap_2d;
     l.tmp = unwrap ? align256(P * sizeof(float)) : 0;
     l.out = align256(P * sizeof(uint16_t));
-    l.small = 256;   // contrast range (2 floats)
+    l.small = 256;   // contrast range (2 floats) at 0, output-queue slot words at 64
     // per-block image min/max partials: a header pair + one float pair per
     // block of the widest image kernel (P / 64 blocks)
     l.mm = align256(sizeof(float2) * (1 + (P + 63) / 64 + 4096));
@@ -107,7 +108,9 @@ Layout layout_of(const hv_compute_settings& s) {
     l.cuts = s.cuts_enabled ? align256(T * P * sizeof(float2)) + 2 * align256(T * std::max(s.width, s.height) * 4) : 0;
     // registration: mask, reference spectrum, work plane, and a shifted copy
     l.reg = s.registration_enabled ? align256(registration_scratch_bytes(P)) + align256(P * sizeof(float)) : 0;
-    l.total = l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
+    // the output queue: OutputBufferSize u16 images
+    l.outq = align256((size_t)s.output_buffer_size * P * sizeof(uint16_t));
+    l.total = l.outq + l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
               l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts + l.reg;
     return l;
 }
@@ -143,6 +146,7 @@ struct Lane {
     uint16_t* out = nullptr;
     uint32_t* mm = nullptr;
     float* range = nullptr;
+    uint32_t* oq_pos = nullptr;  // output-queue slot of this lane's next image, finished-block count
     void* pct = nullptr;
     void* unwrap_ws = nullptr;
     void* mem = nullptr;  // own allocation (lanes >= 1)
@@ -188,6 +192,14 @@ struct hv_pipe {
     void* reg = nullptr;            // registration scratch
     float* reg_tmp = nullptr;
     bool reg_ref_set = false;       // the reference image has been taken
+    // output queue (Queue semantics, queue.cc:149-189): image k in slot k mod oq_cap
+    uint16_t* outq = nullptr;
+    uint32_t oq_cap = 0;
+    uint64_t out_count = 0;         // images produced since creation
+    uint32_t oq_start = 0, oq_size = 0;
+    bool oq_overwritten = false;
+    std::vector<hipEvent_t> oq_ev;  // completion of the image in each slot
+    std::vector<char> oq_ev_rec;
     uint32_t accu_held = 0;         // host mirror of the accumulation queue's size
     // host mirror of the reference's state
     uint32_t batch_index = 0;  // BatchEnv::batch_index
@@ -213,6 +225,44 @@ struct hv_pipe {
 
     hv_status image_work();
     hv_status enqueue_cycle(const void* frames, int slot0);
+    // where lane L's cycle writes its u16 image
+    OutSlot out_slot(const Lane& L) const {
+        OutSlot o;
+        if (!oq_cap) return o;
+        o.ring = outq;
+        o.cap = oq_cap;
+        o.step = timing ? 1u : (uint32_t)nlanes;
+        o.pos = L.oq_pos;
+        return o;
+    }
+    // After a change of lanes, timing or cycle order (next = 0): lane i
+    // writes images out_count + i, out_count + i + nlanes, ...
+    hv_status reset_output_slots() {
+        if (!oq_cap) return HV_OK;
+        for (auto& l : lane)
+            if (l.stream) HV_HIP(hipStreamSynchronize(l.stream));
+        const int n = timing ? 1 : nlanes;
+        for (int i = 0; i < n && i < (int)lane.size(); ++i) {
+            const uint32_t w[2] = {(uint32_t)((out_count + (uint64_t)i) % oq_cap), 0u};
+            HV_HIP(hipMemcpy(lane[i].oq_pos, w, sizeof(w), hipMemcpyHostToDevice));
+        }
+        return HV_OK;
+    }
+    // Queue::enqueue bookkeeping of the image the current lane just queued
+    hv_status output_enqueued() {
+        if (!oq_cap) return HV_OK;
+        const uint32_t slot = (uint32_t)(out_count % oq_cap);
+        HV_HIP(hipEventRecord(oq_ev[slot], lane[cur].stream));
+        oq_ev_rec[slot] = 1;
+        ++out_count;
+        if (oq_size < oq_cap) {
+            ++oq_size;
+        } else {
+            oq_start = (oq_start + 1) % oq_cap;
+            oq_overwritten = true;
+        }
+        return HV_OK;
+    }
     bool shared_ring() const { return nlanes > 1 && lane[1].kind == LANE_SHARED_RING; }
     // min/max folded into the image kernel, accumulators re-armed by the
     // cycle's first kernel (the row pass of a space transform)
@@ -362,7 +412,7 @@ hv_status hv_pipe::image_work() {
     }
     if (s.log_enabled) HV_TRY(launch_log10(L.image, P, L.stream));
     HV_TRY(mark(4));
-    if (!s.contrast_enabled) return launch_float_to_ushort(L.out, L.image, P, L.stream);
+    if (!s.contrast_enabled) return launch_float_to_ushort(L.out, L.image, P, L.stream, out_slot(L));
     switch (s.contrast_mode) {
         case HV_CONTRAST_MINMAX:
             if (!minmax_fused()) {
@@ -372,7 +422,7 @@ hv_status hv_pipe::image_work() {
             }
             // the contrast kernel decodes the accumulators and records the range it applied
             return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.mm, kPipeMm, s.contrast_invert, L.stream,
-                                             L.range);
+                                             L.range, out_slot(L));
         case HV_CONTRAST_PERCENTILE:
             HV_TRY(launch_percentile(L.range, L.image, s.width, s.height, (int)s.contrast_subsample, s.contrast_lower,
                                      s.contrast_upper, 1, L.pct, L.stream, reticle()));
@@ -385,7 +435,8 @@ hv_status hv_pipe::image_work() {
         default:  // MANUAL: range uploaded at create
             break;
     }
-    return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.range, 0, s.contrast_invert, L.stream);
+    return launch_contrast_to_ushort(L.out, L.cimage, L.image, P, L.range, 0, s.contrast_invert, L.stream, nullptr,
+                                     out_slot(L));
 }
 
 hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
@@ -453,6 +504,7 @@ void hv_default_settings(hv_compute_settings* s) {
     s->registration_zone = 0.7f;  // Registration (view_struct.hh)
     s->contrast_reticle_enabled = 0;   // ContrastReticleDisplayEnabled{false} (holovibes.hh:403)
     s->contrast_reticle_scale = 0.5f;  // ContrastReticleScale{0.5f} (holovibes.hh:400)
+    s->output_buffer_size = 1024;      // OutputBufferSize{1024} (holovibes.hh:375)
 }
 
 hv_status hv_validate_settings(const hv_compute_settings* s) {
@@ -510,6 +562,7 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     }
     if (s->renorm_enabled && s->renorm_constant > 31) return fail(HV_ERR_INVALID_ARG, "renorm_constant must be <= 31");
     if (s->img_accu_level > 4096) return fail(HV_ERR_INVALID_ARG, "img_accu_level must be <= 4096");
+    if (s->output_buffer_size > 65536) return fail(HV_ERR_INVALID_ARG, "output_buffer_size must be <= 65536");
     if (s->registration_enabled && !(s->registration_zone > 0.f && s->registration_zone <= 1.f))
         return fail(HV_ERR_INVALID_ARG, "registration_zone must be in (0, 1]");
     if (s->cuts_enabled) {
@@ -569,6 +622,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     L0.tmp = (float*)take(l.tmp);
     L0.out = (uint16_t*)take(l.out);
     L0.range = (float*)take(l.small);
+    L0.oq_pos = (uint32_t*)((char*)L0.range + 64);
     L0.mm = (uint32_t*)take(l.mm);
     L0.pct = take(l.pct);
     L0.unwrap_ws = take(l.unwrap);
@@ -592,6 +646,8 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
         p->cut_xz = (float*)take(cut);
         p->cut_yz = (float*)take(cut);
     }
+    p->outq = (uint16_t*)take(l.outq);
+    p->oq_cap = s->output_buffer_size;
     L0.ring = p->ring;
     L0.cov = p->cov;
     L0.evals = p->evals;
@@ -616,6 +672,18 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
             e = hipErrorLaunchFailure;
         if (e == hipSuccess) e = hipStreamSynchronize(L0.stream);
         if (e != hipSuccess) st = hip_fail(e, "pipe init");
+    }
+    if (st == HV_OK && p->oq_cap) {
+        p->oq_ev.assign(p->oq_cap, nullptr);
+        p->oq_ev_rec.assign(p->oq_cap, 0);
+        for (auto& ev : p->oq_ev) {
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e != hipSuccess) {
+                st = hip_fail(e, "hipEventCreate(output queue)");
+                break;
+            }
+        }
+        if (st == HV_OK) st = p->reset_output_slots();
     }
     if (st != HV_OK) {
         hv_pipe_destroy(p.release());
@@ -718,6 +786,7 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     } else {
         HV_TRY(p->enqueue_cycle(frames, slot0));
     }
+    HV_TRY(p->output_enqueued());
     if (pending) {
         p->autocontrast_pending = false;  // rendering.cc:251-258
         // the frozen range reaches the other lanes in their stream order
@@ -744,7 +813,11 @@ hv_status hv_pipe_set_timing(hv_pipe* p, int enabled) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (enabled && !p->ev[0])
         for (auto& e : p->ev) HV_HIP(hipEventCreate(&e));
-    p->timing = enabled != 0;
+    if (p->timing != (enabled != 0)) {
+        p->timing = enabled != 0;
+        p->next = 0;
+        HV_TRY(p->reset_output_slots());  // timed cycles all run on lane 0
+    }
     for (auto& t : p->stage_ms) t = 0.0;
     p->timed_cycles = 0;
     return HV_OK;
@@ -761,10 +834,65 @@ hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out
                          const float** out_f32_image) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     const Lane& L = p->lane[p->last];
-    if (out_u16) *out_u16 = L.out;
+    if (out_u16)
+        *out_u16 = p->oq_cap && p->out_count ? p->outq + (size_t)((p->out_count - 1) % p->oq_cap) * p->P : L.out;
     if (out_f32_contrast) *out_f32_contrast = p->s.contrast_enabled ? L.cimage : L.image;
     if (out_f32_image) *out_f32_image = L.image;
     return p->have_output ? HV_OK : HV_ERR_NOT_READY;
+}
+
+hv_status hv_pipe_output_queue(hv_pipe* p, const uint16_t** base, uint32_t* max_size, uint32_t* start_index,
+                               uint32_t* size, int* has_overwritten) {
+    if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
+    if (base) *base = p->outq;
+    if (max_size) *max_size = p->oq_cap;
+    if (start_index) *start_index = p->oq_start;
+    if (size) *size = p->oq_size;
+    if (has_overwritten) *has_overwritten = p->oq_overwritten ? 1 : 0;
+    return HV_OK;
+}
+
+hv_status hv_pipe_output_dequeue(hv_pipe* p, void* dst, int memcpy_kind, uint32_t nb, void* stream) {
+    if (!p || !dst) return fail(HV_ERR_INVALID_ARG, "null argument");
+    if (memcpy_kind != (int)hipMemcpyDeviceToHost && memcpy_kind != (int)hipMemcpyDeviceToDevice)
+        return fail(HV_ERR_INVALID_ARG, "output dequeue copies to the host (2) or the device (3)");
+    if (!p->oq_cap) return fail(HV_ERR_INVALID_ARG, "the pipe has no output queue (output_buffer_size 0)");
+    if (nb == 0 || nb > p->oq_size) return fail(HV_ERR_NOT_READY, "fewer images queued than asked for");
+    hipStream_t st = as_stream(stream);
+    const size_t fb = p->P * sizeof(uint16_t);
+    // the newest of the nb images completes last on its lane; wait for each
+    // lane's latest of them (images of one lane complete in order)
+    for (uint32_t k = nb > (uint32_t)p->nlanes ? nb - (uint32_t)p->nlanes : 0; k < nb; ++k) {
+        const uint32_t slot = (p->oq_start + k) % p->oq_cap;
+        if (p->oq_ev_rec[slot]) HV_HIP(hipStreamWaitEvent(st, p->oq_ev[slot], 0));
+    }
+    const uint32_t first = std::min(nb, p->oq_cap - p->oq_start);
+    HV_HIP(hipMemcpyAsync(dst, p->outq + (size_t)p->oq_start * p->P, first * fb, (hipMemcpyKind)memcpy_kind, st));
+    if (nb > first)
+        HV_HIP(hipMemcpyAsync((char*)dst + first * fb, p->outq, (nb - first) * fb, (hipMemcpyKind)memcpy_kind, st));
+    HV_HIP(hipStreamSynchronize(st));
+    p->oq_start = (p->oq_start + nb) % p->oq_cap;
+    p->oq_size -= nb;
+    return HV_OK;
+}
+
+hv_status hv_pipe_output_frame(hv_pipe* p, uint64_t index, void* stream, const uint16_t** frame) {
+    if (!p || !frame) return fail(HV_ERR_INVALID_ARG, "null argument");
+    if (!p->oq_cap) return fail(HV_ERR_INVALID_ARG, "the pipe has no output queue (output_buffer_size 0)");
+    if (index >= p->out_count || p->out_count - index > p->oq_cap) {
+        set_error("image " + std::to_string(index) + " is not in the output queue");
+        return HV_ERR_NOT_READY;
+    }
+    const uint32_t slot = (uint32_t)(index % p->oq_cap);
+    if (stream && p->oq_ev_rec[slot]) HV_HIP(hipStreamWaitEvent(as_stream(stream), p->oq_ev[slot], 0));
+    *frame = p->outq + (size_t)slot * p->P;
+    return HV_OK;
+}
+
+hv_status hv_pipe_output_count(hv_pipe* p, uint64_t* count) {
+    if (!p || !count) return fail(HV_ERR_INVALID_ARG, "null argument");
+    *count = p->out_count;
+    return HV_OK;
 }
 
 hv_status hv_pipe_registration_shift(hv_pipe* p, int* shift_x, int* shift_y) {
@@ -798,7 +926,7 @@ hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled) {
         (void)hipFree(p->rec);
         p->rec = nullptr;
         p->path = choose_path(p->s, p->level);
-        return HV_OK;
+        return p->reset_output_slots();
     }
     if (p->s.time_transformation != HV_TIME_STFT && p->s.time_transformation != HV_TIME_PCA)
         return fail(HV_ERR_UNSUPPORTED, "moments are recorded from the STFT or PCA output");
@@ -808,7 +936,7 @@ hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled) {
     p->path = PATH_THREE_PASS;
     p->drop_graphs();
     p->next = 0;
-    return HV_OK;
+    return p->reset_output_slots();
 }
 
 hv_status hv_pipe_record_frames(hv_pipe* p, const float** frames) {
@@ -841,7 +969,7 @@ hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     p->drop_graphs();
     p->nlanes = 1;  // extra lanes hold one path's buffers: hv_pipe_set_lanes again after a path change
     p->next = 0;
-    return HV_OK;
+    return p->reset_output_slots();
 }
 
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
@@ -855,6 +983,8 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     if (lanes > 1 && p->s.cuts_enabled) return fail(HV_ERR_UNSUPPORTED, "the cuts run one cycle at a time");
     if (lanes > 1 && p->s.registration_enabled)
         return fail(HV_ERR_UNSUPPORTED, "registration runs one cycle at a time");
+    if (p->oq_cap && p->oq_cap < (uint32_t)lanes)
+        return fail(HV_ERR_INVALID_ARG, "the output queue must hold at least one image per lane");
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
     int pmin, pmax;
@@ -923,6 +1053,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         n.tmp = (float*)take(l.tmp);
         n.out = (uint16_t*)take(l.out);
         n.range = (float*)take(l.small);
+        n.oq_pos = (uint32_t*)((char*)n.range + 64);
         n.mm = (uint32_t*)take(l.mm);
         n.pct = take(l.pct);
         n.unwrap_ws = take(l.unwrap);
@@ -935,7 +1066,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
     HV_HIP(hipStreamSynchronize(p->lane[0].stream));
     p->nlanes = lanes;
     p->next = 0;
-    return HV_OK;
+    return p->reset_output_slots();
 }
 
 hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, int n_frames, int iters, float* ms) {
@@ -1024,6 +1155,8 @@ hv_status hv_pipe_destroy(hv_pipe* p) {
         if (l.tt_done) (void)hipEventDestroy(l.tt_done);
     for (auto e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : p->oq_ev)
+        if (e) (void)hipEventDestroy(e);
     if (p->base) (void)hipFree(p->base);
     if (p->rec) (void)hipFree(p->rec);
     if (p->own_stream && p->lane[0].stream) (void)hipStreamDestroy(p->lane[0].stream);
@@ -1034,28 +1167,92 @@ hv_status hv_pipe_destroy(hv_pipe* p) {
 }  // extern "C"
 
 // ===========================================================================
-// server: several sources, admission by HBM budget, latency accounting
+// server: several sources, admission by HBM budget, request slots,
+// continuous batching, latency accounting
 // ===========================================================================
 namespace {
 int64_t now_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
         .count();
 }
+struct Request {
+    int64_t id;
+    const void* frames;
+    int64_t t_arrival;
+};
 struct Inflight {
+    int64_t id;
     int source;
     hipEvent_t done;
     int64_t t_arrival;
-    bool produced;
+    int64_t output_index;  // -1: the cycle produced no image
+};
+struct Source {
+    hv_pipe* pipe = nullptr;
+    size_t bytes = 0;
+    int slots = 2, depth = 16;   // requests on the GPU at once, requests waiting
+    int inflight = 0;
+    std::deque<Request> pending;
 };
 }  // namespace
 
 struct hv_server {
     size_t budget = 0, used = 0;
     int next_id = 0;
-    std::map<int, hv_pipe*> sources;
-    std::map<int, size_t> bytes;
+    int64_t next_request = 0;
+    int slots = 2, depth = 16;
+    uint64_t shed = 0;
+    std::map<int, Source> sources;
     std::deque<Inflight> inflight;
+    std::deque<hv_request_result> results;  // finished or shed, not yet polled
     std::vector<hipEvent_t> free_events;
+
+    // runs one request on its source's pipe (a slot is free)
+    hv_status admit(int id, Source& src, const Request& r) {
+        int produced = 0;
+        HV_TRY(hv_pipe_push_batch(src.pipe, r.frames, &produced));
+        hipEvent_t ev;
+        if (!free_events.empty()) {
+            ev = free_events.back();
+            free_events.pop_back();
+        } else {
+            HV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
+        HV_HIP(hipEventRecord(ev, src.pipe->lane[src.pipe->last].stream));
+        inflight.push_back({r.id, id, ev, r.t_arrival, produced ? (int64_t)src.pipe->out_count - 1 : -1});
+        ++src.inflight;
+        return HV_OK;
+    }
+    // moves finished requests to `results` and admits waiting ones into the
+    // slots that freed, in arrival order per source
+    hv_status progress() {
+        for (auto it = inflight.begin(); it != inflight.end();) {
+            const hipError_t q = hipEventQuery(it->done);
+            if (q == hipErrorNotReady) {
+                ++it;
+                continue;
+            }
+            if (q != hipSuccess) return hip_fail(q, "hipEventQuery");
+            const int64_t t = now_ns();
+            if (it->source >= 0) {
+                results.push_back({it->id, it->source, it->output_index >= 0 ? HV_REQ_DONE : HV_REQ_NO_OUTPUT,
+                                   t - it->t_arrival, it->output_index});
+                auto sit = sources.find(it->source);
+                if (sit != sources.end()) --sit->second.inflight;
+            }
+            free_events.push_back(it->done);
+            it = inflight.erase(it);
+        }
+        for (auto& kv : sources) {
+            Source& src = kv.second;
+            while (src.inflight < src.slots && !src.pending.empty()) {
+                const Request r = src.pending.front();
+                src.pending.pop_front();
+                HV_TRY(admit(kv.first, src, r));
+            }
+        }
+        return HV_OK;
+    }
 };
 
 extern "C" {
@@ -1064,6 +1261,14 @@ hv_status hv_server_create(size_t budget_bytes, hv_server** out) {
     if (!out) return fail(HV_ERR_INVALID_ARG, "out is null");
     *out = new hv_server;
     (*out)->budget = budget_bytes;
+    return HV_OK;
+}
+
+hv_status hv_server_configure(hv_server* s, int slots, int pending) {
+    if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
+    if (slots < 1 || pending < 0) return fail(HV_ERR_INVALID_ARG, "slots must be >= 1 and pending >= 0");
+    s->slots = slots;
+    s->depth = pending;
     return HV_OK;
 }
 
@@ -1076,8 +1281,12 @@ hv_status hv_server_add_source(hv_server* s, const hv_compute_settings* settings
     HV_TRY(hv_pipe_create(settings, nullptr, &p));
     hv_pipe_set_graph(p, 1);
     const int id = s->next_id++;
-    s->sources[id] = p;
-    s->bytes[id] = need;
+    Source src;
+    src.pipe = p;
+    src.bytes = need;
+    src.slots = s->slots;
+    src.depth = s->depth;
+    s->sources[id] = std::move(src);
     s->used += need;
     *source_id = id;
     return HV_OK;
@@ -1087,55 +1296,69 @@ hv_status hv_server_remove_source(hv_server* s, int source_id) {
     if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
     auto it = s->sources.find(source_id);
     if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
-    hv_pipe_sync(it->second);
-    // retire its in-flight entries
+    hv_pipe_sync(it->second.pipe);
+    // retire its in-flight entries; its waiting requests are shed
     for (auto& f : s->inflight)
         if (f.source == source_id) f.source = -1;
-    hv_pipe_destroy(it->second);
-    s->used -= s->bytes[source_id];
-    s->bytes.erase(source_id);
+    for (const Request& r : it->second.pending) {
+        s->results.push_back({r.id, source_id, HV_REQ_SHED, 0, -1});
+        ++s->shed;
+    }
+    hv_pipe_destroy(it->second.pipe);
+    s->used -= it->second.bytes;
     s->sources.erase(it);
     return HV_OK;
 }
 
-hv_status hv_server_submit(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns) {
+hv_status hv_server_submit_request(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns,
+                                   int64_t* request_id) {
     if (!s || !frames) return fail(HV_ERR_INVALID_ARG, "null argument");
     auto it = s->sources.find(source_id);
     if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
-    hv_pipe* p = it->second;
-    int produced = 0;
-    HV_TRY(hv_pipe_push_batch(p, frames, &produced));
-    hipEvent_t ev;
-    if (!s->free_events.empty()) {
-        ev = s->free_events.back();
-        s->free_events.pop_back();
-    } else {
-        HV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    Source& src = it->second;
+    const Request r{s->next_request++, frames, t_arrival_ns};
+    if (request_id) *request_id = r.id;
+    HV_TRY(s->progress());  // free slots first, so that a request never waits behind a finished one
+    if (src.inflight < src.slots && src.pending.empty()) return s->admit(source_id, src, r);
+    if ((int)src.pending.size() >= src.depth) {
+        // a full waiting queue sheds its oldest request (or this one without a queue)
+        const Request old = src.depth > 0 ? src.pending.front() : r;
+        if (src.depth > 0) src.pending.pop_front();
+        s->results.push_back({old.id, source_id, HV_REQ_SHED, 0, -1});
+        ++s->shed;
+        if (src.depth == 0) return HV_OK;
     }
-    HV_HIP(hipEventRecord(ev, p->lane[p->last].stream));
-    s->inflight.push_back({source_id, ev, t_arrival_ns, produced != 0});
+    src.pending.push_back(r);
+    return HV_OK;
+}
+
+hv_status hv_server_submit(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns) {
+    return hv_server_submit_request(s, source_id, frames, t_arrival_ns, nullptr);
+}
+
+hv_status hv_server_poll_requests(hv_server* s, hv_request_result* out, int max_n, int* n_out) {
+    if (!s || !n_out || (max_n > 0 && !out)) return fail(HV_ERR_INVALID_ARG, "null argument");
+    HV_TRY(s->progress());
+    int n = 0;
+    while (n < max_n && !s->results.empty()) {
+        out[n++] = s->results.front();
+        s->results.pop_front();
+    }
+    *n_out = n;
     return HV_OK;
 }
 
 hv_status hv_server_poll(hv_server* s, int* sources, int64_t* latency_ns, int max_n, int* n_out) {
     if (!s || !n_out) return fail(HV_ERR_INVALID_ARG, "null argument");
+    HV_TRY(s->progress());
     int n = 0;
-    // completion order may differ across sources: scan every in-flight batch
-    for (auto itr = s->inflight.begin(); itr != s->inflight.end() && n < max_n;) {
-        hipError_t q = hipEventQuery(itr->done);
-        if (q == hipErrorNotReady) {
-            ++itr;
-            continue;
-        }
-        if (q != hipSuccess) return hip_fail(q, "hipEventQuery");
-        const int64_t t = now_ns();
-        if (itr->produced && itr->source >= 0) {
-            if (sources) sources[n] = itr->source;
-            if (latency_ns) latency_ns[n] = t - itr->t_arrival;
-            ++n;
-        }
-        s->free_events.push_back(itr->done);
-        itr = s->inflight.erase(itr);
+    while (n < max_n && !s->results.empty()) {
+        const hv_request_result r = s->results.front();
+        s->results.pop_front();
+        if (r.status != HV_REQ_DONE) continue;
+        if (sources) sources[n] = r.source;
+        if (latency_ns) latency_ns[n] = r.latency_ns;
+        ++n;
     }
     *n_out = n;
     return HV_OK;
@@ -1143,15 +1366,42 @@ hv_status hv_server_poll(hv_server* s, int* sources, int64_t* latency_ns, int ma
 
 hv_status hv_server_drain(hv_server* s) {
     if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
-    for (auto& kv : s->sources) HV_TRY(hv_pipe_sync(kv.second));
-    return HV_OK;
+    for (;;) {
+        HV_TRY(s->progress());
+        bool waiting = false;
+        for (auto& kv : s->sources) waiting = waiting || !kv.second.pending.empty();
+        if (!waiting) break;
+        // the oldest in-flight request frees a slot
+        if (!s->inflight.empty()) HV_HIP(hipEventSynchronize(s->inflight.front().done));
+    }
+    for (auto& kv : s->sources) HV_TRY(hv_pipe_sync(kv.second.pipe));
+    return s->progress();
 }
 
 hv_status hv_server_output(hv_server* s, int source_id, const uint16_t** out_u16, const float** out_f32_image) {
     if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
     auto it = s->sources.find(source_id);
     if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
-    return hv_pipe_output(it->second, out_u16, nullptr, out_f32_image);
+    return hv_pipe_output(it->second.pipe, out_u16, nullptr, out_f32_image);
+}
+
+hv_status hv_server_output_frame(hv_server* s, int source_id, int64_t output_index, void* stream,
+                                 const uint16_t** frame) {
+    if (!s || output_index < 0) return fail(HV_ERR_INVALID_ARG, "null server or negative output index");
+    auto it = s->sources.find(source_id);
+    if (it == s->sources.end()) return fail(HV_ERR_INVALID_ARG, "unknown source");
+    return hv_pipe_output_frame(it->second.pipe, (uint64_t)output_index, stream, frame);
+}
+
+hv_status hv_server_stats(hv_server* s, uint64_t* shed, uint32_t* pending) {
+    if (!s) return fail(HV_ERR_INVALID_ARG, "server is null");
+    if (shed) *shed = s->shed;
+    if (pending) {
+        uint32_t n = 0;
+        for (auto& kv : s->sources) n += (uint32_t)kv.second.pending.size();
+        *pending = n;
+    }
+    return HV_OK;
 }
 
 hv_status hv_server_memory_bytes(hv_server* s, size_t* bytes) {
@@ -1162,7 +1412,7 @@ hv_status hv_server_memory_bytes(hv_server* s, size_t* bytes) {
 
 hv_status hv_server_destroy(hv_server* s) {
     if (!s) return HV_OK;
-    for (auto& kv : s->sources) hv_pipe_destroy(kv.second);
+    for (auto& kv : s->sources) hv_pipe_destroy(kv.second.pipe);
     for (auto& f : s->inflight) (void)hipEventDestroy(f.done);
     for (auto e : s->free_events) (void)hipEventDestroy(e);
     delete s;

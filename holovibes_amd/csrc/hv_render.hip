@@ -123,14 +123,38 @@ __global__ void __launch_bounds__(256) k_apply_contrast(float* img, size_t n, fl
         img[i] = factor * (img[i] - mn);
 }
 
-__global__ void __launch_bounds__(256) k_float_to_ushort(uint16_t* out, const float* in, size_t n) {
+// The image's destination: `out`, or the output queue slot pos[0].  Every
+// block reads the slot before it counts itself done (out_slot_done), so the
+// last block's advance cannot reach a block still to write.
+__device__ __forceinline__ uint16_t* out_slot_target(uint16_t* out, const OutSlot& o, size_t n) {
+    if (!o.ring) return out;
+    const uint32_t slot = __hip_atomic_load(&o.pos[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return o.ring + (size_t)slot * n;
+}
+__device__ __forceinline__ void out_slot_done(const OutSlot& o) {
+    if (!o.ring) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev = atomicAdd(&o.pos[1], 1u);
+        if (prev == gridDim.x * gridDim.y - 1) {
+            const uint32_t slot = __hip_atomic_load(&o.pos[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&o.pos[0], (slot + o.step) % o.cap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&o.pos[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_float_to_ushort(uint16_t* out, const float* in, size_t n, OutSlot slot) {
+    uint16_t* dst = out_slot_target(out, slot, n);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        out[i] = f2u16(in[i]);
+        dst[i] = f2u16(in[i]);
+    out_slot_done(slot);
 }
 
 __global__ void __launch_bounds__(256)
-    k_contrast_u16(uint16_t* __restrict__ out, float* __restrict__ out_f32, const float* __restrict__ in, size_t n,
-                   const void* range, int encoded, int invert, float* range_out) {
+    k_contrast_u16(uint16_t* __restrict__ out_arg, float* __restrict__ out_f32, const float* __restrict__ in, size_t n,
+                   const void* range, int encoded, int invert, float* range_out, OutSlot slot) {
+    uint16_t* __restrict__ out = out_slot_target(out_arg, slot, n);
     float mn, mx;
     if (encoded) {
         mm_decode_block((const uint32_t*)range, encoded, mn, mx);
@@ -169,6 +193,7 @@ __global__ void __launch_bounds__(256)
         if (out_f32) out_f32[i] = v;
         out[i] = f2u16(v);
     }
+    out_slot_done(slot);
 }
 
 __global__ void __launch_bounds__(256) k_scale(float* img, size_t n, float f) {
@@ -181,16 +206,16 @@ hv_status launch_apply_contrast(float* img, size_t n, float mn, float mx, hipStr
     HV_CHECK_LAUNCH("k_apply_contrast");
     return HV_OK;
 }
-hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st) {
-    k_float_to_ushort<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n);
+hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipStream_t st, OutSlot slot) {
+    k_float_to_ushort<<<blocks_for(n, 256), 256, 0, st>>>(out, in, n, slot);
     HV_CHECK_LAUNCH("k_float_to_ushort");
     return HV_OK;
 }
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
-                                    int encoded, int invert, hipStream_t st, float* range_out) {
+                                    int encoded, int invert, hipStream_t st, float* range_out, OutSlot slot) {
     const unsigned blocks = blocks_for((n + 3) / 4, 256);
     if (encoded == kMmPartials) HV_TRY(collapse_partials((uint32_t*)range, blocks, encoded, st));
-    k_contrast_u16<<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out);
+    k_contrast_u16<<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot);
     HV_CHECK_LAUNCH("k_contrast_u16");
     return HV_OK;
 }

@@ -142,6 +142,38 @@ class Pipe:
         self.sync()
         return t
 
+    def output_queue(self) -> dict:
+        """The output queue's state (Queue getters): base pointer, max_size,
+        start_index, size, has_overwritten, and the images produced so far."""
+        b, m, a, n, ow, c = ctypes.c_void_p(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int(), \
+            ctypes.c_uint64()
+        _lib.call("hv_pipe_output_queue", self.h, ctypes.byref(b), ctypes.byref(m), ctypes.byref(a), ctypes.byref(n),
+                  ctypes.byref(ow))
+        _lib.call("hv_pipe_output_count", self.h, ctypes.byref(c))
+        return {"base": b.value, "max_size": m.value, "start_index": a.value, "size": n.value,
+                "has_overwritten": bool(ow.value), "produced": c.value}
+
+    def output_frame(self, index: int):
+        """The u16 image of the index-th produced image while the queue holds
+        it (numpy), else None."""
+        fr = ctypes.c_void_p()
+        st = _lib.status("hv_pipe_output_frame", self.h, index, None, ctypes.byref(fr))
+        if st == _lib.HV_ERR_NOT_READY:
+            return None
+        if st != _lib.HV_OK:
+            _lib.call("hv_pipe_output_frame", self.h, index, None, ctypes.byref(fr))
+        self.sync()
+        t = torch.empty((self.settings.height, self.settings.width), dtype=torch.int16, device="cuda")
+        copy_d2d(t, fr.value, self.P * 2, self.stream)
+        self.sync()
+        return t.cpu().numpy().view(np.uint16)
+
+    def output_dequeue(self, n: int = 1) -> np.ndarray:
+        """Queue::dequeue of the n oldest images into host memory."""
+        out = np.empty((n, self.settings.height, self.settings.width), np.uint16)
+        _lib.call("hv_pipe_output_dequeue", self.h, ctypes.c_void_p(out.ctypes.data), _D2H, n, None)
+        return out
+
     def contrast_range(self) -> np.ndarray:
         self.sync()
         r = ctypes.c_void_p()
@@ -193,8 +225,45 @@ class Server:
     def remove_source(self, sid: int) -> None:
         _lib.call("hv_server_remove_source", self.h, sid)
 
-    def submit(self, sid: int, frames_ptr: int, t_arrival_ns: int) -> None:
-        _lib.call("hv_server_submit", self.h, sid, ctypes.c_void_p(frames_ptr), t_arrival_ns)
+    def configure(self, slots: int, pending: int) -> None:
+        """Requests on the GPU at once per source, and requests waiting per source."""
+        _lib.call("hv_server_configure", self.h, slots, pending)
+
+    def submit(self, sid: int, frames_ptr: int, t_arrival_ns: int) -> int:
+        """Submits a request; returns its id."""
+        rid = ctypes.c_int64()
+        _lib.call("hv_server_submit_request", self.h, sid, ctypes.c_void_p(frames_ptr), t_arrival_ns,
+                  ctypes.byref(rid))
+        return rid.value
+
+    def poll_requests(self, max_n: int = 4096):
+        """Finished and shed requests: dicts with request_id, source, status
+        (DONE 0, NO_OUTPUT 1, SHED 2), latency_ns, output_index."""
+        from ._lib import RequestResult
+        buf = (RequestResult * max_n)()
+        n = ctypes.c_int()
+        _lib.call("hv_server_poll_requests", self.h, buf, max_n, ctypes.byref(n))
+        return [{"request_id": r.request_id, "source": r.source, "status": r.status, "latency_ns": r.latency_ns,
+                 "output_index": r.output_index} for r in buf[:n.value]]
+
+    def output_frame(self, sid: int, index: int, height: int, width: int):
+        """A request's u16 image from its source's output queue (numpy), or None once overwritten."""
+        fr = ctypes.c_void_p()
+        st = _lib.status("hv_server_output_frame", self.h, sid, index, None, ctypes.byref(fr))
+        if st == _lib.HV_ERR_NOT_READY:
+            return None
+        if st != _lib.HV_OK:
+            _lib.call("hv_server_output_frame", self.h, sid, index, None, ctypes.byref(fr))
+        self.drain()
+        t = torch.empty((height, width), dtype=torch.int16, device="cuda")
+        copy_d2d(t, fr.value, height * width * 2, None)
+        hip().hipStreamSynchronize(None)
+        return t.cpu().numpy().view(np.uint16)
+
+    def stats(self) -> dict:
+        shed, pend = ctypes.c_uint64(), ctypes.c_uint32()
+        _lib.call("hv_server_stats", self.h, ctypes.byref(shed), ctypes.byref(pend))
+        return {"shed": shed.value, "pending": pend.value}
 
     def poll(self, max_n: int = 4096):
         srcs = (ctypes.c_int * max_n)()

@@ -359,6 +359,10 @@ typedef struct hv_compute_settings {
      * exclusive with contrast_subsample > 1. */
     int32_t contrast_reticle_enabled;
     float contrast_reticle_scale;
+    /* OutputBufferSize (Backend/includes/core/holovibes.hh:375): u16 images the
+     * output queue holds (gpu_output_queue, Backend/sources/core/icompute.cc:238-246);
+     * 0: no queue, only the latest image per lane. */
+    uint32_t output_buffer_size;
 } hv_compute_settings;
 
 /* Fills the reference defaults (Backend/includes/core/holovibes.hh:359-447). */
@@ -389,6 +393,29 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced);
  * (before the u16 cast), and the float image before contrast. */
 hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out_f32_contrast,
                          const float** out_f32_image);
+
+/* The output queue (gpu_output_queue, OutputBufferSize frames): every
+ * produced u16 image is written by the cycle's last kernel straight into the
+ * next slot, (index of the produced image) mod max_size, and enqueued as in
+ * Queue::enqueue (Backend/sources/data_struct/queue.cc:149-189, called from
+ * pipe.cc:491-524): when the queue is full the oldest image is overwritten
+ * and has_overwritten set.  base: slot k at base + k * W * H.  A consumer
+ * must read an image before max_size more images are produced. */
+hv_status hv_pipe_output_queue(hv_pipe* p, const uint16_t** base, uint32_t* max_size, uint32_t* start_index,
+                               uint32_t* size, int* has_overwritten);
+/* Queue::dequeue(dest, stream, kind, nb_elts) (queue.cc:385-407): copies the
+ * nb oldest images (memcpy_kind 2 = device to host, 3 = device to device) to
+ * dst on `stream` once they are complete, waits for the copy, and removes
+ * them (a range that wraps around the end is copied in two parts).
+ * HV_ERR_NOT_READY when fewer than nb images are queued. */
+hv_status hv_pipe_output_dequeue(hv_pipe* p, void* dst, int memcpy_kind, uint32_t nb, void* stream);
+/* No reference counterpart: the u16 image of the index-th image the pipe
+ * produced (0-based since creation), while the queue still holds it
+ * (HV_ERR_NOT_READY once overwritten or not yet produced).  `stream` (may be
+ * NULL) waits for the image to be complete. */
+hv_status hv_pipe_output_frame(hv_pipe* p, uint64_t index, void* stream, const uint16_t** frame);
+/* Images produced since creation. */
+hv_status hv_pipe_output_count(hv_pipe* p, uint64_t* count);
 
 /* Current contrast range (device pointer to 2 floats). */
 hv_status hv_pipe_contrast_range(hv_pipe* p, const float** range_dev);
@@ -756,21 +783,60 @@ hv_status hv_server_add_source(hv_server* s, const hv_compute_settings* settings
 /* Releases a source and its memory. */
 hv_status hv_server_remove_source(hv_server* s, int source_id);
 
-/* Submits one batch (device memory, batch_size frames, already landed in HBM)
- * that became available at host time t_arrival_ns (steady clock); the pipe
- * work is queued on the source's stream and overlaps with other sources'.
- * Returns immediately; `frames` must stay valid until the batch completes. */
+/* No reference counterpart (continuous batching).  Every source admits at
+ * most `slots` requests to the GPU at once (their work overlaps other
+ * sources' on the device); later requests wait in a host queue of
+ * `pending` entries, admitted in arrival order as slots free (at submit and
+ * poll); a request arriving at a full queue sheds the oldest waiting one --
+ * newest data wins, as the reference's input queue overwrites its oldest
+ * batch when the consumer lags (Backend/sources/data_struct/batch_input_queue.cc:173-178).
+ * Defaults: 2 slots, 16 pending.  Applies to sources added later. */
+hv_status hv_server_configure(hv_server* s, int slots, int pending);
+
+/* Submits one request: a batch (device memory, batch_size frames, already in
+ * HBM) that became available at host time t_arrival_ns (steady clock).
+ * Returns immediately with its id; `frames` must stay valid until the
+ * request completes or is shed. */
+hv_status hv_server_submit_request(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns,
+                                   int64_t* request_id);
+/* hv_server_submit_request without the id. */
 hv_status hv_server_submit(hv_server* s, int source_id, const void* frames, int64_t t_arrival_ns);
 
-/* Completes finished batches; writes up to max_n (source, latency_ns) pairs.
- * Latency = completion of the batch's u16 image - t_arrival. */
+#define HV_REQ_DONE 0       /* completed; output_index names its image */
+#define HV_REQ_NO_OUTPUT 1  /* completed without an image (window filling, stride discard) */
+#define HV_REQ_SHED 2       /* dropped from a full pending queue, never run */
+
+typedef struct hv_request_result {
+    int64_t request_id;
+    int32_t source;
+    int32_t status;        /* HV_REQ_* */
+    int64_t latency_ns;    /* completion seen - t_arrival (0 when shed) */
+    int64_t output_index;  /* HV_REQ_DONE: index for hv_server_output_frame; -1 otherwise */
+} hv_request_result;
+
+/* Reports finished (and shed) requests, up to max_n, and admits waiting
+ * requests into the slots that freed.  Latency = the time the completion
+ * of the request's u16 image is seen here - t_arrival. */
+hv_status hv_server_poll_requests(hv_server* s, hv_request_result* out, int max_n, int* n_out);
+
+/* hv_server_poll_requests reduced to (source, latency_ns) of the completed
+ * requests that produced an image. */
 hv_status hv_server_poll(hv_server* s, int* sources, int64_t* latency_ns, int max_n, int* n_out);
 
-/* Blocks until every submitted batch has completed. */
+/* Blocks until every submitted request has completed or been shed (waiting
+ * requests are admitted as slots free); results stay for the next poll. */
 hv_status hv_server_drain(hv_server* s);
 
 /* Latest output of a source (device pointers, see hv_pipe_output). */
 hv_status hv_server_output(hv_server* s, int source_id, const uint16_t** out_u16, const float** out_f32_image);
+
+/* The u16 image of a request (its output_index), from the source's output
+ * queue (hv_pipe_output_frame); HV_ERR_NOT_READY once overwritten. */
+hv_status hv_server_output_frame(hv_server* s, int source_id, int64_t output_index, void* stream,
+                                 const uint16_t** frame);
+
+/* Requests shed so far and requests waiting now, over all sources. */
+hv_status hv_server_stats(hv_server* s, uint64_t* shed, uint32_t* pending);
 
 /* Bytes currently admitted. */
 hv_status hv_server_memory_bytes(hv_server* s, size_t* bytes);

@@ -49,7 +49,8 @@ def test_settings_struct_layout(lib):
     # RenormConstant 5, ContrastReticleDisplayEnabled{false}, ContrastReticleScale{0.5}
     assert (s.renorm_enabled, s.renorm_constant, s.img_accu_level, s.log_enabled) == (1, 5, 1, 0)
     assert (s.contrast_reticle_enabled, s.contrast_reticle_scale) == (0, 0.5)
-    assert ctypes.sizeof(ComputeSettings) == 40 * 4
+    assert s.output_buffer_size == 1024  # OutputBufferSize{1024} (holovibes.hh:375)
+    assert ctypes.sizeof(ComputeSettings) == 41 * 4
 
 
 def test_validate_settings(lib):
@@ -87,6 +88,10 @@ def test_memory_accounting(lib):
     assert lib.hv_pipe_memory_bytes(ctypes.byref(s), ctypes.byref(n)) == 0
     P = 512 * 512
     ring = 32 * P * 8
+    outq = 1024 * P * 2  # OutputBufferSize u16 images
+    assert ring + outq < n.value < ring + outq + 32 * P * 2 + 16 * P + 2 ** 20
+    s.output_buffer_size = 0
+    assert lib.hv_pipe_memory_bytes(ctypes.byref(s), ctypes.byref(n)) == 0
     assert ring < n.value < ring + 32 * P * 2 + 16 * P + 2 ** 20
 
 
