@@ -42,7 +42,13 @@ def parse():
     ap.add_argument("--workload", default="metric")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="override a compute setting of the workload (e.g. pca_precision=0)")
-    ap.add_argument("--latency-batches", type=int, default=200)
+    ap.add_argument("--latency-batches", type=int, default=10000,
+                    help="batches of the latency-under-load measurement")
+    ap.add_argument("--load", type=float, action="append", default=None,
+                    help="offered load(s) of the latency measurement, fractions of the measured maximum "
+                         "(default 0.5, 0.65, 0.8; the top-level p50 / p99 are the last one's)")
+    ap.add_argument("--no-fp32-ratio", action="store_true",
+                    help="skip the plain-FP32-path throughput measurements")
     ap.add_argument("--graph", action="store_true",
                     help="replay each cycle from a hipGraph capture (eager launches are faster here)")
     ap.add_argument("--no-graph", action="store_true", help="(default) eager launches")
@@ -132,13 +138,28 @@ def image_error(s, batch, gpu_image):
     return float(np.linalg.norm((gpu_image.astype(np.float64) - r).ravel()) / np.linalg.norm(r.ravel()))
 
 
-def cpu_baseline(name, settings, batches, seconds):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(name, settings, batches, seconds, cores=None):
     from oracle import oracle as o
     o.build()
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
+    cores = cores or host_cores()
     o.set_threads(cores)
     pipe = o.OraclePipe(settings, 32)
     B, S, T = settings.batch_size, settings.time_stride, settings.time_transformation_size
@@ -157,7 +178,58 @@ def cpu_baseline(name, settings, batches, seconds):
             break
     return {"value": frames / el, "unit": "frames/s", "cores": cores, "kind": "port",
             "sample": f"{n} batches of {B} frames ({name}) through the FP32 C++ oracle pipe, "
-                      f"{el:.1f} s on {cores} host threads"}
+                      f"{el:.1f} s on {cores} host threads", "cpu_model": cpu_model()}
+
+
+def throughput(s, dev_batches, level, lanes, steps, warmup):
+    """Frames/s of a pipe at cycle path `level` with `lanes` lanes (eager
+    launches) over `steps` timed pushes after the window fill and warm-up."""
+    import torch
+
+    from holovibes_amd.pipe import Pipe
+    p = Pipe(s, fused=level, lanes=lanes)
+    n = len(dev_batches)
+    k = 0
+    while not p.push(dev_batches[k % n]):
+        k += 1
+    for _ in range(warmup):
+        p.push(dev_batches[k % n])
+        k += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.push(dev_batches[k % n])
+        k += 1
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    path = p.path
+    p.close()
+    return steps * s.batch_size / el, path
+
+
+def fp32_ratios(args, metric_value, metric_settings, metric_batches):
+    """This build's configuration against the plain FP32 path on the same GPU
+    in the same process: the reference's step order (three passes + time
+    transform, no fusion), one lane, FP32 covariance -- for the metric
+    workload and for c2 (Fresnel + PCA)."""
+    import numpy as np
+    import torch
+
+    from holovibes_amd import configs
+    from holovibes_amd.pipe import synth_frames
+    out = {}
+    plain, _ = throughput(metric_settings, metric_batches, 0, 1, args.steps, args.warmup)
+    out["metric"] = {"value": metric_value / plain, "frames_per_s": metric_value, "plain_fp32_frames_per_s": plain}
+    s2 = configs.settings_for("c2")
+    sp = configs.synth_for("c2")
+    host = [synth_frames(sp, k * s2.batch_size, s2.batch_size) for k in range(4)]
+    dev = [torch.from_numpy(h).cuda() for h in host for _ in range(4)]
+    fast, _ = throughput(s2, dev, 2, args.lanes, args.steps, args.warmup)
+    plain2, _ = throughput(configs.settings_for("c2", pca_precision=0), dev, 0, 1, args.steps, args.warmup)
+    out["c2"] = {"value": fast / plain2, "frames_per_s": fast, "plain_fp32_frames_per_s": plain2,
+                 "config": f"split BF16 covariance, {args.lanes} lanes vs FP32 covariance, 1 lane"}
+    out["plain_fp32_path"] = "reference step order (path 0), one lane, eager launches, FP32 covariance"
+    return out
 
 
 def serving(args):
@@ -606,20 +678,36 @@ def main():
         print(json.dumps(line))
         return
 
-    # batch -> image latency, one batch at a time (unloaded)
-    lat = []
-    for _ in range(args.latency_batches):
+    # batch -> image latency under load: batches arrive at a fixed period, a
+    # fraction of the measured maximum rate; completions polled natively
+    loads = args.load or [0.5, 0.65, 0.8]
+    line["latency"] = {"batches_per_load": args.latency_batches, "loads": []}
+    for load in loads:
+        period_ms = ms_per_step / load
+        lat = sorted(x / 1e6 for x in pipe.measure_latency(dev_batches, args.latency_batches, period_ms * 1e6))
+        if not lat:
+            continue
+        p50, p99 = statistics.median(lat), lat[min(len(lat) - 1, int(0.99 * len(lat)))]
+        line["latency"]["loads"].append({"offered_load": load, "frames_per_s": B / (period_ms * 1e-3),
+                                         "batch_period_ms": period_ms, "p50_ms": p50, "p99_ms": p99,
+                                         "max_ms": lat[-1], "images": len(lat), "p99_below_period": p99 < period_ms})
+        line["p50_latency_ms"], line["p99_latency_ms"] = p50, p99
+    ok = [x for x in line["latency"]["loads"] if x["p99_below_period"]]
+    line["latency"]["highest_load_with_p99_below_period"] = max((x["offered_load"] for x in ok), default=None)
+    line["batch_period_ms_at_max_rate"] = ms_per_step
+    # unloaded: one batch at a time
+    lat0 = []
+    for _ in range(200):
         t1 = time.perf_counter_ns()
         made = pipe.push(dev_batches[k % nbatches])
         k += 1
         pipe.sync()
         if made:
-            lat.append((time.perf_counter_ns() - t1) / 1e6)
-    lat.sort()
-    if lat:
-        line["p50_latency_ms"] = statistics.median(lat)
-        line["p99_latency_ms"] = lat[min(len(lat) - 1, int(0.99 * len(lat)))]
-        line["batch_period_ms_at_max_rate"] = ms_per_step
+            lat0.append((time.perf_counter_ns() - t1) / 1e6)
+    lat0.sort()
+    if lat0:
+        line["latency"]["unloaded_p50_ms"] = statistics.median(lat0)
+        line["latency"]["unloaded_p99_ms"] = lat0[min(len(lat0) - 1, int(0.99 * len(lat0)))]
     # image error against the FP64 oracle on one batch through a fresh pipe
     check = Pipe(s, fused=pipe_level)
     made = False
@@ -631,8 +719,12 @@ def main():
             line["err_vs_fp64"] = {"image_rel_l2": err, "oracle": "C++ FP64 restatement of the reference pipe"}
     check.close()
     pipe.close()
+    if not args.no_fp32_ratio and name == "metric" and not overrides:
+        # vs_baseline stays null (no published figure); the same-GPU plain path is the denominator here
+        line["vs_plain_fp32_path"] = fp32_ratios(args, frames_per_s, s, dev_batches)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(name, s, host_batches, args.cpu_seconds)
+        line["cpu_baseline_1thread"] = cpu_baseline(name, s, host_batches, args.cpu_seconds / 2, cores=1)
     print(json.dumps(line))
 
 

@@ -200,6 +200,7 @@ SIGNATURES = {
     "hv_pipe_output_dequeue": [P, P, I, U32, P],
     "hv_pipe_output_frame": [P, U64, P, P],
     "hv_pipe_output_count": [P, P],
+    "hv_pipe_measure_latency": [P, P, I, I, ctypes.c_double, P, P],
     "hv_unpack_12_to_16bit": [P, SZ, P, SZ, P],
     "hv_unpack_10_to_16bit": [P, SZ, P, SZ, P],
     "hv_input_queue_create": [U32, U32, U32, U32, U32, P],

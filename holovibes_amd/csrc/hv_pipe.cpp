@@ -1117,6 +1117,68 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
     return HV_OK;
 }
 
+hv_status hv_pipe_measure_latency(hv_pipe* p, const void* const* frames, int n_frames, int n_batches,
+                                  double period_ns, int64_t* latency_ns, int* n_out) {
+    if (!p || !frames || n_frames < 1 || n_batches < 1 || !latency_ns || !n_out || !(period_ns >= 0.0))
+        return fail(HV_ERR_INVALID_ARG, "measure_latency: bad argument");
+    using clock = std::chrono::steady_clock;
+    const auto ns = [](clock::time_point t) {
+        return std::chrono::duration_cast<std::chrono::nanoseconds>(t.time_since_epoch()).count();
+    };
+    constexpr int kEvents = 64;
+    hipEvent_t ev[kEvents] = {};
+    for (auto& e : ev) HV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    struct Wait {
+        int ev;
+        int64_t t_arrival;
+    };
+    std::deque<Wait> waits;
+    std::vector<int> free_ev;
+    for (int i = kEvents - 1; i >= 0; --i) free_ev.push_back(i);
+    int n = 0;
+    hv_status st = HV_OK;
+    // every outstanding image (lanes may finish out of order); `block` waits
+    // for the oldest first
+    auto poll = [&](bool block) -> hv_status {
+        if (block && !waits.empty()) HV_HIP(hipEventSynchronize(ev[waits.front().ev]));
+        for (auto it = waits.begin(); it != waits.end();) {
+            const hipError_t q = hipEventQuery(ev[it->ev]);
+            if (q == hipErrorNotReady) {
+                ++it;
+                continue;
+            }
+            if (q != hipSuccess) return hip_fail(q, "latency event");
+            latency_ns[n++] = ns(clock::now()) - it->t_arrival;
+            free_ev.push_back(it->ev);
+            it = waits.erase(it);
+        }
+        return HV_OK;
+    };
+    const auto t0 = clock::now();
+    for (int i = 0; i < n_batches && st == HV_OK; ++i) {
+        const auto target = t0 + std::chrono::nanoseconds((int64_t)(period_ns * i));
+        while (clock::now() < target && st == HV_OK) st = poll(false);
+        if (st != HV_OK) break;
+        if (free_ev.empty()) st = poll(true);  // more than kEvents images outstanding
+        if (st != HV_OK) break;
+        int produced = 0;
+        st = hv_pipe_push_batch(p, frames[i % n_frames], &produced);
+        if (st != HV_OK || !produced) continue;
+        const int e = free_ev.back();
+        free_ev.pop_back();
+        const hipError_t r = hipEventRecord(ev[e], p->lane[p->last].stream);
+        if (r != hipSuccess) {
+            st = hip_fail(r, "hipEventRecord(latency)");
+            break;
+        }
+        waits.push_back({e, ns(target)});
+    }
+    while (st == HV_OK && !waits.empty()) st = poll(true);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *n_out = n;
+    return st;
+}
+
 hv_status hv_pipe_path(const hv_pipe* p, int* path) {
     if (!p || !path) return fail(HV_ERR_INVALID_ARG, "null argument");
     *path = p->path;

@@ -107,6 +107,16 @@ class Pipe:
         _lib.call("hv_pipe_time_stage", self.h, stage, arr, len(ptrs), iters, ctypes.byref(ms))
         return ms.value
 
+    def measure_latency(self, frames, n_batches: int, period_ns: float):
+        """Batch -> image latencies (ns) of n_batches pushed at a fixed period
+        (hv_pipe_measure_latency), cycling over `frames`."""
+        ptrs = [f.data_ptr() if isinstance(f, torch.Tensor) else int(f) for f in frames]
+        arr = (ctypes.c_void_p * len(ptrs))(*ptrs)
+        lat = (ctypes.c_int64 * n_batches)()
+        n = ctypes.c_int()
+        _lib.call("hv_pipe_measure_latency", self.h, arr, len(ptrs), n_batches, float(period_ns), lat, ctypes.byref(n))
+        return list(lat[:n.value])
+
     def sync(self) -> None:
         _lib.call("hv_pipe_sync", self.h)
 

@@ -481,6 +481,15 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
  * events.  The pipe's buffers are overwritten; the next cycle rewrites them. */
 hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, int n_frames, int iters, float* ms);
 
+/* No reference counterpart (measurement).  Batch -> image latency under a
+ * fixed offered load: batch i (frames[i mod n_frames]) arrives at host time
+ * t0 + i * period_ns and is pushed then; the completion of each produced
+ * image is polled on the host (an event after its cycle) while waiting for
+ * the next arrival.  Writes one latency (ns, completion seen - arrival) per
+ * produced image into latency_ns (room for n_batches) and their count. */
+hv_status hv_pipe_measure_latency(hv_pipe* p, const void* const* frames, int n_frames, int n_batches,
+                                  double period_ns, int64_t* latency_ns, int* n_out);
+
 /* The time-transformation queue: ring base (T frames, complex), its
  * start index and size (Queue::get_start_index / get_size, queue.hh). */
 hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size);
