@@ -94,14 +94,28 @@ def stage_bytes(s, path):
     space = s.space_transformation
     contrast = ("contrast_u16", P * 4 + P * 2 + P * 4)  # image in, u16 out, contrasted float out
     if path == 2:
+        W = s.width
+        moments = s.img_type in (5, 6, 7)
         pmin = s.p_start
-        pmax = min(pmin + s.p_width, T) if s.p_width else pmin
-        K = pmax - pmin + 1
+        if moments:
+            K = min(pmin + s.p_width, T - 1) - pmin + 1
+        else:
+            K = (min(pmin + s.p_width, T) if s.p_width else pmin) - pmin + 1
+        # the fused kernels hold an image row's K plane rows in LDS (W <= 1024, T <= 64);
+        # otherwise the time DFT, the row FFTs and the inverse row FFTs are passes of their own
+        fused_rows = W in (64, 128, 256, 512, 1024) and T <= 64 and not (W == 1024 and T > 32) and \
+            (K + 1) * (W + W // 8) * 8 <= 160 * 1024
+        fused_image = space == 2 and 64 <= W <= 1024 and not moments
         rows = T * P * d + K * P * 8 + (P * 8 if space == 1 else 0)  # raw window in, K planes out (+ chirp)
+        if not fused_rows:
+            rows += K * P * 16  # the planes' forward row FFT
         cols = K * P * 16 + (P * 8 if space == 2 else 0)
         image = K * P * 8 + P * 4
-        return [("time_dft_row_fft", rows), ("column_fft_lens_ifft" if space == 2 else "column_fft", cols),
-                ("empty", 0), ("row_ifft_image" if space == 2 else "bins_image", image), contrast]
+        if space == 2 and not fused_image:
+            image += K * P * 16  # the planes' inverse row FFT
+        split = "" if fused_rows and (fused_image or space == 1) else "_split"
+        return [("time_dft_row_fft" + split, rows), ("column_fft_lens_ifft" if space == 2 else "column_fft", cols),
+                ("empty", 0), ("row_ifft_image" + split if space == 2 else "bins_image", image), contrast]
     if space == 2:  # angular spectrum: row FFT (+convert), column FFT x lens x IFFT, inverse row FFT
         st0 = nb * P * (d + 8)
         st1 = nb * P * 16 + P * 8

@@ -273,6 +273,143 @@ __global__ void __launch_bounds__(256)
     if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
+// ---------------------------------------------------------------------------
+// Split path (frames wider than 1024, windows longer than 64, moments): the
+// bins-first steps as separate passes over the K planes.
+// ---------------------------------------------------------------------------
+// Time DFT: CW pixels per workgroup, NT threads per pixel, the length-T
+// transform of each pixel's window through LDS (the window read in physical
+// ring-slot order, slot n holding frame (n - slot0) mod T of the batch);
+// only bins [pmin, pmin + K) are written, x chirp for Fresnel.
+template <int T>
+struct BinTimeCfg {
+    static constexpr int E = T >= 16 ? 16 : T;
+    static constexpr int NT = FftShape<T, E>::NT;
+    static constexpr int CW = 64;
+    static constexpr int THREADS = NT * CW;
+};
+
+template <int T, typename TIn>
+__global__ void __launch_bounds__(BinTimeCfg<T>::THREADS)
+    k_bins_time(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp, size_t P,
+                int slot0, int pmin, int K, uint32_t* mm_reset, int mm_slots, int vec) {
+    using C = BinTimeCfg<T>;
+    __shared__ float2 lds[T * C::CW];
+    // the cycle's first kernel re-arms the image min/max accumulators of its last kernels
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
+    const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
+    const size_t p0 = (size_t)blockIdx.x * C::CW, p = p0 + c;
+    const bool valid = p < P;  // P is a multiple of CW: every pixel of the tile exists
+    auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
+    // the tile (T frames of CW pixels) through LDS with 16-byte loads: the
+    // per-pixel loads of the transform would move CW * sizeof(TIn) bytes per
+    // wave instruction
+    constexpr int PX = 16 / sizeof(TIn);  // pixels per 16-byte load
+    constexpr int CHUNKS = T * C::CW / PX;
+    static_assert(C::CW % PX == 0, "a tile row holds whole 16-byte loads");
+    for (int idx = threadIdx.x; idx < CHUNKS; idx += C::THREADS) {
+        const int n = idx / (C::CW / PX), q = idx % (C::CW / PX);
+        const int j = (n - slot0) & (T - 1);  // physical slot n holds frame (n - slot0) mod T
+        const TIn* src = raw + (size_t)j * P + p0 + (size_t)q * PX;
+        float2* dst = lds + n * C::CW + q * PX;
+        if (vec) {
+            const uint4 w = *reinterpret_cast<const uint4*>(src);
+            const TIn* px = reinterpret_cast<const TIn*>(&w);
+#pragma unroll
+            for (int e = 0; e < PX; ++e) dst[e] = make_float2(convert_px<TIn>(px[e]), 0.f);
+        } else {  // a caller's buffer that is not 16-byte aligned
+#pragma unroll
+            for (int e = 0; e < PX; ++e) dst[e] = make_float2(convert_px<TIn>(src[e]), 0.f);
+        }
+    }
+    __syncthreads();
+    auto load = [&](int n) -> float2 { return lds_at(n); };
+    const float2 cf = (chirp && valid) ? chirp[p] : make_float2(1.f, 0.f);
+    auto store = [&](int n, float2 z) {
+        const int k = n - pmin;
+        if (valid && k >= 0 && k < K) planes[(size_t)k * P + p] = chirp ? cmul(z, cf) : z;
+    };
+    fft_run<T, C::E, -1>(t, load, store, lds_at);
+}
+
+template <int T, typename TIn>
+hv_status time_launch_t(const void* raw, float2* planes, const float2* chirp, size_t P, int slot0, int pmin, int K,
+                        uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    using C = BinTimeCfg<T>;
+    if (P % C::CW) return fail(HV_ERR_UNSUPPORTED, "bins-first time DFT: frames of a multiple of 64 pixels");
+    const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
+    const unsigned g = (unsigned)(P / C::CW);
+    k_bins_time<T, TIn><<<g, C::THREADS, 0, st>>>((const TIn*)raw, planes, chirp, P, slot0, pmin, K, mm_reset,
+                                                  mm_slots, vec);
+    HV_CHECK_LAUNCH("k_bins_time");
+    return HV_OK;
+}
+
+template <int T>
+hv_status time_launch_d(const void* raw, int depth, float2* planes, const float2* chirp, size_t P, int slot0,
+                        int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    switch (depth) {
+        case HV_DEPTH_8: return time_launch_t<T, uint8_t>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case HV_DEPTH_16: return time_launch_t<T, uint16_t>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        default: return time_launch_t<T, float>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+    }
+}
+
+hv_status bins_time(const void* raw, int depth, float2* planes, const float2* chirp, size_t P, int T, int slot0,
+                    int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    switch (T) {
+        case 2: return time_launch_d<2>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 4: return time_launch_d<4>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 8: return time_launch_d<8>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 16: return time_launch_d<16>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 32: return time_launch_d<32>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 64: return time_launch_d<64>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 128: return time_launch_d<128>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 256: return time_launch_d<256>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: T must be a power of two in [2, 256]");
+    }
+}
+
+// Moments of the propagated bins (tools_conversion.cu:33-66, tools_compute.cu:38-97):
+// plane k holds bin i = fs + k; f1 = fftfreq(T) * fps in float as the
+// reference computes it; the Nyquist bin counted again for M0 and M2 and
+// cancelled for M1.  `img` receives moment `sel`, whose min/max feeds mm.
+__global__ void __launch_bounds__(256)
+    k_bins_moments(const float2* __restrict__ planes, float* m0, float* m1, float* m2, float* img, size_t P, int T,
+                   int fs, int K, float fps, int sel, uint32_t* mm, int mm_slots) {
+    const float d = fps / T;
+    const int nyq = T / 2;
+    float vmin = INFINITY, vmax = -INFINITY;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int k = 0; k < K; ++k) {
+            const int i = fs + k;
+            const float2 z = planes[(size_t)k * P + p];
+            const float sq = z.x * z.x + z.y * z.y;
+            const float f = i < nyq ? i * d : -((float)T - i) * d;
+            a0 += sq * 1.0f;
+            a1 += sq * f;
+            a2 += sq * (f * f);
+        }
+        if (nyq >= fs && nyq < fs + K) {
+            const float2 z = planes[(size_t)(nyq - fs) * P + p];
+            const float sq = z.x * z.x + z.y * z.y;
+            const float f = -((float)T - nyq) * d;
+            a0 += sq * 1.0f;
+            a1 -= sq * f;
+            a2 += sq * (f * f);
+        }
+        if (m0) m0[p] = a0;
+        if (m1) m1[p] = a1;
+        if (m2) m2[p] = a2;
+        const float v = sel == 0 ? a0 : sel == 1 ? a1 : a2;
+        if (img) img[p] = v;
+        vmin = fminf(vmin, v);
+        vmax = fmaxf(vmax, v);
+    }
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
+}
+
 template <int W, int T, typename TIn>
 hv_status rows_launch(const void* raw, float2* planes, const float2* chirp, int height, int slot0, int pmin, int K,
                       uint32_t* mm_reset, int mm_slots, hipStream_t st) {
@@ -333,14 +470,25 @@ bool bins_first_supported(int width, int height, int T, int K, int depth) {
     return rows_lds_for(width, K) <= 160 * 1024;
 }
 
+bool bins_split_supported(int width, int height, int T, int K, int depth) {
+    if (!fft_len_ok(width) || !fft_len_ok(height)) return false;
+    if (!is_pow2(T) || T < 2 || T > 256 || K < 1 || K > T) return false;
+    return depth == HV_DEPTH_8 || depth == HV_DEPTH_16 || depth == HV_DEPTH_32F;
+}
+
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
                            int mm_slots) {
-    if (!bins_first_supported(width, height, T, K, depth))
-        return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
     if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
         return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
     const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
+    if (!bins_first_supported(width, height, T, K, depth)) {
+        if (!bins_split_supported(width, height, T, K, depth))
+            return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
+        // split: the window's bins, then their forward row FFTs
+        HV_TRY(bins_time(raw, depth, planes, chirp, (size_t)width * height, T, slot0, pmin, K, mm_reset, mm_slots, st));
+        return launch_row_pass(planes, width, height, K, -1, st);
+    }
     switch (width) {
         case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
         case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
@@ -365,10 +513,18 @@ hv_status launch_bins_propagate(const void* raw, int depth, int width, int heigh
     return launch_bins_columns(planes, width, height, K, space, lens, st);
 }
 
+// The inverse row FFT and the image term in one kernel (image rows of
+// W <= 1024 within LDS).
+bool bins_image_fused(int width, int space) {
+    return space == HV_SPACE_ANGULARSP && width >= 64 && width <= 1024;
+}
+
 template <int IMG>
-hv_status bins_image_t(float* out, const float2* planes, int width, int height, int K, int space, float count,
+hv_status bins_image_t(float* out, float2* planes, int width, int height, int K, int space, float count,
                        uint32_t* mm, int mm_slots, hipStream_t st) {
-    if (space == HV_SPACE_FRESNELTR) {
+    if (!bins_image_fused(width, space)) {
+        // split: the planes' inverse row FFT (angular spectrum), then the image term
+        if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
         const size_t P = (size_t)width * height;
         const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
         k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm, mm_slots);
@@ -389,8 +545,21 @@ hv_status bins_image_t(float* out, const float2* planes, int width, int height, 
     return HV_OK;
 }
 
-hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
+hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* planes_c, int width, int height, int T,
+                              int fs, int K, int space, float fps, int mm_moment, uint32_t* mm, hipStream_t st,
+                              int mm_slots) {
+    float2* planes = const_cast<float2*>(planes_c);
+    if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
+    const size_t P = (size_t)width * height;
+    const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
+    k_bins_moments<<<blocks, 256, 0, st>>>(planes, m0, m1, m2, nullptr, P, T, fs, K, fps, mm_moment, mm, mm_slots);
+    HV_CHECK_LAUNCH("k_bins_moments");
+    return HV_OK;
+}
+
+hv_status launch_bins_image(float* out, const float2* planes_c, int width, int height, int K, int space, int img_type,
                             int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
+    float2* planes = const_cast<float2*>(planes_c);
     const float count = (float)(pmax - pmin + 1);
     switch (img_type) {
         case HV_IMG_MODULUS: return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, st);

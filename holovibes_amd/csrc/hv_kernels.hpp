@@ -24,6 +24,8 @@ hv_status launch_fresnel(float2* buf, int batch, const float2* chirp, int width,
 enum { COLPASS_FWD = 0, COLPASS_INV = 1, COLPASS_LENS = 2 };
 hv_status launch_col_pass(float2* buf, const float2* lens, int width, int height, int planes, int mode,
                           hipStream_t st);
+// One row pass over `planes` frames of `buf` (in place, unnormalised): direction -1 forward, +1 inverse.
+hv_status launch_row_pass(float2* buf, int width, int height, int planes, int direction, hipStream_t st);
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
                                    hipEvent_t* marks = nullptr, bool spectral_rows = false,
@@ -47,7 +49,17 @@ hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* rin
 
 // hv_bins.hip: B = S = T STFT images computed bins-first (the STFT bins
 // [pmin, pmin + K) of the raw window are propagated instead of its T frames).
+// bins_first_supported: the fused row kernels fit (W <= 1024, T <= 64, the
+// K plane rows of an image row in LDS); bins_split_supported: the path as
+// separate passes (time DFT, row, column, row, image), any W, H, T.
 bool bins_first_supported(int width, int height, int T, int K, int depth);
+bool bins_split_supported(int width, int height, int T, int K, int depth);
+// Moments M0, M1, M2 (any may be null; `out` receives moment `moment` too, for
+// the image) of the propagated planes holding bins [fs, fs + K) of a length-T
+// window (inverse row FFT first for the angular spectrum).
+hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* planes, int width, int height, int T,
+                              int fs, int K, int space, float fps, int mm_moment, uint32_t* mm, hipStream_t st,
+                              int mm_slots = 1);
 // Bins + row FFTs (x chirp for Fresnel) into planes[0 .. K) (K x P complex),
 // then the column pass over them; launch_bins_propagate runs both, with
 // `mark` (optional) recorded in between.

@@ -751,14 +751,19 @@ __device__ __forceinline__ Rot make_rot(double app, double aqq, double2 apq) {
         R.s = 0.0;
         R.ec = make_double2(1.0, 0.0);
     } else {
-        // t = sgn(tau) / (|tau| + sqrt(1 + tau^2)), c = 1 / sqrt(1 + t^2), s = t c
-        // (reciprocal square roots: one dependent chain each instead of sqrt + divide)
+        // With h = aqq - app, g = 2 |apq|, r = sqrt(h^2 + g^2), the rotation
+        // t = sgn(h) g / (|h| + r) (tau = h / g), c = 1 / sqrt(1 + t^2), s = t c
+        // is c = (r + |h|) u, s = sgn(h) g u with u = 1 / sqrt(2 r (r + |h|)):
+        // a square root and two reciprocal square roots, no division, and
+        // no cancellation for small angles.
         const double inv = HV_JACOBI_RSQRT ? rsqrt(mag2) : 1.0 / sqrt(mag2);
         R.ec = make_double2(apq.x * inv, -apq.y * inv);
-        const double tau = (aqq - app) * (0.5 * inv);
-        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        R.c = HV_JACOBI_RSQRT ? rsqrt(1.0 + t * t) : 1.0 / sqrt(1.0 + t * t);
-        R.s = t * R.c;
+        const double h = aqq - app, ah = fabs(h);
+        const double g = 2.0 * mag2 * inv;
+        const double r = sqrt(fma(h, h, 4.0 * mag2));
+        const double u = HV_JACOBI_RSQRT ? rsqrt(2.0 * r * (r + ah)) : 1.0 / sqrt(2.0 * r * (r + ah));
+        R.c = (r + ah) * u;
+        R.s = (h >= 0 ? g : -g) * u;
     }
     return R;
 }

@@ -49,15 +49,29 @@ bool fused_eligible(const hv_compute_settings& s) {
            row_ifft_stft_supported((int)s.width, (int)s.time_transformation_size);
 }
 
-// B = S = T STFT image with a space transform: the needed bins can be taken
-// before the propagation (the two are linear and commute).
+// The STFT bins an image reads: [pmin, pmax] for the image types, [f_start,
+// f_end] with f_end = min(start + width, T - 1) for the moments (pipe.cc:429-455).
+void bins_range(const hv_compute_settings& s, int& lo, int& K) {
+    if (is_moment_kind(s.img_type)) {
+        lo = (int)s.p_start;
+        K = std::min<int>((int)(s.p_start + s.p_width), (int)s.time_transformation_size - 1) - lo + 1;
+    } else {
+        int pmax;
+        p_range(s, lo, pmax);
+        K = pmax - lo + 1;
+    }
+}
+
+// B = S = T STFT image or moments with a space transform: the needed bins can
+// be taken before the propagation (the two are linear and commute).
 bool bins_eligible(const hv_compute_settings& s) {
     if (!(s.space_transformation == HV_SPACE_ANGULARSP || s.space_transformation == HV_SPACE_FRESNELTR)) return false;
-    if (s.time_transformation != HV_TIME_STFT || !is_image_kind(s.img_type) || !window_replaced(s)) return false;
-    int pmin, pmax;
-    p_range(s, pmin, pmax);
-    return bins_first_supported((int)s.width, (int)s.height, (int)s.time_transformation_size, pmax - pmin + 1,
-                                (int)s.depth);
+    if (s.time_transformation != HV_TIME_STFT || !window_replaced(s)) return false;
+    if (!is_image_kind(s.img_type) && !is_moment_kind(s.img_type)) return false;
+    int lo, K;
+    bins_range(s, lo, K);
+    const int W = (int)s.width, H = (int)s.height, T = (int)s.time_transformation_size;
+    return bins_first_supported(W, H, T, K, (int)s.depth) || bins_split_supported(W, H, T, K, (int)s.depth);
 }
 
 // Execution path of a steady-state cycle.
@@ -339,6 +353,14 @@ hv_status hv_pipe::image_work() {
             HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream));
             HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kPipeMm));
         }
+    } else if (path == PATH_BINS) {
+        // moments of the propagated bins [f_start, f_end] (insert_moments, pipe.cc:429-455)
+        int lo, K;
+        bins_range(s, lo, K);
+        const int k = it - HV_IMG_MOMENTS_0;
+        HV_TRY(launch_bins_moments(k == 0 ? L.image : nullptr, k == 1 ? L.image : nullptr, k == 2 ? L.image : nullptr,
+                                   L.planes, (int)s.width, (int)s.height, Tn, lo, K, s.space_transformation,
+                                   s.camera_fps, k, mm_img, L.stream, kPipeMm));
     } else {
         // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
         const int fs = (int)s.p_start;
@@ -446,11 +468,11 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     if (shared_ring() && last != cur && have_output) HV_HIP(hipStreamWaitEvent(L.stream, lane[last].tt_done, 0));
     HV_TRY(mark(0));
     if (path == PATH_BINS) {
-        int pmin, pmax;
-        p_range(s, pmin, pmax);
-        HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, pmin,
-                                     pmax - pmin + 1, s.space_transformation, lens, L.planes, L.stream,
-                                     timing ? ev[1] : nullptr, minmax_fused() ? L.mm : nullptr, kPipeMm));
+        int lo, K;
+        bins_range(s, lo, K);
+        HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, lo, K,
+                                     s.space_transformation, lens, L.planes, L.stream, timing ? ev[1] : nullptr,
+                                     minmax_fused() ? L.mm : nullptr, kPipeMm));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
@@ -987,9 +1009,9 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         return fail(HV_ERR_INVALID_ARG, "the output queue must hold at least one image per lane");
     HV_TRY(hv_pipe_sync(p));
     p->drop_graphs();
-    int pmin, pmax;
-    p_range(p->s, pmin, pmax);
-    const size_t K = (size_t)(pmax - pmin + 1);
+    int lo, Ki;
+    bins_range(p->s, lo, Ki);
+    const size_t K = (size_t)Ki;
     const Layout& l = p->lay;
     const int kind = p->path == PATH_BINS ? LANE_BINS : window_replaced(p->s) ? LANE_OWN_RING : LANE_SHARED_RING;
     const bool bins = kind == LANE_BINS, own_ring = kind == LANE_OWN_RING;
@@ -1077,9 +1099,9 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
     HV_TRY(hv_pipe_sync(p));
     Lane& L = p->lane[0];
     const hv_compute_settings& s = p->s;
-    int pmin, pmax;
-    p_range(s, pmin, pmax);
-    const int K = pmax - pmin + 1;
+    int pmin, K;
+    bins_range(s, pmin, K);
+    const int pmax = pmin + K - 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HV_HIP(hipEventCreate(&e0));
     HV_HIP(hipEventCreate(&e1));
@@ -1096,8 +1118,13 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
                                          L.stream);
                 break;
             case 3:
-                st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
-                                       s.img_type, pmin, pmax, L.mm, L.stream, kPipeMm);
+                if (is_moment_kind(s.img_type))
+                    st = launch_bins_moments(L.image, nullptr, nullptr, L.planes, (int)s.width, (int)s.height, p->T,
+                                             pmin, K, s.space_transformation, s.camera_fps, 0, L.mm, L.stream,
+                                             kPipeMm);
+                else
+                    st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
+                                           s.img_type, pmin, pmax, L.mm, L.stream, kPipeMm);
                 break;
             default:
                 st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, kPipeMm, s.contrast_invert, L.stream,

@@ -479,6 +479,12 @@ hv_status launch_col_pass(float2* buf, const float2* lens, int width, int height
     }
 }
 
+hv_status launch_row_pass(float2* buf, int width, int height, int planes, int direction, hipStream_t st) {
+    const FrameMap fm{0, 0};
+    if (direction < 0) return row_pass<-1>(buf, IN_C64, buf, nullptr, width, height, planes, fm, fm, 1.0f, st);
+    return row_pass<+1>(buf, IN_C64, buf, nullptr, width, height, planes, fm, fm, 1.0f, st);
+}
+
 hv_status launch_fft2(float2* buf, int batch, int width, int height, int direction, hipStream_t st) {
     FrameMap fm{0, 0};
     if (direction < 0) {

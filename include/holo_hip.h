@@ -446,11 +446,15 @@ hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
  *              inverse row FFT pass is fused with the STFT and image
  *              epilogue; the ring holds each frame after the column pass
  *              instead of the propagated frame;
- *   level 2 -- angular spectrum or Fresnel, T <= 64, W in 64..1024: the
- *              STFT bins [pmin, pmax] of the raw window are taken first and
- *              only those K planes are propagated (the STFT and the
- *              propagation are linear and commute); the ring's first K
- *              frames hold the planes.
+ *   level 2 -- angular spectrum or Fresnel, image types and moments: the
+ *              STFT bins the image reads ([pmin, pmax], or [f_start, f_end]
+ *              for the moments) of the raw window are taken first and only
+ *              those K planes are propagated (the STFT and the propagation
+ *              are linear and commute); the ring's first K frames hold the
+ *              planes.  W <= 1024 and T <= 64 fuse the bins with the row
+ *              FFTs (and, for images, the inverse row FFTs with the image
+ *              term); larger frames and windows run them as passes of their
+ *              own (T up to 256).
  * hv_pipe_set_fused picks the highest eligible path at or below `level`
  * (default: 2); 0 forces the three-pass + STFT sequence.  hv_pipe_path
  * reports the path in use (0, 1 or 2). */

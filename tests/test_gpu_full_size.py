@@ -46,13 +46,34 @@ def test_c3_full_size(oracle):
     check(res, 1e-3)
 
 
-@pytest.mark.parametrize("img_type", [5, 6])
-def test_c5_stft_full_size(oracle, img_type):
+def test_c5_stft_full_size(oracle):
     """c5 STFT variant: 2048^2 u16, B = S = T = 128, angular spectrum,
-    moment 0 and moment 1 over bins 4..60 (camera fps 128)."""
-    print(f"c5_stft 2048x2048 T=128 moment {img_type - 5}", flush=True)
-    s, fl = frames("c5_stft", 1, img_type=img_type)
-    check(run_both(oracle, s, fl), 1e-5)
+    moment 0 and moment 1 over bins 4..60 (camera fps 128), on the
+    bins-first path (57 planes propagated) and on the three-pass path (128
+    frames propagated), against one FP64 oracle run."""
+    from holovibes_amd.constants import HV_IMG_MOMENTS_0
+    from holovibes_amd.pipe import Pipe
+    s, fl = frames("c5_stft", 1)
+    print("c5_stft 2048x2048 T=128: FP64 oracle", flush=True)
+    o = oracle.OraclePipe(s, 64)
+    ref = o.push(fl[0])
+    m_ref = o.moments()
+    o.close()
+    assert rel_l2(ref["image"], m_ref[0]) < 1e-6
+    dev = torch.from_numpy(fl[0].view(np.int16)).cuda()
+    torch.cuda.synchronize()
+    for level in (2, 0):
+        for k in (0, 1):
+            print(f"c5_stft path {level} moment {k}", flush=True)
+            s2 = type(s).from_buffer_copy(s)
+            s2.img_type = HV_IMG_MOMENTS_0 + k
+            p = Pipe(s2, fused=level)
+            assert p.path == level
+            assert p.push(dev)
+            e = rel_l2(p.image().cpu().numpy(), m_ref[k])
+            p.close()
+            print(f"  image rel L2 vs FP64 {e:.3g}", flush=True)
+            assert e < 1e-5, (level, k, e)
 
 
 def test_c5_pca_full_size(oracle):

@@ -178,11 +178,31 @@ def test_pipe_unwrap_rescale_collapse(oracle):
     check(run_both(oracle, s, fl), 1e-3, u16_frac=0.95)
 
 
+@pytest.mark.parametrize("level", [2, 0])
 @pytest.mark.parametrize("img_type", [5, 6, 7])
-def test_pipe_moments(oracle, img_type):
+def test_pipe_moments(oracle, img_type, level):
+    """Moments on the bins-first path (2: fused time DFT + row kernel, then the
+    inverse row pass and the moment epilogue) and the three-pass path (0)."""
     s, fl = frames("c5_stft", 1, width=64, height=64, batch_size=32, time_stride=32, time_transformation_size=32,
                    img_type=img_type, p_start=4, p_width=20, camera_fps=32.0, syn_period=32)
-    check(run_both(oracle, s, fl), 1e-5)
+    check(run_both(oracle, s, fl, fused=level, path=level), 1e-5)
+
+
+@pytest.mark.parametrize("W,H,T,space,img_type,p0,pw", [
+    (64, 64, 128, 2, 5, 4, 56),     # T = 128: the window's DFT in its own kernel, moment 0
+    (64, 32, 256, 2, 6, 100, 60),   # T = 256, moment 1 over a range holding the Nyquist bin
+    (2048, 16, 16, 2, 0, 2, 5),     # W = 2048: split row passes, modulus image
+    (2048, 32, 8, 1, 1, 1, 3),      # Fresnel, W = 2048, squared modulus
+    (128, 64, 128, 1, 5, 0, 127),   # Fresnel, T = 128, every bin (the Nyquist rule at the range edge)
+])
+def test_pipe_bins_split(oracle, W, H, T, space, img_type, p0, pw):
+    """The bins-first path as separate passes (frames wider than 1024 or
+    windows longer than 64): time DFT of the raw window keeping bins
+    [p_start, ...], row FFT, column pass, inverse row FFT, image or moment."""
+    s, fl = frames("c5_stft", 1, width=W, height=H, batch_size=T, time_stride=T, time_transformation_size=T,
+                   space_transformation=space, img_type=img_type, p_start=p0, p_width=pw, camera_fps=float(T),
+                   z_distance=0.35, syn_period=T)
+    check(run_both(oracle, s, fl, path=2), 1e-5)
 
 
 def designed_frames(T, W, H, seed):

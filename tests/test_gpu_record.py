@@ -23,7 +23,9 @@ def _dev(fr):
 
 
 def _moment_images(s, frames, cycles):
-    """Each moment as the pipe's own image type, cycle by cycle."""
+    """Each moment as the pipe's own image type, cycle by cycle, on the
+    three-pass path the record runs (the bins-first moments round
+    differently; both are parity-tested against the oracle)."""
     from holovibes_amd.configs import settings_for  # noqa: F401
     from holovibes_amd.constants import HV_IMG_MOMENTS_0
     from holovibes_amd.pipe import Pipe
@@ -32,7 +34,7 @@ def _moment_images(s, frames, cycles):
     for k in range(3):
         s2 = type(s).from_buffer_copy(s)
         s2.img_type = HV_IMG_MOMENTS_0 + k
-        p = Pipe(s2)
+        p = Pipe(s2, fused=0)
         c = 0
         for b in range(cycles):
             if p.push(_dev(frames[b * B:(b + 1) * B])):
