@@ -74,6 +74,8 @@ def parse():
                     help="frames/s of the workload at one lane with and without recording into a .holo file")
     ap.add_argument("--input-side", action="store_true",
                     help="packed 12/10-bit unpack (HBM roofline) and input-queue enqueue throughput")
+    ap.add_argument("--slots", type=int, default=2, help="serving: requests of one source on the GPU at once")
+    ap.add_argument("--pending", type=int, default=16, help="serving: waiting requests per source before shedding")
     ap.add_argument("--rate-scale", type=float, default=1.0, help="serving: multiply every source's batch rate")
     ap.add_argument("--profile-stages", action="store_true",
                     help="with --profile-steps: after the steps, time the stages and print (for rocprofv3)")
@@ -258,6 +260,7 @@ def serving(args):
 
     torch.cuda.set_device(0)
     srv = Server(configs.SERVING_BUDGET_BYTES)
+    srv.configure(args.slots, args.pending)
     sources = []
     for i, (name, over, rate) in enumerate(configs.SERVING_MIX):
         s = configs.settings_for(name, **over)
@@ -280,7 +283,7 @@ def serving(args):
             srv.submit(src["sid"], src["dev"][src["k"] % 2].data_ptr(), time.perf_counter_ns())
             src["k"] += 1
     srv.drain()
-    srv.poll()
+    srv.poll_requests()
     # Poisson arrivals for every source over the run
     rng = np.random.default_rng(4000)
     events = []
@@ -293,25 +296,30 @@ def serving(args):
             events.append((t, src["sid"]))
     events.sort()
     lat = {src["sid"]: [] for src in sources}
-    frames = {src["sid"]: 0 for src in sources}
+    frames = {src["sid"]: 0 for src in sources}  # frames of requests that completed
+    shed = {src["sid"]: 0 for src in sources}
+
+    def collect():
+        for r in srv.poll_requests():
+            sid_r = r["source"]
+            if r["status"] == 2:
+                shed[sid_r] += 1
+                continue
+            frames[sid_r] += by_sid[sid_r]["B"]
+            if r["status"] == 0:
+                lat[sid_r].append(r["latency_ns"])
+
     t0 = time.perf_counter_ns()
     for t_ev, sid in events:
         target = t0 + int(t_ev * 1e9)
-        while True:
-            now = time.perf_counter_ns()
-            if now >= target:
-                break
-            for s_id, l in srv.poll():
-                lat[s_id].append(l)
+        while time.perf_counter_ns() < target:
+            collect()
         src = by_sid[sid]
         srv.submit(sid, src["dev"][src["k"] % 2].data_ptr(), target)
         src["k"] += 1
-        frames[sid] += src["B"]
-        for s_id, l in srv.poll():
-            lat[s_id].append(l)
+        collect()
     srv.drain()
-    for s_id, l in srv.poll():
-        lat[s_id].append(l)
+    collect()
     el = (time.perf_counter_ns() - t0) / 1e9
     per = []
     all_lat = []
@@ -319,15 +327,20 @@ def serving(args):
         l = sorted(lat[src["sid"]])
         all_lat += l
         per.append({"source": src["name"], "batches_per_s": src["rate"], "batch": src["B"],
-                    "images": len(l), "p50_latency_ms": (l[len(l) // 2] / 1e6) if l else None,
+                    "images": len(l), "shed": shed[src["sid"]],
+                    "p50_latency_ms": (l[len(l) // 2] / 1e6) if l else None,
                     "p99_latency_ms": (l[min(len(l) - 1, int(0.99 * len(l)))] / 1e6) if l else None,
-                    "hbm_bytes": src["bytes"]})
+                    "batch_period_ms": 1e3 / src["rate"], "hbm_bytes": src["bytes"]})
     all_lat.sort()
+    offered = sum(src["B"] * src["rate"] for src in sources)
     line = {"metric": "serving mix: aggregate frames/s + p50/p99 batch->image latency",
             "value": sum(frames.values()) / el, "unit": "frames/s", "n_gpus": 1, "seconds": el,
-            "higher_is_better": True, "data": "synthetic (seeded interferogram generator)",
+            "offered_frames_per_s": offered, "higher_is_better": True,
+            "data": "synthetic (seeded interferogram generator)",
             "p50_latency_ms": all_lat[len(all_lat) // 2] / 1e6 if all_lat else None,
             "p99_latency_ms": all_lat[min(len(all_lat) - 1, int(0.99 * len(all_lat)))] / 1e6 if all_lat else None,
+            "requests_shed": sum(shed.values()), "request_slots_per_source": args.slots,
+            "waiting_places_per_source": args.pending,
             "budget_bytes": configs.SERVING_BUDGET_BYTES, "used_bytes": srv.memory_bytes(),
             "sources": per, "config": {"workload": "serving_mix", "rate_scale": args.rate_scale}}
     srv.close()

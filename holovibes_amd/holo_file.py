@@ -19,6 +19,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ComputeSettings, HoloHeader, HoloInfo
+from .footer_versions import FooterUpgradeError, upgrade_compute_settings, upgrade_footer
 from .constants import (HV_CONTRAST_MANUAL, HV_CONTRAST_REFERENCE, HV_IMG_ARGUMENT, HV_IMG_MODULUS,
                         HV_IMG_MOMENTS_0, HV_IMG_MOMENTS_1, HV_IMG_MOMENTS_2, HV_IMG_SQUARED_MODULUS,
                         HV_SPACE_ANGULARSP, HV_SPACE_FRESNELTR, HV_SPACE_NONE, HV_TIME_NONE, HV_TIME_PCA,
@@ -28,9 +29,6 @@ from .constants import (HV_CONTRAST_MANUAL, HV_CONTRAST_REFERENCE, HV_IMG_ARGUME
 RAW, PROCESSED, MOMENTS = 0, 1, 2
 HEADER_SIZE = 64
 CURRENT_VERSION = 7  # Backend/includes/io/holo_file.hh:98
-# footers of header versions 5, 6 and 7 are identical
-# (Backend/sources/version/holo_file_converter.cc:7-19)
-_SAME_FOOTER_VERSIONS = (5, 6, 7)
 
 # JSON names of the enums (Backend/includes/enum/enum_space_transformation.hh:31-42,
 # enum_time_transformation.hh:25-31, enum_img_type.hh:28-37, enum_computation.hh:23-26);
@@ -172,9 +170,10 @@ class InputHoloFile:
     def import_compute_settings(self) -> dict:
         """The footer's "compute_settings" object, {} without a footer
         (input_holo_file.cc:92-140).  A footer that does not parse is skipped,
-        as in the reference.  Footers of header versions 5 to 7 are read as
-        they are; older ones need the reference's JSON-patch converters,
-        which this reader does not carry, and raise HoloFormatError."""
+        as in the reference.  Footers of header versions 2 to 4 are brought
+        to the current layout first (footer_versions.upgrade_footer); a
+        footer the upgrade cannot convert, or of a version above 7, gives {}
+        as in the reference."""
         if not self.has_footer:
             return {}
         raw = self.footer_bytes()
@@ -184,10 +183,10 @@ class InputHoloFile:
                 self.meta_data = json.loads(raw.decode("utf-8"))
             except (UnicodeDecodeError, json.JSONDecodeError):
                 self.meta_data = {}
-        if self.version > CURRENT_VERSION:
-            return {}  # the converter refuses newer files (holo_file_converter.cc:64-68)
-        if self.version not in _SAME_FOOTER_VERSIONS:
-            raise HoloFormatError(f"footer of header version {self.version} needs the v2-v4 JSON patches")
+        try:  # the footer brought to the version-7 layout (holo_file_converter.cc:54-124)
+            self.meta_data = upgrade_footer(self.meta_data, self.version)
+        except FooterUpgradeError:
+            return {}  # the reference then imports no settings (input_holo_file.cc:135-136)
         cs = self.meta_data.get("compute_settings") if isinstance(self.meta_data, dict) else None
         return cs if isinstance(cs, dict) else {}
 
@@ -410,6 +409,13 @@ def compute_settings_to_json(s: ComputeSettings) -> dict:
             "advanced": advanced}
 
 
+def _enum_name(value, names) -> str:
+    """The JSON name as nlohmann's enum from_json reads it: a name of the
+    table, else the table's first entry."""
+    names = list(names)
+    return value if isinstance(value, str) and value in names else names[0]
+
+
 def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None = None,
                                base: ComputeSettings | None = None) -> tuple[ComputeSettings, list[str]]:
     """The footer's compute settings applied to hv_compute_settings (the
@@ -421,8 +427,13 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     enabled steps the path does not run (band-pass filters, output
     flip/rotation); renormalisation, registration, log10, image accumulation
     and the cuts are mapped.
+    Settings without version "v1" are upgraded first
+    (footer_versions.upgrade_compute_settings; FooterUpgradeError where the
+    reference's converter fails).  Enum names outside the reference's tables
+    read as each table's first entry, as nlohmann does.
     Raises UnsupportedSettings for computations outside the path: raw mode,
     SSA_STFT, phase increase and composite images."""
+    cs = upgrade_compute_settings(cs)  # v0 -> v1 (compute_settings.cc:75-77)
     if base is None:
         s = ComputeSettings()
         _lib.load().hv_default_settings(ctypes.byref(s))
@@ -433,16 +444,16 @@ def compute_settings_from_json(cs: dict, fd: FrameDescriptor, info: dict | None 
     r = cs.get("image_rendering", {})
     v = cs.get("view", {})
     adv = cs.get("advanced", {})
-    mode = r.get("image_mode", "RAW")
+    # a name outside an enum's table reads as the table's first entry (nlohmann's enum from_json)
+    mode = _enum_name(r.get("image_mode", "RAW"), ("RAW", "HOLOGRAM"))
     if mode != "HOLOGRAM":
         raise UnsupportedSettings(f"image_mode {mode}: only HOLOGRAM runs the compute path")
-    space = r.get("space_transformation", "NONE")
-    time = r.get("time_transformation", "NONE")
-    if space not in _SPACE_NAMES:
-        raise UnsupportedSettings(f"space_transformation {space}")
+    space = _enum_name(r.get("space_transformation", "NONE"), _SPACE_NAMES)
+    time = _enum_name(r.get("time_transformation", "NONE"), ("STFT", "PCA", "NONE", "SSA_STFT", "None"))
     if time not in _TIME_NAMES:
         raise UnsupportedSettings(f"time_transformation {time}")
-    img = v.get("image_type", "MODULUS")
+    img = _enum_name(v.get("image_type", "MODULUS"), ("MODULUS", "SQUARED_MODULUS", "ARGUMENT", "PHASE_INCREASE",
+                                                      "COMPOSITE", "MOMENTS_0", "MOMENTS_1", "MOMENTS_2"))
     if img not in _IMG_NAMES:
         raise UnsupportedSettings(f"image_type {img}")
     s.space_transformation = _SPACE_NAMES[space]

@@ -192,9 +192,8 @@ def test_footer_versions_and_parse_failures(holo, tmp_path):
         assert f.data_type == hf.RAW and f.import_compute_settings() == {"version": "v1"}
     with write(8, meta) as f:  # newer than the converter knows
         assert f.import_compute_settings() == {}
-    with write(4, meta) as f:
-        with pytest.raises(holo.HoloFormatError, match="version 4"):
-            f.import_compute_settings()
+    with write(4, meta) as f:  # a version-7 layout under a version-4 header: the upgrade fails
+        assert f.import_compute_settings() == {}
     with write(7, b"{not json") as f:  # skipped, as the reference does
         assert f.has_footer and f.import_compute_settings() == {} and f.import_info() == {}
     with write(7, None) as f:
@@ -325,3 +324,82 @@ def test_large_writes_through_mapping(holo, tmp_path):
         out.correct_number_of_frames(132)
     data = open(path, "rb").read()
     assert data == hf.holo_bytes(frames, 2, footer=holo.dump_footer({"info": {"contiguous": 132}}))
+
+
+# ---- footers of header versions 1 to 6 --------------------------------------------------------
+
+def _upgrades():
+    with open(os.path.join(GOLDEN, "footer_upgrades.json")) as f:
+        return json.load(f)
+
+
+def test_footer_upgrades_match_reference_patches():
+    """Every synthetic footer of header versions 1 to 8 upgraded as the
+    reference's converter does with its own patch files (fixture written by
+    tests/golden/make_footer_upgrades.py from those files): the same
+    document, or a failure where the reference fails."""
+    from holovibes_amd import footer_versions as fv
+    cases = _upgrades()["footers"]
+    assert len(cases) >= 19
+    for c in cases:
+        if c["expected"] is None:
+            with pytest.raises(fv.FooterUpgradeError):
+                fv.upgrade_footer(c["input"], c["version"])
+        else:
+            got = fv.upgrade_footer(c["input"], c["version"])
+            assert got == c["expected"], c["case"]
+    # the input is left as it was
+    c = cases[0]
+    before = json.dumps(c["input"], sort_keys=True)
+    fv.upgrade_footer(c["input"], c["version"])
+    assert json.dumps(c["input"], sort_keys=True) == before
+
+
+def test_compute_settings_upgrade_matches_reference_patch():
+    from holovibes_amd import footer_versions as fv
+    for c in _upgrades()["settings"]:
+        if c["expected"] is None:
+            with pytest.raises(fv.FooterUpgradeError):
+                fv.upgrade_compute_settings(c["input"])
+        else:
+            assert fv.upgrade_compute_settings(c["input"]) == c["expected"], c["case"]
+
+
+def test_version3_recording_loads_its_settings(holo, tmp_path):
+    """A recording with a version-3 header and footer: its settings come
+    through the upgrade into hv_compute_settings (angular spectrum, STFT,
+    T = 16, p = 3, z = 0.25 m, lambda 785 nm, fft shift, contrast range),
+    fields the old footer lacks take the defaults the upgrade adds (batch
+    size 1), and the pixel size comes from the upgraded info."""
+    case = next(c for c in _upgrades()["footers"] if c["case"] == "v3 complete")
+    frames = _frames(2, 16, 16, 2)
+    path = str(tmp_path / "v3.holo")
+    open(path, "wb").write(hf.holo_bytes(frames, 2, hf.RAW, footer=case["input"], version=3))
+    with holo.InputHoloFile(path) as f:
+        cs = f.import_compute_settings()
+        assert cs == case["expected"]["compute_settings"]
+        info = f.import_info()
+    assert info == {"pixel_size": 20.0, "input_fps": 1, "camera_fps": 1}
+    s, _ = holo.compute_settings_from_json(cs, holo.FrameDescriptor(16, 16, 2), info)
+    from holovibes_amd.constants import HV_SPACE_ANGULARSP, HV_TIME_STFT
+    assert (s.space_transformation, s.time_transformation, s.time_transformation_size) == \
+        (HV_SPACE_ANGULARSP, HV_TIME_STFT, 16)
+    assert (s.p_start, s.batch_size, s.time_stride, s.fft_shift) == (3, 1, 1, 1)
+    assert s.z_distance == pytest.approx(0.25) and s.lambda_ == pytest.approx(7.85e-7)
+    assert (s.contrast_min, s.contrast_max) == (0.25, 3.5) and s.pixel_size == 20.0
+    assert s.renorm_enabled == 0
+
+
+def test_unknown_enum_names_read_as_first_entry(holo):
+    """nlohmann reads an enum name outside its table as the table's first
+    entry: an unknown image type is MODULUS, an unknown time transformation
+    STFT; a settings object without a version gets the v0 -> v1 edits (and
+    fails without the keys they remove)."""
+    from holovibes_amd import footer_versions as fv
+    from holovibes_amd.constants import HV_IMG_MODULUS, HV_TIME_STFT
+    cs = {"version": "v1", "image_rendering": {"image_mode": "HOLOGRAM", "time_transformation": "FFT9"},
+          "view": {"image_type": "SQUAREDMODULUS"}}
+    s, _ = holo.compute_settings_from_json(cs, holo.FrameDescriptor(16, 16, 2))
+    assert (s.img_type, s.time_transformation) == (HV_IMG_MODULUS, HV_TIME_STFT)
+    with pytest.raises(fv.FooterUpgradeError):
+        holo.compute_settings_from_json({"image_rendering": {"image_mode": "HOLOGRAM"}}, holo.FrameDescriptor(16, 16, 2))
