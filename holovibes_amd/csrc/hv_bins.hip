@@ -67,10 +67,12 @@ struct BinRowCfg {
     static __device__ __forceinline__ int pad(int n) { return n + (n >> 3); }
 };
 
-// LDS bytes of k_bins_rows: K plane rows + one spare row for idle groups.
+// LDS bytes of k_bins_rows: K plane rows + one spare row for idle groups
+// (+ the T x K8 projection weights for PCA, T > 0).
 template <int W>
-constexpr size_t bins_rows_lds(int K) {
-    return (size_t)(K + 1) * BinRowCfg<W>::PITCH * sizeof(float2);
+constexpr size_t bins_rows_lds(int K, int pca_T = 0) {
+    return (size_t)(K + 1) * BinRowCfg<W>::PITCH * sizeof(float2) +
+           (size_t)pca_T * (pca_T < 8 ? 8 : pca_T) * sizeof(float2);
 }
 
 // The next row's window is prefetched into registers while the current
@@ -108,10 +110,15 @@ __device__ __forceinline__ void real_dft_bins(const float* x, int pmin, int K, E
     }
 }
 
-template <int W, int T, typename TIn>
+// PCA: plane k = sum_t wts[t T + k] x_t over the window in physical slot
+// order (the projection C = H V onto eigenvectors pmin .. pmin + K - 1 of the
+// ring covariance, wts = their row-major float copy from column pmin),
+// instead of DFT bin pmin + k.
+template <int W, int T, typename TIn, bool PCA>
 __global__ void __launch_bounds__(kBinThreads)
     k_bins_rows(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
-                int height, int slot0, int pmin, int K, uint32_t* mm_reset, int mm_slots) {
+                int height, int slot0, int pmin, int K, uint32_t* mm_reset, int mm_slots,
+                const float2* __restrict__ wts) {
     constexpr int rpw = HV_BINS_RPW_ROWS;
     using C = BinRowCfg<W>;
     constexpr int kE = FftShape<W, C::E>::kE;
@@ -120,6 +127,18 @@ __global__ void __launch_bounds__(kBinThreads)
     if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
     const size_t P = (size_t)W * height;
     const int y0 = blockIdx.x * rpw;
+    // PCA: the K x T weights behind the plane rows, slot-major with a row pitch
+    // of KP >= K8 columns (a compile-time stride: constant LDS offsets), zero padded
+    constexpr int KP = T < 8 ? 8 : T;
+    const int K8 = (K + 7) & ~7;
+    float2* lw = rows + (size_t)(K + 1) * C::PITCH;
+    if constexpr (PCA) {
+        for (int i = threadIdx.x; i < T * KP; i += kBinThreads) {
+            const int f = i / KP, k = i % KP;
+            lw[i] = k < K ? wts[(size_t)f * T + k] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+    }
     const int g = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
     // Slot f of the ring holds frame (f - slot0) mod T of the batch.
     auto load_window = [&](int y, int x, float* xr) {
@@ -147,9 +166,52 @@ __global__ void __launch_bounds__(kBinThreads)
             if constexpr (!PF) load_window(y, x, xr[0]);
             const float* xs = xr[PF ? q : 0];
             const float2 c = chirp ? chirp[(size_t)y * W + x] : make_float2(1.f, 0.f);
-            real_dft_bins<T>(xs, pmin, K, [&](int k, float2 v) {
-                rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(v, c) : v;
-            });
+            if constexpr (PCA) {
+                // eight components at a time (sixteen independent FMA chains), the
+                // weights of a slot read from LDS by the whole wave at one address
+#pragma unroll 1
+                for (int kb = 0; kb < K; kb += 8) {
+                    float2 v[8];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) v[b] = make_float2(0.f, 0.f);
+                    // FG slots per window: their weights are loaded, then used; the
+                    // empty asm statements pin the accumulators and memory at the end
+                    // of each window, so that the compiler neither hoists every slot's
+                    // loads nor sinks the FMAs (either needs more registers than a
+                    // wave has)
+                    constexpr int FG = T < 4 ? T : 4;
+#pragma unroll
+                    for (int f0 = 0; f0 < T; f0 += FG) {
+                        float4 w[FG][4];
+#pragma unroll
+                        for (int u = 0; u < FG; ++u)
+#pragma unroll
+                            for (int b2 = 0; b2 < 4; ++b2)
+                                w[u][b2] = reinterpret_cast<const float4*>(lw + kb + (f0 + u) * KP)[b2];
+#pragma unroll
+                        for (int u = 0; u < FG; ++u) {
+                            const float x = xs[f0 + u];
+#pragma unroll
+                            for (int b2 = 0; b2 < 4; ++b2) {
+                                v[2 * b2].x = fmaf(w[u][b2].x, x, v[2 * b2].x);
+                                v[2 * b2].y = fmaf(w[u][b2].y, x, v[2 * b2].y);
+                                v[2 * b2 + 1].x = fmaf(w[u][b2].z, x, v[2 * b2 + 1].x);
+                                v[2 * b2 + 1].y = fmaf(w[u][b2].w, x, v[2 * b2 + 1].y);
+                            }
+                        }
+                        // the window's FMAs complete here and the next window's loads stay after them
+#pragma unroll
+                        for (int b = 0; b < 8; ++b) asm volatile("" : "+v"(v[b].x), "+v"(v[b].y)::"memory");
+                    }
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        if (kb + b < K) rows[(kb + b) * C::PITCH + C::pad(x)] = chirp ? cmul(v[b], c) : v[b];
+                }
+            } else {
+                real_dft_bins<T>(xs, pmin, K, [&](int k, float2 v) {
+                    rows[(k - pmin) * C::PITCH + C::pad(x)] = chirp ? cmul(v, c) : v;
+                });
+            }
         }
         if constexpr (PF) {
             if (i + 1 < rpw && y + 1 < height) {
@@ -183,7 +245,7 @@ __global__ void __launch_bounds__(kBinThreads)
 template <int W, int IMG>
 __global__ void __launch_bounds__(img_threads<W>())
     k_bins_row_ifft_image(const float2* __restrict__ planes, float* __restrict__ out, int height, int K, int rpw,
-                          float count, uint32_t* mm, int mm_slots) {
+                          float count, uint32_t* mm, int mm_slots, float scale) {
     constexpr int kImgThreads = img_threads<W>();
     using C = BinRowCfg<W, kImgThreads>;
     constexpr int kE = FftShape<W, C::E>::kE;
@@ -242,7 +304,7 @@ __global__ void __launch_bounds__(img_threads<W>())
                 float a = 0.f;
 #pragma unroll
                 for (int j2 = 0; j2 < C::G; ++j2) a += part[j2 * W + x];
-                const float val = a / count;
+                const float val = a / count * scale;
                 out[(size_t)y * W + x] = val;
                 vmin = fminf(vmin, val);
                 vmax = fmaxf(vmax, val);
@@ -260,12 +322,12 @@ __global__ void __launch_bounds__(img_threads<W>())
 template <int IMG>
 __global__ void __launch_bounds__(256)
     k_bins_image(const float2* __restrict__ planes, float* __restrict__ out, size_t P, int K, float count,
-                 uint32_t* mm, int mm_slots) {
+                 uint32_t* mm, int mm_slots, float scale) {
     float vmin = INFINITY, vmax = -INFINITY;
     for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
         float a = 0.f;
         for (int k = 0; k < K; ++k) a += image_term_fast<IMG>(planes[(size_t)k * P + p]);
-        const float val = a / count;
+        const float val = a / count * scale;
         out[p] = val;
         vmin = fminf(vmin, val);
         vmax = fmaxf(vmax, val);
@@ -370,6 +432,102 @@ hv_status bins_time(const void* raw, int depth, float2* planes, const float2* ch
     }
 }
 
+// PCA projection for the split path: the tile (T slots x 64 pixels, converted,
+// physical slot order) through LDS; wave w computes components
+// [8 w + 64 i, + 8) of every pixel of the tile (lane = pixel), the weights of
+// a component and slot uniform across the wave (scalar loads).
+constexpr int kPcaTimeThreads = 512;
+constexpr int kPcaTimeCW = 64;
+
+template <int T, typename TIn>
+__global__ void __launch_bounds__(kPcaTimeThreads)
+    k_pca_bins_time(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
+                    size_t P, int slot0, int K, const float2* __restrict__ wts, uint32_t* mm_reset, int mm_slots,
+                    int vec) {
+    constexpr int CW = kPcaTimeCW;
+    __shared__ float lds[T * CW];
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
+    const size_t p0 = (size_t)blockIdx.x * CW;
+    constexpr int PX = 16 / sizeof(TIn);  // pixels per 16-byte load
+    constexpr int CHUNKS = T * CW / PX;
+    for (int idx = threadIdx.x; idx < CHUNKS; idx += kPcaTimeThreads) {
+        const int n = idx / (CW / PX), q = idx % (CW / PX);
+        const int j = (n - slot0) & (T - 1);  // physical slot n holds frame (n - slot0) mod T
+        const TIn* src = raw + (size_t)j * P + p0 + (size_t)q * PX;
+        float* dst = lds + n * CW + q * PX;
+        if (vec) {
+            const uint4 w = *reinterpret_cast<const uint4*>(src);
+            const TIn* px = reinterpret_cast<const TIn*>(&w);
+#pragma unroll
+            for (int e = 0; e < PX; ++e) dst[e] = convert_px<TIn>(px[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < PX; ++e) dst[e] = convert_px<TIn>(src[e]);
+        }
+    }
+    __syncthreads();
+    // the wave index made uniform for the compiler: the weights are then scalar loads
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const size_t p = p0 + lane;
+    const float2 cf = chirp ? chirp[p] : make_float2(1.f, 0.f);
+    constexpr int NB = 8;
+    for (int kb = wave * NB; kb < K; kb += NB * (kPcaTimeThreads / 64)) {
+        float2 acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = make_float2(0.f, 0.f);
+#pragma unroll 4
+        for (int f = 0; f < T; ++f) {
+            const float x = lds[f * CW + lane];
+            const float2* wf = wts + (size_t)f * T + kb;  // 8 consecutive columns: one scalar load
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float2 w = wf[b];
+                acc[b].x = fmaf(w.x, x, acc[b].x);
+                acc[b].y = fmaf(w.y, x, acc[b].y);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (kb + b < K) planes[(size_t)(kb + b) * P + p] = chirp ? cmul(acc[b], cf) : acc[b];
+    }
+}
+
+template <int T, typename TIn>
+hv_status pca_time_t(const void* raw, float2* planes, const float2* chirp, size_t P, int slot0, int K,
+                     const float2* wts, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    if (P % kPcaTimeCW) return fail(HV_ERR_UNSUPPORTED, "bins-first PCA: frames of a multiple of 64 pixels");
+    const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
+    k_pca_bins_time<T, TIn><<<(unsigned)(P / kPcaTimeCW), kPcaTimeThreads, 0, st>>>(
+        (const TIn*)raw, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, vec);
+    HV_CHECK_LAUNCH("k_pca_bins_time");
+    return HV_OK;
+}
+
+template <int T>
+hv_status pca_time_d(const void* raw, int depth, float2* planes, const float2* chirp, size_t P, int slot0, int K,
+                     const float2* wts, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    switch (depth) {
+        case HV_DEPTH_8: return pca_time_t<T, uint8_t>(raw, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case HV_DEPTH_16: return pca_time_t<T, uint16_t>(raw, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        default: return pca_time_t<T, float>(raw, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+    }
+}
+
+hv_status pca_time(const void* raw, int depth, float2* planes, const float2* chirp, size_t P, int T, int slot0,
+                   int K, const float2* wts, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    switch (T) {
+        case 2: return pca_time_d<2>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 4: return pca_time_d<4>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 8: return pca_time_d<8>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 16: return pca_time_d<16>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 32: return pca_time_d<32>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 64: return pca_time_d<64>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 128: return pca_time_d<128>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        case 256: return pca_time_d<256>(raw, depth, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "bins-first PCA: T must be a power of two in [2, 256]");
+    }
+}
+
 // Moments of the propagated bins (tools_conversion.cu:33-66, tools_compute.cu:38-97):
 // plane k holds bin i = fs + k; f1 = fftfreq(T) * fps in float as the
 // reference computes it; the Nyquist bin counted again for M0 and M2 and
@@ -410,64 +568,71 @@ __global__ void __launch_bounds__(256)
     if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
 }
 
-template <int W, int T, typename TIn>
+template <int W, int T, typename TIn, bool PCA>
 hv_status rows_launch(const void* raw, float2* planes, const float2* chirp, int height, int slot0, int pmin, int K,
-                      uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+                      uint32_t* mm_reset, int mm_slots, const float2* wts, hipStream_t st) {
     // the plane rows may take more than the default 64 KiB of dynamic LDS
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bins_rows<W, T, TIn>),
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bins_rows<W, T, TIn, PCA>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_bins_rows)");
     const int rpw = HV_BINS_RPW_ROWS;
-    k_bins_rows<W, T, TIn><<<(height + rpw - 1) / rpw, kBinThreads, bins_rows_lds<W>(K), st>>>(
-        (const TIn*)raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots);
+    k_bins_rows<W, T, TIn, PCA><<<(height + rpw - 1) / rpw, kBinThreads, bins_rows_lds<W>(K, PCA ? T : 0), st>>>(
+        (const TIn*)raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts);
     HV_CHECK_LAUNCH("k_bins_rows");
     return HV_OK;
 }
 
+template <int W, int T, typename TIn>
+hv_status rows_wtd(const void* raw, float2* planes, const float2* chirp, int height, int slot0, int pmin, int K,
+                   uint32_t* mm_reset, int mm_slots, const float2* wts, hipStream_t st) {
+    return wts ? rows_launch<W, T, TIn, true>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st)
+               : rows_launch<W, T, TIn, false>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+}
+
 template <int W, int T>
 hv_status rows_wt(const void* raw, int depth, float2* planes, const float2* chirp, int height, int slot0, int pmin,
-                  int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+                  int K, uint32_t* mm_reset, int mm_slots, const float2* wts, hipStream_t st) {
     switch (depth) {
-        case HV_DEPTH_8: return rows_launch<W, T, uint8_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        case HV_DEPTH_16: return rows_launch<W, T, uint16_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        default: return rows_launch<W, T, float>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case HV_DEPTH_8: return rows_wtd<W, T, uint8_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case HV_DEPTH_16: return rows_wtd<W, T, uint16_t>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        default: return rows_wtd<W, T, float>(raw, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
     }
 }
 
 template <int W>
 hv_status rows_w(const void* raw, int depth, float2* planes, const float2* chirp, int height, int T, int slot0,
-                 int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+                 int pmin, int K, uint32_t* mm_reset, int mm_slots, const float2* wts, hipStream_t st) {
     switch (T) {
-        case 2: return rows_wt<W, 2>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 4: return rows_wt<W, 4>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 8: return rows_wt<W, 8>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 16: return rows_wt<W, 16>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 32: return rows_wt<W, 32>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 64: return rows_wt<W, 64>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 2: return rows_wt<W, 2>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 4: return rows_wt<W, 4>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 8: return rows_wt<W, 8>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 16: return rows_wt<W, 16>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 32: return rows_wt<W, 32>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 64: return rows_wt<W, 64>(raw, depth, planes, chirp, height, slot0, pmin, K, mm_reset, mm_slots, wts, st);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: T must be a power of two in [2, 64]");
     }
 }
 
-size_t rows_lds_for(int width, int K) {
+size_t rows_lds_for(int width, int K, int pca_T) {
     switch (width) {
-        case 64: return bins_rows_lds<64>(K);
-        case 128: return bins_rows_lds<128>(K);
-        case 256: return bins_rows_lds<256>(K);
-        case 512: return bins_rows_lds<512>(K);
-        case 1024: return bins_rows_lds<1024>(K);
+        case 64: return bins_rows_lds<64>(K, pca_T);
+        case 128: return bins_rows_lds<128>(K, pca_T);
+        case 256: return bins_rows_lds<256>(K, pca_T);
+        case 512: return bins_rows_lds<512>(K, pca_T);
+        case 1024: return bins_rows_lds<1024>(K, pca_T);
         default: return 0;
     }
 }
 
 }  // namespace
 
-bool bins_first_supported(int width, int height, int T, int K, int depth) {
+bool bins_first_supported(int width, int height, int T, int K, int depth, bool pca) {
     if (!(width == 64 || width == 128 || width == 256 || width == 512 || width == 1024)) return false;
     if (!is_pow2(height) || height < 16 || height > 4096) return false;
     if (!is_pow2(T) || T < 2 || T > 64 || K < 1 || K > T) return false;
     if (width == 1024 && T > 32) return false;  // two pixels' windows per thread exceed the registers
     if (depth != HV_DEPTH_8 && depth != HV_DEPTH_16 && depth != HV_DEPTH_32F) return false;
-    return rows_lds_for(width, K) <= 160 * 1024;
+    return rows_lds_for(width, K, pca ? T : 0) <= 160 * 1024;
 }
 
 bool bins_split_supported(int width, int height, int T, int K, int depth) {
@@ -478,23 +643,25 @@ bool bins_split_supported(int width, int height, int T, int K, int depth) {
 
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
-                           int mm_slots) {
+                           int mm_slots, const float2* wts) {
     if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
         return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
     const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
-    if (!bins_first_supported(width, height, T, K, depth)) {
+    if (!bins_first_supported(width, height, T, K, depth, wts != nullptr)) {
         if (!bins_split_supported(width, height, T, K, depth))
             return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
-        // split: the window's bins, then their forward row FFTs
-        HV_TRY(bins_time(raw, depth, planes, chirp, (size_t)width * height, T, slot0, pmin, K, mm_reset, mm_slots, st));
+        // split: the window's bins (or PCA components), then their forward row FFTs
+        const size_t P = (size_t)width * height;
+        if (wts) HV_TRY(pca_time(raw, depth, planes, chirp, P, T, slot0, K, wts, mm_reset, mm_slots, st));
+        else HV_TRY(bins_time(raw, depth, planes, chirp, P, T, slot0, pmin, K, mm_reset, mm_slots, st));
         return launch_row_pass(planes, width, height, K, -1, st);
     }
     switch (width) {
-        case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 256: return rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 512: return rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
-        default: return rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 256: return rows_w<256>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        case 512: return rows_w<512>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
+        default: return rows_w<1024>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
     }
 }
 
@@ -506,9 +673,9 @@ hv_status launch_bins_columns(float2* planes, int width, int height, int K, int 
 
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots) {
+                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots, const float2* wts) {
     HV_TRY(launch_bins_rows(raw, depth, width, height, T, slot0, pmin, K, space, lens, planes, st, mm_reset,
-                            mm_slots));
+                            mm_slots, wts));
     if (mark) HV_HIP(hipEventRecord(mark, st));
     return launch_bins_columns(planes, width, height, K, space, lens, st);
 }
@@ -521,24 +688,24 @@ bool bins_image_fused(int width, int space) {
 
 template <int IMG>
 hv_status bins_image_t(float* out, float2* planes, int width, int height, int K, int space, float count,
-                       uint32_t* mm, int mm_slots, hipStream_t st) {
+                       uint32_t* mm, int mm_slots, float scale, hipStream_t st) {
     if (!bins_image_fused(width, space)) {
         // split: the planes' inverse row FFT (angular spectrum), then the image term
         if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
         const size_t P = (size_t)width * height;
         const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
-        k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm, mm_slots);
+        k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm, mm_slots, scale);
         HV_CHECK_LAUNCH("k_bins_image");
         return HV_OK;
     }
     const int rpw = HV_BINS_RPW_IMAGE;
     const dim3 grid((height + rpw - 1) / rpw);
     switch (width) {
-        case 64: k_bins_row_ifft_image<64, IMG><<<grid, img_threads<64>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
-        case 128: k_bins_row_ifft_image<128, IMG><<<grid, img_threads<128>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
-        case 256: k_bins_row_ifft_image<256, IMG><<<grid, img_threads<256>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
-        case 512: k_bins_row_ifft_image<512, IMG><<<grid, img_threads<512>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
-        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, img_threads<1024>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots); break;
+        case 64: k_bins_row_ifft_image<64, IMG><<<grid, img_threads<64>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots, scale); break;
+        case 128: k_bins_row_ifft_image<128, IMG><<<grid, img_threads<128>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots, scale); break;
+        case 256: k_bins_row_ifft_image<256, IMG><<<grid, img_threads<256>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots, scale); break;
+        case 512: k_bins_row_ifft_image<512, IMG><<<grid, img_threads<512>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots, scale); break;
+        case 1024: k_bins_row_ifft_image<1024, IMG><<<grid, img_threads<1024>(), 0, st>>>(planes, out, height, K, rpw, count, mm, mm_slots, scale); break;
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported width");
     }
     HV_CHECK_LAUNCH("k_bins_row_ifft_image");
@@ -558,14 +725,16 @@ hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* pla
 }
 
 hv_status launch_bins_image(float* out, const float2* planes_c, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots) {
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots, float scale) {
     float2* planes = const_cast<float2*>(planes_c);
     const float count = (float)(pmax - pmin + 1);
     switch (img_type) {
-        case HV_IMG_MODULUS: return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, st);
+        case HV_IMG_MODULUS:
+            return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st);
         case HV_IMG_SQUARED_MODULUS:
-            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, st);
-        case HV_IMG_ARGUMENT: return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, mm_slots, st);
+            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st);
+        case HV_IMG_ARGUMENT:
+            return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: image type must be modulus, squared or argument");
     }
 }

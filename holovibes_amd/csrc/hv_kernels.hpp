@@ -52,7 +52,7 @@ hv_status launch_stft_moments(float* m0, float* m1, float* m2, const float2* rin
 // bins_first_supported: the fused row kernels fit (W <= 1024, T <= 64, the
 // K plane rows of an image row in LDS); bins_split_supported: the path as
 // separate passes (time DFT, row, column, row, image), any W, H, T.
-bool bins_first_supported(int width, int height, int T, int K, int depth);
+bool bins_first_supported(int width, int height, int T, int K, int depth, bool pca = false);
 bool bins_split_supported(int width, int height, int T, int K, int depth);
 // Moments M0, M1, M2 (any may be null; `out` receives moment `moment` too, for
 // the image) of the propagated planes holding bins [fs, fs + K) of a length-T
@@ -63,18 +63,22 @@ hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* pla
 // Bins + row FFTs (x chirp for Fresnel) into planes[0 .. K) (K x P complex),
 // then the column pass over them; launch_bins_propagate runs both, with
 // `mark` (optional) recorded in between.
+// wts (bins-first PCA): plane k = sum_t wts[t T + k] x_t (the projection onto
+// eigenvectors pmin .. pmin + K - 1: the row-major float eigenvectors from
+// column pmin, readable 8 columns past the last) instead of DFT bin pmin + k.
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
-                           int mm_slots = 1);
+                           int mm_slots = 1, const float2* wts = nullptr);
 hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
                               hipStream_t st);
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots = 1);
+                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots = 1, const float2* wts = nullptr);
 // Image of the propagated planes: inverse row FFT (angular spectrum) + image
-// term, mean over [pmin, pmax]; mm: encoded min/max (may be null).
+// term, mean over [pmin, pmax], x scale (131071 for PCA images); mm: encoded
+// min/max (may be null).
 hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1);
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1, float scale = 1.f);
 
 // hv_render.hip
 // mm: `slots` accumulator pairs (block_minmax_atomic); the C-ABI uses one.
@@ -122,9 +126,17 @@ size_t pca_workspace_bytes(int T, size_t P, int precision);
 // eigensolver), float2 otherwise (the C-ABI step's).
 hv_status launch_pca_cov(void* cov, int cov_double, const float2* ring, int T, size_t P, int precision, void* ws,
                          hipStream_t st);
+// Bins-first PCA: the covariance (double2, T x T) of the converted raw
+// window, frames in physical ring-slot order (slot n = frame (n - slot0) mod T),
+// equal to the propagated ring's up to a constant factor.
+bool pca_raw_cov_supported(int T, size_t P);
+hv_status launch_pca_cov_raw(double2* cov, const void* raw, int depth, int T, size_t P, int slot0, int precision,
+                             void* ws, hipStream_t st);
 size_t eig_workspace_bytes(int T);
+// evecs32 (optional): the eigenvectors again as float2, row-major (component
+// k of eigenvector j at k T + j)
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const void* cov, int cov_double, int T, void* work,
-                            hipStream_t st);
+                            hipStream_t st, float2* evecs32 = nullptr);
 // img_type: an image type (mean over [pmin, pmax], x131071) or HV_IMG_MOMENTS_k
 // (weights from fps over [pmin, pmax] = [f_start, f_end])
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,

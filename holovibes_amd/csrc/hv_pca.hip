@@ -122,7 +122,11 @@ static CovLayout cov_layout(int T, size_t P, int precision) {
     return l;
 }
 
-size_t pca_workspace_bytes(int T, size_t P, int precision) { return cov_layout(T, P, precision).bytes; }
+static CovLayout raw_cov_layout(int T, size_t P);
+// room for the ring covariance and for the raw-window covariance (bins-first PCA)
+size_t pca_workspace_bytes(int T, size_t P, int precision) {
+    return std::max(cov_layout(T, P, precision).bytes, raw_cov_layout(T, P).bytes);
+}
 
 __device__ __forceinline__ void pair_of(int idx, int& F, int& G) {
     // pairs enumerated (0,0), (1,0), (1,1), (2,0), ... with F >= G
@@ -694,6 +698,462 @@ hv_status launch_pca_cov(void* cov, int cov_double, const float2* ring, int T, s
 }
 
 // ---------------------------------------------------------------------------
+// covariance of the raw window (bins-first PCA)
+// ---------------------------------------------------------------------------
+// Both propagators are unitary up to a constant: the angular spectrum is
+// L = F^H diag(H) F / P with |H| = 1 everywhere (L^H L = I), Fresnel is
+// L = F diag(c) with a unit-modulus chirp (L^H L = P I).  The propagated
+// window X = L H (frames as columns) therefore has X^H X = alpha H^T H: the
+// covariance the reference takes of the ring (matrix_operations.cc:9-37) is
+// the covariance of the converted raw window, a real symmetric matrix, up to
+// a factor that changes no eigenvector and no eigenvalue order.  It is taken
+// here straight from the raw u8 / u16 / f32 frames, in physical ring-slot
+// order (slot n holds frame (n - slot0) mod T of the batch), centred as in
+// launch_pca_cov: real products, a quarter of the complex MFMA work, and
+// d bytes per value read instead of 8.
+
+// N consecutive pixels of a raw frame, loaded as whole words (16-byte aligned
+// buffers) or byte by byte (a caller's buffer that is not), converted on use.
+template <typename TIn, int N>
+struct RawPx {
+    static constexpr int kBytes = N * (int)sizeof(TIn);
+    static constexpr int kWords = kBytes / 4;
+    uint32_t w[kWords];
+    __device__ __forceinline__ void load(const TIn* src, int vec) {
+        if (vec) {
+            if constexpr (kBytes == 4) {
+                w[0] = *reinterpret_cast<const uint32_t*>(src);
+            } else if constexpr (kBytes == 8) {
+                const uint2 v = *reinterpret_cast<const uint2*>(src);
+                w[0] = v.x;
+                w[1] = v.y;
+            } else {
+#pragma unroll
+                for (int i = 0; i < kBytes / 16; ++i) {
+                    const uint4 v = reinterpret_cast<const uint4*>(src)[i];
+                    w[4 * i] = v.x;
+                    w[4 * i + 1] = v.y;
+                    w[4 * i + 2] = v.z;
+                    w[4 * i + 3] = v.w;
+                }
+            }
+        } else {
+            const uint8_t* b = reinterpret_cast<const uint8_t*>(src);
+#pragma unroll
+            for (int i = 0; i < kWords; ++i)
+                w[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
+                       ((uint32_t)b[4 * i + 3] << 24);
+        }
+    }
+    __device__ __forceinline__ float get(int e) const {
+        if constexpr (sizeof(TIn) == 1) return convert_px<uint8_t>((uint8_t)(w[e >> 2] >> (8 * (e & 3))));
+        else if constexpr (sizeof(TIn) == 2) return convert_px<uint16_t>((uint16_t)(w[e >> 1] >> (16 * (e & 1))));
+        else return __uint_as_float(w[e]);
+    }
+};
+
+// Per-pixel window mean (float: integer frames sum exactly) and FP64 partials
+// of m = sum mu^2 (one per workgroup), and -- for the FP16 scale -- max |x| and
+// max |mu|.  A workgroup takes 256 pixels at a time: thread (g, q) sums frames
+// g, g + 4, ... of pixels 4q .. 4q + 3 (one word per frame, the frames' loads
+// independent), and the four frame groups are added in LDS in a fixed order.
+constexpr int kRawMeanThreads = 256;
+template <typename TIn>
+__global__ void __launch_bounds__(kRawMeanThreads)
+    k_raw_mean(float* __restrict__ mean, double* __restrict__ mpart, uint32_t* mm, const TIn* __restrict__ raw,
+               int T, size_t P, int vec) {
+    constexpr int kGroups = 4, kQuads = kRawMeanThreads / kGroups;  // 64 quads = 256 pixels
+    const int q = threadIdx.x % kQuads, g = threadIdx.x / kQuads;
+    const float inv = 1.0f / (float)T;  // a power of two: exact
+    __shared__ float4 part[kGroups][kQuads];
+    float mx = 0.f, mmu = 0.f;
+    double msum = 0.0;
+    for (size_t c0 = (size_t)blockIdx.x * 256; c0 < P; c0 += (size_t)gridDim.x * 256) {
+        const size_t p = c0 + 4 * q;
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+        for (int t = g; t < T; t += kGroups) {
+            RawPx<TIn, 4> v;
+            v.load(raw + (size_t)t * P + p, vec);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float x = v.get(e);
+                acc[e] += x;
+                if (mm) mx = fmaxf(mx, fabsf(x));
+            }
+        }
+        __syncthreads();  // the previous chunk's partial sums have been read
+        part[g][q] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        __syncthreads();
+        if (g == 0) {
+            float4 s = part[0][q];
+#pragma unroll
+            for (int j = 1; j < kGroups; ++j) {
+                const float4 u = part[j][q];
+                s.x += u.x;
+                s.y += u.y;
+                s.z += u.z;
+                s.w += u.w;
+            }
+            const float4 m = make_float4(s.x * inv, s.y * inv, s.z * inv, s.w * inv);
+            reinterpret_cast<float4*>(mean)[p / 4] = m;
+            mmu = fmaxf(mmu, fmaxf(fmaxf(fabsf(m.x), fabsf(m.y)), fmaxf(fabsf(m.z), fabsf(m.w))));
+            msum += (double)m.x * m.x + (double)m.y * m.y + (double)m.z * m.z + (double)m.w * m.w;
+        }
+    }
+    if (mm) {  // uniform: a kernel argument
+        block_minmax_atomic(mx, mx, mm);
+        __syncthreads();  // the helper's LDS is reused by the second call
+        block_minmax_atomic(mmu, mmu, mm + 2);
+    }
+    __shared__ double red[kRawMeanThreads];
+    red[threadIdx.x] = msum;
+    __syncthreads();
+    for (int o = kRawMeanThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mpart[blockIdx.x] = red[0];
+}
+
+// Slice sums of the raw-window tiles in FP64 with the mean terms computed in
+// place (k_pca_mean_terms + k_pca_cov_reduce in one kernel): a workgroup owns
+// 64 elements of one pair's tile -- two rows i0, i0 + 4 and 32 columns -- and
+// its 16 waves sum, each over every 16th slice, the tile elements and the a_t
+// of those 34 frames, and m over the mean pass's partials; the waves' sums are
+// added in a fixed order.  cov(i, j) = (Y^T Y)(i, j) + a_i + a_j + m.
+constexpr int kRawReduceWaves = 16;
+__global__ void __launch_bounds__(64 * kRawReduceWaves)
+    k_raw_cov_reduce(double2* __restrict__ cov64, const float* __restrict__ partial, const float2* __restrict__ apart,
+                     const double* __restrict__ mpart, int mblocks, int T, int slices, int npairs,
+                     const uint32_t* __restrict__ scale_mm, int use_scale) {
+    constexpr int NW = kRawReduceWaves;
+    const int pair = blockIdx.x;
+    int F, G;
+    pair_of(pair, F, G);
+    const int g = threadIdx.x >> 6, el = threadIdx.x & 63;
+    const int r = blockIdx.y;  // MFMA output register: rows 8 (r / 4) + r % 4 (+ 4), all 32 columns
+    const int e = r * 64 + el;
+    const int i0 = 8 * (r / 4) + (r % 4);
+    const int f = el < 32 ? 32 * G + el : 32 * F + i0 + 4 * (el - 32);
+    double re = 0.0;
+#pragma unroll 8
+    for (int sl = g; sl < slices; sl += NW) re += (double)partial[((size_t)sl * npairs + pair) * kTile + e];
+    double a = 0.0;
+    if (el < 34 && f < T) {
+#pragma unroll 8
+        for (int sl = g; sl < slices; sl += NW) a += (double)apart[(size_t)sl * T + f].x;
+    }
+    double m = 0.0;
+    for (int b = (int)threadIdx.x; b < mblocks; b += 64 * NW) m += mpart[b];
+    for (int o = 32; o >= 1; o >>= 1) m += __shfl_xor(m, o, 64);
+    __shared__ double sr[NW][64];
+    __shared__ double ap[NW][34];
+    __shared__ double mp[NW];
+    sr[g][el] = re;
+    if (el < 34) ap[g][el] = a;
+    if (el == 0) mp[g] = m;
+    __syncthreads();
+    if (g != 0) return;
+    re = 0.0;
+    double msh = 0.0, ar = 0.0, ac = 0.0;
+    const int half = el / 32, jj = el % 32;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        re += sr[w][el];
+        msh += mp[w];
+        ar += ap[w][32 + half];
+        ac += ap[w][jj];
+    }
+    if (use_scale) {
+        const double sc = (double)fp16_scale(scale_mm);
+        re *= 1.0 / (sc * sc);  // exact: a power of two
+    }
+    const int fi = 32 * F + i0 + 4 * half, fj = 32 * G + jj;
+    if (fi >= T || fj >= T) return;
+    re += ar + ac + msh;
+    cov64[fi + (size_t)fj * T] = make_double2(re, 0.0);
+    if (F != G) cov64[fj + (size_t)fi * T] = make_double2(re, 0.0);
+}
+
+// The MFMA fragments of 8 centred values (pixels 8 (lane / 32) .. + 7 of a
+// 16-pixel step, the operand layout of both A and B): FP32 values for the
+// 32x32x2 f32 steps (value e in step e), or BF16 / FP16 hi and (split) lo.
+template <int PREC>
+struct RawFrag {
+    static constexpr bool fp32 = PREC == HV_PCA_FP32;
+    static constexpr bool split = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_FP16_SPLIT);
+    static constexpr bool bf = (PREC == HV_PCA_BF16_SPLIT || PREC == HV_PCA_BF16);
+    using V8 = typename std::conditional<bf, v8bf, v8h>::type;
+    using E = typename std::conditional<bf, __bf16, _Float16>::type;
+};
+
+// acc += A^T B over 16 pixels, A and B the fragments of two frame groups
+// (the same registers for a diagonal pair).
+template <int PREC>
+__device__ __forceinline__ v16f raw_products(const float (&a)[8], const float (&b)[8], v16f acc) {
+    using F = RawFrag<PREC>;
+    if constexpr (F::fp32) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[e], b[e], acc, 0, 0, 0);
+        return acc;
+    } else {
+        typename F::V8 ah, bh, al, bl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const typename F::E xa = (typename F::E)a[e], xb = (typename F::E)b[e];
+            ah[e] = xa;
+            bh[e] = xb;
+            if constexpr (F::split) {
+                al[e] = (typename F::E)(a[e] - (float)xa);
+                bl[e] = (typename F::E)(b[e] - (float)xb);
+            }
+        }
+        auto mfma = [](typename F::V8 x, typename F::V8 y, v16f c) {
+            if constexpr (F::bf) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, c, 0, 0, 0);
+            else return __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y, c, 0, 0, 0);
+        };
+        if constexpr (F::split) {  // small cross terms first, then hi * hi
+            acc = mfma(al, bh, acc);
+            acc = mfma(ah, bl, acc);
+        }
+        return mfma(ah, bh, acc);
+    }
+}
+
+// T <= 32 (one frame group): every wave streams its own pixel range, lane l
+// holding frame slot l % 32 (zero past T); the four waves' tiles are summed
+// in LDS into one partial tile per workgroup.
+template <int PREC, typename TIn>
+__global__ void __launch_bounds__(64 * kCovWaves)
+    k_raw_cov(float* __restrict__ partial, float2* __restrict__ apart, const TIn* __restrict__ raw,
+              const float* __restrict__ mean, int T, size_t P, int slices, int slot0, int vec,
+              const uint32_t* __restrict__ scale_mm) {
+    const int slice = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t per_wave = P / ((size_t)slices * kCovWaves);
+    const size_t p_begin = ((size_t)slice * kCovWaves + wave) * per_wave, p_end = p_begin + per_wave;
+    const int f = lane & 31;
+    const bool ok = f < T;
+    const TIn* row = raw + (size_t)(ok ? ((f - slot0) & (T - 1)) : 0) * P + (lane >> 5) * 8;
+    const float* mrow = mean + (lane >> 5) * 8;
+    float scale = 1.0f;
+    if constexpr (PREC == HV_PCA_FP16 || PREC == HV_PCA_FP16_SPLIT) scale = fp16_scale(scale_mm);
+    RawPx<TIn, 8> nx;
+    float4 nm0, nm1;
+    auto load = [&](size_t p0) {
+        if (ok) nx.load(row + p0, vec);
+        nm0 = *reinterpret_cast<const float4*>(mrow + p0);
+        nm1 = *reinterpret_cast<const float4*>(mrow + p0 + 4);
+    };
+    load(p_begin);
+    float aacc = 0.f;
+    v16f acc = {};
+    for (size_t p0 = p_begin; p0 < p_end; p0 += 16) {
+        const float m[8] = {nm0.x, nm0.y, nm0.z, nm0.w, nm1.x, nm1.y, nm1.z, nm1.w};
+        float y[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float c = ok ? nx.get(e) - m[e] : 0.f;
+            aacc = fmaf(c, m[e], aacc);
+            y[e] = c * scale;
+        }
+        if (p0 + 16 < p_end) load(p0 + 16);
+        acc = raw_products<PREC>(y, y, acc);
+    }
+    __shared__ float red[kCovWaves][1024];
+    __shared__ float ared[kCovWaves][64];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) red[wave][r * 64 + lane] = acc[r];
+    ared[wave][lane] = aacc;
+    __syncthreads();
+    float* out = partial + (size_t)slice * kTile;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
+        float s = 0.f;
+#pragma unroll
+        for (int w = 0; w < kCovWaves; ++w) s += red[w][i];
+        out[i] = s;
+    }
+    if (threadIdx.x < 32 && (int)threadIdx.x < T) {
+        float a = 0.f;
+#pragma unroll
+        for (int w = 0; w < kCovWaves; ++w) a += ared[w][threadIdx.x] + ared[w][threadIdx.x + 32];
+        apart[(size_t)slice * T + threadIdx.x] = make_float2(a, 0.f);
+    }
+}
+
+// T = 32 G (G >= 2): as k_pca_cov_groups -- wave g < G converts frame group
+// g's 16-pixel step into LDS fragments once, every wave runs the products of
+// its pairs -- with real fragments.
+template <int PREC, int G, typename TIn>
+__global__ void __launch_bounds__(64 * kCovGroupWaves)
+    k_raw_cov_groups(float* __restrict__ partial, float2* __restrict__ apart, const TIn* __restrict__ raw,
+                     const float* __restrict__ mean, size_t P, int slices, int slot0, int vec,
+                     const uint32_t* __restrict__ scale_mm) {
+    constexpr int NP = G * (G + 1) / 2;
+    constexpr int W = kCovGroupWaves;
+    constexpr int PPW = (NP + W - 1) / W;
+    constexpr int T = 32 * G;
+    __shared__ float4 frag[2][G][2][64];  // 8 centred, scaled values per lane
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const size_t per_slice = P / slices;
+    const size_t p_begin = (size_t)blockIdx.x * per_slice, p_end = p_begin + per_slice;
+    float scale = 1.0f;
+    if constexpr (PREC == HV_PCA_FP16 || PREC == HV_PCA_FP16_SPLIT) scale = fp16_scale(scale_mm);
+    const bool conv = wave < G;
+    const int f = 32 * (conv ? wave : 0) + (lane & 31);
+    const TIn* row = raw + (size_t)((f - slot0) & (T - 1)) * P + (lane >> 5) * 8;
+    const float* mrow = mean + (lane >> 5) * 8;
+    RawPx<TIn, 8> nx;
+    float4 nm0, nm1;
+    auto load = [&](size_t p0) {
+        nx.load(row + p0, vec);
+        nm0 = *reinterpret_cast<const float4*>(mrow + p0);
+        nm1 = *reinterpret_cast<const float4*>(mrow + p0 + 4);
+    };
+    if (conv) load(p_begin);
+    float aacc = 0.f;
+    v16f acc[PPW];
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) acc[j] = v16f{};
+    int buf = 0;
+    for (size_t p0 = p_begin; p0 < p_end; p0 += 16, buf ^= 1) {
+        if (conv) {
+            const float m[8] = {nm0.x, nm0.y, nm0.z, nm0.w, nm1.x, nm1.y, nm1.z, nm1.w};
+            float y[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float c = nx.get(e) - m[e];
+                aacc = fmaf(c, m[e], aacc);
+                y[e] = c * scale;
+            }
+            if (p0 + 16 < p_end) load(p0 + 16);
+            frag[buf][wave][0][lane] = make_float4(y[0], y[1], y[2], y[3]);
+            frag[buf][wave][1][lane] = make_float4(y[4], y[5], y[6], y[7]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PPW; ++j) {
+            const int pair = wave + j * W;
+            if (pair >= NP) break;
+            int F, Gp;
+            pair_of(pair, F, Gp);
+            const float4 a0 = frag[buf][F][0][lane], a1 = frag[buf][F][1][lane];
+            const float4 b0 = frag[buf][Gp][0][lane], b1 = frag[buf][Gp][1][lane];
+            const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+            const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            acc[j] = raw_products<PREC>(a, b, acc[j]);
+        }
+    }
+    if (conv) {
+        const float a = aacc + __shfl_xor(aacc, 32);
+        if (lane < 32) apart[(size_t)blockIdx.x * T + 32 * wave + lane] = make_float2(a, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) {
+        const int pair = wave + j * W;
+        if (pair >= NP) break;
+        float* out = partial + ((size_t)blockIdx.x * NP + pair) * kTile;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[r * 64 + lane] = acc[j][r];
+    }
+}
+
+// Pixel slices of the raw covariance: T <= 32 as the pair kernel (one pair),
+// larger windows as the grouped kernel.
+static int raw_cov_slices(int T, size_t P) { return T <= 32 ? pca_slices_pairs(T, P) : pca_slices_grouped(P); }
+
+static CovLayout raw_cov_layout(int T, size_t P) {
+    CovLayout l;
+    const int slices = raw_cov_slices(T, P);
+    l.partial_floats = (size_t)slices * pca_pairs(T) * kTile;
+    l.off_apart = align_up(l.partial_floats * sizeof(float));
+    l.off_mean = l.off_apart + align_up((size_t)slices * std::max(T, 32) * sizeof(float2));
+    l.off_mpart = l.off_mean + align_up(P * sizeof(float));
+    l.off_terms = l.off_mpart + align_up(kMeanBlocks * sizeof(double));
+    l.offset_scale = l.off_terms + align_up((size_t)(T + 1) * sizeof(double2));
+    l.bytes = l.offset_scale + 256;
+    return l;
+}
+
+template <int PREC, typename TIn>
+static hv_status raw_cov_t(float* partial, float2* apart, const void* raw, const float* mean, int T, size_t P,
+                           int slices, int slot0, int vec, const uint32_t* scale_mm, hipStream_t st) {
+    const TIn* r = (const TIn*)raw;
+    constexpr int tg = 64 * kCovGroupWaves;
+    switch (T) {
+        case 64: k_raw_cov_groups<PREC, 2, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm); break;
+        case 128: k_raw_cov_groups<PREC, 4, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm); break;
+        case 256: k_raw_cov_groups<PREC, 8, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm); break;
+        default:
+            k_raw_cov<PREC, TIn><<<slices, 64 * kCovWaves, 0, st>>>(partial, apart, r, mean, T, P, slices, slot0, vec, scale_mm);
+            break;
+    }
+    HV_CHECK_LAUNCH("k_raw_cov");
+    return HV_OK;
+}
+
+template <typename TIn>
+static hv_status raw_cov_p(int precision, float* partial, float2* apart, const void* raw, const float* mean, int T,
+                           size_t P, int slices, int slot0, int vec, const uint32_t* scale_mm, hipStream_t st) {
+    switch (precision) {
+        case HV_PCA_FP32: return raw_cov_t<HV_PCA_FP32, TIn>(partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st);
+        case HV_PCA_BF16_SPLIT: return raw_cov_t<HV_PCA_BF16_SPLIT, TIn>(partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st);
+        case HV_PCA_FP16_SPLIT: return raw_cov_t<HV_PCA_FP16_SPLIT, TIn>(partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st);
+        case HV_PCA_BF16: return raw_cov_t<HV_PCA_BF16, TIn>(partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st);
+        default: return raw_cov_t<HV_PCA_FP16, TIn>(partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st);
+    }
+}
+
+bool pca_raw_cov_supported(int T, size_t P) {
+    if (!(T == 1 || T == 2 || T == 4 || T == 8 || T == 16 || T == 32 || T == 64 || T == 128 || T == 256)) return false;
+    const int slices = raw_cov_slices(T, P);
+    return P % ((size_t)slices * (T <= 32 ? kCovWaves : 1) * 16) == 0;
+}
+
+hv_status launch_pca_cov_raw(double2* cov, const void* raw, int depth, int T, size_t P, int slot0, int precision,
+                             void* ws, hipStream_t st) {
+    if (!pca_raw_cov_supported(T, P)) return fail(HV_ERR_UNSUPPORTED, "raw-window covariance: unsupported shape");
+    switch (precision) {
+        case HV_PCA_FP32: case HV_PCA_BF16_SPLIT: case HV_PCA_FP16_SPLIT: case HV_PCA_BF16: case HV_PCA_FP16: break;
+        default: return fail(HV_ERR_INVALID_ARG, "unknown PCA precision");
+    }
+    const CovLayout l = raw_cov_layout(T, P);
+    const int slices = raw_cov_slices(T, P), np = pca_pairs(T);
+    char* w = reinterpret_cast<char*>(ws);
+    float* partial = reinterpret_cast<float*>(w);
+    float2* apart = reinterpret_cast<float2*>(w + l.off_apart);
+    float* mean = reinterpret_cast<float*>(w + l.off_mean);
+    double* mpart = reinterpret_cast<double*>(w + l.off_mpart);
+    uint32_t* scale_mm = reinterpret_cast<uint32_t*>(w + l.offset_scale);
+    const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
+    const bool fp16 = precision == HV_PCA_FP16 || precision == HV_PCA_FP16_SPLIT;
+    // the FP16 scale words (max |x|, max |mu|): only the FP16 products use them
+    uint32_t* mm = fp16 ? scale_mm : nullptr;
+    if (fp16) {
+        k_scale_init<<<1, 64, 0, st>>>(scale_mm);
+        HV_CHECK_LAUNCH("k_scale_init");
+    }
+    const int mblocks = (int)std::min<size_t>(kMeanBlocks, P / 256);
+    switch (depth) {
+        case HV_DEPTH_8: k_raw_mean<uint8_t><<<mblocks, kRawMeanThreads, 0, st>>>(mean, mpart, mm, (const uint8_t*)raw, T, P, vec); break;
+        case HV_DEPTH_16: k_raw_mean<uint16_t><<<mblocks, kRawMeanThreads, 0, st>>>(mean, mpart, mm, (const uint16_t*)raw, T, P, vec); break;
+        case HV_DEPTH_32F: k_raw_mean<float><<<mblocks, kRawMeanThreads, 0, st>>>(mean, mpart, mm, (const float*)raw, T, P, vec); break;
+        default: return fail(HV_ERR_INVALID_ARG, "depth must be 1, 2 or 4");
+    }
+    HV_CHECK_LAUNCH("k_raw_mean");
+    switch (depth) {
+        case HV_DEPTH_8: HV_TRY(raw_cov_p<uint8_t>(precision, partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st)); break;
+        case HV_DEPTH_16: HV_TRY(raw_cov_p<uint16_t>(precision, partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st)); break;
+        default: HV_TRY(raw_cov_p<float>(precision, partial, apart, raw, mean, T, P, slices, slot0, vec, scale_mm, st)); break;
+    }
+    k_raw_cov_reduce<<<dim3(np, 1024 / 64), 64 * kRawReduceWaves, 0, st>>>(cov, partial, apart, mpart, mblocks, T,
+                                                                          slices, np, scale_mm, fp16 ? 1 : 0);
+    HV_CHECK_LAUNCH("k_raw_cov_reduce");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 Jacobi
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
@@ -802,7 +1262,7 @@ __device__ __forceinline__ void rot_left(const Rot& R, double2& yp, double2& yq)
 // first of equal magnitudes within a relative 1e-12 -- from the converged
 // diagonal d(j) and the accumulated rotations v(k, j).
 template <int NT, int N, class D, class VA>
-__device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int* order) {
+__device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int* order, float2* evecs32) {
     constexpr int n = N;
     const int tid = threadIdx.x;
     for (int j = tid; j < n; j += NT) {
@@ -834,7 +1294,12 @@ __device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int*
         const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
         const double2 rotp = make_double2(ph.x * inv, -ph.y * inv);
 #pragma unroll 4
-        for (int k = 0; k < n; ++k) evecs[k + (size_t)jj * n] = zmul(v(k, j), rotp);
+        for (int k = 0; k < n; ++k) {
+            const double2 z = zmul(v(k, j), rotp);
+            evecs[k + (size_t)jj * n] = z;
+            // row-major float copy: one component of consecutive eigenvectors is contiguous
+            if (evecs32) evecs32[(size_t)k * n + jj] = make_float2((float)z.x, (float)z.y);
+        }
     }
 }
 
@@ -849,7 +1314,7 @@ struct JacobiLog {
 
 template <int NT, int N, bool DEFER = false>
 __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double2* evecs, Rot* rot, double* red,
-                           int* order, JacobiLog log = JacobiLog{}) {
+                           int* order, float2* evecs32, JacobiLog log = JacobiLog{}) {
     constexpr int n = N, half = N / 2;
     const int tid = threadIdx.x;
     int g = 0;  // rounds done
@@ -925,7 +1390,7 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
         if (tid == 0) *log.rounds = g;
         return;
     }
-    jacobi_finish<NT, N>([&](int j) { return a(j, j).x; }, v, evals, evecs, order);
+    jacobi_finish<NT, N>([&](int j) { return a(j, j).x; }, v, evals, evecs, order, evecs32);
 }
 
 // A covariance element (float2 from the C-ABI step, double2 in the pipe) as double2.
@@ -936,7 +1401,7 @@ __device__ __forceinline__ double2 cov_elem(const float2* c, size_t k) {
 __device__ __forceinline__ double2 cov_elem(const double2* c, size_t k) { return c[k]; }
 
 template <int NT, int N, class CT>
-__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const CT* cov) {
+__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const CT* cov, float2* evecs32) {
     constexpr int LD = N + 1;  // padded columns: row and column walks spread over the banks
     __shared__ double2 A[N * LD], V[N * LD];
     __shared__ Rot rot[N / 2 > 0 ? N / 2 : 1];
@@ -950,7 +1415,7 @@ __global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs
         V[i + j * LD] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
-    jacobi_run<NT, N>(A, V, LD, evals, evecs, rot, red, order);
+    jacobi_run<NT, N>(A, V, LD, evals, evecs, rot, red, order, evecs32);
 }
 
 // T >= 128: A in the workspace, the rotations logged (JacobiLog) instead of
@@ -982,7 +1447,7 @@ __global__ void __launch_bounds__(NT) k_jacobi_global(const CT* cov, char* ws) {
     __syncthreads();
     const JacobiLog log{reinterpret_cast<Rot*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
                         reinterpret_cast<double*>(ws + L::diag_off)};
-    jacobi_run<NT, N, true>(A, nullptr, N, nullptr, nullptr, rot, red, order, log);
+    jacobi_run<NT, N, true>(A, nullptr, N, nullptr, nullptr, rot, red, order, nullptr, log);
 }
 
 // T = 128: the lower triangle of A packed in LDS (132 KB; the full matrix
@@ -1137,13 +1602,14 @@ __global__ void __launch_bounds__(256) k_jacobi_apply(const char* __restrict__ w
 }
 
 template <int NT, int N>
-__global__ void __launch_bounds__(NT) k_jacobi_finish(double* evals, double2* evecs, const char* __restrict__ ws) {
+__global__ void __launch_bounds__(NT) k_jacobi_finish(double* evals, double2* evecs, const char* __restrict__ ws,
+                                                      float2* evecs32) {
     using L = JacobiWs<N>;
     __shared__ int order[N];
     const double* diag = reinterpret_cast<const double*>(ws + L::diag_off);
     const double2* V = reinterpret_cast<const double2*>(ws + L::v_off);
     jacobi_finish<NT, N>([&](int j) { return diag[j]; }, [&](int k, int j) { return V[k + (size_t)j * N]; }, evals,
-                         evecs, order);
+                         evecs, order, evecs32);
 }
 
 size_t eig_workspace_bytes(int T) {
@@ -1158,7 +1624,8 @@ size_t eig_workspace_bytes(int T) {
 #endif
 
 template <int N, class CT>
-static hv_status eig_deferred(double* evals, double2* evecs, const CT* cov, char* ws, hipStream_t st) {
+static hv_status eig_deferred(double* evals, double2* evecs, const CT* cov, char* ws, hipStream_t st,
+                              float2* evecs32) {
     if constexpr (N == 128 && HV_JACOBI_PACKED) {
         constexpr size_t lds = jacobi_packed_lds(N);
         static const hipError_t attr = hipFuncSetAttribute(
@@ -1173,26 +1640,27 @@ static hv_status eig_deferred(double* evals, double2* evecs, const CT* cov, char
     }
     k_jacobi_apply<N><<<N / 4, 256, 0, st>>>(ws, ws);
     HV_CHECK_LAUNCH("k_jacobi_apply");
-    k_jacobi_finish<256, N><<<1, 256, 0, st>>>(evals, evecs, ws);
+    k_jacobi_finish<256, N><<<1, 256, 0, st>>>(evals, evecs, ws, evecs32);
     HV_CHECK_LAUNCH("k_jacobi_finish");
     return HV_OK;
 }
 
 template <class CT>
-static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T, void* work, hipStream_t st) {
+static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T, void* work, hipStream_t st,
+                            float2* e32) {
     switch (T) {
-        case 1: k_jacobi_lds<64, 1, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 2: k_jacobi_lds<64, 2, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 4: k_jacobi_lds<64, 4, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 8: k_jacobi_lds<64, 8, CT><<<1, 64, 0, st>>>(evals, evecs, cov); break;
-        case 16: k_jacobi_lds<256, 16, CT><<<1, 256, 0, st>>>(evals, evecs, cov); break;
-        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov); break;
-        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov); break;
+        case 1: k_jacobi_lds<64, 1, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 2: k_jacobi_lds<64, 2, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 4: k_jacobi_lds<64, 4, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 8: k_jacobi_lds<64, 8, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 16: k_jacobi_lds<256, 16, CT><<<1, 256, 0, st>>>(evals, evecs, cov, e32); break;
+        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov, e32); break;
+        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov, e32); break;
         case 128:
         case 256:
             if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
-            return T == 128 ? eig_deferred<128>(evals, evecs, cov, (char*)work, st)
-                            : eig_deferred<256>(evals, evecs, cov, (char*)work, st);
+            return T == 128 ? eig_deferred<128>(evals, evecs, cov, (char*)work, st, e32)
+                            : eig_deferred<256>(evals, evecs, cov, (char*)work, st, e32);
         default: return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
     }
     HV_CHECK_LAUNCH("k_jacobi");
@@ -1200,9 +1668,9 @@ static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T,
 }
 
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const void* cov, int cov_double, int T, void* work,
-                            hipStream_t st) {
-    return cov_double ? eig_jacobi(evals, evecs, (const double2*)cov, T, work, st)
-                      : eig_jacobi(evals, evecs, (const float2*)cov, T, work, st);
+                            hipStream_t st, float2* evecs32) {
+    return cov_double ? eig_jacobi(evals, evecs, (const double2*)cov, T, work, st, evecs32)
+                      : eig_jacobi(evals, evecs, (const float2*)cov, T, work, st, evecs32);
 }
 
 // ---------------------------------------------------------------------------

@@ -62,15 +62,20 @@ void bins_range(const hv_compute_settings& s, int& lo, int& K) {
     }
 }
 
-// B = S = T STFT image or moments with a space transform: the needed bins can
-// be taken before the propagation (the two are linear and commute).
+// B = S = T STFT or PCA image or moments with a space transform: the needed
+// bins (STFT) or components (PCA) can be taken before the propagation (both
+// are linear combinations of frames and commute with the linear propagator;
+// the PCA covariance of the propagated window is the raw window's up to a
+// constant, the propagators being unitary up to scale).
 bool bins_eligible(const hv_compute_settings& s) {
     if (!(s.space_transformation == HV_SPACE_ANGULARSP || s.space_transformation == HV_SPACE_FRESNELTR)) return false;
-    if (s.time_transformation != HV_TIME_STFT || !window_replaced(s)) return false;
+    if (s.time_transformation != HV_TIME_STFT && s.time_transformation != HV_TIME_PCA) return false;
+    if (!window_replaced(s)) return false;
     if (!is_image_kind(s.img_type) && !is_moment_kind(s.img_type)) return false;
     int lo, K;
     bins_range(s, lo, K);
     const int W = (int)s.width, H = (int)s.height, T = (int)s.time_transformation_size;
+    if (s.time_transformation == HV_TIME_PCA && !pca_raw_cov_supported(T, (size_t)W * H)) return false;
     return bins_first_supported(W, H, T, K, (int)s.depth) || bins_split_supported(W, H, T, K, (int)s.depth);
 }
 
@@ -85,7 +90,8 @@ int choose_path(const hv_compute_settings& s, int level) {
 }
 
 struct Layout {
-    size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, pca_ws, eig_ws;
+    size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, evecs32, pca_ws,
+        eig_ws;
     size_t accu, renorm, cuts, reg, outq;
     size_t total;
 };
@@ -112,6 +118,8 @@ Layout layout_of(const hv_compute_settings& s) {
         l.cov = align256(T * T * sizeof(double2));  // FP64 covariance for the FP64 eigensolver
         l.evals = align256(T * sizeof(double));
         l.evecs = align256(T * T * sizeof(double2));
+        // the bins-first projection's weights (+ 8 columns read past the last)
+        l.evecs32 = align256((T * T + 8) * sizeof(float2));
         l.pca_ws = align256(pca_workspace_bytes((int)T, P, s.pca_precision));
         l.eig_ws = align256(eig_workspace_bytes((int)T));
     }
@@ -125,7 +133,7 @@ Layout layout_of(const hv_compute_settings& s) {
     // the output queue: OutputBufferSize u16 images
     l.outq = align256((size_t)s.output_buffer_size * P * sizeof(uint16_t));
     l.total = l.outq + l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
-              l.evals + l.evecs + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts + l.reg;
+              l.evals + l.evecs + l.evecs32 + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts + l.reg;
     return l;
 }
 
@@ -152,6 +160,7 @@ struct Lane {
     double2* cov = nullptr;
     double* evals = nullptr;
     double2* evecs = nullptr;
+    float2* evecs32 = nullptr;
     void* pca_ws = nullptr;
     void* eig_ws = nullptr;
     float* image = nullptr;
@@ -194,6 +203,7 @@ struct hv_pipe {
     double2* cov = nullptr;
     double* evals = nullptr;
     double2* evecs = nullptr;
+    float2* evecs32 = nullptr;
     void* pca_ws = nullptr;
     void* eig_ws = nullptr;
     float* accu_ring = nullptr;     // img_accu_level images
@@ -333,8 +343,10 @@ hv_status hv_pipe::image_work() {
         int pmin, pmax;
         p_range(s, pmin, pmax);
         if (path == PATH_BINS) {
+            // PCA images are scaled by 2^17 - 1 after the mean (converts.cc:50-61)
+            const float scale = s.time_transformation == HV_TIME_PCA ? (float)((2 << 16) - 1) : 1.0f;
             HV_TRY(launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, pmax - pmin + 1,
-                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kPipeMm));
+                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kPipeMm, scale));
         } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
             HV_TRY(launch_row_ifft_stft_image(L.image, L.ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
                                               L.stream, kPipeMm));
@@ -470,9 +482,16 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     if (path == PATH_BINS) {
         int lo, K;
         bins_range(s, lo, K);
+        const float2* wts = nullptr;
+        if (s.time_transformation == HV_TIME_PCA) {
+            // covariance of the raw window, FP64 eigensolver, then components lo .. lo + K - 1 projected
+            HV_TRY(launch_pca_cov_raw(L.cov, frames, (int)s.depth, T, P, slot0, s.pca_precision, L.pca_ws, L.stream));
+            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, T, L.eig_ws, L.stream, L.evecs32));
+            wts = L.evecs32 + lo;
+        }
         HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, lo, K,
                                      s.space_transformation, lens, L.planes, L.stream, timing ? ev[1] : nullptr,
-                                     minmax_fused() ? L.mm : nullptr, kPipeMm));
+                                     minmax_fused() ? L.mm : nullptr, kPipeMm, wts));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
@@ -651,6 +670,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     p->cov = (double2*)take(l.cov);
     p->evals = (double*)take(l.evals);
     p->evecs = (double2*)take(l.evecs);
+    p->evecs32 = (float2*)take(l.evecs32);
     p->pca_ws = take(l.pca_ws);
     p->eig_ws = take(l.eig_ws);
     if (l.accu) {
@@ -674,6 +694,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     L0.cov = p->cov;
     L0.evals = p->evals;
     L0.evecs = p->evecs;
+    L0.evecs32 = p->evecs32;
     L0.pca_ws = p->pca_ws;
     L0.eig_ws = p->eig_ws;
     // refresh-time work: lens, manual contrast range, zeroed ring
@@ -684,6 +705,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
         e = hipMemcpyAsync(L0.range, r, sizeof(r), hipMemcpyHostToDevice, L0.stream);
         if (e == hipSuccess) e = hipMemsetAsync(p->ring, 0, l.ring, L0.stream);
         if (e == hipSuccess && p->accu_state) e = hipMemsetAsync(p->accu_state, 0, 256, L0.stream);
+        if (e == hipSuccess && p->evecs32) e = hipMemsetAsync(p->evecs32, 0, l.evecs32, L0.stream);
         if (e == hipSuccess && p->reg)
             e = hipMemsetAsync(p->reg, 0, l.reg - align256(p->P * sizeof(float)), L0.stream);
         // the disc is quadrant-swapped when exactly one of Fresnel and the fft shift holds (registration.cc:100-110)
@@ -1030,7 +1052,9 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         Lane n;
         const size_t planes = bins ? align256(K * p->P * sizeof(float2)) : 0;
         const size_t ring = own_ring ? l.ring : 0;
-        const size_t pca = own_ring ? l.cov + l.evals + l.evecs + l.pca_ws + l.eig_ws : 0;
+        // own PCA buffers: window-replaced three-pass lanes and bins-first PCA lanes
+        const bool own_pca = own_ring || (bins && p->s.time_transformation == HV_TIME_PCA);
+        const size_t pca = own_pca ? l.cov + l.evals + l.evecs + l.evecs32 + l.pca_ws + l.eig_ws : 0;
         const size_t total = planes + ring + pca + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct +
                              l.unwrap;
         hipError_t e = hipMalloc(&n.mem, total);
@@ -1054,11 +1078,12 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         };
         n.kind = kind;
         n.planes = (float2*)take(planes);
-        if (own_ring) {
-            n.ring = (float2*)take(ring);
+        if (own_ring) n.ring = (float2*)take(ring);
+        if (own_pca) {
             n.cov = (double2*)take(l.cov);
             n.evals = (double*)take(l.evals);
             n.evecs = (double2*)take(l.evecs);
+            n.evecs32 = (float2*)take(l.evecs32);
             n.pca_ws = take(l.pca_ws);
             n.eig_ws = take(l.eig_ws);
         } else if (!bins) {
@@ -1067,6 +1092,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
             n.cov = z.cov;
             n.evals = z.evals;
             n.evecs = z.evecs;
+            n.evecs32 = z.evecs32;
             n.pca_ws = z.pca_ws;
             n.eig_ws = z.eig_ws;
         }
@@ -1095,7 +1121,10 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
     if (!p || !ms || iters < 1) return fail(HV_ERR_INVALID_ARG, "null argument or iters < 1");
     if (p->path != PATH_BINS) return fail(HV_ERR_UNSUPPORTED, "stage timing covers the bins-first path");
     if (stage == 0 && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 0 needs input frames");
-    if (stage < 0 || stage > 4 || stage == 2) return fail(HV_ERR_INVALID_ARG, "stage must be 0, 1, 3 or 4");
+    const bool pca = p->s.time_transformation == HV_TIME_PCA;
+    if (stage < 0 || stage > 4 || (stage == 2 && !pca))
+        return fail(HV_ERR_INVALID_ARG, "stage must be 0, 1, 3 or 4 (2: PCA covariance + eigensolver)");
+    if (stage == 2 && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 2 needs input frames");
     HV_TRY(hv_pipe_sync(p));
     Lane& L = p->lane[0];
     const hv_compute_settings& s = p->s;
@@ -1111,7 +1140,14 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
         switch (stage) {
             case 0:
                 st = launch_bins_rows(frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
-                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kPipeMm);
+                                      K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kPipeMm,
+                                      pca ? L.evecs32 + pmin : nullptr);
+                break;
+            case 2:
+                st = launch_pca_cov_raw(L.cov, frames[i % n_frames], (int)s.depth, p->T, p->P, 0, s.pca_precision,
+                                        L.pca_ws, L.stream);
+                if (st == HV_OK)
+                    st = launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, p->T, L.eig_ws, L.stream, L.evecs32);
                 break;
             case 1:
                 st = launch_bins_columns(L.planes, (int)s.width, (int)s.height, K, s.space_transformation, p->lens,
@@ -1124,7 +1160,8 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
                                              kPipeMm);
                 else
                     st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
-                                           s.img_type, pmin, pmax, L.mm, L.stream, kPipeMm);
+                                           s.img_type, pmin, pmax, L.mm, L.stream, kPipeMm,
+                                           pca ? (float)((2 << 16) - 1) : 1.0f);
                 break;
             default:
                 st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, kPipeMm, s.contrast_invert, L.stream,

@@ -451,10 +451,14 @@ hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
  *              for the moments) of the raw window are taken first and only
  *              those K planes are propagated (the STFT and the propagation
  *              are linear and commute); the ring's first K frames hold the
- *              planes.  W <= 1024 and T <= 64 fuse the bins with the row
- *              FFTs (and, for images, the inverse row FFTs with the image
- *              term); larger frames and windows run them as passes of their
- *              own (T up to 256).
+ *              planes.  PCA likewise: the propagators are unitary up to a
+ *              constant, so the propagated window's covariance is the raw
+ *              window's up to that constant (same eigenvectors); it is taken
+ *              from the raw frames and the K selected components of the raw
+ *              window are propagated.  W <= 1024 and T <= 64 fuse the bins
+ *              with the row FFTs (and, for images, the inverse row FFTs with
+ *              the image term); larger frames and windows run them as passes
+ *              of their own (T up to 256).
  * hv_pipe_set_fused picks the highest eligible path at or below `level`
  * (default: 2); 0 forces the three-pass + STFT sequence.  hv_pipe_path
  * reports the path in use (0, 1 or 2). */
@@ -479,8 +483,9 @@ hv_status hv_pipe_path(const hv_pipe* p, int* path);
 hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
 
 /* No reference counterpart (measurement).  Mean milliseconds of one launch of
- * a bins-first stage -- 0 bins + row FFTs (cycling over the n_frames input
- * batches), 1 column pass, 3 inverse row FFT + image, 4 contrast + u16 --
+ * a bins-first stage -- 0 bins (or PCA components) + row FFTs (cycling over
+ * the n_frames input batches), 1 column pass, 2 (PCA) the raw window's
+ * covariance + eigensolver, 3 inverse row FFT + image, 4 contrast + u16 --
  * over `iters` launches back to back on lane 0's stream, between two HIP
  * events.  The pipe's buffers are overwritten; the next cycle rewrites them. */
 hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, int n_frames, int iters, float* ms);

@@ -26,13 +26,15 @@ def test_c1_full_size(oracle):
     check(run_both(oracle, s, fl), 1e-5)
 
 
+@pytest.mark.parametrize("level", [2, 0])
 @pytest.mark.parametrize("precision,tol", [(0, 1e-5), (1, 1e-4), (2, 1e-4)])
-def test_c2_full_size(oracle, precision, tol):
+def test_c2_full_size(oracle, precision, tol, level):
     """c2: 512^2 u8, Fresnel, PCA T = 32 (FP32 / split BF16 / split FP16
-    covariance, FP64 Jacobi), squared modulus of eigenvectors 4..31."""
-    print(f"c2 512x512 T=32 pca_precision={precision}", flush=True)
+    covariance, FP64 Jacobi), squared modulus of eigenvectors 4..31, on the
+    bins-first path (2) and in the reference's step order (0)."""
+    print(f"c2 512x512 T=32 pca_precision={precision} path {level}", flush=True)
     s, fl = frames("c2", 2, pca_precision=precision)
-    check(run_both(oracle, s, fl), tol)
+    check(run_both(oracle, s, fl, fused=level, path=level), tol)
 
 
 def test_c3_full_size(oracle):
@@ -80,7 +82,9 @@ def test_c5_pca_full_size(oracle):
     """c5 PCA variant: 2048^2 u16, B = S = T = 128, angular spectrum, PCA
     with the covariance in plain FP32, split BF16 and split FP16 (the
     configured one), FP64 Jacobi; moment 0 and moment 1 of components 4..60
-    against the FP64 oracle at 1e-4 relative L2.  The window's spectrum has
+    on the bins-first path (covariance of the raw window, 57 components
+    propagated) and in the reference's step order, against the FP64 oracle
+    at 1e-4 relative L2.  The window's spectrum has
     13 near-degenerate noise eigenvalues 1e-9 lambda_max apart and the range
     starts among them, so this bar needs the covariance centred and
     assembled in FP64.  The covariance (step function, float output) and the
@@ -101,24 +105,26 @@ def test_c5_pca_full_size(oracle):
     torch.cuda.synchronize()
     errs = {}
     for precision in (0, 1, 2):
-        for k in (0, 1):
-            print(f"c5_pca pca_precision={precision} moment {k}", flush=True)
-            s2 = type(s).from_buffer_copy(s)
-            s2.pca_precision = precision
-            s2.img_type = HV_IMG_MOMENTS_0 + k
-            p = Pipe(s2)
-            assert p.push(dev)
-            errs[(precision, k)] = rel_l2(p.image().cpu().numpy(), m_ref[k])
-            if k == 0:
-                ring, _, _ = p.ring()
-                cov = st.pca_cov(ring, precision).cpu().numpy()
-                ecov = rel_l2(cov, cov64)
-                ev, _ = st.eig_hermitian(torch.from_numpy(cov).cuda())
-                eev = float(np.abs(ev.cpu().numpy() - ev64).max() / ev64[-1])
-                print(f"  covariance rel L2 {ecov:.3g}, eigenvalues max err / lambda_max {eev:.3g}", flush=True)
-                assert ecov < 2e-6 and eev < 1e-6, (precision, ecov, eev)
-                del ring
-            p.close()
-            print(f"  image rel L2 vs FP64 {errs[(precision, k)]:.3g}", flush=True)
+        for level in (2, 0):
+            for k in (0, 1):
+                print(f"c5_pca pca_precision={precision} path {level} moment {k}", flush=True)
+                s2 = type(s).from_buffer_copy(s)
+                s2.pca_precision = precision
+                s2.img_type = HV_IMG_MOMENTS_0 + k
+                p = Pipe(s2, fused=level)
+                assert p.path == level
+                assert p.push(dev)
+                errs[(precision, level, k)] = rel_l2(p.image().cpu().numpy(), m_ref[k])
+                if level == 0 and k == 0:  # the three-pass ring holds the propagated window
+                    ring, _, _ = p.ring()
+                    cov = st.pca_cov(ring, precision).cpu().numpy()
+                    ecov = rel_l2(cov, cov64)
+                    ev, _ = st.eig_hermitian(torch.from_numpy(cov).cuda())
+                    eev = float(np.abs(ev.cpu().numpy() - ev64).max() / ev64[-1])
+                    print(f"  covariance rel L2 {ecov:.3g}, eigenvalues max err / lambda_max {eev:.3g}", flush=True)
+                    assert ecov < 2e-6 and eev < 1e-6, (precision, ecov, eev)
+                    del ring
+                p.close()
+                print(f"  image rel L2 vs FP64 {errs[(precision, level, k)]:.3g}", flush=True)
     for key, e in errs.items():
         assert e < 1e-4, (key, e)

@@ -32,6 +32,9 @@ def run_both(oracle, settings, frames_list, graph=False, fused=True, path=None):
     o = oracle.OraclePipe(settings, 64)
     results = []
     for fr in frames_list:
+        # the kernels read batch_size frames of `depth` bytes per pixel: a mismatched array would be read past its end
+        assert fr.dtype.itemsize == settings.depth and fr.size == settings.batch_size * settings.width * \
+            settings.height, (fr.dtype, fr.shape, settings.depth)
         t = torch.from_numpy(fr.view(np.int16) if fr.dtype == np.uint16 else fr).cuda()
         torch.cuda.synchronize()
         made = p.push(t)
@@ -45,13 +48,13 @@ def run_both(oracle, settings, frames_list, graph=False, fused=True, path=None):
     return results
 
 
-def check(results, img_tol, u16_frac=0.95):
+def check(results, img_tol, u16_frac=0.95, u16_max=1):
     assert results, "no image produced"
     for img, u, rng, ref in results:
         e = rel_l2(img, ref["image"])
         assert e < img_tol, e
         d = np.abs(u.astype(np.int64) - ref["u16"].astype(np.int64))
-        assert d.max() <= 1 and (d == 0).mean() >= u16_frac, (d.max(), (d == 0).mean())
+        assert d.max() <= u16_max and (d == 0).mean() >= u16_frac, (d.max(), (d == 0).mean())
 
 
 def frames(name, n_batches, **over):
@@ -143,11 +146,53 @@ def test_pipe_space_and_image_types(oracle, img_type, space):
     check(run_both(oracle, s, fl), 1e-5 if img_type != 2 else 1e-4)
 
 
+@pytest.mark.parametrize("level", [2, 0])
 @pytest.mark.parametrize("precision,tol", [(0, 1e-5), (1, 1e-4), (2, 1e-4)])
-def test_pipe_pca_fresnel(oracle, precision, tol):
-    """c2 shape at 128 x 128: Fresnel + PCA (T=32), eigenvectors 4..31, squared modulus."""
+def test_pipe_pca_fresnel(oracle, precision, tol, level):
+    """c2 shape at 128 x 128: Fresnel + PCA (T=32), eigenvectors 4..31,
+    squared modulus -- bins-first (2: covariance of the raw window, the 28
+    selected components propagated) and the reference's step order (0)."""
     s, fl = frames("c2", 2, width=128, height=128, pca_precision=precision)
-    check(run_both(oracle, s, fl), tol, u16_frac=0.95)
+    check(run_both(oracle, s, fl, fused=level, path=level), tol, u16_frac=0.95)
+
+
+@pytest.mark.parametrize("W,H,T,depth,space,p0,pw,precision", [
+    (64, 64, 8, 2, 2, 2, 5, 0),       # angular spectrum, u16 (fused row kernel)
+    (128, 64, 16, 1, 1, 4, 11, 1),    # Fresnel, u8, split BF16
+    (128, 128, 64, 2, 2, 4, 59, 2),   # T = 64 (grouped covariance, fused rows), split FP16
+    (2048, 16, 16, 2, 2, 4, 11, 1),   # W = 2048: projection, row and image as passes of their own
+])
+def test_pipe_pca_bins_shapes(oracle, W, H, T, depth, space, p0, pw, precision):
+    """The bins-first PCA path against the FP64 oracle pipe (which propagates
+    every frame and takes the covariance of the propagated ring) on the
+    synthetic interferograms, squared modulus of components p0 .. T - 1:
+    integer depths, both propagators, the single-group and grouped covariance
+    kernels, fused and split projection.  (The sum of squared moduli depends
+    only on the components' span; the interferograms' time covariance has
+    near-degenerate pairs whose individual eigenvectors are not determined.)"""
+    syn = dict(syn_depth=2, syn_bits=12, syn_mean_counts=1024.0) if depth == 2 else {}  # c2's generator is u8
+    s, fl = frames("c2", 2, width=W, height=H, depth=depth, batch_size=T, time_stride=T,
+                   time_transformation_size=T, space_transformation=space, img_type=1, p_start=p0,
+                   p_width=pw, pca_precision=precision, z_distance=0.25 if space == 2 else 0.4, syn_period=T, **syn)
+    print(f"PCA bins {W}x{H} T={T} depth={depth} space={space} prec={precision}", flush=True)
+    check(run_both(oracle, s, fl, path=2), 1e-5 if precision == 0 else 1e-4)
+
+
+@pytest.mark.parametrize("space", [1, 2])
+@pytest.mark.parametrize("img_type,tol", [(0, 1e-5), (1, 1e-5), (2, 1e-4)])
+def test_pipe_pca_bins_image_types(oracle, space, img_type, tol):
+    """Modulus, squared modulus and argument of components 3 .. 8 of T = 16
+    on the bins-first PCA path, on frames with a designed, well-separated
+    time spectrum (the individual eigenvectors are then determined).  The
+    argument image (mean phase of the projections) is within 1e-4 relative
+    L2; its u16 image within 4 counts."""
+    from holovibes_amd.configs import settings_for
+    from holovibes_amd.constants import HV_DEPTH_32F
+    T = 16
+    s = settings_for("c2", width=64, height=32, depth=HV_DEPTH_32F, batch_size=T, time_stride=T,
+                     time_transformation_size=T, space_transformation=space, img_type=img_type, p_start=3,
+                     p_width=5, pca_precision=0, z_distance=0.25 if space == 2 else 0.4)
+    check(run_both(oracle, s, [designed_frames(T, 64, 32, 5)], path=2), tol, u16_max=4 if img_type == 2 else 1)
 
 
 def test_pipe_unwrap_percentile_subsample(oracle):
@@ -218,9 +263,10 @@ def designed_frames(T, W, H, seed):
     return (X * 1000).astype(np.float32).reshape(T, H, W)
 
 
+@pytest.mark.parametrize("level", [2, 0])
 @pytest.mark.parametrize("img_type", [5, 6, 7])
 @pytest.mark.parametrize("T,precision,tol", [(32, 0, 1e-5), (128, 0, 1e-5), (128, 2, 1e-4)])
-def test_pipe_pca_moments(oracle, img_type, T, precision, tol):
+def test_pipe_pca_moments(oracle, img_type, T, precision, tol, level):
     """c5 PCA variant at 32 x 32: moments of the projections onto
     eigenvectors f_start..f_end (register kernel at T = 32, chunked projection
     over two component chunks at T = 128).  The synthetic interferograms'
@@ -232,7 +278,7 @@ def test_pipe_pca_moments(oracle, img_type, T, precision, tol):
     s = settings_for("c5_pca", width=32, height=32, depth=HV_DEPTH_32F, batch_size=T, time_stride=T,
                      time_transformation_size=T, img_type=img_type, p_start=4, p_width=min(56, T - 5),
                      camera_fps=float(T), pca_precision=precision)
-    check(run_both(oracle, s, [designed_frames(T, 32, 32, 17)]), tol)
+    check(run_both(oracle, s, [designed_frames(T, 32, 32, 17)], fused=level, path=level), tol)
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -277,14 +323,21 @@ def test_pipe_metric_full_size(oracle, level):
 
 @pytest.mark.parametrize("lanes", [2, 3])
 @pytest.mark.parametrize("contrast_mode", [0, 2])
-def test_pipe_lanes_match_one_lane(lanes, contrast_mode):
+@pytest.mark.parametrize("kind", ["stft", "pca"])
+def test_pipe_lanes_match_one_lane(lanes, contrast_mode, kind):
     """Cycles side by side on several lanes give the one-lane outputs,
     batch by batch and after a burst of pushes without synchronisation
-    (min/max contrast per cycle; autocontrast frozen after the first)."""
+    (min/max contrast per cycle; autocontrast frozen after the first); STFT
+    bins and PCA components (each lane with its own covariance, eigensolver
+    and projection buffers)."""
     from holovibes_amd.pipe import Pipe
-    s, fl = frames("c1", 6, width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16,
-                   contrast_mode=contrast_mode, contrast_lower=0.5, contrast_upper=99.5)
-    dev = [torch.from_numpy(f.view(np.int16)).cuda() for f in fl]
+    if kind == "stft":
+        s, fl = frames("c1", 6, width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16,
+                       contrast_mode=contrast_mode, contrast_lower=0.5, contrast_upper=99.5)
+    else:
+        s, fl = frames("c2", 6, width=128, height=128, contrast_mode=contrast_mode, contrast_lower=0.5,
+                       contrast_upper=99.5)
+    dev = [torch.from_numpy(f.view(np.int16) if f.dtype == np.uint16 else f).cuda() for f in fl]
     torch.cuda.synchronize()
     one, many = Pipe(s, graph=True), Pipe(s, graph=True, lanes=lanes)
     assert one.path == Pipe.BINS and many.path == Pipe.BINS
@@ -312,8 +365,8 @@ def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
     cfg = {"stft": ("c1", 0, dict(width=128, height=128, batch_size=16, time_stride=16, time_transformation_size=16)),
            "row_stft": ("c1", 1, dict(width=128, height=128, batch_size=16, time_stride=16,
                                       time_transformation_size=16)),
-           "pca": ("c2", 2, dict(width=128, height=128)),
-           "pca_moments": ("c5_pca", 2, dict(width=64, height=64, batch_size=32, time_stride=32,
+           "pca": ("c2", 0, dict(width=128, height=128)),
+           "pca_moments": ("c5_pca", 0, dict(width=64, height=64, batch_size=32, time_stride=32,
                                              time_transformation_size=32, p_start=4, p_width=20, camera_fps=32.0,
                                              syn_period=32)),
            "unwrap": ("c3", 0, dict(width=128, height=128, batch_size=16, time_stride=16,
