@@ -134,9 +134,10 @@ hv_status launch_pca_cov_raw(double2* cov, const void* raw, int depth, int T, si
                              void* ws, hipStream_t st);
 size_t eig_workspace_bytes(int T);
 // evecs32 (optional): the eigenvectors again as float2, row-major (component
-// k of eigenvector j at k T + j)
+// k of eigenvector j at k T + j).  real: cov is real symmetric (its imaginary
+// parts are ignored) and the rotations real (T <= 128).
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const void* cov, int cov_double, int T, void* work,
-                            hipStream_t st, float2* evecs32 = nullptr);
+                            hipStream_t st, float2* evecs32 = nullptr, int real = 0);
 // img_type: an image type (mean over [pmin, pmax], x131071) or HV_IMG_MOMENTS_k
 // (weights from fps over [pmin, pmax] = [f_start, f_end])
 hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs, int T, size_t P, int img_type,

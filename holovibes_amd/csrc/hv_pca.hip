@@ -1257,6 +1257,58 @@ __device__ __forceinline__ void rot_left(const Rot& R, double2& yp, double2& yq)
     yq = nq_;
 }
 
+// Real symmetric matrices (the raw-window covariance): the same rotation with
+// e = sgn(a_pq) folded into s, J = [[c, s], [-s, c]]; every update real.
+struct RotR {
+    double c, s;
+};
+__device__ __forceinline__ RotR make_rot(double app, double aqq, double apq) {
+    RotR R;
+    if (apq == 0.0) {
+        R.c = 1.0;
+        R.s = 0.0;
+    } else {
+        const double h = aqq - app, ah = fabs(h);
+        const double g = 2.0 * fabs(apq);
+        const double r = sqrt(fma(h, h, g * g));
+        const double u = HV_JACOBI_RSQRT ? rsqrt(2.0 * r * (r + ah)) : 1.0 / sqrt(2.0 * r * (r + ah));
+        R.c = (r + ah) * u;
+        const double s = g * u;
+        R.s = ((h >= 0) == (apq >= 0)) ? s : -s;  // sgn(h) sgn(a_pq) g u
+    }
+    return R;
+}
+__device__ __forceinline__ void rot_right(const RotR& R, double& xp, double& xq) {
+    const double np_ = xp * R.c - xq * R.s, nq_ = xp * R.s + xq * R.c;
+    xp = np_;
+    xq = nq_;
+}
+__device__ __forceinline__ void rot_left(const RotR& R, double& yp, double& yq) { rot_right(R, yp, yq); }
+
+// Element helpers of the two matrix kinds (complex double2, real double).
+__device__ __forceinline__ double el_re(double2 x) { return x.x; }
+__device__ __forceinline__ double el_re(double x) { return x; }
+__device__ __forceinline__ double el_norm2(double2 x) { return x.x * x.x + x.y * x.y; }
+__device__ __forceinline__ double el_norm2(double x) { return x * x; }
+__device__ __forceinline__ void el_zero(double2& x) { x = make_double2(0.0, 0.0); }
+__device__ __forceinline__ void el_zero(double& x) { x = 0.0; }
+__device__ __forceinline__ double2 el_conj(double2 x) { return make_double2(x.x, -x.y); }
+__device__ __forceinline__ double el_conj(double x) { return x; }
+__device__ __forceinline__ void el_one(double2& x) { x = make_double2(1.0, 0.0); }
+__device__ __forceinline__ void el_one(double& x) { x = 1.0; }
+__device__ __forceinline__ void el_real(double2& x) { x.y = 0.0; }
+__device__ __forceinline__ void el_real(double&) {}
+__device__ __forceinline__ double2 el_c(double2 x) { return x; }
+__device__ __forceinline__ double2 el_c(double x) { return make_double2(x, 0.0); }
+template <class E>
+using RotOf = decltype(make_rot(0.0, 0.0, E{}));
+// x times the unit phase that makes ph real positive
+__device__ __forceinline__ double2 el_phase_norm(double2 x, double2 ph) {
+    const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
+    return zmul(x, make_double2(ph.x * inv, -ph.y * inv));
+}
+__device__ __forceinline__ double2 el_phase_norm(double x, double ph) { return make_double2(ph < 0.0 ? -x : x, 0.0); }
+
 // Eigenvalues ascending (stable ranks: ties keep their index order) and
 // phase-normalised eigenvectors -- the largest component real positive, the
 // first of equal magnitudes within a relative 1e-12 -- from the converged
@@ -1283,19 +1335,16 @@ __device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int*
         double best = -1.0;
 #pragma unroll 4
         for (int k = 0; k < n; ++k) {
-            const double2 x = v(k, j);
-            const double m2 = x.x * x.x + x.y * x.y;
+            const double m2 = el_norm2(v(k, j));
             if (m2 > best * (1 + 2e-12)) {
                 best = m2;
                 kmax = k;
             }
         }
-        const double2 ph = v(kmax, j);
-        const double inv = 1.0 / sqrt(ph.x * ph.x + ph.y * ph.y);
-        const double2 rotp = make_double2(ph.x * inv, -ph.y * inv);
+        const auto ph = v(kmax, j);
 #pragma unroll 4
         for (int k = 0; k < n; ++k) {
-            const double2 z = zmul(v(k, j), rotp);
+            const double2 z = el_phase_norm(v(k, j), ph);
             evecs[k + (size_t)jj * n] = z;
             // row-major float copy: one component of consecutive eigenvectors is contiguous
             if (evecs32) evecs32[(size_t)k * n + jj] = make_float2((float)z.x, (float)z.y);
@@ -1306,34 +1355,30 @@ __device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int*
 // Rotation log of a deferred run: every round's n/2 rotations in order, then
 // the round count; the eigenvectors are accumulated afterwards row by row
 // (k_jacobi_apply), which many workgroups do in parallel.
+template <class R = Rot>
 struct JacobiLog {
-    Rot* rot;       // HV_JACOBI_MAX_SWEEPS (n - 1) rounds of n / 2
+    R* rot;         // HV_JACOBI_MAX_SWEEPS (n - 1) rounds of n / 2
     int* rounds;
     double* diag;   // converged diagonal, unordered
 };
 
-template <int NT, int N, bool DEFER = false>
-__device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double2* evecs, Rot* rot, double* red,
-                           int* order, float2* evecs32, JacobiLog log = JacobiLog{}) {
+template <int NT, int N, bool DEFER = false, class E = double2>
+__device__ void jacobi_run(E* A, E* V, int ld, double* evals, double2* evecs, RotOf<E>* rot, double* red,
+                           int* order, float2* evecs32, JacobiLog<RotOf<E>> log = JacobiLog<RotOf<E>>{}) {
+    using R = RotOf<E>;
     constexpr int n = N, half = N / 2;
     const int tid = threadIdx.x;
     int g = 0;  // rounds done
-    auto a = [&](int i, int j) -> double2& { return A[i + (size_t)j * ld]; };
-    auto v = [&](int i, int j) -> double2& { return V[i + (size_t)j * ld]; };
+    auto a = [&](int i, int j) -> E& { return A[i + (size_t)j * ld]; };
+    auto v = [&](int i, int j) -> E& { return V[i + (size_t)j * ld]; };
     double fro = 0.0;
-    for (int e = tid; e < n * n; e += NT) {
-        const double2 x = a(e % n, e / n);
-        fro += x.x * x.x + x.y * x.y;
-    }
+    for (int e = tid; e < n * n; e += NT) fro += el_norm2(a(e % n, e / n));
     fro = block_sum<NT>(fro, red);
     for (int sweep = 0; sweep < HV_JACOBI_MAX_SWEEPS && n > 1; ++sweep) {
         double off = 0.0;
         for (int e = tid; e < n * n; e += NT) {
             const int i = e % n, j = e / n;
-            if (i != j) {
-                const double2 x = a(i, j);
-                off += x.x * x.x + x.y * x.y;
-            }
+            if (i != j) off += el_norm2(a(i, j));
         }
         off = block_sum<NT>(off, red);
         // relative off-diagonal norm below 1e-13: far under the FP32 input's own precision
@@ -1342,9 +1387,9 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
             for (int k = tid; k < half; k += NT) {
                 int p, q;
                 rr_pair(n, r, k, p, q);
-                const Rot R = make_rot(a(p, p).x, a(q, q).x, a(p, q));
-                rot[k] = R;
-                if constexpr (DEFER) log.rot[(size_t)g * half + k] = R;
+                const R rk = make_rot(el_re(a(p, p)), el_re(a(q, q)), a(p, q));
+                rot[k] = rk;
+                if constexpr (DEFER) log.rot[(size_t)g * half + k] = rk;
             }
             ++g;
             __syncthreads();
@@ -1354,17 +1399,17 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
                 int p, q, pp, qq;
                 rr_pair(n, r, k, p, q);
                 rr_pair(n, r, kk, pp, qq);
-                const Rot Rr = rot[k], Rc = rot[kk];
-                double2 b00 = a(p, pp), b01 = a(p, qq), b10 = a(q, pp), b11 = a(q, qq);
+                const R Rr = rot[k], Rc = rot[kk];
+                E b00 = a(p, pp), b01 = a(p, qq), b10 = a(q, pp), b11 = a(q, qq);
                 rot_right(Rc, b00, b01);
                 rot_right(Rc, b10, b11);
                 rot_left(Rr, b00, b10);
                 rot_left(Rr, b01, b11);
                 if (k == kk) {  // the annihilated pair: exact zeros, real diagonal
-                    b01 = make_double2(0.0, 0.0);
-                    b10 = make_double2(0.0, 0.0);
-                    b00.y = 0.0;
-                    b11.y = 0.0;
+                    el_zero(b01);
+                    el_zero(b10);
+                    el_real(b00);
+                    el_real(b11);
                 }
                 a(p, pp) = b00;
                 a(p, qq) = b01;
@@ -1376,7 +1421,7 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
                     const int m = task % n, kk = task / n;
                     int pp, qq;
                     rr_pair(n, r, kk, pp, qq);
-                    double2 x = v(m, pp), y = v(m, qq);
+                    E x = v(m, pp), y = v(m, qq);
                     rot_right(rot[kk], x, y);
                     v(m, pp) = x;
                     v(m, qq) = y;
@@ -1386,11 +1431,11 @@ __device__ void jacobi_run(double2* A, double2* V, int ld, double* evals, double
         }
     }
     if constexpr (DEFER) {
-        for (int j = tid; j < n; j += NT) log.diag[j] = a(j, j).x;
+        for (int j = tid; j < n; j += NT) log.diag[j] = el_re(a(j, j));
         if (tid == 0) *log.rounds = g;
         return;
     }
-    jacobi_finish<NT, N>([&](int j) { return a(j, j).x; }, v, evals, evecs, order, evecs32);
+    jacobi_finish<NT, N>([&](int j) { return el_re(a(j, j)); }, v, evals, evecs, order, evecs32);
 }
 
 // A covariance element (float2 from the C-ABI step, double2 in the pipe) as double2.
@@ -1400,22 +1445,29 @@ __device__ __forceinline__ double2 cov_elem(const float2* c, size_t k) {
 }
 __device__ __forceinline__ double2 cov_elem(const double2* c, size_t k) { return c[k]; }
 
-template <int NT, int N, class CT>
+// E = double: the real part of the covariance only (a real symmetric matrix).
+__device__ __forceinline__ void set_cov_el(double2& dst, double2 c, int i, int j) {
+    dst = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
+}
+__device__ __forceinline__ void set_cov_el(double& dst, double2 c, int, int) { dst = c.x; }
+
+template <int NT, int N, class CT, class E = double2>
 __global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const CT* cov, float2* evecs32) {
     constexpr int LD = N + 1;  // padded columns: row and column walks spread over the banks
-    __shared__ double2 A[N * LD], V[N * LD];
-    __shared__ Rot rot[N / 2 > 0 ? N / 2 : 1];
+    __shared__ E A[N * LD], V[N * LD];
+    __shared__ RotOf<E> rot[N / 2 > 0 ? N / 2 : 1];
     __shared__ double red[NT / 64];
     __shared__ int order[N];
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
         // Hermitian from the lower triangle (CUBLAS_FILL_MODE_LOWER)
         const double2 c = i >= j ? cov_elem(cov, i + (size_t)j * N) : cov_elem(cov, j + (size_t)i * N);
-        A[i + j * LD] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
-        V[i + j * LD] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        set_cov_el(A[i + j * LD], c, i, j);
+        el_zero(V[i + j * LD]);
+        if (i == j) el_one(V[i + j * LD]);
     }
     __syncthreads();
-    jacobi_run<NT, N>(A, V, LD, evals, evecs, rot, red, order, evecs32);
+    jacobi_run<NT, N, false, E>(A, V, LD, evals, evecs, rot, red, order, evecs32);
 }
 
 // T >= 128: A in the workspace, the rotations logged (JacobiLog) instead of
@@ -1445,53 +1497,50 @@ __global__ void __launch_bounds__(NT) k_jacobi_global(const CT* cov, char* ws) {
         A[e] = make_double2(c.x, i == j ? 0.0 : (i > j ? c.y : -c.y));
     }
     __syncthreads();
-    const JacobiLog log{reinterpret_cast<Rot*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
-                        reinterpret_cast<double*>(ws + L::diag_off)};
-    jacobi_run<NT, N, true>(A, nullptr, N, nullptr, nullptr, rot, red, order, nullptr, log);
+    const JacobiLog<Rot> log{reinterpret_cast<Rot*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
+                             reinterpret_cast<double*>(ws + L::diag_off)};
+    jacobi_run<NT, N, true, double2>(A, nullptr, N, nullptr, nullptr, rot, red, order, nullptr, log);
 }
 
 // T = 128: the lower triangle of A packed in LDS (132 KB; the full matrix
 // does not fit).  A round's 2 x 2 blocks (k, kk) and (kk, k) hold conjugate
 // elements, so only blocks with k >= kk are updated: every stored element
 // once per round.  Rotations go to the log as in k_jacobi_global.
-template <int NT, int N, class CT>
+template <int NT, int N, class CT, class E = double2>
 __global__ void __launch_bounds__(NT) k_jacobi_packed(const CT* cov, char* ws) {
     using L = JacobiWs<N>;
+    using R = RotOf<E>;
     constexpr int n = N, half = N / 2, nblocks = half * (half + 1) / 2;
-    extern __shared__ double2 Lt[];  // element (i, j), i >= j, at i (i + 1) / 2 + j
-    __shared__ Rot rot[half];
+    extern __shared__ double2 Lraw[];  // element (i, j), i >= j, at i (i + 1) / 2 + j
+    E* Lt = reinterpret_cast<E*>(Lraw);
+    __shared__ R rot[half];
     __shared__ double red[NT / 64];
     const int tid = threadIdx.x;
-    auto at = [&](int i, int j) -> double2& { return Lt[i * (i + 1) / 2 + j]; };
-    auto get = [&](int i, int j) -> double2 {
+    auto at = [&](int i, int j) -> E& { return Lt[i * (i + 1) / 2 + j]; };
+    auto get = [&](int i, int j) -> E {
         if (i >= j) return at(i, j);
-        const double2 x = at(j, i);
-        return make_double2(x.x, -x.y);
+        return el_conj(at(j, i));
     };
-    auto set = [&](int i, int j, double2 v) {
+    auto set = [&](int i, int j, E v) {
         if (i >= j) at(i, j) = v;
-        else at(j, i) = make_double2(v.x, -v.y);
+        else at(j, i) = el_conj(v);
     };
     // lower triangle of cov (CUBLAS_FILL_MODE_LOWER), real diagonal
     for (int i = 0; i < n; ++i)
-        for (int j = tid; j <= i; j += NT) {
-            const double2 c = cov_elem(cov, i + (size_t)j * n);
-            at(i, j) = make_double2(c.x, i == j ? 0.0 : c.y);
-        }
+        for (int j = tid; j <= i; j += NT) set_cov_el(at(i, j), cov_elem(cov, i + (size_t)j * n), i, j);
     __syncthreads();
     auto tri_sum = [&](bool with_diag) {
         double acc = 0.0;
         for (int i = 0; i < n; ++i)
             for (int j = tid; j <= i; j += NT) {
-                const double2 x = at(i, j);
-                const double m2 = x.x * x.x + x.y * x.y;
+                const double m2 = el_norm2(at(i, j));
                 acc += i == j ? (with_diag ? m2 : 0.0) : 2.0 * m2;
             }
         return block_sum<NT>(acc, red);
     };
     const double fro = tri_sum(true);
-    const JacobiLog log{reinterpret_cast<Rot*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
-                        reinterpret_cast<double*>(ws + L::diag_off)};
+    const JacobiLog<R> log{reinterpret_cast<R*>(ws + L::log_off), reinterpret_cast<int*>(ws + L::rounds_off),
+                           reinterpret_cast<double*>(ws + L::diag_off)};
     // this thread's blocks (k, kk), k >= kk, with task = k (k + 1) / 2 + kk (the same every round)
     constexpr int kTasks = (nblocks + NT - 1) / NT;
     int tk[kTasks], tkk[kTasks];
@@ -1515,16 +1564,16 @@ __global__ void __launch_bounds__(NT) k_jacobi_packed(const CT* cov, char* ws) {
             for (int k = tid; k < half; k += NT) {
                 int p, q;
                 rr_pair(n, r, k, p, q);
-                const Rot R = make_rot(at(p, p).x, at(q, q).x, get(p, q));
-                rot[k] = R;
-                log.rot[(size_t)g * half + k] = R;
+                const R rk = make_rot(el_re(at(p, p)), el_re(at(q, q)), get(p, q));
+                rot[k] = rk;
+                log.rot[(size_t)g * half + k] = rk;
             }
             ++g;
             __syncthreads();
             // a round's blocks touch disjoint elements: all loads, then all
             // rotations, then all stores, so that the LDS latencies overlap
             int p[kTasks], q[kTasks], pp[kTasks], qq[kTasks];
-            double2 b[kTasks][4];
+            E b[kTasks][4];
 #pragma unroll
             for (int u = 0; u < kTasks; ++u) {
                 if (tk[u] < 0) continue;
@@ -1538,16 +1587,16 @@ __global__ void __launch_bounds__(NT) k_jacobi_packed(const CT* cov, char* ws) {
 #pragma unroll
             for (int u = 0; u < kTasks; ++u) {
                 if (tk[u] < 0) continue;
-                const Rot Rr = rot[tk[u]], Rc = rot[tkk[u]];
+                const R Rr = rot[tk[u]], Rc = rot[tkk[u]];
                 rot_right(Rc, b[u][0], b[u][1]);
                 rot_right(Rc, b[u][2], b[u][3]);
                 rot_left(Rr, b[u][0], b[u][2]);
                 rot_left(Rr, b[u][1], b[u][3]);
                 if (tk[u] == tkk[u]) {  // the annihilated pair: exact zeros, real diagonal
-                    b[u][1] = make_double2(0.0, 0.0);
-                    b[u][2] = make_double2(0.0, 0.0);
-                    b[u][0].y = 0.0;
-                    b[u][3].y = 0.0;
+                    el_zero(b[u][1]);
+                    el_zero(b[u][2]);
+                    el_real(b[u][0]);
+                    el_real(b[u][3]);
                 }
             }
 #pragma unroll
@@ -1561,27 +1610,32 @@ __global__ void __launch_bounds__(NT) k_jacobi_packed(const CT* cov, char* ws) {
             __syncthreads();
         }
     }
-    for (int j = tid; j < n; j += NT) log.diag[j] = at(j, j).x;
+    for (int j = tid; j < n; j += NT) log.diag[j] = el_re(at(j, j));
     if (tid == 0) *log.rounds = g;
 }
 
-constexpr size_t jacobi_packed_lds(int n) { return (size_t)n * (n + 1) / 2 * sizeof(double2); }
+template <class E = double2>
+constexpr size_t jacobi_packed_lds(int n) { return (size_t)n * (n + 1) / 2 * sizeof(E); }
 
 // V = J_1 J_2 ... J_G row by row: row m of the identity times the logged
 // rotations, one wavefront per row in LDS (a round's rotations touch
 // disjoint pairs; the wavefront's LDS accesses complete in order, so rounds
 // need only a fence).
-template <int N>
+template <int N, class E = double2>
 __global__ void __launch_bounds__(256) k_jacobi_apply(const char* __restrict__ ws_in, char* ws) {
     using L = JacobiWs<N>;
+    using R = RotOf<E>;
     constexpr int half = N / 2;
-    __shared__ double2 rows[4][N];
+    __shared__ E rows[4][N];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = blockIdx.x * 4 + w;
-    double2* row = rows[w];
-    const Rot* log = reinterpret_cast<const Rot*>(ws_in + L::log_off);
+    E* row = rows[w];
+    const R* log = reinterpret_cast<const R*>(ws_in + L::log_off);
     const int G = *reinterpret_cast<const int*>(ws_in + L::rounds_off);
-    for (int j = lane; j < N; j += 64) row[j] = make_double2(j == m ? 1.0 : 0.0, 0.0);
+    for (int j = lane; j < N; j += 64) {
+        el_zero(row[j]);
+        if (j == m) el_one(row[j]);
+    }
     fft_sync<true>();
     for (int g = 0; g < G; ++g) {
         const int r = g % (N - 1);
@@ -1589,25 +1643,25 @@ __global__ void __launch_bounds__(256) k_jacobi_apply(const char* __restrict__ w
         for (int k = lane; k < half; k += 64) {
             int p, q;
             rr_pair(N, r, k, p, q);
-            const Rot R = log[(size_t)g * half + k];
-            double2 x = row[p], y = row[q];
-            rot_right(R, x, y);
+            const R rk = log[(size_t)g * half + k];
+            E x = row[p], y = row[q];
+            rot_right(rk, x, y);
             row[p] = x;
             row[q] = y;
         }
         fft_sync<true>();
     }
-    double2* V = reinterpret_cast<double2*>(ws + L::v_off);
+    E* V = reinterpret_cast<E*>(ws + L::v_off);
     for (int j = lane; j < N; j += 64) V[m + (size_t)j * N] = row[j];
 }
 
-template <int NT, int N>
+template <int NT, int N, class E = double2>
 __global__ void __launch_bounds__(NT) k_jacobi_finish(double* evals, double2* evecs, const char* __restrict__ ws,
                                                       float2* evecs32) {
     using L = JacobiWs<N>;
     __shared__ int order[N];
     const double* diag = reinterpret_cast<const double*>(ws + L::diag_off);
-    const double2* V = reinterpret_cast<const double2*>(ws + L::v_off);
+    const E* V = reinterpret_cast<const E*>(ws + L::v_off);
     jacobi_finish<NT, N>([&](int j) { return diag[j]; }, [&](int k, int j) { return V[k + (size_t)j * N]; }, evals,
                          evecs, order, evecs32);
 }
@@ -1623,44 +1677,46 @@ size_t eig_workspace_bytes(int T) {
 #define HV_JACOBI_NT128 1024    // threads of the packed T = 128 eigensolver
 #endif
 
-template <int N, class CT>
+// E = double: a real symmetric matrix (the real part of cov), real rotations.
+template <int N, class CT, class E = double2>
 static hv_status eig_deferred(double* evals, double2* evecs, const CT* cov, char* ws, hipStream_t st,
                               float2* evecs32) {
     if constexpr (N == 128 && HV_JACOBI_PACKED) {
-        constexpr size_t lds = jacobi_packed_lds(N);
+        constexpr size_t lds = jacobi_packed_lds<E>(N);
         static const hipError_t attr = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&k_jacobi_packed<HV_JACOBI_NT128, N, CT>),
+            reinterpret_cast<const void*>(&k_jacobi_packed<HV_JACOBI_NT128, N, CT, E>),
             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (attr != hipSuccess) return hip_fail(attr, "hipFuncSetAttribute(k_jacobi_packed)");
-        k_jacobi_packed<HV_JACOBI_NT128, N, CT><<<1, HV_JACOBI_NT128, lds, st>>>(cov, ws);
+        k_jacobi_packed<HV_JACOBI_NT128, N, CT, E><<<1, HV_JACOBI_NT128, lds, st>>>(cov, ws);
         HV_CHECK_LAUNCH("k_jacobi_packed");
     } else {
+        if constexpr (!std::is_same<E, double2>::value) return fail(HV_ERR_UNSUPPORTED, "real Jacobi: T = 256");
         k_jacobi_global<1024, N, CT><<<1, 1024, 0, st>>>(cov, ws);
         HV_CHECK_LAUNCH("k_jacobi_global");
     }
-    k_jacobi_apply<N><<<N / 4, 256, 0, st>>>(ws, ws);
+    k_jacobi_apply<N, E><<<N / 4, 256, 0, st>>>(ws, ws);
     HV_CHECK_LAUNCH("k_jacobi_apply");
-    k_jacobi_finish<256, N><<<1, 256, 0, st>>>(evals, evecs, ws, evecs32);
+    k_jacobi_finish<256, N, E><<<1, 256, 0, st>>>(evals, evecs, ws, evecs32);
     HV_CHECK_LAUNCH("k_jacobi_finish");
     return HV_OK;
 }
 
-template <class CT>
+template <class CT, class E = double2>
 static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T, void* work, hipStream_t st,
                             float2* e32) {
     switch (T) {
-        case 1: k_jacobi_lds<64, 1, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 2: k_jacobi_lds<64, 2, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 4: k_jacobi_lds<64, 4, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 8: k_jacobi_lds<64, 8, CT><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 16: k_jacobi_lds<256, 16, CT><<<1, 256, 0, st>>>(evals, evecs, cov, e32); break;
-        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov, e32); break;
-        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 1: k_jacobi_lds<64, 1, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 2: k_jacobi_lds<64, 2, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 4: k_jacobi_lds<64, 4, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 8: k_jacobi_lds<64, 8, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
+        case 16: k_jacobi_lds<256, 16, CT, E><<<1, 256, 0, st>>>(evals, evecs, cov, e32); break;
+        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT, E><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov, e32); break;
+        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT, E><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov, e32); break;
         case 128:
         case 256:
             if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
-            return T == 128 ? eig_deferred<128>(evals, evecs, cov, (char*)work, st, e32)
-                            : eig_deferred<256>(evals, evecs, cov, (char*)work, st, e32);
+            return T == 128 ? eig_deferred<128, CT, E>(evals, evecs, cov, (char*)work, st, e32)
+                            : eig_deferred<256, CT, double2>(evals, evecs, cov, (char*)work, st, e32);
         default: return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
     }
     HV_CHECK_LAUNCH("k_jacobi");
@@ -1668,7 +1724,11 @@ static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T,
 }
 
 hv_status launch_eig_jacobi(double* evals, double2* evecs, const void* cov, int cov_double, int T, void* work,
-                            hipStream_t st, float2* evecs32) {
+                            hipStream_t st, float2* evecs32, int real) {
+    if (real) {
+        if (!cov_double) return fail(HV_ERR_INVALID_ARG, "the real eigensolver takes a double2 covariance");
+        return eig_jacobi<double2, double>(evals, evecs, (const double2*)cov, T, work, st, evecs32);
+    }
     return cov_double ? eig_jacobi(evals, evecs, (const double2*)cov, T, work, st, evecs32)
                       : eig_jacobi(evals, evecs, (const float2*)cov, T, work, st, evecs32);
 }

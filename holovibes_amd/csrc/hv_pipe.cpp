@@ -486,7 +486,8 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
         if (s.time_transformation == HV_TIME_PCA) {
             // covariance of the raw window, FP64 eigensolver, then components lo .. lo + K - 1 projected
             HV_TRY(launch_pca_cov_raw(L.cov, frames, (int)s.depth, T, P, slot0, s.pca_precision, L.pca_ws, L.stream));
-            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, T, L.eig_ws, L.stream, L.evecs32));
+            // the raw window's covariance is real symmetric: real rotations (T <= 128)
+            HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, T, L.eig_ws, L.stream, L.evecs32, T <= 128));
             wts = L.evecs32 + lo;
         }
         HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, lo, K,
@@ -1147,7 +1148,8 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
                 st = launch_pca_cov_raw(L.cov, frames[i % n_frames], (int)s.depth, p->T, p->P, 0, s.pca_precision,
                                         L.pca_ws, L.stream);
                 if (st == HV_OK)
-                    st = launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, p->T, L.eig_ws, L.stream, L.evecs32);
+                    st = launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, p->T, L.eig_ws, L.stream, L.evecs32,
+                                           p->T <= 128);
                 break;
             case 1:
                 st = launch_bins_columns(L.planes, (int)s.width, (int)s.height, K, s.space_transformation, p->lens,
