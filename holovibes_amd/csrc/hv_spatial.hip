@@ -328,7 +328,11 @@ __device__ __forceinline__ int xcd_tile(int bx, int nx) {
 // lens values of its output positions stay in registers for all frames.
 // Successive frames need no barrier between them: a transform's first LDS
 // write follows a barrier inside the engine.
-template <int N, int MODE>
+// ONE (columns <= 512): one frame per workgroup (fpb = 1), its tile's loads
+// issued before the thread builds its twiddles, no next-frame registers: more
+// workgroups per CU and the twiddle work under the load latency (512-long
+// columns x 14 planes: 17.6 -> 15.5 us on MI355X).
+template <int N, int MODE, bool ONE>
 __global__ void __launch_bounds__(ColCfg<N>::THREADS)
     k_col_fft(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ lens, int width,
               FrameMap fin, FrameMap fm, int batch, int fpb) {
@@ -344,6 +348,27 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
         const float2* src = in + fin.out_frame(b < batch ? b : 0) * P + x;
         fft_load<N, C::E>(t, [&](int n) -> float2 { return src[(size_t)n * width]; }, v);
     };
+    if constexpr (ONE) {
+        float2 v[kE], lr[MODE == COL_LENS ? kE : 1];
+        load_frame(b0, v);
+        if constexpr (MODE == COL_LENS) {
+#pragma unroll
+            for (int i = 0; i < kE; ++i) lr[i] = lens[(size_t)fft_out_index<N, C::E>(t, i) * width + x];
+        }
+        float2 tw[FftTw<N, C::E>::COUNT];
+        fft_twiddles<N, C::E>(t, tw);
+        float2* dst = out + fm.out_frame(b0) * P + x;
+        auto store = [&](int n, float2 z) { dst[(size_t)n * width] = z; };
+        if constexpr (MODE == COL_LENS) {
+            fft_compute_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw), [](int, float2) {}, lds_at);
+#pragma unroll
+            for (int k = 0; k < kE; ++k) v[k] = cmul(v[k], lr[k]);
+            fft_compute_adj_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw), store, lds_at);
+        } else {
+            fft_compute_tw<N, C::E, MODE == COL_FWD ? -1 : +1>(t, v, static_cast<const float2*>(tw), store, lds_at);
+        }
+        return;
+    }
     float2 lr[kE];
     if constexpr (MODE == COL_LENS) {
 #pragma unroll
@@ -438,12 +463,19 @@ static hv_status col_pass_n(const float2* in, float2* out, const float2* lens, i
                             FrameMap fin, FrameMap fm, hipStream_t st) {
     using C = ColCfg<N>;
     if (width % C::CW) return fail(HV_ERR_UNSUPPORTED, "width not a multiple of the column tile");
-    // frames per workgroup: pipeline several while keeping >= 512 workgroups
     const int tiles = width / C::CW;
+    if constexpr (N <= 512) {
+        // one frame per workgroup
+        dim3 grid(tiles, batch);
+        k_col_fft<N, MODE, true><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm, batch, 1);
+        HV_CHECK_LAUNCH("k_col_fft");
+        return HV_OK;
+    }
+    // frames per workgroup: pipeline several while keeping >= 512 workgroups
     int fpb = 1;
     while (fpb < 8 && (size_t)tiles * ((batch + fpb * 2 - 1) / (fpb * 2)) >= HV_COL_MIN_WG) fpb *= 2;
     dim3 grid(tiles, (batch + fpb - 1) / fpb);
-    k_col_fft<N, MODE><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm, batch, fpb);
+    k_col_fft<N, MODE, false><<<grid, C::THREADS, 0, st>>>(in, out, lens, width, fin, fm, batch, fpb);
     HV_CHECK_LAUNCH("k_col_fft");
     return HV_OK;
 }
