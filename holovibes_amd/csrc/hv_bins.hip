@@ -683,7 +683,7 @@ hv_status launch_bins_propagate(const void* raw, int depth, int width, int heigh
 // The inverse row FFT and the image term in one kernel (image rows of
 // W <= 1024 within LDS).
 bool bins_image_fused(int width, int space) {
-    return space == HV_SPACE_ANGULARSP && width >= 64 && width <= 1024;
+    return space == HV_SPACE_ANGULARSP && is_pow2(width) && width >= 64 && width <= 1024;
 }
 
 template <int IMG>

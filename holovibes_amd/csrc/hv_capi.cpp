@@ -31,7 +31,7 @@ namespace {
 hv_status check_frame(int width, int height) {
     if (width <= 0 || height <= 0) return fail(HV_ERR_INVALID_ARG, "width and height must be positive");
     if (!fft_len_ok(width) || !fft_len_ok(height))
-        return fail(HV_ERR_UNSUPPORTED, "width and height must be powers of two in [16, 4096]");
+        return fail(HV_ERR_UNSUPPORTED, "width and height must be powers of two or even 2^a 3^b 5^c in [16, 4096]");
     return HV_OK;
 }
 

@@ -52,8 +52,11 @@ hv_status input_queue_enqueue_until(hv_input_queue* q, const void* frames, int m
                                     void* stream, const std::atomic<bool>* stop);
 void input_queue_wake(hv_input_queue* q);
 
-// Geometry accepted by the FFT kernels.
-inline bool fft_len_ok(long n) { return is_pow2(n) && n >= 16 && n <= 4096; }
+// Geometry accepted by the FFT kernels: powers of two in [16, 4096] (the
+// templated kernels) and the other even 2^a 3^b 5^c in that range (the
+// runtime-length kernels, hv_spatial.hip).
+bool fft_len_any_ok(long n);
+inline bool fft_len_ok(long n) { return n >= 16 && n <= 4096 && (is_pow2(n) || fft_len_any_ok(n)); }
 inline bool time_len_ok(long n) { return is_pow2(n) && n >= 1 && n <= 256; }
 
 // ---------------------------------------------------------------------------

@@ -73,7 +73,10 @@ static constexpr int kTile = 2 * 32 * 32;  // floats of one wave's two accumulat
 static int pca_slices_pairs(int T, size_t P) {
     const int np = pca_pairs(T);
     int ns = 1;
-    while (np * ns * 2 <= HV_PCA_COV_WG && (size_t)(ns * 2) * kCovWaves * 256 <= P) ns *= 2;
+    // every wave's pixel range a multiple of 16 pixels (frames whose pixel count is not a power of two)
+    while (np * ns * 2 <= HV_PCA_COV_WG && (size_t)(ns * 2) * kCovWaves * 256 <= P &&
+           P % ((size_t)(ns * 2) * kCovWaves * 16) == 0)
+        ns *= 2;
     return ns;
 }
 

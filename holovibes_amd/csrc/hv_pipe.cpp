@@ -555,7 +555,7 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     if (s->space_transformation != HV_SPACE_NONE || s->time_transformation == HV_TIME_STFT ||
         s->time_transformation == HV_TIME_PCA || (s->img_type == HV_IMG_ARGUMENT && s->unwrap_2d)) {
         if (!fft_len_ok(s->width) || !fft_len_ok(s->height))
-            return fail(HV_ERR_UNSUPPORTED, "frame sides must be powers of two in [16, 4096]");
+            return fail(HV_ERR_UNSUPPORTED, "frame sides must be powers of two or even 2^a 3^b 5^c in [16, 4096]");
     }
     if (((size_t)s->width * s->height) % 256) return fail(HV_ERR_UNSUPPORTED, "frame must hold a multiple of 256 pixels");
     if (s->depth != HV_DEPTH_8 && s->depth != HV_DEPTH_16 && s->depth != HV_DEPTH_32F)

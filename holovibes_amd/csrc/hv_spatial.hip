@@ -48,8 +48,8 @@ hv_status launch_convert(float2* out, const void* in, size_t n, int depth, hipSt
 // ---------------------------------------------------------------------------
 // Angular-spectrum transfer function.  `shift` writes element (x, y) at its
 // corner-shifted position; `scale` multiplies the result (1/(W H) folds the
-// inverse-FFT normalisation of angular_spectrum.cu:61 into the lens; the
-// scale is a power of two, so the product is exact).
+// inverse-FFT normalisation of angular_spectrum.cu:61 into the lens; for
+// power-of-two sides the scale is a power of two and the product exact).
 __global__ void k_lens_as(float2* __restrict__ out, int nx, int ny, float z, float lambda, float x_step,
                           float y_step, int precision, int shift, float scale) {
 #pragma clang fp contract(off)
@@ -401,6 +401,244 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
 }
 
 // ---------------------------------------------------------------------------
+// Frame sides 2^a 3^b 5^c that are not powers of two (camera geometries such
+// as 1536 x 2048 or 1920 x 1080; cufftPlanMany accepts any n,
+// icompute.cc:151-175).  Runtime-length Stockham FFT: the plan's radices
+// (8, 4, 2, then 3s, 5s) are walked at run time, each stage reading buffer a
+// and writing buffer b in LDS (a transform of CW interleaved columns), with
+// twiddles exp(dir 2 pi i e / M) from sincospif of the exact ratio e / M.
+// ---------------------------------------------------------------------------
+struct AnyPlan {
+    int n, nst;
+    int r[16];
+};
+
+static bool any_plan(int n, AnyPlan& p) {
+    if (n < 16 || n > 4096 || (n & 1)) return false;
+    p.n = n;
+    p.nst = 0;
+    int m = n;
+    while (m % 8 == 0) { p.r[p.nst++] = 8; m /= 8; }
+    if (m % 4 == 0) { p.r[p.nst++] = 4; m /= 4; }
+    if (m % 2 == 0) { p.r[p.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.r[p.nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.r[p.nst++] = 5; m /= 5; }
+    return m == 1 && p.nst <= 16;
+}
+
+bool fft_len_any_ok(long n) {
+    AnyPlan p;
+    return n <= 4096 && any_plan((int)n, p);
+}
+
+// exp(dir 2 pi i e / m), 0 <= e < m
+template <int DIR>
+__device__ __forceinline__ float2 tw_any(int e, int m) {
+    float s, c;
+    sincospif(2.0f * (float)e / (float)m, &s, &c);
+    return make_float2(c, DIR < 0 ? -s : s);
+}
+
+// radix-3 and radix-5 DFTs, natural order in and out
+template <int DIR>
+__device__ __forceinline__ void dft3(float2* v) {
+    constexpr float s3 = 0.866025403784438647f;  // sin(2 pi / 3)
+    const float2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    // (-DIR i) s3 t2: forward y1 = m - i s3 t2
+    const float2 q = DIR < 0 ? make_float2(s3 * t2.y, -s3 * t2.x) : make_float2(-s3 * t2.y, s3 * t2.x);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, q);
+    v[2] = csub(m, q);
+}
+template <int DIR>
+__device__ __forceinline__ void dft5(float2* v) {
+    constexpr float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;  // cos(2 pi / 5), cos(4 pi / 5)
+    constexpr float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;   // sin(2 pi / 5), sin(4 pi / 5)
+    const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const float2 b1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const float2 b2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    // forward: y1 = a1 - i b1, y4 = a1 + i b1, y2 = a2 - i b2, y3 = a2 + i b2
+    const float2 q1 = DIR < 0 ? make_float2(b1.y, -b1.x) : make_float2(-b1.y, b1.x);
+    const float2 q2 = DIR < 0 ? make_float2(b2.y, -b2.x) : make_float2(-b2.y, b2.x);
+    v[0] = make_float2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+    v[1] = cadd(a1, q1);
+    v[4] = csub(a1, q1);
+    v[2] = cadd(a2, q2);
+    v[3] = csub(a2, q2);
+}
+
+// One Stockham stage: butterfly j reads a[j + r m], m = n / R, multiplies by
+// w^r (w = exp(dir 2 pi i (j mod ns) / (ns R))), runs the radix-R DFT and
+// writes b[(j / ns) ns R + (j mod ns) + r ns].
+template <int R, int DIR, int CW>
+__device__ __forceinline__ void any_stage(const float2* a, float2* b, int m, int ns, int c, int t, int nt) {
+    for (int j = t; j < m; j += nt) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = a[(j + r * m) * CW + c];
+        const int k = j % ns;
+        if (ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw_any<DIR>(k * r, ns * R));
+        }
+        if constexpr (R == 3) dft3<DIR>(v);
+        else if constexpr (R == 5) dft5<DIR>(v);
+        else dft_reg<R, DIR>(v);
+        const int d = (j / ns) * ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) b[(d + r * ns) * CW + c] = v[r];
+    }
+}
+
+// The whole transform (input in a, natural order); returns the buffer holding
+// the output.  Every thread of the workgroup calls it (it contains barriers).
+template <int DIR, int CW>
+__device__ float2* any_fft(float2* a, float2* b, const AnyPlan& pl, int c, int t, int nt) {
+    const int n = pl.n;
+    int ns = 1;
+    for (int s = 0; s < pl.nst; ++s) {
+        const int R = pl.r[s], m = n / R;
+        switch (R) {
+            case 8: any_stage<8, DIR, CW>(a, b, m, ns, c, t, nt); break;
+            case 4: any_stage<4, DIR, CW>(a, b, m, ns, c, t, nt); break;
+            case 2: any_stage<2, DIR, CW>(a, b, m, ns, c, t, nt); break;
+            case 3: any_stage<3, DIR, CW>(a, b, m, ns, c, t, nt); break;
+            default: any_stage<5, DIR, CW>(a, b, m, ns, c, t, nt); break;
+        }
+        __syncthreads();
+        float2* x = a;
+        a = b;
+        b = x;
+        ns *= R;
+    }
+    return a;
+}
+
+constexpr int kAnyRowThreads = 256;
+
+// Row pass of any supported length: one row of one frame per workgroup.
+template <int KIN, bool CHIRP, int DIR>
+__global__ void __launch_bounds__(kAnyRowThreads)
+    k_any_row(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
+              FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int mm_slots, AnyPlan pl) {
+    extern __shared__ float2 any_sh[];
+    if (mm_reset && blockIdx.x == 0 && blockIdx.y == 0) mm_reset_slots(mm_reset, mm_slots);
+    const int n = pl.n, y = blockIdx.x, b = blockIdx.y;
+    const size_t P = (size_t)n * height, row = (size_t)y * n;
+    const size_t fi = fin.out_frame(b) * P;
+    float2* a = any_sh;
+    for (int i = threadIdx.x; i < n; i += kAnyRowThreads) {
+        float2 v = px_complex<KIN>(load_px<KIN>(in, fi + row + i));
+        if constexpr (CHIRP) v = cmul(v, chirp[row + i]);
+        a[i] = v;
+    }
+    __syncthreads();
+    const float2* res = any_fft<DIR, 1>(a, any_sh + n, pl, 0, threadIdx.x, kAnyRowThreads);
+    float2* dst = out + fm.out_frame(b) * P + row;
+    for (int i = threadIdx.x; i < n; i += kAnyRowThreads) dst[i] = scale == 1.0f ? res[i] : cscale(res[i], scale);
+}
+
+// Column pass of any supported length: CW columns of one frame per
+// workgroup (columns past the frame's width are computed on zeros and not stored).
+template <int MODE, int CW>
+__global__ void __launch_bounds__(64 * CW)
+    k_any_col(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ lens, int width,
+              FrameMap fin, FrameMap fm, AnyPlan pl) {
+    extern __shared__ float2 any_sh[];
+    const int n = pl.n, b = blockIdx.y;
+    const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+    const int x = blockIdx.x * CW + c;
+    const bool valid = x < width;
+    const size_t P = (size_t)n * width;
+    const float2* src = in + fin.out_frame(b) * P + x;
+    float2* a = any_sh;
+    float2* bb = any_sh + (size_t)n * CW;
+    for (int i = t; i < n; i += 64) a[i * CW + c] = valid ? src[(size_t)i * width] : make_float2(0.f, 0.f);
+    __syncthreads();
+    float2* res;
+    if constexpr (MODE == COL_LENS) {
+        res = any_fft<-1, CW>(a, bb, pl, c, t, 64);
+        for (int i = t; i < n; i += 64)
+            if (valid) res[i * CW + c] = cmul(res[i * CW + c], lens[(size_t)i * width + x]);
+        __syncthreads();
+        res = any_fft<+1, CW>(res, res == a ? bb : a, pl, c, t, 64);
+    } else {
+        res = any_fft<MODE == COL_FWD ? -1 : +1, CW>(a, bb, pl, c, t, 64);
+    }
+    float2* dst = out + fm.out_frame(b) * P + x;
+    if (valid)
+        for (int i = t; i < n; i += 64) dst[(size_t)i * width] = res[i * CW + c];
+}
+
+template <class K>
+static hv_status any_lds_attr(K* kernel, size_t bytes) {
+    // more than the default 64 KiB of dynamic LDS for the longest transforms
+    if (bytes > 64 * 1024) HV_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return HV_OK;
+}
+
+template <int DIR>
+static hv_status any_row_pass(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
+                              int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset,
+                              int mm_slots) {
+    AnyPlan pl;
+    if (!any_plan(width, pl)) return fail(HV_ERR_UNSUPPORTED, "width must be an even 2^a 3^b 5^c in [16, 4096]");
+    if (chirp && DIR > 0) return fail(HV_ERR_UNSUPPORTED, "chirp factors on an inverse row pass");
+    const size_t lds = 2 * (size_t)width * sizeof(float2);
+    const dim3 grid(height, batch);
+#define HV_ANY_ROW(KIN, CH)                                                                                      \
+    do {                                                                                                         \
+        HV_TRY(any_lds_attr(&k_any_row<KIN, CH, DIR>, lds));                                                     \
+        k_any_row<KIN, CH, DIR><<<grid, kAnyRowThreads, lds, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, \
+                                                                  mm_slots, pl);                                 \
+    } while (0)
+    if (chirp) {
+        if constexpr (DIR < 0) {
+            switch (kin) {
+                case IN_U8: HV_ANY_ROW(IN_U8, true); break;
+                case IN_U16: HV_ANY_ROW(IN_U16, true); break;
+                case IN_F32: HV_ANY_ROW(IN_F32, true); break;
+                default: HV_ANY_ROW(IN_C64, true); break;
+            }
+        }
+    } else {
+        switch (kin) {
+            case IN_U8: HV_ANY_ROW(IN_U8, false); break;
+            case IN_U16: HV_ANY_ROW(IN_U16, false); break;
+            case IN_F32: HV_ANY_ROW(IN_F32, false); break;
+            default: HV_ANY_ROW(IN_C64, false); break;
+        }
+    }
+#undef HV_ANY_ROW
+    HV_CHECK_LAUNCH("k_any_row");
+    return HV_OK;
+}
+
+template <int MODE>
+static hv_status any_col_pass(const float2* in, float2* out, const float2* lens, int width, int height, int batch,
+                              FrameMap fin, FrameMap fm, hipStream_t st) {
+    AnyPlan pl;
+    if (!any_plan(height, pl)) return fail(HV_ERR_UNSUPPORTED, "height must be an even 2^a 3^b 5^c in [16, 4096]");
+    if (height <= 2048) {
+        constexpr int CW = 4;
+        const size_t lds = 2 * (size_t)height * CW * sizeof(float2);
+        HV_TRY(any_lds_attr(&k_any_col<MODE, CW>, lds));
+        k_any_col<MODE, CW><<<dim3((width + CW - 1) / CW, batch), 64 * CW, lds, st>>>(in, out, lens, width, fin, fm, pl);
+    } else {
+        constexpr int CW = 2;
+        const size_t lds = 2 * (size_t)height * CW * sizeof(float2);
+        HV_TRY(any_lds_attr(&k_any_col<MODE, CW>, lds));
+        k_any_col<MODE, CW><<<dim3((width + CW - 1) / CW, batch), 64 * CW, lds, st>>>(in, out, lens, width, fin, fm, pl);
+    }
+    HV_CHECK_LAUNCH("k_any_col");
+    return HV_OK;
+}
+
+// ---------------------------------------------------------------------------
 // dispatch
 // ---------------------------------------------------------------------------
 template <int N, int DIR>
@@ -454,7 +692,9 @@ static hv_status row_pass(const void* in, int kin, float2* out, const float2* ch
         case 1024: return row_pass_n<1024, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
         case 2048: return row_pass_n<2048, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
         case 4096: return row_pass_n<4096, DIR>(in, kin, out, chirp, height, batch, fin, fm, scale, st, mm_reset, mm_slots);
-        default: return fail(HV_ERR_UNSUPPORTED, "width must be a power of two in [16, 4096]");
+        default:
+            return any_row_pass<DIR>(in, kin, out, chirp, width, height, batch, fin, fm, scale, st, mm_reset,
+                                     mm_slots);
     }
 }
 
@@ -462,7 +702,8 @@ template <int N, int MODE>
 static hv_status col_pass_n(const float2* in, float2* out, const float2* lens, int width, int batch,
                             FrameMap fin, FrameMap fm, hipStream_t st) {
     using C = ColCfg<N>;
-    if (width % C::CW) return fail(HV_ERR_UNSUPPORTED, "width not a multiple of the column tile");
+    // widths that are not a multiple of the column tile take the runtime-length kernel
+    if (width % C::CW) return any_col_pass<MODE>(in, out, lens, width, N, batch, fin, fm, st);
     const int tiles = width / C::CW;
     if constexpr (N <= 512) {
         // one frame per workgroup
@@ -493,7 +734,7 @@ static hv_status col_pass(const float2* in, float2* out, const float2* lens, int
         case 1024: return col_pass_n<1024, MODE>(in, out, lens, width, batch, fin, fm, st);
         case 2048: return col_pass_n<2048, MODE>(in, out, lens, width, batch, fin, fm, st);
         case 4096: return col_pass_n<4096, MODE>(in, out, lens, width, batch, fin, fm, st);
-        default: return fail(HV_ERR_UNSUPPORTED, "height must be a power of two in [16, 4096]");
+        default: return any_col_pass<MODE>(in, out, lens, width, height, batch, fin, fm, st);
     }
 }
 

@@ -54,12 +54,27 @@ bool is_pow2(long n) { return n > 0 && (n & (n - 1)) == 0; }
 // X[k] = sum_n x[n] exp(dir * 2 pi i k n / N), dir = -1 forward, unnormalised,
 // the cuFFT C2C convention used at angular_spectrum.cu:47,59 and stft.cu:12).
 // ---------------------------------------------------------------------------
+// Lengths that are not powers of two (2^a 3^b 5^c camera sides) take a
+// recursive mixed-radix decimation in time: x splits by its smallest prime
+// factor p into p interleaved subsequences of length m = n / p, each is
+// transformed, and X[k + r m] = sum_q W_n^{q (k + r m)} Y_q[k] (a direct
+// p-point combination per output; twiddles from the table of the full length).
 template <typename R>
 struct FftPlan {
     int n = 0;
+    bool pow2 = true;
     std::vector<int> rev;
-    std::vector<C<R>> tw;  // exp(-2 pi i k / n), k < n/2
-    explicit FftPlan(int n_) : n(n_), rev(n_), tw(n_ / 2 > 0 ? n_ / 2 : 1) {
+    std::vector<C<R>> tw;      // exp(-2 pi i k / n), k < n/2
+    std::vector<C<R>> tw_all;  // exp(-2 pi i k / n), k < n (other lengths)
+    explicit FftPlan(int n_) : n(n_), pow2(is_pow2(n_)), rev(n_), tw(n_ / 2 > 0 ? n_ / 2 : 1) {
+        if (!pow2) {
+            tw_all.resize(n);
+            for (int k = 0; k < n; ++k) {
+                double a = -2.0 * M_PI * (double)k / (double)n;
+                tw_all[k] = {(R)std::cos(a), (R)std::sin(a)};
+            }
+            return;
+        }
         int lg = 0;
         while ((1 << lg) < n) ++lg;
         for (int i = 0; i < n; ++i) {
@@ -75,9 +90,42 @@ struct FftPlan {
     }
 };
 
+// out[0 .. len) = DFT of in[0], in[s], ..., in[(len - 1) s]; twiddle
+// W_len^e = tw_all[e (n / len) mod n]; scratch holds len values.
+template <typename R>
+void fft_mixed(const C<R>* in, C<R>* out, int len, int s, const FftPlan<R>& p, int dir, C<R>* scratch) {
+    if (len == 1) {
+        out[0] = in[0];
+        return;
+    }
+    int f = 2;
+    while (len % f) ++f;
+    const int m = len / f;
+    for (int q = 0; q < f; ++q) fft_mixed(in + (size_t)q * s, out + (size_t)q * m, m, s * f, p, dir, scratch);
+    const int step = p.n / len;
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < f; ++r) {
+            C<R> acc{0, 0};
+            for (int q = 0; q < f; ++q) {
+                const long e = ((long)q * (k + (long)r * m)) % len;
+                C<R> w = p.tw_all[(size_t)(e * step)];
+                if (dir > 0) w.im = -w.im;
+                acc = cadd(acc, cmul(out[(size_t)q * m + k], w));
+            }
+            scratch[(size_t)r * m + k] = acc;
+        }
+    }
+    for (int i = 0; i < len; ++i) out[i] = scratch[i];
+}
+
 template <typename R>
 void fft1d(C<R>* a, const FftPlan<R>& p, int dir) {
     const int n = p.n;
+    if (!p.pow2) {
+        std::vector<C<R>> in(a, a + n), scratch(n);
+        fft_mixed(in.data(), a, n, 1, p, dir, scratch.data());
+        return;
+    }
     for (int i = 0; i < n; ++i)
         if (i < p.rev[i]) std::swap(a[i], a[p.rev[i]]);
     for (int len = 2; len <= n; len <<= 1) {
@@ -811,17 +859,17 @@ void or_lens_fresnel(int side, int H, int W, float lambda, float z, float px, in
 }
 
 int or_fft2_f64(double* buf, int B, int W, int H, int dir) {
-    if (!is_pow2(W) || !is_pow2(H)) return HV_ERR_UNSUPPORTED;
+    if (W < 1 || H < 1) return HV_ERR_INVALID_ARG;
     fft2(cp<double>(buf), B, W, H, dir);
     return 0;
 }
 int or_fft2_f32(float* buf, int B, int W, int H, int dir) {
-    if (!is_pow2(W) || !is_pow2(H)) return HV_ERR_UNSUPPORTED;
+    if (W < 1 || H < 1) return HV_ERR_INVALID_ARG;
     fft2(cp<float>(buf), B, W, H, dir);
     return 0;
 }
 int or_fft1_f64(double* buf, int n, int dir) {
-    if (!is_pow2(n)) return HV_ERR_UNSUPPORTED;
+    if (n < 1) return HV_ERR_INVALID_ARG;
     FftPlan<double> p(n);
     fft1d(cp<double>(buf), p, dir);
     return 0;

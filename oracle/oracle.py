@@ -140,9 +140,12 @@ def shift_corners(a: np.ndarray) -> np.ndarray:
 
 
 def fft2(a: np.ndarray, direction: int = -1) -> np.ndarray:
-    """Batched unnormalised 2-D FFT of (..., H, W) complex128 (own radix-2)."""
+    """Batched unnormalised 2-D FFT of (..., H, W) complex128 (own radix-2;
+    mixed radix for other lengths)."""
     a = np.ascontiguousarray(a, np.complex128).copy()
     H, W = a.shape[-2:]
+    if H * W == 0:
+        raise ValueError("unsupported size")
     B = a.size // (H * W)
     rc = lib().or_fft2_f64(_p(a), B, W, H, direction)
     if rc:

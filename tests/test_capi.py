@@ -60,7 +60,13 @@ def test_validate_settings(lib):
     assert lib.hv_validate_settings(ctypes.byref(ok)) == 0
     for name in configs.WORKLOADS:
         assert lib.hv_validate_settings(ctypes.byref(configs.settings_for(name))) == 0, name
-    bad = [dict(time_stride=48), dict(time_stride=16), dict(batch_size=0), dict(width=500),
+    # frame sides: powers of two or other even 2^a 3^b 5^c in [16, 4096]
+    for good in (dict(width=1536), dict(width=1000), dict(width=1350), dict(width=1920, height=1080),
+                 dict(height=768)):
+        s = configs.settings_for("metric", **good)
+        assert lib.hv_validate_settings(ctypes.byref(s)) == 0, good
+    bad = [dict(time_stride=48), dict(time_stride=16), dict(batch_size=0), dict(width=504), dict(width=4100),
+           dict(width=135),
            dict(time_transformation_size=24), dict(p_start=2, p_width=30), dict(depth=3), dict(lambda_=0.0),
            dict(contrast_mode=9), dict(contrast_subsample=3),
            dict(contrast_reticle_enabled=1, contrast_reticle_scale=1.5),
@@ -75,7 +81,8 @@ def test_validate_settings(lib):
 def test_step_functions_reject_bad_arguments_without_gpu(lib):
     # validation happens before any device work
     assert lib.hv_convert_to_complex(None, None, 16, 1, 2, None) == 1
-    assert lib.hv_fft2(ctypes.c_void_p(16), 1, 48, 64, -1, None) == 2  # not a power of two
+    assert lib.hv_fft2(ctypes.c_void_p(16), 1, 56, 64, -1, None) == 2  # 56 = 8 x 7
+    assert lib.hv_fft2(ctypes.c_void_p(16), 1, 45, 64, -1, None) == 2  # odd
     assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 16, 256, 0, 3, 16, None, None) == 1
     assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 12, 256, 0, 3, 5, None, None) == 2
     assert b"pmax" in lib.hv_last_error() or b"range" in lib.hv_last_error() or lib.hv_last_error()

@@ -32,7 +32,7 @@ def rel_l2(a, b):
 # ---------------------------------------------------------------------------
 # FFT (cuFFT C2C semantics)
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("shape", [(2, 16, 32), (1, 64, 64), (3, 32, 16)])
+@pytest.mark.parametrize("shape", [(2, 16, 32), (1, 64, 64), (3, 32, 16), (2, 24, 40), (1, 96, 60), (1, 48, 1536)])
 def test_fft2_matches_numpy(oracle, shape):
     x = crand(*shape)
     assert rel_l2(oracle.fft2(x, -1), np.fft.fft2(x)) < 1e-13
@@ -40,15 +40,26 @@ def test_fft2_matches_numpy(oracle, shape):
     assert rel_l2(oracle.fft2(x, +1), np.fft.ifft2(x) * shape[1] * shape[2]) < 1e-13
 
 
-@pytest.mark.parametrize("n", [1, 2, 8, 16, 128, 256])
+@pytest.mark.parametrize("n", [1, 2, 8, 16, 128, 256, 3, 5, 6, 12, 24, 45, 120, 1080, 1536, 3888, 4000])
 def test_fft1_matches_numpy(oracle, n):
+    # powers of two (radix 2) and camera lengths 2^a 3^b 5^c (mixed radix)
     x = crand(n)
     assert rel_l2(oracle.fft1(x, -1), np.fft.fft(x)) < 1e-13
+    assert rel_l2(oracle.fft1(x, +1), np.fft.ifft(x) * n) < 1e-13
 
 
-def test_fft_rejects_non_power_of_two(oracle):
+def test_fft_mixed_radix_pure_tone(oracle):
+    # a pure tone exp(2 pi i k0 t / n) transforms to n at bin k0 exactly (closed form)
+    for n, k0 in ((24, 5), (1536, 1001), (1080, 7)):
+        x = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+        ref = np.zeros(n, np.complex128)
+        ref[k0] = n
+        assert np.abs(oracle.fft1(x, -1) - ref).max() < 1e-9 * n
+
+
+def test_fft_rejects_empty(oracle):
     with pytest.raises(ValueError):
-        oracle.fft2(crand(1, 12, 16))
+        oracle.fft2(np.zeros((1, 0, 16), np.complex128))
 
 
 # ---------------------------------------------------------------------------
