@@ -37,7 +37,7 @@ hv_status check_frame(int width, int height) {
 
 hv_status check_T(int T) {
     if (T <= 0) return fail(HV_ERR_INVALID_ARG, "T must be positive");
-    if (!time_len_ok(T)) return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
+    if (!time_len_ok(T)) return fail(HV_ERR_UNSUPPORTED, "T must be in [1, 256]");
     return HV_OK;
 }
 

@@ -57,7 +57,9 @@ void input_queue_wake(hv_input_queue* q);
 // runtime-length kernels, hv_spatial.hip).
 bool fft_len_any_ok(long n);
 inline bool fft_len_ok(long n) { return n >= 16 && n <= 4096 && (is_pow2(n) || fft_len_any_ok(n)); }
-inline bool time_len_ok(long n) { return is_pow2(n) && n >= 1 && n <= 256; }
+// Time windows: any length in [1, 256] (powers of two on the FFT kernels,
+// the others by direct DFT of the bins used).
+inline bool time_len_ok(long n) { return n >= 1 && n <= 256; }
 
 // ---------------------------------------------------------------------------
 // device helpers

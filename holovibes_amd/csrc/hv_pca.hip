@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256)
     const size_t P2 = P / 2;
     const float4* r4 = reinterpret_cast<const float4*>(ring);
     float4* m4 = reinterpret_cast<float4*>(mean);
-    const float inv = 1.0f / (float)T;  // a power of two: exact
+    const float inv = 1.0f / (float)T;  // exact for power-of-two windows
     float mx = 0.f, mmu = 0.f;
     double msum = 0.0;
     auto amax = [](float4 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))); };
@@ -647,6 +647,8 @@ __global__ void __launch_bounds__(256)
 
 hv_status launch_pca_cov(void* cov, int cov_double, const float2* ring, int T, size_t P, int precision, void* ws,
                          hipStream_t st) {
+    // frame groups of 32: windows that are not powers of two fit one group
+    if (!is_pow2(T) && T > 32) return fail(HV_ERR_UNSUPPORTED, "PCA covariance: T must be a power of two or below 32");
     const CovLayout l = cov_layout(T, P, precision);
     const int slices = pca_slices(T, P, precision), np = pca_pairs(T);
     const bool grouped = pca_cov_grouped(T, precision);
@@ -767,7 +769,7 @@ __global__ void __launch_bounds__(kRawMeanThreads)
                int T, size_t P, int vec) {
     constexpr int kGroups = 4, kQuads = kRawMeanThreads / kGroups;  // 64 quads = 256 pixels
     const int q = threadIdx.x % kQuads, g = threadIdx.x / kQuads;
-    const float inv = 1.0f / (float)T;  // a power of two: exact
+    const float inv = 1.0f / (float)T;  // exact for power-of-two windows
     __shared__ float4 part[kGroups][kQuads];
     float mx = 0.f, mmu = 0.f;
     double msum = 0.0;
@@ -1299,6 +1301,8 @@ __device__ __forceinline__ double2 el_conj(double2 x) { return make_double2(x.x,
 __device__ __forceinline__ double el_conj(double x) { return x; }
 __device__ __forceinline__ void el_one(double2& x) { x = make_double2(1.0, 0.0); }
 __device__ __forceinline__ void el_one(double& x) { x = 1.0; }
+__device__ __forceinline__ void el_set_real(double2& x, double v) { x = make_double2(v, 0.0); }
+__device__ __forceinline__ void el_set_real(double& x, double v) { x = v; }
 __device__ __forceinline__ void el_real(double2& x) { x.y = 0.0; }
 __device__ __forceinline__ void el_real(double&) {}
 __device__ __forceinline__ double2 el_c(double2 x) { return x; }
@@ -1316,8 +1320,11 @@ __device__ __forceinline__ double2 el_phase_norm(double x, double ph) { return m
 // phase-normalised eigenvectors -- the largest component real positive, the
 // first of equal magnitudes within a relative 1e-12 -- from the converged
 // diagonal d(j) and the accumulated rotations v(k, j).
+// nout < N: the matrix was padded (see k_jacobi_lds); only the first nout
+// eigenpairs, of nout components, are written.
 template <int NT, int N, class D, class VA>
-__device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int* order, float2* evecs32) {
+__device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int* order, float2* evecs32,
+                              int nout = N) {
     constexpr int n = N;
     const int tid = threadIdx.x;
     for (int j = tid; j < n; j += NT) {
@@ -1331,13 +1338,13 @@ __device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int*
         order[rank] = j;
     }
     __syncthreads();
-    for (int jj = tid; jj < n; jj += NT) {
+    for (int jj = tid; jj < nout; jj += NT) {
         const int j = order[jj];
         evals[jj] = d(j);
         int kmax = 0;
         double best = -1.0;
 #pragma unroll 4
-        for (int k = 0; k < n; ++k) {
+        for (int k = 0; k < nout; ++k) {
             const double m2 = el_norm2(v(k, j));
             if (m2 > best * (1 + 2e-12)) {
                 best = m2;
@@ -1346,11 +1353,11 @@ __device__ void jacobi_finish(D&& d, VA&& v, double* evals, double2* evecs, int*
         }
         const auto ph = v(kmax, j);
 #pragma unroll 4
-        for (int k = 0; k < n; ++k) {
+        for (int k = 0; k < nout; ++k) {
             const double2 z = el_phase_norm(v(k, j), ph);
-            evecs[k + (size_t)jj * n] = z;
+            evecs[k + (size_t)jj * nout] = z;
             // row-major float copy: one component of consecutive eigenvectors is contiguous
-            if (evecs32) evecs32[(size_t)k * n + jj] = make_float2((float)z.x, (float)z.y);
+            if (evecs32) evecs32[(size_t)k * nout + jj] = make_float2((float)z.x, (float)z.y);
         }
     }
 }
@@ -1365,9 +1372,12 @@ struct JacobiLog {
     double* diag;   // converged diagonal, unordered
 };
 
+// nreal < N: a matrix padded to N (k_jacobi_lds); the convergence test
+// measures the nreal x nreal block only, as for that matrix alone.
 template <int NT, int N, bool DEFER = false, class E = double2>
 __device__ void jacobi_run(E* A, E* V, int ld, double* evals, double2* evecs, RotOf<E>* rot, double* red,
-                           int* order, float2* evecs32, JacobiLog<RotOf<E>> log = JacobiLog<RotOf<E>>{}) {
+                           int* order, float2* evecs32, JacobiLog<RotOf<E>> log = JacobiLog<RotOf<E>>{},
+                           int nreal = N) {
     using R = RotOf<E>;
     constexpr int n = N, half = N / 2;
     const int tid = threadIdx.x;
@@ -1375,7 +1385,8 @@ __device__ void jacobi_run(E* A, E* V, int ld, double* evals, double2* evecs, Ro
     auto a = [&](int i, int j) -> E& { return A[i + (size_t)j * ld]; };
     auto v = [&](int i, int j) -> E& { return V[i + (size_t)j * ld]; };
     double fro = 0.0;
-    for (int e = tid; e < n * n; e += NT) fro += el_norm2(a(e % n, e / n));
+    for (int e = tid; e < n * n; e += NT)
+        if (e % n < nreal && e / n < nreal) fro += el_norm2(a(e % n, e / n));
     fro = block_sum<NT>(fro, red);
     for (int sweep = 0; sweep < HV_JACOBI_MAX_SWEEPS && n > 1; ++sweep) {
         double off = 0.0;
@@ -1438,7 +1449,7 @@ __device__ void jacobi_run(E* A, E* V, int ld, double* evals, double2* evecs, Ro
         if (tid == 0) *log.rounds = g;
         return;
     }
-    jacobi_finish<NT, N>([&](int j) { return el_re(a(j, j)); }, v, evals, evecs, order, evecs32);
+    jacobi_finish<NT, N>([&](int j) { return el_re(a(j, j)); }, v, evals, evecs, order, evecs32, nreal);
 }
 
 // A covariance element (float2 from the C-ABI step, double2 in the pipe) as double2.
@@ -1454,8 +1465,15 @@ __device__ __forceinline__ void set_cov_el(double2& dst, double2 c, int i, int j
 }
 __device__ __forceinline__ void set_cov_el(double& dst, double2 c, int, int) { dst = c.x; }
 
+// nreal < N (windows that are not powers of two): the nreal x nreal matrix
+// (leading dimension nreal) padded to N with a diagonal block pad * I, pad
+// above every eigenvalue (1 + twice the largest absolute row sum, a
+// Gershgorin bound) and zero coupling: the pad rows never rotate (their
+// off-diagonal entries are exact zeros), the matrix's eigenpairs are the
+// nreal smallest, and the first nreal components of their vectors.
 template <int NT, int N, class CT, class E = double2>
-__global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs, const CT* cov, float2* evecs32) {
+__global__ void __launch_bounds__(NT)
+    k_jacobi_lds(double* evals, double2* evecs, const CT* cov, float2* evecs32, int nreal) {
     constexpr int LD = N + 1;  // padded columns: row and column walks spread over the banks
     __shared__ E A[N * LD], V[N * LD];
     __shared__ RotOf<E> rot[N / 2 > 0 ? N / 2 : 1];
@@ -1463,14 +1481,30 @@ __global__ void __launch_bounds__(NT) k_jacobi_lds(double* evals, double2* evecs
     __shared__ int order[N];
     for (int e = threadIdx.x; e < N * N; e += NT) {
         const int i = e % N, j = e / N;
-        // Hermitian from the lower triangle (CUBLAS_FILL_MODE_LOWER)
-        const double2 c = i >= j ? cov_elem(cov, i + (size_t)j * N) : cov_elem(cov, j + (size_t)i * N);
-        set_cov_el(A[i + j * LD], c, i, j);
+        if (i < nreal && j < nreal) {
+            // Hermitian from the lower triangle (CUBLAS_FILL_MODE_LOWER)
+            const double2 c = i >= j ? cov_elem(cov, i + (size_t)j * nreal) : cov_elem(cov, j + (size_t)i * nreal);
+            set_cov_el(A[i + j * LD], c, i, j);
+        } else {
+            el_zero(A[i + j * LD]);
+        }
         el_zero(V[i + j * LD]);
         if (i == j) el_one(V[i + j * LD]);
     }
     __syncthreads();
-    jacobi_run<NT, N, false, E>(A, V, LD, evals, evecs, rot, red, order, evecs32);
+    if (nreal < N) {
+        if (threadIdx.x == 0) {
+            double bound = 0.0;
+            for (int i = 0; i < nreal; ++i) {
+                double row = 0.0;
+                for (int j = 0; j < nreal; ++j) row += sqrt(el_norm2(A[i + j * LD]));
+                bound = fmax(bound, row);
+            }
+            for (int i = nreal; i < N; ++i) el_set_real(A[i + i * LD], 1.0 + 2.0 * bound);
+        }
+        __syncthreads();
+    }
+    jacobi_run<NT, N, false, E>(A, V, LD, evals, evecs, rot, red, order, evecs32, JacobiLog<RotOf<E>>{}, nreal);
 }
 
 // T >= 128: A in the workspace, the rotations logged (JacobiLog) instead of
@@ -1707,14 +1741,17 @@ static hv_status eig_deferred(double* evals, double2* evecs, const CT* cov, char
 template <class CT, class E = double2>
 static hv_status eig_jacobi(double* evals, double2* evecs, const CT* cov, int T, void* work, hipStream_t st,
                             float2* e32) {
-    switch (T) {
-        case 1: k_jacobi_lds<64, 1, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 2: k_jacobi_lds<64, 2, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 4: k_jacobi_lds<64, 4, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 8: k_jacobi_lds<64, 8, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32); break;
-        case 16: k_jacobi_lds<256, 16, CT, E><<<1, 256, 0, st>>>(evals, evecs, cov, e32); break;
-        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT, E><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov, e32); break;
-        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT, E><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov, e32); break;
+    // windows that are not powers of two (T < 32) run padded to the next power of two
+    const int N = T <= 1 ? T : 1 << ilog2(T);
+    if (N != T && T > 32) return fail(HV_ERR_UNSUPPORTED, "eigensolver: T must be a power of two or below 32");
+    switch (N) {
+        case 1: k_jacobi_lds<64, 1, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32, T); break;
+        case 2: k_jacobi_lds<64, 2, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32, T); break;
+        case 4: k_jacobi_lds<64, 4, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32, T); break;
+        case 8: k_jacobi_lds<64, 8, CT, E><<<1, 64, 0, st>>>(evals, evecs, cov, e32, T); break;
+        case 16: k_jacobi_lds<256, 16, CT, E><<<1, 256, 0, st>>>(evals, evecs, cov, e32, T); break;
+        case 32: k_jacobi_lds<HV_JACOBI_NT32, 32, CT, E><<<1, HV_JACOBI_NT32, 0, st>>>(evals, evecs, cov, e32, T); break;
+        case 64: k_jacobi_lds<HV_JACOBI_NT64, 64, CT, E><<<1, HV_JACOBI_NT64, 0, st>>>(evals, evecs, cov, e32, T); break;
         case 128:
         case 256:
             if (!work) return fail(HV_ERR_INVALID_ARG, "T > 64 needs an eigensolver workspace");
@@ -2034,7 +2071,7 @@ hv_status launch_pca_image(float* out, const float2* ring, const double2* evecs,
             default: return pca_image_mfma<256>(out, ring, evecs, P, po, pmin, pmax, mm, st, mm_slots);
         }
     }
-    if (T <= 64) {
+    if (T <= 64 && is_pow2(T)) {
         const size_t shm = (size_t)T * (pmax - pmin + 1) * sizeof(float2);
         if (shm > 64 * 1024) return fail(HV_ERR_UNSUPPORTED, "PCA eigenvector range too large");
         switch (T) {

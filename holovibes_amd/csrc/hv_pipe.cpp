@@ -567,7 +567,11 @@ hv_status hv_validate_settings(const hv_compute_settings* s) {
     // transform_api.cc:113-116
     if (s->time_transformation_size == 0) return fail(HV_ERR_INVALID_ARG, "time_transformation_size must be >= 1");
     if (!time_len_ok(s->time_transformation_size))
-        return fail(HV_ERR_UNSUPPORTED, "time_transformation_size must be a power of two <= 256");
+        return fail(HV_ERR_UNSUPPORTED, "time_transformation_size must be <= 256");
+    // PCA windows that are not powers of two: one MFMA frame group (T < 32)
+    if (s->time_transformation == HV_TIME_PCA && !is_pow2(s->time_transformation_size) &&
+        s->time_transformation_size > 32)
+        return fail(HV_ERR_UNSUPPORTED, "PCA windows that are not powers of two must be shorter than 32 frames");
     if (s->space_transformation < HV_SPACE_NONE || s->space_transformation > HV_SPACE_ANGULARSP)
         return fail(HV_ERR_INVALID_ARG, "unknown space transformation");
     if (s->time_transformation < HV_TIME_NONE || s->time_transformation > HV_TIME_PCA)

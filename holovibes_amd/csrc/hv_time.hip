@@ -284,8 +284,109 @@ static hv_status bins_launch_t(const float2* ring, size_t P, const TimeParams& p
     return HV_OK;
 }
 
+// Windows whose length is not a power of two (2 <= T <= 256; cufftPlanMany
+// takes any n, icompute.cc:109-115): the bins the output needs by direct DFT,
+// X_k = sum_t x_t exp(-2 pi i k t / T) over the ring's physical slot order,
+// eight bins at a time as the window streams in, twiddles from an LDS table
+// of exp(-2 pi i m / T) built with sincospif.  All T bins for the full
+// spectrum, [pmin, pmax] for an image, [f_start, f_end] for the moments.
+__device__ __forceinline__ float freq1_rt(int i, int T, float d) {
+    if (T % 2 == 0) return i < T / 2 ? i * d : -((float)T - i) * d;
+    return i < (T + 1) / 2 ? i * d : (i - (float)T) * d;
+}
+
+constexpr int kAnyBins = 8;
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_time_any(const float2* __restrict__ ring, size_t P, int T, TimeParams prm) {
+    __shared__ float2 tw[256];
+    for (int i = threadIdx.x; i < T; i += blockDim.x) {
+        float sn, cs;
+        sincospif(2.0f * (float)i / (float)T, &sn, &cs);
+        tw[i] = make_float2(cs, -sn);
+    }
+    __syncthreads();
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < P;
+    const float2* xp = ring + (valid ? p : 0);
+    const int k0 = MODE == TM_FULL ? 0 : prm.pmin, k1 = MODE == TM_FULL ? T - 1 : prm.pmax;
+    const float d = prm.fps / T;
+    float acc = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    float2 nyq = make_float2(0.f, 0.f);
+    for (int kb = k0; kb <= k1; kb += kAnyBins) {
+        const int nk = min(kAnyBins, k1 - kb + 1);
+        float2 sb[kAnyBins];
+        int idx[kAnyBins];  // (k t) mod T of each bin
+#pragma unroll
+        for (int j = 0; j < kAnyBins; ++j) {
+            sb[j] = make_float2(0.f, 0.f);
+            idx[j] = 0;
+        }
+        for (int t = 0; t < T; ++t) {
+            const float2 x = xp[(size_t)t * P];
+#pragma unroll
+            for (int j = 0; j < kAnyBins; ++j) {
+                if (j >= nk) break;  // uniform
+                const float2 w = tw[idx[j]];
+                sb[j].x = fmaf(x.x, w.x, fmaf(-x.y, w.y, sb[j].x));
+                sb[j].y = fmaf(x.x, w.y, fmaf(x.y, w.x, sb[j].y));
+                idx[j] += kb + j;
+                if (idx[j] >= T) idx[j] -= T;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kAnyBins; ++j) {
+            if (j >= nk) break;
+            const int k = kb + j;
+            if constexpr (MODE == TM_FULL) {
+                if (valid) prm.spec[(size_t)k * P + p] = sb[j];
+            } else if constexpr (MODE == TM_IMAGE) {
+                acc += image_term(sb[j], prm.img_type);
+            } else {
+                const float sq = sb[j].x * sb[j].x + sb[j].y * sb[j].y;
+                const float f = freq1_rt(k, T, d);
+                a0 += sq * 1.0f;
+                a1 += sq * f;
+                a2 += sq * (f * f);
+                if (2 * k == T) nyq = sb[j];
+            }
+        }
+    }
+    float vmin = INFINITY, vmax = -INFINITY;
+    if constexpr (MODE == TM_IMAGE) {
+        const float val = acc / (prm.pmax - prm.pmin + 1);
+        if (valid) {
+            prm.out[p] = val;
+            vmin = vmax = val;
+        }
+    } else if constexpr (MODE == TM_MOMENTS) {
+        // the Nyquist bin of an even window counted again for M0, M2 and cancelled for M1 (tools_compute.cu:61-69)
+        if (T % 2 == 0 && T / 2 >= k0 && T / 2 <= k1) {
+            const float sq = nyq.x * nyq.x + nyq.y * nyq.y;
+            const float f = freq1_rt(T / 2, T, d);
+            a0 += sq * 1.0f;
+            a1 -= sq * f;
+            a2 += sq * (f * f);
+        }
+        if (valid) {
+            if (prm.out) prm.out[p] = a0;
+            if (prm.m1) prm.m1[p] = a1;
+            if (prm.m2) prm.m2[p] = a2;
+            const float sel = prm.mm_moment == 0 ? a0 : prm.mm_moment == 1 ? a1 : a2;
+            vmin = vmax = sel;
+        }
+    }
+    if (MODE != TM_FULL && prm.mm) block_minmax_atomic(vmin, vmax, prm.mm, prm.mm_slots != 0 ? prm.mm_slots : 1);
+}
+
 template <int MODE>
 static hv_status time_launch(const float2* ring, int T, size_t P, const TimeParams& prm, hipStream_t st) {
+    if (!is_pow2(T)) {
+        if (T < 2 || T > 256) return fail(HV_ERR_UNSUPPORTED, "T must be in [1, 256]");
+        k_time_any<MODE><<<(unsigned)((P + 255) / 256), 256, 0, st>>>(ring, P, T, prm);
+        HV_CHECK_LAUNCH("k_time_any");
+        return HV_OK;
+    }
     if (MODE == TM_IMAGE && T >= 16 && prm.pmax - prm.pmin + 1 <= kDirectBins && HV_TIME_DIRECT_BINS) {
         switch (T) {
             case 16: return bins_launch_t<16>(ring, P, prm, st);
@@ -305,7 +406,7 @@ static hv_status time_launch(const float2* ring, int T, size_t P, const TimePara
         case 64: return time_launch_t<64, MODE>(ring, P, prm, st);
         case 128: return time_launch_t<128, MODE>(ring, P, prm, st);
         case 256: return time_launch_t<256, MODE>(ring, P, prm, st);
-        default: return fail(HV_ERR_UNSUPPORTED, "T must be a power of two in [1, 256]");
+        default: return fail(HV_ERR_UNSUPPORTED, "T must be in [1, 256]");
     }
 }
 

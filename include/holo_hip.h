@@ -14,8 +14,8 @@
  *
  * Supported geometry: W and H in [16, 4096], powers of two or other even
  * products 2^a 3^b 5^c (e.g. 1536, 1080, 1920), with W*H a multiple of 256;
- * time-window sizes T powers of two in [1, 256].  Other sizes return
- * HV_ERR_UNSUPPORTED.
+ * time-window sizes T in [1, 256] (PCA: powers of two, or below 32).  Other
+ * sizes return HV_ERR_UNSUPPORTED.
  *
  * Each declaration cites the reference interface it replaces
  * (paths relative to the Holovibes 14.8.2 source tree).

@@ -67,7 +67,7 @@ def test_validate_settings(lib):
         assert lib.hv_validate_settings(ctypes.byref(s)) == 0, good
     bad = [dict(time_stride=48), dict(time_stride=16), dict(batch_size=0), dict(width=504), dict(width=4100),
            dict(width=135),
-           dict(time_transformation_size=24), dict(p_start=2, p_width=30), dict(depth=3), dict(lambda_=0.0),
+           dict(time_transformation_size=300), dict(p_start=2, p_width=30), dict(depth=3), dict(lambda_=0.0),
            dict(contrast_mode=9), dict(contrast_subsample=3),
            dict(contrast_reticle_enabled=1, contrast_reticle_scale=1.5),
            dict(contrast_reticle_enabled=1, contrast_subsample=2),
@@ -76,6 +76,14 @@ def test_validate_settings(lib):
         s = configs.settings_for("metric", **over)
         assert lib.hv_validate_settings(ctypes.byref(s)) != 0, over
         assert lib.hv_last_error()
+    # windows that are not powers of two: any length for the STFT, below 32 frames for PCA
+    for T, want_ok in ((24, True), (100, True), (1, True)):
+        s = configs.settings_for("metric", batch_size=T, time_stride=T, time_transformation_size=T, p_start=0,
+                                 p_width=0)
+        assert (lib.hv_validate_settings(ctypes.byref(s)) == 0) == want_ok, T
+    for T, want_ok in ((24, True), (31, True), (48, False), (96, False), (64, True)):
+        s = configs.settings_for("c2", batch_size=T, time_stride=T, time_transformation_size=T, p_width=T - 5)
+        assert (lib.hv_validate_settings(ctypes.byref(s)) == 0) == want_ok, T
 
 
 def test_step_functions_reject_bad_arguments_without_gpu(lib):
@@ -84,7 +92,7 @@ def test_step_functions_reject_bad_arguments_without_gpu(lib):
     assert lib.hv_fft2(ctypes.c_void_p(16), 1, 56, 64, -1, None) == 2  # 56 = 8 x 7
     assert lib.hv_fft2(ctypes.c_void_p(16), 1, 45, 64, -1, None) == 2  # odd
     assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 16, 256, 0, 3, 16, None, None) == 1
-    assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 12, 256, 0, 3, 5, None, None) == 2
+    assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 300, 256, 0, 3, 5, None, None) == 2
     assert b"pmax" in lib.hv_last_error() or b"range" in lib.hv_last_error() or lib.hv_last_error()
 
 

@@ -1,7 +1,9 @@
 """Frame sides that are not powers of two (camera geometries 2^a 3^b 5^c such
-as 1536 x 2048, 1920 x 1080): the runtime-length FFT kernels against the
-oracle's mixed-radix FFT (pinned by numpy.fft at these lengths), and whole
-pipes against the oracle's pipe on the same seeded frames.
+as 1536 x 2048, 1920 x 1080) and time windows that are not (T = 24, ...): the
+runtime-length FFT and time-DFT kernels against the oracle's mixed-radix FFT
+(pinned by numpy.fft at these lengths), the padded eigensolver against
+numpy's eigh, and whole pipes against the oracle's pipe on the same seeded
+frames.
 
 cufftPlanMany accepts any n (Backend/sources/core/icompute.cc:109-115,
 151-175); the reference's path itself is the same for every size.
@@ -119,3 +121,106 @@ def test_pipe_camera_full_frame(oracle, W, H):
         img, u = p.image().cpu().numpy(), p.output_u16()
         p.close()
         check([(img, u, None, ref)], 1e-5)
+
+
+# ---------------------------------------------------------------------------
+# time windows that are not powers of two
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("T", [3, 12, 24, 100])
+def test_stft_other_windows(st, oracle, T):
+    rng = np.random.default_rng(14)
+    ring = crand(rng, T, 16, 32)
+    got = host(st.stft(torch.from_numpy(ring).cuda()))
+    assert rel_l2(got, oracle.stft(ring.astype(np.complex128))) < 1e-6
+
+
+@pytest.mark.parametrize("T,pmin,pmax", [(24, 2, 7), (12, 0, 11), (100, 5, 5), (15, 3, 14)])
+@pytest.mark.parametrize("img_type", [0, 1, 2])
+def test_stft_image_other_windows(st, oracle, T, pmin, pmax, img_type):
+    rng = np.random.default_rng(15)
+    ring = crand(rng, T, 32, 32) * 100
+    img, mm = st.stft_image(torch.from_numpy(ring).cuda(), img_type, pmin, pmax, minmax=True)
+    got = host(img)
+    want = oracle.image_type(oracle.stft(ring.astype(np.complex128)), img_type, pmin, pmax)
+    assert rel_l2(got, want) < (2e-6 if img_type != 2 else 2e-5)
+    r = host(st.decode_minmax(mm))
+    assert r[0] == got.min() and r[1] == got.max()
+
+
+@pytest.mark.parametrize("T,fs,fe", [(24, 2, 20), (24, 0, 23), (15, 1, 13)])
+def test_stft_moments_other_windows(st, oracle, T, fs, fe):
+    rng = np.random.default_rng(16)
+    ring = crand(rng, T, 16, 64)
+    fps = 96.0
+    m = [host(x) for x in st.stft_moments(torch.from_numpy(ring).cuda(), fs, fe, fps)]
+    w = oracle.moments(oracle.stft(ring.astype(np.complex128)), fs, fe, fps)
+    for k in range(3):
+        scale = np.abs(w[0]).max() * (fps if k else 1) ** k
+        assert np.abs(m[k] - w[k]).max() < 2e-5 * scale
+
+
+@pytest.mark.parametrize("T", [3, 12, 24, 31])
+def test_eig_hermitian_other_windows(st, oracle, T):
+    """The eigensolver pads T to the next power of two with a decoupled block
+    above every eigenvalue; the T smallest eigenpairs are the matrix's."""
+    rng = np.random.default_rng(17)
+    A = crand(rng, T, 3 * T).astype(np.complex128)
+    A = (A @ A.conj().T).astype(np.complex64)
+    ev, V = st.eig_hermitian(torch.from_numpy(A).cuda())
+    ev, V = host(ev), host(V)
+    assert ev.shape == (T,) and V.shape == (T, T)
+    A64 = A.astype(np.complex128)
+    w = np.linalg.eigvalsh(A64)
+    assert np.allclose(ev, w, rtol=1e-10, atol=1e-10 * np.abs(w).max())
+    wo, Vo, _ = oracle.eig_jacobi(A64)
+    assert np.abs(V - Vo).max() < 1e-8
+    assert rel_l2(A64 @ V, V * ev[None, :]) < 1e-12
+
+
+def test_pca_image_other_window(st, oracle):
+    rng = np.random.default_rng(18)
+    T = 24
+    ring = crand(rng, T, 32, 32) * 50
+    V = np.linalg.qr(rng.standard_normal((T, T)) + 1j * rng.standard_normal((T, T)))[0]
+    got = host(st.pca_image(torch.from_numpy(ring).cuda(), torch.from_numpy(V.astype(np.complex128)).cuda(), 1, 4, 23))
+    want = oracle.image_type(oracle.pca_project(ring.astype(np.complex128), V), 1, 4, 23) * 131071
+    assert rel_l2(got, want) < 2e-6
+
+
+@pytest.mark.parametrize("img_type", [0, 2])
+def test_pipe_window_24(oracle, img_type):
+    """B = S = T = 24 (three-pass path: the bins-first and fused kernels are
+    built for power-of-two windows)."""
+    s, fl = frames("c1", 2, width=64, height=64, batch_size=24, time_stride=24, time_transformation_size=24,
+                   img_type=img_type, p_start=2, p_width=5)
+    check(run_both(oracle, s, fl, path=0), 1e-5 if img_type == 0 else 1e-4)
+
+
+def test_pipe_window_24_rotation(oracle):
+    """T = 24, B = S = 8: the window rotates in physical slot order (n mod 24)."""
+    s, fl = frames("c1", 5, width=64, height=64, batch_size=8, time_stride=8, time_transformation_size=24,
+                   p_start=1, p_width=4)
+    res = run_both(oracle, s, fl)
+    assert len(res) == 3
+    check(res, 1e-5)
+
+
+def test_pipe_window_24_moments(oracle):
+    from holovibes_amd.constants import HV_IMG_MOMENTS_1
+    s, fl = frames("c5_stft", 1, width=64, height=64, batch_size=24, time_stride=24, time_transformation_size=24,
+                   p_start=2, p_width=18, camera_fps=24.0, img_type=HV_IMG_MOMENTS_1)
+    check(run_both(oracle, s, fl), 1e-5)
+
+
+def test_pipe_window_24_pca(oracle):
+    """PCA over a 24-frame window: MFMA covariance of one frame group, the
+    padded eigensolver, projection onto eigenvectors 4..23."""
+    s, fl = frames("c2", 1, width=64, height=64, batch_size=24, time_stride=24, time_transformation_size=24,
+                   p_start=4, p_width=19)
+    check(run_both(oracle, s, fl), 1e-4)
+
+
+def test_pipe_camera_frame_window_24(oracle):
+    """1536 x 2048 camera frames with T = 24 (c1's other settings)."""
+    s, fl = frames("c1", 1, width=1536, height=2048, batch_size=24, time_stride=24, time_transformation_size=24)
+    check(run_both(oracle, s, fl), 1e-5)

@@ -186,6 +186,31 @@ def test_stft_matches_numpy_and_tone(oracle):
     assert np.allclose(s[5], 32) and np.abs(np.delete(s, 5, axis=0)).max() < 1e-12
 
 
+@pytest.mark.parametrize("T", [3, 12, 24, 100])
+def test_stft_other_windows_match_numpy(oracle, T):
+    # windows that are not powers of two (mixed-radix time FFT), and a pure tone
+    ring = crand(T, 3, 4)
+    assert rel_l2(oracle.stft(ring), np.fft.fft(ring, axis=0)) < 1e-13
+    tone = np.exp(2j * np.pi * 2 * np.arange(T)[:, None, None] / T) * np.ones((1, 2, 2))
+    s = oracle.stft(tone)
+    assert np.allclose(s[2], T) and np.abs(np.delete(s, 2, axis=0)).max() < 1e-11 * T
+
+
+@pytest.mark.parametrize("T,fps", [(24, 200.0), (15, 90.0)])
+def test_moments_other_windows_match_fftfreq(oracle, T, fps):
+    spec = crand(T, 3, 4)
+    f = np.fft.fftfreq(T) * fps
+    s = np.abs(spec) ** 2
+    fs, fe = 1, T - 2
+    m0, m1, m2 = oracle.moments(spec, fs, fe, fps)
+    w0, w1, w2 = s[fs:fe + 1].sum(0), (s * f[:, None, None])[fs:fe + 1].sum(0), (s * f[:, None, None] ** 2)[fs:fe + 1].sum(0)
+    if T % 2 == 0:  # Nyquist rule (tools_compute.cu:61-69): numpy puts the bin on the negative side too
+        n = s[T // 2]
+        w0, w1, w2 = w0 + n, w1 - n * f[T // 2], w2 + n * f[T // 2] ** 2
+    assert np.allclose(m0, w0, rtol=1e-6) and np.allclose(m2, w2, rtol=1e-6)
+    assert np.allclose(m1, w1, rtol=1e-5, atol=1e-6 * np.abs(w0).max() * fps)
+
+
 def test_stft_modulus_invariant_to_ring_rotation(oracle):
     ring = crand(8, 4, 4)
     a = np.abs(oracle.stft(ring))
