@@ -643,7 +643,7 @@ bool bins_split_supported(int width, int height, int T, int K, int depth) {
 
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
-                           int mm_slots, const float2* wts) {
+                           int mm_slots, const float2* wts, float2* tiles) {
     if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
         return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
     const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
@@ -654,6 +654,8 @@ hv_status launch_bins_rows(const void* raw, int depth, int width, int height, in
         const size_t P = (size_t)width * height;
         if (wts) HV_TRY(pca_time(raw, depth, planes, chirp, P, T, slot0, K, wts, mm_reset, mm_slots, st));
         else HV_TRY(bins_time(raw, depth, planes, chirp, P, T, slot0, pmin, K, mm_reset, mm_slots, st));
+        // column-tiled planes: the forward row pass writes the tiles the column pass reads
+        if (tiles) return launch_row_pass_tiled(planes, tiles, width, height, K, -1, st);
         return launch_row_pass(planes, width, height, K, -1, st);
     }
     switch (width) {
@@ -666,18 +668,25 @@ hv_status launch_bins_rows(const void* raw, int depth, int width, int height, in
 }
 
 hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
-                              hipStream_t st) {
+                              hipStream_t st, float2* tiles, const float2* lens_t) {
+    if (tiles) return launch_col_lens_tiled(tiles, lens_t, width, height, K, st);
     return launch_col_pass(planes, space == HV_SPACE_ANGULARSP ? lens : nullptr, width, height, K,
                            space == HV_SPACE_ANGULARSP ? COLPASS_LENS : COLPASS_FWD, st);
 }
 
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots, const float2* wts) {
+                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots, const float2* wts, float2* tiles,
+                                const float2* lens_t) {
     HV_TRY(launch_bins_rows(raw, depth, width, height, T, slot0, pmin, K, space, lens, planes, st, mm_reset,
-                            mm_slots, wts));
+                            mm_slots, wts, tiles));
     if (mark) HV_HIP(hipEventRecord(mark, st));
-    return launch_bins_columns(planes, width, height, K, space, lens, st);
+    return launch_bins_columns(planes, width, height, K, space, lens, st, tiles, lens_t);
+}
+
+bool bins_tiled_supported(int width, int height, int T, int K, int depth, int space, bool pca) {
+    return space == HV_SPACE_ANGULARSP && !bins_first_supported(width, height, T, K, depth, pca) &&
+           bins_split_supported(width, height, T, K, depth) && col_tiled_supported(width, height);
 }
 
 // The inverse row FFT and the image term in one kernel (image rows of
@@ -688,10 +697,12 @@ bool bins_image_fused(int width, int space) {
 
 template <int IMG>
 hv_status bins_image_t(float* out, float2* planes, int width, int height, int K, int space, float count,
-                       uint32_t* mm, int mm_slots, float scale, hipStream_t st) {
-    if (!bins_image_fused(width, space)) {
-        // split: the planes' inverse row FFT (angular spectrum), then the image term
-        if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
+                       uint32_t* mm, int mm_slots, float scale, hipStream_t st, float2* tiles) {
+    if (tiles || !bins_image_fused(width, space)) {
+        // split: the planes' inverse row FFT (angular spectrum; from the column
+        // tiles back to row-major planes), then the image term
+        if (tiles) HV_TRY(launch_row_pass_tiled(tiles, planes, width, height, K, +1, st));
+        else if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
         const size_t P = (size_t)width * height;
         const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
         k_bins_image<IMG><<<blocks, 256, 0, st>>>(planes, out, P, K, count, mm, mm_slots, scale);
@@ -714,9 +725,10 @@ hv_status bins_image_t(float* out, float2* planes, int width, int height, int K,
 
 hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* planes_c, int width, int height, int T,
                               int fs, int K, int space, float fps, int mm_moment, uint32_t* mm, hipStream_t st,
-                              int mm_slots) {
+                              int mm_slots, float2* tiles) {
     float2* planes = const_cast<float2*>(planes_c);
-    if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
+    if (tiles) HV_TRY(launch_row_pass_tiled(tiles, planes, width, height, K, +1, st));
+    else if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
     const size_t P = (size_t)width * height;
     const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
     k_bins_moments<<<blocks, 256, 0, st>>>(planes, m0, m1, m2, nullptr, P, T, fs, K, fps, mm_moment, mm, mm_slots);
@@ -725,16 +737,19 @@ hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* pla
 }
 
 hv_status launch_bins_image(float* out, const float2* planes_c, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots, float scale) {
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots, float scale,
+                            float2* tiles) {
     float2* planes = const_cast<float2*>(planes_c);
     const float count = (float)(pmax - pmin + 1);
     switch (img_type) {
         case HV_IMG_MODULUS:
-            return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st);
+            return bins_image_t<HV_IMG_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st, tiles);
         case HV_IMG_SQUARED_MODULUS:
-            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st);
+            return bins_image_t<HV_IMG_SQUARED_MODULUS>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st,
+                                                        tiles);
         case HV_IMG_ARGUMENT:
-            return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st);
+            return bins_image_t<HV_IMG_ARGUMENT>(out, planes, width, height, K, space, count, mm, mm_slots, scale, st,
+                                                 tiles);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: image type must be modulus, squared or argument");
     }
 }

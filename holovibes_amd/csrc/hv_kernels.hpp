@@ -26,6 +26,18 @@ hv_status launch_col_pass(float2* buf, const float2* lens, int width, int height
                           hipStream_t st);
 // One row pass over `planes` frames of `buf` (in place, unnormalised): direction -1 forward, +1 inverse.
 hv_status launch_row_pass(float2* buf, int width, int height, int planes, int direction, hipStream_t st);
+// Column-tiled planes for 2048- and 4096-long columns (power-of-two widths):
+// element (y, x) at ((x / CW) H + y) CW + x mod CW with CW the column tile of
+// the column pass, so that every column tile is one contiguous block.
+bool col_tiled_supported(int width, int height);
+// direction -1: forward row FFT, row-major in, tiled out; +1: inverse row
+// FFT, tiled in, row-major out (out of place).
+hv_status launch_row_pass_tiled(const float2* in, float2* out, int width, int height, int planes, int direction,
+                                hipStream_t st);
+// column FFT x lens x inverse column FFT of tiled planes (in place), lens tiled too
+hv_status launch_col_lens_tiled(float2* buf, const float2* lens_t, int width, int height, int planes,
+                                hipStream_t st);
+hv_status launch_tile_frame(float2* out, const float2* in, int width, int height, hipStream_t st);
 hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int width, int height, int space,
                                    const float2* lens, float2* ring, int ring_size, int slot0, hipStream_t st,
                                    hipEvent_t* marks = nullptr, bool spectral_rows = false,
@@ -59,7 +71,12 @@ bool bins_split_supported(int width, int height, int T, int K, int depth);
 // window (inverse row FFT first for the angular spectrum).
 hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* planes, int width, int height, int T,
                               int fs, int K, int space, float fps, int mm_moment, uint32_t* mm, hipStream_t st,
-                              int mm_slots = 1);
+                              int mm_slots = 1, float2* tiles = nullptr);
+// The split path with column-tiled planes (angular spectrum, 2048- and
+// 4096-long columns): `tiles` (K x P, apart from `planes`) holds the planes
+// between the forward row pass and the inverse row pass, `lens_t` the lens in
+// the same layout; the inverse row pass writes row-major planes back.
+bool bins_tiled_supported(int width, int height, int T, int K, int depth, int space, bool pca = false);
 // Bins + row FFTs (x chirp for Fresnel) into planes[0 .. K) (K x P complex),
 // then the column pass over them; launch_bins_propagate runs both, with
 // `mark` (optional) recorded in between.
@@ -68,17 +85,19 @@ hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* pla
 // column pmin, readable 8 columns past the last) instead of DFT bin pmin + k.
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
-                           int mm_slots = 1, const float2* wts = nullptr);
+                           int mm_slots = 1, const float2* wts = nullptr, float2* tiles = nullptr);
 hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
-                              hipStream_t st);
+                              hipStream_t st, float2* tiles = nullptr, const float2* lens_t = nullptr);
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
-                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots = 1, const float2* wts = nullptr);
+                                hipEvent_t mark, uint32_t* mm_reset, int mm_slots = 1, const float2* wts = nullptr,
+                                float2* tiles = nullptr, const float2* lens_t = nullptr);
 // Image of the propagated planes: inverse row FFT (angular spectrum) + image
 // term, mean over [pmin, pmax], x scale (131071 for PCA images); mm: encoded
 // min/max (may be null).
 hv_status launch_bins_image(float* out, const float2* planes, int width, int height, int K, int space, int img_type,
-                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1, float scale = 1.f);
+                            int pmin, int pmax, uint32_t* mm, hipStream_t st, int mm_slots = 1, float scale = 1.f,
+                            float2* tiles = nullptr);
 
 // hv_render.hip
 // mm: `slots` accumulator pairs (block_minmax_atomic); the C-ABI uses one.

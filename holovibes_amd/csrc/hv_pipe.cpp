@@ -79,6 +79,18 @@ bool bins_eligible(const hv_compute_settings& s) {
     return bins_first_supported(W, H, T, K, (int)s.depth) || bins_split_supported(W, H, T, K, (int)s.depth);
 }
 
+// The bins-first split path with column-tiled planes (2048- and 4096-long
+// columns, angular spectrum): the tiles take the ring slots after the K
+// planes, so the window must hold 2 K frames.
+bool bins_tiled(const hv_compute_settings& s) {
+    if (!bins_eligible(s)) return false;
+    int lo, K;
+    bins_range(s, lo, K);
+    const int T = (int)s.time_transformation_size;
+    return 2 * K <= T && bins_tiled_supported((int)s.width, (int)s.height, T, K, (int)s.depth,
+                                              s.space_transformation, s.time_transformation == HV_TIME_PCA);
+}
+
 // Execution path of a steady-state cycle.
 enum { PATH_THREE_PASS = 0, PATH_ROW_STFT = 1, PATH_BINS = 2 };
 
@@ -90,7 +102,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 }
 
 struct Layout {
-    size_t ring, lens, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, evecs32, pca_ws,
+    size_t ring, lens, lens_t, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, evecs32, pca_ws,
         eig_ws;
     size_t accu, renorm, cuts, reg, outq;
     size_t total;
@@ -102,6 +114,7 @@ Layout layout_of(const hv_compute_settings& s) {
     const size_t T = s.time_transformation_size;
     l.ring = align256(T * P * sizeof(float2));
     l.lens = s.space_transformation != HV_SPACE_NONE ? align256(P * sizeof(float2)) : 0;
+    l.lens_t = bins_tiled(s) ? l.lens : 0;  // the lens again, column-tiled
     l.staging = align256((size_t)s.batch_size * P * s.depth);
     l.image = align256(P * sizeof(float));
     l.cimage = align256(P * sizeof(float));
@@ -132,7 +145,7 @@ Layout layout_of(const hv_compute_settings& s) {
     l.reg = s.registration_enabled ? align256(registration_scratch_bytes(P)) + align256(P * sizeof(float)) : 0;
     // the output queue: OutputBufferSize u16 images
     l.outq = align256((size_t)s.output_buffer_size * P * sizeof(uint16_t));
-    l.total = l.outq + l.ring + l.lens + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
+    l.total = l.outq + l.ring + l.lens + l.lens_t + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
               l.evals + l.evecs + l.evecs32 + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts + l.reg;
     return l;
 }
@@ -154,6 +167,7 @@ constexpr int kPipeMm = HV_PIPE_MM_MODE;
 struct Lane {
     hipStream_t stream = nullptr;
     float2* planes = nullptr;  // bins-first planes (lane 0: the ring's first K frames)
+    float2* tiles = nullptr;   // column-tiled planes (lane 0: the ring's next K frames), when the path tiles
     // time-window ring and PCA buffers (lane 0: the pipe's; further lanes of a
     // window-replaced three-pass pipe: their own)
     float2* ring = nullptr;
@@ -199,6 +213,7 @@ struct hv_pipe {
     char* base = nullptr;
     float2* ring = nullptr;
     float2* lens = nullptr;
+    float2* lens_t = nullptr;  // the lens column-tiled (bins_tiled)
     void* staging = nullptr;
     double2* cov = nullptr;
     double* evals = nullptr;
@@ -346,7 +361,8 @@ hv_status hv_pipe::image_work() {
             // PCA images are scaled by 2^17 - 1 after the mean (converts.cc:50-61)
             const float scale = s.time_transformation == HV_TIME_PCA ? (float)((2 << 16) - 1) : 1.0f;
             HV_TRY(launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, pmax - pmin + 1,
-                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kPipeMm, scale));
+                                     s.space_transformation, it, pmin, pmax, mm_img, L.stream, kPipeMm, scale,
+                                     L.tiles));
         } else if (s.time_transformation == HV_TIME_STFT && path == PATH_ROW_STFT) {
             HV_TRY(launch_row_ifft_stft_image(L.image, L.ring, (int)s.width, (int)s.height, Tn, it, pmin, pmax, mm_img,
                                               L.stream, kPipeMm));
@@ -372,7 +388,7 @@ hv_status hv_pipe::image_work() {
         const int k = it - HV_IMG_MOMENTS_0;
         HV_TRY(launch_bins_moments(k == 0 ? L.image : nullptr, k == 1 ? L.image : nullptr, k == 2 ? L.image : nullptr,
                                    L.planes, (int)s.width, (int)s.height, Tn, lo, K, s.space_transformation,
-                                   s.camera_fps, k, mm_img, L.stream, kPipeMm));
+                                   s.camera_fps, k, mm_img, L.stream, kPipeMm, L.tiles));
     } else {
         // insert_moments (pipe.cc:429-455): f_end = min(start + width, T - 1)
         const int fs = (int)s.p_start;
@@ -492,7 +508,7 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
         }
         HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, lo, K,
                                      s.space_transformation, lens, L.planes, L.stream, timing ? ev[1] : nullptr,
-                                     minmax_fused() ? L.mm : nullptr, kPipeMm, wts));
+                                     minmax_fused() ? L.mm : nullptr, kPipeMm, wts, L.tiles, lens_t));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
@@ -661,8 +677,14 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     const Layout& l = p->lay;
     p->ring = (float2*)take(l.ring);
     p->lens = (float2*)take(l.lens);
+    p->lens_t = (float2*)take(l.lens_t);
     p->staging = take(l.staging);
     L0.planes = p->ring;
+    if (l.lens_t) {
+        int lo, K;
+        bins_range(*s, lo, K);
+        L0.tiles = p->ring + (size_t)K * p->P;
+    }
     L0.image = (float*)take(l.image);
     L0.cimage = (float*)take(l.cimage);
     L0.tmp = (float*)take(l.tmp);
@@ -705,6 +727,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     // refresh-time work: lens, manual contrast range, zeroed ring
     hv_status st = hv_prepare_lens(p->lens, s->space_transformation, s->width, s->height, s->lambda, s->z_distance,
                                    s->pixel_size, s->lens_precision, L0.stream);
+    if (st == HV_OK && p->lens_t) st = launch_tile_frame(p->lens_t, p->lens, (int)s->width, (int)s->height, L0.stream);
     if (st == HV_OK) {
         const float r[2] = {s->contrast_min, s->contrast_max};
         e = hipMemcpyAsync(L0.range, r, sizeof(r), hipMemcpyHostToDevice, L0.stream);
@@ -1055,7 +1078,8 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         HV_HIP(hipEventCreateWithFlags(&p->lane[0].tt_done, hipEventDisableTiming));
     for (int i = (int)p->lane.size(); i < lanes; ++i) {
         Lane n;
-        const size_t planes = bins ? align256(K * p->P * sizeof(float2)) : 0;
+        // column-tiled bins: the tiles after the planes
+        const size_t planes = bins ? align256((p->lens_t ? 2 : 1) * K * p->P * sizeof(float2)) : 0;
         const size_t ring = own_ring ? l.ring : 0;
         // own PCA buffers: window-replaced three-pass lanes and bins-first PCA lanes
         const bool own_pca = own_ring || (bins && p->s.time_transformation == HV_TIME_PCA);
@@ -1083,6 +1107,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
         };
         n.kind = kind;
         n.planes = (float2*)take(planes);
+        if (bins && p->lens_t) n.tiles = n.planes + K * p->P;
         if (own_ring) n.ring = (float2*)take(ring);
         if (own_pca) {
             n.cov = (double2*)take(l.cov);
@@ -1146,7 +1171,7 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
             case 0:
                 st = launch_bins_rows(frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
                                       K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kPipeMm,
-                                      pca ? L.evecs32 + pmin : nullptr);
+                                      pca ? L.evecs32 + pmin : nullptr, L.tiles);
                 break;
             case 2:
                 st = launch_pca_cov_raw(L.cov, frames[i % n_frames], (int)s.depth, p->T, p->P, 0, s.pca_precision,
@@ -1157,17 +1182,17 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
                 break;
             case 1:
                 st = launch_bins_columns(L.planes, (int)s.width, (int)s.height, K, s.space_transformation, p->lens,
-                                         L.stream);
+                                         L.stream, L.tiles, p->lens_t);
                 break;
             case 3:
                 if (is_moment_kind(s.img_type))
                     st = launch_bins_moments(L.image, nullptr, nullptr, L.planes, (int)s.width, (int)s.height, p->T,
                                              pmin, K, s.space_transformation, s.camera_fps, 0, L.mm, L.stream,
-                                             kPipeMm);
+                                             kPipeMm, L.tiles);
                 else
                     st = launch_bins_image(L.image, L.planes, (int)s.width, (int)s.height, K, s.space_transformation,
                                            s.img_type, pmin, pmax, L.mm, L.stream, kPipeMm,
-                                           pca ? (float)((2 << 16) - 1) : 1.0f);
+                                           pca ? (float)((2 << 16) - 1) : 1.0f, L.tiles);
                 break;
             default:
                 st = launch_contrast_to_ushort(L.out, L.cimage, L.image, p->P, L.mm, kPipeMm, s.contrast_invert, L.stream,

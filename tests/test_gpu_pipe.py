@@ -239,6 +239,10 @@ def test_pipe_moments(oracle, img_type, level):
     (2048, 16, 16, 2, 0, 2, 5),     # W = 2048: split row passes, modulus image
     (2048, 32, 8, 1, 1, 1, 3),      # Fresnel, W = 2048, squared modulus
     (128, 64, 128, 1, 5, 0, 127),   # Fresnel, T = 128, every bin (the Nyquist rule at the range edge)
+    # 2048- and 4096-long columns: the planes column-tiled between the row passes (2 K <= T)
+    (64, 2048, 128, 2, 5, 4, 56),   # four-column tiles, moment 0 over 57 bins
+    (128, 4096, 128, 2, 0, 2, 9),   # two-column tiles, modulus image
+    (64, 2048, 64, 2, 6, 3, 40),    # 2 K > T: row-major planes, moment 1
 ])
 def test_pipe_bins_split(oracle, W, H, T, space, img_type, p0, pw):
     """The bins-first path as separate passes (frames wider than 1024 or
@@ -396,6 +400,25 @@ def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
         many.push(d)
     assert np.array_equal(one.output_u16(), many.output_u16())
     assert np.array_equal(one.contrast_range(), many.contrast_range())
+    one.close()
+    many.close()
+
+
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_pipe_bins_column_tiles_lanes(lanes):
+    """Column-tiled bins (each extra lane holds its planes and tiles): the
+    lanes' outputs are bit-identical to one lane's."""
+    from holovibes_amd.pipe import Pipe
+    s, fl = frames("c5_stft", 4, width=64, height=2048, batch_size=128, time_stride=128, time_transformation_size=128,
+                   p_start=4, p_width=56, camera_fps=128.0, syn_period=128)
+    dev = [torch.from_numpy(f.view(np.int16)).cuda() for f in fl]
+    torch.cuda.synchronize()
+    one, many = Pipe(s, fused=2), Pipe(s, fused=2, lanes=lanes)
+    assert one.path == many.path == Pipe.BINS
+    for d in dev + dev[::-1]:
+        assert one.push(d) and many.push(d)
+        assert np.array_equal(one.output_u16(), many.output_u16())
+        assert torch.equal(one.image(), many.image())
     one.close()
     many.close()
 
