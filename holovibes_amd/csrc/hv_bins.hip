@@ -50,6 +50,10 @@ namespace {
 
 constexpr int kBinThreads = 512;  // time DFT + row FFT kernel
 
+#ifndef HV_BINS_ROW_E
+#define HV_BINS_ROW_E 8         // values per thread of the plane-row FFTs, rows up to 512
+#endif
+
 // inverse row FFT + image kernel: its exchange rows and partial sums stay
 // within LDS
 template <int W>
@@ -59,7 +63,7 @@ constexpr int img_threads() {
 
 template <int W, int THREADS = kBinThreads>
 struct BinRowCfg {
-    static constexpr int E = W >= 1024 ? 16 : 8;
+    static constexpr int E = W >= 1024 ? 16 : HV_BINS_ROW_E;
     static constexpr int NT = FftShape<W, E>::NT;          // threads per row transform
     static constexpr int G = THREADS / NT;                  // row transforms at once
     static constexpr int PITCH = W + W / 8;                  // padded LDS row

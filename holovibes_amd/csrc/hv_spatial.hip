@@ -204,9 +204,24 @@ template <int N>
 struct RowCfg {
     static constexpr int E = N >= 1024 ? 16 : HV_ROW_E_SMALL;
     static constexpr int NT = FftShape<N, E>::NT;
-    static constexpr int M = (HV_ROW_THREADS / NT) > 0 ? (HV_ROW_THREADS / NT) : 1;  // rows per workgroup
-    static constexpr int THREADS = M * NT;
+    static constexpr int M0 = (HV_ROW_THREADS / NT) > 0 ? (HV_ROW_THREADS / NT) : 1;  // rows per workgroup
+    static constexpr int THREADS0 = M0 * NT;
     static constexpr int PITCH = N + N / 8;                      // padded LDS row
+};
+
+#ifndef HV_ROW_TILE_MUL
+#define HV_ROW_TILE_MUL 2       // rows per workgroup x this when the row pass writes column tiles
+#endif
+
+// Rows per workgroup of a row pass with layout LAY (1: writes column tiles,
+// see LAY_TO_TILES below): tiles hold 2^tl columns x H rows, so 4-row groups
+// write whole 128-byte lines of 4-column tiles (57 planes of 2048^2:
+// 1009 -> 910 us; doubling the rows of the pass that reads the tiles made it
+// 45 us slower).
+template <int N, int LAY>
+struct RowCfgL : RowCfg<N> {
+    static constexpr int M = RowCfg<N>::M0 * (LAY == 1 ? HV_ROW_TILE_MUL : 1);
+    static constexpr int THREADS = M * RowCfg<N>::NT;
 };
 
 // Frames of the batch map to output frame (slot0 + b) mod ring (ring > 0), or
@@ -255,10 +270,10 @@ enum { LAY_ROWS = 0, LAY_TO_TILES = 1, LAY_FROM_TILES = 2 };
 // of group i + 1 (pixels, and chirp factors with CHIRP) are issued before
 // group i is computed, so a workgroup keeps two groups of input in flight.
 template <int N, int DIR, int KIN, bool CHIRP, int LAY = LAY_ROWS>
-__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+__global__ void __launch_bounds__((RowCfgL<N, LAY>::THREADS))
     k_row_fft(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
               FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int mm_slots, int iters, int tl = 0) {
-    using C = RowCfg<N>;
+    using C = RowCfgL<N, LAY>;
     constexpr int kE = FftShape<N, C::E>::kE;
     __shared__ float2 lds[C::M * C::PITCH];
     // the cycle's first kernel re-arms the image min/max accumulators of its last kernels
@@ -673,7 +688,7 @@ template <int N, int DIR>
 static hv_status row_pass_n(const void* in, int kin, float2* out, const float2* chirp, int height, int batch,
                             FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset,
                             int mm_slots) {
-    using C = RowCfg<N>;
+    using C = RowCfgL<N, LAY_ROWS>;
     // enough workgroups to fill the chip, each with a few row groups to pipeline
     const int groups = (height + C::M - 1) / C::M;
     int iters = 1;
@@ -777,7 +792,7 @@ int col_tile_log2(int height) {
 
 template <int N, int DIR, int LAY>
 static hv_status row_pass_lay_n(const float2* in, float2* out, int height, int batch, int tl, hipStream_t st) {
-    using C = RowCfg<N>;
+    using C = RowCfgL<N, LAY>;
     const int groups = (height + C::M - 1) / C::M;
     int iters = 1;
     while (iters < 8 && groups % (iters * 2) == 0 && (size_t)(groups / (iters * 2)) * batch >= HV_ROW_MIN_WG) iters *= 2;
