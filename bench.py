@@ -618,7 +618,8 @@ def main():
         key, val = kv.split("=", 1)
         overrides[key] = float(val) if "." in val or "e" in val.lower() else int(val)
     s = configs.settings_for(name, **overrides)
-    sp = configs.synth_for(name)
+    # the source follows overridden frame sizes, depths and windows
+    sp = configs.synth_for(name, width=s.width, height=s.height, depth=s.depth, period=s.time_transformation_size)
     B, S, T, P = s.batch_size, s.time_stride, s.time_transformation_size, s.width * s.height
     host_batches = [synth_frames(sp, k * B, B) for k in range(4)]
     to_dev = (lambda a: torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).cuda())

@@ -447,14 +447,19 @@ __global__ void __launch_bounds__(ColCfg<N>::THREADS)
 // Frame sides 2^a 3^b 5^c that are not powers of two (camera geometries such
 // as 1536 x 2048 or 1920 x 1080; cufftPlanMany accepts any n,
 // icompute.cc:151-175).  Runtime-length Stockham FFT: the plan's radices
-// (8, 4, 2, then 3s, 5s) are walked at run time, each stage reading buffer a
-// and writing buffer b in LDS (a transform of CW interleaved columns), with
-// twiddles exp(dir 2 pi i e / M) from sincospif of the exact ratio e / M.
+// (8, 4, 2, then 3s, 5s) are walked at run time, each stage in place in LDS
+// (a transform of CW interleaved columns): every thread first reads the
+// inputs of all its butterflies into registers (n / nt values), the
+// workgroup synchronises, then each writes its outputs.  Twiddles come from
+// an LDS table of exp(-2 pi i k / n), k < n, built once per workgroup with
+// sincospif: exp(dir 2 pi i e / M) = table[e n / M] for M | n.
 // ---------------------------------------------------------------------------
 struct AnyPlan {
     int n, nst;
     int r[16];
 };
+
+constexpr int kAnyVals = 40;  // values per thread a stage may hold (n / nt <= 32 with nt = 64 or 128)
 
 static bool any_plan(int n, AnyPlan& p) {
     if (n < 16 || n > 4096 || (n & 1)) return false;
@@ -469,17 +474,18 @@ static bool any_plan(int n, AnyPlan& p) {
     return m == 1 && p.nst <= 16;
 }
 
+// every stage's butterflies fit the threads' registers: ceil(n / (R nt)) <= kAnyVals / R
+static bool any_plan_fits(const AnyPlan& p, int nt) {
+    for (int s = 0; s < p.nst; ++s) {
+        const int R = p.r[s], m = p.n / R;
+        if ((m + nt - 1) / nt > kAnyVals / R) return false;
+    }
+    return true;
+}
+
 bool fft_len_any_ok(long n) {
     AnyPlan p;
     return n <= 4096 && any_plan((int)n, p);
-}
-
-// exp(dir 2 pi i e / m), 0 <= e < m
-template <int DIR>
-__device__ __forceinline__ float2 tw_any(int e, int m) {
-    float s, c;
-    sincospif(2.0f * (float)e / (float)m, &s, &c);
-    return make_float2(c, DIR < 0 ? -s : s);
 }
 
 // radix-3 and radix-5 DFTs, natural order in and out
@@ -513,81 +519,114 @@ __device__ __forceinline__ void dft5(float2* v) {
     v[3] = csub(a2, q2);
 }
 
-// One Stockham stage: butterfly j reads a[j + r m], m = n / R, multiplies by
-// w^r (w = exp(dir 2 pi i (j mod ns) / (ns R))), runs the radix-R DFT and
-// writes b[(j / ns) ns R + (j mod ns) + r ns].
-template <int R, int DIR, int CW>
-__device__ __forceinline__ void any_stage(const float2* a, float2* b, int m, int ns, int c, int t, int nt) {
-    for (int j = t; j < m; j += nt) {
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = a[(j + r * m) * CW + c];
-        const int k = j % ns;
-        if (ns > 1) {
-#pragma unroll
-            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw_any<DIR>(k * r, ns * R));
-        }
-        if constexpr (R == 3) dft3<DIR>(v);
-        else if constexpr (R == 5) dft5<DIR>(v);
-        else dft_reg<R, DIR>(v);
-        const int d = (j / ns) * ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) b[(d + r * ns) * CW + c] = v[r];
+// table[k] = exp(-2 pi i k / n), k < n, by the whole workgroup
+__device__ __forceinline__ void any_twiddle_table(float2* table, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        float s, c;
+        sincospif(2.0f * (float)k / (float)n, &s, &c);
+        table[k] = make_float2(c, -s);
     }
 }
 
-// The whole transform (input in a, natural order); returns the buffer holding
-// the output.  Every thread of the workgroup calls it (it contains barriers).
+// One Stockham stage in place: butterfly j reads a[j + r m], m = n / R,
+// multiplies by w^r (w = exp(dir 2 pi i (j mod ns) / (ns R))), runs the
+// radix-R DFT and writes a[(j / ns) ns R + (j mod ns) + r ns].
+template <int R, int DIR, int CW>
+__device__ __forceinline__ void any_stage(float2* a, const float2* table, int n, int ns, int c, int t, int nt) {
+    constexpr int MAXB = kAnyVals / R;
+    const int m = n / R, step = n / (ns * R);
+    float2 v[MAXB * R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = t + b * nt;
+        if (j < m) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = a[(j + r * m) * CW + c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = t + b * nt;
+        if (j < m) {
+            float2* x = v + b * R;
+            const int k = j % ns;
+            if (ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = table[k * r * step];  // exp(-2 pi i k r / (ns R))
+                    x[r] = DIR < 0 ? cmul(x[r], w) : cmulc(x[r], w);
+                }
+            }
+            if constexpr (R == 3) dft3<DIR>(x);
+            else if constexpr (R == 5) dft5<DIR>(x);
+            else dft_reg<R, DIR>(x);
+            const int d = (j / ns) * ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) a[(d + r * ns) * CW + c] = x[r];
+        }
+    }
+    __syncthreads();
+}
+
+// The whole transform, in place in a (natural order in and out).  Every
+// thread of the workgroup calls it (it contains barriers); the twiddle table
+// is ready.
 template <int DIR, int CW>
-__device__ float2* any_fft(float2* a, float2* b, const AnyPlan& pl, int c, int t, int nt) {
+__device__ void any_fft(float2* a, const float2* table, const AnyPlan& pl, int c, int t, int nt) {
     const int n = pl.n;
     int ns = 1;
     for (int s = 0; s < pl.nst; ++s) {
-        const int R = pl.r[s], m = n / R;
+        const int R = pl.r[s];
         switch (R) {
-            case 8: any_stage<8, DIR, CW>(a, b, m, ns, c, t, nt); break;
-            case 4: any_stage<4, DIR, CW>(a, b, m, ns, c, t, nt); break;
-            case 2: any_stage<2, DIR, CW>(a, b, m, ns, c, t, nt); break;
-            case 3: any_stage<3, DIR, CW>(a, b, m, ns, c, t, nt); break;
-            default: any_stage<5, DIR, CW>(a, b, m, ns, c, t, nt); break;
+            case 8: any_stage<8, DIR, CW>(a, table, n, ns, c, t, nt); break;
+            case 4: any_stage<4, DIR, CW>(a, table, n, ns, c, t, nt); break;
+            case 2: any_stage<2, DIR, CW>(a, table, n, ns, c, t, nt); break;
+            case 3: any_stage<3, DIR, CW>(a, table, n, ns, c, t, nt); break;
+            default: any_stage<5, DIR, CW>(a, table, n, ns, c, t, nt); break;
         }
-        __syncthreads();
-        float2* x = a;
-        a = b;
-        b = x;
         ns *= R;
     }
-    return a;
 }
 
 constexpr int kAnyRowThreads = 256;
 
-// Row pass of any supported length: one row of one frame per workgroup.
-template <int KIN, bool CHIRP, int DIR>
+// Row pass of any supported length: RPW rows of one frame per workgroup
+// (256 / RPW threads per row), sharing one twiddle table; rows past the
+// frame's height are computed on row 0 and not stored.
+template <int KIN, bool CHIRP, int DIR, int RPW>
 __global__ void __launch_bounds__(kAnyRowThreads)
     k_any_row(const void* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ chirp, int height,
               FrameMap fin, FrameMap fm, float scale, uint32_t* mm_reset, int mm_slots, AnyPlan pl) {
+    constexpr int NT = kAnyRowThreads / RPW;
     extern __shared__ float2 any_sh[];
     if (mm_reset && blockIdx.x == 0 && blockIdx.y == 0) mm_reset_slots(mm_reset, mm_slots);
-    const int n = pl.n, y = blockIdx.x, b = blockIdx.y;
-    const size_t P = (size_t)n * height, row = (size_t)y * n;
+    const int n = pl.n, b = blockIdx.y;
+    const int g = threadIdx.x / NT, t = threadIdx.x % NT;
+    const int y = blockIdx.x * RPW + g;
+    const bool valid = y < height;
+    const size_t P = (size_t)n * height, row = (size_t)(valid ? y : 0) * n;
     const size_t fi = fin.out_frame(b) * P;
-    float2* a = any_sh;
-    for (int i = threadIdx.x; i < n; i += kAnyRowThreads) {
+    float2* a = any_sh + (size_t)g * n;
+    float2* table = any_sh + (size_t)RPW * n;
+    for (int i = t; i < n; i += NT) {
         float2 v = px_complex<KIN>(load_px<KIN>(in, fi + row + i));
         if constexpr (CHIRP) v = cmul(v, chirp[row + i]);
         a[i] = v;
     }
+    any_twiddle_table(table, n);
     __syncthreads();
-    const float2* res = any_fft<DIR, 1>(a, any_sh + n, pl, 0, threadIdx.x, kAnyRowThreads);
+    any_fft<DIR, 1>(a, table, pl, 0, t, NT);
     float2* dst = out + fm.out_frame(b) * P + row;
-    for (int i = threadIdx.x; i < n; i += kAnyRowThreads) dst[i] = scale == 1.0f ? res[i] : cscale(res[i], scale);
+    if (valid)
+        for (int i = t; i < n; i += NT) dst[i] = scale == 1.0f ? a[i] : cscale(a[i], scale);
 }
 
-// Column pass of any supported length: CW columns of one frame per
-// workgroup (columns past the frame's width are computed on zeros and not stored).
-template <int MODE, int CW>
-__global__ void __launch_bounds__(64 * CW)
+// Column pass of any supported length: CW columns of one frame per workgroup,
+// NT threads per column (columns past the frame's width are computed on zeros
+// and not stored).
+template <int MODE, int CW, int NT>
+__global__ void __launch_bounds__(NT * CW)
     k_any_col(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ lens, int width,
               FrameMap fin, FrameMap fm, AnyPlan pl) {
     extern __shared__ float2 any_sh[];
@@ -598,22 +637,22 @@ __global__ void __launch_bounds__(64 * CW)
     const size_t P = (size_t)n * width;
     const float2* src = in + fin.out_frame(b) * P + x;
     float2* a = any_sh;
-    float2* bb = any_sh + (size_t)n * CW;
-    for (int i = t; i < n; i += 64) a[i * CW + c] = valid ? src[(size_t)i * width] : make_float2(0.f, 0.f);
+    float2* table = any_sh + (size_t)n * CW;
+    for (int i = t; i < n; i += NT) a[i * CW + c] = valid ? src[(size_t)i * width] : make_float2(0.f, 0.f);
+    any_twiddle_table(table, n);
     __syncthreads();
-    float2* res;
     if constexpr (MODE == COL_LENS) {
-        res = any_fft<-1, CW>(a, bb, pl, c, t, 64);
-        for (int i = t; i < n; i += 64)
-            if (valid) res[i * CW + c] = cmul(res[i * CW + c], lens[(size_t)i * width + x]);
+        any_fft<-1, CW>(a, table, pl, c, t, NT);
+        for (int i = t; i < n; i += NT)
+            if (valid) a[i * CW + c] = cmul(a[i * CW + c], lens[(size_t)i * width + x]);
         __syncthreads();
-        res = any_fft<+1, CW>(res, res == a ? bb : a, pl, c, t, 64);
+        any_fft<+1, CW>(a, table, pl, c, t, NT);
     } else {
-        res = any_fft<MODE == COL_FWD ? -1 : +1, CW>(a, bb, pl, c, t, 64);
+        any_fft<MODE == COL_FWD ? -1 : +1, CW>(a, table, pl, c, t, NT);
     }
     float2* dst = out + fm.out_frame(b) * P + x;
     if (valid)
-        for (int i = t; i < n; i += 64) dst[(size_t)i * width] = res[i * CW + c];
+        for (int i = t; i < n; i += NT) dst[(size_t)i * width] = a[i * CW + c];
 }
 
 template <class K>
@@ -624,20 +663,18 @@ static hv_status any_lds_attr(K* kernel, size_t bytes) {
     return HV_OK;
 }
 
-template <int DIR>
-static hv_status any_row_pass(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
-                              int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset,
-                              int mm_slots) {
-    AnyPlan pl;
-    if (!any_plan(width, pl)) return fail(HV_ERR_UNSUPPORTED, "width must be an even 2^a 3^b 5^c in [16, 4096]");
-    if (chirp && DIR > 0) return fail(HV_ERR_UNSUPPORTED, "chirp factors on an inverse row pass");
-    const size_t lds = 2 * (size_t)width * sizeof(float2);
-    const dim3 grid(height, batch);
-#define HV_ANY_ROW(KIN, CH)                                                                                      \
-    do {                                                                                                         \
-        HV_TRY(any_lds_attr(&k_any_row<KIN, CH, DIR>, lds));                                                     \
-        k_any_row<KIN, CH, DIR><<<grid, kAnyRowThreads, lds, st>>>(in, out, chirp, height, fin, fm, scale, mm_reset, \
-                                                                  mm_slots, pl);                                 \
+template <int DIR, int RPW>
+static hv_status any_row_launch(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
+                                int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset,
+                                int mm_slots, const AnyPlan& pl) {
+    if (!any_plan_fits(pl, kAnyRowThreads / RPW)) return fail(HV_ERR_UNSUPPORTED, "row length not supported");
+    const size_t lds = (size_t)(RPW + 1) * width * sizeof(float2);  // the rows and the twiddle table
+    const dim3 grid((height + RPW - 1) / RPW, batch);
+#define HV_ANY_ROW(KIN, CH)                                                                                       \
+    do {                                                                                                          \
+        HV_TRY(any_lds_attr(&k_any_row<KIN, CH, DIR, RPW>, lds));                                                 \
+        k_any_row<KIN, CH, DIR, RPW><<<grid, kAnyRowThreads, lds, st>>>(in, out, chirp, height, fin, fm, scale,   \
+                                                                        mm_reset, mm_slots, pl);                  \
     } while (0)
     if (chirp) {
         if constexpr (DIR < 0) {
@@ -661,24 +698,40 @@ static hv_status any_row_pass(const void* in, int kin, float2* out, const float2
     return HV_OK;
 }
 
+template <int DIR>
+static hv_status any_row_pass(const void* in, int kin, float2* out, const float2* chirp, int width, int height,
+                              int batch, FrameMap fin, FrameMap fm, float scale, hipStream_t st, uint32_t* mm_reset,
+                              int mm_slots) {
+    AnyPlan pl;
+    if (!any_plan(width, pl)) return fail(HV_ERR_UNSUPPORTED, "width must be an even 2^a 3^b 5^c in [16, 4096]");
+    if (chirp && DIR > 0) return fail(HV_ERR_UNSUPPORTED, "chirp factors on an inverse row pass");
+    // four rows per workgroup share the twiddle table (two above 2048: 128 threads per row)
+    if (width <= 2048)
+        return any_row_launch<DIR, 4>(in, kin, out, chirp, width, height, batch, fin, fm, scale, st, mm_reset, mm_slots,
+                                      pl);
+    return any_row_launch<DIR, 2>(in, kin, out, chirp, width, height, batch, fin, fm, scale, st, mm_reset, mm_slots,
+                                  pl);
+}
+
+template <int MODE, int CW, int NT>
+static hv_status any_col_launch(const float2* in, float2* out, const float2* lens, int width, int height, int batch,
+                                FrameMap fin, FrameMap fm, const AnyPlan& pl, hipStream_t st) {
+    if (!any_plan_fits(pl, NT)) return fail(HV_ERR_UNSUPPORTED, "column length not supported");
+    const size_t lds = (size_t)height * (CW + 1) * sizeof(float2);  // the tile and the twiddle table
+    HV_TRY(any_lds_attr(&k_any_col<MODE, CW, NT>, lds));
+    k_any_col<MODE, CW, NT><<<dim3((width + CW - 1) / CW, batch), NT * CW, lds, st>>>(in, out, lens, width, fin, fm, pl);
+    HV_CHECK_LAUNCH("k_any_col");
+    return HV_OK;
+}
+
 template <int MODE>
 static hv_status any_col_pass(const float2* in, float2* out, const float2* lens, int width, int height, int batch,
                               FrameMap fin, FrameMap fm, hipStream_t st) {
     AnyPlan pl;
     if (!any_plan(height, pl)) return fail(HV_ERR_UNSUPPORTED, "height must be an even 2^a 3^b 5^c in [16, 4096]");
-    if (height <= 2048) {
-        constexpr int CW = 4;
-        const size_t lds = 2 * (size_t)height * CW * sizeof(float2);
-        HV_TRY(any_lds_attr(&k_any_col<MODE, CW>, lds));
-        k_any_col<MODE, CW><<<dim3((width + CW - 1) / CW, batch), 64 * CW, lds, st>>>(in, out, lens, width, fin, fm, pl);
-    } else {
-        constexpr int CW = 2;
-        const size_t lds = 2 * (size_t)height * CW * sizeof(float2);
-        HV_TRY(any_lds_attr(&k_any_col<MODE, CW>, lds));
-        k_any_col<MODE, CW><<<dim3((width + CW - 1) / CW, batch), 64 * CW, lds, st>>>(in, out, lens, width, fin, fm, pl);
-    }
-    HV_CHECK_LAUNCH("k_any_col");
-    return HV_OK;
+    // 4-column tiles (32-byte runs): 2048 x 4 x 8 B + the table = 80 KiB, two workgroups per CU
+    if (height <= 2048) return any_col_launch<MODE, 4, 64>(in, out, lens, width, height, batch, fin, fm, pl, st);
+    return any_col_launch<MODE, 2, 128>(in, out, lens, width, height, batch, fin, fm, pl, st);
 }
 
 // ---------------------------------------------------------------------------
