@@ -71,6 +71,16 @@ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
+// 2-D unwrap helpers (tools_unwrap.cu): the FFT frequency index of the
+// circularly shifted fx / fy grids (kernel_init_unwrap_2d + circ_shift by
+// (-W/2, -H/2)), and eq_unwrap_2d's product for one element, g 2 pi i conj(i z).
+__device__ __forceinline__ float kfreq(int i, int n) { return (float)(i < n / 2 ? i : i - n); }
+__device__ __forceinline__ float2 unwrap_eq_term(float2 g, float2 zz) {
+    const float2 tpi = make_float2(0.f, (float)(2.0 * M_PI));
+    const float2 zi = make_float2(zz.x * 0.f - zz.y * 1.f, -(zz.x * 1.f + zz.y * 0.f));
+    return cmul(cmul(g, tpi), zi);
+}
+
 // Order-preserving float <-> uint mapping for atomic min/max on floats.
 __device__ __forceinline__ uint32_t float_to_ordered(float f) {
     uint32_t u = __float_as_uint(f);

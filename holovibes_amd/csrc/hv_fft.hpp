@@ -425,7 +425,11 @@ __device__ __forceinline__ void fft_compute_adj_tw(int t, float2* v, const TW& t
             for (int k = 0; k < NB; ++k) {
                 const int j = t + k * NT;
 #pragma unroll
-                for (int r = 0; r < R; ++r) store(j + r * (N / R), v[k * R + r]);
+                for (int r = 0; r < R; ++r) {
+                    // a storer may also take the register slot, the one fft_load filled with element n
+                    if constexpr (std::is_invocable_v<Store, int, float2, int>) store(j + r * (N / R), v[k * R + r], k * R + r);
+                    else store(j + r * (N / R), v[k * R + r]);
+                }
             }
         } else {
             // the forward stage s read positions j + r N/R; stage s - 1 wrote d + r' Ns'

@@ -135,8 +135,15 @@ hv_status launch_percentile_static(float* range, const float* img, int width, in
                                    int clamp, hipStream_t st);
 uint32_t reticle_radius(int width, int height, float scale);
 size_t reticle_pixels(int width, int height, float scale);
-hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots = 1);
-hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st);
+// have_mm: mm already holds the input's min/max (launch_unwrap_2d with mm)
+hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots = 1,
+                                bool have_mm = false);
+hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st,
+                           uint32_t* mm = nullptr, int slots = 1);
+// Power-of-two sides in [16, 4096]: four fused passes (hv_spatial.hip)
+bool unwrap_fused_supported(int width, int height);
+hv_status launch_unwrap_2d_fused(float* out, const float* phase, int width, int height, void* scratch,
+                                 hipStream_t st, uint32_t* mm, int slots);
 size_t unwrap_scratch_bytes(int width, int height);
 
 // hv_pca.hip

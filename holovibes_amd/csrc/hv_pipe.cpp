@@ -378,8 +378,9 @@ hv_status hv_pipe::image_work() {
         }
         if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
         if (unwrap) {
-            HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream));
-            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kPipeMm));
+            // the unwrap's last pass leaves its output's min/max in mm for the rescale
+            HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream, L.mm, kPipeMm));
+            HV_TRY(launch_rescale_unwrap(L.image, L.tmp, P, L.mm, L.stream, kPipeMm, true));
         }
     } else if (path == PATH_BINS) {
         // moments of the propagated bins [f_start, f_end] (insert_moments, pipe.cc:429-455)

@@ -413,9 +413,12 @@ __global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float*
     }
 }
 
-hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots) {
-    HV_TRY(launch_minmax_init(mm, st, slots));
-    HV_TRY(launch_minmax(mm, in, n, st, slots));
+hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots,
+                                bool have_mm) {
+    if (!have_mm) {
+        HV_TRY(launch_minmax_init(mm, st, slots));
+        HV_TRY(launch_minmax(mm, in, n, st, slots));
+    }
     const unsigned blocks = blocks_for(n, 256);
     HV_TRY(collapse_partials(mm, blocks, slots, st));
     k_rescale_unwrap<<<blocks, 256, 0, st>>>(out, in, n, mm, slots);
@@ -426,10 +429,6 @@ hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t*
 // ---------------------------------------------------------------------------
 // unwrap_2d (tools_unwrap.cu:269-369): periodic least-squares unwrapping
 // ---------------------------------------------------------------------------
-// k: the FFT frequency index of the circularly shifted fx/fy grids
-// (kernel_init_unwrap_2d + circ_shift by (-W/2, -H/2)).
-__device__ __forceinline__ float kfreq(int i, int n) { return (float)(i < n / 2 ? i : i - n); }
-
 size_t unwrap_scratch_bytes(int width, int height) { return (size_t)width * height * sizeof(float2) * 3; }
 
 __global__ void k_unwrap_init(float2* z, float2* g, const float* phase, size_t P) {
@@ -455,11 +454,9 @@ __global__ void k_unwrap_grad(float2* gx, float2* gy, int width, int height) {
 __global__ void k_unwrap_eq(float2* gx, float2* gy, const float2* z, size_t P) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    const float2 tpi = make_float2(0.f, (float)(2.0 * M_PI));
     const float2 zz = z[i];
-    const float2 zi = make_float2(zz.x * 0.f - zz.y * 1.f, -(zz.x * 1.f + zz.y * 0.f));
-    gx[i] = cmul(cmul(gx[i], tpi), zi);
-    gy[i] = cmul(cmul(gy[i], tpi), zi);
+    gx[i] = unwrap_eq_term(gx[i], zz);
+    gy[i] = unwrap_eq_term(gy[i], zz);
 }
 
 // kernel_norm_ratio + kernel_add_complex_frames: G = Gx fx/|f|^2 + Gy fy/|f|^2, 0 at DC
@@ -484,7 +481,14 @@ __global__ void k_unwrap_last(float* out, const float2* g, size_t P) {
     if (i < P) out[i] = (float)((double)g[i].y / -(2.0 * M_PI));
 }
 
-hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st) {
+// The steps above one after another (frame sides that are not powers of
+// two); powers of two run the fused passes of launch_unwrap_2d_fused.
+// mm: the output's min/max accumulated there (re-armed first), for the
+// rescale; nullptr: none.
+hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height, void* scratch, hipStream_t st,
+                           uint32_t* mm, int slots) {
+    if (unwrap_fused_supported(width, height))
+        return launch_unwrap_2d_fused(out, phase, width, height, scratch, st, mm, slots);
     const size_t P = (size_t)width * height;
     float2* z = reinterpret_cast<float2*>(scratch);
     float2* gx = z + P;
@@ -505,6 +509,10 @@ hv_status launch_unwrap_2d(float* out, const float* phase, int width, int height
     HV_TRY(launch_fft2(gx, 1, width, height, +1, st));
     k_unwrap_last<<<g1, 256, 0, st>>>(out, gx, P);
     HV_CHECK_LAUNCH("k_unwrap_last");
+    if (mm) {
+        HV_TRY(launch_minmax_init(mm, st, slots));
+        HV_TRY(launch_minmax(mm, out, P, st, slots));
+    }
     return HV_OK;
 }
 

@@ -1007,4 +1007,238 @@ hv_status launch_propagate_to_ring(const void* raw, int depth, int batch, int wi
     return HV_OK;
 }
 
+// ---------------------------------------------------------------------------
+// 2-D unwrap (unwrap_2d, tools_unwrap.cu:269-369) in four passes, power-of-two
+// sides.  The reference's steps: z = exp(i phase); (gx, gy) = IFFT2(FFT2(z)
+// (fx, fy)); g' = g 2 pi i conj(i z); G = FFT2(gx') fx / |f|^2 + FFT2(gy')
+// fy / |f|^2 (0 at the origin); out = Im IFFT2(G) / -2 pi -- seven 2-D
+// transforms of one or two frames and five element-wise kernels.  fx depends
+// on the row frequency only, so the column transforms around its product
+// cancel to a factor H: gx = H IFFT_row(fx FFT_row(z)), a 1-D filter along
+// rows; likewise gy = W IFFT_col(fy FFT_col(z)).  The passes:
+//   1. columns: phase -> z -> gy' -> column FFT                   (Gy_c)
+//   2. rows, two jobs: phase -> z -> gx' -> row FFT (Gx_r), and Gy_c -> row FFT (Gy)
+//   3. columns: Gx_r -> column FFT (Gx), G from Gx and Gy, inverse column FFT
+//   4. rows: inverse row FFT, Im / -2 pi, the output's min/max
+// Every product and ratio is the reference's element formula (kfreq,
+// unwrap_eq_term, the |f|^2 ratio); transforms are unnormalised as cuFFT's.
+// ---------------------------------------------------------------------------
+// One frame only: 1024-long columns in 4-column tiles (256 workgroups
+// instead of 128 of 8 columns), the other lengths as the column pass.
+template <int N>
+struct UnwrapColCfg : ColCfg<N> {
+    static constexpr int CW = N == 1024 ? 4 : ColCfg<N>::CW;
+    static constexpr int THREADS = ColCfg<N>::NT * CW;
+};
+
+template <int N>
+__global__ void __launch_bounds__(UnwrapColCfg<N>::THREADS)
+    k_unwrap_cols_a(const float* __restrict__ phase, float2* __restrict__ gyc, int width, uint32_t* mm_reset,
+                    int mm_slots) {
+    using C = UnwrapColCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
+    __shared__ float2 lds[N * C::CW];
+    // the unwrap's last pass accumulates the output's min/max
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
+    const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
+    const int x = xcd_tile<(16 / C::CW)>(blockIdx.x, gridDim.x) * C::CW + c;
+    auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
+    float ph[kE];
+    fft_load_slots<N, C::E>(t, [&](int i, int n) { ph[i] = phase[(size_t)n * width + x]; });
+    float2 tw[FftTw<N, C::E>::COUNT];
+    fft_twiddles<N, C::E>(t, tw);
+    float2 z[kE], v[kE];
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+        sincosf(ph[i], &z[i].y, &z[i].x);
+        v[i] = z[i];
+    }
+    fft_compute_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw), [](int, float2) {}, lds_at);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+        const float fy = kfreq(fft_out_index<N, C::E>(t, i), N);
+        v[i] = make_float2(v[i].x * fy, v[i].y * fy);
+    }
+    const float wf = (float)width;
+    fft_compute_adj_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw), [&](int, float2 g, int slot) {
+        v[slot] = unwrap_eq_term(make_float2(g.x * wf, g.y * wf), z[slot]);
+    }, lds_at);
+    float2* dst = gyc + x;
+    fft_compute_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw),
+                                [&](int n, float2 g) { dst[(size_t)n * width] = g; }, lds_at);
+}
+
+// blockIdx.y 0: phase rows -> gx' -> row FFT into gxr; 1: gy rows (column
+// transformed) -> row FFT in place.
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS0)
+    k_unwrap_rows_a(const float* __restrict__ phase, float2* __restrict__ gxr, float2* __restrict__ gy, int height) {
+    using C = RowCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
+    __shared__ float2 lds[C::M0 * C::PITCH];
+    const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    const int y = blockIdx.x * C::M0 + m;
+    const bool valid = y < height;
+    const size_t row = (size_t)(valid ? y : 0) * N;
+    float2* my = lds + m * C::PITCH;
+    auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
+    float2 tw[FftTw<N, C::E>::COUNT];
+    float2 v[kE];
+    if (blockIdx.y == 0) {
+        float2 z[kE];
+        fft_load_slots<N, C::E>(t, [&](int i, int n) {
+            sincosf(phase[row + n], &z[i].y, &z[i].x);
+        });
+        fft_twiddles<N, C::E>(t, tw);
+#pragma unroll
+        for (int i = 0; i < kE; ++i) v[i] = z[i];
+        fft_compute_tw<N, C::E, -1, false, (C::NT <= 64)>(t, v, static_cast<const float2*>(tw), [](int, float2) {}, lds_at);
+#pragma unroll
+        for (int i = 0; i < kE; ++i) {
+            const float fx = kfreq(fft_out_index<N, C::E>(t, i), N);
+            v[i] = make_float2(v[i].x * fx, v[i].y * fx);
+        }
+        const float hf = (float)height;
+        fft_compute_adj_tw<N, C::E, -1, (C::NT <= 64)>(t, v, static_cast<const float2*>(tw), [&](int, float2 g, int slot) {
+            v[slot] = unwrap_eq_term(make_float2(g.x * hf, g.y * hf), z[slot]);
+        }, lds_at);
+        float2* dst = gxr + row;
+        fft_compute_tw<N, C::E, -1, false, (C::NT <= 64)>(t, v, static_cast<const float2*>(tw), [&](int n, float2 g) {
+            if (valid) dst[n] = g;
+        }, lds_at);
+    } else {
+        fft_load<N, C::E>(t, [&](int n) -> float2 { return gy[row + n]; }, v);
+        fft_twiddles<N, C::E>(t, tw);
+        float2* dst = gy + row;
+        // a row is loaded whole before its first exchange (one wavefront, or a barrier): in place is safe
+        fft_compute_tw<N, C::E, -1, false, (C::NT <= 64)>(t, v, static_cast<const float2*>(tw), [&](int n, float2 g) {
+            if (valid) dst[n] = g;
+        }, lds_at);
+    }
+}
+
+// Gx = column FFT of gxr; G = Gx fx / |f|^2 + Gy fy / |f|^2 (kernel_norm_ratio
+// + kernel_add_complex_frames, 0 where |f| = 0); inverse column FFT, in place.
+template <int N>
+__global__ void __launch_bounds__(UnwrapColCfg<N>::THREADS)
+    k_unwrap_cols_b(float2* __restrict__ g, const float2* __restrict__ gy, int width) {
+    using C = UnwrapColCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
+    __shared__ float2 lds[N * C::CW];
+    const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
+    const int x = xcd_tile<(16 / C::CW)>(blockIdx.x, gridDim.x) * C::CW + c;
+    auto lds_at = [&](int n) -> float2& { return lds[n * C::CW + c]; };
+    float2 v[kE], b[kE];
+    fft_load<N, C::E>(t, [&](int n) -> float2 { return g[(size_t)n * width + x]; }, v);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) b[i] = gy[(size_t)fft_out_index<N, C::E>(t, i) * width + x];
+    float2 tw[FftTw<N, C::E>::COUNT];
+    fft_twiddles<N, C::E>(t, tw);
+    fft_compute_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw), [](int, float2) {}, lds_at);
+    const float fx = kfreq(x, width);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+        const float fy = kfreq(fft_out_index<N, C::E>(t, i), N);
+        const float norm = fx * fx + fy * fy;
+        float2 a = make_float2(0.f, 0.f), e = make_float2(0.f, 0.f);
+        if (norm != 0) {
+            const float cx = fx / norm, cy = fy / norm;
+            a = make_float2(v[i].x * cx, v[i].y * cx);
+            e = make_float2(b[i].x * cy, b[i].y * cy);
+        }
+        v[i] = cadd(a, e);
+    }
+    float2* dst = g + x;
+    fft_compute_adj_tw<N, C::E, -1>(t, v, static_cast<const float2*>(tw),
+                                    [&](int n, float2 z) { dst[(size_t)n * width] = z; }, lds_at);
+}
+
+// Inverse row FFT, out = Im / -2 pi (kernel_unwrap2d_last_step), the
+// output's min/max into mm (nullptr: none).
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS0)
+    k_unwrap_rows_b(const float2* __restrict__ g, float* __restrict__ out, int height, uint32_t* mm, int mm_slots) {
+    using C = RowCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
+    __shared__ float2 lds[C::M0 * C::PITCH];
+    const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    const int y = blockIdx.x * C::M0 + m;
+    const bool valid = y < height;
+    const size_t row = (size_t)(valid ? y : 0) * N;
+    float2* my = lds + m * C::PITCH;
+    auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
+    float2 v[kE];
+    fft_load<N, C::E>(t, [&](int n) -> float2 { return g[row + n]; }, v);
+    float2 tw[FftTw<N, C::E>::COUNT];
+    fft_twiddles<N, C::E>(t, tw);
+    float vmin = INFINITY, vmax = -INFINITY;
+    float* dst = out + row;
+    fft_compute_tw<N, C::E, +1, false, (C::NT <= 64)>(t, v, static_cast<const float2*>(tw), [&](int n, float2 z) {
+        if (!valid) return;
+        const float o = (float)((double)z.y / -(2.0 * M_PI));
+        dst[n] = o;
+        vmin = fminf(vmin, o);
+        vmax = fmaxf(vmax, o);
+    }, lds_at);
+    if (mm) block_minmax_atomic(vmin, vmax, mm, mm_slots);
+}
+
+template <int NX, int NY>
+static hv_status unwrap_fused_n(float* out, const float* phase, void* scratch, hipStream_t st, uint32_t* mm,
+                                int slots) {
+    const size_t P = (size_t)NX * NY;
+    float2* gx = reinterpret_cast<float2*>(scratch);
+    float2* gy = gx + P;
+    using CC = UnwrapColCfg<NY>;
+    using RC = RowCfg<NX>;
+    const unsigned col_groups = NX / CC::CW, row_groups = (NY + RC::M0 - 1) / RC::M0;
+    k_unwrap_cols_a<NY><<<col_groups, CC::THREADS, 0, st>>>(phase, gy, NX, mm, slots);
+    HV_CHECK_LAUNCH("k_unwrap_cols_a");
+    k_unwrap_rows_a<NX><<<dim3(row_groups, 2), RC::THREADS0, 0, st>>>(phase, gx, gy, NY);
+    HV_CHECK_LAUNCH("k_unwrap_rows_a");
+    k_unwrap_cols_b<NY><<<col_groups, CC::THREADS, 0, st>>>(gx, gy, NX);
+    HV_CHECK_LAUNCH("k_unwrap_cols_b");
+    k_unwrap_rows_b<NX><<<row_groups, RC::THREADS0, 0, st>>>(gx, out, NY, mm, slots);
+    HV_CHECK_LAUNCH("k_unwrap_rows_b");
+    return HV_OK;
+}
+
+bool unwrap_fused_supported(int width, int height) {
+    return is_pow2(width) && is_pow2(height) && width >= 16 && height >= 16 && width <= 4096 && height <= 4096;
+}
+
+template <int NX>
+static hv_status unwrap_fused_x(float* out, const float* phase, int height, void* scratch, hipStream_t st,
+                                uint32_t* mm, int slots) {
+    switch (height) {
+        case 16: return unwrap_fused_n<NX, 16>(out, phase, scratch, st, mm, slots);
+        case 32: return unwrap_fused_n<NX, 32>(out, phase, scratch, st, mm, slots);
+        case 64: return unwrap_fused_n<NX, 64>(out, phase, scratch, st, mm, slots);
+        case 128: return unwrap_fused_n<NX, 128>(out, phase, scratch, st, mm, slots);
+        case 256: return unwrap_fused_n<NX, 256>(out, phase, scratch, st, mm, slots);
+        case 512: return unwrap_fused_n<NX, 512>(out, phase, scratch, st, mm, slots);
+        case 1024: return unwrap_fused_n<NX, 1024>(out, phase, scratch, st, mm, slots);
+        case 2048: return unwrap_fused_n<NX, 2048>(out, phase, scratch, st, mm, slots);
+        case 4096: return unwrap_fused_n<NX, 4096>(out, phase, scratch, st, mm, slots);
+        default: return fail(HV_ERR_UNSUPPORTED, "fused unwrap: height");
+    }
+}
+
+// Scratch: two complex frames (the step's unwrap_scratch_bytes holds three).
+hv_status launch_unwrap_2d_fused(float* out, const float* phase, int width, int height, void* scratch,
+                                 hipStream_t st, uint32_t* mm, int slots) {
+    if (!unwrap_fused_supported(width, height)) return fail(HV_ERR_UNSUPPORTED, "fused unwrap: frame sides");
+    switch (width) {
+        case 16: return unwrap_fused_x<16>(out, phase, height, scratch, st, mm, slots);
+        case 32: return unwrap_fused_x<32>(out, phase, height, scratch, st, mm, slots);
+        case 64: return unwrap_fused_x<64>(out, phase, height, scratch, st, mm, slots);
+        case 128: return unwrap_fused_x<128>(out, phase, height, scratch, st, mm, slots);
+        case 256: return unwrap_fused_x<256>(out, phase, height, scratch, st, mm, slots);
+        case 512: return unwrap_fused_x<512>(out, phase, height, scratch, st, mm, slots);
+        case 1024: return unwrap_fused_x<1024>(out, phase, height, scratch, st, mm, slots);
+        case 2048: return unwrap_fused_x<2048>(out, phase, height, scratch, st, mm, slots);
+        default: return unwrap_fused_x<4096>(out, phase, height, scratch, st, mm, slots);
+    }
+}
+
 }  // namespace hv

@@ -257,10 +257,13 @@ def test_pca_moments(st, oracle, T, fs, fe, moment):
     assert np.abs(got - w[moment]).max() < 2e-6 * scale
 
 
-def test_unwrap_2d(st, oracle):
-    n = 256
-    y, x = np.mgrid[0:n, 0:n]
-    phi = 4.0 * np.sin(2 * np.pi * x / n) * np.cos(2 * np.pi * y / n) + 1.5 * np.cos(2 * np.pi * 2 * y / n)
+# (width, height): power-of-two sides run the four fused passes (square,
+# wide, tall with 2048-long columns, the smallest), the others the step-by-step
+# 2-D transforms
+@pytest.mark.parametrize("W,H", [(256, 256), (512, 128), (64, 2048), (16, 16), (96, 80)])
+def test_unwrap_2d(st, oracle, W, H):
+    y, x = np.mgrid[0:H, 0:W]
+    phi = 4.0 * np.sin(2 * np.pi * x / W) * np.cos(2 * np.pi * y / H) + 1.5 * np.cos(2 * np.pi * 2 * y / H)
     wrapped = np.angle(np.exp(1j * phi)).astype(np.float32)
     got = host(st.unwrap_2d(dev(wrapped)))
     want = oracle.unwrap2d(wrapped.astype(np.float64))
