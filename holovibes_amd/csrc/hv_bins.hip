@@ -342,6 +342,12 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------------
 // Split path (frames wider than 1024, windows longer than 64, moments): the
 // bins-first steps as separate passes over the K planes.
+#ifndef HV_BINS_TIME_REAL
+#define HV_BINS_TIME_REAL 1     // T >= 32: the real window's DFT through a half-length complex FFT
+#endif
+#ifndef HV_BINS_TIME_LDS_KB
+#define HV_BINS_TIME_LDS_KB 64  // LDS tile of the real time DFT (sets its pixels per workgroup)
+#endif
 // ---------------------------------------------------------------------------
 // Time DFT: CW pixels per workgroup, NT threads per pixel, the length-T
 // transform of each pixel's window through LDS (the window read in physical
@@ -398,9 +404,101 @@ __global__ void __launch_bounds__(BinTimeCfg<T>::THREADS)
     fft_run<T, C::E, -1>(t, load, store, lds_at);
 }
 
+// T >= 32: the window is real, so each pixel's length-T DFT is one complex
+// FFT of half the length, z_m = x_2m + i x_2m+1, then the split of
+// real_dft_bins (X_k = E_k + w^k O_k, conjugate symmetry above T / 2): half
+// the LDS tile (T / 2 x 64 pixels: two or more workgroups per CU at T = 256,
+// four at T = 128) and half the butterflies.  Thread t of a pixel writes bins
+// pmin + t, pmin + t + NT, ..., so a wavefront (64 pixels, one t) writes 512
+// contiguous bytes of one plane per store.
+// CW pixels per workgroup: the widest tile of HV_BINS_TIME_LDS_KB KiB of LDS
+// (each frame of the window read as CW-pixel runs: 64 pixels of u16 frames,
+// 128-byte runs, took 1.14 ms at c5 -- 57 bins of 128 frames of 2048^2 --
+// against 0.92 ms with 128 pixels), or 64 pixels when the frame is not a
+// multiple of that width.
+template <int T>
+constexpr int bins_time_cw() {
+    return HV_BINS_TIME_LDS_KB * 1024 / (T / 2 * (int)sizeof(float2));
+}
+
+template <int T, int CWT>
+struct BinTimeRealCfg {
+    static constexpr int M = T / 2;  // complex points
+    static constexpr int E = M >= 16 ? 16 : M;
+    static constexpr int NT = FftShape<M, E>::NT;  // threads per pixel
+    static constexpr int CW = CWT;
+    static constexpr int THREADS = NT * CW;
+};
+
+template <int T, typename TIn, int CWT>
+__global__ void __launch_bounds__((BinTimeRealCfg<T, CWT>::THREADS))
+    k_bins_time_real(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
+                     size_t P, int slot0, int pmin, int K, uint32_t* mm_reset, int mm_slots, int vec) {
+    using C = BinTimeRealCfg<T, CWT>;
+    __shared__ float2 lds[C::M * C::CW];
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
+    const int c = threadIdx.x % C::CW, t = threadIdx.x / C::CW;
+    const size_t p0 = (size_t)blockIdx.x * C::CW, p = p0 + c;
+    auto lds_at = [&](int m) -> float2& { return lds[m * C::CW + c]; };
+    // sample n of the window (physical slot n holds frame (n - slot0) mod T,
+    // as in k_bins_time) into z_{n/2}, real part for even n
+    float* ldsf = reinterpret_cast<float*>(lds);
+    constexpr int PX = 16 / sizeof(TIn);  // pixels per 16-byte load
+    constexpr int CHUNKS = T * C::CW / PX;
+    static_assert(C::CW % PX == 0, "a tile row holds whole 16-byte loads");
+    for (int idx = threadIdx.x; idx < CHUNKS; idx += C::THREADS) {
+        const int n = idx / (C::CW / PX), q = idx % (C::CW / PX);
+        const int j = (n - slot0) & (T - 1);
+        const TIn* src = raw + (size_t)j * P + p0 + (size_t)q * PX;
+        float* dst = ldsf + ((size_t)(n >> 1) * C::CW + q * PX) * 2 + (n & 1);
+        if (vec) {
+            const uint4 w = *reinterpret_cast<const uint4*>(src);
+            const TIn* px = reinterpret_cast<const TIn*>(&w);
+#pragma unroll
+            for (int e = 0; e < PX; ++e) dst[2 * e] = convert_px<TIn>(px[e]);
+        } else {  // a caller's buffer that is not 16-byte aligned
+#pragma unroll
+            for (int e = 0; e < PX; ++e) dst[2 * e] = convert_px<TIn>(src[e]);
+        }
+    }
+    __syncthreads();
+    // Z = FFT_M(z), left in the tile in natural order
+    fft_run<C::M, C::E, -1, true>(t, [&](int m) -> float2 { return lds_at(m); },
+                                  [&](int m, float2 z) { lds_at(m) = z; }, lds_at);
+    __syncthreads();
+    const float2 cf = chirp ? chirp[p] : make_float2(1.f, 0.f);
+    for (int k = pmin + t; k < pmin + K; k += C::NT) {
+        const int kk = k <= C::M ? k : T - k;
+        const float2 a = lds_at(kk % C::M), bc = lds_at((C::M - kk) % C::M);
+        const float2 b = make_float2(bc.x, -bc.y);
+        const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+        const float2 d = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y - b.y));
+        const float2 o = make_float2(d.y, -d.x);  // d / i
+        const float2 w = twiddle<-1>(kk, clog2(T));
+        float2 v = make_float2(e.x + (w.x * o.x - w.y * o.y), e.y + (w.x * o.y + w.y * o.x));
+        if (k > C::M) v.y = -v.y;
+        planes[(size_t)(k - pmin) * P + p] = chirp ? cmul(v, cf) : v;
+    }
+}
+
 template <int T, typename TIn>
 hv_status time_launch_t(const void* raw, float2* planes, const float2* chirp, size_t P, int slot0, int pmin, int K,
                         uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+    if constexpr (T >= 32 && HV_BINS_TIME_REAL) {
+        constexpr int CWB = bins_time_cw<T>();
+        static_assert(CWB >= 64 && BinTimeRealCfg<T, CWB>::THREADS <= 1024, "time DFT tile");
+        if (P % 64) return fail(HV_ERR_UNSUPPORTED, "bins-first time DFT: frames of a multiple of 64 pixels");
+        const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
+        if (P % CWB == 0) {
+            k_bins_time_real<T, TIn, CWB><<<(unsigned)(P / CWB), BinTimeRealCfg<T, CWB>::THREADS, 0, st>>>(
+                (const TIn*)raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, vec);
+        } else {
+            k_bins_time_real<T, TIn, 64><<<(unsigned)(P / 64), BinTimeRealCfg<T, 64>::THREADS, 0, st>>>(
+                (const TIn*)raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, vec);
+        }
+        HV_CHECK_LAUNCH("k_bins_time_real");
+        return HV_OK;
+    }
     using C = BinTimeCfg<T>;
     if (P % C::CW) return fail(HV_ERR_UNSUPPORTED, "bins-first time DFT: frames of a multiple of 64 pixels");
     const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
