@@ -594,11 +594,192 @@ __global__ void __launch_bounds__(kPcaTimeThreads)
     }
 }
 
+// The projection on MFMA (integer frames, T = 32, 64, 128, at most 64
+// components): the planes are the product of the window (pixels x T frames)
+// and the T x 2K real weight matrix (component k's real and imaginary
+// weights in columns 2k, 2k + 1), as v_mfma_f32_32x32x16_f16 steps on split
+// operands: frames x / 16 = hi + lo exactly (converted pixels have at most 16
+// significant bits), weights 256 w = hi + lo (22 bits), three products per
+// step (lo hi, hi lo, hi hi) accumulated in FP32; the powers of two are
+// divided out exactly.  Each workgroup walks 64-pixel chunks (persistent,
+// one grid of resident workgroups): the chunk's T x 64 window is staged in
+// LDS as pixel rows of FP16 hi / lo (8 x 8 blocks transposed in registers),
+// wave w runs columns 32 w .. 32 w + 31 with their B fragments held in
+// registers for every chunk, and writes its 16 components through a
+// wave-private LDS tile as 256-byte plane runs.
+#ifndef HV_PCA_TIME_MFMA
+#define HV_PCA_TIME_MFMA 1
+#endif
+constexpr int kPcaMfmaWaves = 4;
+constexpr int kPcaMfmaCW = 64;
+
+typedef _Float16 v8h_p __attribute__((ext_vector_type(8)));
+typedef float v16f_p __attribute__((ext_vector_type(16)));
+
+template <int T>
+struct PcaMfmaCfg {
+    static constexpr int KS = T / 16;                          // k-steps of 16 frames
+    static constexpr int PITCH = T + 8;                        // halfs per pixel row (16-byte pad)
+    static constexpr int TILE = kPcaMfmaCW * PITCH;            // halfs of the hi (or lo) tile
+};
+
+template <typename TIn>
+constexpr bool pca_time_mfma_ok(int T, int K) {
+    return HV_PCA_TIME_MFMA && !std::is_same<TIn, float>::value && (T == 32 || T == 64 || T == 128) && K <= 64;
+}
+
+// Eight pixels of one frame, converted.
+template <typename TIn>
+__device__ __forceinline__ void load8_px(const TIn* src, int vec, float* x) {
+    if constexpr (sizeof(TIn) == 2) {
+        if (vec) {
+            const uint4 w = *reinterpret_cast<const uint4*>(src);
+            const TIn* px = reinterpret_cast<const TIn*>(&w);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = convert_px<TIn>(px[e]);
+            return;
+        }
+    } else {
+        if (vec) {
+            const uint2 w = *reinterpret_cast<const uint2*>(src);
+            const TIn* px = reinterpret_cast<const TIn*>(&w);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = convert_px<TIn>(px[e]);
+            return;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = convert_px<TIn>(src[e]);
+}
+
+template <int T, typename TIn>
+__global__ void __launch_bounds__(64 * kPcaMfmaWaves)
+    k_pca_time_mfma(const TIn* __restrict__ raw, float2* __restrict__ planes, const float2* __restrict__ chirp,
+                    size_t P, int slot0, int K, const float2* __restrict__ wts, uint32_t* mm_reset, int mm_slots,
+                    int vec) {
+    using C = PcaMfmaCfg<T>;
+    constexpr float XS = 1.f / 16, WS = 256.f, OS = 1.f / (XS * WS);
+    __shared__ __align__(16) _Float16 th[C::TILE], tl[C::TILE];
+    __shared__ float dl[kPcaMfmaWaves][32 * 33];
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, l32 = lane & 31;
+    const bool active = 32 * wave < 2 * K;  // uniform across the wave
+    // B fragments: lane l holds columns 32 w + l % 32, frames 16 s + 8 (l / 32) + e
+    v8h_p bh[C::KS], bl[C::KS];
+    {
+        const int n = 32 * wave + l32, j = n >> 1;
+#pragma unroll
+        for (int s = 0; s < C::KS; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                float v = 0.f;
+                if (j < K) {
+                    const float2 w = wts[(size_t)(16 * s + 8 * half + e) * T + j];
+                    v = ((n & 1) ? w.y : w.x) * WS;
+                }
+                const _Float16 h = (_Float16)v;
+                bh[s][e] = h;
+                bl[s][e] = (_Float16)(v - (float)h);
+            }
+    }
+    float* d = dl[wave];
+    const size_t nchunks = P / kPcaMfmaCW;
+    for (size_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        const size_t p0 = chunk * kPcaMfmaCW;
+        __syncthreads();  // the previous chunk's tile is read
+        // T blocks of 8 frames x 8 pixels (window frame f from physical slot (f - slot0) mod T)
+        for (int b = threadIdx.x; b < T; b += 64 * kPcaMfmaWaves) {
+            const int fg = b >> 3, pg = b & 7;
+            float x[8][8];
+#pragma unroll
+            for (int fi = 0; fi < 8; ++fi) {
+                const int slot = (8 * fg + fi - slot0) & (T - 1);
+                load8_px<TIn>(raw + (size_t)slot * P + p0 + 8 * pg, vec, x[fi]);
+            }
+#pragma unroll
+            for (int pi = 0; pi < 8; ++pi) {
+                v8h_p hi, lo;
+#pragma unroll
+                for (int fi = 0; fi < 8; ++fi) {
+                    const float v = x[fi][pi] * XS;
+                    const _Float16 h = (_Float16)v;
+                    hi[fi] = h;
+                    lo[fi] = (_Float16)(v - (float)h);
+                }
+                *reinterpret_cast<v8h_p*>(th + (8 * pg + pi) * C::PITCH + 8 * fg) = hi;
+                *reinterpret_cast<v8h_p*>(tl + (8 * pg + pi) * C::PITCH + 8 * fg) = lo;
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+#pragma unroll
+        for (int mt = 0; mt < kPcaMfmaCW / 32; ++mt) {
+            v16f_p acc = {};
+            const int row = (32 * mt + l32) * C::PITCH + 8 * half;
+#pragma unroll
+            for (int s = 0; s < C::KS; ++s) {
+                const v8h_p xh = *reinterpret_cast<const v8h_p*>(th + row + 16 * s);
+                const v8h_p xl = *reinterpret_cast<const v8h_p*>(tl + row + 16 * s);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(xl, bh[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bl[s], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(xh, bh[s], acc, 0, 0, 0);
+            }
+            // register r of lane l: pixel 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+#pragma unroll
+            for (int r = 0; r < 16; ++r) d[l32 * 33 + 8 * (r / 4) + 4 * half + (r % 4)] = acc[r];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int cl = 2 * q + half, k = 16 * wave + cl;
+                if (k < K) {
+                    const size_t p = p0 + 32 * mt + l32;
+                    float2 v = make_float2(d[(2 * cl) * 33 + l32] * OS, d[(2 * cl + 1) * 33 + l32] * OS);
+                    if (chirp) v = cmul(v, chirp[p]);
+                    planes[(size_t)k * P + p] = v;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
+// Resident workgroups of k_pca_time_mfma on the current device.
+template <int T, typename TIn>
+static hv_status pca_mfma_grid(unsigned& grid) {
+    static int per_cu = 0, cus = 0;
+    if (!per_cu) {
+        int dev = 0;
+        HV_HIP(hipGetDevice(&dev));
+        HV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&k_pca_time_mfma<T, TIn>), 64 * kPcaMfmaWaves, 0));
+        if (per_cu < 1) per_cu = 1;
+    }
+    grid = (unsigned)(cus * per_cu);
+    return HV_OK;
+}
+
 template <int T, typename TIn>
 hv_status pca_time_t(const void* raw, float2* planes, const float2* chirp, size_t P, int slot0, int K,
                      const float2* wts, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
     if (P % kPcaTimeCW) return fail(HV_ERR_UNSUPPORTED, "bins-first PCA: frames of a multiple of 64 pixels");
     const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
+    if constexpr (T == 32 || T == 64 || T == 128) {
+        if (pca_time_mfma_ok<TIn>(T, K)) {
+            unsigned grid = 0;
+            HV_TRY((pca_mfma_grid<T, TIn>(grid)));
+            const size_t nchunks = P / kPcaMfmaCW;
+            if (grid > nchunks) grid = (unsigned)nchunks;
+            k_pca_time_mfma<T, TIn><<<grid, 64 * kPcaMfmaWaves, 0, st>>>((const TIn*)raw, planes, chirp, P, slot0,
+                                                                         K, wts, mm_reset, mm_slots, vec);
+            HV_CHECK_LAUNCH("k_pca_time_mfma");
+            return HV_OK;
+        }
+    }
     k_pca_bins_time<T, TIn><<<(unsigned)(P / kPcaTimeCW), kPcaTimeThreads, 0, st>>>(
         (const TIn*)raw, planes, chirp, P, slot0, K, wts, mm_reset, mm_slots, vec);
     HV_CHECK_LAUNCH("k_pca_bins_time");
