@@ -989,7 +989,9 @@ __global__ void __launch_bounds__(64 * kCovWaves)
 
 // T = 32 G (G >= 2): as k_pca_cov_groups -- wave g < G converts frame group
 // g's 16-pixel step into LDS fragments once, every wave runs the products of
-// its pairs -- with real fragments.
+// its pairs -- with real fragments.  Half precisions store the fragments
+// already converted (hi and the split lo), so a group's conversion is done
+// once instead of once per pair that reads it.
 template <int PREC, int G, typename TIn>
 __global__ void __launch_bounds__(64 * kCovGroupWaves)
     k_raw_cov_groups(float* __restrict__ partial, float2* __restrict__ apart, const TIn* __restrict__ raw,
@@ -999,7 +1001,12 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
     constexpr int W = kCovGroupWaves;
     constexpr int PPW = (NP + W - 1) / W;
     constexpr int T = 32 * G;
-    __shared__ float4 frag[2][G][2][64];  // 8 centred, scaled values per lane
+    using FR = RawFrag<PREC>;
+    // 8 centred, scaled values per lane: FP32, or converted once (hi, and the
+    // split lo) for every pair that reads them
+    constexpr int NH = FR::fp32 ? 1 : (FR::split ? 2 : 1);
+    __shared__ float4 frag[2][FR::fp32 ? G : 1][2][64];
+    __shared__ typename FR::V8 hfrag[2][FR::fp32 ? 1 : G][NH][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const size_t per_slice = P / slices;
     const size_t p_begin = (size_t)blockIdx.x * per_slice, p_end = p_begin + per_slice;
@@ -1033,8 +1040,20 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
                 y[e] = c * scale;
             }
             if (p0 + 16 < p_end) load(p0 + 16);
-            frag[buf][wave][0][lane] = make_float4(y[0], y[1], y[2], y[3]);
-            frag[buf][wave][1][lane] = make_float4(y[4], y[5], y[6], y[7]);
+            if constexpr (FR::fp32) {
+                frag[buf][wave][0][lane] = make_float4(y[0], y[1], y[2], y[3]);
+                frag[buf][wave][1][lane] = make_float4(y[4], y[5], y[6], y[7]);
+            } else {
+                typename FR::V8 h, l;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const typename FR::E x = (typename FR::E)y[e];
+                    h[e] = x;
+                    if constexpr (FR::split) l[e] = (typename FR::E)(y[e] - (float)x);
+                }
+                hfrag[buf][wave][0][lane] = h;
+                if constexpr (FR::split) hfrag[buf][wave][1][lane] = l;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -1043,11 +1062,125 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
             if (pair >= NP) break;
             int F, Gp;
             pair_of(pair, F, Gp);
-            const float4 a0 = frag[buf][F][0][lane], a1 = frag[buf][F][1][lane];
-            const float4 b0 = frag[buf][Gp][0][lane], b1 = frag[buf][Gp][1][lane];
-            const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-            const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-            acc[j] = raw_products<PREC>(a, b, acc[j]);
+            if constexpr (FR::fp32) {
+                const float4 a0 = frag[buf][F][0][lane], a1 = frag[buf][F][1][lane];
+                const float4 b0 = frag[buf][Gp][0][lane], b1 = frag[buf][Gp][1][lane];
+                const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+                const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+                acc[j] = raw_products<PREC>(a, b, acc[j]);
+            } else {
+                auto mfma = [](typename FR::V8 x, typename FR::V8 y2, v16f c) {
+                    if constexpr (FR::bf) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y2, c, 0, 0, 0);
+                    else return __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y2, c, 0, 0, 0);
+                };
+                const typename FR::V8 ah = hfrag[buf][F][0][lane], bh = hfrag[buf][Gp][0][lane];
+                if constexpr (FR::split) {  // small cross terms first, then hi * hi (raw_products' order)
+                    acc[j] = mfma(hfrag[buf][F][1][lane], bh, acc[j]);
+                    acc[j] = mfma(ah, hfrag[buf][Gp][1][lane], acc[j]);
+                }
+                acc[j] = mfma(ah, bh, acc[j]);
+            }
+        }
+    }
+    if (conv) {
+        const float a = aacc + __shfl_xor(aacc, 32);
+        if (lane < 32) apart[(size_t)blockIdx.x * T + 32 * wave + lane] = make_float2(a, 0.f);
+    }
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) {
+        const int pair = wave + j * W;
+        if (pair >= NP) break;
+        float* out = partial + ((size_t)blockIdx.x * NP + pair) * kTile;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) out[r * 64 + lane] = acc[j][r];
+    }
+}
+
+// Half precisions, slices of a multiple of 64 pixels: the same products over
+// 64-pixel steps, one barrier per step instead of four.  Lane (frame f, half h)
+// of wave g < G loads pixels 32 h .. 32 h + 31 of the step (64-byte runs for
+// u16, the next step's loads issued before this step's are converted) and
+// gives k-step kk the 8 values 32 h + 8 kk + e, for A and B alike; the step's
+// 64 pixel means are staged in LDS by the last wave one step ahead.
+template <int PREC, int G, typename TIn>
+__global__ void __launch_bounds__(64 * kCovGroupWaves)
+    k_raw_cov_groups_h(float* __restrict__ partial, float2* __restrict__ apart, const TIn* __restrict__ raw,
+                       const float* __restrict__ mean, size_t P, int slices, int slot0, int vec,
+                       const uint32_t* __restrict__ scale_mm) {
+    using FR = RawFrag<PREC>;
+    static_assert(!FR::fp32, "half precisions");
+    constexpr int NP = G * (G + 1) / 2;
+    constexpr int W = kCovGroupWaves;
+    constexpr int PPW = (NP + W - 1) / W;
+    constexpr int T = 32 * G;
+    constexpr int NH = FR::split ? 2 : 1;
+    constexpr int SS = 64;  // pixels per step
+    __shared__ typename FR::V8 hfrag[2][4][G][NH][64];
+    __shared__ float4 mu[2][SS / 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5;
+    const size_t per_slice = P / slices;
+    const size_t p_begin = (size_t)blockIdx.x * per_slice, p_end = p_begin + per_slice;
+    float scale = 1.0f;
+    if constexpr (PREC == HV_PCA_FP16 || PREC == HV_PCA_FP16_SPLIT) scale = fp16_scale(scale_mm);
+    const bool conv = wave < G, stager = wave == W - 1 && lane < SS / 4;
+    const int f = 32 * (conv ? wave : 0) + (lane & 31);
+    const TIn* row = raw + (size_t)((f - slot0) & (T - 1)) * P + 32 * h;
+    RawPx<TIn, 32> nx;
+    if (conv) nx.load(row + p_begin, vec);
+    if (stager) mu[0][lane] = reinterpret_cast<const float4*>(mean + p_begin)[lane];
+    __syncthreads();
+    float aacc = 0.f;
+    v16f acc[PPW];
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) acc[j] = v16f{};
+    int buf = 0;
+    for (size_t p0 = p_begin; p0 < p_end; p0 += SS, buf ^= 1) {
+        const bool more = p0 + SS < p_end;
+        if (conv) {
+            const float* m = reinterpret_cast<const float*>(mu[buf]) + 32 * h;
+            float xs[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) xs[i] = nx.get(i);
+            if (more) nx.load(row + p0 + SS, vec);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                typename FR::V8 hv, lv;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float me = m[8 * kk + e];
+                    const float c = xs[8 * kk + e] - me;
+                    aacc = fmaf(c, me, aacc);
+                    const float y = c * scale;
+                    const typename FR::E x = (typename FR::E)y;
+                    hv[e] = x;
+                    if constexpr (FR::split) lv[e] = (typename FR::E)(y - (float)x);
+                }
+                hfrag[buf][kk][wave][0][lane] = hv;
+                if constexpr (FR::split) hfrag[buf][kk][wave][1][lane] = lv;
+            }
+        }
+        // the next step's means (read from the other buffer after the barrier)
+        if (stager && more) mu[buf ^ 1][lane] = reinterpret_cast<const float4*>(mean + p0 + SS)[lane];
+        __syncthreads();
+        auto mfma = [](typename FR::V8 x, typename FR::V8 y2, v16f c) {
+            if constexpr (FR::bf) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y2, c, 0, 0, 0);
+            else return __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y2, c, 0, 0, 0);
+        };
+#pragma unroll
+        for (int j = 0; j < PPW; ++j) {
+            const int pair = wave + j * W;
+            if (pair >= NP) break;
+            int F, Gp;
+            pair_of(pair, F, Gp);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const typename FR::V8 ah = hfrag[buf][kk][F][0][lane], bh = hfrag[buf][kk][Gp][0][lane];
+                if constexpr (FR::split) {  // small cross terms first, then hi * hi
+                    acc[j] = mfma(hfrag[buf][kk][F][1][lane], bh, acc[j]);
+                    acc[j] = mfma(ah, hfrag[buf][kk][Gp][1][lane], acc[j]);
+                }
+                acc[j] = mfma(ah, bh, acc[j]);
+            }
         }
     }
     if (conv) {
@@ -1066,7 +1199,15 @@ __global__ void __launch_bounds__(64 * kCovGroupWaves)
 
 // Pixel slices of the raw covariance: T <= 32 as the pair kernel (one pair),
 // larger windows as the grouped kernel.
-static int raw_cov_slices(int T, size_t P) { return T <= 32 ? pca_slices_pairs(T, P) : pca_slices_grouped(P); }
+// T >= 64: slices of a multiple of 64 pixels where the frame allows
+// (k_raw_cov_groups_h), else of 16 (k_raw_cov_groups).
+static int raw_cov_slices(int T, size_t P) {
+    if (T <= 32) return pca_slices_pairs(T, P);
+    if (P % 64) return pca_slices_grouped(P);
+    int ns = 1;
+    while (ns * 2 <= kCovGroupSlices && (size_t)(ns * 2) * 128 <= P && P % ((size_t)(ns * 2) * 64) == 0) ns *= 2;
+    return ns;
+}
 
 static CovLayout raw_cov_layout(int T, size_t P) {
     CovLayout l;
@@ -1086,6 +1227,15 @@ static hv_status raw_cov_t(float* partial, float2* apart, const void* raw, const
                            int slices, int slot0, int vec, const uint32_t* scale_mm, hipStream_t st) {
     const TIn* r = (const TIn*)raw;
     constexpr int tg = 64 * kCovGroupWaves;
+    if constexpr (!RawFrag<PREC>::fp32) {
+        // T = 256: eight groups' step fragments would take 128 KiB of LDS
+        if ((T == 64 || T == 128) && (P / slices) % 64 == 0) {
+            if (T == 64) k_raw_cov_groups_h<PREC, 2, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm);
+            else k_raw_cov_groups_h<PREC, 4, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm);
+            HV_CHECK_LAUNCH("k_raw_cov_groups_h");
+            return HV_OK;
+        }
+    }
     switch (T) {
         case 64: k_raw_cov_groups<PREC, 2, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm); break;
         case 128: k_raw_cov_groups<PREC, 4, TIn><<<slices, tg, 0, st>>>(partial, apart, r, mean, P, slices, slot0, vec, scale_mm); break;
