@@ -871,6 +871,164 @@ static hv_status row_pass_lay(const float2* in, float2* out, int width, int heig
     }
 }
 
+// The split path's last pass on column-tiled planes: inverse row FFT of
+// every plane and the per-pixel epilogue in one kernel, so the propagated
+// planes never return to HBM.  Each workgroup owns M rows and walks the K
+// planes (the next plane's tile values loaded before the current one is
+// transformed); a thread's outputs sit in the same register slots for every
+// plane, where their sums accumulate:
+//   EPI = kEpiMoments: moments (k_bins_moments' formulas: bins fs + k with
+//            f = fftfreq(T) fps, the Nyquist bin counted twice, its M1 terms
+//            cancelling -- added where the bin comes, not after the loop);
+//   EPI = HV_IMG_*: image term mean over the planes x scale (k_bins_image).
+// Then the outputs and the min/max partials of the image (or selected moment).
+constexpr int kEpiMoments = -1;  // (image types are HV_IMG_MODULUS = 0, ...)
+struct EpiArgs {
+    float* m0;
+    float* m1;
+    float* m2;
+    float* img;
+    int T, fs, K, sel;
+    float fps, count, scale;
+    uint32_t* mm;
+    int mm_slots;
+};
+
+// 8 values per thread (the plane sums, the next plane's values and the
+// transform in registers: 16 values per thread left one wavefront per SIMD),
+// rows per workgroup so that the tile runs of consecutive rows are read
+// together (two rows: 64-byte runs of 4-column tiles).
+template <int N>
+struct EpiCfg {
+    static constexpr int E = 8;
+    static constexpr int NT = FftShape<N, E>::NT;
+    static constexpr int M = NT >= 512 ? 1 : (NT >= 256 ? 2 : 512 / NT);
+    static constexpr int THREADS = M * NT;
+    static constexpr int PITCH = N + N / 8;
+};
+
+template <int N, int EPI>
+__global__ void __launch_bounds__(EpiCfg<N>::THREADS)
+    k_row_ifft_epi_tiled(const float2* __restrict__ tiles, int height, int tl, EpiArgs ea) {
+    using C = EpiCfg<N>;
+    constexpr int kE = FftShape<N, C::E>::kE;
+    __shared__ float2 lds[C::M * C::PITCH];
+    const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
+    const size_t P = (size_t)N * height;
+    const int y = blockIdx.x * C::M + m;
+    const bool valid = y < height;
+    const int yc = valid ? y : 0;
+    float2* my = lds + m * C::PITCH;
+    auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
+    // the slots' element offsets within a plane, the same for every plane
+    uint32_t off[kE];
+    fft_load_slots<N, C::E>(t, [&](int i, int n) { off[i] = (uint32_t)tiled_off(yc, n, height, tl); });
+    float2 px[kE];
+    auto load_plane = [&](int k) {
+        const float2* src = tiles + (size_t)k * P;
+#pragma unroll
+        for (int i = 0; i < kE; ++i) px[i] = src[off[i]];
+    };
+    load_plane(0);
+    constexpr bool MOM = EPI == kEpiMoments;
+    float a0[kE], a1[MOM ? kE : 1], a2[MOM ? kE : 1];
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+        a0[i] = 0.f;
+        if constexpr (MOM) {
+            a1[i] = 0.f;
+            a2[i] = 0.f;
+        }
+    }
+    const float d = MOM ? ea.fps / ea.T : 0.f;
+    const int nyq = ea.T / 2;
+    for (int k = 0; k < ea.K; ++k) {
+        float2 v[kE];
+#pragma unroll
+        for (int i = 0; i < kE; ++i) v[i] = px[i];
+        if (k + 1 < ea.K) load_plane(k + 1);
+        const int bin = ea.fs + k;
+        const float f = bin < nyq ? bin * d : -((float)ea.T - bin) * d;
+        // twiddles generated per stage: the sums and the next plane's values hold the registers
+        fft_compute_tw<N, C::E, +1, false, (C::NT <= 64)>(t, v, TwOnTheFly{}, [&](int, float2 z, int slot) {
+            if constexpr (MOM) {
+                const float sq = z.x * z.x + z.y * z.y;
+                a0[slot] += sq * 1.0f;
+                a1[slot] += sq * f;
+                a2[slot] += sq * (f * f);
+                if (bin == nyq) {  // the Nyquist bin counted again: its frequency's sign cancels in M1
+                    a0[slot] += sq * 1.0f;
+                    a1[slot] -= sq * f;
+                    a2[slot] += sq * (f * f);
+                }
+            } else {
+                a0[slot] += image_term_fast<EPI>(z);
+            }
+        }, lds_at);
+    }
+    float vmin = INFINITY, vmax = -INFINITY;
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < kE; ++i) {
+            const size_t p = (size_t)y * N + fft_out_index<N, C::E>(t, i);
+            float v;
+            if constexpr (MOM) {
+                if (ea.m0) ea.m0[p] = a0[i];
+                if (ea.m1) ea.m1[p] = a1[i];
+                if (ea.m2) ea.m2[p] = a2[i];
+                v = ea.sel == 0 ? a0[i] : ea.sel == 1 ? a1[i] : a2[i];
+            } else {
+                v = a0[i] / ea.count * ea.scale;
+            }
+            if (ea.img) ea.img[p] = v;
+            vmin = fminf(vmin, v);
+            vmax = fmaxf(vmax, v);
+        }
+    }
+    if (ea.mm) block_minmax_atomic(vmin, vmax, ea.mm, ea.mm_slots);
+}
+
+template <int N, int EPI>
+static hv_status row_ifft_epi_n(const float2* tiles, int height, int tl, const EpiArgs& ea, hipStream_t st) {
+    using C = EpiCfg<N>;
+    k_row_ifft_epi_tiled<N, EPI><<<(height + C::M - 1) / C::M, C::THREADS, 0, st>>>(tiles, height, tl, ea);
+    HV_CHECK_LAUNCH("k_row_ifft_epi_tiled");
+    return HV_OK;
+}
+
+template <int EPI>
+static hv_status row_ifft_epi(const float2* tiles, int width, int height, const EpiArgs& ea, hipStream_t st) {
+    const int tl = col_tile_log2(height);
+    switch (width) {
+        case 64: return row_ifft_epi_n<64, EPI>(tiles, height, tl, ea, st);
+        case 128: return row_ifft_epi_n<128, EPI>(tiles, height, tl, ea, st);
+        case 256: return row_ifft_epi_n<256, EPI>(tiles, height, tl, ea, st);
+        case 512: return row_ifft_epi_n<512, EPI>(tiles, height, tl, ea, st);
+        case 1024: return row_ifft_epi_n<1024, EPI>(tiles, height, tl, ea, st);
+        case 2048: return row_ifft_epi_n<2048, EPI>(tiles, height, tl, ea, st);
+        case 4096: return row_ifft_epi_n<4096, EPI>(tiles, height, tl, ea, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "column-tiled planes: width must be a power of two in [64, 4096]");
+    }
+}
+
+hv_status launch_row_ifft_moments_tiled(float* m0, float* m1, float* m2, float* img, const float2* tiles, int width,
+                                        int height, int T, int fs, int K, float fps, int sel, uint32_t* mm,
+                                        int mm_slots, hipStream_t st) {
+    const EpiArgs ea{m0, m1, m2, img, T, fs, K, sel, fps, 1.f, 1.f, mm, mm_slots};
+    return row_ifft_epi<kEpiMoments>(tiles, width, height, ea, st);
+}
+
+hv_status launch_row_ifft_image_tiled(float* out, const float2* tiles, int width, int height, int K, int img_type,
+                                      float count, float scale, uint32_t* mm, int mm_slots, hipStream_t st) {
+    const EpiArgs ea{nullptr, nullptr, nullptr, out, 0, 0, K, 0, 0.f, count, scale, mm, mm_slots};
+    switch (img_type) {
+        case HV_IMG_MODULUS: return row_ifft_epi<HV_IMG_MODULUS>(tiles, width, height, ea, st);
+        case HV_IMG_SQUARED_MODULUS: return row_ifft_epi<HV_IMG_SQUARED_MODULUS>(tiles, width, height, ea, st);
+        case HV_IMG_ARGUMENT: return row_ifft_epi<HV_IMG_ARGUMENT>(tiles, width, height, ea, st);
+        default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: image type must be modulus, squared or argument");
+    }
+}
+
 bool col_tiled_supported(int width, int height) {
     const int tl = col_tile_log2(height);
     return tl >= 0 && is_pow2(width) && width >= 64 && width <= 4096 && width % (1 << tl) == 0;

@@ -242,6 +242,10 @@ def test_pipe_moments(oracle, img_type, level):
     # 2048- and 4096-long columns: the planes column-tiled between the row passes (2 K <= T)
     (64, 2048, 128, 2, 5, 4, 56),   # four-column tiles, moment 0 over 57 bins
     (128, 4096, 128, 2, 0, 2, 9),   # two-column tiles, modulus image
+    (64, 2048, 128, 2, 6, 40, 60),  # tiles, moment 1 over bins 40..100 (the Nyquist bin inside)
+    (64, 2048, 128, 2, 7, 2, 30),   # tiles, moment 2
+    (64, 2048, 64, 2, 2, 1, 20),    # tiles, argument image
+    (32, 2048, 32, 2, 1, 0, 15),    # tiles, squared modulus, bins 0..15 (DC)
     (64, 2048, 64, 2, 6, 3, 40),    # 2 K > T: row-major planes, moment 1
 ])
 def test_pipe_bins_split(oracle, W, H, T, space, img_type, p0, pw):
@@ -251,7 +255,8 @@ def test_pipe_bins_split(oracle, W, H, T, space, img_type, p0, pw):
     s, fl = frames("c5_stft", 1, width=W, height=H, batch_size=T, time_stride=T, time_transformation_size=T,
                    space_transformation=space, img_type=img_type, p_start=p0, p_width=pw, camera_fps=float(T),
                    z_distance=0.35, syn_period=T)
-    check(run_both(oracle, s, fl, path=2), 1e-5)
+    # the argument's atan2 turns rounding at small |z| into larger angle steps (as for the other argument images)
+    check(run_both(oracle, s, fl, path=2), 1e-5, u16_max=4 if img_type == 2 else 1)
 
 
 def designed_frames(T, W, H, seed):
