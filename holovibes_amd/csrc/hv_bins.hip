@@ -981,10 +981,13 @@ bool bins_image_fused(int width, int space) {
 template <int IMG>
 hv_status bins_image_t(float* out, float2* planes, int width, int height, int K, int space, float count,
                        uint32_t* mm, int mm_slots, float scale, hipStream_t st, float2* tiles) {
-    // column-tiled planes: the inverse row FFT and the image term in one kernel
-    if (tiles) return launch_row_ifft_image_tiled(out, tiles, width, height, K, IMG, count, scale, mm, mm_slots, st);
+    // column-tiled planes, or row-major ones too long for the image rows in
+    // LDS: the inverse row FFT and the image term in one kernel
+    if (tiles) return launch_row_ifft_image(out, tiles, width, height, K, IMG, count, scale, mm, mm_slots, st, true);
     if (!bins_image_fused(width, space)) {
-        // split: the planes' inverse row FFT (angular spectrum), then the image term
+        if (space == HV_SPACE_ANGULARSP && row_ifft_epi_supported(width))
+            return launch_row_ifft_image(out, planes, width, height, K, IMG, count, scale, mm, mm_slots, st, false);
+        // the planes' inverse row FFT (angular spectrum, other widths), then the image term
         if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
         const size_t P = (size_t)width * height;
         const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);
@@ -1010,10 +1013,13 @@ hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* pla
                               int fs, int K, int space, float fps, int mm_moment, uint32_t* mm, hipStream_t st,
                               int mm_slots, float2* tiles) {
     float2* planes = const_cast<float2*>(planes_c);
-    // column-tiled planes: the inverse row FFT and the moments in one kernel
+    // angular spectrum: the inverse row FFT and the moments in one kernel (column-tiled or row-major planes)
     if (tiles)
-        return launch_row_ifft_moments_tiled(m0, m1, m2, nullptr, tiles, width, height, T, fs, K, fps, mm_moment, mm,
-                                             mm_slots, st);
+        return launch_row_ifft_moments(m0, m1, m2, nullptr, tiles, width, height, T, fs, K, fps, mm_moment, mm,
+                                       mm_slots, st, true);
+    if (space == HV_SPACE_ANGULARSP && row_ifft_epi_supported(width))
+        return launch_row_ifft_moments(m0, m1, m2, nullptr, planes, width, height, T, fs, K, fps, mm_moment, mm,
+                                       mm_slots, st, false);
     if (space == HV_SPACE_ANGULARSP) HV_TRY(launch_row_pass(planes, width, height, K, +1, st));
     const size_t P = (size_t)width * height;
     const int blocks = (int)std::min<size_t>((P + 255) / 256, 2048);

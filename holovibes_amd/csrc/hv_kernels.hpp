@@ -32,13 +32,15 @@ hv_status launch_row_pass(float2* buf, int width, int height, int planes, int di
 bool col_tiled_supported(int width, int height);
 // direction -1: forward row FFT, row-major in, tiled out; +1: inverse row
 // FFT, tiled in, row-major out (out of place).
-// Inverse row FFT of K column-tiled planes with the moments (k_bins_moments'
-// outputs, img = moment sel) or the image term mean in the same kernel.
-hv_status launch_row_ifft_moments_tiled(float* m0, float* m1, float* m2, float* img, const float2* tiles, int width,
-                                        int height, int T, int fs, int K, float fps, int sel, uint32_t* mm,
-                                        int mm_slots, hipStream_t st);
-hv_status launch_row_ifft_image_tiled(float* out, const float2* tiles, int width, int height, int K, int img_type,
-                                      float count, float scale, uint32_t* mm, int mm_slots, hipStream_t st);
+// Inverse row FFT of K planes (column-tiled, or row-major) with the moments
+// (k_bins_moments' outputs, img = moment sel) or the image term mean in the
+// same kernel; widths row_ifft_epi_supported.
+bool row_ifft_epi_supported(int width);
+hv_status launch_row_ifft_moments(float* m0, float* m1, float* m2, float* img, const float2* planes, int width,
+                                  int height, int T, int fs, int K, float fps, int sel, uint32_t* mm, int mm_slots,
+                                  hipStream_t st, bool tiled);
+hv_status launch_row_ifft_image(float* out, const float2* planes, int width, int height, int K, int img_type,
+                                float count, float scale, uint32_t* mm, int mm_slots, hipStream_t st, bool tiled);
 hv_status launch_row_pass_tiled(const float2* in, float2* out, int width, int height, int planes, int direction,
                                 hipStream_t st);
 // column FFT x lens x inverse column FFT of tiled planes (in place), lens tiled too

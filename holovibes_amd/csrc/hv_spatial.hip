@@ -871,9 +871,9 @@ static hv_status row_pass_lay(const float2* in, float2* out, int width, int heig
     }
 }
 
-// The split path's last pass on column-tiled planes: inverse row FFT of
-// every plane and the per-pixel epilogue in one kernel, so the propagated
-// planes never return to HBM.  Each workgroup owns M rows and walks the K
+// The split path's last pass (column-tiled or row-major planes): inverse row
+// FFT of every plane and the per-pixel epilogue in one kernel, so the
+// propagated planes never return to HBM.  Each workgroup owns M rows and walks the K
 // planes (the next plane's tile values loaded before the current one is
 // transformed); a thread's outputs sit in the same register slots for every
 // plane, where their sums accumulate:
@@ -898,17 +898,23 @@ struct EpiArgs {
 // transform in registers: 16 values per thread left one wavefront per SIMD),
 // rows per workgroup so that the tile runs of consecutive rows are read
 // together (two rows: 64-byte runs of 4-column tiles).
+#ifndef HV_EPI_ROWS
+#define HV_EPI_ROWS 2           // rows per workgroup of the tiled epilogue pass for 2048-long rows
+#endif
+#ifndef HV_EPI_WAVES_PER_EU
+#define HV_EPI_WAVES_PER_EU 1
+#endif
 template <int N>
 struct EpiCfg {
     static constexpr int E = 8;
     static constexpr int NT = FftShape<N, E>::NT;
-    static constexpr int M = NT >= 512 ? 1 : (NT >= 256 ? 2 : 512 / NT);
+    static constexpr int M = NT >= 512 ? 1 : (NT >= 256 ? HV_EPI_ROWS : 512 / NT);
     static constexpr int THREADS = M * NT;
     static constexpr int PITCH = N + N / 8;
 };
 
 template <int N, int EPI>
-__global__ void __launch_bounds__(EpiCfg<N>::THREADS)
+__global__ void __launch_bounds__(EpiCfg<N>::THREADS) __attribute__((amdgpu_waves_per_eu(HV_EPI_WAVES_PER_EU)))
     k_row_ifft_epi_tiled(const float2* __restrict__ tiles, int height, int tl, EpiArgs ea) {
     using C = EpiCfg<N>;
     constexpr int kE = FftShape<N, C::E>::kE;
@@ -921,8 +927,11 @@ __global__ void __launch_bounds__(EpiCfg<N>::THREADS)
     float2* my = lds + m * C::PITCH;
     auto lds_at = [&](int n) -> float2& { return my[n + (n >> 3)]; };
     // the slots' element offsets within a plane, the same for every plane
+    // (tl < 0: row-major planes)
     uint32_t off[kE];
-    fft_load_slots<N, C::E>(t, [&](int i, int n) { off[i] = (uint32_t)tiled_off(yc, n, height, tl); });
+    fft_load_slots<N, C::E>(t, [&](int i, int n) {
+        off[i] = tl >= 0 ? (uint32_t)tiled_off(yc, n, height, tl) : (uint32_t)(yc * N + n);
+    });
     float2 px[kE];
     auto load_plane = [&](int k) {
         const float2* src = tiles + (size_t)k * P;
@@ -996,9 +1005,12 @@ static hv_status row_ifft_epi_n(const float2* tiles, int height, int tl, const E
     return HV_OK;
 }
 
+bool row_ifft_epi_supported(int width) { return is_pow2(width) && width >= 64 && width <= 4096; }
+
 template <int EPI>
-static hv_status row_ifft_epi(const float2* tiles, int width, int height, const EpiArgs& ea, hipStream_t st) {
-    const int tl = col_tile_log2(height);
+static hv_status row_ifft_epi(const float2* tiles, int width, int height, const EpiArgs& ea, hipStream_t st,
+                              bool tiled) {
+    const int tl = tiled ? col_tile_log2(height) : -1;
     switch (width) {
         case 64: return row_ifft_epi_n<64, EPI>(tiles, height, tl, ea, st);
         case 128: return row_ifft_epi_n<128, EPI>(tiles, height, tl, ea, st);
@@ -1011,20 +1023,20 @@ static hv_status row_ifft_epi(const float2* tiles, int width, int height, const 
     }
 }
 
-hv_status launch_row_ifft_moments_tiled(float* m0, float* m1, float* m2, float* img, const float2* tiles, int width,
-                                        int height, int T, int fs, int K, float fps, int sel, uint32_t* mm,
-                                        int mm_slots, hipStream_t st) {
+hv_status launch_row_ifft_moments(float* m0, float* m1, float* m2, float* img, const float2* planes, int width,
+                                  int height, int T, int fs, int K, float fps, int sel, uint32_t* mm, int mm_slots,
+                                  hipStream_t st, bool tiled) {
     const EpiArgs ea{m0, m1, m2, img, T, fs, K, sel, fps, 1.f, 1.f, mm, mm_slots};
-    return row_ifft_epi<kEpiMoments>(tiles, width, height, ea, st);
+    return row_ifft_epi<kEpiMoments>(planes, width, height, ea, st, tiled);
 }
 
-hv_status launch_row_ifft_image_tiled(float* out, const float2* tiles, int width, int height, int K, int img_type,
-                                      float count, float scale, uint32_t* mm, int mm_slots, hipStream_t st) {
+hv_status launch_row_ifft_image(float* out, const float2* planes, int width, int height, int K, int img_type,
+                                float count, float scale, uint32_t* mm, int mm_slots, hipStream_t st, bool tiled) {
     const EpiArgs ea{nullptr, nullptr, nullptr, out, 0, 0, K, 0, 0.f, count, scale, mm, mm_slots};
     switch (img_type) {
-        case HV_IMG_MODULUS: return row_ifft_epi<HV_IMG_MODULUS>(tiles, width, height, ea, st);
-        case HV_IMG_SQUARED_MODULUS: return row_ifft_epi<HV_IMG_SQUARED_MODULUS>(tiles, width, height, ea, st);
-        case HV_IMG_ARGUMENT: return row_ifft_epi<HV_IMG_ARGUMENT>(tiles, width, height, ea, st);
+        case HV_IMG_MODULUS: return row_ifft_epi<HV_IMG_MODULUS>(planes, width, height, ea, st, tiled);
+        case HV_IMG_SQUARED_MODULUS: return row_ifft_epi<HV_IMG_SQUARED_MODULUS>(planes, width, height, ea, st, tiled);
+        case HV_IMG_ARGUMENT: return row_ifft_epi<HV_IMG_ARGUMENT>(planes, width, height, ea, st, tiled);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: image type must be modulus, squared or argument");
     }
 }
