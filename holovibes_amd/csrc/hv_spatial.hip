@@ -921,7 +921,17 @@ __global__ void __launch_bounds__(EpiCfg<N>::THREADS) __attribute__((amdgpu_wave
     __shared__ float2 lds[C::M * C::PITCH];
     const int m = threadIdx.x / C::NT, t = threadIdx.x % C::NT;
     const size_t P = (size_t)N * height;
-    const int y = blockIdx.x * C::M + m;
+    // tiles: a 128-byte line of a tile holds 16 >> tl rows; the workgroups
+    // whose rows share lines go to one XCD one after another (its L2 then
+    // fetches each line once)
+    int grp = blockIdx.x;
+    if (tl >= 0) {
+        const int lg = (16 >> tl) / C::M;
+        if (lg == 2) grp = xcd_tile<2>(blockIdx.x, gridDim.x);
+        else if (lg == 4) grp = xcd_tile<4>(blockIdx.x, gridDim.x);
+        else if (lg >= 8) grp = xcd_tile<8>(blockIdx.x, gridDim.x);
+    }
+    const int y = grp * C::M + m;
     const bool valid = y < height;
     const int yc = valid ? y : 0;
     float2* my = lds + m * C::PITCH;
