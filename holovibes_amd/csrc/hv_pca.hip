@@ -1338,7 +1338,7 @@ __device__ __forceinline__ void rr_pair(int n, int r, int k, int& p, int& q) {
 #define HV_JACOBI_MAX_SWEEPS 40
 #endif
 #ifndef HV_JACOBI_NT32
-#define HV_JACOBI_NT32 512      // threads of the T <= 32 eigensolver
+#define HV_JACOBI_NT32 1024     // threads of the T <= 32 eigensolver (A's blocks and V's row pairs on separate threads)
 #endif
 #ifndef HV_JACOBI_NT64
 #define HV_JACOBI_NT64 1024     // threads of the T <= 64 eigensolver
@@ -1581,7 +1581,9 @@ __device__ void jacobi_run(E* A, E* V, int ld, double* evals, double2* evecs, Ro
                 a(q, qq) = b11;
             }
             if constexpr (!DEFER) {
-                for (int task = tid; task < half * n; task += NT) {
+                // with threads for both, V's row pairs go to the threads past A's blocks, in parallel
+                constexpr int v0 = NT >= half * half + half * n ? half * half : 0;
+                for (int task = tid - v0; task >= 0 && task < half * n; task += NT) {
                     const int m = task % n, kk = task / n;
                     int pp, qq;
                     rr_pair(n, r, kk, pp, qq);
