@@ -30,8 +30,9 @@ hv_status launch_row_pass(float2* buf, int width, int height, int planes, int di
 // element (y, x) at ((x / CW) H + y) CW + x mod CW with CW the column tile of
 // the column pass, so that every column tile is one contiguous block.
 bool col_tiled_supported(int width, int height);
-// direction -1: forward row FFT, row-major in, tiled out; +1: inverse row
-// FFT, tiled in, row-major out (out of place).
+// Forward row FFT (direction -1), row-major in, tiled out (out of place).
+hv_status launch_row_pass_tiled(const float2* in, float2* out, int width, int height, int planes, int direction,
+                                hipStream_t st);
 // Inverse row FFT of K planes (column-tiled, or row-major) with the moments
 // (k_bins_moments' outputs, img = moment sel) or the image term mean in the
 // same kernel; widths row_ifft_epi_supported.
@@ -41,8 +42,6 @@ hv_status launch_row_ifft_moments(float* m0, float* m1, float* m2, float* img, c
                                   hipStream_t st, bool tiled);
 hv_status launch_row_ifft_image(float* out, const float2* planes, int width, int height, int K, int img_type,
                                 float count, float scale, uint32_t* mm, int mm_slots, hipStream_t st, bool tiled);
-hv_status launch_row_pass_tiled(const float2* in, float2* out, int width, int height, int planes, int direction,
-                                hipStream_t st);
 // column FFT x lens x inverse column FFT of tiled planes (in place), lens tiled too
 hv_status launch_col_lens_tiled(float2* buf, const float2* lens_t, int width, int height, int planes,
                                 hipStream_t st);

@@ -263,8 +263,8 @@ __device__ __forceinline__ size_t tiled_off(int y, int x, int height, int tl) {
 }
 
 // Row-pass layouts: LAY_ROWS row-major in and out; LAY_TO_TILES row-major in,
-// column-tiled out; LAY_FROM_TILES column-tiled in, row-major out.
-enum { LAY_ROWS = 0, LAY_TO_TILES = 1, LAY_FROM_TILES = 2 };
+// column-tiled out (the tiles are read back by k_row_ifft_epi_tiled).
+enum { LAY_ROWS = 0, LAY_TO_TILES = 1 };
 
 // Each workgroup transforms `iters` groups of M rows of one frame; the loads
 // of group i + 1 (pixels, and chirp factors with CHIRP) are issued before
@@ -292,8 +292,7 @@ __global__ void __launch_bounds__((RowCfgL<N, LAY>::THREADS))
         const int yc = yr < height ? yr : 0;
         const size_t row = (size_t)yc * N;
         fft_load_slots<N, C::E>(t, [&](int i, int n) {
-            if constexpr (LAY == LAY_FROM_TILES) px[i] = load_px<KIN>(in, in_frame + tiled_off(yc, n, height, tl));
-            else px[i] = load_px<KIN>(in, in_frame + row + n);
+            px[i] = load_px<KIN>(in, in_frame + row + n);
             if constexpr (CHIRP) cf[i] = chirp[row + n];
         });
     };
@@ -1060,9 +1059,8 @@ hv_status launch_row_pass_tiled(const float2* in, float2* out, int width, int he
                                 hipStream_t st) {
     if (!col_tiled_supported(width, height)) return fail(HV_ERR_UNSUPPORTED, "column-tiled planes: unsupported shape");
     if (in == out) return fail(HV_ERR_INVALID_ARG, "a layout-changing row pass runs out of place");
-    const int tl = col_tile_log2(height);
-    if (direction < 0) return row_pass_lay<-1, LAY_TO_TILES>(in, out, width, height, planes, tl, st);
-    return row_pass_lay<+1, LAY_FROM_TILES>(in, out, width, height, planes, tl, st);
+    if (direction >= 0) return fail(HV_ERR_UNSUPPORTED, "column tiles are written by the forward row pass only");
+    return row_pass_lay<-1, LAY_TO_TILES>(in, out, width, height, planes, col_tile_log2(height), st);
 }
 
 template <int N>
