@@ -1,7 +1,8 @@
 """Benchmark of the per-batch compute path on one MI355X.
 
 A step is one Pipe cycle (hv_pipe_push_batch) over one batch of synthetic
-frames already resident in HBM: conversion + angular spectrum into the
+frames already resident in HBM (the timed batches pushed through one native
+loop, hv_pipe_push_batches, as the compute worker runs its cycles): conversion + angular spectrum into the
 time-window ring, STFT + image type, contrast, u16.  Default workload:
 512 x 512 u16, B = S = T = 32 (see holovibes_amd/configs.py "metric").
 
@@ -60,6 +61,8 @@ def parse():
                     help="highest cycle path allowed: 0 three passes + STFT, 1 fused row-IFFT + STFT, 2 bins-first")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--python-push", action="store_true",
+                    help="time one Python call per batch (hv_pipe_push_batch) instead of the native batch loop")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="only run warmup + this many steps (for rocprofv3), print the JSON without extras")
     ap.add_argument("--stage-iters", type=int, default=50)
@@ -220,10 +223,9 @@ def throughput(s, dev_batches, level, lanes, steps, warmup):
         p.push(dev_batches[k % n])
         k += 1
     torch.cuda.synchronize()
+    ptrs = Pipe.batch_pointers([dev_batches[(k + i) % n] for i in range(steps)])
     t0 = time.perf_counter()
-    for _ in range(steps):
-        p.push(dev_batches[k % n])
-        k += 1
+    p.push_many(ptrs)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     path = p.path
@@ -655,10 +657,17 @@ def main():
         k += 1
     torch.cuda.synchronize()
     steps = args.profile_steps or args.steps
+    # the timed batches go through one native loop (hv_pipe_push_batches, the
+    # compute worker's cycle loop), or one Python call each with --python-push
+    seq = [dev_batches[(k + i) % nbatches] for i in range(steps)]
+    ptrs = Pipe.batch_pointers(seq)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        pipe.push(dev_batches[k % nbatches])
-        k += 1
+    if args.python_push:
+        for b in seq:
+            pipe.push(b)
+    else:
+        pipe.push_many(ptrs)
+    k += steps
     t_host = time.perf_counter() - t0
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
@@ -666,7 +675,9 @@ def main():
     frames_per_s = steps * B / el
     line = {"metric": "frames/s + p99 batch->image latency at 512^2 T=32; % HBM roofline; err vs FP64",
             "value": frames_per_s, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-            "ms_per_step": ms_per_step, "host_submit_ms_per_step": t_host / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "ms_per_step": ms_per_step, "host_submit_ms_per_step": t_host / steps * 1e3,
+            "host_loop": "python, one hv_pipe_push_batch call per batch" if args.python_push else
+            "native: hv_pipe_push_batches over the timed batches", "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic (seeded interferogram generator)",
             "config": {"workload": name, "overrides": overrides, "width": s.width, "height": s.height, "depth_bytes": s.depth,
                        "batch": B, "stride": S, "T": T, "space": ["none", "fresnel", "angular_spectrum"][

@@ -163,6 +163,7 @@ SIGNATURES = {
     "hv_pipe_create": [P, P, P],
     "hv_pipe_memory_bytes": [P, P],
     "hv_pipe_push_batch": [P, P, P],
+    "hv_pipe_push_batches": [P, P, I, P],
     "hv_pipe_output": [P, P, P, P],
     "hv_pipe_contrast_range": [P, P],
     "hv_pipe_set_graph": [P, I],

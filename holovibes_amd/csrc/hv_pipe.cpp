@@ -882,6 +882,17 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     return HV_OK;
 }
 
+hv_status hv_pipe_push_batches(hv_pipe* p, const void* const* frames, int n, int* produced) {
+    if (produced) *produced = 0;
+    if (!p || (!frames && n > 0) || n < 0) return fail(HV_ERR_INVALID_ARG, "null argument or negative count");
+    for (int i = 0; i < n; ++i) {
+        int one = 0;
+        HV_TRY(hv_pipe_push_batch(p, frames[i], &one));
+        if (produced) *produced += one;
+    }
+    return HV_OK;
+}
+
 hv_status hv_pipe_set_timing(hv_pipe* p, int enabled) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (enabled && !p->ev[0])

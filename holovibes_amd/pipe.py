@@ -80,6 +80,20 @@ class Pipe:
         _lib.call("hv_pipe_push_batch", self.h, ctypes.c_void_p(ptr), ctypes.byref(produced))
         return bool(produced.value)
 
+    @staticmethod
+    def batch_pointers(batches) -> ctypes.Array:
+        """Device pointers of a batch sequence, for push_many."""
+        return (ctypes.c_void_p * len(batches))(
+            *[b.data_ptr() if isinstance(b, torch.Tensor) else int(b) for b in batches])
+
+    def push_many(self, pointers: ctypes.Array) -> int:
+        """hv_pipe_push_batches: every batch of `pointers` (batch_pointers) in
+        one native loop; returns the number of output images written."""
+        produced = ctypes.c_int()
+        _lib.call("hv_pipe_push_batches", self.h, ctypes.cast(pointers, ctypes.c_void_p), len(pointers),
+                  ctypes.byref(produced))
+        return produced.value
+
     @property
     def path(self) -> int:
         """Cycle path in use: THREE_PASS, ROW_STFT or BINS."""

@@ -391,6 +391,13 @@ hv_status hv_pipe_memory_bytes(const hv_compute_settings* s, size_t* bytes);
  * the cycle exited early.  Asynchronous: work is queued on the pipe stream. */
 hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced);
 
+/* hv_pipe_push_batch for n batches in order, in one call: the loop of the
+ * compute worker (Pipe::exec's cycle loop, pipe.cc:650-676, run by
+ * compute_worker.cc:30-32) for a caller that holds several batches in device
+ * memory.  *produced (may be NULL) counts the output images written.  Stops
+ * at the first failing push and returns its status. */
+hv_status hv_pipe_push_batches(hv_pipe* p, const void* const* frames, int n, int* produced);
+
 /* Device pointers of the latest output: u16 image, float image after contrast
  * (before the u16 cast), and the float image before contrast. */
 hv_status hv_pipe_output(hv_pipe* p, const uint16_t** out_u16, const float** out_f32_contrast,

@@ -94,6 +94,9 @@ def test_step_functions_reject_bad_arguments_without_gpu(lib):
     assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 16, 256, 0, 3, 16, None, None) == 1
     assert lib.hv_stft_image(ctypes.c_void_p(16), ctypes.c_void_p(16), 300, 256, 0, 3, 5, None, None) == 2
     assert b"pmax" in lib.hv_last_error() or b"range" in lib.hv_last_error() or lib.hv_last_error()
+    # the multi-batch push checks its arguments before touching the pipe
+    assert lib.hv_pipe_push_batches(None, None, 1, None) == 1
+    assert lib.hv_pipe_push_batches(ctypes.c_void_p(16), None, -1, None) == 1
 
 
 def test_memory_accounting(lib):

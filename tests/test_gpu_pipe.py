@@ -428,6 +428,24 @@ def test_pipe_bins_column_tiles_lanes(lanes):
     many.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_pipe_push_batches_matches_push(lanes):
+    """hv_pipe_push_batches runs the cycles of per-batch pushes: the same
+    number of images and the same last image (ring still filling at first)."""
+    from holovibes_amd.pipe import Pipe
+    s, fl = frames("c1", 7, width=64, height=64, batch_size=8, time_stride=8, time_transformation_size=16)
+    dev = [torch.from_numpy(f.view(np.int16)).cuda() for f in fl]
+    torch.cuda.synchronize()
+    one, many = Pipe(s, lanes=lanes), Pipe(s, lanes=lanes)
+    made = sum(bool(one.push(d)) for d in dev)
+    assert made == len(dev) - 1
+    assert many.push_many(Pipe.batch_pointers(dev)) == made
+    assert np.array_equal(one.output_u16(), many.output_u16())
+    assert torch.equal(one.image(), many.image())
+    one.close()
+    many.close()
+
+
 def test_pipe_lanes_need_own_streams():
     from holovibes_amd import _lib
     from holovibes_amd.pipe import Pipe
