@@ -481,9 +481,173 @@ __global__ void __launch_bounds__((BinTimeRealCfg<T, CWT>::THREADS))
     }
 }
 
+// Few bins (K <= HV_BINS_DIRECT_K) in their own pass: each thread sums the
+// window of 8 pixels directly, X_k = sum_n x_n e^{-2 pi i k n / T} over the
+// physical slots n (slot n holding frame (n - slot0) mod T), each frame read
+// once with 8-pixel loads, G frames' loads in flight, no LDS and no barrier
+// (the transform kernels above stage a tile of every pixel's window in LDS
+// and compute all T bins: latency-bound when only a few are kept).  The sums
+// run over x_n - x_0: for k != 0 mod T the constant drops out exactly, so the
+// partial sums stay at the scale of the window's variation, not of its mean
+// (T x_0 is added back to bin 0).  With `ins_src`, the cycle's batch is
+// inserted on the way: slots (ws0 + b) mod T, b < nb, are read from the batch
+// and written to ins_dst (the raw ring being read), so the ring's update and
+// the transform's read are one pass.
+#ifndef HV_BINS_DIRECT_K
+#define HV_BINS_DIRECT_K 4
+#endif
+constexpr int kDirectThreads = 256;
+constexpr int kDirectG = 8;  // frames whose loads are issued together
+
+template <typename TIn>
+struct Px8 {  // 8 pixels as loaded: 8, 16 or 32 bytes
+    uint4 a, b;
+};
+
+template <typename TIn>
+__device__ __forceinline__ void px8_load(const TIn* src, int vec, Px8<TIn>& v) {
+    if (vec) {
+        if constexpr (sizeof(TIn) == 1) {
+            const uint2 w = *reinterpret_cast<const uint2*>(src);
+            v.a = make_uint4(w.x, w.y, 0u, 0u);
+        } else {
+            v.a = *reinterpret_cast<const uint4*>(src);
+            if constexpr (sizeof(TIn) == 4) v.b = *reinterpret_cast<const uint4*>(src + 4);
+        }
+    } else {  // a caller's buffer that is not 16-byte aligned
+        TIn* px = reinterpret_cast<TIn*>(&v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) px[e] = src[e];
+    }
+}
+
+template <typename TIn>
+__device__ __forceinline__ void px8_store(TIn* dst, const Px8<TIn>& v) {  // dst 16-byte aligned (the raw ring)
+    if constexpr (sizeof(TIn) == 1) {
+        *reinterpret_cast<uint2*>(dst) = make_uint2(v.a.x, v.a.y);
+    } else {
+        *reinterpret_cast<uint4*>(dst) = v.a;
+        if constexpr (sizeof(TIn) == 4) *reinterpret_cast<uint4*>(dst + 4) = v.b;
+    }
+}
+
+template <int T, typename TIn, int KT>
+__global__ void __launch_bounds__(kDirectThreads)
+    k_bins_time_direct(const TIn* raw, int slot0, TIn* ins_dst, const TIn* __restrict__ ins_src, int nb, int ws0,
+                       float2* __restrict__ planes, const float2* __restrict__ chirp, size_t P, int pmin, int K,
+                       uint32_t* mm_reset, int mm_slots, int vec) {
+    if (mm_reset && blockIdx.x == 0) mm_reset_slots(mm_reset, mm_slots);
+    const size_t p = ((size_t)blockIdx.x * kDirectThreads + threadIdx.x) * 8;
+    if (p >= P) return;  // P is a multiple of 8: a thread's 8 pixels exist together
+    constexpr int LG = clog2(T);
+    constexpr int G = T < kDirectG ? T : kDirectG;
+    float2 acc[KT][8];
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[k][e] = make_float2(0.f, 0.f);
+    float x0[8];
+    for (int n0 = 0; n0 < T; n0 += G) {
+        Px8<TIn> v[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int n = n0 + g;
+            const int r = (n - ws0) & (T - 1);
+            const bool ins = ins_src && r < nb;  // uniform
+            const TIn* src = ins ? ins_src + (size_t)r * P + p : raw + (size_t)((n - slot0) & (T - 1)) * P + p;
+            px8_load<TIn>(src, ins ? 1 : vec, v[g]);
+        }
+        if (ins_src) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                if (((n0 + g - ws0) & (T - 1)) < nb) px8_store<TIn>(ins_dst + (size_t)(n0 + g) * P + p, v[g]);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const TIn* px = reinterpret_cast<const TIn*>(&v[g]);
+            float x[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[e] = convert_px<TIn>(px[e]);
+            if (n0 == 0 && g == 0) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x0[e] = x[e];
+            }
+            const int n = n0 + g;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                if (k >= K) break;
+                const float2 w = twiddle<-1>(((pmin + k) * n) & (T - 1), LG);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float d = x[e] - x0[e];
+                    acc[k][e].x = fmaf(d, w.x, acc[k][e].x);
+                    acc[k][e].y = fmaf(d, w.y, acc[k][e].y);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        if (k >= K) break;
+        const bool dc = ((pmin + k) & (T - 1)) == 0;
+        float2 out[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float2 z = acc[k][e];
+            if (dc) z.x += (float)T * x0[e];
+            out[e] = chirp ? cmul(z, chirp[p + e]) : z;
+        }
+        float4* dst = reinterpret_cast<float4*>(planes + (size_t)k * P + p);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dst[e] = make_float4(out[2 * e].x, out[2 * e].y, out[2 * e + 1].x, out[2 * e + 1].y);
+    }
+}
+
+// The cycle's batch into the raw ring by copies (the kernels that stage the
+// window in LDS read the ring after it): at most two contiguous runs.
+hv_status raw_insert_copy(const RawInsert& ins, size_t frame_bytes, int T, hipStream_t st) {
+    if (!ins.batch) return HV_OK;
+    if (ins.n < 1 || ins.n >= T || ins.slot0 < 0 || ins.slot0 >= T)
+        return fail(HV_ERR_INVALID_ARG, "raw ring insert: 0 < n < T and 0 <= slot0 < T");
+    const int n1 = std::min(ins.n, T - ins.slot0);
+    HV_HIP(hipMemcpyAsync((char*)ins.ring + (size_t)ins.slot0 * frame_bytes, ins.batch, (size_t)n1 * frame_bytes,
+                          hipMemcpyDeviceToDevice, st));
+    if (ins.n > n1)
+        HV_HIP(hipMemcpyAsync(ins.ring, (const char*)ins.batch + (size_t)n1 * frame_bytes,
+                              (size_t)(ins.n - n1) * frame_bytes, hipMemcpyDeviceToDevice, st));
+    return HV_OK;
+}
+
+template <int T, typename TIn, int KT>
+hv_status time_direct_t(const void* raw, float2* planes, const float2* chirp, size_t P, int slot0, int pmin, int K,
+                        uint32_t* mm_reset, int mm_slots, hipStream_t st, const RawInsert& ins) {
+    if (P % 8) return fail(HV_ERR_UNSUPPORTED, "bins-first time DFT: frames of a multiple of 8 pixels");
+    if (ins.batch) {
+        if (ins.ring != raw || slot0 != 0 || ins.n < 1 || ins.n >= T || ins.slot0 < 0 || ins.slot0 >= T)
+            return fail(HV_ERR_INVALID_ARG, "raw ring insert: the ring read in slot order, 0 < n < T");
+        if ((reinterpret_cast<uintptr_t>(ins.ring) & 15) || (reinterpret_cast<uintptr_t>(ins.batch) & 15)) {
+            HV_TRY(raw_insert_copy(ins, P * sizeof(TIn), T, st));
+            return time_direct_t<T, TIn, KT>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, RawInsert{});
+        }
+    }
+    const int vec = (reinterpret_cast<uintptr_t>(raw) & 15) == 0;
+    const unsigned g = (unsigned)((P / 8 + kDirectThreads - 1) / kDirectThreads);
+    k_bins_time_direct<T, TIn, KT><<<g, kDirectThreads, 0, st>>>(
+        (const TIn*)raw, slot0, (TIn*)ins.ring, (const TIn*)ins.batch, ins.n, ins.slot0, planes, chirp, P, pmin, K,
+        mm_reset, mm_slots, vec);
+    HV_CHECK_LAUNCH("k_bins_time_direct");
+    return HV_OK;
+}
+
 template <int T, typename TIn>
 hv_status time_launch_t(const void* raw, float2* planes, const float2* chirp, size_t P, int slot0, int pmin, int K,
-                        uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+                        uint32_t* mm_reset, int mm_slots, hipStream_t st, const RawInsert& ins) {
+    if (K <= HV_BINS_DIRECT_K) {
+        if (K == 1) return time_direct_t<T, TIn, 1>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        if (K == 2) return time_direct_t<T, TIn, 2>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        return time_direct_t<T, TIn, 4>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+    }
+    HV_TRY(raw_insert_copy(ins, P * sizeof(TIn), T, st));
     if constexpr (T >= 32 && HV_BINS_TIME_REAL) {
         constexpr int CWB = bins_time_cw<T>();
         static_assert(CWB >= 64 && BinTimeRealCfg<T, CWB>::THREADS <= 1024, "time DFT tile");
@@ -511,25 +675,25 @@ hv_status time_launch_t(const void* raw, float2* planes, const float2* chirp, si
 
 template <int T>
 hv_status time_launch_d(const void* raw, int depth, float2* planes, const float2* chirp, size_t P, int slot0,
-                        int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+                        int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st, const RawInsert& ins) {
     switch (depth) {
-        case HV_DEPTH_8: return time_launch_t<T, uint8_t>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case HV_DEPTH_16: return time_launch_t<T, uint16_t>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        default: return time_launch_t<T, float>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case HV_DEPTH_8: return time_launch_t<T, uint8_t>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case HV_DEPTH_16: return time_launch_t<T, uint16_t>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        default: return time_launch_t<T, float>(raw, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
     }
 }
 
 hv_status bins_time(const void* raw, int depth, float2* planes, const float2* chirp, size_t P, int T, int slot0,
-                    int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st) {
+                    int pmin, int K, uint32_t* mm_reset, int mm_slots, hipStream_t st, const RawInsert& ins) {
     switch (T) {
-        case 2: return time_launch_d<2>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 4: return time_launch_d<4>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 8: return time_launch_d<8>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 16: return time_launch_d<16>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 32: return time_launch_d<32>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 64: return time_launch_d<64>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 128: return time_launch_d<128>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
-        case 256: return time_launch_d<256>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st);
+        case 2: return time_launch_d<2>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 4: return time_launch_d<4>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 8: return time_launch_d<8>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 16: return time_launch_d<16>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 32: return time_launch_d<32>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 64: return time_launch_d<64>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 128: return time_launch_d<128>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
+        case 256: return time_launch_d<256>(raw, depth, planes, chirp, P, slot0, pmin, K, mm_reset, mm_slots, st, ins);
         default: return fail(HV_ERR_UNSUPPORTED, "bins-first path: T must be a power of two in [2, 256]");
     }
 }
@@ -926,7 +1090,7 @@ bool bins_split_supported(int width, int height, int T, int K, int depth) {
 
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
-                           int mm_slots, const float2* wts, float2* tiles) {
+                           int mm_slots, const float2* wts, float2* tiles, RawInsert ins) {
     if (space != HV_SPACE_ANGULARSP && space != HV_SPACE_FRESNELTR)
         return fail(HV_ERR_UNSUPPORTED, "bins-first path needs a space transform");
     const float2* chirp = space == HV_SPACE_FRESNELTR ? lens : nullptr;
@@ -935,12 +1099,17 @@ hv_status launch_bins_rows(const void* raw, int depth, int width, int height, in
             return fail(HV_ERR_UNSUPPORTED, "bins-first path: unsupported shape");
         // split: the window's bins (or PCA components), then their forward row FFTs
         const size_t P = (size_t)width * height;
-        if (wts) HV_TRY(pca_time(raw, depth, planes, chirp, P, T, slot0, K, wts, mm_reset, mm_slots, st));
-        else HV_TRY(bins_time(raw, depth, planes, chirp, P, T, slot0, pmin, K, mm_reset, mm_slots, st));
+        if (wts) {
+            HV_TRY(raw_insert_copy(ins, P * depth, T, st));
+            HV_TRY(pca_time(raw, depth, planes, chirp, P, T, slot0, K, wts, mm_reset, mm_slots, st));
+        } else {
+            HV_TRY(bins_time(raw, depth, planes, chirp, P, T, slot0, pmin, K, mm_reset, mm_slots, st, ins));
+        }
         // column-tiled planes: the forward row pass writes the tiles the column pass reads
         if (tiles) return launch_row_pass_tiled(planes, tiles, width, height, K, -1, st);
         return launch_row_pass(planes, width, height, K, -1, st);
     }
+    HV_TRY(raw_insert_copy(ins, (size_t)width * height * depth, T, st));
     switch (width) {
         case 64: return rows_w<64>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
         case 128: return rows_w<128>(raw, depth, planes, chirp, height, T, slot0, pmin, K, mm_reset, mm_slots, wts, st);
@@ -960,9 +1129,9 @@ hv_status launch_bins_columns(float2* planes, int width, int height, int K, int 
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
                                 hipEvent_t mark, uint32_t* mm_reset, int mm_slots, const float2* wts, float2* tiles,
-                                const float2* lens_t) {
+                                const float2* lens_t, RawInsert ins) {
     HV_TRY(launch_bins_rows(raw, depth, width, height, T, slot0, pmin, K, space, lens, planes, st, mm_reset,
-                            mm_slots, wts, tiles));
+                            mm_slots, wts, tiles, ins));
     if (mark) HV_HIP(hipEventRecord(mark, st));
     return launch_bins_columns(planes, width, height, K, space, lens, st, tiles, lens_t);
 }

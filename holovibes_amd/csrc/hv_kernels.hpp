@@ -85,6 +85,16 @@ hv_status launch_bins_moments(float* m0, float* m1, float* m2, const float2* pla
 // between the forward row pass and the inverse row pass, `lens_t` the lens in
 // the same layout; the inverse row pass writes row-major planes back.
 bool bins_tiled_supported(int width, int height, int T, int K, int depth, int space, bool pca = false);
+// The cycle's batch on its way into a sliding window's raw ring: `n` frames
+// of `batch` (0 < n < T) go to slots (slot0 + b) mod T of `ring`, the window
+// the bins read (in slot order); the few-bin time DFT writes them as it reads
+// the window, the other kernels after copies.  Empty (batch null): nothing.
+struct RawInsert {
+    void* ring = nullptr;
+    const void* batch = nullptr;
+    int n = 0;
+    int slot0 = 0;
+};
 // Bins + row FFTs (x chirp for Fresnel) into planes[0 .. K) (K x P complex),
 // then the column pass over them; launch_bins_propagate runs both, with
 // `mark` (optional) recorded in between.
@@ -93,13 +103,14 @@ bool bins_tiled_supported(int width, int height, int T, int K, int depth, int sp
 // column pmin, readable 8 columns past the last) instead of DFT bin pmin + k.
 hv_status launch_bins_rows(const void* raw, int depth, int width, int height, int T, int slot0, int pmin, int K,
                            int space, const float2* lens, float2* planes, hipStream_t st, uint32_t* mm_reset,
-                           int mm_slots = 1, const float2* wts = nullptr, float2* tiles = nullptr);
+                           int mm_slots = 1, const float2* wts = nullptr, float2* tiles = nullptr,
+                           RawInsert ins = {});
 hv_status launch_bins_columns(float2* planes, int width, int height, int K, int space, const float2* lens,
                               hipStream_t st, float2* tiles = nullptr, const float2* lens_t = nullptr);
 hv_status launch_bins_propagate(const void* raw, int depth, int width, int height, int T, int slot0, int pmin,
                                 int K, int space, const float2* lens, float2* planes, hipStream_t st,
                                 hipEvent_t mark, uint32_t* mm_reset, int mm_slots = 1, const float2* wts = nullptr,
-                                float2* tiles = nullptr, const float2* lens_t = nullptr);
+                                float2* tiles = nullptr, const float2* lens_t = nullptr, RawInsert ins = {});
 // Image of the propagated planes: inverse row FFT (angular spectrum) + image
 // term, mean over [pmin, pmax], x scale (131071 for PCA images); mm: encoded
 // min/max (may be null).

@@ -62,15 +62,20 @@ void bins_range(const hv_compute_settings& s, int& lo, int& K) {
     }
 }
 
-// B = S = T STFT or PCA image or moments with a space transform: the needed
+// B = S = T (or B < T, from a raw ring) STFT or PCA image or moments with a space transform: the needed
 // bins (STFT) or components (PCA) can be taken before the propagation (both
 // are linear combinations of frames and commute with the linear propagator;
 // the PCA covariance of the propagated window is the raw window's up to a
 // constant, the propagators being unitary up to scale).
+// B < T (a sliding window): the raw frames of the window are kept in a raw
+// ring (slot of frame n = n mod T, as the reference's ring) and the bins are
+// taken from it each cycle; the propagation stays per bin.
+bool raw_window(const hv_compute_settings& s) { return s.batch_size < s.time_transformation_size; }
+
 bool bins_eligible(const hv_compute_settings& s) {
     if (!(s.space_transformation == HV_SPACE_ANGULARSP || s.space_transformation == HV_SPACE_FRESNELTR)) return false;
     if (s.time_transformation != HV_TIME_STFT && s.time_transformation != HV_TIME_PCA) return false;
-    if (!window_replaced(s)) return false;
+    if (!window_replaced(s) && !raw_window(s)) return false;
     if (!is_image_kind(s.img_type) && !is_moment_kind(s.img_type)) return false;
     int lo, K;
     bins_range(s, lo, K);
@@ -102,7 +107,7 @@ int choose_path(const hv_compute_settings& s, int level) {
 }
 
 struct Layout {
-    size_t ring, lens, lens_t, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, evecs32, pca_ws,
+    size_t ring, raw, lens, lens_t, staging, image, cimage, tmp, out, small, mm, pct, unwrap, cov, evals, evecs, evecs32, pca_ws,
         eig_ws;
     size_t accu, renorm, cuts, reg, outq;
     size_t total;
@@ -113,6 +118,8 @@ Layout layout_of(const hv_compute_settings& s) {
     const size_t P = (size_t)s.width * s.height;
     const size_t T = s.time_transformation_size;
     l.ring = align256(T * P * sizeof(float2));
+    // the raw frames of a sliding window (bins-first with B < T)
+    l.raw = raw_window(s) && choose_path(s, PATH_BINS) == PATH_BINS ? align256(T * P * s.depth) : 0;
     l.lens = s.space_transformation != HV_SPACE_NONE ? align256(P * sizeof(float2)) : 0;
     l.lens_t = bins_tiled(s) ? l.lens : 0;  // the lens again, column-tiled
     l.staging = align256((size_t)s.batch_size * P * s.depth);
@@ -145,7 +152,7 @@ Layout layout_of(const hv_compute_settings& s) {
     l.reg = s.registration_enabled ? align256(registration_scratch_bytes(P)) + align256(P * sizeof(float)) : 0;
     // the output queue: OutputBufferSize u16 images
     l.outq = align256((size_t)s.output_buffer_size * P * sizeof(uint16_t));
-    l.total = l.outq + l.ring + l.lens + l.lens_t + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
+    l.total = l.outq + l.ring + l.raw + l.lens + l.lens_t + l.staging + l.image + l.cimage + l.tmp + l.out + l.small + l.mm + l.pct + l.unwrap + l.cov +
               l.evals + l.evecs + l.evecs32 + l.pca_ws + l.eig_ws + l.accu + l.renorm + l.cuts + l.reg;
     return l;
 }
@@ -212,6 +219,7 @@ struct hv_pipe {
     Layout lay{};
     char* base = nullptr;
     float2* ring = nullptr;
+    void* raw = nullptr;            // sliding-window bins-first: the window's raw frames, slot n = frame n mod T
     float2* lens = nullptr;
     float2* lens_t = nullptr;  // the lens column-tiled (bins_tiled)
     void* staging = nullptr;
@@ -264,6 +272,14 @@ struct hv_pipe {
 
     hv_status image_work();
     hv_status enqueue_cycle(const void* frames, int slot0);
+    // The propagated ring of the window the raw ring holds, for path `to`
+    // (the sliding-window bins-first path keeps only the raw frames); the
+    // pipe's streams are idle.
+    hv_status complex_ring_from_raw(int to) {
+        if (!raw_ring()) return HV_OK;
+        return launch_propagate_to_ring(raw, (int)s.depth, T, (int)s.width, (int)s.height, s.space_transformation,
+                                        lens, ring, T, 0, lane[0].stream, nullptr, to == PATH_ROW_STFT);
+    }
     // where lane L's cycle writes its u16 image
     OutSlot out_slot(const Lane& L) const {
         OutSlot o;
@@ -302,7 +318,24 @@ struct hv_pipe {
         }
         return HV_OK;
     }
-    bool shared_ring() const { return nlanes > 1 && lane[1].kind == LANE_SHARED_RING; }
+    // the bins-first path reads a sliding window (B < T) from the raw ring
+    bool raw_ring() const { return raw && path == PATH_BINS; }
+    // the raw ring follows the window while the bins-first path runs or may
+    // resume (a moments record holds the three-pass path meanwhile)
+    bool raw_keep() const { return raw && (path == PATH_BINS || (rec && level >= PATH_BINS)); }
+    // cycles on several lanes write and read one window: the three-pass ring or the raw ring
+    bool shared_ring() const { return nlanes > 1 && (lane[1].kind == LANE_SHARED_RING || raw_ring()); }
+    // Queue::enqueue_multiple of raw frames (queue.cc:332-376): batch frame b
+    // to slot (slot0 + b) mod T, B < T, so at most two contiguous runs
+    hv_status raw_store(const void* frames, int slot0, hipStream_t st) {
+        const size_t fb = P * s.depth;
+        const int n1 = std::min(B, T - slot0);
+        HV_HIP(hipMemcpyAsync((char*)raw + (size_t)slot0 * fb, frames, (size_t)n1 * fb, hipMemcpyDeviceToDevice, st));
+        if (B > n1)
+            HV_HIP(hipMemcpyAsync(raw, (const char*)frames + (size_t)n1 * fb, (size_t)(B - n1) * fb,
+                                  hipMemcpyDeviceToDevice, st));
+        return HV_OK;
+    }
     // min/max folded into the image kernel, accumulators re-armed by the
     // cycle's first kernel (the row pass of a space transform)
     bool minmax_fused() const {
@@ -376,7 +409,8 @@ hv_status hv_pipe::image_work() {
             HV_TRY(launch_complex_to_image(L.image, L.ring, it, pmin, pmax, P, L.stream));
             if (mm_img) HV_TRY(launch_minmax(L.mm, L.image, P, L.stream, kPipeMm));
         }
-        if (shared_ring()) HV_HIP(hipEventRecord(L.tt_done, L.stream));  // the ring has been read
+        // the ring has been read (the bins-first path records it after its time DFT)
+        if (shared_ring() && path != PATH_BINS) HV_HIP(hipEventRecord(L.tt_done, L.stream));
         if (unwrap) {
             // the unwrap's last pass leaves its output's min/max in mm for the rescale
             HV_TRY(launch_unwrap_2d(L.tmp, L.image, s.width, s.height, L.unwrap_ws, L.stream, L.mm, kPipeMm));
@@ -496,20 +530,38 @@ hv_status hv_pipe::enqueue_cycle(const void* frames, int slot0) {
     // transform reads (and reads the slots it keeps)
     if (shared_ring() && last != cur && have_output) HV_HIP(hipStreamWaitEvent(L.stream, lane[last].tt_done, 0));
     HV_TRY(mark(0));
+    // STFT bins of a sliding window: the batch enters the raw ring within the
+    // bins launch (the few-bin transform writes it as it reads the window)
+    const bool insert = raw_ring() && s.time_transformation == HV_TIME_STFT;
+    if (raw_keep() && !insert) HV_TRY(raw_store(frames, slot0, L.stream));
     if (path == PATH_BINS) {
         int lo, K;
         bins_range(s, lo, K);
+        // B = T: the batch is the window (frame j in slot (slot0 + j) mod T);
+        // B < T: the raw ring, in slot order
+        const void* win = raw_ring() ? raw : frames;
+        const int w0 = raw_ring() ? 0 : slot0;
         const float2* wts = nullptr;
         if (s.time_transformation == HV_TIME_PCA) {
             // covariance of the raw window, FP64 eigensolver, then components lo .. lo + K - 1 projected
-            HV_TRY(launch_pca_cov_raw(L.cov, frames, (int)s.depth, T, P, slot0, s.pca_precision, L.pca_ws, L.stream));
+            HV_TRY(launch_pca_cov_raw(L.cov, win, (int)s.depth, T, P, w0, s.pca_precision, L.pca_ws, L.stream));
             // the raw window's covariance is real symmetric: real rotations (T <= 128)
             HV_TRY(launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, T, L.eig_ws, L.stream, L.evecs32, T <= 128));
             wts = L.evecs32 + lo;
         }
-        HV_TRY(launch_bins_propagate(frames, (int)s.depth, (int)s.width, (int)s.height, T, slot0, lo, K,
-                                     s.space_transformation, lens, L.planes, L.stream, timing ? ev[1] : nullptr,
-                                     minmax_fused() ? L.mm : nullptr, kPipeMm, wts, L.tiles, lens_t));
+        // the event after the time DFT: the stage timer's, or the one the
+        // next lane's raw-ring write waits for
+        RawInsert ins;
+        if (insert) {
+            ins.ring = raw;
+            ins.batch = frames;
+            ins.n = B;
+            ins.slot0 = slot0;
+        }
+        HV_TRY(launch_bins_propagate(win, (int)s.depth, (int)s.width, (int)s.height, T, w0, lo, K,
+                                     s.space_transformation, lens, L.planes, L.stream,
+                                     timing ? ev[1] : shared_ring() ? L.tt_done : nullptr,
+                                     minmax_fused() ? L.mm : nullptr, kPipeMm, wts, L.tiles, lens_t, ins));
         HV_TRY(mark(2));
     } else {
         HV_TRY(launch_propagate_to_ring(frames, (int)s.depth, B, (int)s.width, (int)s.height,
@@ -677,6 +729,7 @@ hv_status hv_pipe_create(const hv_compute_settings* s, void* stream, hv_pipe** o
     };
     const Layout& l = p->lay;
     p->ring = (float2*)take(l.ring);
+    p->raw = take(l.raw);
     p->lens = (float2*)take(l.lens);
     p->lens_t = (float2*)take(l.lens_t);
     p->staging = take(l.staging);
@@ -795,6 +848,8 @@ hv_status hv_pipe_push_batch(hv_pipe* p, const void* frames, int* produced) {
     // insert_wait_time_transformation_size (pipe.cc:418-427)
     const bool full = p->q_size >= maxq;
     if (!full) {
+        if (p->raw_keep()) HV_TRY(p->raw_store(frames, slot0, p->lane[0].stream));
+        if (p->raw_ring()) return HV_OK;
         return launch_propagate_to_ring(frames, (int)p->s.depth, p->B, (int)p->s.width, (int)p->s.height,
                                         p->s.space_transformation, p->lens, p->ring, p->T, slot0,
                                         p->lane[0].stream);
@@ -1016,7 +1071,9 @@ hv_status hv_pipe_set_record_moments(hv_pipe* p, int enabled) {
         return fail(HV_ERR_UNSUPPORTED, "moments are recorded from the STFT or PCA output");
     if (p->nlanes > 1) return fail(HV_ERR_UNSUPPORTED, "the moments record runs one cycle at a time");
     if (!p->rec) HV_HIP(hipMalloc((void**)&p->rec, 3 * p->P * sizeof(float)));
-    // the moments read the whole time window, which only the three-pass path keeps
+    // the moments read the whole time window, which only the three-pass path
+    // keeps; a sliding window's ring is rebuilt from its raw frames
+    HV_TRY(p->complex_ring_from_raw(PATH_THREE_PASS));
     p->path = PATH_THREE_PASS;
     p->drop_graphs();
     p->next = 0;
@@ -1047,9 +1104,15 @@ hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t*
 hv_status hv_pipe_set_fused(hv_pipe* p, int level) {
     if (!p) return fail(HV_ERR_INVALID_ARG, "pipe is null");
     if (level < 0) return fail(HV_ERR_INVALID_ARG, "fusion level must be >= 0");
-    p->level = level;
-    p->path = p->rec ? PATH_THREE_PASS : choose_path(p->s, level);
     HV_TRY(hv_pipe_sync(p));
+    const bool was_raw = p->raw_ring();
+    const int path = p->rec ? PATH_THREE_PASS : choose_path(p->s, level);
+    // leaving the sliding-window bins-first path: its raw window gives the
+    // ring; entering it: the raw ring did not follow the window, which refills
+    if (path != PATH_BINS) HV_TRY(p->complex_ring_from_raw(path));
+    p->level = level;
+    p->path = path;
+    if (!was_raw && p->raw_ring()) p->q_start = p->q_size = 0;
     p->drop_graphs();
     p->nlanes = 1;  // extra lanes hold one path's buffers: hv_pipe_set_lanes again after a path change
     p->next = 0;
@@ -1162,11 +1225,13 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes) {
 hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, int n_frames, int iters, float* ms) {
     if (!p || !ms || iters < 1) return fail(HV_ERR_INVALID_ARG, "null argument or iters < 1");
     if (p->path != PATH_BINS) return fail(HV_ERR_UNSUPPORTED, "stage timing covers the bins-first path");
-    if (stage == 0 && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 0 needs input frames");
+    // a sliding window (B < T) is read from the raw ring, a replaced one from the input batches
+    const bool raw = p->raw_ring();
+    if (stage == 0 && !raw && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 0 needs input frames");
     const bool pca = p->s.time_transformation == HV_TIME_PCA;
     if (stage < 0 || stage > 4 || (stage == 2 && !pca))
         return fail(HV_ERR_INVALID_ARG, "stage must be 0, 1, 3 or 4 (2: PCA covariance + eigensolver)");
-    if (stage == 2 && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 2 needs input frames");
+    if (stage == 2 && !raw && (!frames || n_frames < 1)) return fail(HV_ERR_INVALID_ARG, "stage 2 needs input frames");
     HV_TRY(hv_pipe_sync(p));
     Lane& L = p->lane[0];
     const hv_compute_settings& s = p->s;
@@ -1181,12 +1246,12 @@ hv_status hv_pipe_time_stage(hv_pipe* p, int stage, const void* const* frames, i
     for (int i = 0; i < iters && st == HV_OK && e == hipSuccess; ++i) {
         switch (stage) {
             case 0:
-                st = launch_bins_rows(frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
+                st = launch_bins_rows(raw ? p->raw : frames[i % n_frames], (int)s.depth, (int)s.width, (int)s.height, p->T, 0, pmin,
                                       K, s.space_transformation, p->lens, L.planes, L.stream, L.mm, kPipeMm,
                                       pca ? L.evecs32 + pmin : nullptr, L.tiles);
                 break;
             case 2:
-                st = launch_pca_cov_raw(L.cov, frames[i % n_frames], (int)s.depth, p->T, p->P, 0, s.pca_precision,
+                st = launch_pca_cov_raw(L.cov, raw ? p->raw : frames[i % n_frames], (int)s.depth, p->T, p->P, 0, s.pca_precision,
                                         L.pca_ws, L.stream);
                 if (st == HV_OK)
                     st = launch_eig_jacobi(L.evals, L.evecs, L.cov, 1, p->T, L.eig_ws, L.stream, L.evecs32,

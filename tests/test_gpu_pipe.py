@@ -130,6 +130,51 @@ def test_pipe_stride_discard(oracle):
     check(res, 1e-5)
 
 
+@pytest.mark.parametrize("name,over,tol", [
+    # B < T on the bins-first path: the raw frames of the window in their own ring
+    ("c1", dict(width=64, height=64, batch_size=4, time_stride=4, time_transformation_size=16), 1e-5),
+    # 3 does not divide 8: batches straddle the ring's end (two runs per write)
+    ("c1", dict(width=64, height=64, batch_size=3, time_stride=3, time_transformation_size=8, p_start=1,
+                p_width=4), 1e-5),
+    # S = 2 B: every other batch dropped before the ring
+    ("c1", dict(width=64, height=64, batch_size=4, time_stride=8, time_transformation_size=16), 1e-5),
+    ("c1", dict(width=64, height=32, batch_size=8, time_stride=8, time_transformation_size=32, depth=1,
+                space_transformation=1, z_distance=0.4, syn_depth=1, syn_bits=8, syn_mean_counts=64.0), 1e-5),
+    ("c5_stft", dict(width=64, height=64, batch_size=16, time_stride=16, time_transformation_size=64, p_start=4,
+                     p_width=40, camera_fps=64.0, syn_period=64), 1e-5),
+    # split path: W = 2048 (time DFT, row, column and epilogue kernels), then column tiles (H = 2048)
+    ("c1", dict(width=2048, height=16, batch_size=8, time_stride=8, time_transformation_size=32, p_start=2,
+                p_width=9), 1e-5),
+    ("c5_stft", dict(width=64, height=2048, batch_size=32, time_stride=32, time_transformation_size=128,
+                     p_start=4, p_width=56, camera_fps=128.0, syn_period=128), 1e-5),
+    # PCA of a sliding window: covariance, eigensolver and projection from the raw ring
+    ("c2", dict(width=64, height=64, batch_size=8, time_stride=8, time_transformation_size=32), 1e-4),
+    # K <= 4 on the split path: the direct few-bin transform inserts the batch into the raw ring as it reads
+    ("c1", dict(width=2048, height=16, batch_size=8, time_stride=8, time_transformation_size=32, p_start=2,
+                p_width=1), 1e-5),
+    ("c1", dict(width=2048, height=16, batch_size=24, time_stride=24, time_transformation_size=64, p_start=0,
+                p_width=3, as_f32=True), 1e-5),  # bin 0 (the window's first frame added back), float frames
+    ("c1", dict(width=2048, height=16, batch_size=5, time_stride=5, time_transformation_size=16, p_start=3,
+                p_width=0, depth=1, syn_depth=1, syn_bits=8, syn_mean_counts=64.0), 1e-5),
+])
+def test_pipe_sliding_window_bins(oracle, name, over, tol):
+    """Sliding windows (B < T) on the bins-first path, batch by batch
+    against the oracle while the window fills and rotates."""
+    over = dict(over)
+    as_f32 = over.pop("as_f32", False)
+    T, B = over["time_transformation_size"], over["batch_size"]
+    # five images: the window fills after ceil(T / B) processed batches, one batch in S / B processed
+    n = (-(-T // B) + 4) * (over["time_stride"] // B)
+    if as_f32:
+        over["depth"] = 4
+    s, fl = frames(name, n, **over)
+    if as_f32:
+        fl = [f.astype(np.float32) * 0.5 for f in fl]
+    res = run_both(oracle, s, fl, path=2)
+    assert len(res) >= 5
+    check(res, tol)
+
+
 def test_pipe_oversized_batch(oracle):
     s, fl = frames("c1", 2, width=32, height=32, batch_size=8, time_stride=8, time_transformation_size=4,
                    p_start=1, p_width=2)
@@ -419,6 +464,67 @@ def test_pipe_lanes_other_paths_match_one_lane(case, lanes):
     assert np.array_equal(one.contrast_range(), many.contrast_range())
     one.close()
     many.close()
+
+
+@pytest.mark.parametrize("case", ["stft", "unwrap", "pca"])
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_pipe_sliding_bins_lanes_match_one_lane(case, lanes):
+    """Sliding windows on the bins-first path share the raw ring across the
+    lanes (each cycle's write waits until the previous cycle's time DFT has
+    read it): the lanes' outputs are bit-identical to one lane's."""
+    from holovibes_amd.pipe import Pipe
+    name, over = {"stft": ("c1", dict(width=128, height=64, batch_size=4, time_stride=4,
+                                      time_transformation_size=16)),
+                  "unwrap": ("c3", dict(width=128, height=128, batch_size=8, time_stride=8,
+                                        time_transformation_size=16, syn_period=16)),
+                  "pca": ("c2", dict(width=128, height=128, batch_size=8, time_stride=8,
+                                     time_transformation_size=32))}[case]
+    s, fl = frames(name, 8, **over)
+    conv = (lambda f: torch.from_numpy(f.view(np.int16) if f.dtype == np.uint16 else f).cuda())
+    dev = [conv(f) for f in fl]
+    torch.cuda.synchronize()
+    one, many = Pipe(s, graph=True), Pipe(s, graph=True, lanes=lanes)
+    assert one.path == many.path == Pipe.BINS
+    for d in dev:
+        a, b = one.push(d), many.push(d)
+        assert a == b
+        if a:
+            assert np.array_equal(one.output_u16(), many.output_u16())
+            assert torch.equal(one.image(), many.image())
+    for d in dev + dev[::-1]:
+        one.push(d)
+        many.push(d)
+    assert np.array_equal(one.output_u16(), many.output_u16())
+    assert torch.equal(one.image(), many.image())
+    one.close()
+    many.close()
+
+
+def test_pipe_sliding_bins_path_switch(oracle):
+    """Leaving the sliding-window bins-first path mid-stream (hv_pipe_set_fused
+    0): the three-pass ring is rebuilt from the raw window and the next
+    cycles match the oracle; coming back, the window refills first."""
+    from holovibes_amd.pipe import Pipe
+    from holovibes_amd import _lib
+    s, fl = frames("c1", 10, width=64, height=64, batch_size=4, time_stride=4, time_transformation_size=16)
+    p = Pipe(s)
+    o = oracle.OraclePipe(s, 64)
+    assert p.path == Pipe.BINS
+    dev = [torch.from_numpy(f.view(np.int16)).cuda() for f in fl]
+    torch.cuda.synchronize()
+    for k in range(6):
+        assert p.push(dev[k]) == (o.push(fl[k]) is not None)
+    _lib.call("hv_pipe_set_fused", p.h, 0)
+    assert p.path == Pipe.THREE_PASS
+    for k in range(6, 8):
+        ref = o.push(fl[k])
+        assert p.push(dev[k]) and ref is not None
+        assert rel_l2(p.image().cpu().numpy(), ref["image"]) < 1e-5
+    _lib.call("hv_pipe_set_fused", p.h, 2)
+    assert p.path == Pipe.BINS
+    # the window refills: T / B - 1 pushes without an image
+    assert not p.push(dev[8]) and not p.push(dev[9])
+    p.close()
 
 
 @pytest.mark.parametrize("lanes", [2, 3])

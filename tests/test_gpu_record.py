@@ -155,6 +155,59 @@ def test_moments_record_close_restores_graph_pipe(tmp_path):
     eager.close()
 
 
+def test_moments_record_sliding_window_bins(tmp_path):
+    """A sliding window (B < T) on the bins-first path holds the raw frames
+    only: a moments record started mid-stream rebuilds the three-pass ring
+    from them (the recorded moments equal a three-pass pipe's), the raw ring
+    follows the window meanwhile, and after the record the bins-first
+    cycles equal those of a pipe that never left the path."""
+    from holovibes_amd.configs import settings_for
+    from holovibes_amd.constants import HV_IMG_MOMENTS_0
+    from holovibes_amd.pipe import Pipe
+    from holovibes_amd.record import Recorder
+    s = settings_for("c1", width=128, height=128, batch_size=8, time_stride=8, time_transformation_size=16,
+                     camera_fps=64.0, p_start=2, p_width=5)
+    B, n = s.batch_size, 8
+    frames = _frames("c1", n * B, width=128, height=128)
+    batches = [_dev(frames[b * B:(b + 1) * B]) for b in range(n)]
+    torch.cuda.synchronize()
+    # the moments of cycles 1..3 (batches 2..4) from three-pass pipes
+    expect = []
+    for k in range(3):
+        s2 = type(s).from_buffer_copy(s)
+        s2.img_type = HV_IMG_MOMENTS_0 + k
+        p = Pipe(s2, fused=0)
+        imgs = [p.image().cpu().numpy() for b in batches[:5] if p.push(b)]
+        p.close()
+        assert len(imgs) == 4
+        expect.append(imgs[1:])
+    graph, eager = Pipe(s, graph=True), Pipe(s)
+    assert graph.path == Pipe.BINS
+    for b in batches[:2]:
+        graph.push(b)
+        eager.push(b)
+    rec = Recorder(str(tmp_path / "m.holo"), graph, "moments", 3 * 3)
+    assert graph.path == Pipe.THREE_PASS
+    for b in batches[2:5]:
+        made = graph.push(b)
+        assert made
+        rec.after_push(b, made)
+        eager.push(b)
+    assert rec.close(input_fps=100) == 9
+    assert graph.path == Pipe.BINS
+    for b in batches[5:]:
+        assert graph.push(b) and eager.push(b)
+        assert np.array_equal(graph.output_u16(), eager.output_u16())
+        assert torch.equal(graph.image(), eager.image())
+    graph.close()
+    eager.close()
+    p = hf.parse(open(str(tmp_path / "m.holo"), "rb").read())
+    got = np.frombuffer(p["frames"], np.float32).reshape(3, 3, s.height, s.width)
+    for c in range(3):
+        for k in range(3):
+            assert np.array_equal(got[c, k], expect[k][c]), (c, k)
+
+
 def test_recorder_refuses_lanes(tmp_path):
     """The record copies a cycle's output on the pipe's first stream, so it
     needs the cycles in order on one lane."""
