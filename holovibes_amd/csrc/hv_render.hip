@@ -125,10 +125,14 @@ __global__ void __launch_bounds__(256) k_apply_contrast(float* img, size_t n, fl
 
 // The image's destination: `out`, or the output queue slot pos[0].  Every
 // block reads the slot before it counts itself done (out_slot_done), so the
-// last block's advance cannot reach a block still to write.
+// last block's advance cannot reach a block still to write.  One thread per
+// block reads it (device-scope accesses to one address are served one after
+// another: a read per wavefront cost ~10 us over 4096 wavefronts).
 __device__ __forceinline__ uint16_t* out_slot_target(uint16_t* out, const OutSlot& o, size_t n) {
     if (!o.ring) return out;
-    const uint32_t slot = __hip_atomic_load(&o.pos[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ uint32_t slot;
+    if (threadIdx.x == 0) slot = __hip_atomic_load(&o.pos[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
     return o.ring + (size_t)slot * n;
 }
 __device__ __forceinline__ void out_slot_done(const OutSlot& o) {
@@ -151,9 +155,14 @@ __global__ void __launch_bounds__(256) k_float_to_ushort(uint16_t* out, const fl
     out_slot_done(slot);
 }
 
+// kContrastV float4 per thread, their loads issued together: a quarter of the
+// blocks, so a quarter of the output slot's device-scope accesses and of the
+// blocks' min/max decodes.
+constexpr int kContrastV = 4;
+
 __global__ void __launch_bounds__(256)
     k_contrast_u16(uint16_t* __restrict__ out_arg, float* __restrict__ out_f32, const float* __restrict__ in, size_t n,
-                   const void* range, int encoded, int invert, float* range_out, OutSlot slot) {
+                   const void* range, int encoded, int invert, float* range_out, OutSlot slot, int vec) {
     uint16_t* __restrict__ out = out_slot_target(out_arg, slot, n);
     float mn, mx;
     if (encoded) {
@@ -172,21 +181,33 @@ __global__ void __launch_bounds__(256)
         mx = t;
     }
     const float factor = (unsigned short)65535 / (mx - mn + FLT_EPSILON);
-    const size_t n4 = n / 4;
+    const size_t n4 = vec ? n / 4 : 0;  // unaligned buffers: the element loop below takes every pixel
     const float4* in4 = reinterpret_cast<const float4*>(in);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-        float4 v = in4[i];
-        v.x = factor * (v.x - mn);
-        v.y = factor * (v.y - mn);
-        v.z = factor * (v.z - mn);
-        v.w = factor * (v.w - mn);
-        if (out_f32) reinterpret_cast<float4*>(out_f32)[i] = v;
-        ushort4 u;
-        u.x = f2u16(v.x);
-        u.y = f2u16(v.y);
-        u.z = f2u16(v.z);
-        u.w = f2u16(v.w);
-        reinterpret_cast<ushort4*>(out)[i] = u;
+    const size_t step = (size_t)gridDim.x * blockDim.x * kContrastV;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x * kContrastV + threadIdx.x; i0 < n4; i0 += step) {
+        float4 v[kContrastV];
+#pragma unroll
+        for (int j = 0; j < kContrastV; ++j) {
+            const size_t i = i0 + (size_t)j * blockDim.x;
+            if (i < n4) v[j] = in4[i];
+        }
+#pragma unroll
+        for (int j = 0; j < kContrastV; ++j) {
+            const size_t i = i0 + (size_t)j * blockDim.x;
+            if (i >= n4) break;
+            float4 w = v[j];
+            w.x = factor * (w.x - mn);
+            w.y = factor * (w.y - mn);
+            w.z = factor * (w.z - mn);
+            w.w = factor * (w.w - mn);
+            if (out_f32) reinterpret_cast<float4*>(out_f32)[i] = w;
+            ushort4 u;
+            u.x = f2u16(w.x);
+            u.y = f2u16(w.y);
+            u.z = f2u16(w.z);
+            u.w = f2u16(w.w);
+            reinterpret_cast<ushort4*>(out)[i] = u;
+        }
     }
     for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const float v = factor * (in[i] - mn);
@@ -213,9 +234,12 @@ hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipSt
 }
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
                                     int encoded, int invert, hipStream_t st, float* range_out, OutSlot slot) {
-    const unsigned blocks = blocks_for((n + 3) / 4, 256);
+    const unsigned blocks = blocks_for((n + 3) / 4, 256 * kContrastV);
     if (encoded == kMmPartials) HV_TRY(collapse_partials((uint32_t*)range, blocks, encoded, st));
-    k_contrast_u16<<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot);
+    const int vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out_f32)) & 15) == 0 &&
+                    ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(slot.ring)) & 7) == 0 &&
+                    (!slot.ring || n % 4 == 0);  // queue slots of n pixels stay 8-byte aligned
+    k_contrast_u16<<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot, vec);
     HV_CHECK_LAUNCH("k_contrast_u16");
     return HV_OK;
 }
@@ -402,15 +426,32 @@ hv_status launch_percentile(float* range, const float* img, int width, int heigh
 // ---------------------------------------------------------------------------
 // rescale_float_unwrap2d (tools_conversion.cu:315-335), device min/max
 // ---------------------------------------------------------------------------
+// kContrastV float4 per thread like k_contrast_u16: at most 256 blocks for a
+// 1024^2 image, which decode the unwrap's min/max partials directly.
 __global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float* in, size_t n, const uint32_t* mm,
-                                                        int slots) {
+                                                        int slots, int vec) {
     float mn, mx;
     mm_decode_block(mm, slots, mn, mx);
     const float m = (unsigned short)65535;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const float v = in[i];
-        out[i] = mn < 0.f ? (v + fabsf(mn)) / (fabsf(mn) + mx) * m : (v - mn) / (mx - mn) * m;
+    auto f = [&](float v) { return mn < 0.f ? (v + fabsf(mn)) / (fabsf(mn) + mx) * m : (v - mn) / (mx - mn) * m; };
+    const size_t n4 = vec ? n / 4 : 0;
+    const size_t step = (size_t)gridDim.x * blockDim.x * kContrastV;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x * kContrastV + threadIdx.x; i0 < n4; i0 += step) {
+        float4 v[kContrastV];
+#pragma unroll
+        for (int j = 0; j < kContrastV; ++j) {
+            const size_t i = i0 + (size_t)j * blockDim.x;
+            if (i < n4) v[j] = reinterpret_cast<const float4*>(in)[i];
+        }
+#pragma unroll
+        for (int j = 0; j < kContrastV; ++j) {
+            const size_t i = i0 + (size_t)j * blockDim.x;
+            if (i >= n4) break;
+            reinterpret_cast<float4*>(out)[i] = make_float4(f(v[j].x), f(v[j].y), f(v[j].z), f(v[j].w));
+        }
     }
+    for (size_t i = n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = f(in[i]);
 }
 
 hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t* mm, hipStream_t st, int slots,
@@ -419,9 +460,10 @@ hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t*
         HV_TRY(launch_minmax_init(mm, st, slots));
         HV_TRY(launch_minmax(mm, in, n, st, slots));
     }
-    const unsigned blocks = blocks_for(n, 256);
+    const unsigned blocks = blocks_for((n + 3) / 4, 256 * kContrastV);
     HV_TRY(collapse_partials(mm, blocks, slots, st));
-    k_rescale_unwrap<<<blocks, 256, 0, st>>>(out, in, n, mm, slots);
+    const int vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    k_rescale_unwrap<<<blocks, 256, 0, st>>>(out, in, n, mm, slots, vec);
     HV_CHECK_LAUNCH("k_rescale_unwrap");
     return HV_OK;
 }
