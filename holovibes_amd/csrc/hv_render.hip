@@ -363,6 +363,85 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The same radix select in one workgroup when the visited pixels fit in LDS
+// (subsampled images: c3's 8 x 8 subsample of 1024^2 is 16384 keys): the
+// keys are read once, the three digit passes and both selections run on
+// them in LDS, and the range is written -- one launch instead of seven,
+// with the same result (the selection is exact).
+constexpr int kPctSmallMax = 16384;
+constexpr int kPctSmallThreads = 1024;
+
+__global__ void __launch_bounds__(kPctSmallThreads)
+    k_pct_small(const float* __restrict__ img, int in_w, int out_w, int step, int n, uint32_t r0, uint32_t r1,
+                float* out, int clamp) {
+    __shared__ uint32_t keys[kPctSmallMax];
+    __shared__ uint32_t h[2][kPctBins];
+    __shared__ uint32_t part[2][kPctSmallThreads / 2];
+    __shared__ uint32_t sel_bin[2], sel_before[2];
+    constexpr int PER = 2 * kPctBins / kPctSmallThreads;  // bins per thread of its selection's half
+    const int t = threadIdx.x;
+    for (int i = t; i < n; i += kPctSmallThreads)
+        keys[i] = float_to_ordered(img[step > 1 ? subsample_index((size_t)i, in_w, out_w, step) : (size_t)i]);
+    uint32_t prefix0 = 0, prefix1 = 0, rank0 = r0, rank1 = r1;
+    const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
+    for (int d = 0; d < 3; ++d) {
+        for (int i = t; i < 2 * kPctBins; i += kPctSmallThreads) (&h[0][0])[i] = 0;
+        __syncthreads();
+        const int shift = shifts[d], width = widths[d], hs = shift + width;
+        const uint32_t mask = (1u << width) - 1;
+        for (int i = t; i < n; i += kPctSmallThreads) {
+            const uint32_t key = keys[i];
+            const uint32_t digit = (key >> shift) & mask;
+            const uint32_t high = hs >= 32 ? 0u : (key >> hs);
+            if (high == prefix0) atomicAdd(&h[0][digit], 1u);
+            if (high == prefix1) atomicAdd(&h[1][digit], 1u);
+        }
+        __syncthreads();
+        // threads [0, 512) select for the lower percentile, [512, 1024) for the upper
+        const int q = t / (kPctSmallThreads / 2), u = t % (kPctSmallThreads / 2);
+        const uint32_t rank = q ? rank1 : rank0;
+        uint32_t local[PER], sum = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int b = u * PER + k;
+            local[k] = b < (1 << width) ? h[q][b] : 0u;
+            sum += local[k];
+        }
+        part[q][u] = sum;
+        __syncthreads();
+        for (int o = 1; o < kPctSmallThreads / 2; o <<= 1) {  // inclusive scan of each half's partial sums
+            const uint32_t v = u >= o ? part[q][u - o] : 0u;
+            __syncthreads();
+            part[q][u] += v;
+            __syncthreads();
+        }
+        const uint32_t before = u > 0 ? part[q][u - 1] : 0u;
+        if (rank >= before && rank < part[q][u]) {
+            uint32_t c = before;
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                if (rank < c + local[k]) {
+                    sel_bin[q] = u * PER + k;
+                    sel_before[q] = c;
+                    break;
+                }
+                c += local[k];
+            }
+        }
+        __syncthreads();
+        prefix0 = (prefix0 << width) | sel_bin[0];
+        prefix1 = (prefix1 << width) | sel_bin[1];
+        rank0 -= sel_before[0];
+        rank1 -= sel_before[1];
+        __syncthreads();  // sel_* and h are rewritten by the next pass
+    }
+    if (t < 2) {
+        float v = ordered_to_float(t ? prefix1 : prefix0);
+        if (clamp) v = v > 1.0f ? v : 1.0f;  // contrast_api.cc:139-140
+        out[t] = v;
+    }
+}
+
 uint32_t reticle_radius(int width, int height, float scale) {
     // compute_percentile_xy_view (percentile.cu:166-170): float product, truncated
     return (uint32_t)(scale * (float)std::min(width, height) / 2);
@@ -409,6 +488,12 @@ hv_status launch_percentile(float* range, const float* img, int width, int heigh
         if (r >= count) r = (uint32_t)count - 1;  // p = 100 would read one past the end
         return r;
     };
+    if (!z.disc && n <= (size_t)kPctSmallMax) {
+        k_pct_small<<<1, kPctSmallThreads, 0, st>>>(img, width, out_w, step, (int)n, rank_of(lo), rank_of(hi), range,
+                                                    clamp);
+        HV_CHECK_LAUNCH("k_pct_small");
+        return HV_OK;
+    }
     PctScratch* s = reinterpret_cast<PctScratch*>(scratch);
     k_pct_init<<<1, 256, 0, st>>>(s, rank_of(lo), rank_of(hi));
     HV_CHECK_LAUNCH("k_pct_init");

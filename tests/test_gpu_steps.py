@@ -312,6 +312,20 @@ def test_percentile_exact(st, oracle, step, lo, hi):
     assert np.array_equal(got_c, np.maximum(want, 1.0))
 
 
+@pytest.mark.parametrize("H,W,step", [(256, 256, 2), (258, 256, 2), (64, 64, 1), (1024, 1024, 8)])
+def test_percentile_one_workgroup_boundary(st, oracle, H, W, step):
+    """Up to 16384 visited pixels the select runs in one workgroup on keys
+    held in LDS (256^2 / 2: exactly 16384; 258 x 256 / 2: one row more, the
+    multi-pass select); both equal the oracle's order statistics."""
+    rng = np.random.default_rng(17)
+    img = (rng.standard_normal((H, W)) * 1000).astype(np.float32)
+    img[0, 0] = -0.0
+    src = oracle.subsample(img, step) if step > 1 else img
+    for lo, hi in [(0.5, 99.5), (0.0, 100.0), (50.0, 50.0)]:
+        got = host(st.percentile_range(dev(img), lo, hi, step, clamp=False))
+        assert np.array_equal(got, oracle.percentile(src, [lo, hi])), (lo, hi)
+
+
 @pytest.mark.parametrize("W,H,scale", [(128, 256, 0.5), (256, 256, 1.0), (64, 32, 0.3), (512, 128, 0.0)])
 def test_percentile_reticle_exact(st, oracle, W, H, scale):
     """Percentiles over the contrast reticle's centred disc equal the oracle's
