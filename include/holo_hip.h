@@ -449,8 +449,8 @@ hv_status hv_pipe_input_buffer(hv_pipe* p, void** frames);
 hv_status hv_pipe_set_timing(hv_pipe* p, int enabled);
 hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
 
-/* No reference counterpart.  With B = S = T every batch replaces the whole
- * window, which allows two shorter cycle paths for STFT images:
+/* No reference counterpart.  Two shorter cycle paths than the three passes
+ * (level 1 needs B = S = T, every batch replacing the whole window):
  *   level 1 -- angular spectrum, T <= 32, a T x W tile within LDS: the
  *              inverse row FFT pass is fused with the STFT and image
  *              epilogue; the ring holds each frame after the column pass
@@ -467,10 +467,18 @@ hv_status hv_pipe_stage_times(hv_pipe* p, float* ms5, int* cycles);
  *              window are propagated.  W <= 1024 and T <= 64 fuse the bins
  *              with the row FFTs (and, for images, the inverse row FFTs with
  *              the image term); larger frames and windows run them as passes
- *              of their own (T up to 256).
+ *              of their own (T up to 256).  B = S = T: the bins are taken
+ *              from the batch itself.  B < T (a sliding window): the pipe
+ *              keeps the window's raw frames in a raw ring (T frames of
+ *              `depth` bytes, slot n holding frame n mod T as the reference's
+ *              ring) and takes the bins from it; up to 4 STFT bins are summed
+ *              directly, the batch written into the raw ring as the window
+ *              is read.
  * hv_pipe_set_fused picks the highest eligible path at or below `level`
  * (default: 2); 0 forces the three-pass + STFT sequence.  hv_pipe_path
- * reports the path in use (0, 1 or 2). */
+ * reports the path in use (0, 1 or 2).  Leaving the sliding-window level 2
+ * path rebuilds the three-pass ring from the raw window; entering it, the
+ * window refills first (T / B batches without an image). */
 hv_status hv_pipe_set_fused(hv_pipe* p, int level);
 hv_status hv_pipe_path(const hv_pipe* p, int* path);
 
@@ -480,11 +488,12 @@ hv_status hv_pipe_path(const hv_pipe* p, int* path);
  * cycle n runs on lane n mod lanes.  When every batch replaces the whole
  * window (B = S = T) cycles share no device state and each lane also has its
  * own bins-first planes or time-window ring and PCA buffers; otherwise the
- * lanes share the ring: a cycle's propagation waits until the previous
- * cycle's time transform has read it, and only the image post-processing
- * (unwrap, percentile, contrast) overlaps the next cycle (eager launches,
- * no graphs).  Only for a pipe that owns its streams (created without a
- * stream).  Outputs (hv_pipe_output, hv_pipe_contrast_range) are the latest
+ * lanes share the ring (the three-pass ring, or the raw ring of the
+ * bins-first path with their own planes): a cycle's write into it waits
+ * until the previous cycle's time transform has read it, and the rest of
+ * the cycle (the image post-processing; on the bins-first path also the
+ * bins' propagation) overlaps the next cycle (eager launches, no graphs).
+ * Only for a pipe that owns its streams (created without a stream).  Outputs (hv_pipe_output, hv_pipe_contrast_range) are the latest
  * cycle's; call hv_pipe_sync before reading them.  A frozen autocontrast
  * range reaches every lane in stream order.  Changing the path
  * (hv_pipe_set_fused) returns to one lane; the frames a lane reads must stay
@@ -493,7 +502,7 @@ hv_status hv_pipe_set_lanes(hv_pipe* p, int lanes);
 
 /* No reference counterpart (measurement).  Mean milliseconds of one launch of
  * a bins-first stage -- 0 bins (or PCA components) + row FFTs (cycling over
- * the n_frames input batches), 1 column pass, 2 (PCA) the raw window's
+ * the n_frames input batches; a sliding window: from the raw ring), 1 column pass, 2 (PCA) the raw window's
  * covariance + eigensolver, 3 inverse row FFT + image, 4 contrast + u16 --
  * over `iters` launches back to back on lane 0's stream, between two HIP
  * events.  The pipe's buffers are overwritten; the next cycle rewrites them. */
@@ -509,7 +518,9 @@ hv_status hv_pipe_measure_latency(hv_pipe* p, const void* const* frames, int n_f
                                   double period_ns, int64_t* latency_ns, int* n_out);
 
 /* The time-transformation queue: ring base (T frames, complex), its
- * start index and size (Queue::get_start_index / get_size, queue.hh). */
+ * start index and size (Queue::get_start_index / get_size, queue.hh).  On
+ * the bins-first path (level 2) the ring's first frames hold the bins'
+ * planes instead of the propagated window. */
 hv_status hv_pipe_ring(hv_pipe* p, const void** ring, uint32_t* start, uint32_t* size);
 
 /* The pipe's stream (hipStream_t). */
