@@ -155,11 +155,16 @@ __global__ void __launch_bounds__(256) k_float_to_ushort(uint16_t* out, const fl
     out_slot_done(slot);
 }
 
-// kContrastV float4 per thread, their loads issued together: a quarter of the
-// blocks, so a quarter of the output slot's device-scope accesses and of the
-// blocks' min/max decodes.
-constexpr int kContrastV = 4;
+// V float4 per thread, their loads issued together, V chosen so that an
+// image of 1024^2 or more still gets 256 blocks: fewer blocks mean fewer of
+// the output slot's device-scope accesses and of the blocks' min/max
+// decodes; smaller images keep one float4 per thread and 256 blocks or less.
+static int float4_per_thread(size_t n) {
+    const size_t n4 = n / 4;
+    return n4 >= (size_t)4 * 256 * 256 ? 4 : n4 >= (size_t)2 * 256 * 256 ? 2 : 1;
+}
 
+template <int V>
 __global__ void __launch_bounds__(256)
     k_contrast_u16(uint16_t* __restrict__ out_arg, float* __restrict__ out_f32, const float* __restrict__ in, size_t n,
                    const void* range, int encoded, int invert, float* range_out, OutSlot slot, int vec) {
@@ -183,16 +188,16 @@ __global__ void __launch_bounds__(256)
     const float factor = (unsigned short)65535 / (mx - mn + FLT_EPSILON);
     const size_t n4 = vec ? n / 4 : 0;  // unaligned buffers: the element loop below takes every pixel
     const float4* in4 = reinterpret_cast<const float4*>(in);
-    const size_t step = (size_t)gridDim.x * blockDim.x * kContrastV;
-    for (size_t i0 = (size_t)blockIdx.x * blockDim.x * kContrastV + threadIdx.x; i0 < n4; i0 += step) {
-        float4 v[kContrastV];
+    const size_t step = (size_t)gridDim.x * blockDim.x * V;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x * V + threadIdx.x; i0 < n4; i0 += step) {
+        float4 v[V];
 #pragma unroll
-        for (int j = 0; j < kContrastV; ++j) {
+        for (int j = 0; j < V; ++j) {
             const size_t i = i0 + (size_t)j * blockDim.x;
             if (i < n4) v[j] = in4[i];
         }
 #pragma unroll
-        for (int j = 0; j < kContrastV; ++j) {
+        for (int j = 0; j < V; ++j) {
             const size_t i = i0 + (size_t)j * blockDim.x;
             if (i >= n4) break;
             float4 w = v[j];
@@ -234,12 +239,15 @@ hv_status launch_float_to_ushort(uint16_t* out, const float* in, size_t n, hipSt
 }
 hv_status launch_contrast_to_ushort(uint16_t* out, float* out_f32, const float* in, size_t n, const void* range,
                                     int encoded, int invert, hipStream_t st, float* range_out, OutSlot slot) {
-    const unsigned blocks = blocks_for((n + 3) / 4, 256 * kContrastV);
+    const int V = float4_per_thread(n);
+    const unsigned blocks = blocks_for((n + 3) / 4, 256 * V);
     if (encoded == kMmPartials) HV_TRY(collapse_partials((uint32_t*)range, blocks, encoded, st));
     const int vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out_f32)) & 15) == 0 &&
                     ((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(slot.ring)) & 7) == 0 &&
                     (!slot.ring || n % 4 == 0);  // queue slots of n pixels stay 8-byte aligned
-    k_contrast_u16<<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot, vec);
+    if (V == 4) k_contrast_u16<4><<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot, vec);
+    else if (V == 2) k_contrast_u16<2><<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot, vec);
+    else k_contrast_u16<1><<<blocks, 256, 0, st>>>(out, out_f32, in, n, range, encoded, invert, range_out, slot, vec);
     HV_CHECK_LAUNCH("k_contrast_u16");
     return HV_OK;
 }
@@ -522,8 +530,9 @@ hv_status launch_percentile(float* range, const float* img, int width, int heigh
 // ---------------------------------------------------------------------------
 // rescale_float_unwrap2d (tools_conversion.cu:315-335), device min/max
 // ---------------------------------------------------------------------------
-// kContrastV float4 per thread like k_contrast_u16: at most 256 blocks for a
-// 1024^2 image, which decode the unwrap's min/max partials directly.
+// V float4 per thread like k_contrast_u16: 256 blocks for a 1024^2 image,
+// which decode the unwrap's min/max partials directly.
+template <int V>
 __global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float* in, size_t n, const uint32_t* mm,
                                                         int slots, int vec) {
     float mn, mx;
@@ -531,16 +540,16 @@ __global__ void __launch_bounds__(256) k_rescale_unwrap(float* out, const float*
     const float m = (unsigned short)65535;
     auto f = [&](float v) { return mn < 0.f ? (v + fabsf(mn)) / (fabsf(mn) + mx) * m : (v - mn) / (mx - mn) * m; };
     const size_t n4 = vec ? n / 4 : 0;
-    const size_t step = (size_t)gridDim.x * blockDim.x * kContrastV;
-    for (size_t i0 = (size_t)blockIdx.x * blockDim.x * kContrastV + threadIdx.x; i0 < n4; i0 += step) {
-        float4 v[kContrastV];
+    const size_t step = (size_t)gridDim.x * blockDim.x * V;
+    for (size_t i0 = (size_t)blockIdx.x * blockDim.x * V + threadIdx.x; i0 < n4; i0 += step) {
+        float4 v[V];
 #pragma unroll
-        for (int j = 0; j < kContrastV; ++j) {
+        for (int j = 0; j < V; ++j) {
             const size_t i = i0 + (size_t)j * blockDim.x;
             if (i < n4) v[j] = reinterpret_cast<const float4*>(in)[i];
         }
 #pragma unroll
-        for (int j = 0; j < kContrastV; ++j) {
+        for (int j = 0; j < V; ++j) {
             const size_t i = i0 + (size_t)j * blockDim.x;
             if (i >= n4) break;
             reinterpret_cast<float4*>(out)[i] = make_float4(f(v[j].x), f(v[j].y), f(v[j].z), f(v[j].w));
@@ -556,10 +565,13 @@ hv_status launch_rescale_unwrap(float* out, const float* in, size_t n, uint32_t*
         HV_TRY(launch_minmax_init(mm, st, slots));
         HV_TRY(launch_minmax(mm, in, n, st, slots));
     }
-    const unsigned blocks = blocks_for((n + 3) / 4, 256 * kContrastV);
+    const int V = float4_per_thread(n);
+    const unsigned blocks = blocks_for((n + 3) / 4, 256 * V);
     HV_TRY(collapse_partials(mm, blocks, slots, st));
     const int vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
-    k_rescale_unwrap<<<blocks, 256, 0, st>>>(out, in, n, mm, slots, vec);
+    if (V == 4) k_rescale_unwrap<4><<<blocks, 256, 0, st>>>(out, in, n, mm, slots, vec);
+    else if (V == 2) k_rescale_unwrap<2><<<blocks, 256, 0, st>>>(out, in, n, mm, slots, vec);
+    else k_rescale_unwrap<1><<<blocks, 256, 0, st>>>(out, in, n, mm, slots, vec);
     HV_CHECK_LAUNCH("k_rescale_unwrap");
     return HV_OK;
 }
