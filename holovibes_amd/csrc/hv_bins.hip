@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(BinTimeCfg<T>::THREADS)
 // contiguous bytes of one plane per store.
 // CW pixels per workgroup: the widest tile of HV_BINS_TIME_LDS_KB KiB of LDS
 // (each frame of the window read as CW-pixel runs: 64 pixels of u16 frames,
-// 128-byte runs, took 1.14 ms at c5 -- 57 bins of 128 frames of 2048^2 --
+// 128-byte runs, took 1.14 ms for 57 bins of 128 frames of 2048^2 u16 --
 // against 0.92 ms with 128 pixels), or 64 pixels when the frame is not a
 // multiple of that width.
 template <int T>
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__((BinTimeRealCfg<T, CWT>::THREADS))
 // Few bins (K <= HV_BINS_DIRECT_K) in their own pass: each thread sums the
 // window of 8 pixels directly, X_k = sum_n x_n e^{-2 pi i k n / T} over the
 // physical slots n (slot n holding frame (n - slot0) mod T), each frame read
-// once with 8-pixel loads, G frames' loads in flight, no LDS and no barrier
+// once with 8-pixel loads, kDirectG frames' loads in flight, no LDS and no barrier
 // (the transform kernels above stage a tile of every pixel's window in LDS
 // and compute all T bins: latency-bound when only a few are kept).  The sums
 // run over x_n - x_0: for k != 0 mod T the constant drops out exactly, so the
@@ -500,7 +500,13 @@ __global__ void __launch_bounds__((BinTimeRealCfg<T, CWT>::THREADS))
 #define HV_BINS_DIRECT_K 4
 #endif
 constexpr int kDirectThreads = 256;
-constexpr int kDirectG = 8;  // frames whose loads are issued together
+// frames whose loads are issued together: 1024^2 u16, T = 64, one bin, 3 lanes ran at
+// 327 k / 324 k / 340 k / 358 k / 328 k frames/s with 16 / 8 / 4 / 2 / 1 (the
+// fewer registers, the more of the other lanes' kernels beside it)
+#ifndef HV_BINS_DIRECT_G
+#define HV_BINS_DIRECT_G 2
+#endif
+constexpr int kDirectG = HV_BINS_DIRECT_G;
 
 template <typename TIn>
 struct Px8 {  // 8 pixels as loaded: 8, 16 or 32 bytes
@@ -1158,7 +1164,7 @@ hv_status bins_image_t(float* out, float2* planes, int width, int height, int K,
     if (tiles) return launch_row_ifft_image(out, tiles, width, height, K, IMG, count, scale, mm, mm_slots, st, true);
     // a few planes: the fused kernel transforms G plane rows per round (8 at
     // W = 1024), padding rows included; the epilogue kernel takes the planes
-    // one after another (c3, one plane of 1024^2: 21.0 -> 7 us)
+    // one after another (one plane of 1024^2: 21.0 -> 7 us)
     if (space == HV_SPACE_ANGULARSP && K <= HV_BINS_IMAGE_EPI_K && row_ifft_epi_supported(width))
         return launch_row_ifft_image(out, planes, width, height, K, IMG, count, scale, mm, mm_slots, st, false);
     if (!bins_image_fused(width, space)) {

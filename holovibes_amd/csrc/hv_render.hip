@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // The same radix select in one workgroup when the visited pixels fit in LDS
-// (subsampled images: c3's 8 x 8 subsample of 1024^2 is 16384 keys): the
+// (subsampled images: the 8 x 8 subsample of 1024^2 is 16384 keys): the
 // keys are read once, the three digit passes and both selections run on
 // them in LDS, and the range is written -- one launch instead of seven,
 // with the same result (the selection is exact).
