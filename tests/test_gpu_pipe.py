@@ -156,6 +156,11 @@ def test_pipe_stride_discard(oracle):
                 p_width=3, as_f32=True), 1e-5),  # bin 0 (the window's first frame added back), float frames
     ("c1", dict(width=2048, height=16, batch_size=5, time_stride=5, time_transformation_size=16, p_start=3,
                 p_width=0, depth=1, syn_depth=1, syn_bits=8, syn_mean_counts=64.0), 1e-5),
+    # 1024-long columns with few planes (AS lens, Fresnel forward)
+    ("c1", dict(width=64, height=1024, batch_size=8, time_stride=8, time_transformation_size=32, p_start=2,
+                p_width=1), 1e-5),
+    ("c1", dict(width=128, height=1024, batch_size=8, time_stride=8, time_transformation_size=32, p_start=1,
+                p_width=0, space_transformation=1, z_distance=0.4), 1e-5),
 ])
 def test_pipe_sliding_window_bins(oracle, name, over, tol):
     """Sliding windows (B < T) on the bins-first path, batch by batch
