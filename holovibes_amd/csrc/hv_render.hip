@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(kPctSmallThreads)
                 float* out, int clamp) {
     __shared__ uint32_t keys[kPctSmallMax];
     __shared__ uint32_t h[2][kPctBins];
-    __shared__ uint32_t part[2][kPctSmallThreads / 2];
+    __shared__ uint32_t wsum[2][kPctSmallThreads / 2 / 64];
     __shared__ uint32_t sel_bin[2], sel_before[2];
     constexpr int PER = 2 * kPctBins / kPctSmallThreads;  // bins per thread of its selection's half
     const int t = threadIdx.x;
@@ -408,34 +408,49 @@ __global__ void __launch_bounds__(kPctSmallThreads)
         __syncthreads();
         const int shift = shifts[d], width = widths[d], hs = shift + width;
         const uint32_t mask = (1u << width) - 1;
-        for (int i = t; i < n; i += kPctSmallThreads) {
-            const uint32_t key = keys[i];
+        // both selections share the first digit's histogram (both prefixes are empty)
+        const bool shared = d == 0;
+        uint32_t kv[KPT];
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            const int i = t + j * kPctSmallThreads;
+            kv[j] = i < n ? keys[i] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+            if (t + j * kPctSmallThreads >= n) break;
+            const uint32_t key = kv[j];
             const uint32_t digit = (key >> shift) & mask;
             const uint32_t high = hs >= 32 ? 0u : (key >> hs);
             if (high == prefix0) atomicAdd(&h[0][digit], 1u);
-            if (high == prefix1) atomicAdd(&h[1][digit], 1u);
+            if (!shared && high == prefix1) atomicAdd(&h[1][digit], 1u);
         }
         __syncthreads();
         // threads [0, 512) select for the lower percentile, [512, 1024) for the upper
         const int q = t / (kPctSmallThreads / 2), u = t % (kPctSmallThreads / 2);
+        const int qh = shared ? 0 : q;
         const uint32_t rank = q ? rank1 : rank0;
         uint32_t local[PER], sum = 0;
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int b = u * PER + k;
-            local[k] = b < (1 << width) ? h[q][b] : 0u;
+            local[k] = b < (1 << width) ? h[qh][b] : 0u;
             sum += local[k];
         }
-        part[q][u] = sum;
-        __syncthreads();
-        for (int o = 1; o < kPctSmallThreads / 2; o <<= 1) {  // inclusive scan of each half's partial sums
-            const uint32_t v = u >= o ? part[q][u - o] : 0u;
-            __syncthreads();
-            part[q][u] += v;
-            __syncthreads();
+        // inclusive scan of each half's partial sums: within each wavefront by
+        // shuffles, then the half's wavefront totals
+        const int lane = u & 63, wv = u >> 6;
+        uint32_t incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
         }
-        const uint32_t before = u > 0 ? part[q][u - 1] : 0u;
-        if (rank >= before && rank < part[q][u]) {
+        if (lane == 63) wsum[q][wv] = incl;
+        __syncthreads();
+        for (int w = 0; w < wv; ++w) incl += wsum[q][w];
+        const uint32_t before = incl - sum;
+        if (rank >= before && rank < incl) {
             uint32_t c = before;
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
