@@ -113,6 +113,25 @@ def test_memory_accounting(lib):
     assert ring < n.value < ring + 32 * P * 2 + 16 * P + 2 ** 20
 
 
+def test_memory_accounting_raw_ring(lib):
+    """A sliding window (B < T) on the bins-first path adds the raw-frame
+    ring, T frames of `depth` bytes, to the pipe's memory; a window replaced
+    by every batch, or a cycle without a time transform, has none."""
+    from holovibes_amd import configs
+
+    def nbytes(**over):
+        s = configs.settings_for("c3", **over)
+        n = ctypes.c_size_t()
+        assert lib.hv_pipe_memory_bytes(ctypes.byref(s), ctypes.byref(n)) == 0
+        return n.value
+
+    P, T = 1024 * 1024, 64
+    assert nbytes() - nbytes(time_transformation=0) == T * P * 2
+    assert nbytes(depth=1) - nbytes(depth=1, time_transformation=0) == T * P
+    # B = T: the bins are taken from the batch
+    assert nbytes(batch_size=64, time_stride=64) - nbytes(batch_size=64, time_stride=64, time_transformation=0) == 0
+
+
 def test_synth_deterministic_and_calibrated():
     from holovibes_amd import configs
     from holovibes_amd.pipe import synth_frames
